@@ -1,0 +1,352 @@
+"""Thin ctypes binding over libaxs_hip.so's C ABI (include/axs.h).
+
+This is test/bench plumbing: it loads the in-tree HIP library, declares the
+C signatures and wraps a level in a small Python class.  It never computes
+anything itself and raises if the library is missing -- there is no CPU
+fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libaxs_hip.so")
+
+AXS_OK = 0
+AXS_FP64, AXS_FP32C, AXS_FP32 = 0, 1, 2
+AXS_MASK_NONE, AXS_MASK_A, AXS_MASK_B = 0, 1, 2
+AXS_FIELD_MASKED_A_DOT, AXS_FIELD_PI1, AXS_FIELD_PI2 = 0, 1, 2
+DBL_MAX = np.finfo(np.float64).max
+
+
+class AxsError(RuntimeError):
+    pass
+
+
+class Background(C.Structure):
+    _fields_ = [("a_inv", C.c_double), ("c0", C.c_double), ("c1", C.c_double),
+                ("tau_switch", C.c_double), ("has_switch", C.c_int),
+                ("flat_space", C.c_int), ("flat_m_r", C.c_double)]
+
+
+class RhsCoeffs(C.Structure):
+    _fields_ = [("curvature_coeff", C.c_double), ("lambda_", C.c_double),
+                ("R_squared", C.c_double)]
+
+
+class TaggerParams(C.Structure):
+    _fields_ = [("gradient_threshold", C.c_double),
+                ("radial_gradient_threshold", C.c_double),
+                ("amplitude_threshold", C.c_double), ("R", C.c_double)]
+
+
+class Masking(C.Structure):
+    _fields_ = [("scheme", C.c_int), ("threshold", C.c_double)]
+
+
+_P = C.c_void_p
+_I = C.c_int
+_D = C.c_double
+_PD = C.POINTER(C.c_double)
+_PI64 = C.POINTER(C.c_int64)
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "axs_ctx_create": (_I, [_I, C.POINTER(_P)]),
+    "axs_ctx_destroy": (_I, [_P]),
+    "axs_ctx_synchronize": (_I, [_P]),
+    "axs_ctx_stream": (_P, [_P]),
+    "axs_last_error": (C.c_char_p, []),
+    "axs_num_tile_variants": (_I, []),
+    "axs_background_eval": (_I, [C.POINTER(Background), _I, _D, _PD]),
+    "axs_background_rhs_coeffs": (_I, [C.POINTER(Background), _D,
+                                       C.POINTER(RhsCoeffs)]),
+    "axs_level_create": (_I, [_P, _I, _I, _I, _D, _I, C.POINTER(_P)]),
+    "axs_level_destroy": (_I, [_P]),
+    "axs_level_upload": (_I, [_P, _PD]),
+    "axs_level_download": (_I, [_P, _PD]),
+    "axs_level_download_raw": (_I, [_P, _P, _P]),
+    "axs_level_device_ptrs": (_I, [_P, C.POINTER(_P), C.POINTER(_P)]),
+    "axs_level_set_homogeneous": (_I, [_P, _D, _D]),
+    "axs_level_set_tile": (_I, [_P, _I, _I]),
+    "axs_level_get_tile": (_I, [_P, C.POINTER(_I), C.POINTER(_I)]),
+    "axs_rhs_eval": (_I, [_P, C.POINTER(RhsCoeffs), _P]),
+    "axs_rk4_step": (_I, [_P, C.POINTER(RhsCoeffs), _D]),
+    "axs_rk4_advance": (_I, [_P, C.POINTER(Background), _D, _D, _I, _I]),
+    "axs_count_plaquettes": (_I, [_P, _P, _PI64]),
+    "axs_windings_field": (_I, [_P, _P]),
+    "axs_tag_cells": (_I, [_P, C.POINTER(TaggerParams), _P, _PI64]),
+    "axs_energy_sums": (_I, [_P, _D, _D, _D, C.POINTER(Masking), _P, _PD]),
+    "axs_fill_field": (_I, [_P, _I, C.POINTER(Masking), _D, _P]),
+    "axs_level_spectrum": (_I, [_P, _I, C.POINTER(Masking), _D, _PD, _PI64,
+                                _PD, _PD]),
+    "axs_level_fourier_ic": (_I, [_P, _I, _D, _D, C.c_uint64, _I]),
+    "axs_level_zero_components": (_I, [_P, _I, _I]),
+    "axs_level_compare": (_I, [_P, _P, _PD, _PD]),
+    "axs_boxlevel_create": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_I),
+                                 C.POINTER(_I), _I, _D, _I, C.POINTER(_P)]),
+    "axs_boxlevel_destroy": (_I, [_P]),
+    "axs_boxlevel_num_cells": (C.c_int64, [_P]),
+    "axs_boxlevel_upload_dense": (_I, [_P, _PD]),
+    "axs_boxlevel_download_dense": (_I, [_P, _PD]),
+    "axs_boxlevel_fill_ghosts": (_I, [_P, _P, _D]),
+    "axs_boxlevel_rk4_step": (_I, [_P, C.POINTER(RhsCoeffs), _D]),
+    "axs_boxlevel_tag": (_I, [_P, C.POINTER(TaggerParams), _P, _PI64]),
+    "axs_adjoint_grad_energy": (_I, [_P, C.POINTER(Background), _D, _D, _D,
+                                     _I, _I, _PD, _PD, _PD]),
+    "axs_forward_grad_energy": (_I, [_P, C.POINTER(Background), _D, _D, _D,
+                                     _I, _PD]),
+}
+
+_lib: Optional[C.CDLL] = None
+
+
+def declared_symbols() -> list[str]:
+    return list(_SIGS)
+
+
+def load(path: str = LIB_PATH) -> C.CDLL:
+    """Loads libaxs_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise AxsError(f"{path} not built: run __graft_entry__.build() or "
+                       "python -m axion_strings_adaptive_amd.build")
+    lib = C.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name, None)
+        if fn is None:
+            continue
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int) -> None:
+    if rc != AXS_OK:
+        msg = load().axs_last_error().decode(errors="replace")
+        raise AxsError(f"axs error {rc}: {msg}")
+
+
+def dptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_PD)
+
+
+def make_background(a_inv=2.0, c0=0.0, c1=0.0, tau_switch=0.0,
+                    has_switch=False, flat_space=False, flat_m_r=1.0):
+    return Background(a_inv, c0, c1, tau_switch, int(has_switch),
+                      int(flat_space), flat_m_r)
+
+
+def background_eval(bg: Background, which: int, x: float) -> float:
+    out = C.c_double()
+    check(load().axs_background_eval(C.byref(bg), which, x, C.byref(out)))
+    return out.value
+
+
+def rhs_coeffs(bg: Background, tau: float) -> RhsCoeffs:
+    c = RhsCoeffs()
+    check(load().axs_background_rhs_coeffs(C.byref(bg), tau, C.byref(c)))
+    return c
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        h = _P()
+        check(self.lib.axs_ctx_create(device, C.byref(h)))
+        self.h = h
+
+    def synchronize(self):
+        check(self.lib.axs_ctx_synchronize(self.h))
+
+    def close(self):
+        if self.h:
+            self.lib.axs_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceBuffer:
+    """Raw device memory owned through torch (plumbing only)."""
+
+    def __init__(self, nbytes: int):
+        import torch
+        self.t = torch.empty(max(nbytes, 1), dtype=torch.uint8, device="cuda")
+        self.ptr = self.t.data_ptr()
+        self.nbytes = nbytes
+
+    def to_numpy(self, dtype) -> np.ndarray:
+        import torch
+        torch.cuda.synchronize()
+        return self.t.cpu().numpy()[: self.nbytes].view(dtype).copy()
+
+    def zero(self):
+        import torch
+        self.t.zero_()
+        torch.cuda.synchronize()
+
+    def from_numpy(self, a: np.ndarray):
+        import torch
+        b = np.ascontiguousarray(a).view(np.uint8).reshape(-1)
+        self.t[: b.size].copy_(torch.from_numpy(b))
+        torch.cuda.synchronize()
+
+
+class Level:
+    """A uniform periodic level (axs_level) owned by the HIP library."""
+
+    def __init__(self, ctx: Context, n, dx: float, precision: int = AXS_FP64):
+        if isinstance(n, int):
+            n = (n, n, n)
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.nx, self.ny, self.nz = n
+        self.dx = dx
+        self.precision = precision
+        h = _P()
+        check(self.lib.axs_level_create(ctx.h, self.nx, self.ny, self.nz, dx,
+                                        precision, C.byref(h)))
+        self.h = h
+
+    @property
+    def ncell(self) -> int:
+        return self.nx * self.ny * self.nz
+
+    @property
+    def word(self) -> int:
+        return 8 if self.precision == AXS_FP64 else 4
+
+    def upload(self, state: np.ndarray):
+        s = np.ascontiguousarray(state, dtype=np.float64).reshape(-1)
+        assert s.size == 4 * self.ncell
+        check(self.lib.axs_level_upload(self.h, dptr(s)))
+
+    def download(self) -> np.ndarray:
+        out = np.empty(4 * self.ncell, dtype=np.float64)
+        check(self.lib.axs_level_download(self.h, dptr(out)))
+        return out.reshape(4, self.nz, self.ny, self.nx)
+
+    def download_raw(self):
+        dt = np.float64 if self.precision == AXS_FP64 else np.float32
+        hi = np.empty(4 * self.ncell, dtype=dt)
+        lo = np.empty(4 * self.ncell, dtype=np.float32) \
+            if self.precision == AXS_FP32C else None
+        check(self.lib.axs_level_download_raw(
+            self.h, hi.ctypes.data_as(_P),
+            lo.ctypes.data_as(_P) if lo is not None else None))
+        return hi, lo
+
+    def set_homogeneous(self, R: float, Rp: float):
+        check(self.lib.axs_level_set_homogeneous(self.h, R, Rp))
+
+    def set_tile(self, variant: int, z_chunk: int = 0):
+        check(self.lib.axs_level_set_tile(self.h, variant, z_chunk))
+
+    def rhs(self, coeffs: RhsCoeffs) -> np.ndarray:
+        buf = DeviceBuffer(4 * self.ncell * self.word)
+        check(self.lib.axs_rhs_eval(self.h, C.byref(coeffs), _P(buf.ptr)))
+        self.ctx.synchronize()
+        dt = np.float64 if self.precision == AXS_FP64 else np.float32
+        return buf.to_numpy(dt).reshape(4, self.nz, self.ny, self.nx)
+
+    def rk4_step(self, coeffs, dt: float):
+        arr = (RhsCoeffs * 4)(*coeffs)
+        check(self.lib.axs_rk4_step(self.h, arr, dt))
+
+    def advance(self, bg: Background, tau: float, dt: float, nsteps: int,
+                use_graph: bool = False):
+        check(self.lib.axs_rk4_advance(self.h, C.byref(bg), tau, dt, nsteps,
+                                       int(use_graph)))
+
+    def count_plaquettes(self, mask: Optional[DeviceBuffer] = None):
+        out = (C.c_int64 * 2)()
+        check(self.lib.axs_count_plaquettes(
+            self.h, _P(mask.ptr) if mask is not None else None, out))
+        return int(out[0]), int(out[1])
+
+    def windings_field(self) -> np.ndarray:
+        buf = DeviceBuffer(3 * self.ncell)
+        check(self.lib.axs_windings_field(self.h, _P(buf.ptr)))
+        self.ctx.synchronize()
+        return buf.to_numpy(np.int8).reshape(self.nz, self.ny, self.nx, 3)
+
+    def tag(self, grad_thr=DBL_MAX, rgrad_thr=DBL_MAX, amp_thr=0.0, R=1.0,
+            tags: Optional[DeviceBuffer] = None):
+        if tags is None:
+            tags = DeviceBuffer(self.ncell)
+            tags.zero()
+        p = TaggerParams(grad_thr, rgrad_thr, amp_thr, R)
+        counts = (C.c_int64 * 2)()
+        check(self.lib.axs_tag_cells(self.h, C.byref(p), _P(tags.ptr), counts))
+        self.ctx.synchronize()
+        return (tags.to_numpy(np.uint8).reshape(self.nz, self.ny, self.nx),
+                (int(counts[0]), int(counts[1])))
+
+    def energy_sums(self, R, lam, RpR, scheme=AXS_MASK_NONE, thr=0.8,
+                    mask: Optional[DeviceBuffer] = None) -> np.ndarray:
+        out = np.zeros(14)
+        m = Masking(scheme, thr)
+        check(self.lib.axs_energy_sums(
+            self.h, R, lam, RpR, C.byref(m),
+            _P(mask.ptr) if mask is not None else None, dptr(out)))
+        return out
+
+    def fill_field(self, kind, scheme=AXS_MASK_NONE, thr=0.8, R=1.0):
+        w = 8 if self.precision == AXS_FP64 else 4
+        buf = DeviceBuffer(self.ncell * w)
+        m = Masking(scheme, thr)
+        check(self.lib.axs_fill_field(self.h, kind, C.byref(m), R,
+                                      _P(buf.ptr)))
+        self.ctx.synchronize()
+        dt = np.float64 if w == 8 else np.float32
+        return buf.to_numpy(dt).reshape(self.nz, self.ny, self.nx)
+
+    def spectrum(self, kind, scheme=AXS_MASK_NONE, thr=0.8, R=1.0):
+        n_shells = max(self.nx, self.ny, self.nz) // 2 + 1
+        ss = np.zeros(n_shells)
+        sc = np.zeros(n_shells, dtype=np.int64)
+        tot = np.zeros(2)
+        sumsq = C.c_double()
+        m = Masking(scheme, thr)
+        check(self.lib.axs_level_spectrum(
+            self.h, kind, C.byref(m), R, dptr(ss),
+            sc.ctypes.data_as(_PI64), dptr(tot), C.byref(sumsq)))
+        return ss, sc, tot, sumsq.value
+
+    def fourier_ic(self, comp, k_max_cells, target_ms, seed, comp_seed_offset):
+        check(self.lib.axs_level_fourier_ic(self.h, comp, k_max_cells,
+                                            target_ms, seed, comp_seed_offset))
+
+    def zero_components(self, c0, ncomp):
+        check(self.lib.axs_level_zero_components(self.h, c0, ncomp))
+
+    def compare(self, other: "Level"):
+        a, b = C.c_double(), C.c_double()
+        check(self.lib.axs_level_compare(self.h, other.h, C.byref(a),
+                                         C.byref(b)))
+        return a.value, b.value
+
+    def close(self):
+        if self.h:
+            self.lib.axs_level_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,672 @@
+// axs_capi.hip -- C ABI of libaxs_hip.so: contexts, levels, RK4 driving
+// (direct launches or a captured hipGraph), state transfer.
+#include "axs_internal.hpp"
+#include "host/Background.hpp"
+
+#include <hipfft/hipfft.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace axs
+{
+static thread_local std::string g_last_error;
+
+int set_error(int code, const std::string &msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+// ---------------------------------------------------------------- kernels
+__global__ void k_upload_convert(const double *src, float *hi, float *lo,
+                                 size_t n4, int with_lo)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n4;
+         i += (size_t)gridDim.x * blockDim.x)
+    {
+        const double v = src[i];
+        const float h = (float)v;
+        hi[i] = h;
+        if (with_lo)
+            lo[i] = (float)(v - (double)h);
+    }
+}
+
+__global__ void k_download_convert(const float *hi, const float *lo,
+                                   double *dst, size_t n4)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n4;
+         i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = (double)hi[i] + (lo ? (double)lo[i] : 0.0);
+}
+
+template <typename T>
+__global__ void k_set_homogeneous(T *y, float *lo, size_t n, double R,
+                                  double Rp)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+    {
+        y[i] = (T)R;
+        y[n + i] = (T)0;
+        y[2 * n + i] = (T)Rp;
+        y[3 * n + i] = (T)0;
+        if (lo)
+        {
+            lo[i] = (float)(R - (double)(T)R);
+            lo[n + i] = 0.0f;
+            lo[2 * n + i] = (float)(Rp - (double)(T)Rp);
+            lo[3 * n + i] = 0.0f;
+        }
+    }
+}
+
+template <typename T>
+__global__ void k_zero_range(T *y, float *lo, size_t begin, size_t end)
+{
+    for (size_t i = begin + blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+         i < end; i += (size_t)gridDim.x * blockDim.x)
+    {
+        y[i] = (T)0;
+        if (lo)
+            lo[i] = 0.0f;
+    }
+}
+
+template <typename TA, typename TB>
+__global__ void k_compare(const TA *a, const float *alo, const TB *b,
+                          const float *blo, size_t n, double *partials)
+{
+    double m = 0.0, mb = 0.0;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+    {
+        const double a1 = (double)a[i] + (alo ? (double)alo[i] : 0.0);
+        const double a2 = (double)a[n + i] + (alo ? (double)alo[n + i] : 0.0);
+        const double b1 = (double)b[i] + (blo ? (double)blo[i] : 0.0);
+        const double b2 = (double)b[n + i] + (blo ? (double)blo[n + i] : 0.0);
+        const double d1 = a1 - b1, d2 = a2 - b2;
+        m = fmax(m, sqrt(d1 * d1 + d2 * d2));
+        mb = fmax(mb, sqrt(b1 * b1 + b2 * b2));
+    }
+    __shared__ double s0[256], s1[256];
+    s0[threadIdx.x] = m;
+    s1[threadIdx.x] = mb;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1)
+    {
+        if ((int)threadIdx.x < o)
+        {
+            s0[threadIdx.x] = fmax(s0[threadIdx.x], s0[threadIdx.x + o]);
+            s1[threadIdx.x] = fmax(s1[threadIdx.x], s1[threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+    {
+        partials[2 * blockIdx.x] = s0[0];
+        partials[2 * blockIdx.x + 1] = s1[0];
+    }
+}
+
+static int grid_for(size_t n)
+{
+    size_t b = (n + 255) / 256;
+    if (b > 8192)
+        b = 8192;
+    if (b < 1)
+        b = 1;
+    return (int)b;
+}
+
+static Background make_background(const axs_background *bg)
+{
+    CTauSchedule s{};
+    s.c0 = bg->c0;
+    s.c1 = bg->c1;
+    s.tau_switch = bg->tau_switch;
+    s.has_switch = bg->has_switch != 0;
+    return Background(bg->a_inv, s);
+}
+
+// Stage scalars of specific_eval_rhs (AxionStringsLevel.cpp:859-885).
+static void rhs_coeffs_at(const axs_background *bg, double tau,
+                          axs_rhs_coeffs *out)
+{
+    if (bg->flat_space)
+    {
+        const FlatBackground f(bg->flat_m_r);
+        out->curvature_coeff = f.curvature_term_coeff(tau);
+        out->lambda = f.lambda(tau);
+        out->R_squared = 1.0;
+        return;
+    }
+    const Background b = make_background(bg);
+    out->curvature_coeff = b.curvature_term_coeff(tau);
+    out->lambda = b.lambda(tau);
+    const double R = b.R(tau);
+    out->R_squared = R * R;
+}
+
+static int level_words(const axs_level *L) { return (int)L->word; }
+
+} // namespace axs
+
+using namespace axs;
+
+// ================================================================ C ABI ==
+extern "C" {
+
+const char *axs_last_error(void) { return g_last_error.c_str(); }
+
+int axs_num_tile_variants(void) { return axs::num_tile_variants(); }
+
+int axs_ctx_create(int device, axs_ctx **out)
+{
+    AXS_CHECK_ARG(out != nullptr, "axs_ctx_create: out is NULL");
+    *out = nullptr;
+    int ndev = 0;
+    AXS_HIP_TRY(hipGetDeviceCount(&ndev));
+    AXS_CHECK_ARG(device >= 0 && device < ndev,
+                  "axs_ctx_create: no such HIP device");
+    AXS_HIP_TRY(hipSetDevice(device));
+    axs_ctx *c = new axs_ctx{};
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) !=
+            hipSuccess ||
+        hipMalloc(&c->d_counts, 8 * sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc(&c->d_partials, (size_t)kMaxPartials * 16 * sizeof(double)) !=
+            hipSuccess ||
+        hipMalloc(&c->d_result, 64 * sizeof(double)) != hipSuccess ||
+        hipHostMalloc(&c->h_result, 64 * sizeof(double)) != hipSuccess ||
+        hipHostMalloc(&c->h_counts, 8 * sizeof(unsigned long long)) !=
+            hipSuccess)
+    {
+        delete c;
+        return set_error(AXS_ERR_NOMEM, "axs_ctx_create: allocation failed");
+    }
+    *out = c;
+    return AXS_OK;
+}
+
+int axs_ctx_destroy(axs_ctx *c)
+{
+    if (!c)
+        return AXS_OK;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    hipFree(c->d_counts);
+    hipFree(c->d_partials);
+    hipFree(c->d_result);
+    hipHostFree(c->h_result);
+    hipHostFree(c->h_counts);
+    hipStreamDestroy(c->stream);
+    delete c;
+    return AXS_OK;
+}
+
+int axs_ctx_synchronize(axs_ctx *c)
+{
+    AXS_CHECK_ARG(c, "axs_ctx_synchronize: NULL context");
+    AXS_HIP_TRY(hipStreamSynchronize(c->stream));
+    return AXS_OK;
+}
+
+void *axs_ctx_stream(axs_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+int axs_background_eval(const axs_background *bg, int which, double x,
+                        double *out)
+{
+    AXS_CHECK_ARG(bg && out, "axs_background_eval: NULL argument");
+    if (bg->flat_space)
+    {
+        const FlatBackground f(bg->flat_m_r);
+        switch (which)
+        {
+        case 0: *out = f.R(x); return AXS_OK;
+        case 1: *out = f.R_prime(x); return AXS_OK;
+        case 2: *out = f.curvature_term_coeff(x); return AXS_OK;
+        case 3: *out = f.lambda(x); return AXS_OK;
+        case 8: *out = 0.0; return AXS_OK;
+        default:
+            return set_error(AXS_ERR_ARG,
+                             "axs_background_eval: not defined in flat space");
+        }
+    }
+    AXS_CHECK_ARG(bg->a_inv > 1.0, "axs_background_eval: a_inv must be > 1");
+    const Background b = make_background(bg);
+    switch (which)
+    {
+    case 0: *out = b.R(x); break;
+    case 1: *out = b.R_prime(x); break;
+    case 2: *out = b.curvature_term_coeff(x); break;
+    case 3: *out = b.lambda(x); break;
+    case 4: *out = b.H(x); break;
+    case 5: *out = b.H_over_mr_direct(x); break;
+    case 6: *out = b.H_over_mr_closed_form(x); break;
+    case 7: *out = b.tau_from_log_mr_over_h(x); break;
+    case 8: *out = 1.0 / (b.b_inv * x); break; // AxionStringsLevel.cpp:1283
+    default:
+        return set_error(AXS_ERR_ARG, "axs_background_eval: bad selector");
+    }
+    return AXS_OK;
+}
+
+int axs_background_rhs_coeffs(const axs_background *bg, double tau,
+                              axs_rhs_coeffs *out)
+{
+    AXS_CHECK_ARG(bg && out, "axs_background_rhs_coeffs: NULL argument");
+    AXS_CHECK_ARG(bg->flat_space || bg->a_inv > 1.0,
+                  "axs_background_rhs_coeffs: a_inv must be > 1");
+    rhs_coeffs_at(bg, tau, out);
+    return AXS_OK;
+}
+
+int axs_level_create(axs_ctx *ctx, int nx, int ny, int nz, double dx,
+                     int precision, axs_level **out)
+{
+    AXS_CHECK_ARG(ctx && out, "axs_level_create: NULL argument");
+    *out = nullptr;
+    AXS_CHECK_ARG(nx >= 4 && ny >= 4 && nz >= 4,
+                  "axs_level_create: every dimension must be >= 4");
+    AXS_CHECK_ARG((long long)nx * ny * nz * 4 < (1LL << 32),
+                  "axs_level_create: 4*nx*ny*nz must be < 2^32");
+    AXS_CHECK_ARG(dx > 0.0, "axs_level_create: dx must be > 0");
+    AXS_CHECK_ARG(precision == AXS_FP64 || precision == AXS_FP32C ||
+                      precision == AXS_FP32,
+                  "axs_level_create: unknown precision");
+    AXS_HIP_TRY(hipSetDevice(ctx->device));
+    axs_level *L = new axs_level{};
+    L->ctx = ctx;
+    L->g = Grid{nx, ny, nz, (uint32_t)((long long)nx * ny * nz)};
+    L->dx = dx;
+    L->precision = precision;
+    L->word = precision == AXS_FP64 ? 8 : 4;
+    L->tile_variant = -1;
+    L->z_chunk = 0;
+    const size_t bytes4 = (size_t)4 * L->g.ncell * L->word;
+    bool ok = hipMalloc(&L->y, bytes4) == hipSuccess &&
+              hipMalloc(&L->acc, bytes4) == hipSuccess;
+    // ya and yb are one allocation so that together they can hold a
+    // 4-component double array (FP32 transfers, FFT scratch).
+    void *yab = nullptr;
+    ok = ok && hipMalloc(&yab, 2 * bytes4) == hipSuccess;
+    if (ok)
+    {
+        L->ya = yab;
+        L->yb = (char *)yab + bytes4;
+    }
+    if (ok && precision == AXS_FP32C)
+        ok = hipMalloc(&L->comp, bytes4) == hipSuccess;
+    ok = ok &&
+         hipMalloc(&L->d_coeff, (size_t)kCoeffTableSteps * 12 * sizeof(double)) ==
+             hipSuccess &&
+         hipMalloc(&L->d_step, sizeof(int)) == hipSuccess;
+    if (!ok)
+    {
+        hipFree(L->y);
+        hipFree(L->acc);
+        hipFree(yab);
+        hipFree(L->comp);
+        hipFree(L->d_coeff);
+        hipFree(L->d_step);
+        delete L;
+        return set_error(AXS_ERR_NOMEM,
+                         "axs_level_create: device allocation failed");
+    }
+    AXS_HIP_TRY(hipMemsetAsync(L->y, 0, bytes4, ctx->stream));
+    if (L->comp)
+        AXS_HIP_TRY(hipMemsetAsync(L->comp, 0, bytes4, ctx->stream));
+    AXS_HIP_TRY(hipMemsetAsync(L->d_step, 0, sizeof(int), ctx->stream));
+    AXS_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    *out = L;
+    return AXS_OK;
+}
+
+static void destroy_graph(axs_level *L)
+{
+    if (L->graph_valid)
+    {
+        hipGraphExecDestroy(L->graph_exec);
+        hipGraphDestroy(L->graph);
+        L->graph_valid = 0;
+    }
+}
+
+int axs_level_destroy(axs_level *L)
+{
+    if (!L)
+        return AXS_OK;
+    hipSetDevice(L->ctx->device);
+    hipStreamSynchronize(L->ctx->stream);
+    destroy_graph(L);
+    if (L->fft_r2c)
+        hipfftDestroy((hipfftHandle)L->fft_r2c);
+    if (L->fft_c2r)
+        hipfftDestroy((hipfftHandle)L->fft_c2r);
+    hipFree(L->y);
+    hipFree(L->acc);
+    hipFree(L->ya); // yb is part of this allocation
+    hipFree(L->comp);
+    hipFree(L->d_coeff);
+    hipFree(L->d_step);
+    delete L;
+    return AXS_OK;
+}
+
+int axs_level_set_tile(axs_level *L, int variant, int z_chunk)
+{
+    AXS_CHECK_ARG(L, "axs_level_set_tile: NULL level");
+    AXS_CHECK_ARG(variant >= -1 && variant < axs::num_tile_variants(),
+                  "axs_level_set_tile: bad variant");
+    AXS_CHECK_ARG(z_chunk >= 0, "axs_level_set_tile: bad z_chunk");
+    if (variant != L->tile_variant || z_chunk != L->z_chunk)
+        destroy_graph(L);
+    L->tile_variant = variant;
+    L->z_chunk = z_chunk;
+    return AXS_OK;
+}
+
+int axs_level_get_tile(axs_level *L, int *variant, int *z_chunk)
+{
+    AXS_CHECK_ARG(L && variant && z_chunk, "axs_level_get_tile: NULL");
+    *variant = L->tile_variant;
+    *z_chunk = L->z_chunk;
+    return AXS_OK;
+}
+
+int axs_level_device_ptrs(axs_level *L, void **state, void **comp)
+{
+    AXS_CHECK_ARG(L, "axs_level_device_ptrs: NULL level");
+    if (state)
+        *state = L->y;
+    if (comp)
+        *comp = L->comp;
+    return AXS_OK;
+}
+
+int axs_level_upload(axs_level *L, const double *host)
+{
+    AXS_CHECK_ARG(L && host, "axs_level_upload: NULL argument");
+    const size_t n4 = (size_t)4 * L->g.ncell;
+    hipStream_t s = L->ctx->stream;
+    if (L->precision == AXS_FP64)
+    {
+        AXS_HIP_TRY(hipMemcpyAsync(L->y, host, n4 * 8, hipMemcpyHostToDevice, s));
+    }
+    else
+    {
+        // stage the doubles in the (ya|yb) scratch, then split into hi/lo
+        AXS_HIP_TRY(
+            hipMemcpyAsync(L->ya, host, n4 * 8, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_upload_convert, dim3(grid_for(n4)), dim3(256), 0,
+                           s, (const double *)L->ya, (float *)L->y, L->comp, n4,
+                           L->comp ? 1 : 0);
+        AXS_HIP_TRY(hipGetLastError());
+    }
+    AXS_HIP_TRY(hipStreamSynchronize(s));
+    return AXS_OK;
+}
+
+int axs_level_download(axs_level *L, double *host)
+{
+    AXS_CHECK_ARG(L && host, "axs_level_download: NULL argument");
+    const size_t n4 = (size_t)4 * L->g.ncell;
+    hipStream_t s = L->ctx->stream;
+    if (L->precision == AXS_FP64)
+    {
+        AXS_HIP_TRY(hipMemcpyAsync(host, L->y, n4 * 8, hipMemcpyDeviceToHost, s));
+    }
+    else
+    {
+        hipLaunchKernelGGL(k_download_convert, dim3(grid_for(n4)), dim3(256), 0,
+                           s, (const float *)L->y, (const float *)L->comp,
+                           (double *)L->ya, n4);
+        AXS_HIP_TRY(hipGetLastError());
+        AXS_HIP_TRY(
+            hipMemcpyAsync(host, L->ya, n4 * 8, hipMemcpyDeviceToHost, s));
+    }
+    AXS_HIP_TRY(hipStreamSynchronize(s));
+    return AXS_OK;
+}
+
+int axs_level_download_raw(axs_level *L, void *hi, void *lo)
+{
+    AXS_CHECK_ARG(L && hi, "axs_level_download_raw: NULL argument");
+    const size_t bytes4 = (size_t)4 * L->g.ncell * L->word;
+    hipStream_t s = L->ctx->stream;
+    AXS_HIP_TRY(hipMemcpyAsync(hi, L->y, bytes4, hipMemcpyDeviceToHost, s));
+    if (lo && L->comp)
+        AXS_HIP_TRY(
+            hipMemcpyAsync(lo, L->comp, bytes4, hipMemcpyDeviceToHost, s));
+    AXS_HIP_TRY(hipStreamSynchronize(s));
+    return AXS_OK;
+}
+
+int axs_level_set_homogeneous(axs_level *L, double R, double Rp)
+{
+    AXS_CHECK_ARG(L, "axs_level_set_homogeneous: NULL level");
+    const size_t n = L->g.ncell;
+    hipStream_t s = L->ctx->stream;
+    if (L->precision == AXS_FP64)
+        hipLaunchKernelGGL(k_set_homogeneous<double>, dim3(grid_for(n)),
+                           dim3(256), 0, s, (double *)L->y, (float *)nullptr, n,
+                           R, Rp);
+    else
+        hipLaunchKernelGGL(k_set_homogeneous<float>, dim3(grid_for(n)),
+                           dim3(256), 0, s, (float *)L->y, L->comp, n, R, Rp);
+    AXS_HIP_TRY(hipGetLastError());
+    return AXS_OK;
+}
+
+int axs_level_zero_components(axs_level *L, int c0, int ncomp)
+{
+    AXS_CHECK_ARG(L, "axs_level_zero_components: NULL level");
+    AXS_CHECK_ARG(c0 >= 0 && ncomp >= 0 && c0 + ncomp <= 4,
+                  "axs_level_zero_components: bad component range");
+    const size_t n = L->g.ncell;
+    const size_t b = (size_t)c0 * n, e = (size_t)(c0 + ncomp) * n;
+    hipStream_t s = L->ctx->stream;
+    if (L->precision == AXS_FP64)
+        hipLaunchKernelGGL(k_zero_range<double>, dim3(grid_for(e - b)),
+                           dim3(256), 0, s, (double *)L->y, (float *)nullptr, b,
+                           e);
+    else
+        hipLaunchKernelGGL(k_zero_range<float>, dim3(grid_for(e - b)), dim3(256),
+                           0, s, (float *)L->y, L->comp, b, e);
+    AXS_HIP_TRY(hipGetLastError());
+    return AXS_OK;
+}
+
+int axs_level_compare(axs_level *a, axs_level *b, double *max_dpsi,
+                      double *max_psi_b)
+{
+    AXS_CHECK_ARG(a && b && max_dpsi && max_psi_b,
+                  "axs_level_compare: NULL argument");
+    AXS_CHECK_ARG(a->g.nx == b->g.nx && a->g.ny == b->g.ny &&
+                      a->g.nz == b->g.nz,
+                  "axs_level_compare: shapes differ");
+    AXS_CHECK_ARG(a->ctx == b->ctx, "axs_level_compare: different contexts");
+    const size_t n = a->g.ncell;
+    hipStream_t s = a->ctx->stream;
+    const int nb = grid_for(n) > 4096 ? 4096 : grid_for(n);
+    double *part = a->ctx->d_partials;
+    const bool a64 = a->precision == AXS_FP64, b64 = b->precision == AXS_FP64;
+    if (a64 && b64)
+        hipLaunchKernelGGL((k_compare<double, double>), dim3(nb), dim3(256), 0, s,
+                           (const double *)a->y, nullptr, (const double *)b->y,
+                           nullptr, n, part);
+    else if (!a64 && b64)
+        hipLaunchKernelGGL((k_compare<float, double>), dim3(nb), dim3(256), 0, s,
+                           (const float *)a->y, a->comp, (const double *)b->y,
+                           nullptr, n, part);
+    else if (a64 && !b64)
+        hipLaunchKernelGGL((k_compare<double, float>), dim3(nb), dim3(256), 0, s,
+                           (const double *)a->y, nullptr, (const float *)b->y,
+                           b->comp, n, part);
+    else
+        hipLaunchKernelGGL((k_compare<float, float>), dim3(nb), dim3(256), 0, s,
+                           (const float *)a->y, a->comp, (const float *)b->y,
+                           b->comp, n, part);
+    AXS_HIP_TRY(hipGetLastError());
+    std::vector<double> h(2 * nb);
+    AXS_HIP_TRY(hipMemcpyAsync(h.data(), part, h.size() * sizeof(double),
+                               hipMemcpyDeviceToHost, s));
+    AXS_HIP_TRY(hipStreamSynchronize(s));
+    double m = 0.0, mb = 0.0;
+    for (int i = 0; i < nb; ++i)
+    {
+        m = std::fmax(m, h[2 * i]);
+        mb = std::fmax(mb, h[2 * i + 1]);
+    }
+    *max_dpsi = m;
+    *max_psi_b = mb;
+    return AXS_OK;
+}
+
+// ------------------------------------------------------------ RHS / RK4 --
+static int upload_coeffs(axs_level *L, const double *h, size_t count,
+                         double *dst)
+{
+    AXS_HIP_TRY(hipMemcpyAsync(dst, h, count * sizeof(double),
+                               hipMemcpyHostToDevice, L->ctx->stream));
+    // the host buffer may be a temporary: make the copy complete first
+    AXS_HIP_TRY(hipStreamSynchronize(L->ctx->stream));
+    return AXS_OK;
+}
+
+int axs_rhs_eval(axs_level *L, const axs_rhs_coeffs *c, void *rhs)
+{
+    AXS_CHECK_ARG(L && c && rhs, "axs_rhs_eval: NULL argument");
+    destroy_graph(L); // the coefficient table row 0 is overwritten
+    const double h[3] = {c->curvature_coeff, c->lambda, c->R_squared};
+    int rc = upload_coeffs(L, h, 3, L->d_coeff);
+    if (rc)
+        return rc;
+    return launch_rhs_eval(L, L->d_coeff, rhs);
+}
+
+static int run_step(axs_level *L, const int *step_ptr, double dt)
+{
+    int rc;
+    if ((rc = launch_rk_stage(L, 1, L->y, L->ya, L->d_coeff, step_ptr, 0, dt)))
+        return rc;
+    if ((rc = launch_rk_stage(L, 2, L->ya, L->yb, L->d_coeff, step_ptr, 1, dt)))
+        return rc;
+    if ((rc = launch_rk_stage(L, 3, L->yb, L->ya, L->d_coeff, step_ptr, 2, dt)))
+        return rc;
+    return launch_rk_stage(L, 4, L->ya, L->y, L->d_coeff, step_ptr, 3, dt);
+}
+
+int axs_rk4_step(axs_level *L, const axs_rhs_coeffs coeffs[4], double dt)
+{
+    AXS_CHECK_ARG(L && coeffs, "axs_rk4_step: NULL argument");
+    destroy_graph(L);
+    double h[12];
+    for (int s = 0; s < 4; ++s)
+    {
+        h[3 * s] = coeffs[s].curvature_coeff;
+        h[3 * s + 1] = coeffs[s].lambda;
+        h[3 * s + 2] = coeffs[s].R_squared;
+    }
+    int rc = upload_coeffs(L, h, 12, L->d_coeff);
+    if (rc)
+        return rc;
+    return run_step(L, nullptr, dt);
+}
+
+static int capture_graph(axs_level *L, double dt)
+{
+    destroy_graph(L);
+    hipStream_t s = L->ctx->stream;
+    AXS_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int rc = run_step(L, L->d_step, dt);
+    if (rc == AXS_OK)
+        rc = launch_step_advance(L, 0);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc != AXS_OK)
+    {
+        if (g)
+            hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess)
+        return set_error(AXS_ERR_HIP, std::string("hipStreamEndCapture: ") +
+                                          hipGetErrorString(e));
+    hipGraphExec_t ge = nullptr;
+    e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    if (e != hipSuccess)
+    {
+        hipGraphDestroy(g);
+        return set_error(AXS_ERR_HIP, std::string("hipGraphInstantiate: ") +
+                                          hipGetErrorString(e));
+    }
+    L->graph = g;
+    L->graph_exec = ge;
+    L->graph_valid = 1;
+    L->graph_key[0] = dt;
+    return AXS_OK;
+}
+
+int axs_rk4_advance(axs_level *L, const axs_background *bg, double tau,
+                    double dt, int nsteps, int use_graph)
+{
+    AXS_CHECK_ARG(L && bg, "axs_rk4_advance: NULL argument");
+    AXS_CHECK_ARG(nsteps >= 0, "axs_rk4_advance: nsteps must be >= 0");
+    AXS_CHECK_ARG(dt > 0.0, "axs_rk4_advance: dt must be > 0");
+    AXS_CHECK_ARG(bg->flat_space || bg->a_inv > 1.0,
+                  "axs_rk4_advance: a_inv must be > 1");
+    std::vector<double> table;
+    int done = 0;
+    while (done < nsteps)
+    {
+        const int chunk = std::min(nsteps - done, kCoeffTableSteps);
+        table.assign((size_t)chunk * 12, 0.0);
+        for (int i = 0; i < chunk; ++i)
+        {
+            // stage times tau, tau+dt/2, tau+dt/2, tau+dt; tau += dt per step
+            const double ts[4] = {tau, tau + 0.5 * dt, tau + 0.5 * dt,
+                                  tau + dt};
+            for (int s = 0; s < 4; ++s)
+            {
+                axs_rhs_coeffs c;
+                rhs_coeffs_at(bg, ts[s], &c);
+                table[(size_t)i * 12 + 3 * s] = c.curvature_coeff;
+                table[(size_t)i * 12 + 3 * s + 1] = c.lambda;
+                table[(size_t)i * 12 + 3 * s + 2] = c.R_squared;
+            }
+            tau += dt;
+        }
+        int rc = upload_coeffs(L, table.data(), table.size(), L->d_coeff);
+        if (rc)
+            return rc;
+        if ((rc = launch_step_advance(L, 1)))
+            return rc;
+        if (use_graph)
+        {
+            if (!L->graph_valid || L->graph_key[0] != dt)
+                if ((rc = capture_graph(L, dt)))
+                    return rc;
+            for (int i = 0; i < chunk; ++i)
+                AXS_HIP_TRY(hipGraphLaunch(L->graph_exec, L->ctx->stream));
+        }
+        else
+        {
+            for (int i = 0; i < chunk; ++i)
+            {
+                if ((rc = run_step(L, L->d_step, dt)))
+                    return rc;
+                if ((rc = launch_step_advance(L, 0)))
+                    return rc;
+            }
+        }
+        done += chunk;
+    }
+    return AXS_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,617 @@
+// axs_diag.hip -- string finding, tagging and energy reductions on a uniform
+// periodic level.
+//
+// Windings (StringFinder.hpp:45-122): every vertex phase atan2(psi2, psi1)
+// is evaluated once, in FP64 on the FP64-promoted state exactly like the
+// reference (StringFinder.hpp:49-53), into a 3-deep ring of LDS planes;
+// each cell then reads the 7 corner phases of its 3 low-corner plaquettes
+// from LDS.  Pierced-plaquette counts are formed per wavefront with
+// __ballot / __popcll and one 64-bit integer atomic per wavefront, so the
+// counts are exact and order independent.
+//
+// Energies (EnergyKernel.hpp:106-233): FP64 per-cell arithmetic, per-thread
+// accumulation, then a fixed-order block tree and a fixed-order second pass
+// over the block partials: the 14 sums are bitwise reproducible run to run.
+#include "axs_internal.hpp"
+
+#include <cfloat>
+#include <cmath>
+#include <vector>
+
+namespace axs
+{
+
+template <typename T>
+__device__ __forceinline__ double ld(const T *p, const float *lo, uint32_t i)
+{
+    return lo ? (double)p[i] + (double)lo[i] : (double)p[i];
+}
+
+__device__ __forceinline__ int wrapc(int i, int n)
+{
+    return i < 0 ? i + n : (i >= n ? i - n : i);
+}
+
+// ---- PlaquetteWinding.hpp:24-51, on the device -------------------------
+__device__ __forceinline__ double wrap_angle_diff_d(double delta)
+{
+    const double pi = 3.14159265358979323846;
+    const double two_pi = 2.0 * pi;
+    delta = fmod(delta, two_pi);
+    if (delta <= -pi)
+        delta += two_pi;
+    else if (delta > pi)
+        delta -= two_pi;
+    return delta;
+}
+
+__device__ __forceinline__ int plaquette_winding_d(double t1, double t2,
+                                                   double t3, double t4)
+{
+#pragma clang fp contract(off)
+    const double two_pi = 2.0 * 3.14159265358979323846;
+    const double total = wrap_angle_diff_d(t2 - t1) + wrap_angle_diff_d(t3 - t2) +
+                         wrap_angle_diff_d(t4 - t3) + wrap_angle_diff_d(t1 - t4);
+    return (int)lround(total / two_pi);
+}
+
+// FourthOrderDerivatives diff1 / diff2 (published 4th-order forms)
+__device__ __forceinline__ double d1_ref(double fm2, double fm1, double fp1,
+                                         double fp2, double i1)
+{
+#pragma clang fp contract(off)
+    const double w_far = 8.33333333333333333333e-2;
+    const double w_near = 6.66666666666666666667e-1;
+    return (w_far * fm2 - w_near * fm1 + w_near * fp1 - w_far * fp2) * i1;
+}
+__device__ __forceinline__ double d2_ref(double fm2, double fm1, double f0,
+                                         double fp1, double fp2, double i2)
+{
+#pragma clang fp contract(off)
+    const double w_far = 8.33333333333333333333e-2;
+    const double w_near = 1.33333333333333333333e+0;
+    const double w_local = 2.50000000000000000000e+0;
+    return (-w_far * fm2 + w_near * fm1 - w_local * f0 + w_near * fp1 -
+            w_far * fp2) *
+           i2;
+}
+
+struct StarLoader
+{
+    int nx, ny, nz;
+    uint32_t plane;
+    template <typename T>
+    __device__ __forceinline__ double at(const T *c, const float *lo, int x,
+                                         int y, int z) const
+    {
+        const uint32_t i = (uint32_t)wrapc(z, nz) * plane +
+                           (uint32_t)wrapc(y, ny) * nx + wrapc(x, nx);
+        return ld(c, lo, i);
+    }
+};
+
+// --------------------------------------------------- windings / tagging --
+struct WindParams
+{
+    int nx, ny, nz;
+    uint32_t n;
+    int z_chunk, tiles_x, tiles_y, tiles_z;
+    const uint8_t *mask;    // optional: skip cells with mask == 0
+    int8_t *wfield;         // optional: per-cell windings
+    uint8_t *tags;          // optional: tagger output (only sets)
+    unsigned long long *counts; // [0] plain, [1] weighted
+    // tagger extras
+    double dx, grad_thr, rgrad_thr, amp_thr, R;
+    int use_lap, use_rgrad;
+};
+
+constexpr int WTY = 4;
+
+template <typename T>
+__global__ void __launch_bounds__(64 * WTY)
+    k_windings(const T *y, const float *lo, WindParams p)
+{
+    constexpr int TX = 64;
+    __shared__ double th[3][WTY + 1][TX + 1];
+
+    const int bid = blockIdx.x;
+    const int bx = bid % p.tiles_x;
+    const int rem = bid / p.tiles_x;
+    const int by = rem % p.tiles_y;
+    const int bz = rem / p.tiles_y;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int nx = p.nx, ny = p.ny, nz = p.nz;
+    const uint32_t n = p.n;
+    const uint32_t plane = (uint32_t)nx * ny;
+    const int gx = bx * TX + tx, gy = by * WTY + ty;
+    const bool active = gx < nx && gy < ny;
+    const int xw = gx % nx, yw = gy % ny;
+    const int xh = (bx * TX + TX) % nx;         // x halo column
+    const int yh = (by * WTY + WTY) % ny;       // y halo row
+    const T *p1 = y;
+    const T *p2 = y + n;
+    const float *l1 = lo, *l2 = lo ? lo + n : nullptr;
+
+    auto phase = [&](int xx, int yy, int zz) -> double {
+        const uint32_t i = (uint32_t)zz * plane + (uint32_t)yy * nx + xx;
+        return atan2(ld(p2, l2, i), ld(p1, l1, i));
+    };
+    // fills ring slot s with the phases of plane zz (tile + 1 halo each way)
+    auto fill = [&](int s, int zz) {
+        th[s][ty][tx] = phase(xw, yw, zz);
+        if (tx == 0)
+            th[s][ty][TX] = phase(xh, yw, zz);
+        if (ty == 0)
+        {
+            th[s][WTY][tx] = phase(xw, yh, zz);
+            if (tx == 0)
+                th[s][WTY][TX] = phase(xh, yh, zz);
+        }
+    };
+
+    const int z_begin = bz * p.z_chunk;
+    const int z_end = min(z_begin + p.z_chunk, nz);
+    fill(z_begin % 3, z_begin);
+
+    unsigned long long plain_w = 0; // per wavefront (uniform)
+    unsigned long long weighted = 0; // per lane
+    const StarLoader S{nx, ny, nz, plane};
+    const double i1 = 1.0 / p.dx, i2 = 1.0 / (p.dx * p.dx);
+
+    for (int z = z_begin; z < z_end; ++z)
+    {
+        const int s0 = z % 3, s1 = (z + 1) % 3;
+        fill(s1, wrapc(z + 1, nz));
+        __syncthreads();
+        int wxy = 0, wyz = 0, wzx = 0;
+        bool in = active;
+        const uint32_t id = (uint32_t)z * plane + (uint32_t)gy * nx + gx;
+        if (in && p.mask && p.mask[id] == 0)
+            in = false;
+        if (in)
+        {
+            const double t000 = th[s0][ty][tx];
+            const double t100 = th[s0][ty][tx + 1];
+            const double t110 = th[s0][ty + 1][tx + 1];
+            const double t010 = th[s0][ty + 1][tx];
+            const double t001 = th[s1][ty][tx];
+            const double t101 = th[s1][ty][tx + 1];
+            const double t011 = th[s1][ty + 1][tx];
+            wxy = plaquette_winding_d(t000, t100, t110, t010);
+            wyz = plaquette_winding_d(t000, t010, t011, t001);
+            wzx = plaquette_winding_d(t000, t001, t101, t100);
+            weighted += (unsigned long long)(abs(wxy) + abs(wyz) + abs(wzx));
+            if (p.wfield)
+            {
+                p.wfield[3 * (size_t)id] = (int8_t)wxy;
+                p.wfield[3 * (size_t)id + 1] = (int8_t)wyz;
+                p.wfield[3 * (size_t)id + 2] = (int8_t)wzx;
+            }
+            if (p.tags)
+            {
+                bool tag = (wxy != 0 || wyz != 0 || wzx != 0);
+                const double psi1 = ld(p1, l1, id), psi2 = ld(p2, l2, id);
+                if (!tag && p.amp_thr > 0.0 &&
+                    sqrt(psi1 * psi1 + psi2 * psi2) / p.R < p.amp_thr)
+                    tag = true;
+                if (!tag && p.use_lap)
+                {
+                    const double dx2 = p.dx * p.dx;
+                    double lap[2];
+                    for (int c = 0; c < 2; ++c)
+                    {
+                        const T *q = c ? p2 : p1;
+                        const float *ql = c ? l2 : l1;
+                        const double f0 = ld(q, ql, id);
+                        double r = 0.0;
+                        r += d2_ref(S.at(q, ql, gx - 2, gy, z),
+                                    S.at(q, ql, gx - 1, gy, z), f0,
+                                    S.at(q, ql, gx + 1, gy, z),
+                                    S.at(q, ql, gx + 2, gy, z), i2);
+                        r += d2_ref(S.at(q, ql, gx, gy - 2, z),
+                                    S.at(q, ql, gx, gy - 1, z), f0,
+                                    S.at(q, ql, gx, gy + 1, z),
+                                    S.at(q, ql, gx, gy + 2, z), i2);
+                        r += d2_ref(S.at(q, ql, gx, gy, z - 2),
+                                    S.at(q, ql, gx, gy, z - 1), f0,
+                                    S.at(q, ql, gx, gy, z + 1),
+                                    S.at(q, ql, gx, gy, z + 2), i2);
+                        lap[c] = r;
+                    }
+                    if (dx2 * fabs(lap[0]) > p.grad_thr ||
+                        dx2 * fabs(lap[1]) > p.grad_thr)
+                        tag = true;
+                }
+                if (!tag && p.use_rgrad)
+                {
+                    const double psq = psi1 * psi1 + psi2 * psi2;
+                    double g1sq = 0.0, g2sq = 0.0, gtsq = 0.0;
+                    for (int d = 0; d < 3; ++d)
+                    {
+                        const int ex = d == 0, ey = d == 1, ez = d == 2;
+                        const double a =
+                            d1_ref(S.at(p1, l1, gx - 2 * ex, gy - 2 * ey, z - 2 * ez),
+                                   S.at(p1, l1, gx - ex, gy - ey, z - ez),
+                                   S.at(p1, l1, gx + ex, gy + ey, z + ez),
+                                   S.at(p1, l1, gx + 2 * ex, gy + 2 * ey, z + 2 * ez),
+                                   i1);
+                        const double b =
+                            d1_ref(S.at(p2, l2, gx - 2 * ex, gy - 2 * ey, z - 2 * ez),
+                                   S.at(p2, l2, gx - ex, gy - ey, z - ez),
+                                   S.at(p2, l2, gx + ex, gy + ey, z + ez),
+                                   S.at(p2, l2, gx + 2 * ex, gy + 2 * ey, z + 2 * ez),
+                                   i1);
+                        g1sq += a * a;
+                        g2sq += b * b;
+                        const double dth =
+                            psq > 0.0 ? (psi1 * b - psi2 * a) / psq : 0.0;
+                        gtsq += dth * dth;
+                    }
+                    if (p.dx * sqrt(g1sq + g2sq - psq * gtsq) > p.rgrad_thr)
+                        tag = true;
+                }
+                if (tag)
+                    p.tags[id] = 1;
+            }
+        }
+        // pierced plaquettes of this wavefront, by ballot / popcount
+        plain_w += __popcll(__ballot(wxy != 0)) + __popcll(__ballot(wyz != 0)) +
+                   __popcll(__ballot(wzx != 0));
+    }
+    // wavefront reduction of the per-lane weighted count
+    for (int o = 32; o > 0; o >>= 1)
+        weighted += __shfl_xor(weighted, o, 64);
+    if ((threadIdx.x & 63) == 0)
+    {
+        atomicAdd(&p.counts[0], plain_w);
+        atomicAdd(&p.counts[1], weighted);
+    }
+}
+
+__global__ void k_zero_counts(unsigned long long *c)
+{
+    if (threadIdx.x < 8)
+        c[threadIdx.x] = 0ull;
+}
+
+// ------------------------------------------------------------- energies --
+struct EnergyParams
+{
+    int nx, ny, nz;
+    uint32_t n;
+    double dx, R, lambda, RpR;
+    int scheme;
+    double thr;
+    const uint8_t *mask;
+    double *partials; // [gridDim.x][14]
+};
+
+__device__ __forceinline__ double masking_weight_d(int scheme, double thr,
+                                                   double psi1, double psi2,
+                                                   double R)
+{
+    if (scheme == AXS_MASK_NONE)
+        return 1.0;
+    const double mod = sqrt(psi1 * psi1 + psi2 * psi2) / R;
+    if (scheme == AXS_MASK_B)
+        return (mod >= thr) ? 1.0 : 0.0;
+    const double f_a = 1.4142135623730951;
+    const double r = mod - 1.0;
+    const double f = 1.0 + r / f_a;
+    return f * f;
+}
+
+constexpr int EB = 256;
+
+template <typename T>
+__global__ void __launch_bounds__(EB)
+    k_energy(const T *y, const float *lo, EnergyParams p)
+{
+    const uint32_t n = p.n;
+    const int nx = p.nx, ny = p.ny, nz = p.nz;
+    const uint32_t plane = (uint32_t)nx * ny;
+    const StarLoader S{nx, ny, nz, plane};
+    const T *p1 = y, *p2 = y + n, *q1 = y + 2 * n, *q2 = y + 3 * n;
+    const float *l1 = lo, *l2 = lo ? lo + n : nullptr,
+                *m1 = lo ? lo + 2 * n : nullptr, *m2 = lo ? lo + 3 * n : nullptr;
+    const double i1 = 1.0 / p.dx;
+    const double R = p.R, R2 = R * R, R4 = R2 * R2;
+    const double f_a = 1.4142135623730951;
+    double acc[14];
+#pragma unroll
+    for (int q = 0; q < 14; ++q)
+        acc[q] = 0.0;
+    for (uint32_t id = blockIdx.x * EB + threadIdx.x; id < n;
+         id += gridDim.x * EB)
+    {
+        if (p.mask && p.mask[id] == 0)
+            continue;
+        const int x = id % nx;
+        const int yy = (id / nx) % ny;
+        const int z = id / plane;
+        const double psi1 = ld(p1, l1, id), psi2 = ld(p2, l2, id);
+        const double Pi1 = ld(q1, m1, id), Pi2 = ld(q2, m2, id);
+        const double psi_sq = psi1 * psi1 + psi2 * psi2;
+        double g1sq = 0.0, g2sq = 0.0, gtsq = 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+        {
+            const int ex = d == 0, ey = d == 1, ez = d == 2;
+            const double a =
+                d1_ref(S.at(p1, l1, x - 2 * ex, yy - 2 * ey, z - 2 * ez),
+                       S.at(p1, l1, x - ex, yy - ey, z - ez),
+                       S.at(p1, l1, x + ex, yy + ey, z + ez),
+                       S.at(p1, l1, x + 2 * ex, yy + 2 * ey, z + 2 * ez), i1);
+            const double b =
+                d1_ref(S.at(p2, l2, x - 2 * ex, yy - 2 * ey, z - 2 * ez),
+                       S.at(p2, l2, x - ex, yy - ey, z - ez),
+                       S.at(p2, l2, x + ex, yy + ey, z + ez),
+                       S.at(p2, l2, x + 2 * ex, yy + 2 * ey, z + 2 * ez), i1);
+            g1sq += a * a;
+            g2sq += b * b;
+            const double dth =
+                (psi_sq > 0.0) ? (psi1 * b - psi2 * a) / psi_sq : 0.0;
+            gtsq += dth * dth;
+        }
+        // Energy.hpp:200-325
+        const double c1 = Pi1 - p.RpR * psi1, c2 = Pi2 - p.RpR * psi2;
+        const double dpot = psi1 * psi1 + psi2 * psi2 - R * R;
+        const double pot = 0.25 * p.lambda * dpot * dpot;
+        const double rho = (c1 * c1 + c2 * c2 + (g1sq + g2sq) + pot) / R4;
+        const double tp =
+            (psi_sq > 0.0) ? (psi1 * Pi2 - Pi1 * psi2) / psi_sq : 0.0;
+        const double a_dot = f_a * tp / R;
+        const double rak = 0.5 * a_dot * a_dot;
+        const double rag = 0.5 * f_a * f_a * gtsq / R2;
+        double rrk = 0.0;
+        if (psi_sq > 0.0)
+        {
+            const double rd = (psi1 * c1 + psi2 * c2) / sqrt(psi_sq);
+            rrk = (rd * rd) / R4;
+        }
+        const double rrg = (g1sq + g2sq - psi_sq * gtsq) / R4;
+        const double rrm = pot / R4;
+        const double w = masking_weight_d(p.scheme, p.thr, psi1, psi2, R);
+        acc[0] += rho;
+        acc[1] += rho * w;
+        acc[2] += rak;
+        acc[3] += rak * w;
+        acc[4] += rag;
+        acc[5] += rag * w;
+        acc[6] += rrk;
+        acc[7] += rrk * w;
+        acc[8] += rrg;
+        acc[9] += rrg * w;
+        acc[10] += rrm;
+        acc[11] += rrm * w;
+        acc[12] += w;
+        acc[13] += 1.0;
+    }
+    __shared__ double red[EB];
+    for (int q = 0; q < 14; ++q)
+    {
+        red[threadIdx.x] = acc[q];
+        __syncthreads();
+        for (int o = EB / 2; o > 0; o >>= 1)
+        {
+            if ((int)threadIdx.x < o)
+                red[threadIdx.x] += red[threadIdx.x + o];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0)
+            p.partials[(size_t)blockIdx.x * 14 + q] = red[0];
+        __syncthreads();
+    }
+}
+
+// Sums nrow rows of `width` partials column-wise, in fixed order.
+__global__ void k_sum_partials(const double *part, int nrow, int width,
+                               double *out)
+{
+    __shared__ double red[256];
+    const int col = blockIdx.x;
+    double s = 0.0;
+    for (int r = threadIdx.x; r < nrow; r += 256)
+        s += part[(size_t)r * width + col];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1)
+    {
+        if ((int)threadIdx.x < o)
+            red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        out[col] = red[0];
+}
+
+// ------------------------------------------------------------ field fill --
+template <typename T, typename B>
+__global__ void k_fill(const T *y, const float *lo, uint32_t n, int kind,
+                       int scheme, double thr, double R, B *buf)
+{
+    for (uint32_t id = blockIdx.x * 256 + threadIdx.x; id < n;
+         id += gridDim.x * 256)
+    {
+        double v;
+        if (kind == AXS_FIELD_MASKED_A_DOT)
+        {
+            // masked_a_dot, Masking.hpp:96-118
+            const double psi1 = ld(y, lo, id);
+            const double psi2 = ld(y + n, lo ? lo + n : nullptr, id);
+            const double Pi1 = ld(y + 2 * n, lo ? lo + 2 * n : nullptr, id);
+            const double Pi2 = ld(y + 3 * n, lo ? lo + 3 * n : nullptr, id);
+            const double raw = psi1 * Pi2 - Pi1 * psi2;
+            if (scheme == AXS_MASK_A)
+                v = raw;
+            else if (scheme == AXS_MASK_B &&
+                     masking_weight_d(scheme, thr, psi1, psi2, R) == 0.0)
+                v = 0.0;
+            else
+                v = raw / (psi1 * psi1 + psi2 * psi2);
+        }
+        else
+        {
+            const uint32_t off = (uint32_t)(kind + 1) * n;
+            v = ld(y + off, lo ? lo + off : nullptr, id);
+        }
+        buf[id] = (B)v;
+    }
+}
+
+static int nblocks(size_t n, int bs, int cap)
+{
+    size_t b = (n + bs - 1) / bs;
+    if (b > (size_t)cap)
+        b = cap;
+    return b < 1 ? 1 : (int)b;
+}
+
+// shared launcher for windings / tags
+static int run_windings(axs_level *L, const uint8_t *mask, int8_t *wfield,
+                        uint8_t *tags, const axs_tagger_params *tp,
+                        int64_t out[2])
+{
+    hipStream_t s = L->ctx->stream;
+    WindParams p{};
+    p.nx = L->g.nx;
+    p.ny = L->g.ny;
+    p.nz = L->g.nz;
+    p.n = L->g.ncell;
+    p.mask = mask;
+    p.wfield = wfield;
+    p.tags = tags;
+    p.counts = L->ctx->d_counts;
+    p.dx = L->dx;
+    p.grad_thr = tp ? tp->gradient_threshold : DBL_MAX;
+    p.rgrad_thr = tp ? tp->radial_gradient_threshold : DBL_MAX;
+    p.amp_thr = tp ? tp->amplitude_threshold : 0.0;
+    p.R = tp ? tp->R : 1.0;
+    p.use_lap = (tp && tp->gradient_threshold < DBL_MAX) ? 1 : 0;
+    p.use_rgrad = (tp && tp->radial_gradient_threshold < DBL_MAX) ? 1 : 0;
+    p.tiles_x = (p.nx + 63) / 64;
+    p.tiles_y = (p.ny + WTY - 1) / WTY;
+    int zc = 32;
+    while (zc > 2 && (long)p.tiles_x * p.tiles_y * ((p.nz + zc - 1) / zc) < 4096)
+        zc >>= 1;
+    if (zc > p.nz)
+        zc = p.nz;
+    p.z_chunk = zc;
+    p.tiles_z = (p.nz + zc - 1) / zc;
+    const int ntiles = p.tiles_x * p.tiles_y * p.tiles_z;
+    hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, s, p.counts);
+    if (L->precision == AXS_FP64)
+        hipLaunchKernelGGL(k_windings<double>, dim3(ntiles), dim3(64, WTY), 0,
+                           s, (const double *)L->y, (const float *)nullptr, p);
+    else
+        hipLaunchKernelGGL(k_windings<float>, dim3(ntiles), dim3(64, WTY), 0, s,
+                           (const float *)L->y, (const float *)L->comp, p);
+    AXS_HIP_TRY(hipGetLastError());
+    if (out)
+    {
+        AXS_HIP_TRY(hipMemcpyAsync(L->ctx->h_counts, p.counts,
+                                   2 * sizeof(unsigned long long),
+                                   hipMemcpyDeviceToHost, s));
+        AXS_HIP_TRY(hipStreamSynchronize(s));
+        out[0] = (int64_t)L->ctx->h_counts[0];
+        out[1] = (int64_t)L->ctx->h_counts[1];
+    }
+    return AXS_OK;
+}
+
+} // namespace axs
+
+using namespace axs;
+
+extern "C" {
+
+int axs_count_plaquettes(axs_level *L, const uint8_t *mask, int64_t out[2])
+{
+    AXS_CHECK_ARG(L && out, "axs_count_plaquettes: NULL argument");
+    return run_windings(L, mask, nullptr, nullptr, nullptr, out);
+}
+
+int axs_windings_field(axs_level *L, int8_t *w)
+{
+    AXS_CHECK_ARG(L && w, "axs_windings_field: NULL argument");
+    int64_t c[2];
+    return run_windings(L, nullptr, w, nullptr, nullptr, c);
+}
+
+int axs_tag_cells(axs_level *L, const axs_tagger_params *tp, uint8_t *tags,
+                  int64_t counts[2])
+{
+    AXS_CHECK_ARG(L && tp && tags, "axs_tag_cells: NULL argument");
+    AXS_CHECK_ARG(tp->gradient_threshold > 0.0 &&
+                      tp->radial_gradient_threshold > 0.0,
+                  "axs_tag_cells: thresholds must be > 0");
+    AXS_CHECK_ARG(tp->amplitude_threshold <= 0.0 || tp->R > 0.0,
+                  "axs_tag_cells: amplitude criterion needs R > 0");
+    int64_t c[2];
+    int rc = run_windings(L, nullptr, nullptr, tags, tp, counts ? counts : c);
+    return rc;
+}
+
+int axs_energy_sums(axs_level *L, double R, double lambda, double RpR,
+                    const axs_masking *m, const uint8_t *mask, double out[14])
+{
+    AXS_CHECK_ARG(L && m && out, "axs_energy_sums: NULL argument");
+    AXS_CHECK_ARG(R > 0.0, "axs_energy_sums: R must be > 0");
+    AXS_CHECK_ARG(m->scheme >= 0 && m->scheme <= 2,
+                  "axs_energy_sums: unknown masking scheme");
+    hipStream_t s = L->ctx->stream;
+    EnergyParams p{};
+    p.nx = L->g.nx;
+    p.ny = L->g.ny;
+    p.nz = L->g.nz;
+    p.n = L->g.ncell;
+    p.dx = L->dx;
+    p.R = R;
+    p.lambda = lambda;
+    p.RpR = RpR;
+    p.scheme = m->scheme;
+    p.thr = m->threshold;
+    p.mask = mask;
+    p.partials = L->ctx->d_partials;
+    const int nb = nblocks(p.n, EB, 4096);
+    if (L->precision == AXS_FP64)
+        hipLaunchKernelGGL(k_energy<double>, dim3(nb), dim3(EB), 0, s,
+                           (const double *)L->y, (const float *)nullptr, p);
+    else
+        hipLaunchKernelGGL(k_energy<float>, dim3(nb), dim3(EB), 0, s,
+                           (const float *)L->y, (const float *)L->comp, p);
+    AXS_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_sum_partials, dim3(14), dim3(256), 0, s, p.partials,
+                       nb, 14, L->ctx->d_result);
+    AXS_HIP_TRY(hipGetLastError());
+    AXS_HIP_TRY(hipMemcpyAsync(L->ctx->h_result, L->ctx->d_result,
+                               14 * sizeof(double), hipMemcpyDeviceToHost, s));
+    AXS_HIP_TRY(hipStreamSynchronize(s));
+    for (int q = 0; q < 14; ++q)
+        out[q] = L->ctx->h_result[q];
+    return AXS_OK;
+}
+
+int axs_fill_field(axs_level *L, int kind, const axs_masking *m, double R,
+                   void *buf)
+{
+    AXS_CHECK_ARG(L && m && buf, "axs_fill_field: NULL argument");
+    AXS_CHECK_ARG(kind >= 0 && kind <= 2, "axs_fill_field: unknown kind");
+    AXS_CHECK_ARG(m->scheme >= 0 && m->scheme <= 2,
+                  "axs_fill_field: unknown masking scheme");
+    hipStream_t s = L->ctx->stream;
+    const uint32_t n = L->g.ncell;
+    const int nb = nblocks(n, 256, 8192);
+    if (L->precision == AXS_FP64)
+        hipLaunchKernelGGL((k_fill<double, double>), dim3(nb), dim3(256), 0, s,
+                           (const double *)L->y, (const float *)nullptr, n, kind,
+                           m->scheme, m->threshold, R, (double *)buf);
+    else
+        hipLaunchKernelGGL((k_fill<float, float>), dim3(nb), dim3(256), 0, s,
+                           (const float *)L->y, (const float *)L->comp, n, kind,
+                           m->scheme, m->threshold, R, (float *)buf);
+    AXS_HIP_TRY(hipGetLastError());
+    return AXS_OK;
+}
+
+} // extern "C"

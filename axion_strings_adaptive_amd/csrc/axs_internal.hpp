@@ -1,0 +1,97 @@
+// axs_internal.hpp -- internal types shared by the HIP translation units of
+// libaxs_hip.so (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "../../include/axs.h"
+
+namespace axs
+{
+
+// Records the message returned by axs_last_error() and yields the code.
+int set_error(int code, const std::string &msg);
+
+#define AXS_HIP_TRY(expr)                                                     \
+    do                                                                        \
+    {                                                                         \
+        hipError_t e_ = (expr);                                               \
+        if (e_ != hipSuccess)                                                 \
+            return ::axs::set_error(AXS_ERR_HIP,                              \
+                                    std::string(#expr) + ": " +               \
+                                        hipGetErrorString(e_));               \
+    } while (0)
+
+#define AXS_CHECK_ARG(cond, msg)                                              \
+    do                                                                        \
+    {                                                                         \
+        if (!(cond))                                                          \
+            return ::axs::set_error(AXS_ERR_ARG, (msg));                      \
+    } while (0)
+
+// Maximum RK4 steps whose stage scalars one table upload holds.
+constexpr int kCoeffTableSteps = 4096;
+// Maximum partial-sum slots (blocks) a reduction pass may produce.
+constexpr int kMaxPartials = 65536;
+
+struct Grid
+{
+    int nx, ny, nz;
+    uint32_t ncell; // nx*ny*nz, < 2^31 (checked at creation)
+};
+
+} // namespace axs
+
+struct axs_ctx
+{
+    int device;
+    hipStream_t stream;
+    // small device scratch: counters and reduction partials
+    unsigned long long *d_counts; // [8]
+    double *d_partials;           // [kMaxPartials * 16]
+    double *d_result;             // [64]
+    double *h_result;             // pinned [64]
+    unsigned long long *h_counts; // pinned [8]
+};
+
+struct axs_level
+{
+    axs_ctx *ctx;
+    axs::Grid g;
+    double dx;
+    int precision;
+    size_t word;        // bytes per stored word (8 or 4)
+    void *y;            // y_n: 4 comps
+    float *comp;        // FP32C: low part of y_n (4 comps), else nullptr
+    void *acc;          // RK accumulator (4 comps)
+    void *ya;           // stage state A (4 comps)
+    void *yb;           // stage state B (4 comps)
+    double *d_coeff;    // [kCoeffTableSteps][4][3]
+    int *d_step;        // device step counter (graph replays)
+    int tile_variant;   // stage-kernel tile variant
+    int z_chunk;        // z planes per block
+    // captured one-step graph
+    hipGraph_t graph;
+    hipGraphExec_t graph_exec;
+    int graph_valid;
+    double graph_key[2]; // dt and table identity at capture
+    // FFT plans (lazily created)
+    void *fft_r2c;  // hipfftHandle stored as void*
+    void *fft_c2r;
+    size_t fft_work_bytes;
+};
+
+namespace axs
+{
+// Launchers implemented in the .hip files.
+int launch_rk_stage(axs_level *L, int stage, const void *yin, void *yout,
+                    const double *coeff_table, const int *step_ptr,
+                    int coeff_stage, double dt);
+int launch_rhs_eval(axs_level *L, const double *coeff3_dev, void *rhs);
+int launch_step_advance(axs_level *L, int reset);
+int num_tile_variants();
+} // namespace axs

@@ -1,0 +1,276 @@
+/*
+ * axs.h -- C ABI of libaxs_hip.so, the MI355X (gfx950) implementation of the
+ * axion-string simulator's per-step hot path.
+ *
+ * Every entry point returns an int status (AXS_OK = 0) and never throws.
+ * The text of the last error is available from axs_last_error().  No entry
+ * point allocates device memory except the *_create calls and
+ * axs_level_spectrum's one-time FFT plan creation.
+ *
+ * Data model.  A level (axs_level) is one uniform, periodic grid of
+ * nx*ny*nz cells with spacing dx that owns its state in HBM, structure of
+ * arrays, component-major, x fastest:
+ *     state[c][z][y][x],  c in {psi1, psi2, Pi1, Pi2}
+ * (the reference's StateVariables.hpp:9-17 component order).  Periodic
+ * ghost cells are realised by index wrapping inside the kernels, which is
+ * what the reference's FillBoundary(periodicity) provides
+ * (AxionStringsLevel.cpp:820,1255).  Precision modes:
+ *   AXS_FP64  : double storage and arithmetic;
+ *   AXS_FP32C : float storage, float stage arithmetic, and a float
+ *               compensation array carrying the low part of y_n, updated
+ *               by Kahan summation at each RK4 step (state = hi + lo);
+ *   AXS_FP32  : plain float (the reference's PRECISION=FLOAT build).
+ * A box level (axs_boxlevel) is a list of boxes of one AMR level, each with
+ * its own 3 ghost layers, advanced by one persistent launch per RK stage.
+ *
+ * All calls on one context are ordered on that context's HIP stream.
+ * Calls that return host values (counts, sums, spectra) synchronise that
+ * stream.  A context is not thread-safe.
+ */
+#ifndef AXS_H_
+#define AXS_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------ status ------------------------------ */
+#define AXS_OK 0
+#define AXS_ERR_ARG 1         /* invalid argument / shape / precision */
+#define AXS_ERR_HIP 2         /* HIP runtime or hipFFT failure */
+#define AXS_ERR_NOMEM 3       /* device allocation failed */
+#define AXS_ERR_UNSUPPORTED 4 /* valid request this build does not serve */
+
+#define AXS_FP64 0
+#define AXS_FP32C 1
+#define AXS_FP32 2
+
+/* Masking schemes, in the reference's enum order (Masking.hpp:39-44). */
+#define AXS_MASK_NONE 0
+#define AXS_MASK_A 1
+#define AXS_MASK_B 2
+
+/* Field kinds written into the spectrum buffer. */
+#define AXS_FIELD_MASKED_A_DOT 0 /* fill_masked_a_dot_buffer (reference) */
+#define AXS_FIELD_PI1 1          /* Pi1, for the |Pi|^2 spectrum */
+#define AXS_FIELD_PI2 2          /* Pi2 */
+
+typedef struct axs_ctx axs_ctx;
+typedef struct axs_level axs_level;
+typedef struct axs_boxlevel axs_boxlevel;
+
+/* Background of Background.hpp:21-58 (CTauSchedule + a_inv), plus the
+ * flat-space background of FlatBackground.hpp:26-44. */
+typedef struct
+{
+    double a_inv;
+    double c0;
+    double c1;
+    double tau_switch;
+    int has_switch;
+    int flat_space;  /* 1: R = 1, curvature 0, lambda = m_r^2 */
+    double flat_m_r; /* FlatBackground m_r (flat_space only) */
+} axs_background;
+
+/* The three scalars of AxionStringsRHS's constructor
+ * (AxionStringsRHS.hpp:32-37): curvature_coeff, lambda, R_squared. */
+typedef struct
+{
+    double curvature_coeff;
+    double lambda;
+    double R_squared;
+} axs_rhs_coeffs;
+
+/* StringTaggerParams (StringTagger.hpp:83-111) plus one extension: an
+ * amplitude criterion |psi|/R < amplitude_threshold (0 = off, the default,
+ * which reproduces the reference tagger exactly). */
+typedef struct
+{
+    double gradient_threshold;        /* DBL_MAX = off (reference default) */
+    double radial_gradient_threshold; /* DBL_MAX = off (reference default) */
+    double amplitude_threshold;       /* extension; 0 = off */
+    double R;                         /* R(tau), used by the extension */
+} axs_tagger_params;
+
+/* MaskingParams (Masking.hpp:46-54). */
+typedef struct
+{
+    int scheme;       /* AXS_MASK_* */
+    double threshold; /* scheme B top-hat on |psi|/R */
+} axs_masking;
+
+/* ------------------------- context / errors ------------------------- */
+/* Creates a context on HIP device `device` with its own stream. */
+int axs_ctx_create(int device, axs_ctx **out);
+int axs_ctx_destroy(axs_ctx *ctx);
+int axs_ctx_synchronize(axs_ctx *ctx);
+/* The HIP stream the context launches on (a hipStream_t). */
+void *axs_ctx_stream(axs_ctx *ctx);
+const char *axs_last_error(void);
+/* Number of compiled stage-kernel tile variants (for the autotuner). */
+int axs_num_tile_variants(void);
+
+/* ------------------------- host background -------------------------- */
+/* Background.hpp:53-133.  which: 0 R, 1 R', 2 curvature_term_coeff,
+ * 3 lambda, 4 H, 5 H_over_mr_direct, 6 H_over_mr_closed_form,
+ * 7 tau_from_log_mr_over_h (x is log(m_r/H)), 8 R'/R. */
+int axs_background_eval(const axs_background *bg, int which, double x,
+                        double *out);
+/* The AxionStringsRHS scalars at conformal time tau, as
+ * AxionStringsLevel::specific_eval_rhs computes them
+ * (AxionStringsLevel.cpp:859-885, evolving phase or flat space). */
+int axs_background_rhs_coeffs(const axs_background *bg, double tau,
+                              axs_rhs_coeffs *out);
+
+/* ------------------------------ levels ------------------------------ */
+int axs_level_create(axs_ctx *ctx, int nx, int ny, int nz, double dx,
+                     int precision, axs_level **out);
+int axs_level_destroy(axs_level *lvl);
+/* Copies a host double SoA state (4*nx*ny*nz) in / out.  FP32C stores the
+ * value as hi + lo (lo in the compensation array); download returns
+ * hi + lo. */
+int axs_level_upload(axs_level *lvl, const double *host_state);
+int axs_level_download(axs_level *lvl, double *host_state);
+/* Download of the raw stored words: FP64 -> 4n doubles; FP32/FP32C -> 4n
+ * floats of hi (and, for FP32C, 4n floats of lo into `lo` if non-NULL). */
+int axs_level_download_raw(axs_level *lvl, void *hi, void *lo);
+/* Device pointers of the state (y_n) and compensation arrays. */
+int axs_level_device_ptrs(axs_level *lvl, void **state, void **comp);
+/* Homogeneous exact solution psi = (R, 0), Pi = (R', 0)
+ * (AxionStringsLevel.cpp:824-847). */
+int axs_level_set_homogeneous(axs_level *lvl, double R, double R_prime);
+/* Selects the stage-kernel tile variant (autotuner output); -1 = default. */
+int axs_level_set_tile(axs_level *lvl, int variant, int z_chunk);
+int axs_level_get_tile(axs_level *lvl, int *variant, int *z_chunk);
+
+/* ------------------- RHS and RK4 (the hot path) --------------------- */
+/* AxionStringsRHS<>::operator() over every cell of a periodic level
+ * (AxionStringsRHS.impl.hpp:25-60 launched as in
+ * AxionStringsLevel.cpp:909-918): rhs = (Pi1, Pi2, lap psi1 + c psi1 -
+ * (lambda/2)(|psi|^2 - R^2) psi1, ...).  `rhs` is a device array of
+ * 4*nx*ny*nz words of the level's precision (float for FP32C/FP32). */
+int axs_rhs_eval(axs_level *lvl, const axs_rhs_coeffs *c, void *rhs);
+/* One classic RK4 step with explicit stage scalars (stage s at tau,
+ * tau+dt/2, tau+dt/2, tau+dt): four fused RHS + RK-stage launches.  This
+ * replaces GRTeclyn's RK4 stage driver around specific_eval_rhs
+ * (AxionStringsLevel.cpp:850-921). */
+int axs_rk4_step(axs_level *lvl, const axs_rhs_coeffs coeffs[4], double dt);
+/* nsteps RK4 steps from conformal time tau with the background's stage
+ * scalars (host FP64), uploaded once as a table.  use_graph = 1 replays a
+ * captured hipGraph of one step (4 stages + step-counter advance). */
+int axs_rk4_advance(axs_level *lvl, const axs_background *bg, double tau,
+                    double dt, int nsteps, int use_graph);
+
+/* ------------------------ strings / tagging ------------------------- */
+/* count_plaquettes (StringFinder.hpp:76-122): out[0] = n_p_plain,
+ * out[1] = n_p_weighted.  mask: optional device uint8 per cell (0 skips). */
+int axs_count_plaquettes(axs_level *lvl, const uint8_t *mask, int64_t out[2]);
+/* plaquette_windings_at for every cell (StringFinder.hpp:45-68), into a
+ * device int8 array w[cell*3 + {xy, yz, zx}]. */
+int axs_windings_field(axs_level *lvl, int8_t *w);
+/* StringTagger::operator() for every cell (StringTagger.hpp:240-274):
+ * sets tags[cell] = 1 where a criterion holds, never clears.  tags is a
+ * device uint8 array.  counts (optional, host) receives the plaquette
+ * counts of the same pass. */
+int axs_tag_cells(axs_level *lvl, const axs_tagger_params *p, uint8_t *tags,
+                  int64_t counts[2]);
+
+/* ----------------------------- energies ----------------------------- */
+/* compute_total_energy_sums (EnergyKernel.hpp:106-233): the 14 raw sums in
+ * TotalEnergySums order. mask: optional device uint8 per cell. */
+int axs_energy_sums(axs_level *lvl, double R, double lambda,
+                    double R_prime_over_R, const axs_masking *m,
+                    const uint8_t *mask, double out[14]);
+
+/* ------------------------- spectrum / masking ----------------------- */
+/* fill_masked_a_dot_buffer (MaskedFieldBuffer.hpp:15-35) and the Pi1/Pi2
+ * fill variants, into a device array of nx*ny*nz words of the level's
+ * arithmetic type (double for FP64, float otherwise). */
+int axs_fill_field(axs_level *lvl, int kind, const axs_masking *m, double R,
+                   void *buf);
+/* compute_spectrum (SpectrumKernel.hpp:52-153) of the level's field `kind`
+ * (filled internally into the level's stage scratch, R2C on the device,
+ * then shell binning on the device).  Accumulates (+=) into
+ * shell_sum[n_shells], shell_count[n_shells], tot[2] = {full_cube,
+ * inscribed_sphere}; n_shells = max(nx,ny,nz)/2 + 1.  Also returns the
+ * real-space sum of squares of the filled buffer in *sum_sq if non-NULL. */
+int axs_level_spectrum(axs_level *lvl, int kind, const axs_masking *m,
+                       double R, double *shell_sum, int64_t *shell_count,
+                       double tot[2], double *sum_sq);
+
+/* ---------------------------- initial data -------------------------- */
+/* FourierIC::generate (FourierIC.hpp:237-285) into component `comp`
+ * (0 psi1 / 1 psi2) of the level, on the device: hash-seeded spectral
+ * fill, Hermitian projection of the self-conjugate planes, C2R, real-space
+ * rescale to target_mean_square.  Uses the stage scratch as FFT memory. */
+int axs_level_fourier_ic(axs_level *lvl, int comp, double k_max_cells,
+                         double target_mean_square, uint64_t seed,
+                         int comp_seed_offset);
+/* Sets components c0..c0+ncomp-1 to zero (the reference's
+ * state_new.setVal(0.0, c_Pi1, 2, 0), AxionStringsLevel.cpp:818). */
+int axs_level_zero_components(axs_level *lvl, int c0, int ncomp);
+
+/* --------------------- comparisons (FP32C vs FP64) ------------------ */
+/* max over cells of |psi_a - psi_b| (complex modulus of the difference)
+ * and the max of |psi_b|, for two levels of equal shape. */
+int axs_level_compare(axs_level *a, axs_level *b, double *max_dpsi,
+                      double *max_psi_b);
+
+/* -------------------------- AMR box levels -------------------------- */
+/* A level made of nboxes boxes, box b covering cells lo[3b..3b+2] to
+ * hi[3b..3b+2] (inclusive) of a periodic index space of size
+ * domain[0..2], each box stored with nghost (>= 3) ghost layers,
+ * component-major per box.  Boxes must not overlap. */
+int axs_boxlevel_create(axs_ctx *ctx, int nboxes, const int *lo,
+                        const int *hi, const int domain[3], int nghost,
+                        double dx, int precision, axs_boxlevel **out);
+int axs_boxlevel_destroy(axs_boxlevel *bl);
+/* Number of cells (valid, without ghosts) over all boxes. */
+int64_t axs_boxlevel_num_cells(axs_boxlevel *bl);
+/* Copies values from / to a dense periodic host array of the whole domain
+ * (4 * domain cells, doubles); cells outside every box are ignored. */
+int axs_boxlevel_upload_dense(axs_boxlevel *bl, const double *dense);
+int axs_boxlevel_download_dense(axs_boxlevel *bl, double *dense);
+/* Fills each box's ghost cells: from overlapping boxes of the same level
+ * where they exist, else from the coarse level `coarse` (ratio 2) by
+ * tensor-product quartic interpolation in space of a time-interpolated
+ * coarse state (coarse_old*(1-t_frac) + coarse_new*t_frac; pass
+ * coarse_old = NULL for t_frac = 1).  coarse may be NULL when the boxes
+ * tile the whole periodic domain. */
+int axs_boxlevel_fill_ghosts(axs_boxlevel *bl, const axs_level *coarse,
+                             double t_frac);
+/* One RK4 step of every box of the level, each stage one persistent
+ * launch over the box work-list; same-level ghost fill between stages. */
+int axs_boxlevel_rk4_step(axs_boxlevel *bl, const axs_rhs_coeffs coeffs[4],
+                          double dt);
+/* Tags of every box cell (StringTagger semantics) into a dense device
+ * uint8 array of the whole domain; returns plaquette counts. */
+int axs_boxlevel_tag(axs_boxlevel *bl, const axs_tagger_params *p,
+                     uint8_t *dense_tags, int64_t counts[2]);
+
+/* ------------------------------ adjoint ----------------------------- */
+/* Gradient of J = sum_cells (|grad psi1|^2 + |grad psi2|^2) dx^3 / R_f^4
+ * (the gradient term of rho_tot_pointwise, Energy.hpp:205-215, volume
+ * summed as in EnergyKernel.hpp:301-316) at the end of nsteps RK4 steps,
+ * with respect to the initial state (4n doubles, written to grad_state)
+ * and to lambda0 (the overall scale of lambda(tau), *grad_lambda0).
+ * Forward in the level's precision, adjoint accumulated in FP64,
+ * checkpoints every `checkpoint_every` steps with recompute in between.
+ * On return the level holds the final state; *J receives the objective. */
+int axs_adjoint_grad_energy(axs_level *lvl, const axs_background *bg,
+                            double lambda0, double tau, double dt,
+                            int nsteps, int checkpoint_every, double *J,
+                            double *grad_state, double *grad_lambda0);
+/* Forward only: the objective J after nsteps (same definition). */
+int axs_forward_grad_energy(axs_level *lvl, const axs_background *bg,
+                            double lambda0, double tau, double dt,
+                            int nsteps, double *J);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AXS_H_ */

@@ -1,0 +1,137 @@
+// ref_shim.cpp -- TEST INFRASTRUCTURE ONLY.
+//
+// Exposes the reference's own AMReX-free headers, compiled from source where
+// they lie (AxionStrings/ of the reference checkout, passed with -I by
+// oracle/Makefile), behind a C ABI, so tests/ can check the CPU oracle
+// (axs_oracle.c) and the product's host Background against the reference
+// code itself.  Only headers that compile with nothing but <cmath> are used
+// (Background.hpp, PlaquetteWinding.hpp, Masking.hpp, Energy.hpp,
+// XiFormula.hpp); everything else in the reference needs the absent
+// GRTeclyn/AMReX submodules and is not built.  The output library goes to
+// oracle/_ref/ and is never shipped in git.
+#include "Background.hpp"
+#include "Energy.hpp"
+#include "Masking.hpp"
+#include "PlaquetteWinding.hpp"
+#include "XiFormula.hpp"
+
+namespace
+{
+Background make_bg(double a_inv, double c0, double c1, double tau_switch,
+                   int has_switch)
+{
+    CTauSchedule s{};
+    s.c0 = c0;
+    s.c1 = c1;
+    s.tau_switch = tau_switch;
+    s.has_switch = has_switch != 0;
+    return Background(a_inv, s);
+}
+} // namespace
+
+extern "C"
+{
+    // which: 0 R, 1 R', 2 curvature, 3 lambda, 4 H, 5 H/m_r direct,
+    //        6 H/m_r closed form, 7 tau_from_log_mr_over_h (tau = log value)
+    double ref_background(double a_inv, double c0, double c1,
+                          double tau_switch, int has_switch, int which,
+                          double tau)
+    {
+        const Background b = make_bg(a_inv, c0, c1, tau_switch, has_switch);
+        switch (which)
+        {
+        case 0:
+            return b.R(tau);
+        case 1:
+            return b.R_prime(tau);
+        case 2:
+            return b.curvature_term_coeff(tau);
+        case 3:
+            return b.lambda(tau);
+        case 4:
+            return b.H(tau);
+        case 5:
+            return b.H_over_mr_direct(tau);
+        case 6:
+            return b.H_over_mr_closed_form(tau);
+        case 7:
+            return b.tau_from_log_mr_over_h(tau);
+        default:
+            return 0.0;
+        }
+    }
+
+    int ref_plaquette_winding(double t1, double t2, double t3, double t4)
+    {
+        return plaquette_winding(t1, t2, t3, t4);
+    }
+
+    double ref_wrap_angle_diff(double d) { return wrap_angle_diff(d); }
+
+    double ref_masking_weight(int scheme, double thr, double psi1, double psi2,
+                              double R)
+    {
+        MaskingParams p{};
+        p.scheme = scheme == 0   ? MaskingScheme::None
+                   : scheme == 1 ? MaskingScheme::A
+                                 : MaskingScheme::B;
+        p.threshold = thr;
+        return masking_weight(p, psi1, psi2, R);
+    }
+
+    double ref_masked_a_dot(int scheme, double thr, double psi1, double psi2,
+                            double Pi1, double Pi2, double R)
+    {
+        MaskingParams p{};
+        p.scheme = scheme == 0   ? MaskingScheme::None
+                   : scheme == 1 ? MaskingScheme::A
+                                 : MaskingScheme::B;
+        p.threshold = thr;
+        return masked_a_dot(p, psi1, psi2, Pi1, Pi2, R);
+    }
+
+    double ref_rho_tot_pointwise(double psi1, double psi2, double Pi1,
+                                 double Pi2, double g1, double g2, double R,
+                                 double lambda, double RpR)
+    {
+        return rho_tot_pointwise(psi1, psi2, Pi1, Pi2, g1, g2, R, lambda, RpR);
+    }
+    double ref_axion_kinetic(double tp, double R)
+    {
+        return axion_kinetic_energy_pointwise(tp, R);
+    }
+    double ref_axion_gradient(double g, double R)
+    {
+        return axion_gradient_energy_pointwise(g, R);
+    }
+    double ref_radial_kinetic(double psi1, double psi2, double Pi1, double Pi2,
+                              double R, double RpR)
+    {
+        return radial_kinetic_energy_pointwise(psi1, psi2, Pi1, Pi2, R, RpR);
+    }
+    double ref_radial_gradient(double g1, double g2, double psq, double gt,
+                               double R)
+    {
+        return radial_gradient_energy_pointwise(g1, g2, psq, gt, R);
+    }
+    double ref_radial_mass(double psi1, double psi2, double R, double lambda)
+    {
+        return radial_mass_energy_pointwise(psi1, psi2, R, lambda);
+    }
+
+    double ref_xi_from_comoving_length(double ell, double L, double a_inv,
+                                       double tau)
+    {
+        return xi_from_comoving_length(ell, L, a_inv, tau);
+    }
+    double ref_xi_from_plaquette_count(double N_p, double dx, double L,
+                                       double a_inv, double tau)
+    {
+        return xi_from_plaquette_count(N_p, dx, L, a_inv, tau);
+    }
+    double ref_plaquette_count_for_target_xi(double xi, int N, double L,
+                                             double a_inv, double tau)
+    {
+        return plaquette_count_for_target_xi(xi, N, L, a_inv, tau);
+    }
+}

@@ -1,0 +1,283 @@
+"""Parity of the HIP path (libaxs_hip.so, through its C ABI) with the CPU
+oracle on the same seeded inputs.
+
+Bars: FP64 RHS / RK4 bit-exact (same operation order, contraction off);
+windings, counts and tags bit-exact; energy sums, spectra and the IC within
+1e-12 .. 1e-10 relative (reduction order and FFT rounding differ)."""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+DBL_MAX = np.finfo(np.float64).max
+
+
+def _capi():
+    from axion_strings_adaptive_amd import capi
+    return capi
+
+
+def random_state(shape, seed=0, amp=1.0):
+    nx, ny, nz = shape
+    rng = np.random.default_rng(seed)
+    return amp * rng.normal(size=(4, nz, ny, nx))
+
+
+def seeded(oracle, n, seed=42):
+    return oracle.seeded_ic(n, seed=seed)
+
+
+@pytest.mark.parametrize("shape", [(32, 32, 32), (64, 64, 64), (40, 24, 36),
+                                   (4, 8, 12), (128, 16, 8)])
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_rhs_fp64_bitexact(ctx, oracle, shape, variant):
+    capi = _capi()
+    s = random_state(shape, seed=sum(shape))
+    lvl = capi.Level(ctx, shape, 0.5, capi.AXS_FP64)
+    lvl.set_tile(variant, 0)
+    lvl.upload(s)
+    c = capi.RhsCoeffs(0.013, 0.7, 1.9)
+    got = lvl.rhs(c)
+    want = oracle.rhs(s, 0.5, 0.013, 0.7, 1.9)
+    assert np.array_equal(got, want)
+
+
+def test_rhs_fp32_close(ctx, oracle):
+    capi = _capi()
+    shape = (32, 32, 32)
+    s = random_state(shape, seed=3)
+    lvl = capi.Level(ctx, shape, 0.5, capi.AXS_FP32C)
+    lvl.upload(s)
+    got = lvl.rhs(capi.RhsCoeffs(0.0, 0.7, 1.0))
+    s32 = lvl.download_raw()[0].astype(np.float64).reshape(s.shape)
+    want = oracle.rhs(s32, 0.5, 0.0, 0.7, 1.0)
+    scale = np.max(np.abs(want))
+    assert np.max(np.abs(got - want)) / scale < 1e-6
+
+
+def test_rk4_fp64_bitexact_config1(ctx, oracle):
+    """64^3 seeded IC, c0 = 0, dtau = 0.1: 10 RK4 steps, direct and graph."""
+    capi = _capi()
+    n, dx, dt = 64, 0.5, 0.1
+    s0 = seeded(oracle, n)
+    ob = oracle.bg(2.0)
+    bg = capi.make_background(2.0)
+    want = s0.copy()
+    oracle.advance(want, dx, ob, 1.0, dt, 10)
+    for use_graph in (False, True):
+        lvl = capi.Level(ctx, n, dx, capi.AXS_FP64)
+        lvl.upload(s0)
+        lvl.advance(bg, 1.0, dt, 10, use_graph=use_graph)
+        got = lvl.download()
+        assert np.array_equal(got, want), f"graph={use_graph}"
+
+
+def test_rk4_step_explicit_coeffs_moore(ctx, oracle):
+    """Explicit stage scalars (axs_rk4_step) across the fat->Moore switch."""
+    capi = _capi()
+    n, dx, dt = 32, 0.5, 0.1
+    s = seeded(oracle, n, seed=7)
+    ob = oracle.bg(2.0, c0=0.0, c1=2.0, tau_switch=1.0, has_switch=True)
+    lvl = capi.Level(ctx, n, dx, capi.AXS_FP64)
+    lvl.upload(s)
+    tau = 1.0
+    for _ in range(5):
+        c = oracle.stage_coeffs(ob, tau, dt)
+        lvl.rk4_step([capi.RhsCoeffs(*c[3 * k:3 * k + 3]) for k in range(4)], dt)
+        oracle.rk4_step(s, dx, c, dt)
+        tau += dt
+    assert np.array_equal(lvl.download(), s)
+
+
+@pytest.mark.parametrize("variant,zc", [(0, 4), (1, 0), (2, 16), (1, 64)])
+def test_tile_variants_identical(ctx, oracle, variant, zc):
+    capi = _capi()
+    n = 64
+    s0 = seeded(oracle, n, seed=11)
+    bg = capi.make_background(2.0, c0=1.0)
+    lvl = capi.Level(ctx, n, 0.5, capi.AXS_FP64)
+    lvl.upload(s0)
+    lvl.set_tile(variant, zc)
+    lvl.advance(bg, 1.0, 0.1, 3)
+    ref = capi.Level(ctx, n, 0.5, capi.AXS_FP64)
+    ref.upload(s0)
+    ref.advance(bg, 1.0, 0.1, 3)
+    assert np.array_equal(lvl.download(), ref.download())
+
+
+def test_fp32c_tracks_fp64(ctx, oracle):
+    capi = _capi()
+    n, dx, dt = 64, 0.5, 0.1
+    s0 = seeded(oracle, n)
+    bg = capi.make_background(2.0, c0=0.0, c1=2.0, tau_switch=1.0,
+                              has_switch=True)
+    a = capi.Level(ctx, n, dx, capi.AXS_FP32C)
+    b = capi.Level(ctx, n, dx, capi.AXS_FP64)
+    c = capi.Level(ctx, n, dx, capi.AXS_FP32)
+    for L in (a, b, c):
+        L.upload(s0)
+        L.advance(bg, 1.0, dt, 40, use_graph=True)
+    d32c, psimax = a.compare(b)
+    d32, _ = c.compare(b)
+    assert d32c / psimax < 2e-5
+    assert d32c <= d32  # compensation never hurts
+
+
+def evolved(ctx, oracle, n=64, steps=30, seed=42, precision=0):
+    capi = _capi()
+    lvl = capi.Level(ctx, n, 0.5, precision)
+    lvl.upload(seeded(oracle, n, seed))
+    lvl.advance(capi.make_background(2.0), 1.0, 0.1, steps)
+    return lvl
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_windings_and_counts_bitexact(ctx, oracle, precision):
+    lvl = evolved(ctx, oracle, precision=precision)
+    s = lvl.download()
+    w_gpu = lvl.windings_field()
+    w_cpu = oracle.windings_field(s)
+    assert np.array_equal(w_gpu, w_cpu)
+    counts = lvl.count_plaquettes()
+    assert counts == oracle.count_plaquettes(s)
+    assert counts[0] > 0
+
+
+def test_counts_with_mask_and_string_lattice(ctx, oracle):
+    capi = _capi()
+    for n in (16, 24):
+        i = np.arange(n)
+        sx = np.sin(2 * np.pi * (i + 0.3) / n)
+        s = np.zeros((4, n, n, n))
+        s[0] = sx[None, None, :]
+        s[1] = sx[None, :, None]
+        lvl = capi.Level(ctx, n, 1.0, capi.AXS_FP64)
+        lvl.upload(s)
+        assert lvl.count_plaquettes() == (4 * n, 4 * n)
+    lvl = evolved(ctx, oracle, n=32, steps=20)
+    s = lvl.download()
+    rng = np.random.default_rng(1)
+    mask = (rng.random((32, 32, 32)) < 0.6).astype(np.uint8)
+    mb = capi.DeviceBuffer(mask.size)
+    mb.from_numpy(mask)
+    assert lvl.count_plaquettes(mb) == oracle.count_plaquettes(s, mask)
+
+
+@pytest.mark.parametrize("thr", [(DBL_MAX, DBL_MAX, 0.0), (0.02, DBL_MAX, 0.0),
+                                 (DBL_MAX, 0.3, 0.0), (DBL_MAX, DBL_MAX, 0.5)])
+def test_tagger_bitexact(ctx, oracle, thr):
+    lvl = evolved(ctx, oracle, n=32, steps=20)
+    s = lvl.download()
+    R = 3.0
+    tags, counts = lvl.tag(thr[0], thr[1], thr[2], R)
+    want = oracle.tag_cells(s, 0.5, thr[0], thr[1], thr[2], R)
+    assert np.array_equal(tags, want)
+    assert counts == oracle.count_plaquettes(s)
+
+
+@pytest.mark.parametrize("scheme,thr", [(0, 0.8), (1, 0.8), (2, 0.8), (2, 0.5)])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_energy_sums_parity(ctx, oracle, scheme, thr, precision):
+    lvl = evolved(ctx, oracle, n=32, steps=20, precision=precision)
+    s = lvl.download()
+    tau = 3.0
+    R, lam, RpR = tau, 1.0, 1.0 / tau
+    got = lvl.energy_sums(R, lam, RpR, scheme, thr)
+    want = oracle.energy_sums(s, 0.5, R, lam, RpR, scheme, thr)
+    np.testing.assert_allclose(got, want, rtol=1e-11, atol=1e-300)
+
+
+def test_energy_sums_with_mask(ctx, oracle):
+    capi = _capi()
+    lvl = evolved(ctx, oracle, n=32, steps=10)
+    s = lvl.download()
+    mask = (np.random.default_rng(2).random((32, 32, 32)) < 0.5).astype(np.uint8)
+    mb = capi.DeviceBuffer(mask.size)
+    mb.from_numpy(mask)
+    got = lvl.energy_sums(2.0, 0.5, 0.5, 2, 0.8, mask=mb)
+    want = oracle.energy_sums(s, 0.5, 2.0, 0.5, 0.5, 2, 0.8, mask=mask)
+    np.testing.assert_allclose(got, want, rtol=1e-11)
+    assert got[13] == mask.sum()
+
+
+@pytest.mark.parametrize("kind,scheme", [(0, 0), (0, 1), (0, 2), (1, 0), (2, 0)])
+def test_fill_field_parity(ctx, oracle, kind, scheme):
+    lvl = evolved(ctx, oracle, n=32, steps=10)
+    s = lvl.download()
+    got = lvl.fill_field(kind, scheme, 0.8, 2.0)
+    want = oracle.fill_field(s, kind, scheme, 0.8, 2.0)
+    np.testing.assert_allclose(got, want, rtol=1e-14, atol=1e-300)
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+@pytest.mark.parametrize("shape", [(32, 32, 32), (64, 32, 16)])
+def test_spectrum_parity(ctx, oracle, kind, shape):
+    capi = _capi()
+    nx, ny, nz = shape
+    lvl = capi.Level(ctx, shape, 0.5, capi.AXS_FP64)
+    lvl.upload(oracle.seeded_ic(shape))
+    lvl.advance(capi.make_background(2.0), 1.0, 0.1, 10)
+    s = lvl.download()
+    ss, sc, tot, ssq = lvl.spectrum(kind, capi.AXS_MASK_B, 0.5, 2.0)
+    field = oracle.fill_field(s, kind, 2, 0.5, 2.0)
+    wss, wsc, wtot = oracle.spectrum(field)
+    assert np.array_equal(sc, wsc)
+    np.testing.assert_allclose(ss, wss, rtol=1e-10, atol=1e-12 * wss.max())
+    np.testing.assert_allclose(tot, wtot, rtol=1e-11)
+    assert ssq == pytest.approx(np.sum(field * field), rel=1e-12)
+    # Parseval on the device result
+    assert tot[0] == pytest.approx(nx * ny * nz * ssq, rel=1e-10)
+
+
+def test_spectrum_fp32c_close(ctx, oracle):
+    capi = _capi()
+    lvl = evolved(ctx, oracle, n=32, steps=10, precision=capi.AXS_FP32C)
+    s = lvl.download()
+    ss, sc, tot, _ = lvl.spectrum(capi.AXS_FIELD_PI1)
+    wss, wsc, wtot = oracle.spectrum(oracle.fill_field(s, 1))
+    assert np.array_equal(sc, wsc)
+    np.testing.assert_allclose(ss, wss, rtol=2e-5, atol=1e-6 * wss.max())
+
+
+@pytest.mark.parametrize("shape", [(32, 32, 32), (64, 64, 64), (32, 16, 8)])
+def test_fourier_ic_parity(ctx, oracle, shape):
+    capi = _capi()
+    nx, ny, nz = shape
+    kmax = max(shape) / 8.0
+    for prec in (capi.AXS_FP64, capi.AXS_FP32C):
+        lvl = capi.Level(ctx, shape, 0.5, prec)
+        lvl.fourier_ic(0, kmax, 0.5, 42, 0)
+        lvl.fourier_ic(1, kmax, 0.5, 42, 1)
+        lvl.zero_components(2, 2)
+        got = lvl.download()
+        for c in (0, 1):
+            want = oracle.fourier_ic(shape, kmax, 0.5, 42, c)
+            tol = 1e-12 if prec == capi.AXS_FP64 else 1e-12
+            assert np.max(np.abs(got[c] - want)) < tol
+        assert np.all(got[2:] == 0.0)
+
+
+def test_homogeneous_exact_solution_gpu(ctx, oracle):
+    capi = _capi()
+    bg = capi.make_background(2.0)
+    for prec, tol in [(capi.AXS_FP64, 1e-13), (capi.AXS_FP32C, 1e-9)]:
+        lvl = capi.Level(ctx, 16, 0.5, prec)
+        lvl.set_homogeneous(capi.background_eval(bg, 0, 1.0),
+                            capi.background_eval(bg, 1, 1.0))
+        lvl.advance(bg, 1.0, 0.1, 100, use_graph=True)
+        s = lvl.download()
+        R = capi.background_eval(bg, 0, 1.0 + 100 * 0.1)
+        assert np.max(np.abs(s[0] - R)) / R < tol
+        assert np.max(np.abs(s[1])) == 0.0
+
+
+def test_errors_are_reported(ctx):
+    capi = _capi()
+    with pytest.raises(capi.AxsError, match=">= 4"):
+        capi.Level(ctx, (2, 8, 8), 0.5)
+    lvl = capi.Level(ctx, 8, 0.5)
+    with pytest.raises(capi.AxsError, match="comp"):
+        lvl.fourier_ic(3, 1.0, 0.5, 1, 0)
+    with pytest.raises(capi.AxsError):
+        lvl.set_tile(7, 0)
