@@ -78,7 +78,11 @@ _SIGS = {
     "axs_rhs_eval": (_I, [_P, C.POINTER(RhsCoeffs), _P]),
     "axs_rk4_step": (_I, [_P, C.POINTER(RhsCoeffs), _D]),
     "axs_rk4_advance": (_I, [_P, C.POINTER(Background), _D, _D, _I, _I]),
+    "axs_level_enable_timing": (_I, [_P, _I]),
+    "axs_level_stage_times": (_I, [_P, _PD, _PI64]),
     "axs_count_plaquettes": (_I, [_P, _P, _PI64]),
+    "axs_velocity_at_pierced_corners": (_I, [_P, _D, _D, _D, _P, _PD, _PI64,
+                                             _PI64]),
     "axs_windings_field": (_I, [_P, _P]),
     "axs_tag_cells": (_I, [_P, C.POINTER(TaggerParams), _P, _PI64]),
     "axs_energy_sums": (_I, [_P, _D, _D, _D, C.POINTER(Masking), _P, _PD]),
@@ -272,11 +276,31 @@ class Level:
         check(self.lib.axs_rk4_advance(self.h, C.byref(bg), tau, dt, nsteps,
                                        int(use_graph)))
 
+    def enable_timing(self, on: bool = True):
+        check(self.lib.axs_level_enable_timing(self.h, int(on)))
+
+    def stage_times(self):
+        """(ms per stage summed over the timed steps, number of steps)."""
+        ms = np.zeros(4)
+        n = C.c_int64()
+        check(self.lib.axs_level_stage_times(self.h, dptr(ms), C.byref(n)))
+        return ms, n.value
+
     def count_plaquettes(self, mask: Optional[DeviceBuffer] = None):
         out = (C.c_int64 * 2)()
         check(self.lib.axs_count_plaquettes(
             self.h, _P(mask.ptr) if mask is not None else None, out))
         return int(out[0]), int(out[1])
+
+    def velocity(self, R, RpR, m_r, mask: Optional[DeviceBuffer] = None):
+        """(sum gamma^2 v^2, corner count, (n_p_plain, n_p_weighted))."""
+        s = C.c_double()
+        n = C.c_int64()
+        counts = (C.c_int64 * 2)()
+        check(self.lib.axs_velocity_at_pierced_corners(
+            self.h, R, RpR, m_r, _P(mask.ptr) if mask is not None else None,
+            C.byref(s), C.byref(n), counts))
+        return s.value, n.value, (int(counts[0]), int(counts[1]))
 
     def windings_field(self) -> np.ndarray:
         buf = DeviceBuffer(3 * self.ncell)

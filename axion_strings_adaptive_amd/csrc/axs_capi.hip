@@ -343,6 +343,9 @@ int axs_level_destroy(axs_level *L)
     hipSetDevice(L->ctx->device);
     hipStreamSynchronize(L->ctx->stream);
     destroy_graph(L);
+    if (L->timing)
+        for (int i = 0; i < 5; ++i)
+            hipEventDestroy(L->ev[i]);
     if (L->fft_r2c)
         hipfftDestroy((hipfftHandle)L->fft_r2c);
     if (L->fft_c2r)
@@ -549,8 +552,35 @@ int axs_rhs_eval(axs_level *L, const axs_rhs_coeffs *c, void *rhs)
     return launch_rhs_eval(L, L->d_coeff, rhs);
 }
 
+static int run_step_timed(axs_level *L, const int *step_ptr, double dt)
+{
+    hipStream_t s = L->ctx->stream;
+    const void *in[4] = {L->y, L->ya, L->yb, L->ya};
+    void *out[4] = {L->ya, L->yb, L->ya, L->y};
+    AXS_HIP_TRY(hipEventRecord(L->ev[0], s));
+    for (int st = 0; st < 4; ++st)
+    {
+        int rc = launch_rk_stage(L, st + 1, in[st], out[st], L->d_coeff,
+                                 step_ptr, st, dt);
+        if (rc)
+            return rc;
+        AXS_HIP_TRY(hipEventRecord(L->ev[st + 1], s));
+    }
+    AXS_HIP_TRY(hipEventSynchronize(L->ev[4]));
+    for (int st = 0; st < 4; ++st)
+    {
+        float ms = 0.0f;
+        AXS_HIP_TRY(hipEventElapsedTime(&ms, L->ev[st], L->ev[st + 1]));
+        L->stage_ms[st] += ms;
+    }
+    L->timed_steps += 1;
+    return AXS_OK;
+}
+
 static int run_step(axs_level *L, const int *step_ptr, double dt)
 {
+    if (L->timing)
+        return run_step_timed(L, step_ptr, dt);
     int rc;
     if ((rc = launch_rk_stage(L, 1, L->y, L->ya, L->d_coeff, step_ptr, 0, dt)))
         return rc;
@@ -646,7 +676,7 @@ int axs_rk4_advance(axs_level *L, const axs_background *bg, double tau,
             return rc;
         if ((rc = launch_step_advance(L, 1)))
             return rc;
-        if (use_graph)
+        if (use_graph && !L->timing)
         {
             if (!L->graph_valid || L->graph_key[0] != dt)
                 if ((rc = capture_graph(L, dt)))
@@ -666,6 +696,38 @@ int axs_rk4_advance(axs_level *L, const axs_background *bg, double tau,
         }
         done += chunk;
     }
+    return AXS_OK;
+}
+
+int axs_level_enable_timing(axs_level *L, int on)
+{
+    AXS_CHECK_ARG(L, "axs_level_enable_timing: NULL level");
+    if (on && !L->timing)
+    {
+        for (int i = 0; i < 5; ++i)
+            AXS_HIP_TRY(hipEventCreate(&L->ev[i]));
+        L->timing = 1;
+    }
+    else if (!on && L->timing)
+    {
+        AXS_HIP_TRY(hipStreamSynchronize(L->ctx->stream));
+        for (int i = 0; i < 5; ++i)
+            hipEventDestroy(L->ev[i]);
+        L->timing = 0;
+    }
+    return AXS_OK;
+}
+
+int axs_level_stage_times(axs_level *L, double ms[4], int64_t *nsteps)
+{
+    AXS_CHECK_ARG(L && ms && nsteps, "axs_level_stage_times: NULL argument");
+    for (int i = 0; i < 4; ++i)
+    {
+        ms[i] = L->stage_ms[i];
+        L->stage_ms[i] = 0.0;
+    }
+    *nsteps = L->timed_steps;
+    L->timed_steps = 0;
     return AXS_OK;
 }
 

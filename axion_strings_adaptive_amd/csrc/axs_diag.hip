@@ -103,7 +103,29 @@ struct WindParams
     // tagger extras
     double dx, grad_thr, rgrad_thr, amp_thr, R;
     int use_lap, use_rgrad;
+    // string velocities at pierced corners (VelocityKernel.hpp:37-141)
+    int do_vel;
+    double vel_R, vel_RpR, vel_mr;
+    double *vel_partials; // [gridDim.x]
 };
+
+// gamma_sq_v_sq_from_phi, Velocity.hpp:62-78
+__device__ __forceinline__ double gamma_sq_v_sq_d(double phi1, double phi2,
+                                                  double pd1, double pd2,
+                                                  double m_r)
+{
+    const double c1 = 0.41222;
+    const double e1 = -0.025763;
+    const double phi_sq = phi1 * phi1 + phi2 * phi2;
+    const double phi_dot_sq = pd1 * pd1 + pd2 * pd2;
+    const double re = phi1 * pd1 + phi2 * pd2;
+    const double m_r_sq = m_r * m_r;
+    const double c1_sq = c1 * c1;
+    const double term1 = (phi_dot_sq / (2.0 * m_r_sq * c1_sq)) *
+                         (1.0 - e1 * phi_sq / (c1 * c1_sq));
+    const double term2 = (e1 / (4.0 * m_r_sq * c1_sq * c1_sq * c1)) * re * re;
+    return term1 - term2;
+}
 
 constexpr int WTY = 4;
 
@@ -155,8 +177,23 @@ __global__ void __launch_bounds__(64 * WTY)
 
     unsigned long long plain_w = 0; // per wavefront (uniform)
     unsigned long long weighted = 0; // per lane
+    unsigned long long vcount = 0;   // per lane
+    double vsum = 0.0;               // per lane
     const StarLoader S{nx, ny, nz, plane};
     const double i1 = 1.0 / p.dx, i2 = 1.0 / (p.dx * p.dx);
+    const T *q1 = y + 2 * (size_t)n, *q2 = y + 3 * (size_t)n;
+    const float *m1 = lo ? lo + 2 * (size_t)n : nullptr;
+    const float *m2 = lo ? lo + 3 * (size_t)n : nullptr;
+    // gamma^2 v^2 at one plaquette corner, VelocityKernel.hpp:82-96
+    auto corner = [&](int xx, int yy, int zz) -> double {
+        const double R = p.vel_R, RpR = p.vel_RpR;
+        const double psi1 = S.at(p1, l1, xx, yy, zz);
+        const double psi2 = S.at(p2, l2, xx, yy, zz);
+        const double Pi1 = S.at(q1, m1, xx, yy, zz);
+        const double Pi2 = S.at(q2, m2, xx, yy, zz);
+        return gamma_sq_v_sq_d(psi1 / R, psi2 / R, (Pi1 - RpR * psi1) / (R * R),
+                               (Pi2 - RpR * psi2) / (R * R), p.vel_mr);
+    };
 
     for (int z = z_begin; z < z_end; ++z)
     {
@@ -181,6 +218,28 @@ __global__ void __launch_bounds__(64 * WTY)
             wyz = plaquette_winding_d(t000, t010, t011, t001);
             wzx = plaquette_winding_d(t000, t001, t101, t100);
             weighted += (unsigned long long)(abs(wxy) + abs(wyz) + abs(wzx));
+            if (p.do_vel && (wxy != 0 || wyz != 0 || wzx != 0))
+            {
+                const double c000 = corner(gx, gy, z);
+                if (wxy != 0)
+                {
+                    vsum += c000 + corner(gx + 1, gy, z) +
+                            corner(gx + 1, gy + 1, z) + corner(gx, gy + 1, z);
+                    vcount += 4;
+                }
+                if (wyz != 0)
+                {
+                    vsum += c000 + corner(gx, gy + 1, z) +
+                            corner(gx, gy + 1, z + 1) + corner(gx, gy, z + 1);
+                    vcount += 4;
+                }
+                if (wzx != 0)
+                {
+                    vsum += c000 + corner(gx, gy, z + 1) +
+                            corner(gx + 1, gy, z + 1) + corner(gx + 1, gy, z);
+                    vcount += 4;
+                }
+            }
             if (p.wfield)
             {
                 p.wfield[3 * (size_t)id] = (int8_t)wxy;
@@ -258,13 +317,34 @@ __global__ void __launch_bounds__(64 * WTY)
         plain_w += __popcll(__ballot(wxy != 0)) + __popcll(__ballot(wyz != 0)) +
                    __popcll(__ballot(wzx != 0));
     }
-    // wavefront reduction of the per-lane weighted count
+    // wavefront reduction of the per-lane counts
     for (int o = 32; o > 0; o >>= 1)
+    {
         weighted += __shfl_xor(weighted, o, 64);
+        vcount += __shfl_xor(vcount, o, 64);
+    }
     if ((threadIdx.x & 63) == 0)
     {
         atomicAdd(&p.counts[0], plain_w);
         atomicAdd(&p.counts[1], weighted);
+        if (p.do_vel)
+            atomicAdd(&p.counts[2], vcount);
+    }
+    if (p.do_vel)
+    {
+        // fixed-order block reduction of the velocity sum -> one partial
+        __shared__ double red[64 * WTY];
+        const int t = threadIdx.y * 64 + threadIdx.x;
+        red[t] = vsum;
+        __syncthreads();
+        for (int o = 32 * WTY; o > 0; o >>= 1)
+        {
+            if (t < o)
+                red[t] += red[t + o];
+            __syncthreads();
+        }
+        if (t == 0)
+            p.vel_partials[blockIdx.x] = red[0];
     }
 }
 
@@ -468,9 +548,16 @@ static int nblocks(size_t n, int bs, int cap)
 }
 
 // shared launcher for windings / tags
+struct VelArgs
+{
+    double R, RpR, m_r;
+    double *sum;
+    int64_t *count;
+};
+
 static int run_windings(axs_level *L, const uint8_t *mask, int8_t *wfield,
                         uint8_t *tags, const axs_tagger_params *tp,
-                        int64_t out[2])
+                        int64_t out[2], const VelArgs *vel = nullptr)
 {
     hipStream_t s = L->ctx->stream;
     WindParams p{};
@@ -499,6 +586,16 @@ static int run_windings(axs_level *L, const uint8_t *mask, int8_t *wfield,
     p.z_chunk = zc;
     p.tiles_z = (p.nz + zc - 1) / zc;
     const int ntiles = p.tiles_x * p.tiles_y * p.tiles_z;
+    if (vel)
+    {
+        AXS_CHECK_ARG(ntiles <= kMaxPartials * 16,
+                      "axs_velocity_at_pierced_corners: grid too large");
+        p.do_vel = 1;
+        p.vel_R = vel->R;
+        p.vel_RpR = vel->RpR;
+        p.vel_mr = vel->m_r;
+        p.vel_partials = L->ctx->d_partials;
+    }
     hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, s, p.counts);
     if (L->precision == AXS_FP64)
         hipLaunchKernelGGL(k_windings<double>, dim3(ntiles), dim3(64, WTY), 0,
@@ -507,14 +604,31 @@ static int run_windings(axs_level *L, const uint8_t *mask, int8_t *wfield,
         hipLaunchKernelGGL(k_windings<float>, dim3(ntiles), dim3(64, WTY), 0, s,
                            (const float *)L->y, (const float *)L->comp, p);
     AXS_HIP_TRY(hipGetLastError());
-    if (out)
+    if (vel)
+    {
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s,
+                           (const double *)p.vel_partials, ntiles, 1,
+                           L->ctx->d_result);
+        AXS_HIP_TRY(hipGetLastError());
+        AXS_HIP_TRY(hipMemcpyAsync(L->ctx->h_result, L->ctx->d_result,
+                                   sizeof(double), hipMemcpyDeviceToHost, s));
+    }
+    if (out || vel)
     {
         AXS_HIP_TRY(hipMemcpyAsync(L->ctx->h_counts, p.counts,
-                                   2 * sizeof(unsigned long long),
+                                   3 * sizeof(unsigned long long),
                                    hipMemcpyDeviceToHost, s));
         AXS_HIP_TRY(hipStreamSynchronize(s));
-        out[0] = (int64_t)L->ctx->h_counts[0];
-        out[1] = (int64_t)L->ctx->h_counts[1];
+        if (out)
+        {
+            out[0] = (int64_t)L->ctx->h_counts[0];
+            out[1] = (int64_t)L->ctx->h_counts[1];
+        }
+        if (vel)
+        {
+            *vel->sum = L->ctx->h_result[0];
+            *vel->count = (int64_t)L->ctx->h_counts[2];
+        }
     }
     return AXS_OK;
 }
@@ -529,6 +643,21 @@ int axs_count_plaquettes(axs_level *L, const uint8_t *mask, int64_t out[2])
 {
     AXS_CHECK_ARG(L && out, "axs_count_plaquettes: NULL argument");
     return run_windings(L, mask, nullptr, nullptr, nullptr, out);
+}
+
+int axs_velocity_at_pierced_corners(axs_level *L, double R, double RpR,
+                                    double m_r, const uint8_t *mask,
+                                    double *sum_gamma_sq_v_sq, int64_t *count,
+                                    int64_t counts[2])
+{
+    AXS_CHECK_ARG(L && sum_gamma_sq_v_sq && count,
+                  "axs_velocity_at_pierced_corners: NULL argument");
+    AXS_CHECK_ARG(R > 0.0 && m_r > 0.0,
+                  "axs_velocity_at_pierced_corners: R and m_r must be > 0");
+    VelArgs v{R, RpR, m_r, sum_gamma_sq_v_sq, count};
+    int64_t c[2];
+    return run_windings(L, mask, nullptr, nullptr, nullptr, counts ? counts : c,
+                        &v);
 }
 
 int axs_windings_field(axs_level *L, int8_t *w)

@@ -83,6 +83,11 @@ struct axs_level
     void *fft_r2c;  // hipfftHandle stored as void*
     void *fft_c2r;
     size_t fft_work_bytes;
+    // stage timing
+    int timing;
+    hipEvent_t ev[5];
+    double stage_ms[4];
+    long long timed_steps;
 };
 
 namespace axs

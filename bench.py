@@ -1,0 +1,214 @@
+#!/usr/bin/env python3
+"""Benchmark: cell-updates/s of classic RK4 on a 1024^3 periodic level,
+FP32 + Kahan compensation versus FP64, on one MI355X.
+
+Workload (the precision-form configuration): 1024^3 cells, dx = 0.5,
+radiation era a = tau from tau = 1, lambda = eta = 1 with the c = 2 (Moore)
+schedule reached through a switch at tau = 1 (c0 = 0, c1 = 2), dtau = 0.1;
+psi1, psi2 hash-seeded Gaussian fields (seed 42, modes 0 < |p| <= N/8,
+<psi_i^2> = 0.5), Pi = 0, generated on the device.  One step = one RK4 step
+of every cell (four fused RHS + RK-stage launches).  The FP32C run is the
+headline `value`; the FP64 run on the same IC is timed the same way and
+reported beside it together with the precision metrics (max |dphi|/eta,
+relative energy error, delta xi) at the end of the timed steps.
+
+The timed region holds exactly K steps, bracketed by a device
+synchronisation on both sides, with all data resident in HBM.  Stage
+kernels are launched directly with HIP events recorded on the level's own
+stream between them, which gives the per-launch durations used for the
+roofline figure.  The CPU baseline is the FP64 OpenMP oracle timed on a
+bounded sample (a 256^3 level, 2 RK4 steps) of the same per-cell work.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("cell-updates/s 1024^3 RK4: FP32-compensated vs FP64 at <=1e-5 "
+          "rel energy err")
+HBM_PEAK_GBS = 8000.0
+# Algorithmic HBM bytes per cell of each stage launch (4 components):
+#   FP32C (S = 16 B): stage1 3S, stage2 5S, stage3 5S, stage4 6S (y, lo rd/wr)
+#   FP64  (S = 32 B): stage1 3S, stage2 5S, stage3 5S, stage4 4S
+STAGE_BYTES = {1: [48, 80, 80, 96], 0: [96, 160, 160, 128]}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_level(capi, ctx, n, precision, bg, dt, warmup, steps, tile=None):
+    lvl = capi.Level(ctx, n, 0.5, precision)
+    kmax = n / 8.0
+    lvl.fourier_ic(0, kmax, 0.5, 42, 0)
+    lvl.fourier_ic(1, kmax, 0.5, 42, 1)
+    lvl.zero_components(2, 2)
+    if tile is not None:
+        lvl.set_tile(*tile)
+    tau = 1.0
+    lvl.advance(bg, tau, dt, warmup, use_graph=True)
+    tau += warmup * dt
+    ctx.synchronize()
+    lvl.enable_timing(True)
+    lvl.stage_times()  # reset
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    lvl.advance(bg, tau, dt, steps, use_graph=False)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    ms, nsteps = lvl.stage_times()
+    lvl.enable_timing(False)
+    tau_end = 1.0 + (warmup + steps) * dt
+    return lvl, (t1 - t0), ms, nsteps, tau_end
+
+
+def diagnostics(capi, lvl, bg, tau, n):
+    R = capi.background_eval(bg, 0, tau)
+    lam = capi.background_eval(bg, 3, tau)
+    RpR = capi.background_eval(bg, 8, tau)
+    e = lvl.energy_sums(R, lam, RpR, capi.AXS_MASK_B, 0.8)
+    plain, weighted = lvl.count_plaquettes()
+    dx = 0.5
+    L = n * dx
+    ell = (2.0 / 3.0) * plain * dx
+    a_inv = bg.a_inv
+    xi = ell / L ** 3 * ((a_inv - 1) ** 3 / a_inv ** 2) * tau * tau
+    return {"rho_tot_unscreened": e[0] / e[13], "n_p": plain, "xi": xi, "R": R}
+
+
+def cpu_baseline(n_sample=256, steps=2):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle_py as o
+    L = o.load()
+    s = np.zeros((4, n_sample, n_sample, n_sample))
+    rng = np.random.default_rng(0)
+    s[0] = 1.0 + 0.1 * rng.standard_normal(s[0].shape)
+    s[1] = 0.1 * rng.standard_normal(s[1].shape)
+    b = o.bg(2.0, c0=0.0, c1=2.0, tau_switch=1.0, has_switch=True)
+    t0 = time.perf_counter()
+    o.advance(s, 0.5, b, 1.0, 0.1, steps)
+    dt = time.perf_counter() - t0
+    return {"value": n_sample ** 3 * steps / dt, "unit": "cell-updates/s",
+            "cores": int(L.or_num_threads()), "kind": "port",
+            "sample": f"FP64 OpenMP oracle, {n_sample}^3 periodic level, "
+                      f"{steps} RK4 steps ({dt:.1f} s)"}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1024)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tile", type=str, default="",
+                    help="variant,z_chunk (default: autotuned cache or built-in)")
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        return 2
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if world != 1:
+        print("bench.py runs as a single process on one GPU", file=sys.stderr)
+        return 2
+
+    from axion_strings_adaptive_amd import capi
+    import torch
+    if not torch.cuda.is_available():
+        print("no HIP device", file=sys.stderr)
+        return 2
+    ctx = capi.Context(0)
+    n, dt = args.n, 0.1
+    bg = capi.make_background(2.0, c0=0.0, c1=2.0, tau_switch=1.0,
+                              has_switch=True)
+    tile = None
+    if args.tile:
+        v, z = (int(t) for t in args.tile.split(","))
+        tile = (v, z)
+    else:
+        from axion_strings_adaptive_amd import autotune
+        tile = autotune.cached_choice(n, capi.AXS_FP32C)
+
+    log(f"[bench] FP32C {n}^3: IC + {args.warmup} warmup + {args.steps} timed steps")
+    a, wall32, ms32, ns32, tau_end = run_level(
+        capi, ctx, n, capi.AXS_FP32C, bg, dt, args.warmup, args.steps, tile)
+    ncell = n ** 3
+    value = ncell * args.steps / wall32
+    d32 = diagnostics(capi, a, bg, tau_end, n)
+    log(f"[bench] FP32C {value:.4e} cell-updates/s, {1e3 * wall32 / args.steps:.2f} ms/step")
+
+    res64 = None
+    if not args.no_fp64:
+        log(f"[bench] FP64 {n}^3 same IC and steps")
+        b, wall64, ms64, ns64, _ = run_level(
+            capi, ctx, n, capi.AXS_FP64, bg, dt, args.warmup, args.steps,
+            autotune.cached_choice(n, capi.AXS_FP64) if not args.tile else tile)
+        d64 = diagnostics(capi, b, bg, tau_end, n)
+        dpsi, _ = a.compare(b)
+        value64 = ncell * args.steps / wall64
+        res64 = {
+            "value": value64, "ms_per_step": 1e3 * wall64 / args.steps,
+            "stage_ms": [m / ns64 for m in ms64],
+            "speedup_fp32c_over_fp64": value / value64,
+            "max_dphi_over_eta": dpsi / d64["R"],
+            "rel_energy_err": abs(d32["rho_tot_unscreened"] -
+                                  d64["rho_tot_unscreened"]) /
+            abs(d64["rho_tot_unscreened"]),
+            "delta_xi": d32["xi"] - d64["xi"], "xi_fp64": d64["xi"],
+            "tau": tau_end,
+        }
+        log(f"[bench] FP64 {value64:.4e} cell-updates/s; ratio {value / value64:.3f}")
+        b.close()
+
+    # roofline of the dominant kernel: the stage-2/3 launch (two per step)
+    st = [m / ns32 for m in ms32]  # ms per launch of each stage
+    bytes23 = STAGE_BYTES[1][1] * ncell
+    achieved = bytes23 / (0.5 * (st[1] + st[2]) * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                "traffic": None,
+                "kernel": "k_stage<float, FP32C, MODE_S23> (RK stages 2/3)",
+                "stage_ms": st,
+                "stage_gbs": [STAGE_BYTES[1][i] * ncell / (st[i] * 1e-3) / 1e9
+                              for i in range(4)],
+                "step_gbs": sum(STAGE_BYTES[1]) * ncell /
+                (sum(st) * 1e-3) / 1e9}
+    a.close()
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline (oracle, bounded sample)")
+        cpu = cpu_baseline()
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "cell-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall32 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{n}^3 periodic RK4, dx=0.5, dtau=0.1, a=tau "
+                               "from tau=1, c0=0 -> c1=2 at tau=1, seeded "
+                               "Gaussian IC (seed 42, |p|<=N/8)",
+                   "precision": "fp32 storage + Kahan-compensated y_n "
+                                "(fp32 hi + fp32 lo)",
+                   "tile": list(tile) if tile else None},
+        "fp64": res64, "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -164,10 +164,27 @@ int axs_rk4_step(axs_level *lvl, const axs_rhs_coeffs coeffs[4], double dt);
 int axs_rk4_advance(axs_level *lvl, const axs_background *bg, double tau,
                     double dt, int nsteps, int use_graph);
 
+/* Per-stage kernel timing with HIP events recorded on the context stream
+ * around each stage launch of axs_rk4_advance (forces direct launches while
+ * on).  ms[s] accumulates the milliseconds of stage s+1, *nsteps the steps
+ * timed; axs_level_stage_times resets the accumulators. */
+int axs_level_enable_timing(axs_level *lvl, int on);
+int axs_level_stage_times(axs_level *lvl, double ms[4], int64_t *nsteps);
+
 /* ------------------------ strings / tagging ------------------------- */
 /* count_plaquettes (StringFinder.hpp:76-122): out[0] = n_p_plain,
  * out[1] = n_p_weighted.  mask: optional device uint8 per cell (0 skips). */
 int axs_count_plaquettes(axs_level *lvl, const uint8_t *mask, int64_t out[2]);
+/* compute_velocity_at_pierced_corners (VelocityKernel.hpp:37-141):
+ * gamma^2 v^2 (Velocity.hpp:62-78) summed over the 4 corners of every
+ * pierced plaquette (*sum_gamma_sq_v_sq) and the number of corner
+ * evaluations (*count), in the same pass as the plaquette counts
+ * (counts, optional).  mask: optional device uint8 per cell. */
+int axs_velocity_at_pierced_corners(axs_level *lvl, double R,
+                                    double R_prime_over_R, double m_r,
+                                    const uint8_t *mask,
+                                    double *sum_gamma_sq_v_sq, int64_t *count,
+                                    int64_t counts[2]);
 /* plaquette_windings_at for every cell (StringFinder.hpp:45-68), into a
  * device int8 array w[cell*3 + {xy, yz, zx}]. */
 int axs_windings_field(axs_level *lvl, int8_t *w);

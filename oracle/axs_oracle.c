@@ -449,6 +449,73 @@ void or_windings_field(const double *s, int nx, int ny, int nz, int8_t *wf)
             }
 }
 
+/* ========== gamma_sq_v_sq_from_phi, Velocity.hpp:62-78 ============== */
+double or_gamma_sq_v_sq_from_phi(double phi1, double phi2, double pd1,
+                                 double pd2, double m_r)
+{
+    const double c1 = 0.41222;
+    const double e1 = -0.025763;
+    const double phi_sq = phi1 * phi1 + phi2 * phi2;
+    const double phi_dot_sq = pd1 * pd1 + pd2 * pd2;
+    const double re = phi1 * pd1 + phi2 * pd2;
+    const double m_r_sq = m_r * m_r;
+    const double c1_sq = c1 * c1;
+    const double term1 = (phi_dot_sq / (2.0 * m_r_sq * c1_sq)) *
+                         (1.0 - e1 * phi_sq / (c1 * c1_sq));
+    const double term2 = (e1 / (4.0 * m_r_sq * c1_sq * c1_sq * c1)) * re * re;
+    return term1 - term2;
+}
+
+/* compute_velocity_at_pierced_corners, VelocityKernel.hpp:37-141 */
+static double gvv_at(const double *s, int x, int y, int z, int nx, int ny,
+                     int nz, double R, double RpR, double m_r)
+{
+    const double psi1 = F(0, x, y, z), psi2 = F(1, x, y, z);
+    const double Pi1 = F(2, x, y, z), Pi2 = F(3, x, y, z);
+    const double phi1 = psi1 / R, phi2 = psi2 / R;
+    const double pd1 = (Pi1 - RpR * psi1) / (R * R);
+    const double pd2 = (Pi2 - RpR * psi2) / (R * R);
+    return or_gamma_sq_v_sq_from_phi(phi1, phi2, pd1, pd2, m_r);
+}
+
+void or_velocity(const double *s, int nx, int ny, int nz, double R,
+                 double RpR, double m_r, const uint8_t *mask, double *sum_out,
+                 int64_t *count_out)
+{
+    double sum = 0.0;
+    int64_t count = 0;
+#pragma omp parallel for collapse(2) reduction(+ : sum, count)
+    for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x)
+            {
+                const size_t id = ((size_t)z * ny + y) * nx + x;
+                if (mask && mask[id] == 0)
+                    continue;
+                int w[3];
+                or_windings_at(s, nx, ny, nz, x, y, z, w);
+#define G(a, b, c) gvv_at(s, x + (a), y + (b), z + (c), nx, ny, nz, R, RpR, m_r)
+                if (w[0] != 0)
+                {
+                    sum += G(0, 0, 0) + G(1, 0, 0) + G(1, 1, 0) + G(0, 1, 0);
+                    count += 4;
+                }
+                if (w[1] != 0)
+                {
+                    sum += G(0, 0, 0) + G(0, 1, 0) + G(0, 1, 1) + G(0, 0, 1);
+                    count += 4;
+                }
+                if (w[2] != 0)
+                {
+                    sum += G(0, 0, 0) + G(0, 0, 1) + G(1, 0, 1) + G(1, 0, 0);
+                    count += 4;
+                }
+#undef G
+            }
+    *sum_out = sum;
+    *count_out = count;
+}
+
 /* ============ StringTagger::operator(), StringTagger.hpp:240-341 ===== */
 void or_tag_cells(const double *s, int nx, int ny, int nz, double dx,
                   double gradient_threshold, double radial_gradient_threshold,

@@ -102,6 +102,16 @@ void or_count_plaquettes(const double *state, int nx, int ny, int nz,
 void or_windings_field(const double *state, int nx, int ny, int nz,
                        int8_t *w);
 
+/* ------- string velocities (Velocity.hpp:53-78, VelocityKernel.hpp) --- */
+double or_gamma_sq_v_sq_from_phi(double phi1, double phi2, double pd1,
+                                 double pd2, double m_r);
+/* compute_velocity_at_pierced_corners (VelocityKernel.hpp:37-141):
+ * *sum = sum of gamma^2 v^2 over the 4 corners of every pierced plaquette,
+ * *count = number of corner evaluations.  mask may be NULL. */
+void or_velocity(const double *state, int nx, int ny, int nz, double R,
+                 double RpR, double m_r, const uint8_t *mask, double *sum,
+                 int64_t *count);
+
 /* ---------------- tagger (StringTagger.hpp:83-218) ------------------ */
 /* Only sets tags (never clears), like the reference.  amp_thr <= 0 turns
  * the optional amplitude criterion |psi|/R < amp_thr off. */

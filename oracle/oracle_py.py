@@ -86,6 +86,10 @@ def load():
     lib.or_spectrum.restype = _I
     lib.or_spectrum.argtypes = [_PD, _I, _I, _I, _PD, C.POINTER(C.c_int64),
                                 _PD]
+    lib.or_gamma_sq_v_sq_from_phi.restype = _D
+    lib.or_gamma_sq_v_sq_from_phi.argtypes = [_D] * 5
+    lib.or_velocity.argtypes = [_PD, _I, _I, _I, _D, _D, _D, _P, _PD,
+                                C.POINTER(C.c_int64)]
     lib.or_hash_u64.restype = C.c_uint64
     lib.or_hash_u64.argtypes = [C.c_uint64]
     lib.or_uniform_from_hash.restype = _D
@@ -154,6 +158,18 @@ def count_plaquettes(state, mask=None):
                                m.ctypes.data_as(_P) if m is not None else None,
                                out)
     return int(out[0]), int(out[1])
+
+
+def velocity(state, R, RpR, m_r, mask=None):
+    s = np.ascontiguousarray(state, dtype=np.float64)
+    nx, ny, nz = shape_of(s)
+    out = C.c_double()
+    cnt = C.c_int64()
+    m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+    load().or_velocity(_d(s), nx, ny, nz, R, RpR, m_r,
+                       m.ctypes.data_as(_P) if m is not None else None,
+                       C.byref(out), C.byref(cnt))
+    return out.value, cnt.value
 
 
 def windings_field(state):
@@ -243,6 +259,8 @@ def ref_lib():
                  ("ref_radial_mass", 4)]:
         getattr(lib, n).restype = _D
         getattr(lib, n).argtypes = [_D] * k
+    lib.ref_gamma_sq_v_sq_from_phi.restype = _D
+    lib.ref_gamma_sq_v_sq_from_phi.argtypes = [_D] * 5
     lib.ref_xi_from_comoving_length.restype = _D
     lib.ref_xi_from_comoving_length.argtypes = [_D] * 4
     lib.ref_xi_from_plaquette_count.restype = _D

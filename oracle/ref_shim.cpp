@@ -6,13 +6,14 @@
 // (axs_oracle.c) and the product's host Background against the reference
 // code itself.  Only headers that compile with nothing but <cmath> are used
 // (Background.hpp, PlaquetteWinding.hpp, Masking.hpp, Energy.hpp,
-// XiFormula.hpp); everything else in the reference needs the absent
+// XiFormula.hpp, Velocity.hpp); everything else in the reference needs the absent
 // GRTeclyn/AMReX submodules and is not built.  The output library goes to
 // oracle/_ref/ and is never shipped in git.
 #include "Background.hpp"
 #include "Energy.hpp"
 #include "Masking.hpp"
 #include "PlaquetteWinding.hpp"
+#include "Velocity.hpp"
 #include "XiFormula.hpp"
 
 namespace
@@ -117,6 +118,12 @@ extern "C"
     double ref_radial_mass(double psi1, double psi2, double R, double lambda)
     {
         return radial_mass_energy_pointwise(psi1, psi2, R, lambda);
+    }
+
+    double ref_gamma_sq_v_sq_from_phi(double phi1, double phi2, double pd1,
+                                      double pd2, double m_r)
+    {
+        return gamma_sq_v_sq_from_phi(phi1, phi2, pd1, pd2, m_r);
     }
 
     double ref_xi_from_comoving_length(double ell, double L, double a_inv,

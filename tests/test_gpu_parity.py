@@ -281,3 +281,21 @@ def test_errors_are_reported(ctx):
         lvl.fourier_ic(3, 1.0, 0.5, 1, 0)
     with pytest.raises(capi.AxsError):
         lvl.set_tile(7, 0)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_velocity_parity(ctx, oracle, precision):
+    capi = _capi()
+    lvl = evolved(ctx, oracle, n=32, steps=20, precision=precision)
+    s = lvl.download()
+    R, RpR, mr = 3.0, 1.0 / 3.0, 1.0
+    total, count, counts = lvl.velocity(R, RpR, mr)
+    wt, wc = oracle.velocity(s, R, RpR, mr)
+    assert count == wc and counts == oracle.count_plaquettes(s)
+    assert total == pytest.approx(wt, rel=1e-11)
+    mask = (np.random.default_rng(4).random((32, 32, 32)) < 0.5).astype(np.uint8)
+    mb = capi.DeviceBuffer(mask.size)
+    mb.from_numpy(mask)
+    total, count, _ = lvl.velocity(R, RpR, mr, mask=mb)
+    wt, wc = oracle.velocity(s, R, RpR, mr, mask=mask)
+    assert count == wc and total == pytest.approx(wt, rel=1e-11)

@@ -565,3 +565,42 @@ def test_fourier_ic_invariants_and_modes(oracle):
                 assert X[k, j, i] == pytest.approx(scale * want, rel=1e-10,
                                                    abs=1e-10)
     assert abs(scale.imag) < 1e-10 and scale.real > 0
+
+
+# ---------------------------------------------------------------- velocity
+def test_velocity_formula_matches_reference_header(oracle, ref):
+    L = oracle.load()
+    rng = np.random.default_rng(10)
+    for _ in range(3000):
+        v = rng.normal(size=5)
+        v[4] = abs(v[4]) + 0.05
+        assert L.or_gamma_sq_v_sq_from_phi(*v) == \
+            ref.ref_gamma_sq_v_sq_from_phi(*v)
+
+
+def test_velocity_static_field_and_corner_count(oracle):
+    """Pi = 0 and R'/R = 0 give phi_dot = 0, so gamma^2 v^2 = 0 at every
+    corner; the string lattice pierces 4n xy plaquettes -> 16n corners."""
+    for n in (8, 12):
+        i = np.arange(n)
+        sx = np.sin(2 * np.pi * (i + 0.3) / n)
+        s = np.zeros((4, n, n, n))
+        s[0] = sx[None, None, :]
+        s[1] = sx[None, :, None]
+        total, count = oracle.velocity(s, 1.0, 0.0, 1.0)
+        assert count == 16 * n and total == 0.0
+        # a uniform rotation psi -> psi e^{i w tau}: Pi = i w psi, so at every
+        # corner gamma^2 v^2 = term1 - term2 with re(phi* phi_dot) = 0
+        w = 0.3
+        s[2], s[3] = -w * s[1], w * s[0]
+        total, count = oracle.velocity(s, 1.0, 0.0, 1.0)
+        L = oracle.load()
+        wf = oracle.windings_field(s)
+        acc = 0.0
+        for z, yy, x in zip(*np.nonzero(wf[..., 0])):
+            for (a, b) in [(0, 0), (1, 0), (1, 1), (0, 1)]:
+                xx, yb = (x + a) % n, (yy + b) % n
+                p1, p2 = s[0, z, yb, xx], s[1, z, yb, xx]
+                acc += L.or_gamma_sq_v_sq_from_phi(p1, p2, -w * p2, w * p1,
+                                                   1.0)
+        assert total == pytest.approx(acc, rel=1e-12)
