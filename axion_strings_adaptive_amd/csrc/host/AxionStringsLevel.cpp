@@ -1,0 +1,382 @@
+// AxionStringsLevel.cpp -- see AxionStringsLevel.hpp.
+#include "AxionStringsLevel.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <iostream>
+#include <vector>
+
+namespace axs_host
+{
+
+static int precision_from(const std::string &s, const ParmParse &pp)
+{
+    if (s == "fp64" || s == "double" || s == "DOUBLE")
+        return AXS_FP64;
+    if (s == "fp32c")
+        return AXS_FP32C;
+    if (s == "fp32" || s == "float" || s == "FLOAT")
+        return AXS_FP32;
+    pp.error("precision", "must be fp64, fp32c or fp32");
+}
+
+AxionStringsLevel::AxionStringsLevel(const ParamTable &params,
+                                     const std::string &out_dir)
+    : m_params(params), m_out_dir(out_dir)
+{
+    Params::check_params(params);
+    ParmParse amr(params, "amr"), geom(params, "geometry"),
+        evo(params, "evolution"), as(params, "axion_strings");
+    std::vector<int> n_cell;
+    amr.getarr("n_cell", n_cell);
+    if (n_cell.size() != 3)
+        amr.error("n_cell", "needs 3 values");
+    int max_level = 0;
+    amr.queryAdd("max_level", max_level);
+    if (max_level != 0)
+        amr.error("max_level", "the uniform-level driver runs max_level = 0; "
+                               "refined levels use the box-level API");
+    std::vector<double> extent;
+    geom.getarr("prob_extent", extent);
+    if (extent.size() != 3)
+        geom.error("prob_extent", "needs 3 values");
+    for (int d = 0; d < 3; ++d)
+        m_n[d] = n_cell[d];
+    m_dx = extent[0] / n_cell[0];
+    for (int d = 1; d < 3; ++d)
+        if (std::abs(extent[d] / n_cell[d] - m_dx) > 1e-12 * m_dx)
+            geom.error("prob_extent", "cells must be cubic (equal dx)");
+    m_L = extent[0];
+    double dt_multiplier = 0.0;
+    evo.get("dt_multiplier", dt_multiplier);
+    m_dt = dt_multiplier * m_dx; // AxionStringsParams.hpp:359-361
+    evo.queryAdd("max_steps", m_max_steps);
+    evo.queryAdd("stop_time", m_stop_time);
+
+    std::string prec = "fp64";
+    as.queryAdd("precision", prec);
+    m_precision = precision_from(prec, as);
+    as.queryAdd("use_graph", m_use_graph);
+    as.queryAdd("flat_space", m_flat);
+    m_masking = Params::read_masking_params(params, "axion_strings.masking");
+    m_tagging = Params::read_tagging_params(params);
+    if (m_flat)
+    {
+        ParmParse(params, "axion_strings.flat").queryAdd("m_r", m_flat_mr);
+        long every = 0;
+        ParmParse(params, "axion_strings.flat")
+            .queryAdd("output_cadence_steps", every);
+        m_cadence.every_steps = every > 0 ? every : 1;
+        m_tau_i = 0.0;
+        m_cbg = axs_background{2.0, 0, 0, 0, 0, 1, m_flat_mr};
+    }
+    else
+    {
+        m_bg = Params::read_background(params);
+        m_tau_i = Params::read_tau_i(params, m_bg);
+        m_cadence = Params::read_output_cadence(params);
+        m_next_log = m_cadence.first_log_mr_over_h;
+        m_cbg = axs_background{m_bg.a_inv,           m_bg.c_sched.c0,
+                               m_bg.c_sched.c1,      m_bg.c_sched.tau_switch,
+                               m_bg.c_sched.has_switch, 0, 1.0};
+    }
+    m_tau = m_tau_i;
+    as.queryAdd("compute_spectrum", m_compute_spectrum);
+    std::string field = "a_dot";
+    as.queryAdd("spectrum_field", field);
+    if (field == "pi")
+        m_spectrum_kind = -1;
+    else if (field != "a_dot")
+        as.error("spectrum_field", "must be a_dot or pi");
+
+    axs_check(axs_ctx_create(0, &m_ctx), "axs_ctx_create");
+    axs_check(axs_level_create(m_ctx, m_n[0], m_n[1], m_n[2], m_dx,
+                               m_precision, &m_level),
+              "axs_level_create");
+}
+
+AxionStringsLevel::~AxionStringsLevel()
+{
+    if (m_scalars)
+        std::fclose(m_scalars);
+    if (m_spectrum)
+        std::fclose(m_spectrum);
+    axs_level_destroy(m_level);
+    axs_ctx_destroy(m_ctx);
+}
+
+// AxionStringsLevel::initData, AxionStringsLevel.cpp:786-847
+void AxionStringsLevel::initData()
+{
+    std::string ic_mode = "homogeneous";
+    ParmParse(m_params, "axion_strings").queryAdd("ic_mode", ic_mode);
+    if (ic_mode == "fourier")
+    {
+        const FourierICParams f =
+            Params::read_fourier_ic_params(m_params, "axion_strings");
+        const double m_r =
+            m_flat ? m_flat_mr : std::sqrt(m_bg.lambda(m_tau_i));
+        const double k_max_cells = f.k_max_over_mr * m_r * m_dx;
+        FourierIC::generate(m_level, 0, k_max_cells, f.mean_square_variance,
+                            f.seed, 0);
+        FourierIC::generate(m_level, 1, k_max_cells, f.mean_square_variance,
+                            f.seed, 1);
+        axs_check(axs_level_zero_components(m_level, 2, 2), "setVal(Pi)");
+        return;
+    }
+    const double R = m_flat ? 1.0 : m_bg.R(m_tau_i);
+    const double Rp = m_flat ? 0.0 : m_bg.R_prime(m_tau_i);
+    axs_check(axs_level_set_homogeneous(m_level, R, Rp), "initData");
+}
+
+// The log(m_r/H) (or step) output cadence of AxionStringsLevel.cpp:
+// 1198-1248, evaluated after a step that ends at tau_after / step_after.
+// `commit` advances the thresholds like the reference's while loops.
+struct CadenceState
+{
+    double next_log, next_moore;
+    bool moore_init;
+    double H_switch;
+};
+
+static bool cadence_due(const Background &bg, const OutputCadence &c,
+                        CadenceState &st, double tau, long step)
+{
+    if (c.every_steps > 0)
+        return step % c.every_steps == 0;
+    const double log_mr_over_h = -std::log(bg.H_over_mr_direct(tau));
+    const bool in_moore =
+        bg.c_sched.has_switch && tau >= bg.c_sched.tau_switch;
+    if (in_moore)
+    {
+        if (!st.moore_init)
+        {
+            st.H_switch = bg.H(bg.c_sched.tau_switch);
+            st.next_moore = 0.0;
+            st.moore_init = true;
+        }
+        const double d = std::log(st.H_switch / bg.H(tau));
+        if (d < st.next_moore)
+            return false;
+        while (st.next_moore <= d)
+            st.next_moore += c.delta_log_mr_over_h;
+        return true;
+    }
+    if (log_mr_over_h < st.next_log)
+        return false;
+    while (st.next_log <= log_mr_over_h)
+        st.next_log += c.delta_log_mr_over_h;
+    return true;
+}
+
+long AxionStringsLevel::steps_until_next_output()
+{
+    CadenceState st{m_next_log, m_next_moore_range, m_moore_init,
+                    m_moore_H_switch};
+    double tau = m_tau;
+    for (long k = 1; k < 100000000; ++k)
+    {
+        tau += m_dt;
+        if (cadence_due(m_bg, m_cadence, st, tau, m_step + k))
+            return k;
+    }
+    return 100000000;
+}
+
+bool AxionStringsLevel::output_due()
+{
+    CadenceState st{m_next_log, m_next_moore_range, m_moore_init,
+                    m_moore_H_switch};
+    const bool due = cadence_due(m_bg, m_cadence, st, m_tau, m_step);
+    m_next_log = st.next_log;
+    m_next_moore_range = st.next_moore;
+    m_moore_init = st.moore_init;
+    m_moore_H_switch = st.H_switch;
+    return due;
+}
+
+void AxionStringsLevel::evolve()
+{
+    while (true)
+    {
+        long k = steps_until_next_output();
+        if (m_max_steps >= 0)
+            k = std::min(k, m_max_steps - m_step);
+        if (m_stop_time >= 0.0)
+        {
+            // stop once tau - tau_i reaches evolution.stop_time
+            long left = 0;
+            double t = m_tau;
+            while (t - m_tau_i < m_stop_time - 1e-12 * m_dt && left < k)
+            {
+                t += m_dt;
+                ++left;
+            }
+            k = std::min(k, left);
+        }
+        if (k <= 0)
+            break;
+        axs_check(axs_rk4_advance(m_level, &m_cbg, m_tau, m_dt, (int)k,
+                                  m_use_graph ? 1 : 0),
+                  "axs_rk4_advance");
+        for (long j = 0; j < k; ++j)
+            m_tau += m_dt;
+        m_step += k;
+        if (output_due())
+            diagnostics();
+    }
+    axs_check(axs_ctx_synchronize(m_ctx), "synchronize");
+}
+
+// specific_post_timestep's snapshot, AxionStringsLevel.cpp:1256-1764
+void AxionStringsLevel::diagnostics()
+{
+    const double tau = m_tau;
+    const long n_cells = (long)m_n[0] * m_n[1] * m_n[2];
+    double R = 1.0, RpR = 0.0, lambda = m_flat_mr * m_flat_mr,
+           m_r_over_h = 0.0, H = 0.0;
+    if (!m_flat)
+    {
+        R = m_bg.R(tau);
+        RpR = 1.0 / (m_bg.b_inv * tau); // AxionStringsLevel.cpp:1283
+        lambda = m_bg.lambda(tau);
+        m_r_over_h = 1.0 / m_bg.H_over_mr_direct(tau);
+        H = m_bg.H(tau);
+    }
+    const double m_r = std::sqrt(lambda);
+
+    const PlaquetteCounts counts = count_plaquettes(m_level);
+    const double ell = (2.0 / 3.0) * (double)counts.n_p_plain * m_dx;
+    const double ell_w = (2.0 / 3.0) * (double)counts.n_p_weighted * m_dx;
+    const double xi = m_flat ? 0.0 : xi_from_comoving_length(ell, m_L, m_bg.a_inv, tau);
+    const double xi_w =
+        m_flat ? 0.0 : xi_from_comoving_length(ell_w, m_L, m_bg.a_inv, tau);
+
+    const VelocityResult vel =
+        compute_velocity_at_pierced_corners(m_level, R, RpR, m_r);
+    const double vw_sum = vel.sum_gamma_sq_v_sq * m_dx;
+    const double vw_cnt = (double)vel.count * m_dx;
+
+    const TotalEnergyResult e = compute_composite_total_energy(
+        {EnergyLevelInput{m_level, m_dx, nullptr}}, R, lambda, RpR, m_masking);
+
+    const double sum_core = e.n_total * e.rho_tot_unscreened -
+                            e.n_unmasked * e.rho_tot_screened;
+    const double tail = e.n_unmasked * (e.rho_axion_gradient_screened -
+                                        e.rho_axion_kinetic_screened);
+    const double t_core = ell > 0.0 ? (R * R) * sum_core / ell : 0.0;
+    const double t_tail = ell > 0.0 ? (R * R) * (sum_core + tail) / ell : 0.0;
+    VelocityResult vr{vw_sum, vel.count};
+    (void)vw_cnt;
+    write_network_scalars_row(counts, xi, xi_w, m_r_over_h, e, vr, t_core,
+                              t_tail);
+    std::cout << "  [AxionStrings] tau = " << tau << "  step = " << m_step
+              << "  N_p = " << counts.n_p_plain
+              << "  N_p_W = " << counts.n_p_weighted << "  xi = " << xi
+              << "  m_r/H = " << m_r_over_h
+              << "  rho_tot unscreened = " << e.rho_tot_unscreened
+              << "  screened = " << e.rho_tot_screened << "\n";
+
+    if (!m_compute_spectrum)
+        return;
+    const int n_max = std::max(m_n[0], std::max(m_n[1], m_n[2]));
+    MaskingParams none{};
+    none.scheme = MaskingScheme::None;
+    const Spectrum s_scr = compute_spectrum(m_level, m_spectrum_kind, m_masking,
+                                            R, n_max);
+    const Spectrum s_uns =
+        compute_spectrum(m_level, m_spectrum_kind, none, R, n_max);
+    const double nsq = (double)n_cells * (double)n_cells;
+    const bool first = (m_spectrum == nullptr);
+    if (first)
+    {
+        m_spectrum = std::fopen((m_out_dir + "/axion_spectrum.dat").c_str(), "w");
+        if (!m_spectrum)
+            throw AxsError("cannot write axion_spectrum.dat");
+        std::fprintf(m_spectrum,
+                     "# tau mode_index k_over_H shell_average_screened "
+                     "shell_average_unscreened full_cube_energy_screened "
+                     "full_cube_energy_unscreened "
+                     "inscribed_sphere_energy_screened "
+                     "inscribed_sphere_energy_unscreened "
+                     "real_space_mean_sq_screened real_space_mean_sq_unscreened "
+                     "parseval_full_screened parseval_full_unscreened "
+                     "parseval_inscribed_screened "
+                     "parseval_inscribed_unscreened\n");
+    }
+    const double k_over_h_per_mode = H > 0.0 ? 2.0 * M_PI / (m_L * H) : 0.0;
+    for (size_t i = 0; i < s_scr.shell_average.size(); ++i)
+        std::fprintf(m_spectrum,
+                     "%.10e %zu %.10e %.10e %.10e %.10e %.10e %.10e %.10e "
+                     "%.10e %.10e %.10e %.10e %.10e %.10e\n",
+                     tau, i, k_over_h_per_mode * (double)i,
+                     s_scr.shell_average[i],
+                     i < s_uns.shell_average.size() ? s_uns.shell_average[i] : 0.0,
+                     s_scr.full_cube_energy, s_uns.full_cube_energy,
+                     s_scr.inscribed_sphere_energy, s_uns.inscribed_sphere_energy,
+                     s_scr.real_space_sum_sq / n_cells,
+                     s_uns.real_space_sum_sq / n_cells,
+                     s_scr.full_cube_energy / nsq, s_uns.full_cube_energy / nsq,
+                     s_scr.inscribed_sphere_energy / nsq,
+                     s_uns.inscribed_sphere_energy / nsq);
+    std::fflush(m_spectrum);
+}
+
+void AxionStringsLevel::write_network_scalars_row(
+    const PlaquetteCounts &c, double xi, double xi_w, double m_r_over_h,
+    const TotalEnergyResult &e, const VelocityResult &v, double tension_core,
+    double tension_tail)
+{
+    if (!m_scalars)
+    {
+        m_scalars =
+            std::fopen((m_out_dir + "/network_scalars.dat").c_str(), "w");
+        if (!m_scalars)
+            throw AxsError("cannot write network_scalars.dat");
+        // column names of AxionStringsLevel.cpp:1541-1551
+        std::fprintf(m_scalars,
+                     "# tau N_p N_p_weighted xi xi_weighted m_r_over_H "
+                     "rho_tot_unscreened rho_tot_screened "
+                     "rho_axion_kin_unscreened rho_axion_kin_screened "
+                     "rho_axion_grad_unscreened rho_axion_grad_screened "
+                     "rho_radial_kin_unscreened rho_radial_kin_screened "
+                     "rho_radial_grad_unscreened rho_radial_grad_screened "
+                     "rho_radial_mass_unscreened rho_radial_mass_screened "
+                     "n_total n_unmasked mean_gamma_sq_v_sq mean_gamma "
+                     "n_velocity_corners tension_core_only "
+                     "tension_core_plus_tail\n");
+    }
+    const double wcount = (double)v.count * m_dx;
+    const double mean_g2v2 = wcount > 0.0 ? v.sum_gamma_sq_v_sq / wcount : 0.0;
+    const double row[25] = {m_tau,
+                            (double)c.n_p_plain,
+                            (double)c.n_p_weighted,
+                            xi,
+                            xi_w,
+                            m_r_over_h,
+                            e.rho_tot_unscreened,
+                            e.rho_tot_screened,
+                            e.rho_axion_kinetic_unscreened,
+                            e.rho_axion_kinetic_screened,
+                            e.rho_axion_gradient_unscreened,
+                            e.rho_axion_gradient_screened,
+                            e.rho_radial_kinetic_unscreened,
+                            e.rho_radial_kinetic_screened,
+                            e.rho_radial_gradient_unscreened,
+                            e.rho_radial_gradient_screened,
+                            e.rho_radial_mass_unscreened,
+                            e.rho_radial_mass_screened,
+                            e.n_total,
+                            e.n_unmasked,
+                            mean_g2v2,
+                            std::sqrt(1.0 + mean_g2v2),
+                            (double)v.count,
+                            tension_core,
+                            tension_tail};
+    for (int i = 0; i < 25; ++i)
+        std::fprintf(m_scalars, i ? " %.10e" : "%.10e", row[i]);
+    std::fprintf(m_scalars, "\n");
+    std::fflush(m_scalars);
+}
+
+} // namespace axs_host

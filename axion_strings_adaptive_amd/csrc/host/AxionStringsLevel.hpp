@@ -1,0 +1,76 @@
+// AxionStringsLevel.hpp -- the single-level driver around the hot path:
+// initial data, the RK4 evolution, and the cadence diagnostics with their
+// output rows.
+//
+// Follows the reference's AxionStringsLevel (AxionStringsLevel.cpp):
+//   initData            :210-848  (ic_mode fourier :786-822, homogeneous
+//                                  :824-847)
+//   specific_eval_rhs   :850-921  (stage scalars; here one device table)
+//   specific_post_timestep :923-1914  (log(m_r/H) cadence :1198-1248,
+//                                  counts/xi/energy/velocity :1299-1361,
+//                                  network_scalars.dat :1536-1579,
+//                                  spectrum + axion_spectrum.dat :1593-1764)
+// on one uniform periodic level, the level-0 path of the reference.
+#pragma once
+
+#include "AxionStringsOps.hpp"
+#include "AxionStringsParams.hpp"
+
+#include <cstdio>
+#include <string>
+
+namespace axs_host
+{
+
+class AxionStringsLevel
+{
+  public:
+    AxionStringsLevel(const ParamTable &params, const std::string &out_dir);
+    ~AxionStringsLevel();
+
+    void initData();
+    // Runs until evolution.max_steps / evolution.stop_time / tau_f.
+    void evolve();
+    // One diagnostics snapshot at the current time (also writes rows).
+    void diagnostics();
+
+    double tau() const { return m_tau; }
+    long step() const { return m_step; }
+    axs_level *level() { return m_level; }
+
+  private:
+    bool output_due();
+    long steps_until_next_output();
+    void write_network_scalars_row(const PlaquetteCounts &c, double xi,
+                                   double xi_w, double m_r_over_h,
+                                   const TotalEnergyResult &e,
+                                   const VelocityResult &v, double tension_core,
+                                   double tension_tail);
+
+    const ParamTable &m_params;
+    std::string m_out_dir;
+    axs_ctx *m_ctx{nullptr};
+    axs_level *m_level{nullptr};
+    int m_n[3]{};
+    double m_dx{}, m_L{}, m_dt{};
+    int m_precision{AXS_FP64};
+    bool m_flat{false};
+    double m_flat_mr{1.0};
+    Background m_bg;
+    axs_background m_cbg{};
+    double m_tau_i{}, m_tau{}, m_stop_time{-1.0};
+    long m_step{0}, m_max_steps{-1};
+    bool m_use_graph{true};
+    MaskingParams m_masking;
+    StringTaggerParams m_tagging;
+    OutputCadence m_cadence;
+    double m_next_log{};
+    bool m_moore_init{false};
+    double m_moore_H_switch{}, m_next_moore_range{};
+    bool m_compute_spectrum{false};
+    int m_spectrum_kind{AXS_FIELD_MASKED_A_DOT};
+    FILE *m_scalars{nullptr};
+    FILE *m_spectrum{nullptr};
+};
+
+} // namespace axs_host

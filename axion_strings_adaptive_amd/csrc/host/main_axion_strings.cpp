@@ -1,0 +1,60 @@
+// main_axion_strings.cpp -- command-line driver (the reference's
+// Main_AxionStrings.cpp:10-80 run loop, single level).
+//
+//   axion_strings params.txt [key=value ...] [--check] [--out DIR]
+//
+// --check only parses and validates the parameters (the reference's
+// just_check_params), without touching a GPU.
+#include "AxionStringsLevel.hpp"
+
+#include <cstring>
+#include <iostream>
+
+int main(int argc, char **argv)
+{
+    using namespace axs_host;
+    if (argc < 2)
+    {
+        std::cerr << "usage: " << argv[0]
+                  << " params.txt [key=value ...] [--check] [--out DIR]\n";
+        return 2;
+    }
+    ParamTable table;
+    bool check_only = false;
+    std::string out_dir = ".";
+    try
+    {
+        table.read_file(argv[1]);
+        for (int i = 2; i < argc; ++i)
+        {
+            if (!std::strcmp(argv[i], "--check"))
+                check_only = true;
+            else if (!std::strcmp(argv[i], "--out") && i + 1 < argc)
+                out_dir = argv[++i];
+            else
+                table.add_line(argv[i]);
+        }
+        Params::check_params(table);
+        if (check_only)
+        {
+            std::cout << "parameters ok\n";
+            return 0;
+        }
+        AxionStringsLevel level(table, out_dir);
+        level.initData();
+        level.evolve();
+        std::cout << "done: tau = " << level.tau() << "  steps = "
+                  << level.step() << "\n";
+    }
+    catch (const ParamError &e)
+    {
+        std::cerr << "parameter error: " << e.what() << "\n";
+        return 3;
+    }
+    catch (const std::exception &e)
+    {
+        std::cerr << "error: " << e.what() << "\n";
+        return 1;
+    }
+    return 0;
+}
