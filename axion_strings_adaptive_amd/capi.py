@@ -98,9 +98,16 @@ _SIGS = {
     "axs_boxlevel_num_cells": (C.c_int64, [_P]),
     "axs_boxlevel_upload_dense": (_I, [_P, _PD]),
     "axs_boxlevel_download_dense": (_I, [_P, _PD]),
-    "axs_boxlevel_fill_ghosts": (_I, [_P, _P, _D]),
-    "axs_boxlevel_rk4_step": (_I, [_P, C.POINTER(RhsCoeffs), _D]),
-    "axs_boxlevel_tag": (_I, [_P, C.POINTER(TaggerParams), _P, _PI64]),
+    "axs_boxlevel_copy_from_level": (_I, [_P, _P]),
+    "axs_boxlevel_copy_to_level": (_I, [_P, _P]),
+    "axs_boxlevel_snapshot_coarse": (_I, [_P, _P]),
+    "axs_boxlevel_fill_ghosts": (_I, [_P, _P, _D, _I]),
+    "axs_boxlevel_rk4_step": (_I, [_P, C.POINTER(RhsCoeffs), _D, _P, _D, _D]),
+    "axs_boxlevel_tag": (_I, [_P, C.POINTER(TaggerParams), _P, _P, _PI64]),
+    "axs_boxlevel_energy_sums": (_I, [_P, _D, _D, _D, C.POINTER(Masking), _P,
+                                      _PD]),
+    "axs_boxlevel_coverage_mask": (_I, [_P, C.POINTER(_I), _P]),
+    "axs_boxlevel_average_down": (_I, [_P, _P]),
     "axs_adjoint_grad_energy": (_I, [_P, C.POINTER(Background), _D, _D, _D,
                                      _I, _I, _PD, _PD, _PD]),
     "axs_forward_grad_energy": (_I, [_P, C.POINTER(Background), _D, _D, _D,
@@ -367,6 +374,107 @@ class Level:
     def close(self):
         if self.h:
             self.lib.axs_level_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BoxLevel:
+    """An AMR level stored as boxes with 3 ghost layers (axs_boxlevel)."""
+
+    def __init__(self, ctx: Context, boxes, domain, dx: float,
+                 precision: int = AXS_FP64):
+        """boxes: list of ((lo_x, lo_y, lo_z), (hi_x, hi_y, hi_z)), inclusive."""
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.domain = tuple(domain)
+        self.dx = dx
+        self.precision = precision
+        self.boxes = [tuple(map(tuple, b)) for b in boxes]
+        lo = (C.c_int * (3 * len(boxes)))(*[v for b in boxes for v in b[0]])
+        hi = (C.c_int * (3 * len(boxes)))(*[v for b in boxes for v in b[1]])
+        dom = (C.c_int * 3)(*domain)
+        h = _P()
+        check(self.lib.axs_boxlevel_create(ctx.h, len(boxes), lo, hi, dom, 3,
+                                           dx, precision, C.byref(h)))
+        self.h = h
+
+    @property
+    def num_cells(self) -> int:
+        return int(self.lib.axs_boxlevel_num_cells(self.h))
+
+    def upload_dense(self, dense: np.ndarray):
+        d = np.ascontiguousarray(dense, dtype=np.float64)
+        check(self.lib.axs_boxlevel_upload_dense(self.h, dptr(d)))
+
+    def download_dense(self, fill=np.nan) -> np.ndarray:
+        nx, ny, nz = self.domain
+        d = np.full((4, nz, ny, nx), fill)
+        check(self.lib.axs_boxlevel_download_dense(self.h, dptr(d)))
+        return d
+
+    def copy_from_level(self, lvl: Level):
+        check(self.lib.axs_boxlevel_copy_from_level(self.h, lvl.h))
+
+    def copy_to_level(self, lvl: Level):
+        check(self.lib.axs_boxlevel_copy_to_level(self.h, lvl.h))
+
+    def snapshot_coarse(self, coarse: "BoxLevel"):
+        check(self.lib.axs_boxlevel_snapshot_coarse(self.h, coarse.h))
+
+    def fill_ghosts(self, coarse: Optional["BoxLevel"] = None, t_frac=1.0,
+                    interior=False):
+        check(self.lib.axs_boxlevel_fill_ghosts(
+            self.h, coarse.h if coarse is not None else None, t_frac,
+            int(interior)))
+
+    def rk4_step(self, coeffs, dt, coarse: Optional["BoxLevel"] = None,
+                 t0_frac=0.0, dt_frac=1.0):
+        arr = (RhsCoeffs * 4)(*coeffs)
+        check(self.lib.axs_boxlevel_rk4_step(
+            self.h, arr, dt, coarse.h if coarse is not None else None,
+            t0_frac, dt_frac))
+
+    def tag(self, amp_thr=0.0, R=1.0, tags: Optional[DeviceBuffer] = None,
+            cover: Optional[DeviceBuffer] = None):
+        nx, ny, nz = self.domain
+        if tags is None:
+            tags = DeviceBuffer(nx * ny * nz)
+            tags.zero()
+        p = TaggerParams(DBL_MAX, DBL_MAX, amp_thr, R)
+        counts = (C.c_int64 * 2)()
+        check(self.lib.axs_boxlevel_tag(
+            self.h, C.byref(p), _P(tags.ptr),
+            _P(cover.ptr) if cover is not None else None, counts))
+        self.ctx.synchronize()
+        return tags, (int(counts[0]), int(counts[1]))
+
+    def energy_sums(self, R, lam, RpR, scheme=AXS_MASK_NONE, thr=0.8,
+                    cover: Optional[DeviceBuffer] = None) -> np.ndarray:
+        out = np.zeros(14)
+        m = Masking(scheme, thr)
+        check(self.lib.axs_boxlevel_energy_sums(
+            self.h, R, lam, RpR, C.byref(m),
+            _P(cover.ptr) if cover is not None else None, dptr(out)))
+        return out
+
+    def coverage_mask(self, coarse_domain) -> DeviceBuffer:
+        cd = (C.c_int * 3)(*coarse_domain)
+        buf = DeviceBuffer(int(np.prod(coarse_domain)))
+        check(self.lib.axs_boxlevel_coverage_mask(self.h, cd, _P(buf.ptr)))
+        self.ctx.synchronize()
+        return buf
+
+    def average_down_to(self, coarse: "BoxLevel"):
+        check(self.lib.axs_boxlevel_average_down(self.h, coarse.h))
+
+    def close(self):
+        if self.h:
+            self.lib.axs_boxlevel_destroy(self.h)
             self.h = None
 
     def __del__(self):

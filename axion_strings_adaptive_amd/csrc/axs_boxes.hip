@@ -1,36 +1,1350 @@
-// axs_boxes.hip -- AMR box levels (work-list launches).  Not built yet:
-// every entry point reports AXS_ERR_UNSUPPORTED.
+// axs_boxes.hip -- AMR box levels: one refinement level stored as a list
+// of boxes with 3 ghost layers each, advanced by persistent work-list
+// launches (the MultiFab-of-boxes path of the reference's AMR hierarchy,
+// AxionStringsLevel.cpp and GRTeclyn's GRAmrLevel RK4 with FillPatch).
+//
+// Layout: component-major over the whole level; box b's grown region
+// (n + 2 g cells per edge) starts at off[b] of every component array.
+// A block-owner map (8^3-cell blocks, the blocking factor) gives, for any
+// cell of the periodic domain, the box that owns it, so ghost fills and
+// coarse lookups need no host-built index lists.
+//
+// Work-list: every box is cut into items of H rows x full box width W
+// (W * H <= 256 lanes), each item streams through the box's z extent with
+// the same register-queue + LDS-plane scheme as the uniform kernel, and a
+// persistent grid strides over the items.
 #include "axs_internal.hpp"
+#include "axs_stage_math.hpp"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace axs
+{
+
+constexpr int kBlock = 8;   // blocking factor of box corners
+constexpr int kBoxLds = 576; // (W + 4) (H + 4) <= 544 for 4 <= W <= 64
+
+struct BoxD
+{
+    int lo[3];
+    int n[3];
+    uint32_t off; // grown-region start in every component array
+    int pad;
+};
+
+struct WorkItem
+{
+    int box, y0, h, w;
+};
+
+} // namespace axs
+
+struct axs_boxlevel
+{
+    axs_ctx *ctx;
+    int nb;
+    int dom[3];
+    int ng;
+    double dx;
+    int precision;
+    size_t word;
+    std::vector<axs::BoxD> hbox;
+    axs::BoxD *dbox;
+    int *downer;  // [dom/8]^3 block owners (-1 = not covered)
+    int bdim[3];  // dom / 8
+    uint32_t total; // grown cells per component
+    int64_t nvalid;
+    void *y, *acc, *ya, *yb;
+    float *comp;
+    axs::WorkItem *ditems;
+    int nitems;
+    double *dcoeff;
+    void *snap;      // coarse snapshot (y [+ comp]) for time interpolation
+    float *snap_lo;
+    size_t snap_bytes;
+    const axs_boxlevel *snap_of;
+    unsigned long long *derr;
+};
+
+namespace axs
+{
+
+// ------------------------------------------------------------- views ----
+struct LevelView
+{
+    const BoxD *box;
+    const int *owner;
+    int dom[3], bdim[3];
+    int ng;
+    uint32_t total;
+};
+
+static LevelView view_of(const axs_boxlevel *b)
+{
+    LevelView v{};
+    v.box = b->dbox;
+    v.owner = b->downer;
+    for (int d = 0; d < 3; ++d)
+    {
+        v.dom[d] = b->dom[d];
+        v.bdim[d] = b->bdim[d];
+    }
+    v.ng = b->ng;
+    v.total = b->total;
+    return v;
+}
+
+__device__ __forceinline__ int pmod(int a, int n)
+{
+    a %= n;
+    return a < 0 ? a + n : a;
+}
+
+// storage index (component 0) of global cell (x, y, z) inside box `bi`'s
+// grown region, the cell given in that box's index frame
+__device__ __forceinline__ uint32_t sidx(const BoxD &B, int ng, int lx, int ly,
+                                         int lz)
+{
+    const uint32_t gx = B.n[0] + 2 * ng, gy = B.n[1] + 2 * ng;
+    return B.off + ((uint32_t)(lz + ng) * gy + (uint32_t)(ly + ng)) * gx +
+           (uint32_t)(lx + ng);
+}
+
+// owner box of a periodic global cell and its storage index, or -1
+__device__ __forceinline__ int locate(const LevelView &v, int x, int y, int z,
+                                      uint32_t &idx)
+{
+    x = pmod(x, v.dom[0]);
+    y = pmod(y, v.dom[1]);
+    z = pmod(z, v.dom[2]);
+    const int o = v.owner[((z / kBlock) * v.bdim[1] + y / kBlock) * v.bdim[0] +
+                          x / kBlock];
+    if (o < 0)
+        return -1;
+    const BoxD B = v.box[o];
+    idx = sidx(B, v.ng, x - B.lo[0], y - B.lo[1], z - B.lo[2]);
+    return o;
+}
+
+// 5-point Lagrange weights L_m(t) = prod_{k != m} (t - k)/(m - k), nodes
+// m = -2..2, at offset t = -1/4 (even fine index) and +1/4 (odd) from the
+// nearest coarse cell (exact binary fractions).
+__constant__ double kQuartW[2][5] = {
+    {-0.02197265625, 0.205078125, 0.9228515625, -0.123046875, 0.01708984375},
+    {0.01708984375, -0.123046875, 0.9228515625, 0.205078125, -0.02197265625}};
+
+template <typename T> struct CoarseSrc
+{
+    LevelView v;
+    const T *cur;
+    const float *cur_lo;
+    const T *snap;
+    const float *snap_lo;
+    double t_frac;
+};
+
+template <typename T>
+__device__ double coarse_value(const CoarseSrc<T> &c, int comp, int cx, int cy,
+                               int cz, unsigned long long *err)
+{
+    uint32_t i;
+    if (locate(c.v, cx, cy, cz, i) < 0)
+    {
+        atomicAdd(err, 1ull);
+        return 0.0;
+    }
+    const uint32_t o = (uint32_t)comp * c.v.total + i;
+    double v = (double)c.cur[o] + (c.cur_lo ? (double)c.cur_lo[o] : 0.0);
+    if (c.snap && c.t_frac < 1.0)
+    {
+        const double s =
+            (double)c.snap[o] + (c.snap_lo ? (double)c.snap_lo[o] : 0.0);
+        v = (1.0 - c.t_frac) * s + c.t_frac * v;
+    }
+    return v;
+}
+
+// Tensor-product quartic interpolation of coarse component `comp` at the
+// fine cell (fx, fy, fz) (ratio 2, cell centred).
+template <typename T>
+__device__ double interp_quartic(const CoarseSrc<T> &c, int comp, int fx,
+                                 int fy, int fz, unsigned long long *err)
+{
+    const int cx = fx >> 1, cy = fy >> 1, cz = fz >> 1; // floor for fx >= 0
+    const double *wx = kQuartW[fx & 1], *wy = kQuartW[fy & 1],
+                 *wz = kQuartW[fz & 1];
+    double acc = 0.0;
+    for (int k = 0; k < 5; ++k)
+    {
+        double ay = 0.0;
+        for (int j = 0; j < 5; ++j)
+        {
+            double ax = 0.0;
+            for (int i = 0; i < 5; ++i)
+                ax += wx[i] * coarse_value(c, comp, cx + i - 2, cy + j - 2,
+                                           cz + k - 2, err);
+            ay += wy[j] * ax;
+        }
+        acc += wz[k] * ay;
+    }
+    return acc;
+}
+
+// ghost fill (and optional interior prolongation) of one buffer
+template <typename T>
+__global__ void k_fill(LevelView f, T *buf, float *lo, int have_coarse,
+                       CoarseSrc<T> c, int interior, unsigned long long *err)
+{
+    const BoxD B = f.box[blockIdx.y];
+    const int ng = f.ng;
+    const int gx = B.n[0] + 2 * ng, gy = B.n[1] + 2 * ng, gz = B.n[2] + 2 * ng;
+    const uint32_t cells = (uint32_t)gx * gy * gz;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < cells;
+         t += gridDim.x * blockDim.x)
+    {
+        const int lx = (int)(t % gx) - ng, ly = (int)((t / gx) % gy) - ng,
+                  lz = (int)(t / ((uint32_t)gx * gy)) - ng;
+        const bool valid = lx >= 0 && lx < B.n[0] && ly >= 0 && ly < B.n[1] &&
+                           lz >= 0 && lz < B.n[2];
+        if (valid && !interior)
+            continue;
+        const uint32_t dst = B.off + t;
+        const int X = B.lo[0] + lx, Y = B.lo[1] + ly, Z = B.lo[2] + lz;
+        uint32_t src;
+        if (!interior && locate(f, X, Y, Z, src) >= 0)
+        {
+            for (int comp = 0; comp < 4; ++comp)
+            {
+                buf[comp * f.total + dst] = buf[comp * f.total + src];
+                if (lo)
+                    lo[comp * f.total + dst] = lo[comp * f.total + src];
+            }
+            continue;
+        }
+        if (!have_coarse)
+        {
+            atomicAdd(err, 1ull);
+            continue;
+        }
+        const int px = pmod(X, f.dom[0]), py = pmod(Y, f.dom[1]),
+                  pz = pmod(Z, f.dom[2]);
+        for (int comp = 0; comp < 4; ++comp)
+        {
+            const double v = interp_quartic(c, comp, px, py, pz, err);
+            const T h = (T)v;
+            buf[comp * f.total + dst] = h;
+            if (lo)
+                lo[comp * f.total + dst] = (float)(v - (double)h);
+        }
+    }
+}
+
+// --------------------------------------------------- work-list stage ----
+template <typename T> struct BoxStageParams
+{
+    const BoxD *box;
+    const WorkItem *items;
+    int nitems;
+    int ng;
+    uint32_t total;
+    const T *yin;
+    const T *yn;
+    T *acc;
+    T *yout;
+    float *comp;
+    const double *coeff; // 12 doubles: stage s at [3s .. 3s+2]
+    int coeff_stage;
+    double a, h6;
+    float h6_lo;
+    double inv_dx2;
+};
+
+template <typename T> struct StarOfB;
+template <> struct StarOfB<double>
+{
+    using type = Star;
+};
+template <> struct StarOfB<float>
+{
+    using type = StarF;
+};
+
+template <typename T, int PREC, int MODE>
+__global__ void __launch_bounds__(256) k_box_stage(BoxStageParams<T> p)
+{
+    __shared__ T sm[2][2][kBoxLds];
+    const double *cf = p.coeff + 3 * p.coeff_stage;
+    const double cc = cf[0], lam = cf[1], R2 = cf[2];
+    const T i2 = (T)p.inv_dx2;
+    const uint32_t tot = p.total;
+    const int ng = p.ng;
+    for (int it = blockIdx.x; it < p.nitems; it += gridDim.x)
+    {
+        const WorkItem w = p.items[it];
+        const BoxD B = p.box[w.box];
+        const int W = w.w, H = w.h, nyb = B.n[1], nzb = B.n[2];
+        const int SW = W + 4;
+        const int t = threadIdx.x;
+        const int lx = t % W, ly = t / W;
+        const bool lane = ly < H; // has a tile slot
+        const int yy = w.y0 + ly;
+        const bool active = lane && yy < nyb;
+        const int yl = min(yy, nyb + 1); // clamped load row (in the ghosts)
+        const uint32_t gxs = W + 2 * ng, gys = nyb + 2 * ng;
+        const uint32_t plane = gxs * gys;
+        auto at = [&](int x, int y, int z) -> uint32_t {
+            return B.off + ((uint32_t)(z + ng) * gys + (uint32_t)(y + ng)) * gxs +
+                   (uint32_t)(x + ng);
+        };
+        const bool hxl = lane && lx < 2, hxr = lane && lx >= W - 2;
+        const bool hyl = lane && ly < 2, hyr = lane && ly >= H - 2;
+        const int hxx = hxl ? lx - 2 : W + (lx - (W - 2));
+        const int hyy = hyl ? w.y0 + ly - 2
+                            : min(w.y0 + H + (ly - (H - 2)), nyb + 1);
+        const T *p1 = p.yin, *p2 = p.yin + tot;
+        T q1[5] = {0, 0, 0, 0, 0}, q2[5] = {0, 0, 0, 0, 0};
+        if (lane)
+            for (int i = 0; i < 5; ++i)
+            {
+                const uint32_t o = at(lx, yl, i - 2);
+                q1[i] = p1[o];
+                q2[i] = p2[o];
+            }
+        T hx1 = 0, hx2 = 0, hy1 = 0, hy2 = 0;
+        if (hxl || hxr)
+        {
+            hx1 = p1[at(hxx, yl, 0)];
+            hx2 = p2[at(hxx, yl, 0)];
+        }
+        if (hyl || hyr)
+        {
+            hy1 = p1[at(lx, hyy, 0)];
+            hy2 = p2[at(lx, hyy, 0)];
+        }
+        for (int z = 0; z < nzb; ++z)
+        {
+            const int b = z & 1;
+            T *s0 = sm[b][0], *s1 = sm[b][1];
+            if (lane)
+            {
+                s0[(ly + 2) * SW + lx + 2] = q1[2];
+                s1[(ly + 2) * SW + lx + 2] = q2[2];
+                if (hxl)
+                {
+                    s0[(ly + 2) * SW + lx] = hx1;
+                    s1[(ly + 2) * SW + lx] = hx2;
+                }
+                else if (hxr)
+                {
+                    s0[(ly + 2) * SW + lx + 4] = hx1;
+                    s1[(ly + 2) * SW + lx + 4] = hx2;
+                }
+                if (hyl)
+                {
+                    s0[ly * SW + lx + 2] = hy1;
+                    s1[ly * SW + lx + 2] = hy2;
+                }
+                else if (hyr)
+                {
+                    s0[(ly + 4) * SW + lx + 2] = hy1;
+                    s1[(ly + 4) * SW + lx + 2] = hy2;
+                }
+            }
+            T nq1 = q1[4], nq2 = q2[4];
+            if (lane && z + 1 < nzb)
+            {
+                const uint32_t o3 = at(lx, yl, z + 3);
+                nq1 = p1[o3];
+                nq2 = p2[o3];
+                if (hxl || hxr)
+                {
+                    hx1 = p1[at(hxx, yl, z + 1)];
+                    hx2 = p2[at(hxx, yl, z + 1)];
+                }
+                if (hyl || hyr)
+                {
+                    hy1 = p1[at(lx, hyy, z + 1)];
+                    hy2 = p2[at(lx, hyy, z + 1)];
+                }
+            }
+            const uint32_t oc = active ? at(lx, yy, z) : 0;
+            T pi[2] = {0, 0}, yn[4] = {0, 0, 0, 0}, ac[4] = {0, 0, 0, 0};
+            float lo[4] = {0, 0, 0, 0};
+            if (active)
+            {
+                pi[0] = p.yin[2 * tot + oc];
+                pi[1] = p.yin[3 * tot + oc];
+                if (MODE == MODE_S23 || MODE == MODE_S4)
+                    for (int c = 0; c < 4; ++c)
+                    {
+                        yn[c] = p.yn[c * tot + oc];
+                        ac[c] = p.acc[c * tot + oc];
+                    }
+                if (MODE == MODE_S4 && PREC == AXS_FP32C)
+                    for (int c = 0; c < 4; ++c)
+                        lo[c] = p.comp[c * tot + oc];
+            }
+            __syncthreads();
+            if (active)
+            {
+                const int r = (ly + 2) * SW + lx + 2;
+                using S = typename StarOfB<T>::type;
+                const S a1{s0[r - 2], s0[r - 1], s0[r + 1], s0[r + 2],
+                           s0[r - 2 * SW], s0[r - SW], s0[r + SW], s0[r + 2 * SW],
+                           q1[0], q1[1], q1[3], q1[4], q1[2]};
+                const S a2{s1[r - 2], s1[r - 1], s1[r + 1], s1[r + 2],
+                           s1[r - 2 * SW], s1[r - SW], s1[r + SW], s1[r + 2 * SW],
+                           q2[0], q2[1], q2[3], q2[4], q2[2]};
+                T F1, F2;
+                force(star_lap(a1, i2), star_lap(a2, i2), q1[2], q2[2], cc, lam,
+                      R2, F1, F2);
+                const T k[4] = {pi[0], pi[1], F1, F2};
+                if (MODE == MODE_S1)
+                {
+                    yn[0] = q1[2];
+                    yn[1] = q2[2];
+                    yn[2] = pi[0];
+                    yn[3] = pi[1];
+                }
+                T acc_out[4], y_out[4];
+                stage_update<T, PREC, MODE>(k, yn, ac, lo, p.a, p.h6, p.h6_lo,
+                                            acc_out, y_out);
+                for (int c = 0; c < 4; ++c)
+                {
+                    p.yout[c * tot + oc] = y_out[c];
+                    if (MODE == MODE_S1 || MODE == MODE_S23)
+                        p.acc[c * tot + oc] = acc_out[c];
+                    if (MODE == MODE_S4 && PREC == AXS_FP32C)
+                        p.comp[c * tot + oc] = lo[c];
+                }
+            }
+            for (int i = 0; i < 4; ++i)
+            {
+                q1[i] = q1[i + 1];
+                q2[i] = q2[i + 1];
+            }
+            q1[4] = nq1;
+            q2[4] = nq2;
+            (void)plane;
+        }
+        __syncthreads(); // LDS reuse by the next item
+    }
+}
+
+// -------------------------------------------------- windings / tagging --
+__device__ __forceinline__ double wrap_diff(double delta)
+{
+    const double pi = 3.14159265358979323846;
+    const double two_pi = 2.0 * pi;
+    delta = fmod(delta, two_pi);
+    if (delta <= -pi)
+        delta += two_pi;
+    else if (delta > pi)
+        delta -= two_pi;
+    return delta;
+}
+__device__ __forceinline__ int pwind(double t1, double t2, double t3, double t4)
+{
+#pragma clang fp contract(off)
+    const double two_pi = 2.0 * 3.14159265358979323846;
+    const double total = wrap_diff(t2 - t1) + wrap_diff(t3 - t2) +
+                         wrap_diff(t4 - t3) + wrap_diff(t1 - t4);
+    return (int)lround(total / two_pi);
+}
+
+template <typename T>
+__global__ void k_box_tag(LevelView v, const T *y, const float *lo,
+                          uint8_t *tags, const uint8_t *cover, double amp_thr,
+                          double R, unsigned long long *counts)
+{
+    const BoxD B = v.box[blockIdx.y];
+    const uint32_t cells = (uint32_t)B.n[0] * B.n[1] * B.n[2];
+    const uint32_t tot = v.total;
+    unsigned long long plain = 0, weighted = 0;
+    const int iters = (cells + gridDim.x * blockDim.x - 1) / (gridDim.x * blockDim.x);
+    for (int r = 0; r < iters; ++r)
+    {
+        const uint32_t t = (r * gridDim.x + blockIdx.x) * blockDim.x + threadIdx.x;
+        int wxy = 0, wyz = 0, wzx = 0;
+        if (t < cells)
+        {
+            const int lx = t % B.n[0], ly = (t / B.n[0]) % B.n[1],
+                      lz = t / ((uint32_t)B.n[0] * B.n[1]);
+            const int X = B.lo[0] + lx, Y = B.lo[1] + ly, Z = B.lo[2] + lz;
+            const uint32_t dense =
+                ((uint32_t)Z * v.dom[1] + Y) * v.dom[0] + X;
+            if (!cover || cover[dense])
+            {
+                auto ph = [&](int a, int b, int c) -> double {
+                    const uint32_t i = sidx(B, v.ng, lx + a, ly + b, lz + c);
+                    const double p1 = (double)y[i] + (lo ? (double)lo[i] : 0.0);
+                    const double p2 = (double)y[tot + i] +
+                                      (lo ? (double)lo[tot + i] : 0.0);
+                    return atan2(p2, p1);
+                };
+                const double t000 = ph(0, 0, 0), t100 = ph(1, 0, 0),
+                             t110 = ph(1, 1, 0), t010 = ph(0, 1, 0),
+                             t001 = ph(0, 0, 1), t101 = ph(1, 0, 1),
+                             t011 = ph(0, 1, 1);
+                wxy = pwind(t000, t100, t110, t010);
+                wyz = pwind(t000, t010, t011, t001);
+                wzx = pwind(t000, t001, t101, t100);
+                weighted += abs(wxy) + abs(wyz) + abs(wzx);
+                if (tags)
+                {
+                    bool tag = wxy != 0 || wyz != 0 || wzx != 0;
+                    if (!tag && amp_thr > 0.0)
+                    {
+                        const uint32_t i = sidx(B, v.ng, lx, ly, lz);
+                        const double p1 = (double)y[i] + (lo ? (double)lo[i] : 0.0);
+                        const double p2 = (double)y[tot + i] +
+                                          (lo ? (double)lo[tot + i] : 0.0);
+                        tag = sqrt(p1 * p1 + p2 * p2) / R < amp_thr;
+                    }
+                    if (tag)
+                        tags[dense] = 1;
+                }
+            }
+        }
+        plain += __popcll(__ballot(wxy != 0)) + __popcll(__ballot(wyz != 0)) +
+                 __popcll(__ballot(wzx != 0));
+    }
+    for (int o = 32; o > 0; o >>= 1)
+        weighted += __shfl_xor(weighted, o, 64);
+    if ((threadIdx.x & 63) == 0)
+    {
+        atomicAdd(&counts[0], plain);
+        atomicAdd(&counts[1], weighted);
+    }
+}
+
+// ------------------------------------------------------------- energy ----
+__device__ __forceinline__ double d1r(double fm2, double fm1, double fp1,
+                                      double fp2, double i1)
+{
+#pragma clang fp contract(off)
+    const double w_far = 8.33333333333333333333e-2;
+    const double w_near = 6.66666666666666666667e-1;
+    return (w_far * fm2 - w_near * fm1 + w_near * fp1 - w_far * fp2) * i1;
+}
+
+template <typename T>
+__global__ void k_box_energy(LevelView v, const T *y, const float *lo,
+                             const uint8_t *cover, double dx, double R,
+                             double lambda, double RpR, int scheme, double thr,
+                             double *partials)
+{
+    const BoxD B = v.box[blockIdx.y];
+    const uint32_t cells = (uint32_t)B.n[0] * B.n[1] * B.n[2];
+    const uint32_t tot = v.total;
+    const double i1 = 1.0 / dx, R2 = R * R, R4 = R2 * R2, f_a = 1.4142135623730951;
+    double acc[14];
+    for (int q = 0; q < 14; ++q)
+        acc[q] = 0.0;
+    auto val = [&](int comp, uint32_t i) -> double {
+        const uint32_t o = comp * tot + i;
+        return (double)y[o] + (lo ? (double)lo[o] : 0.0);
+    };
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < cells;
+         t += gridDim.x * blockDim.x)
+    {
+        const int lx = t % B.n[0], ly = (t / B.n[0]) % B.n[1],
+                  lz = t / ((uint32_t)B.n[0] * B.n[1]);
+        const uint32_t dense = ((uint32_t)(B.lo[2] + lz) * v.dom[1] +
+                                (B.lo[1] + ly)) *
+                                   v.dom[0] +
+                               (B.lo[0] + lx);
+        if (cover && !cover[dense])
+            continue;
+        const uint32_t c = sidx(B, v.ng, lx, ly, lz);
+        const double psi1 = val(0, c), psi2 = val(1, c), Pi1 = val(2, c),
+                     Pi2 = val(3, c);
+        const double psi_sq = psi1 * psi1 + psi2 * psi2;
+        double g1sq = 0.0, g2sq = 0.0, gtsq = 0.0;
+        for (int d = 0; d < 3; ++d)
+        {
+            const int ex = d == 0, ey = d == 1, ez = d == 2;
+            auto s = [&](int comp, int m) -> double {
+                return val(comp, sidx(B, v.ng, lx + m * ex, ly + m * ey,
+                                      lz + m * ez));
+            };
+            const double a = d1r(s(0, -2), s(0, -1), s(0, 1), s(0, 2), i1);
+            const double b = d1r(s(1, -2), s(1, -1), s(1, 1), s(1, 2), i1);
+            g1sq += a * a;
+            g2sq += b * b;
+            const double dth = psi_sq > 0.0 ? (psi1 * b - psi2 * a) / psi_sq : 0.0;
+            gtsq += dth * dth;
+        }
+        const double c1 = Pi1 - RpR * psi1, c2 = Pi2 - RpR * psi2;
+        const double dp = psi_sq - R2;
+        const double pot = 0.25 * lambda * dp * dp;
+        const double rho = (c1 * c1 + c2 * c2 + (g1sq + g2sq) + pot) / R4;
+        const double tp = psi_sq > 0.0 ? (psi1 * Pi2 - Pi1 * psi2) / psi_sq : 0.0;
+        const double adot = f_a * tp / R;
+        const double rak = 0.5 * adot * adot;
+        const double rag = 0.5 * f_a * f_a * gtsq / R2;
+        double rrk = 0.0;
+        if (psi_sq > 0.0)
+        {
+            const double rd = (psi1 * c1 + psi2 * c2) / sqrt(psi_sq);
+            rrk = rd * rd / R4;
+        }
+        const double rrg = (g1sq + g2sq - psi_sq * gtsq) / R4;
+        const double rrm = pot / R4;
+        double wgt = 1.0;
+        if (scheme != AXS_MASK_NONE)
+        {
+            const double mod = sqrt(psi_sq) / R;
+            if (scheme == AXS_MASK_B)
+                wgt = mod >= thr ? 1.0 : 0.0;
+            else
+            {
+                const double f = 1.0 + (mod - 1.0) / f_a;
+                wgt = f * f;
+            }
+        }
+        const double vals[14] = {rho, rho * wgt, rak, rak * wgt, rag, rag * wgt,
+                                 rrk, rrk * wgt, rrg, rrg * wgt, rrm, rrm * wgt,
+                                 wgt, 1.0};
+        for (int q = 0; q < 14; ++q)
+            acc[q] += vals[q];
+    }
+    __shared__ double red[256];
+    const uint32_t row = blockIdx.y * gridDim.x + blockIdx.x;
+    for (int q = 0; q < 14; ++q)
+    {
+        red[threadIdx.x] = acc[q];
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1)
+        {
+            if ((int)threadIdx.x < o)
+                red[threadIdx.x] += red[threadIdx.x + o];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0)
+            partials[(size_t)row * 14 + q] = red[0];
+        __syncthreads();
+    }
+}
+
+__global__ void k_box_sum_cols(const double *part, int nrow, int width,
+                               double *out)
+{
+    __shared__ double red[256];
+    const int col = blockIdx.x;
+    double s = 0.0;
+    for (int r = threadIdx.x; r < nrow; r += 256)
+        s += part[(size_t)r * width + col];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1)
+    {
+        if ((int)threadIdx.x < o)
+            red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        out[col] = red[0];
+}
+
+// ------------------------------------------- masks, average down, copies --
+__global__ void k_set_u8(uint8_t *m, size_t n, uint8_t v)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        m[i] = v;
+}
+
+__global__ void k_coverage(LevelView f, int cdx, int cdy, uint8_t *mask)
+{
+    const BoxD B = f.box[blockIdx.y];
+    const int cx = B.n[0] / 2, cy = B.n[1] / 2, cz = B.n[2] / 2;
+    const uint32_t cells = (uint32_t)cx * cy * cz;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < cells;
+         t += gridDim.x * blockDim.x)
+    {
+        const int x = B.lo[0] / 2 + t % cx, y = B.lo[1] / 2 + (t / cx) % cy,
+                  z = B.lo[2] / 2 + t / ((uint32_t)cx * cy);
+        mask[((size_t)z * cdy + y) * cdx + x] = 0;
+    }
+}
+
+template <typename T>
+__global__ void k_average_down(LevelView f, const T *fy, const float *flo,
+                               LevelView c, T *cy_, float *clo,
+                               unsigned long long *err)
+{
+    const BoxD B = f.box[blockIdx.y];
+    const int nx = B.n[0] / 2, ny = B.n[1] / 2, nz = B.n[2] / 2;
+    const uint32_t cells = (uint32_t)nx * ny * nz;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < cells;
+         t += gridDim.x * blockDim.x)
+    {
+        const int ix = t % nx, iy = (t / nx) % ny, iz = t / ((uint32_t)nx * ny);
+        uint32_t ci;
+        if (locate(c, B.lo[0] / 2 + ix, B.lo[1] / 2 + iy, B.lo[2] / 2 + iz, ci) <
+            0)
+        {
+            atomicAdd(err, 1ull);
+            continue;
+        }
+        for (int comp = 0; comp < 4; ++comp)
+        {
+            double s = 0.0;
+            for (int k = 0; k < 2; ++k)
+                for (int j = 0; j < 2; ++j)
+                    for (int i = 0; i < 2; ++i)
+                    {
+                        const uint32_t o = comp * f.total +
+                                           sidx(B, f.ng, 2 * ix + i, 2 * iy + j,
+                                                2 * iz + k);
+                        s += (double)fy[o] + (flo ? (double)flo[o] : 0.0);
+                    }
+            const double v = 0.125 * s;
+            const T h = (T)v;
+            cy_[comp * c.total + ci] = h;
+            if (clo)
+                clo[comp * c.total + ci] = (float)(v - (double)h);
+        }
+    }
+}
+
+template <typename T>
+__global__ void k_copy_level(LevelView v, T *by, float *blo, T *uy, float *ulo,
+                             uint32_t un, int to_level)
+{
+    const BoxD B = v.box[blockIdx.y];
+    const uint32_t cells = (uint32_t)B.n[0] * B.n[1] * B.n[2];
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < cells;
+         t += gridDim.x * blockDim.x)
+    {
+        const int lx = t % B.n[0], ly = (t / B.n[0]) % B.n[1],
+                  lz = t / ((uint32_t)B.n[0] * B.n[1]);
+        const uint32_t u = ((uint32_t)(B.lo[2] + lz) * v.dom[1] +
+                            (B.lo[1] + ly)) *
+                               v.dom[0] +
+                           (B.lo[0] + lx);
+        const uint32_t b = sidx(B, v.ng, lx, ly, lz);
+        for (int comp = 0; comp < 4; ++comp)
+        {
+            if (to_level)
+            {
+                uy[comp * un + u] = by[comp * v.total + b];
+                if (ulo && blo)
+                    ulo[comp * un + u] = blo[comp * v.total + b];
+            }
+            else
+            {
+                by[comp * v.total + b] = uy[comp * un + u];
+                if (ulo && blo)
+                    blo[comp * v.total + b] = ulo[comp * un + u];
+            }
+        }
+    }
+}
+
+__global__ void k_split(const double *src, float *hi, float *lo, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+    {
+        const double v = src[i];
+        const float h = (float)v;
+        hi[i] = h;
+        if (lo)
+            lo[i] = (float)(v - (double)h);
+    }
+}
+
+__global__ void k_join(const float *hi, const float *lo, double *dst, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = (double)hi[i] + (lo ? (double)lo[i] : 0.0);
+}
+
+__global__ void k_zero_err(unsigned long long *e)
+{
+    if (threadIdx.x < 4)
+        e[threadIdx.x] = 0ull;
+}
+
+static int gblk(size_t n, int cap = 4096)
+{
+    size_t b = (n + 255) / 256;
+    return (int)std::max<size_t>(1, std::min<size_t>(b, cap));
+}
+
+// per-box 2D grid: x = chunks of the box's cells, y = boxes
+static dim3 box_grid(const axs_boxlevel *b, size_t max_cells)
+{
+    const size_t chunks = std::min<size_t>((max_cells + 255) / 256, 64);
+    return dim3((unsigned)std::max<size_t>(1, chunks), (unsigned)b->nb);
+}
+
+static size_t max_grown(const axs_boxlevel *b)
+{
+    size_t m = 0;
+    for (const BoxD &B : b->hbox)
+        m = std::max(m, (size_t)(B.n[0] + 2 * b->ng) * (B.n[1] + 2 * b->ng) *
+                            (B.n[2] + 2 * b->ng));
+    return m;
+}
+
+static size_t max_valid(const axs_boxlevel *b)
+{
+    size_t m = 0;
+    for (const BoxD &B : b->hbox)
+        m = std::max(m, (size_t)B.n[0] * B.n[1] * B.n[2]);
+    return m;
+}
+
+static int check_err(axs_boxlevel *bl, const char *what)
+{
+    unsigned long long e = 0;
+    AXS_HIP_TRY(hipMemcpyAsync(&e, bl->derr, sizeof(e), hipMemcpyDeviceToHost,
+                               bl->ctx->stream));
+    AXS_HIP_TRY(hipStreamSynchronize(bl->ctx->stream));
+    if (e)
+        return set_error(AXS_ERR_ARG,
+                         std::string(what) + ": " + std::to_string(e) +
+                             " cells had no owning box (proper nesting / "
+                             "coarse coverage violated)");
+    return AXS_OK;
+}
+
+template <typename T>
+static int fill_t(axs_boxlevel *bl, void *buf, float *lo,
+                  const axs_boxlevel *coarse, double t_frac, int interior)
+{
+    CoarseSrc<T> c{};
+    if (coarse)
+    {
+        c.v = view_of(coarse);
+        c.cur = (const T *)coarse->y;
+        c.cur_lo = coarse->comp;
+        if (bl->snap && bl->snap_of == coarse)
+        {
+            c.snap = (const T *)bl->snap;
+            c.snap_lo = bl->snap_lo;
+        }
+        c.t_frac = t_frac;
+    }
+    hipLaunchKernelGGL(k_fill<T>, box_grid(bl, max_grown(bl)), dim3(256), 0,
+                       bl->ctx->stream, view_of(bl), (T *)buf, lo,
+                       coarse ? 1 : 0, c, interior, bl->derr);
+    AXS_HIP_TRY(hipGetLastError());
+    return AXS_OK;
+}
+
+static int fill(axs_boxlevel *bl, void *buf, float *lo,
+                const axs_boxlevel *coarse, double t_frac, int interior)
+{
+    if (bl->precision == AXS_FP64)
+        return fill_t<double>(bl, buf, lo, coarse, t_frac, interior);
+    return fill_t<float>(bl, buf, lo, coarse, t_frac, interior);
+}
+
+template <typename T, int PREC>
+static int stage_t(axs_boxlevel *bl, int stage, const void *yin, void *yout,
+                   double dt)
+{
+    BoxStageParams<T> p{};
+    p.box = bl->dbox;
+    p.items = bl->ditems;
+    p.nitems = bl->nitems;
+    p.ng = bl->ng;
+    p.total = bl->total;
+    p.yin = (const T *)yin;
+    p.yn = (const T *)bl->y;
+    p.acc = (T *)bl->acc;
+    p.yout = (T *)yout;
+    p.comp = bl->comp;
+    p.coeff = bl->dcoeff;
+    p.coeff_stage = stage - 1;
+    p.a = stage == 3 ? dt : 0.5 * dt;
+    p.h6 = dt / 6.0;
+    p.h6_lo = (float)(p.h6 - (double)(float)p.h6);
+    p.inv_dx2 = 1.0 / (bl->dx * bl->dx);
+    const int grid = std::min(bl->nitems, 2048);
+    hipStream_t s = bl->ctx->stream;
+    switch (stage)
+    {
+    case 1:
+        hipLaunchKernelGGL((k_box_stage<T, PREC, MODE_S1>), dim3(grid), dim3(256),
+                           0, s, p);
+        break;
+    case 4:
+        hipLaunchKernelGGL((k_box_stage<T, PREC, MODE_S4>), dim3(grid), dim3(256),
+                           0, s, p);
+        break;
+    default:
+        hipLaunchKernelGGL((k_box_stage<T, PREC, MODE_S23>), dim3(grid),
+                           dim3(256), 0, s, p);
+    }
+    AXS_HIP_TRY(hipGetLastError());
+    return AXS_OK;
+}
+
+static int stage(axs_boxlevel *bl, int st, const void *yin, void *yout,
+                 double dt)
+{
+    if (bl->precision == AXS_FP64)
+        return stage_t<double, AXS_FP64>(bl, st, yin, yout, dt);
+    if (bl->precision == AXS_FP32C)
+        return stage_t<float, AXS_FP32C>(bl, st, yin, yout, dt);
+    return stage_t<float, AXS_FP32>(bl, st, yin, yout, dt);
+}
+
+} // namespace axs
+
+using namespace axs;
 
 extern "C" {
-int axs_boxlevel_create(axs_ctx *, int, const int *, const int *, const int *,
-                        int, double, int, axs_boxlevel **out)
+
+int axs_boxlevel_create(axs_ctx *ctx, int nboxes, const int *lo, const int *hi,
+                        const int domain[3], int nghost, double dx,
+                        int precision, axs_boxlevel **out)
 {
-    if (out)
-        *out = nullptr;
-    return axs::set_error(AXS_ERR_UNSUPPORTED, "box levels not built");
+    AXS_CHECK_ARG(ctx && lo && hi && domain && out,
+                  "axs_boxlevel_create: NULL argument");
+    *out = nullptr;
+    AXS_CHECK_ARG(nboxes >= 1 && nboxes <= 65535,
+                  "axs_boxlevel_create: 1 <= nboxes <= 65535");
+    AXS_CHECK_ARG(nghost == 3, "axs_boxlevel_create: nghost must be 3");
+    AXS_CHECK_ARG(dx > 0.0, "axs_boxlevel_create: dx must be > 0");
+    AXS_CHECK_ARG(precision == AXS_FP64 || precision == AXS_FP32C ||
+                      precision == AXS_FP32,
+                  "axs_boxlevel_create: unknown precision");
+    for (int d = 0; d < 3; ++d)
+        AXS_CHECK_ARG(domain[d] >= kBlock && domain[d] % kBlock == 0,
+                      "axs_boxlevel_create: domain must be a multiple of 8");
+    axs_boxlevel *b = new axs_boxlevel{};
+    b->ctx = ctx;
+    b->nb = nboxes;
+    b->ng = nghost;
+    b->dx = dx;
+    b->precision = precision;
+    b->word = precision == AXS_FP64 ? 8 : 4;
+    for (int d = 0; d < 3; ++d)
+    {
+        b->dom[d] = domain[d];
+        b->bdim[d] = domain[d] / kBlock;
+    }
+    std::vector<int> owner((size_t)b->bdim[0] * b->bdim[1] * b->bdim[2], -1);
+    uint64_t off = 0;
+    int64_t nvalid = 0;
+    std::vector<WorkItem> items;
+    for (int i = 0; i < nboxes; ++i)
+    {
+        BoxD B{};
+        for (int d = 0; d < 3; ++d)
+        {
+            B.lo[d] = lo[3 * i + d];
+            B.n[d] = hi[3 * i + d] - lo[3 * i + d] + 1;
+            if (B.lo[d] < 0 || B.lo[d] % kBlock || B.n[d] % kBlock ||
+                B.n[d] < kBlock || B.n[d] > 64 || B.lo[d] + B.n[d] > domain[d])
+            {
+                delete b;
+                return set_error(AXS_ERR_ARG,
+                                 "axs_boxlevel_create: box " + std::to_string(i) +
+                                     " must lie in the domain with corners and "
+                                     "edges on multiples of 8, edges 8..64");
+            }
+        }
+        B.off = (uint32_t)off;
+        off += (uint64_t)(B.n[0] + 2 * nghost) * (B.n[1] + 2 * nghost) *
+               (B.n[2] + 2 * nghost);
+        nvalid += (int64_t)B.n[0] * B.n[1] * B.n[2];
+        for (int z = B.lo[2] / kBlock; z < (B.lo[2] + B.n[2]) / kBlock; ++z)
+            for (int y = B.lo[1] / kBlock; y < (B.lo[1] + B.n[1]) / kBlock; ++y)
+                for (int x = B.lo[0] / kBlock; x < (B.lo[0] + B.n[0]) / kBlock;
+                     ++x)
+                {
+                    int &o = owner[((size_t)z * b->bdim[1] + y) * b->bdim[0] + x];
+                    if (o >= 0)
+                    {
+                        delete b;
+                        return set_error(AXS_ERR_ARG,
+                                         "axs_boxlevel_create: boxes overlap");
+                    }
+                    o = i;
+                }
+        const int W = B.n[0];
+        const int H = std::max(1, 256 / W);
+        for (int y0 = 0; y0 < B.n[1]; y0 += H)
+            items.push_back(WorkItem{i, y0, std::min(H, 256 / W), W});
+        b->hbox.push_back(B);
+    }
+    if (off * 4 >= (1ull << 32))
+    {
+        delete b;
+        return set_error(AXS_ERR_ARG, "axs_boxlevel_create: level too large");
+    }
+    b->total = (uint32_t)off;
+    b->nvalid = nvalid;
+    b->nitems = (int)items.size();
+    const size_t bytes4 = (size_t)4 * off * b->word;
+    void *yab = nullptr;
+    bool ok = hipMalloc(&b->y, bytes4) == hipSuccess &&
+              hipMalloc(&b->acc, bytes4) == hipSuccess &&
+              hipMalloc(&yab, 2 * bytes4) == hipSuccess &&
+              hipMalloc(&b->dbox, nboxes * sizeof(BoxD)) == hipSuccess &&
+              hipMalloc(&b->downer, owner.size() * sizeof(int)) == hipSuccess &&
+              hipMalloc(&b->ditems, items.size() * sizeof(WorkItem)) ==
+                  hipSuccess &&
+              hipMalloc(&b->dcoeff, 12 * sizeof(double)) == hipSuccess &&
+              hipMalloc(&b->derr, 4 * sizeof(unsigned long long)) == hipSuccess;
+    if (ok && precision == AXS_FP32C)
+        ok = hipMalloc(&b->comp, bytes4) == hipSuccess;
+    if (!ok)
+    {
+        hipFree(b->y);
+        hipFree(b->acc);
+        hipFree(yab);
+        hipFree(b->dbox);
+        hipFree(b->downer);
+        hipFree(b->ditems);
+        hipFree(b->dcoeff);
+        hipFree(b->derr);
+        hipFree(b->comp);
+        delete b;
+        return set_error(AXS_ERR_NOMEM, "axs_boxlevel_create: out of memory");
+    }
+    b->ya = yab;
+    b->yb = (char *)yab + bytes4;
+    hipStream_t s = ctx->stream;
+    AXS_HIP_TRY(hipMemcpyAsync(b->dbox, b->hbox.data(), nboxes * sizeof(BoxD),
+                               hipMemcpyHostToDevice, s));
+    AXS_HIP_TRY(hipMemcpyAsync(b->downer, owner.data(),
+                               owner.size() * sizeof(int), hipMemcpyHostToDevice,
+                               s));
+    AXS_HIP_TRY(hipMemcpyAsync(b->ditems, items.data(),
+                               items.size() * sizeof(WorkItem),
+                               hipMemcpyHostToDevice, s));
+    AXS_HIP_TRY(hipMemsetAsync(b->y, 0, bytes4, s));
+    if (b->comp)
+        AXS_HIP_TRY(hipMemsetAsync(b->comp, 0, bytes4, s));
+    AXS_HIP_TRY(hipStreamSynchronize(s));
+    *out = b;
+    return AXS_OK;
 }
-int axs_boxlevel_destroy(axs_boxlevel *) { return AXS_OK; }
-int64_t axs_boxlevel_num_cells(axs_boxlevel *) { return 0; }
-int axs_boxlevel_upload_dense(axs_boxlevel *, const double *)
+
+int axs_boxlevel_destroy(axs_boxlevel *b)
 {
-    return axs::set_error(AXS_ERR_UNSUPPORTED, "box levels not built");
+    if (!b)
+        return AXS_OK;
+    hipStreamSynchronize(b->ctx->stream);
+    hipFree(b->y);
+    hipFree(b->acc);
+    hipFree(b->ya);
+    hipFree(b->dbox);
+    hipFree(b->downer);
+    hipFree(b->ditems);
+    hipFree(b->dcoeff);
+    hipFree(b->derr);
+    hipFree(b->comp);
+    hipFree(b->snap);
+    delete b;
+    return AXS_OK;
 }
-int axs_boxlevel_download_dense(axs_boxlevel *, double *)
+
+int64_t axs_boxlevel_num_cells(axs_boxlevel *b) { return b ? b->nvalid : 0; }
+
+int axs_boxlevel_upload_dense(axs_boxlevel *b, const double *dense)
 {
-    return axs::set_error(AXS_ERR_UNSUPPORTED, "box levels not built");
+    AXS_CHECK_ARG(b && dense, "axs_boxlevel_upload_dense: NULL argument");
+    const size_t n4 = (size_t)4 * b->total;
+    const size_t nd = (size_t)b->dom[0] * b->dom[1] * b->dom[2];
+    std::vector<double> st(n4, 0.0);
+    for (const BoxD &B : b->hbox)
+    {
+        const int gx = B.n[0] + 2 * b->ng, gy = B.n[1] + 2 * b->ng;
+        for (int c = 0; c < 4; ++c)
+            for (int z = 0; z < B.n[2]; ++z)
+                for (int y = 0; y < B.n[1]; ++y)
+                    for (int x = 0; x < B.n[0]; ++x)
+                        st[(size_t)c * b->total + B.off +
+                           ((size_t)(z + b->ng) * gy + (y + b->ng)) * gx +
+                           (x + b->ng)] =
+                            dense[c * nd +
+                                  ((size_t)(B.lo[2] + z) * b->dom[1] +
+                                   (B.lo[1] + y)) *
+                                      b->dom[0] +
+                                  (B.lo[0] + x)];
+    }
+    hipStream_t s = b->ctx->stream;
+    if (b->precision == AXS_FP64)
+        AXS_HIP_TRY(hipMemcpyAsync(b->y, st.data(), n4 * 8,
+                                   hipMemcpyHostToDevice, s));
+    else
+    {
+        AXS_HIP_TRY(hipMemcpyAsync(b->ya, st.data(), n4 * 8,
+                                   hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_split, dim3(gblk(n4)), dim3(256), 0, s,
+                           (const double *)b->ya, (float *)b->y, b->comp, n4);
+        AXS_HIP_TRY(hipGetLastError());
+    }
+    AXS_HIP_TRY(hipStreamSynchronize(s));
+    return AXS_OK;
 }
-int axs_boxlevel_fill_ghosts(axs_boxlevel *, const axs_level *, double)
+
+int axs_boxlevel_download_dense(axs_boxlevel *b, double *dense)
 {
-    return axs::set_error(AXS_ERR_UNSUPPORTED, "box levels not built");
+    AXS_CHECK_ARG(b && dense, "axs_boxlevel_download_dense: NULL argument");
+    const size_t n4 = (size_t)4 * b->total;
+    const size_t nd = (size_t)b->dom[0] * b->dom[1] * b->dom[2];
+    std::vector<double> st(n4);
+    hipStream_t s = b->ctx->stream;
+    if (b->precision == AXS_FP64)
+        AXS_HIP_TRY(hipMemcpyAsync(st.data(), b->y, n4 * 8,
+                                   hipMemcpyDeviceToHost, s));
+    else
+    {
+        hipLaunchKernelGGL(k_join, dim3(gblk(n4)), dim3(256), 0, s,
+                           (const float *)b->y, (const float *)b->comp,
+                           (double *)b->ya, n4);
+        AXS_HIP_TRY(hipGetLastError());
+        AXS_HIP_TRY(hipMemcpyAsync(st.data(), b->ya, n4 * 8,
+                                   hipMemcpyDeviceToHost, s));
+    }
+    AXS_HIP_TRY(hipStreamSynchronize(s));
+    for (const BoxD &B : b->hbox)
+    {
+        const int gx = B.n[0] + 2 * b->ng, gy = B.n[1] + 2 * b->ng;
+        for (int c = 0; c < 4; ++c)
+            for (int z = 0; z < B.n[2]; ++z)
+                for (int y = 0; y < B.n[1]; ++y)
+                    for (int x = 0; x < B.n[0]; ++x)
+                        dense[c * nd +
+                              ((size_t)(B.lo[2] + z) * b->dom[1] + (B.lo[1] + y)) *
+                                  b->dom[0] +
+                              (B.lo[0] + x)] =
+                            st[(size_t)c * b->total + B.off +
+                               ((size_t)(z + b->ng) * gy + (y + b->ng)) * gx +
+                               (x + b->ng)];
+    }
+    return AXS_OK;
 }
-int axs_boxlevel_rk4_step(axs_boxlevel *, const axs_rhs_coeffs *, double)
+
+static int copy_level(const axs_boxlevel *b, axs_level *L, int to_level)
 {
-    return axs::set_error(AXS_ERR_UNSUPPORTED, "box levels not built");
+    AXS_CHECK_ARG(b && L, "axs_boxlevel_copy: NULL argument");
+    AXS_CHECK_ARG(L->g.nx == b->dom[0] && L->g.ny == b->dom[1] &&
+                      L->g.nz == b->dom[2],
+                  "axs_boxlevel_copy: domains differ");
+    AXS_CHECK_ARG(L->precision == b->precision,
+                  "axs_boxlevel_copy: precisions differ");
+    AXS_CHECK_ARG(L->ctx == b->ctx, "axs_boxlevel_copy: contexts differ");
+    hipStream_t s = b->ctx->stream;
+    const dim3 g = box_grid(b, max_valid(b));
+    if (b->precision == AXS_FP64)
+        hipLaunchKernelGGL(k_copy_level<double>, g, dim3(256), 0, s, view_of(b),
+                           (double *)b->y, (float *)nullptr, (double *)L->y,
+                           (float *)nullptr, L->g.ncell, to_level);
+    else
+        hipLaunchKernelGGL(k_copy_level<float>, g, dim3(256), 0, s, view_of(b),
+                           (float *)b->y, b->comp, (float *)L->y, L->comp,
+                           L->g.ncell, to_level);
+    AXS_HIP_TRY(hipGetLastError());
+    return AXS_OK;
 }
-int axs_boxlevel_tag(axs_boxlevel *, const axs_tagger_params *, uint8_t *,
-                     int64_t *)
+
+int axs_boxlevel_copy_from_level(axs_boxlevel *b, const axs_level *src)
 {
-    return axs::set_error(AXS_ERR_UNSUPPORTED, "box levels not built");
+    return copy_level(b, const_cast<axs_level *>(src), 0);
 }
+
+int axs_boxlevel_copy_to_level(const axs_boxlevel *b, axs_level *dst)
+{
+    return copy_level(b, dst, 1);
 }
+
+int axs_boxlevel_snapshot_coarse(axs_boxlevel *b, const axs_boxlevel *c)
+{
+    AXS_CHECK_ARG(b && c, "axs_boxlevel_snapshot_coarse: NULL argument");
+    AXS_CHECK_ARG(c->precision == b->precision,
+                  "axs_boxlevel_snapshot_coarse: precisions differ");
+    const size_t sb = (size_t)4 * c->total * c->word;
+    const size_t lb = c->comp ? (size_t)4 * c->total * 4 : 0;
+    if (b->snap_bytes != sb + lb)
+    {
+        hipFree(b->snap);
+        b->snap = nullptr;
+        AXS_HIP_TRY(hipMalloc(&b->snap, sb + lb));
+        b->snap_bytes = sb + lb;
+    }
+    b->snap_lo = lb ? (float *)((char *)b->snap + sb) : nullptr;
+    b->snap_of = c;
+    AXS_HIP_TRY(hipMemcpyAsync(b->snap, c->y, sb, hipMemcpyDeviceToDevice,
+                               b->ctx->stream));
+    if (lb)
+        AXS_HIP_TRY(hipMemcpyAsync(b->snap_lo, c->comp, lb,
+                                   hipMemcpyDeviceToDevice, b->ctx->stream));
+    return AXS_OK;
+}
+
+int axs_boxlevel_fill_ghosts(axs_boxlevel *b, const axs_boxlevel *coarse,
+                             double t_frac, int interior)
+{
+    AXS_CHECK_ARG(b, "axs_boxlevel_fill_ghosts: NULL level");
+    AXS_CHECK_ARG(!interior || coarse,
+                  "axs_boxlevel_fill_ghosts: interior fill needs a coarse level");
+    if (coarse)
+    {
+        AXS_CHECK_ARG(coarse->precision == b->precision,
+                      "axs_boxlevel_fill_ghosts: precisions differ");
+        for (int d = 0; d < 3; ++d)
+            AXS_CHECK_ARG(coarse->dom[d] * 2 == b->dom[d],
+                          "axs_boxlevel_fill_ghosts: coarse domain must be half");
+    }
+    hipLaunchKernelGGL(k_zero_err, dim3(1), dim3(64), 0, b->ctx->stream, b->derr);
+    int rc = fill(b, b->y, b->comp, coarse, t_frac, interior);
+    if (rc)
+        return rc;
+    return check_err(b, "axs_boxlevel_fill_ghosts");
+}
+
+int axs_boxlevel_rk4_step(axs_boxlevel *b, const axs_rhs_coeffs coeffs[4],
+                          double dt, const axs_boxlevel *coarse, double t0,
+                          double df)
+{
+    AXS_CHECK_ARG(b && coeffs, "axs_boxlevel_rk4_step: NULL argument");
+    if (coarse)
+        for (int d = 0; d < 3; ++d)
+            AXS_CHECK_ARG(coarse->dom[d] * 2 == b->dom[d] &&
+                              coarse->precision == b->precision,
+                          "axs_boxlevel_rk4_step: coarse level mismatch");
+    double h[12];
+    for (int s = 0; s < 4; ++s)
+    {
+        h[3 * s] = coeffs[s].curvature_coeff;
+        h[3 * s + 1] = coeffs[s].lambda;
+        h[3 * s + 2] = coeffs[s].R_squared;
+    }
+    hipStream_t s = b->ctx->stream;
+    AXS_HIP_TRY(hipMemcpyAsync(b->dcoeff, h, sizeof(h), hipMemcpyHostToDevice, s));
+    AXS_HIP_TRY(hipStreamSynchronize(s));
+    hipLaunchKernelGGL(k_zero_err, dim3(1), dim3(64), 0, s, b->derr);
+    const double tf[4] = {t0, t0 + 0.5 * df, t0 + 0.5 * df, t0 + df};
+    void *in[4] = {b->y, b->ya, b->yb, b->ya};
+    void *out[4] = {b->ya, b->yb, b->ya, b->y};
+    for (int st = 0; st < 4; ++st)
+    {
+        int rc = fill(b, in[st], st == 0 ? b->comp : nullptr, coarse, tf[st], 0);
+        if (!rc)
+            rc = stage(b, st + 1, in[st], out[st], dt);
+        if (rc)
+            return rc;
+    }
+    return check_err(b, "axs_boxlevel_rk4_step");
+}
+
+int axs_boxlevel_tag(axs_boxlevel *b, const axs_tagger_params *p,
+                     uint8_t *dense_tags, const uint8_t *cover,
+                     int64_t counts[2])
+{
+    AXS_CHECK_ARG(b && counts, "axs_boxlevel_tag: NULL argument");
+    AXS_CHECK_ARG(!p || (p->gradient_threshold == DBL_MAX &&
+                         p->radial_gradient_threshold == DBL_MAX),
+                  "axs_boxlevel_tag: the Laplacian / radial-gradient "
+                  "criteria are served on uniform levels only");
+    hipStream_t s = b->ctx->stream;
+    hipLaunchKernelGGL(k_zero_err, dim3(1), dim3(64), 0, s, b->ctx->d_counts);
+    const dim3 g = box_grid(b, max_valid(b));
+    const double amp = p ? p->amplitude_threshold : 0.0;
+    const double R = p ? p->R : 1.0;
+    if (b->precision == AXS_FP64)
+        hipLaunchKernelGGL(k_box_tag<double>, g, dim3(256), 0, s, view_of(b),
+                           (const double *)b->y, (const float *)nullptr,
+                           dense_tags, cover, amp, R, b->ctx->d_counts);
+    else
+        hipLaunchKernelGGL(k_box_tag<float>, g, dim3(256), 0, s, view_of(b),
+                           (const float *)b->y, (const float *)b->comp,
+                           dense_tags, cover, amp, R, b->ctx->d_counts);
+    AXS_HIP_TRY(hipGetLastError());
+    AXS_HIP_TRY(hipMemcpyAsync(b->ctx->h_counts, b->ctx->d_counts,
+                               2 * sizeof(unsigned long long),
+                               hipMemcpyDeviceToHost, s));
+    AXS_HIP_TRY(hipStreamSynchronize(s));
+    counts[0] = (int64_t)b->ctx->h_counts[0];
+    counts[1] = (int64_t)b->ctx->h_counts[1];
+    return AXS_OK;
+}
+
+int axs_boxlevel_energy_sums(axs_boxlevel *b, double R, double lambda,
+                             double RpR, const axs_masking *m,
+                             const uint8_t *cover, double out[14])
+{
+    AXS_CHECK_ARG(b && m && out, "axs_boxlevel_energy_sums: NULL argument");
+    AXS_CHECK_ARG(R > 0.0, "axs_boxlevel_energy_sums: R must be > 0");
+    hipStream_t s = b->ctx->stream;
+    const dim3 g = box_grid(b, max_valid(b));
+    const size_t rows = (size_t)g.x * g.y;
+    AXS_CHECK_ARG(rows * 14 + 14 <= (size_t)kMaxPartials * 16,
+                  "axs_boxlevel_energy_sums: too many boxes");
+    double *part = b->ctx->d_partials;
+    if (b->precision == AXS_FP64)
+        hipLaunchKernelGGL(k_box_energy<double>, g, dim3(256), 0, s, view_of(b),
+                           (const double *)b->y, (const float *)nullptr, cover,
+                           b->dx, R, lambda, RpR, m->scheme, m->threshold, part);
+    else
+        hipLaunchKernelGGL(k_box_energy<float>, g, dim3(256), 0, s, view_of(b),
+                           (const float *)b->y, (const float *)b->comp, cover,
+                           b->dx, R, lambda, RpR, m->scheme, m->threshold, part);
+    hipLaunchKernelGGL(k_box_sum_cols, dim3(14), dim3(256), 0, s,
+                       (const double *)part, (int)rows, 14, b->ctx->d_result);
+    AXS_HIP_TRY(hipGetLastError());
+    AXS_HIP_TRY(hipMemcpyAsync(b->ctx->h_result, b->ctx->d_result,
+                               14 * sizeof(double), hipMemcpyDeviceToHost, s));
+    AXS_HIP_TRY(hipStreamSynchronize(s));
+    for (int q = 0; q < 14; ++q)
+        out[q] = b->ctx->h_result[q];
+    return AXS_OK;
+}
+
+int axs_boxlevel_coverage_mask(const axs_boxlevel *f, const int cd[3],
+                               uint8_t *mask)
+{
+    AXS_CHECK_ARG(f && cd && mask, "axs_boxlevel_coverage_mask: NULL argument");
+    for (int d = 0; d < 3; ++d)
+        AXS_CHECK_ARG(cd[d] * 2 == f->dom[d],
+                      "axs_boxlevel_coverage_mask: coarse domain must be half");
+    hipStream_t s = f->ctx->stream;
+    const size_t n = (size_t)cd[0] * cd[1] * cd[2];
+    hipLaunchKernelGGL(k_set_u8, dim3(gblk(n)), dim3(256), 0, s, mask, n,
+                       (uint8_t)1);
+    hipLaunchKernelGGL(k_coverage, box_grid(f, max_valid(f) / 8), dim3(256), 0, s,
+                       view_of(f), cd[0], cd[1], mask);
+    AXS_HIP_TRY(hipGetLastError());
+    return AXS_OK;
+}
+
+int axs_boxlevel_average_down(const axs_boxlevel *f, axs_boxlevel *c)
+{
+    AXS_CHECK_ARG(f && c, "axs_boxlevel_average_down: NULL argument");
+    for (int d = 0; d < 3; ++d)
+        AXS_CHECK_ARG(c->dom[d] * 2 == f->dom[d],
+                      "axs_boxlevel_average_down: coarse domain must be half");
+    AXS_CHECK_ARG(c->precision == f->precision,
+                  "axs_boxlevel_average_down: precisions differ");
+    hipStream_t s = f->ctx->stream;
+    hipLaunchKernelGGL(k_zero_err, dim3(1), dim3(64), 0, s, c->derr);
+    const dim3 g = box_grid(f, max_valid(f) / 8);
+    if (f->precision == AXS_FP64)
+        hipLaunchKernelGGL(k_average_down<double>, g, dim3(256), 0, s, view_of(f),
+                           (const double *)f->y, (const float *)nullptr,
+                           view_of(c), (double *)c->y, (float *)nullptr, c->derr);
+    else
+        hipLaunchKernelGGL(k_average_down<float>, g, dim3(256), 0, s, view_of(f),
+                           (const float *)f->y, (const float *)f->comp,
+                           view_of(c), (float *)c->y, c->comp, c->derr);
+    AXS_HIP_TRY(hipGetLastError());
+    return check_err(c, "axs_boxlevel_average_down");
+}
+
+} // extern "C"

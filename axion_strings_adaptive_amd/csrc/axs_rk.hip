@@ -5,32 +5,22 @@
 //   * psi1/psi2 of the stage input are read once per cell per stage: the z
 //     neighbours (radius 2) live in a 5-deep register queue per thread, the
 //     x/y neighbours come from a double-buffered LDS plane with a 2-cell halo
-//     (one barrier per plane);
+//     (one barrier per plane); periodic ghosts are index wrapping;
 //   * the pointwise RHS (AxionStringsRHS.impl.hpp:83-118) and the RK4 stage
-//     axpy are applied in registers and the stage outputs written once.
+//     axpy are applied in registers (axs_stage_math.hpp) and the stage
+//     outputs written once.
 // Byte traffic per cell and stage (4 components of S bytes each):
 //   stage 1: read y_n, write acc, write y_1                     = 3 S
 //   stage 2,3: read y_s, y_n, acc, write acc, y_{s+1}             = 5 S
 //   stage 4: read y_3, y_n, acc, write y_{n+1} (+ read/write lo)  = 4 S (6 S)
 // = 17 S per RK4 step in FP64 (544 B/cell), 19 S in FP32C (304 B/cell).
-//
-// FP64 arithmetic follows the CPU oracle's operation order with contraction
-// off, so FP64 results are bit-identical to it.  FP32 arithmetic writes the
-// 4th-order Laplacian on differences from the centre value (exact by
-// Sterbenz' lemma for a smooth field), which removes the cancellation of the
-// -30 f0 term when |psi| ~ R >> 1.
+// Tiles are mapped XCD-aware: consecutive tiles (sharing halo rows) are
+// dispatched to the same XCD and its L2.
 #include "axs_internal.hpp"
+#include "axs_stage_math.hpp"
 
 namespace axs
 {
-
-enum
-{
-    MODE_RHS = 0,
-    MODE_S1 = 1,
-    MODE_S23 = 2,
-    MODE_S4 = 3
-};
 
 template <typename T> struct StageParams
 {
@@ -56,48 +46,15 @@ template <typename T> struct StageParams
 __device__ __forceinline__ int wrap_lo(int i, int n) { return i < 0 ? i + n : i; }
 __device__ __forceinline__ int wrap_hi(int i, int n) { return i >= n ? i - n : i; }
 
-// ---------------------------------------------------------------- FP64 --
-// Exact restatement of the oracle's arithmetic (contraction off).
-__device__ __forceinline__ double diff2_ref(double fm2, double fm1, double f0,
-                                            double fp1, double fp2, double i2)
+template <typename T> struct StarOf;
+template <> struct StarOf<double>
 {
-#pragma clang fp contract(off)
-    const double w_far = 8.33333333333333333333e-2;
-    const double w_near = 1.33333333333333333333e+0;
-    const double w_local = 2.50000000000000000000e+0;
-    return (-w_far * fm2 + w_near * fm1 - w_local * f0 + w_near * fp1 -
-            w_far * fp2) *
-           i2;
-}
-
-__device__ __forceinline__ void rhs_pi_f64(double lap1, double lap2,
-                                           double psi1, double psi2,
-                                           double cc, double lam, double R2,
-                                           double &F1, double &F2)
+    using type = Star;
+};
+template <> struct StarOf<float>
 {
-#pragma clang fp contract(off)
-    const double d = psi1 * psi1 + psi2 * psi2 - R2;
-    const double pc = 0.5 * lam * d;
-    F1 = lap1 + cc * psi1 - pc * psi1;
-    F2 = lap2 + cc * psi2 - pc * psi2;
-}
-
-__device__ __forceinline__ double axpy_f64(double y, double a, double k)
-{
-#pragma clang fp contract(off)
-    return y + a * k;
-}
-__device__ __forceinline__ double acc2_f64(double acc, double k)
-{
-#pragma clang fp contract(off)
-    return acc + 2.0 * k;
-}
-__device__ __forceinline__ double final_f64(double y, double h6, double acc,
-                                            double k)
-{
-#pragma clang fp contract(off)
-    return y + h6 * (acc + k);
-}
+    using type = StarF;
+};
 
 template <typename T, int PREC, int MODE, int TY>
 __global__ void __launch_bounds__(64 * TY)
@@ -187,7 +144,8 @@ __global__ void __launch_bounds__(64 * TY)
     // stage scalars
     const double *cf =
         p.coeff + 12 * (p.step ? *p.step : 0) + 3 * p.coeff_stage;
-    const double cc_d = cf[0], lam_d = cf[1], R2_d = cf[2];
+    const double cc = cf[0], lam = cf[1], R2 = cf[2];
+    const T i2 = (T)p.inv_dx2;
 
     for (int z = z_begin; z < z_end; ++z)
     {
@@ -218,8 +176,7 @@ __global__ void __launch_bounds__(64 * TY)
 
         // ---- prefetch next plane (queue head + halos) ----
         T nq1 = q1[4], nq2 = q2[4];
-        const bool more = z + 1 < z_end;
-        if (more)
+        if (z + 1 < z_end)
         {
             const uint32_t o3 = zoff(z + 3) + rowc;
             nq1 = psi1[o3];
@@ -239,31 +196,27 @@ __global__ void __launch_bounds__(64 * TY)
 
         // ---- centre loads of this plane ----
         const uint32_t oc = (uint32_t)z * plane + rowc;
-        T pi1 = 0, pi2 = 0;
-        T yn0 = 0, yn1 = 0, yn2 = 0, yn3 = 0;
-        T ac0 = 0, ac1 = 0, ac2 = 0, ac3 = 0;
-        float lo0 = 0, lo1 = 0, lo2 = 0, lo3 = 0;
+        T pi[2] = {0, 0};
+        T yn[4] = {0, 0, 0, 0}, ac[4] = {0, 0, 0, 0};
+        float lo[4] = {0, 0, 0, 0};
         if (active)
         {
-            pi1 = p.yin[2 * n + oc];
-            pi2 = p.yin[3 * n + oc];
+            pi[0] = p.yin[2 * n + oc];
+            pi[1] = p.yin[3 * n + oc];
             if (MODE == MODE_S23 || MODE == MODE_S4)
             {
-                yn0 = p.yn[oc];
-                yn1 = p.yn[n + oc];
-                yn2 = p.yn[2 * n + oc];
-                yn3 = p.yn[3 * n + oc];
-                ac0 = p.acc[oc];
-                ac1 = p.acc[n + oc];
-                ac2 = p.acc[2 * n + oc];
-                ac3 = p.acc[3 * n + oc];
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                {
+                    yn[c] = p.yn[c * n + oc];
+                    ac[c] = p.acc[c * n + oc];
+                }
             }
             if (MODE == MODE_S4 && PREC == AXS_FP32C)
             {
-                lo0 = p.comp[oc];
-                lo1 = p.comp[n + oc];
-                lo2 = p.comp[2 * n + oc];
-                lo3 = p.comp[3 * n + oc];
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    lo[c] = p.comp[c * n + oc];
             }
         }
 
@@ -272,168 +225,40 @@ __global__ void __launch_bounds__(64 * TY)
         if (active)
         {
             const int cy = ty + 2, cx = tx + 2;
+            using S = typename StarOf<T>::type;
+            const S s1{sm[b][0][cy][cx - 2], sm[b][0][cy][cx - 1],
+                       sm[b][0][cy][cx + 1], sm[b][0][cy][cx + 2],
+                       sm[b][0][cy - 2][cx], sm[b][0][cy - 1][cx],
+                       sm[b][0][cy + 1][cx], sm[b][0][cy + 2][cx],
+                       q1[0], q1[1], q1[3], q1[4], q1[2]};
+            const S s2{sm[b][1][cy][cx - 2], sm[b][1][cy][cx - 1],
+                       sm[b][1][cy][cx + 1], sm[b][1][cy][cx + 2],
+                       sm[b][1][cy - 2][cx], sm[b][1][cy - 1][cx],
+                       sm[b][1][cy + 1][cx], sm[b][1][cy + 2][cx],
+                       q2[0], q2[1], q2[3], q2[4], q2[2]};
             T F1, F2;
-            if constexpr (PREC == AXS_FP64)
+            force(star_lap(s1, i2), star_lap(s2, i2), q1[2], q2[2], cc, lam,
+                  R2, F1, F2);
+            const T k[4] = {pi[0], pi[1], F1, F2};
+            if (MODE == MODE_S1)
             {
-                const double i2 = p.inv_dx2;
-                double l1 = 0.0, l2 = 0.0;
-                l1 += diff2_ref(sm[b][0][cy][cx - 2], sm[b][0][cy][cx - 1],
-                                q1[2], sm[b][0][cy][cx + 1],
-                                sm[b][0][cy][cx + 2], i2);
-                l1 += diff2_ref(sm[b][0][cy - 2][cx], sm[b][0][cy - 1][cx],
-                                q1[2], sm[b][0][cy + 1][cx],
-                                sm[b][0][cy + 2][cx], i2);
-                l1 += diff2_ref(q1[0], q1[1], q1[2], q1[3], q1[4], i2);
-                l2 += diff2_ref(sm[b][1][cy][cx - 2], sm[b][1][cy][cx - 1],
-                                q2[2], sm[b][1][cy][cx + 1],
-                                sm[b][1][cy][cx + 2], i2);
-                l2 += diff2_ref(sm[b][1][cy - 2][cx], sm[b][1][cy - 1][cx],
-                                q2[2], sm[b][1][cy + 1][cx],
-                                sm[b][1][cy + 2][cx], i2);
-                l2 += diff2_ref(q2[0], q2[1], q2[2], q2[3], q2[4], i2);
-                double f1, f2;
-                rhs_pi_f64(l1, l2, q1[2], q2[2], cc_d, lam_d, R2_d, f1, f2);
-                F1 = f1;
-                F2 = f2;
+                // y_n is the stage input itself
+                yn[0] = q1[2];
+                yn[1] = q2[2];
+                yn[2] = pi[0];
+                yn[3] = pi[1];
             }
-            else
-            {
-                const float w_near = 4.0f / 3.0f, w_far = 1.0f / 12.0f;
-                const float i2 = (float)p.inv_dx2;
-                const float c1 = q1[2], c2 = q2[2];
-                const float n1 = ((sm[b][0][cy][cx - 1] - c1) +
-                                  (sm[b][0][cy][cx + 1] - c1)) +
-                                 ((sm[b][0][cy - 1][cx] - c1) +
-                                  (sm[b][0][cy + 1][cx] - c1)) +
-                                 ((q1[1] - c1) + (q1[3] - c1));
-                const float f1 = ((sm[b][0][cy][cx - 2] - c1) +
-                                  (sm[b][0][cy][cx + 2] - c1)) +
-                                 ((sm[b][0][cy - 2][cx] - c1) +
-                                  (sm[b][0][cy + 2][cx] - c1)) +
-                                 ((q1[0] - c1) + (q1[4] - c1));
-                const float n2 = ((sm[b][1][cy][cx - 1] - c2) +
-                                  (sm[b][1][cy][cx + 1] - c2)) +
-                                 ((sm[b][1][cy - 1][cx] - c2) +
-                                  (sm[b][1][cy + 1][cx] - c2)) +
-                                 ((q2[1] - c2) + (q2[3] - c2));
-                const float f2 = ((sm[b][1][cy][cx - 2] - c2) +
-                                  (sm[b][1][cy][cx + 2] - c2)) +
-                                 ((sm[b][1][cy - 2][cx] - c2) +
-                                  (sm[b][1][cy + 2][cx] - c2)) +
-                                 ((q2[0] - c2) + (q2[4] - c2));
-                const float lap1 = (w_near * n1 - w_far * f1) * i2;
-                const float lap2 = (w_near * n2 - w_far * f2) * i2;
-                const float d = c1 * c1 + c2 * c2 - (float)R2_d;
-                const float pc = 0.5f * (float)lam_d * d;
-                const float ccf = (float)cc_d;
-                F1 = lap1 + ccf * c1 - pc * c1;
-                F2 = lap2 + ccf * c2 - pc * c2;
-            }
-
-            // k = (Pi1, Pi2, F1, F2)
-            if constexpr (MODE == MODE_RHS)
-            {
-                p.yout[oc] = pi1;
-                p.yout[n + oc] = pi2;
-                p.yout[2 * n + oc] = F1;
-                p.yout[3 * n + oc] = F2;
-            }
-            else if constexpr (MODE == MODE_S1)
-            {
-                // y_n == y_in at stage 1
-                p.acc[oc] = pi1;
-                p.acc[n + oc] = pi2;
-                p.acc[2 * n + oc] = F1;
-                p.acc[3 * n + oc] = F2;
-                if constexpr (PREC == AXS_FP64)
-                {
-                    const double a = p.a;
-                    p.yout[oc] = axpy_f64(q1[2], a, pi1);
-                    p.yout[n + oc] = axpy_f64(q2[2], a, pi2);
-                    p.yout[2 * n + oc] = axpy_f64(pi1, a, F1);
-                    p.yout[3 * n + oc] = axpy_f64(pi2, a, F2);
-                }
-                else
-                {
-                    const float a = (float)p.a;
-                    p.yout[oc] = q1[2] + a * pi1;
-                    p.yout[n + oc] = q2[2] + a * pi2;
-                    p.yout[2 * n + oc] = pi1 + a * F1;
-                    p.yout[3 * n + oc] = pi2 + a * F2;
-                }
-            }
-            else if constexpr (MODE == MODE_S23)
-            {
-                if constexpr (PREC == AXS_FP64)
-                {
-                    const double a = p.a;
-                    p.acc[oc] = acc2_f64(ac0, pi1);
-                    p.acc[n + oc] = acc2_f64(ac1, pi2);
-                    p.acc[2 * n + oc] = acc2_f64(ac2, F1);
-                    p.acc[3 * n + oc] = acc2_f64(ac3, F2);
-                    p.yout[oc] = axpy_f64(yn0, a, pi1);
-                    p.yout[n + oc] = axpy_f64(yn1, a, pi2);
-                    p.yout[2 * n + oc] = axpy_f64(yn2, a, F1);
-                    p.yout[3 * n + oc] = axpy_f64(yn3, a, F2);
-                }
-                else
-                {
-                    const float a = (float)p.a;
-                    p.acc[oc] = ac0 + 2.0f * pi1;
-                    p.acc[n + oc] = ac1 + 2.0f * pi2;
-                    p.acc[2 * n + oc] = ac2 + 2.0f * F1;
-                    p.acc[3 * n + oc] = ac3 + 2.0f * F2;
-                    p.yout[oc] = yn0 + a * pi1;
-                    p.yout[n + oc] = yn1 + a * pi2;
-                    p.yout[2 * n + oc] = yn2 + a * F1;
-                    p.yout[3 * n + oc] = yn3 + a * F2;
-                }
-            }
-            else // MODE_S4
-            {
-                if constexpr (PREC == AXS_FP64)
-                {
-                    const double h6 = p.h6;
-                    p.yout[oc] = final_f64(yn0, h6, ac0, pi1);
-                    p.yout[n + oc] = final_f64(yn1, h6, ac1, pi2);
-                    p.yout[2 * n + oc] = final_f64(yn2, h6, ac2, F1);
-                    p.yout[3 * n + oc] = final_f64(yn3, h6, ac3, F2);
-                }
-                else if constexpr (PREC == AXS_FP32)
-                {
-                    const float h6 = (float)p.h6;
-                    p.yout[oc] = yn0 + h6 * (ac0 + pi1);
-                    p.yout[n + oc] = yn1 + h6 * (ac1 + pi2);
-                    p.yout[2 * n + oc] = yn2 + h6 * (ac2 + F1);
-                    p.yout[3 * n + oc] = yn3 + h6 * (ac3 + F2);
-                }
-                else
-                {
-                    // Kahan: true y_n = hi + lo; add delta = (dt/6)(acc + k)
-                    // with dt/6 carried as hi + lo floats.
-                    const float h6 = (float)p.h6, h6l = p.h6_lo;
-                    float hv[4] = {yn0, yn1, yn2, yn3};
-                    float lv[4] = {lo0, lo1, lo2, lo3};
-                    const float sv[4] = {ac0 + pi1, ac1 + pi2, ac2 + F1,
-                                         ac3 + F2};
+            T acc_out[4], y_out[4];
+            stage_update<T, PREC, MODE>(k, yn, ac, lo, p.a, p.h6, p.h6_lo,
+                                        acc_out, y_out);
 #pragma unroll
-                    for (int c = 0; c < 4; ++c)
-                    {
-                        const float delta = h6 * sv[c] + h6l * sv[c];
-                        const float yk = delta + lv[c];
-                        const float t = hv[c] + yk;
-                        lv[c] = yk - (t - hv[c]);
-                        hv[c] = t;
-                    }
-                    p.yout[oc] = hv[0];
-                    p.yout[n + oc] = hv[1];
-                    p.yout[2 * n + oc] = hv[2];
-                    p.yout[3 * n + oc] = hv[3];
-                    p.comp[oc] = lv[0];
-                    p.comp[n + oc] = lv[1];
-                    p.comp[2 * n + oc] = lv[2];
-                    p.comp[3 * n + oc] = lv[3];
-                }
+            for (int c = 0; c < 4; ++c)
+            {
+                p.yout[c * n + oc] = y_out[c];
+                if (MODE == MODE_S1 || MODE == MODE_S23)
+                    p.acc[c * n + oc] = acc_out[c];
+                if (MODE == MODE_S4 && PREC == AXS_FP32C)
+                    p.comp[c * n + oc] = lo[c];
             }
         }
 

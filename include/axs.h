@@ -237,36 +237,75 @@ int axs_level_compare(axs_level *a, axs_level *b, double *max_dpsi,
                       double *max_psi_b);
 
 /* -------------------------- AMR box levels -------------------------- */
-/* A level made of nboxes boxes, box b covering cells lo[3b..3b+2] to
- * hi[3b..3b+2] (inclusive) of a periodic index space of size
- * domain[0..2], each box stored with nghost (>= 3) ghost layers,
- * component-major per box.  Boxes must not overlap. */
+/* A refinement level made of nboxes boxes, box b covering cells
+ * lo[3b..3b+2] to hi[3b..3b+2] (inclusive) of a periodic index space of
+ * size domain[0..2] (the level's own index space; ratio 2 to the next
+ * coarser level), each box stored with nghost (3) ghost layers.  Box
+ * corners and the domain must be multiples of 8 cells (the blocking
+ * factor), boxes must not overlap, and box edges must be 8..64 cells
+ * (the reference's blocking_factor / max_grid_size range).  This is the
+ * MultiFab of one AMR level of the reference (AxionStringsLevel's state,
+ * boxes from amr.blocking_factor / amr.max_grid_size). */
 int axs_boxlevel_create(axs_ctx *ctx, int nboxes, const int *lo,
                         const int *hi, const int domain[3], int nghost,
                         double dx, int precision, axs_boxlevel **out);
 int axs_boxlevel_destroy(axs_boxlevel *bl);
-/* Number of cells (valid, without ghosts) over all boxes. */
+/* Number of valid cells (without ghosts) over all boxes. */
 int64_t axs_boxlevel_num_cells(axs_boxlevel *bl);
 /* Copies values from / to a dense periodic host array of the whole domain
- * (4 * domain cells, doubles); cells outside every box are ignored. */
+ * (4 * domain cells, doubles, x fastest); cells outside every box are not
+ * read / not written. */
 int axs_boxlevel_upload_dense(axs_boxlevel *bl, const double *dense);
 int axs_boxlevel_download_dense(axs_boxlevel *bl, double *dense);
-/* Fills each box's ghost cells: from overlapping boxes of the same level
- * where they exist, else from the coarse level `coarse` (ratio 2) by
- * tensor-product quartic interpolation in space of a time-interpolated
- * coarse state (coarse_old*(1-t_frac) + coarse_new*t_frac; pass
- * coarse_old = NULL for t_frac = 1).  coarse may be NULL when the boxes
- * tile the whole periodic domain. */
-int axs_boxlevel_fill_ghosts(axs_boxlevel *bl, const axs_level *coarse,
-                             double t_frac);
-/* One RK4 step of every box of the level, each stage one persistent
- * launch over the box work-list; same-level ghost fill between stages. */
+/* Copies every box's valid cells from a uniform level of the same domain
+ * and precision (level-0 initial data). */
+int axs_boxlevel_copy_from_level(axs_boxlevel *bl, const axs_level *src);
+/* Snapshot of the coarse level's current state, the t = 0 end of the time
+ * interpolation used by fill_ghosts / rk4_step (take it before the coarse
+ * level advances). */
+int axs_boxlevel_snapshot_coarse(axs_boxlevel *bl, const axs_boxlevel *coarse);
+/* FillPatch of the state: each box's ghost cells from the valid cells of
+ * the level's boxes (periodic) where they exist, else from `coarse` (ratio
+ * 2) by tensor-product quartic (5-point Lagrange) interpolation of the
+ * coarse state interpolated linearly in time between the snapshot
+ * (t_frac = 0) and the coarse level's current state (t_frac = 1).
+ * coarse may be NULL when the boxes tile the whole periodic domain.
+ * interior = 1 also interpolates the valid cells from the coarse level
+ * (initial data of a new level). */
+int axs_boxlevel_fill_ghosts(axs_boxlevel *bl, const axs_boxlevel *coarse,
+                             double t_frac, int interior);
+/* One RK4 step of every box, each stage one persistent launch over the
+ * box work-list, preceded by the ghost fill of its input at the stage time
+ * (t0_frac, t0_frac + dt_frac/2, t0_frac + dt_frac/2, t0_frac + dt_frac of
+ * the coarse interval; a ratio-2 subcycled fine step has dt_frac = 0.5). */
 int axs_boxlevel_rk4_step(axs_boxlevel *bl, const axs_rhs_coeffs coeffs[4],
-                          double dt);
-/* Tags of every box cell (StringTagger semantics) into a dense device
- * uint8 array of the whole domain; returns plaquette counts. */
+                          double dt, const axs_boxlevel *coarse,
+                          double t0_frac, double dt_frac);
+/* StringTagger over every valid box cell into a dense device uint8 array
+ * of the level's domain (sets only); plaquette counts of the same pass.
+ * Ghosts must be filled.  cover: optional dense device uint8 coverage mask
+ * (0 = covered by a finer level, excluded from the counts). */
 int axs_boxlevel_tag(axs_boxlevel *bl, const axs_tagger_params *p,
-                     uint8_t *dense_tags, int64_t counts[2]);
+                     uint8_t *dense_tags, const uint8_t *cover,
+                     int64_t counts[2]);
+/* compute_total_energy_sums over the level's valid cells, excluding cells
+ * where the dense coverage mask is 0 (EnergyKernel.hpp:106-233). */
+int axs_boxlevel_energy_sums(axs_boxlevel *bl, double R, double lambda,
+                             double R_prime_over_R, const axs_masking *m,
+                             const uint8_t *cover, double out[14]);
+/* Dense uint8 coverage mask of the coarse domain (ratio 2): 0 where a
+ * coarse cell is covered by a box of `fine`, 1 elsewhere
+ * (amrex::makeFineMask, AxionStringsLevel.cpp:33-85). */
+int axs_boxlevel_coverage_mask(const axs_boxlevel *fine,
+                               const int coarse_domain[3], uint8_t *mask);
+/* average_down of the fine level's valid cells onto the covered valid
+ * cells of the coarse box level (mean of the 8 children;
+ * collapse_to_level0, AxionStringsLevel.cpp:110-134). */
+int axs_boxlevel_average_down(const axs_boxlevel *fine, axs_boxlevel *coarse);
+/* Copies every box's valid cells into a uniform level of the same domain
+ * and precision (cells outside the boxes are left as they are); with the
+ * spectrum entry points this gives the level-0 FFT of the reference. */
+int axs_boxlevel_copy_to_level(const axs_boxlevel *bl, axs_level *dst);
 
 /* ------------------------------ adjoint ----------------------------- */
 /* Gradient of J = sum_cells (|grad psi1|^2 + |grad psi2|^2) dx^3 / R_f^4
