@@ -8,8 +8,10 @@ fallback.
 
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import weakref
 from typing import Optional
 
 import numpy as np
@@ -169,18 +171,37 @@ def rhs_coeffs(bg: Background, tau: float) -> RhsCoeffs:
     return c
 
 
+_live_contexts: "weakref.WeakSet" = weakref.WeakSet()
+
+
+@atexit.register
+def _close_all_contexts():
+    # Before interpreter teardown: levels first, then their contexts, while
+    # the HIP runtime is still loaded.
+    for c in list(_live_contexts):
+        c.close()
+
+
 class Context:
     def __init__(self, device: int = 0):
         self.lib = load()
         h = _P()
         check(self.lib.axs_ctx_create(device, C.byref(h)))
         self.h = h
+        self._children = weakref.WeakSet()
+        _live_contexts.add(self)
+
+    def adopt(self, obj):
+        self._children.add(obj)
 
     def synchronize(self):
         check(self.lib.axs_ctx_synchronize(self.h))
 
     def close(self):
+        """Destroys every level still alive on this context, then itself."""
         if self.h:
+            for child in list(self._children):
+                child.close()
             self.lib.axs_ctx_destroy(self.h)
             self.h = None
 
@@ -232,6 +253,7 @@ class Level:
         check(self.lib.axs_level_create(ctx.h, self.nx, self.ny, self.nz, dx,
                                         precision, C.byref(h)))
         self.h = h
+        ctx.adopt(self)
 
     @property
     def ncell(self) -> int:
@@ -402,6 +424,7 @@ class BoxLevel:
         check(self.lib.axs_boxlevel_create(ctx.h, len(boxes), lo, hi, dom, 3,
                                            dx, precision, C.byref(h)))
         self.h = h
+        ctx.adopt(self)
 
     @property
     def num_cells(self) -> int:

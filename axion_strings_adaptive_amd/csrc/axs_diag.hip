@@ -381,6 +381,23 @@ __device__ __forceinline__ double masking_weight_d(int scheme, double thr,
     return f * f;
 }
 
+// masked_a_dot, Masking.hpp:96-118 (contraction off: the numerator
+// psi1 Pi2 - Pi1 psi2 cancels, so it is rounded exactly as written)
+__device__ __forceinline__ double masked_a_dot_d(int scheme, double thr,
+                                                 double psi1, double psi2,
+                                                 double Pi1, double Pi2,
+                                                 double R)
+{
+#pragma clang fp contract(off)
+    const double raw = psi1 * Pi2 - Pi1 * psi2;
+    if (scheme == AXS_MASK_A)
+        return raw;
+    if (scheme == AXS_MASK_B &&
+        masking_weight_d(scheme, thr, psi1, psi2, R) == 0.0)
+        return 0.0;
+    return raw / (psi1 * psi1 + psi2 * psi2);
+}
+
 constexpr int EB = 256;
 
 template <typename T>
@@ -521,14 +538,7 @@ __global__ void k_fill(const T *y, const float *lo, uint32_t n, int kind,
             const double psi2 = ld(y + n, lo ? lo + n : nullptr, id);
             const double Pi1 = ld(y + 2 * n, lo ? lo + 2 * n : nullptr, id);
             const double Pi2 = ld(y + 3 * n, lo ? lo + 3 * n : nullptr, id);
-            const double raw = psi1 * Pi2 - Pi1 * psi2;
-            if (scheme == AXS_MASK_A)
-                v = raw;
-            else if (scheme == AXS_MASK_B &&
-                     masking_weight_d(scheme, thr, psi1, psi2, R) == 0.0)
-                v = 0.0;
-            else
-                v = raw / (psi1 * psi1 + psi2 * psi2);
+            v = masked_a_dot_d(scheme, thr, psi1, psi2, Pi1, Pi2, R);
         }
         else
         {
