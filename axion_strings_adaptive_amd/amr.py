@@ -1,0 +1,220 @@
+"""Static AMR hierarchy on box levels: the batching form of the hot path.
+
+Level 0 is a periodic base grid (tiled into boxes); each finer level
+(ratio 2) is built from the string tagger on the level below -- tag where
+the plaquette winding is nonzero (StringTagger.hpp:240-274) or, as an
+extension, where |psi|/R < amplitude_threshold -- grown by n_error_buf cells
+(the reference's amr.n_error_buf), cut into blocking-factor blocks and
+merged along x into boxes of at most max_grid_size cells
+(amr.blocking_factor / amr.max_grid_size), and kept properly nested one
+block inside the coarser level's coverage.
+
+One coarse step advances level l once and level l+1 twice (subcycling,
+docs/conventions.md:239-241): the coarse state is snapshotted, the coarse
+level takes its RK4 step, the fine level takes two half steps whose
+coarse-fine ghosts interpolate the coarse state in time, then the fine
+valid cells are averaged down onto the coarse level.  Each RK stage of each
+level is one persistent launch over that level's box work-list.
+
+    python -m axion_strings_adaptive_amd.amr --n0 256 --levels 2 --steps 4
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import capi
+
+
+def dilate_periodic(mask: np.ndarray, r: int) -> np.ndarray:
+    """Periodic dilation of a boolean [z, y, x] mask by an r-cell cube."""
+    out = mask.copy()
+    for ax in range(3):
+        acc = out.copy()
+        for s in range(1, r + 1):
+            acc |= np.roll(out, s, axis=ax) | np.roll(out, -s, axis=ax)
+        out = acc
+    return out
+
+
+def blocks_from_tags(tags: np.ndarray, block: int) -> np.ndarray:
+    """Boolean [bz, by, bx] of blocks containing any tagged cell."""
+    nz, ny, nx = tags.shape
+    t = tags.reshape(nz // block, block, ny // block, block, nx // block, block)
+    return t.any(axis=(1, 3, 5))
+
+
+def boxes_from_blocks(blk: np.ndarray, fine_block: int, max_blocks_x: int):
+    """Fine-level boxes (ratio 2): one box per run of up to max_blocks_x
+    consecutive tagged coarse blocks along x."""
+    boxes = []
+    bz, by, bx = blk.shape
+    for k in range(bz):
+        for j in range(by):
+            i = 0
+            while i < bx:
+                if not blk[k, j, i]:
+                    i += 1
+                    continue
+                i0 = i
+                while i < bx and blk[k, j, i] and i - i0 < max_blocks_x:
+                    i += 1
+                lo = (i0 * fine_block, j * fine_block, k * fine_block)
+                hi = (i * fine_block - 1, (j + 1) * fine_block - 1,
+                      (k + 1) * fine_block - 1)
+                boxes.append((lo, hi))
+    return boxes
+
+
+def erode_periodic(mask: np.ndarray, r: int) -> np.ndarray:
+    return ~dilate_periodic(~mask, r)
+
+
+class Hierarchy:
+    def __init__(self, ctx: capi.Context, n0: int = 256, levels: int = 2,
+                 dx0: float = 0.5, precision: int = capi.AXS_FP32C,
+                 amp_thr: float = 0.5, n_error_buf: int = 2,
+                 blocking_factor: int = 16, max_grid_size: int = 64,
+                 seed: int = 42, bg=None, tau0: float = 1.0,
+                 dt_multiplier: float = 0.2, verbose: bool = True,
+                 force_full_refinement: bool = False):
+        self.ctx = ctx
+        self.bg = bg if bg is not None else capi.make_background(
+            2.0, c0=0.0, c1=2.0, tau_switch=1.0, has_switch=True)
+        self.tau = tau0
+        self.precision = precision
+        self.dx = [dx0 / 2 ** l for l in range(levels + 1)]
+        self.dt0 = dt_multiplier * dx0
+        self.amp_thr = amp_thr
+        self.verbose = verbose
+        self.levels: list[capi.BoxLevel] = []
+        self.domains = []
+        # ---- level 0: seeded IC on a uniform level, copied into boxes ----
+        base = capi.Level(ctx, n0, dx0, precision)
+        base.fourier_ic(0, n0 / 8.0, 0.5, seed, 0)
+        base.fourier_ic(1, n0 / 8.0, 0.5, seed, 1)
+        base.zero_components(2, 2)
+        e0 = min(max_grid_size, n0)
+        cuts = [(p, min(p + e0, n0) - 1) for p in range(0, n0, e0)]
+        boxes0 = [((x0, y0, z0), (x1, y1, z1)) for (z0, z1) in cuts
+                  for (y0, y1) in cuts for (x0, x1) in cuts]
+        L0 = capi.BoxLevel(ctx, boxes0, (n0,) * 3, dx0, precision)
+        L0.copy_from_level(base)
+        base.close()
+        L0.fill_ghosts()
+        self.levels.append(L0)
+        self.domains.append((n0,) * 3)
+        # ---- finer levels from the tagger ----
+        R = capi.background_eval(self.bg, 0, self.tau)
+        covered = np.ones((n0 // 8,) * 3, bool)  # level-0 blocks of 8 cells
+        for l in range(1, levels + 1):
+            coarse = self.levels[-1]
+            cd = self.domains[-1]
+            tags_buf, _ = coarse.tag(amp_thr=amp_thr, R=R)
+            tags = tags_buf.to_numpy(np.uint8).reshape(cd[2], cd[1], cd[0]) > 0
+            if force_full_refinement:  # axion_strings.tagging.force_full_refinement
+                tags[:] = True
+            tags = dilate_periodic(tags, n_error_buf)
+            cblock = blocking_factor // 2          # coarse cells per fine box edge
+            blk = blocks_from_tags(tags, cblock)
+            if l > 1:  # proper nesting: one block inside the coarse coverage
+                blk &= erode_periodic(covered, 1)
+            boxes = boxes_from_blocks(blk, blocking_factor,
+                                      max_grid_size // blocking_factor)
+            if not boxes:
+                break
+            fd = tuple(2 * d for d in cd)
+            Lf = capi.BoxLevel(ctx, boxes, fd, self.dx[l], precision)
+            Lf.fill_ghosts(coarse, 1.0, interior=True)
+            self.levels.append(Lf)
+            self.domains.append(fd)
+            # coverage of the new level in blocks of 8 of its own cells
+            covered = np.kron(blk, np.ones((2, 2, 2), bool))
+            if verbose:
+                print(f"[amr] level {l}: {len(boxes)} boxes, "
+                      f"{Lf.num_cells} cells", file=sys.stderr, flush=True)
+        self.ncells = [L.num_cells for L in self.levels]
+
+    def coeffs(self, tau, dt):
+        return [capi.rhs_coeffs(self.bg, t) for t in
+                (tau, tau + 0.5 * dt, tau + 0.5 * dt, tau + dt)]
+
+    def _advance(self, l: int, tau: float, dt: float, t0: float, df: float):
+        L = self.levels[l]
+        coarse = self.levels[l - 1] if l > 0 else None
+        if l + 1 < len(self.levels):
+            self.levels[l + 1].snapshot_coarse(L)
+        L.rk4_step(self.coeffs(tau, dt), dt, coarse, t0, df)
+        if l + 1 < len(self.levels):
+            self._advance(l + 1, tau, 0.5 * dt, 0.0, 0.5)
+            self._advance(l + 1, tau + 0.5 * dt, 0.5 * dt, 0.5, 0.5)
+            self.levels[l + 1].average_down_to(L)
+
+    def step(self):
+        """One coarse step of the whole hierarchy."""
+        self._advance(0, self.tau, self.dt0, 0.0, 1.0)
+        self.tau += self.dt0
+
+    def cell_updates_per_coarse_step(self) -> int:
+        return sum(n * 2 ** l for l, n in enumerate(self.ncells))
+
+    def composite_energy(self, scheme=capi.AXS_MASK_B, thr=0.8):
+        """compute_composite_total_energy (EnergyKernel.hpp:287-339) over the
+        hierarchy, each level masked by the coverage of the next finer."""
+        R = capi.background_eval(self.bg, 0, self.tau)
+        lam = capi.background_eval(self.bg, 3, self.tau)
+        RpR = capi.background_eval(self.bg, 8, self.tau)
+        acc = np.zeros(14)
+        for l, L in enumerate(self.levels):
+            L.fill_ghosts(self.levels[l - 1] if l > 0 else None)
+            cover = None
+            if l + 1 < len(self.levels):
+                cover = self.levels[l + 1].coverage_mask(self.domains[l])
+            e = L.energy_sums(R, lam, RpR, scheme, thr, cover=cover)
+            acc += e * self.dx[l] ** 3
+        n_total, n_unmasked = acc[13], acc[12]
+        return {"rho_tot_unscreened": acc[0] / n_total,
+                "rho_tot_screened": acc[1] / n_unmasked if n_unmasked else 0.0,
+                "n_total": n_total, "n_unmasked": n_unmasked}
+
+    def close(self):
+        for L in self.levels:
+            L.close()
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--n0", type=int, default=256)
+    ap.add_argument("--levels", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--precision", type=int, default=capi.AXS_FP32C)
+    args = ap.parse_args(argv)
+    ctx = capi.Context(0)
+    h = Hierarchy(ctx, args.n0, args.levels, precision=args.precision)
+    h.step()  # warm-up
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        h.step()
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    e = h.composite_energy()
+    print(json.dumps({
+        "levels": len(h.levels), "cells": h.ncells,
+        "boxes": [len(L.boxes) for L in h.levels],
+        "cell_updates_per_coarse_step": h.cell_updates_per_coarse_step(),
+        "cell_updates_per_s": h.cell_updates_per_coarse_step() * args.steps / dt,
+        "ms_per_coarse_step": 1e3 * dt / args.steps,
+        "n_total": e["n_total"], "L_tilde^3": (args.n0 * 0.5) ** 3,
+        "rho_tot_unscreened": e["rho_tot_unscreened"]}))
+    h.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -113,7 +113,20 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile", type=str, default="",
                     help="variant,z_chunk (default: autotuned cache or built-in)")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run 3 device copies of this many MiB (counter "
+                         "calibration: read N + write N bytes per copy)")
     args = ap.parse_args(argv)
+    if args.copy_probe_mib:
+        import torch
+        nbytes = args.copy_probe_mib << 20
+        a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": nbytes}))
+        return 0
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
