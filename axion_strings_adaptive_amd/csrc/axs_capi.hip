@@ -272,8 +272,9 @@ int axs_level_create(axs_ctx *ctx, int nx, int ny, int nz, double dx,
     *out = nullptr;
     AXS_CHECK_ARG(nx >= 4 && ny >= 4 && nz >= 4,
                   "axs_level_create: every dimension must be >= 4");
-    AXS_CHECK_ARG((long long)nx * ny * nz * 4 < (1LL << 32),
-                  "axs_level_create: 4*nx*ny*nz must be < 2^32");
+    // component offsets c*n + i are 32-bit: 4 n <= 2^32 (1024^3 fits)
+    AXS_CHECK_ARG((long long)nx * ny * nz * 4 <= (1LL << 32),
+                  "axs_level_create: 4*nx*ny*nz must be <= 2^32");
     AXS_CHECK_ARG(dx > 0.0, "axs_level_create: dx must be > 0");
     AXS_CHECK_ARG(precision == AXS_FP64 || precision == AXS_FP32C ||
                       precision == AXS_FP32,
