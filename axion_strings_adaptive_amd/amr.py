@@ -82,8 +82,9 @@ class Hierarchy:
                  blocking_factor: int = 16, max_grid_size: int = 64,
                  seed: int = 42, bg=None, tau0: float = 1.0,
                  dt_multiplier: float = 0.2, verbose: bool = True,
-                 force_full_refinement: bool = False):
+                 force_full_refinement: bool = False, use_graph: bool = True):
         self.ctx = ctx
+        self.use_graph = use_graph
         self.bg = bg if bg is not None else capi.make_background(
             2.0, c0=0.0, c1=2.0, tau_switch=1.0, has_switch=True)
         self.tau = tau0
@@ -149,7 +150,8 @@ class Hierarchy:
         coarse = self.levels[l - 1] if l > 0 else None
         if l + 1 < len(self.levels):
             self.levels[l + 1].snapshot_coarse(L)
-        L.rk4_step(self.coeffs(tau, dt), dt, coarse, t0, df)
+        L.rk4_step(self.coeffs(tau, dt), dt, coarse, t0, df,
+                   use_graph=self.use_graph)
         if l + 1 < len(self.levels):
             self._advance(l + 1, tau, 0.5 * dt, 0.0, 0.5)
             self._advance(l + 1, tau + 0.5 * dt, 0.5 * dt, 0.5, 0.5)

@@ -105,6 +105,8 @@ _SIGS = {
     "axs_boxlevel_snapshot_coarse": (_I, [_P, _P]),
     "axs_boxlevel_fill_ghosts": (_I, [_P, _P, _D, _I]),
     "axs_boxlevel_rk4_step": (_I, [_P, C.POINTER(RhsCoeffs), _D, _P, _D, _D]),
+    "axs_boxlevel_rk4_step_graph": (_I, [_P, C.POINTER(RhsCoeffs), _D, _P, _D,
+                                         _D]),
     "axs_boxlevel_tag": (_I, [_P, C.POINTER(TaggerParams), _P, _P, _PI64]),
     "axs_boxlevel_energy_sums": (_I, [_P, _D, _D, _D, C.POINTER(Masking), _P,
                                       _PD]),
@@ -456,11 +458,12 @@ class BoxLevel:
             int(interior)))
 
     def rk4_step(self, coeffs, dt, coarse: Optional["BoxLevel"] = None,
-                 t0_frac=0.0, dt_frac=1.0):
+                 t0_frac=0.0, dt_frac=1.0, use_graph=False):
         arr = (RhsCoeffs * 4)(*coeffs)
-        check(self.lib.axs_boxlevel_rk4_step(
-            self.h, arr, dt, coarse.h if coarse is not None else None,
-            t0_frac, dt_frac))
+        fn = (self.lib.axs_boxlevel_rk4_step_graph if use_graph
+              else self.lib.axs_boxlevel_rk4_step)
+        check(fn(self.h, arr, dt, coarse.h if coarse is not None else None,
+                 t0_frac, dt_frac))
 
     def tag(self, amp_thr=0.0, R=1.0, tags: Optional[DeviceBuffer] = None,
             cover: Optional[DeviceBuffer] = None):

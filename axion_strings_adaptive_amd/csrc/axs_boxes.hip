@@ -68,6 +68,15 @@ struct axs_boxlevel
     size_t snap_bytes;
     const axs_boxlevel *snap_of;
     unsigned long long *derr;
+    // captured step graphs, keyed by (dt, coarse, t0_frac, dt_frac)
+    struct GraphEntry
+    {
+        double dt, t0, df;
+        const axs_boxlevel *coarse;
+        hipGraph_t g;
+        hipGraphExec_t ge;
+    };
+    std::vector<GraphEntry> graphs;
 };
 
 namespace axs
@@ -1039,6 +1048,11 @@ int axs_boxlevel_destroy(axs_boxlevel *b)
     if (!b)
         return AXS_OK;
     hipStreamSynchronize(b->ctx->stream);
+    for (auto &e : b->graphs)
+    {
+        hipGraphExecDestroy(e.ge);
+        hipGraphDestroy(e.g);
+    }
     hipFree(b->y);
     hipFree(b->acc);
     hipFree(b->ya);
@@ -1173,6 +1187,14 @@ int axs_boxlevel_snapshot_coarse(axs_boxlevel *b, const axs_boxlevel *c)
     const size_t lb = c->comp ? (size_t)4 * c->total * 4 : 0;
     if (b->snap_bytes != sb + lb)
     {
+        // captured graphs hold the old snapshot address
+        AXS_HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+        for (auto &e : b->graphs)
+        {
+            hipGraphExecDestroy(e.ge);
+            hipGraphDestroy(e.g);
+        }
+        b->graphs.clear();
         hipFree(b->snap);
         b->snap = nullptr;
         AXS_HIP_TRY(hipMalloc(&b->snap, sb + lb));
@@ -1242,6 +1264,67 @@ int axs_boxlevel_rk4_step(axs_boxlevel *b, const axs_rhs_coeffs coeffs[4],
             return rc;
     }
     return check_err(b, "axs_boxlevel_rk4_step");
+}
+
+int axs_boxlevel_rk4_step_graph(axs_boxlevel *b, const axs_rhs_coeffs coeffs[4],
+                                double dt, const axs_boxlevel *coarse,
+                                double t0, double df)
+{
+    AXS_CHECK_ARG(b && coeffs, "axs_boxlevel_rk4_step_graph: NULL argument");
+    if (coarse)
+        for (int d = 0; d < 3; ++d)
+            AXS_CHECK_ARG(coarse->dom[d] * 2 == b->dom[d] &&
+                              coarse->precision == b->precision,
+                          "axs_boxlevel_rk4_step_graph: coarse level mismatch");
+    double h[12];
+    for (int s = 0; s < 4; ++s)
+    {
+        h[3 * s] = coeffs[s].curvature_coeff;
+        h[3 * s + 1] = coeffs[s].lambda;
+        h[3 * s + 2] = coeffs[s].R_squared;
+    }
+    hipStream_t s = b->ctx->stream;
+    AXS_HIP_TRY(hipMemcpyAsync(b->dcoeff, h, sizeof(h), hipMemcpyHostToDevice, s));
+    AXS_HIP_TRY(hipStreamSynchronize(s));
+    axs_boxlevel::GraphEntry *hit = nullptr;
+    for (auto &e : b->graphs)
+        if (e.dt == dt && e.t0 == t0 && e.df == df && e.coarse == coarse)
+            hit = &e;
+    if (!hit)
+    {
+        AXS_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const double tf[4] = {t0, t0 + 0.5 * df, t0 + 0.5 * df, t0 + df};
+        void *in[4] = {b->y, b->ya, b->yb, b->ya};
+        void *out[4] = {b->ya, b->yb, b->ya, b->y};
+        int rc = AXS_OK;
+        for (int st = 0; st < 4 && rc == AXS_OK; ++st)
+        {
+            rc = fill(b, in[st], st == 0 ? b->comp : nullptr, coarse, tf[st], 0);
+            if (rc == AXS_OK)
+                rc = stage(b, st + 1, in[st], out[st], dt);
+        }
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(s, &g);
+        if (rc != AXS_OK || e != hipSuccess)
+        {
+            if (g)
+                hipGraphDestroy(g);
+            return rc != AXS_OK
+                       ? rc
+                       : set_error(AXS_ERR_HIP, std::string("hipStreamEndCapture: ") +
+                                                    hipGetErrorString(e));
+        }
+        hipGraphExec_t ge = nullptr;
+        if (hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess)
+        {
+            hipGraphDestroy(g);
+            return set_error(AXS_ERR_HIP, "hipGraphInstantiate failed");
+        }
+        b->graphs.push_back({dt, t0, df, coarse, g, ge});
+        hit = &b->graphs.back();
+    }
+    AXS_HIP_TRY(hipGraphLaunch(hit->ge, s));
+    return AXS_OK;
 }
 
 int axs_boxlevel_tag(axs_boxlevel *b, const axs_tagger_params *p,

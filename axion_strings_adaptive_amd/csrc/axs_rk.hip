@@ -41,6 +41,7 @@ template <typename T> struct StageParams
     int z_chunk;
     int tiles_x, tiles_y, tiles_z;
     int xcd_remap;
+    int nontemporal; // stream the stage outputs past the caches
 };
 
 __device__ __forceinline__ int wrap_lo(int i, int n) { return i < 0 ? i + n : i; }
@@ -56,11 +57,57 @@ template <> struct StarOf<float>
     using type = StarF;
 };
 
-template <typename T, int PREC, int MODE, int TY>
+// VX consecutive x cells per lane: vector loads/stores of VX words
+template <typename T, int VX> struct VecOf
+{
+    typedef T type __attribute__((ext_vector_type(VX)));
+};
+
+template <typename T, int VX>
+__device__ __forceinline__ void vload(const T *p, T (&o)[VX])
+{
+    if constexpr (VX == 1)
+        o[0] = p[0];
+    else
+    {
+        const typename VecOf<T, VX>::type v =
+            *reinterpret_cast<const typename VecOf<T, VX>::type *>(p);
+#pragma unroll
+        for (int i = 0; i < VX; ++i)
+            o[i] = v[i];
+    }
+}
+
+template <typename T, int VX>
+__device__ __forceinline__ void vstore(T *p, const T (&x)[VX], bool nt)
+{
+    if constexpr (VX == 1)
+    {
+        if (nt)
+            __builtin_nontemporal_store(x[0], p);
+        else
+            p[0] = x[0];
+    }
+    else
+    {
+        typename VecOf<T, VX>::type v;
+#pragma unroll
+        for (int i = 0; i < VX; ++i)
+            v[i] = x[i];
+        auto *q = reinterpret_cast<typename VecOf<T, VX>::type *>(p);
+        if (nt)
+            __builtin_nontemporal_store(v, q);
+        else
+            *q = v;
+    }
+}
+
+// Each workgroup: a (64 VX) x TY column tile, streaming a chunk of z.
+template <typename T, int PREC, int MODE, int TY, int VX>
 __global__ void __launch_bounds__(64 * TY)
     k_stage(StageParams<T> p)
 {
-    constexpr int TX = 64;
+    constexpr int TX = 64 * VX; // tile width in cells
     constexpr int HX = TX + 4;
     constexpr int HY = TY + 4;
     __shared__ T sm[2][2][HY][HX];
@@ -82,20 +129,27 @@ __global__ void __launch_bounds__(64 * TY)
     const int nx = p.nx, ny = p.ny, nz = p.nz;
     const uint32_t n = p.n;
     const uint32_t plane = (uint32_t)nx * (uint32_t)ny;
-    const int gx = bx * TX + tx;
+    const int gx = bx * TX + tx * VX; // first of this lane's VX cells
     const int gy = by * TY + ty;
-    const bool active = gx < nx && gy < ny;
+    const bool active = gx < nx && gy < ny; // nx % VX == 0
     const int xw = gx % nx;
     const int yw = gy % ny;
     const uint32_t rowc = (uint32_t)yw * nx + xw;
+    const bool nt = p.nontemporal != 0;
 
-    // halo roles
-    const bool hxl = tx < 2, hxr = tx >= TX - 2;
+    // halo roles: VX == 1 -> lanes 0,1 / 62,63 one cell each;
+    // VX >= 2 -> lane 0 / lane 63 one aligned pair each
+    constexpr int HL = VX == 1 ? 2 : 1;
+    const bool hxl = tx < HL, hxr = tx >= 64 - HL;
     const bool hyl = ty < 2, hyr = ty >= TY - 2;
     uint32_t rowhx = 0, rowhy = 0;
     if (hxl || hxr)
     {
-        int hx = hxl ? (bx * TX - 2 + tx) : (bx * TX + TX + (tx - (TX - 2)));
+        int hx;
+        if constexpr (VX == 1)
+            hx = hxl ? (bx * TX - 2 + tx) : (bx * TX + TX + (tx - 62));
+        else
+            hx = hxl ? bx * TX - 2 : bx * TX + TX;
         hx = ((hx % nx) + nx) % nx;
         rowhx = (uint32_t)yw * nx + hx;
     }
@@ -105,6 +159,7 @@ __global__ void __launch_bounds__(64 * TY)
         hy = ((hy % ny) + ny) % ny;
         rowhy = (uint32_t)hy * nx + xw;
     }
+    constexpr int HW = VX == 1 ? 1 : 2; // halo words per x-halo lane
 
     const int z_begin = bz * p.z_chunk;
     const int z_end = min(z_begin + p.z_chunk, nz);
@@ -118,26 +173,26 @@ __global__ void __launch_bounds__(64 * TY)
     };
 
     // register queues: psi at z-2 .. z+2
-    T q1[5], q2[5];
+    T q1[5][VX], q2[5][VX];
 #pragma unroll
     for (int i = 0; i < 5; ++i)
     {
         const uint32_t o = zoff(z_begin - 2 + i) + rowc;
-        q1[i] = psi1[o];
-        q2[i] = psi2[o];
+        vload<T, VX>(psi1 + o, q1[i]);
+        vload<T, VX>(psi2 + o, q2[i]);
     }
-    T hx1 = 0, hx2 = 0, hy1 = 0, hy2 = 0;
+    T hx1[HW], hx2[HW], hy1[VX], hy2[VX];
     {
         const uint32_t o = zoff(z_begin);
         if (hxl || hxr)
         {
-            hx1 = psi1[o + rowhx];
-            hx2 = psi2[o + rowhx];
+            vload<T, HW>(psi1 + o + rowhx, hx1);
+            vload<T, HW>(psi2 + o + rowhx, hx2);
         }
         if (hyl || hyr)
         {
-            hy1 = psi1[o + rowhy];
-            hy2 = psi2[o + rowhy];
+            vload<T, VX>(psi1 + o + rowhy, hy1);
+            vload<T, VX>(psi2 + o + rowhy, hy2);
         }
     }
 
@@ -150,73 +205,120 @@ __global__ void __launch_bounds__(64 * TY)
     for (int z = z_begin; z < z_end; ++z)
     {
         const int b = (z - z_begin) & 1;
+        const int cx0 = tx * VX + 2;
         // ---- stage the plane into LDS ----
-        sm[b][0][ty + 2][tx + 2] = q1[2];
-        sm[b][1][ty + 2][tx + 2] = q2[2];
-        if (hxl)
+#pragma unroll
+        for (int v = 0; v < VX; ++v)
         {
-            sm[b][0][ty + 2][tx] = hx1;
-            sm[b][1][ty + 2][tx] = hx2;
+            sm[b][0][ty + 2][cx0 + v] = q1[2][v];
+            sm[b][1][ty + 2][cx0 + v] = q2[2][v];
         }
-        else if (hxr)
+        if constexpr (VX == 1)
         {
-            sm[b][0][ty + 2][tx + 4] = hx1;
-            sm[b][1][ty + 2][tx + 4] = hx2;
+            if (hxl)
+            {
+                sm[b][0][ty + 2][tx] = hx1[0];
+                sm[b][1][ty + 2][tx] = hx2[0];
+            }
+            else if (hxr)
+            {
+                sm[b][0][ty + 2][tx + 4] = hx1[0];
+                sm[b][1][ty + 2][tx + 4] = hx2[0];
+            }
+        }
+        else
+        {
+            if (hxl)
+            {
+                sm[b][0][ty + 2][0] = hx1[0];
+                sm[b][0][ty + 2][1] = hx1[1];
+                sm[b][1][ty + 2][0] = hx2[0];
+                sm[b][1][ty + 2][1] = hx2[1];
+            }
+            else if (hxr)
+            {
+                sm[b][0][ty + 2][TX + 2] = hx1[0];
+                sm[b][0][ty + 2][TX + 3] = hx1[1];
+                sm[b][1][ty + 2][TX + 2] = hx2[0];
+                sm[b][1][ty + 2][TX + 3] = hx2[1];
+            }
         }
         if (hyl)
         {
-            sm[b][0][ty][tx + 2] = hy1;
-            sm[b][1][ty][tx + 2] = hy2;
+#pragma unroll
+            for (int v = 0; v < VX; ++v)
+            {
+                sm[b][0][ty][cx0 + v] = hy1[v];
+                sm[b][1][ty][cx0 + v] = hy2[v];
+            }
         }
         else if (hyr)
         {
-            sm[b][0][ty + 4][tx + 2] = hy1;
-            sm[b][1][ty + 4][tx + 2] = hy2;
+#pragma unroll
+            for (int v = 0; v < VX; ++v)
+            {
+                sm[b][0][ty + 4][cx0 + v] = hy1[v];
+                sm[b][1][ty + 4][cx0 + v] = hy2[v];
+            }
         }
 
         // ---- prefetch next plane (queue head + halos) ----
-        T nq1 = q1[4], nq2 = q2[4];
+        T nq1[VX], nq2[VX];
+#pragma unroll
+        for (int v = 0; v < VX; ++v)
+        {
+            nq1[v] = q1[4][v];
+            nq2[v] = q2[4][v];
+        }
         if (z + 1 < z_end)
         {
             const uint32_t o3 = zoff(z + 3) + rowc;
-            nq1 = psi1[o3];
-            nq2 = psi2[o3];
+            vload<T, VX>(psi1 + o3, nq1);
+            vload<T, VX>(psi2 + o3, nq2);
             const uint32_t o1 = zoff(z + 1);
             if (hxl || hxr)
             {
-                hx1 = psi1[o1 + rowhx];
-                hx2 = psi2[o1 + rowhx];
+                vload<T, HW>(psi1 + o1 + rowhx, hx1);
+                vload<T, HW>(psi2 + o1 + rowhx, hx2);
             }
             if (hyl || hyr)
             {
-                hy1 = psi1[o1 + rowhy];
-                hy2 = psi2[o1 + rowhy];
+                vload<T, VX>(psi1 + o1 + rowhy, hy1);
+                vload<T, VX>(psi2 + o1 + rowhy, hy2);
             }
         }
 
         // ---- centre loads of this plane ----
         const uint32_t oc = (uint32_t)z * plane + rowc;
-        T pi[2] = {0, 0};
-        T yn[4] = {0, 0, 0, 0}, ac[4] = {0, 0, 0, 0};
-        float lo[4] = {0, 0, 0, 0};
+        T pi[2][VX], yn[4][VX], ac[4][VX];
+        float lo[4][VX];
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int v = 0; v < VX; ++v)
+            {
+                yn[c][v] = 0;
+                ac[c][v] = 0;
+                lo[c][v] = 0.0f;
+            }
         if (active)
         {
-            pi[0] = p.yin[2 * n + oc];
-            pi[1] = p.yin[3 * n + oc];
+            vload<T, VX>(p.yin + 2 * n + oc, pi[0]);
+            vload<T, VX>(p.yin + 3 * n + oc, pi[1]);
             if (MODE == MODE_S23 || MODE == MODE_S4)
             {
 #pragma unroll
                 for (int c = 0; c < 4; ++c)
                 {
-                    yn[c] = p.yn[c * n + oc];
-                    ac[c] = p.acc[c * n + oc];
+                    vload<T, VX>(p.yn + c * n + oc, yn[c]);
+                    vload<T, VX>(p.acc + c * n + oc, ac[c]);
                 }
             }
             if (MODE == MODE_S4 && PREC == AXS_FP32C)
             {
 #pragma unroll
                 for (int c = 0; c < 4; ++c)
-                    lo[c] = p.comp[c * n + oc];
+                    vload<float, VX>(p.comp + c * n + oc, lo[c]);
             }
         }
 
@@ -224,53 +326,81 @@ __global__ void __launch_bounds__(64 * TY)
 
         if (active)
         {
-            const int cy = ty + 2, cx = tx + 2;
-            using S = typename StarOf<T>::type;
-            const S s1{sm[b][0][cy][cx - 2], sm[b][0][cy][cx - 1],
-                       sm[b][0][cy][cx + 1], sm[b][0][cy][cx + 2],
-                       sm[b][0][cy - 2][cx], sm[b][0][cy - 1][cx],
-                       sm[b][0][cy + 1][cx], sm[b][0][cy + 2][cx],
-                       q1[0], q1[1], q1[3], q1[4], q1[2]};
-            const S s2{sm[b][1][cy][cx - 2], sm[b][1][cy][cx - 1],
-                       sm[b][1][cy][cx + 1], sm[b][1][cy][cx + 2],
-                       sm[b][1][cy - 2][cx], sm[b][1][cy - 1][cx],
-                       sm[b][1][cy + 1][cx], sm[b][1][cy + 2][cx],
-                       q2[0], q2[1], q2[3], q2[4], q2[2]};
-            T F1, F2;
-            force(star_lap(s1, i2), star_lap(s2, i2), q1[2], q2[2], cc, lam,
-                  R2, F1, F2);
-            const T k[4] = {pi[0], pi[1], F1, F2};
-            if (MODE == MODE_S1)
+            const int cy = ty + 2;
+            T out_y[4][VX], out_a[4][VX];
+#pragma unroll
+            for (int v = 0; v < VX; ++v)
             {
-                // y_n is the stage input itself
-                yn[0] = q1[2];
-                yn[1] = q2[2];
-                yn[2] = pi[0];
-                yn[3] = pi[1];
+                const int cx = cx0 + v;
+                using S = typename StarOf<T>::type;
+                const S s1{sm[b][0][cy][cx - 2], sm[b][0][cy][cx - 1],
+                           sm[b][0][cy][cx + 1], sm[b][0][cy][cx + 2],
+                           sm[b][0][cy - 2][cx], sm[b][0][cy - 1][cx],
+                           sm[b][0][cy + 1][cx], sm[b][0][cy + 2][cx],
+                           q1[0][v], q1[1][v], q1[3][v], q1[4][v], q1[2][v]};
+                const S s2{sm[b][1][cy][cx - 2], sm[b][1][cy][cx - 1],
+                           sm[b][1][cy][cx + 1], sm[b][1][cy][cx + 2],
+                           sm[b][1][cy - 2][cx], sm[b][1][cy - 1][cx],
+                           sm[b][1][cy + 1][cx], sm[b][1][cy + 2][cx],
+                           q2[0][v], q2[1][v], q2[3][v], q2[4][v], q2[2][v]};
+                T F1, F2;
+                force(star_lap(s1, i2), star_lap(s2, i2), q1[2][v], q2[2][v],
+                      cc, lam, R2, F1, F2);
+                const T k[4] = {pi[0][v], pi[1][v], F1, F2};
+                T ynv[4], acv[4];
+                float lov[4];
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                {
+                    ynv[c] = yn[c][v];
+                    acv[c] = ac[c][v];
+                    lov[c] = lo[c][v];
+                }
+                if (MODE == MODE_S1)
+                {
+                    // y_n is the stage input itself
+                    ynv[0] = q1[2][v];
+                    ynv[1] = q2[2][v];
+                    ynv[2] = pi[0][v];
+                    ynv[3] = pi[1][v];
+                }
+                T acc_out[4], y_out[4];
+                stage_update<T, PREC, MODE>(k, ynv, acv, lov, p.a, p.h6,
+                                            p.h6_lo, acc_out, y_out);
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                {
+                    out_y[c][v] = y_out[c];
+                    out_a[c][v] = acc_out[c];
+                    lo[c][v] = lov[c];
+                }
             }
-            T acc_out[4], y_out[4];
-            stage_update<T, PREC, MODE>(k, yn, ac, lo, p.a, p.h6, p.h6_lo,
-                                        acc_out, y_out);
 #pragma unroll
             for (int c = 0; c < 4; ++c)
             {
-                p.yout[c * n + oc] = y_out[c];
+                vstore<T, VX>(p.yout + c * n + oc, out_y[c], nt);
                 if (MODE == MODE_S1 || MODE == MODE_S23)
-                    p.acc[c * n + oc] = acc_out[c];
+                    vstore<T, VX>(p.acc + c * n + oc, out_a[c], nt);
                 if (MODE == MODE_S4 && PREC == AXS_FP32C)
-                    p.comp[c * n + oc] = lo[c];
+                    vstore<float, VX>(p.comp + c * n + oc, lo[c], nt);
             }
         }
 
         // ---- advance the queue ----
 #pragma unroll
         for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int v = 0; v < VX; ++v)
+            {
+                q1[i][v] = q1[i + 1][v];
+                q2[i][v] = q2[i + 1][v];
+            }
+#pragma unroll
+        for (int v = 0; v < VX; ++v)
         {
-            q1[i] = q1[i + 1];
-            q2[i] = q2[i + 1];
+            q1[4][v] = nq1[v];
+            q2[4][v] = nq2[v];
         }
-        q1[4] = nq1;
-        q2[4] = nq2;
     }
 }
 
@@ -281,47 +411,67 @@ __global__ void k_step_advance(int *step, int reset)
 }
 
 // ------------------------------------------------------------ launchers --
-static const int kTileTY[3] = {4, 8, 16};
+// Tile variants: (TY, VX) pairs; variants >= kBaseVariants use
+// non-temporal stores for the stage outputs.
+struct TileShape
+{
+    int ty, vx;
+};
+static const TileShape kTiles[] = {{4, 1}, {8, 1}, {16, 1}, {4, 2},
+                                   {8, 2}, {4, 4}, {8, 4}};
+constexpr int kBaseVariants = 7;
 
-int num_tile_variants() { return 3; }
+int num_tile_variants() { return 2 * kBaseVariants; }
 
-template <typename T, int PREC, int MODE, int TY>
+template <typename T, int PREC, int MODE, int TY, int VX>
 static int launch_t(const StageParams<T> &p, int ntiles, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_stage<T, PREC, MODE, TY>), dim3(ntiles),
+    hipLaunchKernelGGL((k_stage<T, PREC, MODE, TY, VX>), dim3(ntiles),
                        dim3(64, TY), 0, s, p);
     return AXS_OK;
 }
 
 template <typename T, int PREC, int MODE>
-static int launch_ty(int variant, const StageParams<T> &p, int ntiles,
+static int launch_ty(int base, const StageParams<T> &p, int ntiles,
                      hipStream_t s)
 {
-    switch (variant)
+    switch (base)
     {
     case 0:
-        return launch_t<T, PREC, MODE, 4>(p, ntiles, s);
+        return launch_t<T, PREC, MODE, 4, 1>(p, ntiles, s);
     case 1:
-        return launch_t<T, PREC, MODE, 8>(p, ntiles, s);
+        return launch_t<T, PREC, MODE, 8, 1>(p, ntiles, s);
+    case 2:
+        return launch_t<T, PREC, MODE, 16, 1>(p, ntiles, s);
+    case 3:
+        return launch_t<T, PREC, MODE, 4, 2>(p, ntiles, s);
+    case 4:
+        return launch_t<T, PREC, MODE, 8, 2>(p, ntiles, s);
     default:
-        return launch_t<T, PREC, MODE, 16>(p, ntiles, s);
+        // FP64 x4 tiles would exceed 64 KiB of static LDS
+        if constexpr (sizeof(T) == 8)
+            return launch_t<T, PREC, MODE, 8, 2>(p, ntiles, s);
+        else if (base == 5)
+            return launch_t<T, PREC, MODE, 4, 4>(p, ntiles, s);
+        else
+            return launch_t<T, PREC, MODE, 8, 4>(p, ntiles, s);
     }
 }
 
 template <typename T, int PREC>
-static int launch_mode(int mode, int variant, const StageParams<T> &p,
-                       int ntiles, hipStream_t s)
+static int launch_mode(int mode, int base, const StageParams<T> &p, int ntiles,
+                       hipStream_t s)
 {
     switch (mode)
     {
     case MODE_RHS:
-        return launch_ty<T, PREC, MODE_RHS>(variant, p, ntiles, s);
+        return launch_ty<T, PREC, MODE_RHS>(base, p, ntiles, s);
     case MODE_S1:
-        return launch_ty<T, PREC, MODE_S1>(variant, p, ntiles, s);
+        return launch_ty<T, PREC, MODE_S1>(base, p, ntiles, s);
     case MODE_S23:
-        return launch_ty<T, PREC, MODE_S23>(variant, p, ntiles, s);
+        return launch_ty<T, PREC, MODE_S23>(base, p, ntiles, s);
     default:
-        return launch_ty<T, PREC, MODE_S4>(variant, p, ntiles, s);
+        return launch_ty<T, PREC, MODE_S4>(base, p, ntiles, s);
     }
 }
 
@@ -347,10 +497,18 @@ static int fill_and_launch(axs_level *L, int mode, const void *yin, void *yout,
     p.nz = L->g.nz;
     p.n = L->g.ncell;
     p.inv_dx2 = 1.0 / (L->dx * L->dx);
-    const int variant =
-        (L->tile_variant >= 0 && L->tile_variant < 3) ? L->tile_variant : 1;
-    const int TY = kTileTY[variant];
-    p.tiles_x = (p.nx + 63) / 64;
+    int variant = (L->tile_variant >= 0 && L->tile_variant < num_tile_variants())
+                      ? L->tile_variant
+                      : 1;
+    p.nontemporal = variant >= kBaseVariants ? 1 : 0;
+    int base = variant % kBaseVariants;
+    if (sizeof(T) == 8 && kTiles[base].vx == 4)
+        base = 4; // FP64 x4 tiles would exceed 64 KiB of static LDS
+    // vector variants need whole vectors per row
+    if (p.nx % kTiles[base].vx != 0)
+        base = 1;
+    const int TY = kTiles[base].ty, VX = kTiles[base].vx;
+    p.tiles_x = (p.nx + 64 * VX - 1) / (64 * VX);
     p.tiles_y = (p.ny + TY - 1) / TY;
     int zc = L->z_chunk;
     if (zc <= 0)
@@ -368,13 +526,13 @@ static int fill_and_launch(axs_level *L, int mode, const void *yin, void *yout,
     p.xcd_remap = (ntiles % 8 == 0) ? 1 : 0;
     int rc;
     if constexpr (sizeof(T) == 8)
-        rc = launch_mode<T, AXS_FP64>(mode, variant, p, (int)ntiles,
+        rc = launch_mode<T, AXS_FP64>(mode, base, p, (int)ntiles,
                                       L->ctx->stream);
     else if (L->precision == AXS_FP32C)
-        rc = launch_mode<T, AXS_FP32C>(mode, variant, p, (int)ntiles,
+        rc = launch_mode<T, AXS_FP32C>(mode, base, p, (int)ntiles,
                                        L->ctx->stream);
     else
-        rc = launch_mode<T, AXS_FP32>(mode, variant, p, (int)ntiles,
+        rc = launch_mode<T, AXS_FP32>(mode, base, p, (int)ntiles,
                                       L->ctx->stream);
     if (rc != AXS_OK)
         return rc;

@@ -281,6 +281,14 @@ int axs_boxlevel_fill_ghosts(axs_boxlevel *bl, const axs_boxlevel *coarse,
 int axs_boxlevel_rk4_step(axs_boxlevel *bl, const axs_rhs_coeffs coeffs[4],
                           double dt, const axs_boxlevel *coarse,
                           double t0_frac, double dt_frac);
+/* The same step replayed from a hipGraph captured on first use for this
+ * (dt, coarse, t0_frac, dt_frac): the 4 ghost fills and 4 work-list stage
+ * launches are one graph launch; the stage scalars are copied to the device
+ * before it.  Cells without an owner are not reported in this mode. */
+int axs_boxlevel_rk4_step_graph(axs_boxlevel *bl,
+                                const axs_rhs_coeffs coeffs[4], double dt,
+                                const axs_boxlevel *coarse, double t0_frac,
+                                double dt_frac);
 /* StringTagger over every valid box cell into a dense device uint8 array
  * of the level's domain (sets only); plaquette counts of the same pass.
  * Ghosts must be filled.  cover: optional dense device uint8 coverage mask

@@ -31,6 +31,20 @@ def test_hierarchy_volume_and_nesting(ctx):
     h.close()
 
 
+def test_graph_replay_matches_direct_launches(ctx):
+    from axion_strings_adaptive_amd import amr, capi
+    out = []
+    for use_graph in (False, True):
+        h = amr.Hierarchy(ctx, n0=32, levels=2, precision=capi.AXS_FP32C,
+                          verbose=False, use_graph=use_graph)
+        for _ in range(3):
+            h.step()
+        out.append([L.download_dense(fill=0.0) for L in h.levels])
+        h.close()
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+
+
 def test_fully_refined_hierarchy_tracks_uniform_fine_grid(ctx, oracle):
     """With force_full_refinement, level 1 covers the whole domain: its own
     periodic ghosts make it independent of level 0, so after one coarse

@@ -29,8 +29,8 @@ def seeded(oracle, n, seed=42):
 
 
 @pytest.mark.parametrize("shape", [(32, 32, 32), (64, 64, 64), (40, 24, 36),
-                                   (4, 8, 12), (128, 16, 8)])
-@pytest.mark.parametrize("variant", [0, 1, 2])
+                                   (4, 8, 12), (128, 16, 8), (36, 8, 8)])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 8, 12])
 def test_rhs_fp64_bitexact(ctx, oracle, shape, variant):
     capi = _capi()
     s = random_state(shape, seed=sum(shape))
@@ -90,17 +90,20 @@ def test_rk4_step_explicit_coeffs_moore(ctx, oracle):
     assert np.array_equal(lvl.download(), s)
 
 
-@pytest.mark.parametrize("variant,zc", [(0, 4), (1, 0), (2, 16), (1, 64)])
-def test_tile_variants_identical(ctx, oracle, variant, zc):
+@pytest.mark.parametrize("variant,zc", [(0, 4), (1, 0), (2, 16), (1, 64),
+                                        (3, 8), (4, 0), (5, 32), (6, 0),
+                                        (9, 0), (13, 16)])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_tile_variants_identical(ctx, oracle, variant, zc, precision):
     capi = _capi()
     n = 64
     s0 = seeded(oracle, n, seed=11)
     bg = capi.make_background(2.0, c0=1.0)
-    lvl = capi.Level(ctx, n, 0.5, capi.AXS_FP64)
+    lvl = capi.Level(ctx, n, 0.5, precision)
     lvl.upload(s0)
     lvl.set_tile(variant, zc)
     lvl.advance(bg, 1.0, 0.1, 3)
-    ref = capi.Level(ctx, n, 0.5, capi.AXS_FP64)
+    ref = capi.Level(ctx, n, 0.5, precision)
     ref.upload(s0)
     ref.advance(bg, 1.0, 0.1, 3)
     assert np.array_equal(lvl.download(), ref.download())
