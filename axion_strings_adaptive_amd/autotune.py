@@ -41,7 +41,8 @@ def cached_choice(n: int, precision: int):
     return (e["variant"], e["z_chunk"]) if e else None
 
 
-def tune(ctx, n: int, precision: int, steps: int = 2, verbose=True):
+def tune(ctx, n: int, precision: int, steps: int = 2, verbose=True,
+         cache_path: str = CACHE):
     from . import capi
     lvl = capi.Level(ctx, n, 0.5, precision)
     lvl.fourier_ic(0, n / 8.0, 0.5, 42, 0)
@@ -69,11 +70,15 @@ def tune(ctx, n: int, precision: int, steps: int = 2, verbose=True):
     lvl.close()
     t, v, zc = min(results)
     cache = load_cache()
+    if cache_path != CACHE and os.path.exists(cache_path):
+        with open(cache_path) as f:
+            cache.update(json.load(f))
     cache[_key(n, precision)] = {"variant": v, "z_chunk": zc, "ms_per_step": t,
                                  "candidates": [[a, b, c] for a, b, c in
                                                 sorted(results)]}
-    with open(CACHE, "w") as f:
-        json.dump(cache, f, indent=1, sort_keys=True)
+    for path in {CACHE, cache_path}:
+        with open(path, "w") as f:
+            json.dump(cache, f, indent=1, sort_keys=True)
     return v, zc, t
 
 
@@ -81,13 +86,16 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--n", type=int, nargs="+", default=[1024])
     ap.add_argument("--precision", type=int, nargs="+", default=[1, 0])
+    ap.add_argument("--out", default=CACHE,
+                    help="also write the cache to this path")
     args = ap.parse_args(argv)
     from . import capi
     ctx = capi.Context(0)
     for n in args.n:
         for p in args.precision:
             print(json.dumps({"n": n, "precision": p,
-                              "best": tune(ctx, n, p)}), flush=True)
+                              "best": tune(ctx, n, p, cache_path=args.out)}),
+                  flush=True)
     ctx.close()
 
 

@@ -75,11 +75,13 @@ _SIGS = {
     "axs_level_download_raw": (_I, [_P, _P, _P]),
     "axs_level_device_ptrs": (_I, [_P, C.POINTER(_P), C.POINTER(_P)]),
     "axs_level_set_homogeneous": (_I, [_P, _D, _D]),
+    "axs_level_handoff_rescale": (_I, [_P, _D, _D, _D]),
     "axs_level_set_tile": (_I, [_P, _I, _I]),
     "axs_level_get_tile": (_I, [_P, C.POINTER(_I), C.POINTER(_I)]),
     "axs_rhs_eval": (_I, [_P, C.POINTER(RhsCoeffs), _P]),
     "axs_rk4_step": (_I, [_P, C.POINTER(RhsCoeffs), _D]),
     "axs_rk4_advance": (_I, [_P, C.POINTER(Background), _D, _D, _I, _I]),
+    "axs_rk4_advance_coeffs": (_I, [_P, C.POINTER(RhsCoeffs), _D, _I, _I]),
     "axs_level_enable_timing": (_I, [_P, _I]),
     "axs_level_stage_times": (_I, [_P, _PD, _PI64]),
     "axs_count_plaquettes": (_I, [_P, _P, _PI64]),
@@ -288,6 +290,10 @@ class Level:
     def set_homogeneous(self, R: float, Rp: float):
         check(self.lib.axs_level_set_homogeneous(self.h, R, Rp))
 
+    def handoff_rescale(self, kappa: float, common_term: float, R_pre: float):
+        check(self.lib.axs_level_handoff_rescale(self.h, kappa, common_term,
+                                                 R_pre))
+
     def set_tile(self, variant: int, z_chunk: int = 0):
         check(self.lib.axs_level_set_tile(self.h, variant, z_chunk))
 
@@ -306,6 +312,14 @@ class Level:
                 use_graph: bool = False):
         check(self.lib.axs_rk4_advance(self.h, C.byref(bg), tau, dt, nsteps,
                                        int(use_graph)))
+
+    def advance_coeffs(self, coeffs, dt: float, use_graph: bool = False):
+        """len(coeffs) // 4 RK4 steps; coeffs[4 i + s] = stage s of step i."""
+        assert len(coeffs) % 4 == 0
+        arr = (RhsCoeffs * len(coeffs))(*coeffs)
+        check(self.lib.axs_rk4_advance_coeffs(self.h, arr, dt,
+                                              len(coeffs) // 4,
+                                              int(use_graph)))
 
     def enable_timing(self, on: bool = True):
         check(self.lib.axs_level_enable_timing(self.h, int(on)))

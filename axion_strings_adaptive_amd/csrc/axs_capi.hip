@@ -64,6 +64,35 @@ __global__ void k_set_homogeneous(T *y, float *lo, size_t n, double R,
     }
 }
 
+// The pre-evolution -> main handoff (AxionStringsLevel.cpp:2106-2124):
+// psi <- kappa psi, Pi <- kappa^2 Pi + (psi / R_pre) common_term, evaluated
+// in FP64 left to right (FP32C: on hi + lo, re-split afterwards).
+template <typename T>
+__global__ void k_handoff_rescale(T *y, float *lo, size_t n, double kappa,
+                                  double common, double R_pre)
+{
+#pragma clang fp contract(off)
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+    {
+        double v[4];
+        for (int c = 0; c < 4; ++c)
+            v[c] = (double)y[c * n + i] + (lo ? (double)lo[c * n + i] : 0.0);
+        double o[4];
+        o[0] = kappa * v[0];
+        o[1] = kappa * v[1];
+        o[2] = kappa * kappa * v[2] + (v[0] / R_pre) * common;
+        o[3] = kappa * kappa * v[3] + (v[1] / R_pre) * common;
+        for (int c = 0; c < 4; ++c)
+        {
+            const T hi = (T)o[c];
+            y[c * n + i] = hi;
+            if (lo)
+                lo[c * n + i] = (float)(o[c] - (double)hi);
+        }
+    }
+}
+
 template <typename T>
 __global__ void k_zero_range(T *y, float *lo, size_t begin, size_t end)
 {
@@ -466,6 +495,26 @@ int axs_level_set_homogeneous(axs_level *L, double R, double Rp)
     return AXS_OK;
 }
 
+int axs_level_handoff_rescale(axs_level *L, double kappa, double common_term,
+                              double R_pre)
+{
+    AXS_CHECK_ARG(L, "axs_level_handoff_rescale: NULL level");
+    AXS_CHECK_ARG(R_pre != 0.0, "axs_level_handoff_rescale: R_pre must be != 0");
+    const size_t n = L->g.ncell;
+    hipStream_t s = L->ctx->stream;
+    if (L->precision == AXS_FP64)
+        hipLaunchKernelGGL(k_handoff_rescale<double>, dim3(grid_for(n)),
+                           dim3(256), 0, s, (double *)L->y, (float *)nullptr, n,
+                           kappa, common_term, R_pre);
+    else
+        hipLaunchKernelGGL(k_handoff_rescale<float>, dim3(grid_for(n)),
+                           dim3(256), 0, s, (float *)L->y,
+                           L->precision == AXS_FP32C ? L->comp : nullptr, n,
+                           kappa, common_term, R_pre);
+    AXS_HIP_TRY(hipGetLastError());
+    return AXS_OK;
+}
+
 int axs_level_zero_components(axs_level *L, int c0, int ncomp)
 {
     AXS_CHECK_ARG(L, "axs_level_zero_components: NULL level");
@@ -643,14 +692,14 @@ static int capture_graph(axs_level *L, double dt)
     return AXS_OK;
 }
 
-int axs_rk4_advance(axs_level *L, const axs_background *bg, double tau,
-                    double dt, int nsteps, int use_graph)
+} // extern "C"
+
+// nsteps RK4 steps whose stage scalars come from row(i, s, &c) (step i,
+// stage s), uploaded as one device table per chunk of kCoeffTableSteps.
+template <typename RowFn>
+static int advance_table(axs_level *L, double dt, int nsteps, int use_graph,
+                         RowFn row)
 {
-    AXS_CHECK_ARG(L && bg, "axs_rk4_advance: NULL argument");
-    AXS_CHECK_ARG(nsteps >= 0, "axs_rk4_advance: nsteps must be >= 0");
-    AXS_CHECK_ARG(dt > 0.0, "axs_rk4_advance: dt must be > 0");
-    AXS_CHECK_ARG(bg->flat_space || bg->a_inv > 1.0,
-                  "axs_rk4_advance: a_inv must be > 1");
     std::vector<double> table;
     int done = 0;
     while (done < nsteps)
@@ -658,20 +707,14 @@ int axs_rk4_advance(axs_level *L, const axs_background *bg, double tau,
         const int chunk = std::min(nsteps - done, kCoeffTableSteps);
         table.assign((size_t)chunk * 12, 0.0);
         for (int i = 0; i < chunk; ++i)
-        {
-            // stage times tau, tau+dt/2, tau+dt/2, tau+dt; tau += dt per step
-            const double ts[4] = {tau, tau + 0.5 * dt, tau + 0.5 * dt,
-                                  tau + dt};
             for (int s = 0; s < 4; ++s)
             {
                 axs_rhs_coeffs c;
-                rhs_coeffs_at(bg, ts[s], &c);
+                row(done + i, s, &c);
                 table[(size_t)i * 12 + 3 * s] = c.curvature_coeff;
                 table[(size_t)i * 12 + 3 * s + 1] = c.lambda;
                 table[(size_t)i * 12 + 3 * s + 2] = c.R_squared;
             }
-            tau += dt;
-        }
         int rc = upload_coeffs(L, table.data(), table.size(), L->d_coeff);
         if (rc)
             return rc;
@@ -698,6 +741,47 @@ int axs_rk4_advance(axs_level *L, const axs_background *bg, double tau,
         done += chunk;
     }
     return AXS_OK;
+}
+
+extern "C" {
+
+int axs_rk4_advance(axs_level *L, const axs_background *bg, double tau,
+                    double dt, int nsteps, int use_graph)
+{
+    AXS_CHECK_ARG(L && bg, "axs_rk4_advance: NULL argument");
+    AXS_CHECK_ARG(nsteps >= 0, "axs_rk4_advance: nsteps must be >= 0");
+    AXS_CHECK_ARG(dt > 0.0, "axs_rk4_advance: dt must be > 0");
+    AXS_CHECK_ARG(bg->flat_space || bg->a_inv > 1.0,
+                  "axs_rk4_advance: a_inv must be > 1");
+    // stage times tau, tau+dt/2, tau+dt/2, tau+dt; tau += dt per step
+    int cur = -1;
+    double ts[4] = {0, 0, 0, 0};
+    return advance_table(L, dt, nsteps, use_graph,
+                         [&](int i, int s, axs_rhs_coeffs *c) {
+                             while (cur < i)
+                             {
+                                 if (cur >= 0)
+                                     tau += dt;
+                                 ts[0] = tau;
+                                 ts[1] = ts[2] = tau + 0.5 * dt;
+                                 ts[3] = tau + dt;
+                                 ++cur;
+                             }
+                             rhs_coeffs_at(bg, ts[s], c);
+                         });
+}
+
+int axs_rk4_advance_coeffs(axs_level *L, const axs_rhs_coeffs *coeffs,
+                           double dt, int nsteps, int use_graph)
+{
+    AXS_CHECK_ARG(L && (coeffs || nsteps == 0),
+                  "axs_rk4_advance_coeffs: NULL argument");
+    AXS_CHECK_ARG(nsteps >= 0, "axs_rk4_advance_coeffs: nsteps must be >= 0");
+    AXS_CHECK_ARG(dt > 0.0, "axs_rk4_advance_coeffs: dt must be > 0");
+    return advance_table(L, dt, nsteps, use_graph,
+                         [&](int i, int s, axs_rhs_coeffs *c) {
+                             *c = coeffs[(size_t)i * 4 + s];
+                         });
 }
 
 int axs_level_enable_timing(axs_level *L, int on)

@@ -8,7 +8,10 @@
 // z, then AxionStringsRHS.impl.hpp:105-117 term by term.  FP32 writes the
 // Laplacian on differences from the centre value (exact by Sterbenz' lemma
 // for a smooth field), removing the cancellation of the -30 f0 term when
-// |psi| ~ R >> 1.
+// |psi| ~ R >> 1.  Every FP32 expression is also written out with explicit
+// fmaf and contraction off, so FP32 results do not depend on how a kernel's
+// shape (tile, cells per lane) lets the compiler fuse operations: every tile
+// variant and the box kernel produce identical bits.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -60,13 +63,14 @@ __device__ __forceinline__ double star_lap(const Star &s, double i2)
 
 __device__ __forceinline__ float star_lap(const StarF &s, float i2)
 {
+#pragma clang fp contract(off)
     const float w_near = 4.0f / 3.0f, w_far = 1.0f / 12.0f;
     const float c = s.c;
     const float nsum = ((s.xm1 - c) + (s.xp1 - c)) + ((s.ym1 - c) + (s.yp1 - c)) +
                        ((s.zm1 - c) + (s.zp1 - c));
     const float fsum = ((s.xm2 - c) + (s.xp2 - c)) + ((s.ym2 - c) + (s.yp2 - c)) +
                        ((s.zm2 - c) + (s.zp2 - c));
-    return (w_near * nsum - w_far * fsum) * i2;
+    return fmaf(w_near, nsum, -(w_far * fsum)) * i2;
 }
 
 // Pi' = lap psi + c psi - (lambda/2)(|psi|^2 - R^2) psi
@@ -85,11 +89,12 @@ __device__ __forceinline__ void force(float lap1, float lap2, float psi1,
                                       float psi2, double cc, double lam,
                                       double R2, float &F1, float &F2)
 {
-    const float d = psi1 * psi1 + psi2 * psi2 - (float)R2;
-    const float pc = 0.5f * (float)lam * d;
+#pragma clang fp contract(off)
+    const float d = fmaf(psi1, psi1, fmaf(psi2, psi2, -(float)R2));
+    const float pc = (0.5f * (float)lam) * d;
     const float ccf = (float)cc;
-    F1 = lap1 + ccf * psi1 - pc * psi1;
-    F2 = lap2 + ccf * psi2 - pc * psi2;
+    F1 = fmaf(-pc, psi1, fmaf(ccf, psi1, lap1));
+    F2 = fmaf(-pc, psi2, fmaf(ccf, psi2, lap2));
 }
 
 __device__ __forceinline__ double axpy_f64(double y, double a, double k)
@@ -119,6 +124,7 @@ __device__ __forceinline__ void stage_update(const T k[4], const T yn[4],
                                              double a, double h6, float h6_lo,
                                              T acc_out[4], T y_out[4])
 {
+#pragma clang fp contract(off)
     if constexpr (MODE == MODE_RHS)
     {
         for (int c = 0; c < 4; ++c)
@@ -132,7 +138,7 @@ __device__ __forceinline__ void stage_update(const T k[4], const T yn[4],
             if constexpr (PREC == AXS_FP64)
                 y_out[c] = axpy_f64(yn[c], a, k[c]);
             else
-                y_out[c] = yn[c] + (float)a * k[c];
+                y_out[c] = fmaf((float)a, k[c], yn[c]);
         }
     }
     else if constexpr (MODE == MODE_S23)
@@ -146,8 +152,8 @@ __device__ __forceinline__ void stage_update(const T k[4], const T yn[4],
             }
             else
             {
-                acc_out[c] = ac[c] + 2.0f * k[c];
-                y_out[c] = yn[c] + (float)a * k[c];
+                acc_out[c] = fmaf(2.0f, k[c], ac[c]);
+                y_out[c] = fmaf((float)a, k[c], yn[c]);
             }
         }
     }
@@ -158,13 +164,13 @@ __device__ __forceinline__ void stage_update(const T k[4], const T yn[4],
             if constexpr (PREC == AXS_FP64)
                 y_out[c] = final_f64(yn[c], h6, ac[c], k[c]);
             else if constexpr (PREC == AXS_FP32)
-                y_out[c] = yn[c] + (float)h6 * (ac[c] + k[c]);
+                y_out[c] = fmaf((float)h6, ac[c] + k[c], yn[c]);
             else
             {
                 // Kahan: true y_n = hi + lo; add delta = (dt/6)(acc + k)
                 // with dt/6 carried as hi + lo floats.
                 const float sv = ac[c] + k[c];
-                const float delta = (float)h6 * sv + h6_lo * sv;
+                const float delta = fmaf((float)h6, sv, h6_lo * sv);
                 const float yk = delta + lo[c];
                 const float t = yn[c] + yk;
                 lo[c] = yk - (t - yn[c]);

@@ -81,6 +81,19 @@ AxionStringsLevel::AxionStringsLevel(const ParamTable &params,
                                m_bg.c_sched.has_switch, 0, 1.0};
     }
     m_tau = m_tau_i;
+    m_tau_offset = m_tau_i;
+    std::string ic_mode = "homogeneous";
+    as.queryAdd("ic_mode", ic_mode);
+    if (!m_flat && ic_mode == "fourier_relaxed")
+    {
+        // AxionStringsLevel.cpp:192-203
+        m_pre = Params::read_pre_evolution_background(params);
+        m_xi_target = Params::read_xi_target(params);
+        m_xi_cadence = Params::read_xi_check_cadence(params);
+        m_xi_check_interval = m_xi_cadence.coarse_interval;
+        m_handoff_n_periods = Params::read_handoff_transition_n_periods(params);
+        m_phase = Phase::Relaxing;
+    }
     as.queryAdd("compute_spectrum", m_compute_spectrum);
     std::string field = "a_dot";
     as.queryAdd("spectrum_field", field);
@@ -101,6 +114,8 @@ AxionStringsLevel::~AxionStringsLevel()
         std::fclose(m_scalars);
     if (m_spectrum)
         std::fclose(m_spectrum);
+    if (m_pre_scalars)
+        std::fclose(m_pre_scalars);
     axs_level_destroy(m_level);
     axs_ctx_destroy(m_ctx);
 }
@@ -110,12 +125,15 @@ void AxionStringsLevel::initData()
 {
     std::string ic_mode = "homogeneous";
     ParmParse(m_params, "axion_strings").queryAdd("ic_mode", ic_mode);
-    if (ic_mode == "fourier")
+    const bool relaxed = m_phase == Phase::Relaxing;
+    if (ic_mode == "fourier" || relaxed)
     {
-        const FourierICParams f =
-            Params::read_fourier_ic_params(m_params, "axion_strings");
+        const FourierICParams f = Params::read_fourier_ic_params(
+            m_params,
+            relaxed ? "axion_strings.pre_evolution" : "axion_strings");
         const double m_r =
-            m_flat ? m_flat_mr : std::sqrt(m_bg.lambda(m_tau_i));
+            relaxed ? std::sqrt(m_pre.lambda(0.0))
+                    : (m_flat ? m_flat_mr : std::sqrt(m_bg.lambda(m_tau_i)));
         const double k_max_cells = f.k_max_over_mr * m_r * m_dx;
         FourierIC::generate(m_level, 0, k_max_cells, f.mean_square_variance,
                             f.seed, 0);
@@ -195,37 +213,251 @@ bool AxionStringsLevel::output_due()
     return due;
 }
 
+// Steps left of k under evolution.max_steps and evolution.stop_time (the
+// latter on the level clock a_time, which counts through the relaxing
+// phase like the reference's curTime()).
+static long clamp_steps(long k, long step, long max_steps, double a_time,
+                        double stop_time, double dt)
+{
+    if (max_steps >= 0)
+        k = std::min(k, max_steps - step);
+    if (stop_time >= 0.0)
+    {
+        long left = 0;
+        double t = a_time;
+        while (t < stop_time - 1e-12 * dt && left < k)
+        {
+            t += dt;
+            ++left;
+        }
+        k = std::min(k, left);
+    }
+    return k;
+}
+
+// Main-phase stage scalars at conformal time tau: the Background's
+// (AxionStringsLevel.cpp:878-885), smoothly blended from the frozen
+// pre-evolution values during the post-handoff window (:889-907).
+void AxionStringsLevel::stage_coeffs_at(double tau, axs_rhs_coeffs *c) const
+{
+    axs_check(axs_background_rhs_coeffs(&m_cbg, tau, c),
+              "axs_background_rhs_coeffs");
+    if (m_handoff_dtau > 0.0)
+    {
+        const double t = (tau - m_handoff_tau_start) / m_handoff_dtau;
+        if (t < 1.0)
+        {
+            const double tc = std::clamp(t, 0.0, 1.0);
+            const double s = tc * tc * (3.0 - 2.0 * tc);
+            c->lambda = (1.0 - s) * m_handoff_lambda_start + s * c->lambda;
+            c->curvature_coeff = (1.0 - s) * m_handoff_curv_start +
+                                 s * c->curvature_coeff;
+        }
+    }
+}
+
 void AxionStringsLevel::evolve()
 {
-    while (true)
+    if (m_phase == Phase::Relaxing)
+        evolve_relaxing();
+    while (m_phase == Phase::Evolving)
     {
         long k = steps_until_next_output();
-        if (m_max_steps >= 0)
-            k = std::min(k, m_max_steps - m_step);
-        if (m_stop_time >= 0.0)
-        {
-            // stop once tau - tau_i reaches evolution.stop_time
-            long left = 0;
-            double t = m_tau;
-            while (t - m_tau_i < m_stop_time - 1e-12 * m_dt && left < k)
-            {
-                t += m_dt;
-                ++left;
-            }
-            k = std::min(k, left);
-        }
+        k = clamp_steps(k, m_step, m_max_steps, m_a_time, m_stop_time, m_dt);
         if (k <= 0)
             break;
-        axs_check(axs_rk4_advance(m_level, &m_cbg, m_tau, m_dt, (int)k,
-                                  m_use_graph ? 1 : 0),
-                  "axs_rk4_advance");
+        const bool blending =
+            m_handoff_dtau > 0.0 &&
+            m_tau < m_handoff_tau_start + m_handoff_dtau + m_dt;
+        if (blending)
+        {
+            std::vector<axs_rhs_coeffs> table((size_t)k * 4);
+            double tau = m_tau;
+            for (long j = 0; j < k; ++j)
+            {
+                const double ts[4] = {tau, tau + 0.5 * m_dt, tau + 0.5 * m_dt,
+                                      tau + m_dt};
+                for (int st = 0; st < 4; ++st)
+                    stage_coeffs_at(ts[st], &table[(size_t)j * 4 + st]);
+                tau += m_dt;
+            }
+            axs_check(axs_rk4_advance_coeffs(m_level, table.data(), m_dt,
+                                             (int)k, m_use_graph ? 1 : 0),
+                      "axs_rk4_advance_coeffs");
+        }
+        else
+            axs_check(axs_rk4_advance(m_level, &m_cbg, m_tau, m_dt, (int)k,
+                                      m_use_graph ? 1 : 0),
+                      "axs_rk4_advance");
         for (long j = 0; j < k; ++j)
+        {
             m_tau += m_dt;
+            m_a_time += m_dt;
+        }
         m_step += k;
         if (output_due())
             diagnostics();
     }
     axs_check(axs_ctx_synchronize(m_ctx), "synchronize");
+}
+
+// The relaxing phase: RK4 steps on the pre-evolution background with its
+// own clock tau_pre = a_time (AxionStringsLevel.cpp:869-876), a xi check
+// every m_xi_check_interval steps (:1044-1191).
+void AxionStringsLevel::evolve_relaxing()
+{
+    while (m_phase == Phase::Relaxing)
+    {
+        const long want = m_xi_check_interval;
+        const long k = clamp_steps(want, m_step, m_max_steps, m_a_time,
+                                   m_stop_time, m_dt);
+        if (k <= 0)
+            return;
+        std::vector<axs_rhs_coeffs> table((size_t)k * 4);
+        double t = m_a_time;
+        for (long j = 0; j < k; ++j)
+        {
+            const double ts[4] = {t, t + 0.5 * m_dt, t + 0.5 * m_dt, t + m_dt};
+            for (int st = 0; st < 4; ++st)
+            {
+                axs_rhs_coeffs &c = table[(size_t)j * 4 + st];
+                const double R = m_pre.R(ts[st]);
+                c.curvature_coeff = m_pre.curvature_term_coeff(ts[st]);
+                c.lambda = m_pre.lambda(ts[st]);
+                c.R_squared = R * R;
+            }
+            t += m_dt;
+        }
+        axs_check(axs_rk4_advance_coeffs(m_level, table.data(), m_dt, (int)k,
+                                         m_use_graph ? 1 : 0),
+                  "axs_rk4_advance_coeffs");
+        for (long j = 0; j < k; ++j)
+            m_a_time += m_dt;
+        m_step += k;
+        if (k < want)
+            return; // stopped by max_steps / stop_time
+        relax_check();
+    }
+}
+
+void AxionStringsLevel::relax_check()
+{
+    const PlaquetteCounts counts = count_plaquettes(m_level);
+    const double tau_pre = m_a_time;
+    // xi at the main run's start tau_i (AxionStringsLevel.cpp:1063-1071)
+    const double xi = xi_from_plaquette_count((double)counts.n_p_plain, m_dx,
+                                              m_L, m_bg.a_inv, m_tau_i);
+    const double ratio = xi / m_xi_target;
+    const double r_pre_now = m_pre.R(tau_pre);
+    const double lambda_pre_now = m_pre.lambda(tau_pre);
+    const TotalEnergyResult e = compute_composite_total_energy(
+        {EnergyLevelInput{m_level, m_dx, nullptr}}, r_pre_now, lambda_pre_now,
+        1.0, m_masking);
+    const double rad_total = e.rho_radial_kinetic_unscreened +
+                             e.rho_radial_gradient_unscreened +
+                             e.rho_radial_mass_unscreened;
+    const double axion_total =
+        e.rho_axion_kinetic_unscreened + e.rho_axion_gradient_unscreened;
+    std::cout << "  [AxionStrings pre-evolution RADIAL] tau_pre = " << tau_pre
+              << "  R_pre = " << r_pre_now
+              << "  m_r_pre = " << std::sqrt(lambda_pre_now)
+              << "  rho_radial_total = " << rad_total
+              << "  rho_axion_total = " << axion_total
+              << "  radial/axion = " << rad_total / axion_total
+              << "  rho_tot = " << e.rho_tot_unscreened << "\n";
+    if (!m_pre_scalars)
+    {
+        m_pre_scalars = std::fopen(
+            (m_out_dir + "/pre_evolution_scalars.dat").c_str(), "w");
+        if (!m_pre_scalars)
+            throw AxsError("cannot write pre_evolution_scalars.dat");
+        // column names of AxionStringsLevel.cpp:1122-1125 after the time
+        std::fprintf(m_pre_scalars,
+                     "# time tau_pre R_pre m_r_pre N_p xi xi_target "
+                     "rho_radial_kin rho_radial_grad rho_radial_mass "
+                     "rho_axion_kin rho_axion_grad rho_tot\n");
+    }
+    const double row[13] = {tau_pre,
+                            tau_pre,
+                            r_pre_now,
+                            std::sqrt(lambda_pre_now),
+                            (double)counts.n_p_plain,
+                            xi,
+                            m_xi_target,
+                            e.rho_radial_kinetic_unscreened,
+                            e.rho_radial_gradient_unscreened,
+                            e.rho_radial_mass_unscreened,
+                            e.rho_axion_kinetic_unscreened,
+                            e.rho_axion_gradient_unscreened,
+                            e.rho_tot_unscreened};
+    for (int i = 0; i < 13; ++i)
+        std::fprintf(m_pre_scalars, i ? " %.10e" : "%.10e", row[i]);
+    std::fprintf(m_pre_scalars, "\n");
+    std::fflush(m_pre_scalars);
+    const double n_p_required =
+        xi > 0.0 ? (double)counts.n_p_plain * (m_xi_target / xi) : INFINITY;
+    std::cout << "  [AxionStrings pre-evolution] tau_pre = " << tau_pre
+              << "  N_p = " << counts.n_p_plain << "  xi(at tau_i) = " << xi
+              << "  target = " << m_xi_target << "  ratio = " << ratio << "\n"
+              << "  [AxionStrings pre-evolution] RELAXATION PROGRESS: N_p / "
+                 "N_p_required = "
+              << counts.n_p_plain << " / " << n_p_required << " = " << ratio
+              << "\n";
+    if (ratio <= 1.0)
+    {
+        std::cout << "  [AxionStrings pre-evolution] target xi reached -- "
+                     "transitioning in place to the main evolution.\n";
+        apply_pre_evolution_to_main_rescale();
+        m_phase = Phase::Evolving;
+        return;
+    }
+    if (ratio <= m_xi_cadence.fine_threshold)
+        m_xi_check_interval = m_xi_cadence.fine_interval;
+    else if (ratio <= m_xi_cadence.medium_threshold)
+        m_xi_check_interval = m_xi_cadence.medium_interval;
+    else
+        m_xi_check_interval = m_xi_cadence.coarse_interval;
+}
+
+// AxionStringsLevel::apply_pre_evolution_to_main_rescale,
+// AxionStringsLevel.cpp:2027-2125
+void AxionStringsLevel::apply_pre_evolution_to_main_rescale()
+{
+    const double tau_pre_end = m_a_time;
+    const double R_pre = m_pre.R(tau_pre_end);
+    const double R_pre_prime = m_pre.R_prime(tau_pre_end);
+    const double R_main = m_bg.R(m_tau_i);
+    const double R_main_prime = m_bg.R_prime(m_tau_i);
+    const double kappa = R_main / R_pre;
+    const double common_term = R_main_prime - kappa * kappa * R_pre_prime;
+    std::cout << "  [AxionStrings] pre-evolution -> main handoff: "
+              << "tau_pre_end = " << tau_pre_end << ", R_pre = " << R_pre
+              << ", R_main(tau_i) = " << R_main << ", kappa = " << kappa
+              << "\n";
+    if (m_handoff_n_periods > 0.0)
+    {
+        const double lambda_start = m_pre.lambda(tau_pre_end);
+        const double curvature_start = m_pre.curvature_term_coeff(tau_pre_end);
+        const double lambda_main = m_bg.lambda(m_tau_i);
+        const double m_r_for_period = lambda_main > 0.0
+                                          ? std::sqrt(lambda_main)
+                                          : std::sqrt(lambda_start);
+        m_handoff_lambda_start = lambda_start;
+        m_handoff_curv_start = curvature_start;
+        m_handoff_tau_start = m_tau_i;
+        m_handoff_dtau = m_handoff_n_periods * 2.0 * M_PI / m_r_for_period;
+        std::cout << "  [AxionStrings] handoff transition: lambda "
+                  << lambda_start << " -> " << lambda_main
+                  << ", curvature_term_coeff " << curvature_start << " -> "
+                  << m_bg.curvature_term_coeff(m_tau_i) << ", over d(tau) = "
+                  << m_handoff_dtau << " (" << m_handoff_n_periods
+                  << " main-schedule core periods)\n";
+    }
+    // the level clock keeps counting; tau = offset + a_time = tau_i now
+    m_tau_offset = m_tau_i - tau_pre_end;
+    m_tau = m_tau_i;
+    axs_check(axs_level_handoff_rescale(m_level, kappa, common_term, R_pre),
+              "axs_level_handoff_rescale");
 }
 
 // specific_post_timestep's snapshot, AxionStringsLevel.cpp:1256-1764

@@ -3,9 +3,14 @@
 // output rows.
 //
 // Follows the reference's AxionStringsLevel (AxionStringsLevel.cpp):
-//   initData            :210-848  (ic_mode fourier :786-822, homogeneous
-//                                  :824-847)
-//   specific_eval_rhs   :850-921  (stage scalars; here one device table)
+//   variableSetUp       :136-208  (phase selection :192-207)
+//   initData            :210-848  (ic_mode fourier / fourier_relaxed
+//                                  :784-822, homogeneous :824-847)
+//   specific_eval_rhs   :850-921  (stage scalars of the relaxing phase,
+//                                  the main schedule and the post-handoff
+//                                  blend; here one device table)
+//   relaxing-phase xi check :1044-1191 (pre_evolution_scalars.dat)
+//   apply_pre_evolution_to_main_rescale :2027-2125
 //   specific_post_timestep :923-1914  (log(m_r/H) cadence :1198-1248,
 //                                  counts/xi/energy/velocity :1299-1361,
 //                                  network_scalars.dat :1536-1579,
@@ -38,7 +43,18 @@ class AxionStringsLevel
     long step() const { return m_step; }
     axs_level *level() { return m_level; }
 
+    enum class Phase
+    {
+        Relaxing,
+        Evolving
+    };
+    Phase phase() const { return m_phase; }
+
   private:
+    void evolve_relaxing();
+    void relax_check();
+    void apply_pre_evolution_to_main_rescale();
+    void stage_coeffs_at(double a_time, axs_rhs_coeffs *c) const;
     bool output_due();
     long steps_until_next_output();
     void write_network_scalars_row(const PlaquetteCounts &c, double xi,
@@ -71,6 +87,18 @@ class AxionStringsLevel
     int m_spectrum_kind{AXS_FIELD_MASKED_A_DOT};
     FILE *m_scalars{nullptr};
     FILE *m_spectrum{nullptr};
+    FILE *m_pre_scalars{nullptr};
+    // pre-evolution (ic_mode = fourier_relaxed)
+    Phase m_phase{Phase::Evolving};
+    PreEvolutionBackground m_pre;
+    double m_xi_target{};
+    XiCheckCadence m_xi_cadence;
+    long m_xi_check_interval{50};
+    double m_handoff_n_periods{0.0};
+    double m_a_time{0.0};     // the level clock: tau_pre, then on through
+    double m_tau_offset{0.0}; // tau = m_tau_offset + m_a_time (main phase)
+    double m_handoff_lambda_start{}, m_handoff_curv_start{};
+    double m_handoff_tau_start{}, m_handoff_dtau{0.0};
 };
 
 } // namespace axs_host

@@ -54,6 +54,17 @@ struct FourierICParams
     std::uint64_t seed{};
 };
 
+// The adaptive xi-check cadence of the relaxing phase
+// (AxionStringsParams.hpp:227-234).
+struct XiCheckCadence
+{
+    long coarse_interval{50};
+    long medium_interval{10};
+    long fine_interval{2};
+    double medium_threshold{1.5};
+    double fine_threshold{1.1};
+};
+
 struct OutputCadence
 {
     double first_log_mr_over_h{};
@@ -125,6 +136,57 @@ inline double read_tau_i(const ParamTable &t, const Background &b)
                  "mode from the very start) -- reach Moore only via a switch "
                  "(axion_strings.c1/tau_switch), not as the starting c0");
     return b.tau_from_log_mr_over_h(l);
+}
+
+// read_pre_evolution_background, AxionStringsParams.hpp:135-145
+inline PreEvolutionBackground read_pre_evolution_background(const ParamTable &t)
+{
+    ParmParse pp(t, "axion_strings.pre_evolution");
+    double gamma_pre = 1.0;
+    pp.get("gamma", gamma_pre);
+    if (gamma_pre <= 0.0)
+        pp.error("gamma", "must be > 0");
+    return PreEvolutionBackground(gamma_pre);
+}
+
+// read_xi_target, AxionStringsParams.hpp:152-162
+inline double read_xi_target(const ParamTable &t)
+{
+    ParmParse pp(t, "axion_strings.pre_evolution");
+    double xi_target{};
+    pp.get("xi_target", xi_target);
+    if (xi_target <= 0.0)
+        pp.error("xi_target", "must be > 0");
+    return xi_target;
+}
+
+// read_handoff_transition_n_periods, AxionStringsParams.hpp:185-195
+inline double read_handoff_transition_n_periods(const ParamTable &t)
+{
+    ParmParse pp(t, "axion_strings.pre_evolution");
+    double n_periods = 3.0;
+    pp.queryAdd("handoff_transition_n_periods", n_periods);
+    if (n_periods < 0.0)
+        pp.error("handoff_transition_n_periods", "must be >= 0");
+    return n_periods;
+}
+
+// read_xi_check_cadence, AxionStringsParams.hpp:240-261
+inline XiCheckCadence read_xi_check_cadence(const ParamTable &t)
+{
+    ParmParse pp(t, "axion_strings.pre_evolution");
+    XiCheckCadence c{};
+    pp.queryAdd("check_interval_coarse", c.coarse_interval);
+    pp.queryAdd("check_interval_medium", c.medium_interval);
+    pp.queryAdd("check_interval_fine", c.fine_interval);
+    pp.queryAdd("check_threshold_medium", c.medium_threshold);
+    pp.queryAdd("check_threshold_fine", c.fine_threshold);
+    if (c.coarse_interval < 1 || c.medium_interval < 1 || c.fine_interval < 1)
+        pp.error("check_interval_coarse", "all check intervals must be >= 1");
+    if (!(c.fine_threshold > 1.0 && c.fine_threshold < c.medium_threshold))
+        pp.error("check_threshold_medium",
+                 "must have 1 < check_threshold_fine < check_threshold_medium");
+    return c;
 }
 
 // read_fourier_ic_params, AxionStringsParams.hpp:207-225
@@ -224,12 +286,20 @@ inline void check_params(const ParamTable &t)
     read_tau_i(t, b);
     std::string ic_mode = "homogeneous";
     ParmParse(t, "axion_strings").queryAdd("ic_mode", ic_mode);
-    if (ic_mode == "fourier")
+    if (ic_mode == "fourier_relaxed")
+    {
+        read_pre_evolution_background(t);
+        read_xi_target(t);
+        read_fourier_ic_params(t, "axion_strings.pre_evolution");
+        read_xi_check_cadence(t);
+        read_handoff_transition_n_periods(t);
+    }
+    else if (ic_mode == "fourier")
         read_fourier_ic_params(t, "axion_strings");
     else if (ic_mode != "homogeneous")
         ParmParse(t, "axion_strings")
-            .error("ic_mode", "this build supports \"fourier\" and "
-                              "\"homogeneous\"");
+            .error("ic_mode", "this build supports \"fourier_relaxed\", "
+                              "\"fourier\" and \"homogeneous\"");
     read_output_cadence(t);
     bool compute_spectrum = false;
     ParmParse sp(t, "axion_strings");

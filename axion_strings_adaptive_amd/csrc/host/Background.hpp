@@ -6,8 +6,9 @@
 // AxionStrings/Background.hpp:21-134 (CTauSchedule :21-32, R :55-58,
 // R_prime :60-64, curvature_term_coeff :67-70, lambda :75-86,
 // H :104-107, H_over_mr_direct :112-116, H_over_mr_closed_form :92-96,
-// tau_from_log_mr_over_h :129-133) and FlatBackground.hpp:26-44, so that a
-// caller of the reference finds the same interface.  Evaluated on the host
+// tau_from_log_mr_over_h :129-133), FlatBackground.hpp:26-44 and
+// PreEvolutionBackground.hpp:45-85, so that a caller of the reference finds
+// the same interface.  Evaluated on the host
 // in FP64 only (never in a kernel).
 #pragma once
 
@@ -97,4 +98,34 @@ class FlatBackground
     [[nodiscard]] double R_prime(double) const { return 0.0; }
     [[nodiscard]] double curvature_term_coeff(double) const { return 0.0; }
     [[nodiscard]] double lambda(double) const { return m_r * m_r; }
+};
+
+// The relaxation background: in the pre-evolution conformal time tau_pre,
+// R = exp(tau_pre) (R0 = alpha = 1), a constant curvature coefficient 1 and
+// lambda = gamma_pre^2 / R^2, so that m_r/H = gamma_pre and R m_r are both
+// constant (PreEvolutionBackground.hpp:45-85).
+class PreEvolutionBackground
+{
+  public:
+    double gamma_pre{1.0};
+    PreEvolutionBackground() = default;
+    explicit PreEvolutionBackground(double a_gamma_pre) : gamma_pre(a_gamma_pre)
+    {
+    }
+    [[nodiscard]] double R(double tau_pre) const { return std::exp(tau_pre); }
+    [[nodiscard]] double R_prime(double tau_pre) const
+    {
+        return std::exp(tau_pre);
+    }
+    [[nodiscard]] double curvature_term_coeff(double) const { return 1.0; }
+    [[nodiscard]] double lambda(double tau_pre) const
+    {
+        const double r = R(tau_pre);
+        return gamma_pre * gamma_pre / (r * r);
+    }
+    [[nodiscard]] double m_r_over_H_direct(double tau_pre) const
+    {
+        const double hubble = R_prime(tau_pre) / (R(tau_pre) * R(tau_pre));
+        return std::sqrt(lambda(tau_pre)) / hubble;
+    }
 };

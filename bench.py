@@ -17,7 +17,8 @@ synchronisation on both sides, with all data resident in HBM.  Stage
 kernels are launched directly with HIP events recorded on the level's own
 stream between them, which gives the per-launch durations used for the
 roofline figure.  The CPU baseline is the FP64 OpenMP oracle timed on a
-bounded sample (a 256^3 level, 2 RK4 steps) of the same per-cell work.
+bounded sample (a 512^3 level, 5 RK4 steps, about 10-20 s on 16 host
+threads) of the same per-cell work.
 """
 
 from __future__ import annotations
@@ -84,7 +85,7 @@ def diagnostics(capi, lvl, bg, tau, n):
     return {"rho_tot_unscreened": e[0] / e[13], "n_p": plain, "xi": xi, "R": R}
 
 
-def cpu_baseline(n_sample=256, steps=2):
+def cpu_baseline(n_sample=512, steps=5):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import oracle_py as o
@@ -111,6 +112,10 @@ def main(argv=None):
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-n", type=int, default=512,
+                    help="CPU baseline sample: level size")
+    ap.add_argument("--cpu-steps", type=int, default=5,
+                    help="CPU baseline sample: RK4 steps")
     ap.add_argument("--tile", type=str, default="",
                     help="variant,z_chunk (default: autotuned cache or built-in)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -202,7 +207,7 @@ def main(argv=None):
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle, bounded sample)")
-        cpu = cpu_baseline()
+        cpu = cpu_baseline(args.cpu_n, args.cpu_steps)
 
     out = {
         "metric": METRIC, "value": value, "unit": "cell-updates/s",

@@ -142,6 +142,13 @@ int axs_level_device_ptrs(axs_level *lvl, void **state, void **comp);
 /* Homogeneous exact solution psi = (R, 0), Pi = (R', 0)
  * (AxionStringsLevel.cpp:824-847). */
 int axs_level_set_homogeneous(axs_level *lvl, double R, double R_prime);
+/* The pre-evolution -> main handoff rescale of the state in place
+ * (AxionStringsLevel::apply_pre_evolution_to_main_rescale,
+ * AxionStringsLevel.cpp:2027-2125): psi <- kappa psi and
+ * Pi <- kappa^2 Pi + (psi / R_pre) common_term, with kappa = R_main / R_pre
+ * and common_term = R_main' - kappa^2 R_pre' computed by the caller. */
+int axs_level_handoff_rescale(axs_level *lvl, double kappa,
+                              double common_term, double R_pre);
 /* Selects the stage-kernel tile variant (autotuner output); -1 = default. */
 int axs_level_set_tile(axs_level *lvl, int variant, int z_chunk);
 int axs_level_get_tile(axs_level *lvl, int *variant, int *z_chunk);
@@ -163,6 +170,15 @@ int axs_rk4_step(axs_level *lvl, const axs_rhs_coeffs coeffs[4], double dt);
  * captured hipGraph of one step (4 stages + step-counter advance). */
 int axs_rk4_advance(axs_level *lvl, const axs_background *bg, double tau,
                     double dt, int nsteps, int use_graph);
+
+/* nsteps RK4 steps with caller-supplied stage scalars: coeffs[4*i + s] are
+ * the scalars of stage s of step i.  Covers the stage-scalar selections of
+ * specific_eval_rhs (AxionStringsLevel.cpp:850-911) that are not a
+ * Background: the pre-evolution phase (PreEvolutionBackground.hpp, :869-876)
+ * and the post-handoff smoothstep blend of lambda and the curvature
+ * coefficient (:889-907).  Same launches as axs_rk4_advance. */
+int axs_rk4_advance_coeffs(axs_level *lvl, const axs_rhs_coeffs *coeffs,
+                           double dt, int nsteps, int use_graph);
 
 /* Per-stage kernel timing with HIP events recorded on the context stream
  * around each stage launch of axs_rk4_advance (forces direct launches while

@@ -121,6 +121,112 @@ void or_stage_coeffs(const or_background *b, double tau, double dt,
     }
 }
 
+/* ============ PreEvolutionBackground.hpp:45-85 ======================= */
+/* R0 = alpha = 1: R = exp(tau_pre) = R', constant curvature coefficient 1,
+ * lambda = gamma^2 / R^2. */
+double or_pre_R(double gamma_pre, double tau_pre)
+{
+    (void)gamma_pre;
+    return exp(tau_pre);
+}
+
+double or_pre_R_prime(double gamma_pre, double tau_pre)
+{
+    (void)gamma_pre;
+    return exp(tau_pre);
+}
+
+double or_pre_curvature_term_coeff(double gamma_pre, double tau_pre)
+{
+    (void)gamma_pre;
+    (void)tau_pre;
+    return 1.0;
+}
+
+double or_pre_lambda(double gamma_pre, double tau_pre)
+{
+    const double r = or_pre_R(gamma_pre, tau_pre);
+    return gamma_pre * gamma_pre / (r * r);
+}
+
+double or_pre_m_r_over_H_direct(double gamma_pre, double tau_pre)
+{
+    const double R = or_pre_R(gamma_pre, tau_pre);
+    const double hubble = or_pre_R_prime(gamma_pre, tau_pre) / (R * R);
+    return sqrt(or_pre_lambda(gamma_pre, tau_pre)) / hubble;
+}
+
+/* AxionStringsLevel.cpp:869-876: curvature, lambda and R^2 of the
+ * pre-evolution background at the stage time. */
+void or_pre_stage_coeffs(double gamma_pre, double tau_pre, double dt,
+                         double coeffs[12])
+{
+    const double t[4] = {tau_pre, tau_pre + 0.5 * dt, tau_pre + 0.5 * dt,
+                         tau_pre + dt};
+    for (int s = 0; s < 4; ++s)
+    {
+        const double R = or_pre_R(gamma_pre, t[s]);
+        coeffs[3 * s + 0] = or_pre_curvature_term_coeff(gamma_pre, t[s]);
+        coeffs[3 * s + 1] = or_pre_lambda(gamma_pre, t[s]);
+        coeffs[3 * s + 2] = R * R;
+    }
+}
+
+/* ============ AxionStringsLevel.cpp:2027-2125 (handoff) ================ */
+void or_handoff_constants(const or_background *b, double gamma_pre,
+                          double tau_pre_end, double tau_i, double out[4])
+{
+    const double R_pre = or_pre_R(gamma_pre, tau_pre_end);
+    const double R_pre_prime = or_pre_R_prime(gamma_pre, tau_pre_end);
+    const double R_main = or_bg_R(b, tau_i);
+    const double R_main_prime = or_bg_R_prime(b, tau_i);
+    const double kappa = R_main / R_pre;
+    out[0] = kappa;
+    out[1] = R_main_prime - kappa * kappa * R_pre_prime;
+    out[2] = R_pre;
+    out[3] = R_main;
+}
+
+void or_handoff_rescale(double *state, size_t ncell, double kappa,
+                        double common_term, double R_pre)
+{
+    double *p1 = state, *p2 = state + ncell, *q1 = state + 2 * ncell,
+           *q2 = state + 3 * ncell;
+    for (size_t i = 0; i < ncell; ++i)
+    {
+        const double psi1 = p1[i], psi2 = p2[i], Pi1 = q1[i], Pi2 = q2[i];
+        p1[i] = kappa * psi1;
+        p2[i] = kappa * psi2;
+        q1[i] = kappa * kappa * Pi1 + (psi1 / R_pre) * common_term;
+        q2[i] = kappa * kappa * Pi2 + (psi2 / R_pre) * common_term;
+    }
+}
+
+void or_handoff_blend(double tau, double tau_start, double dtau,
+                      double lambda_start, double curv_start, double *lambda,
+                      double *curv)
+{
+    if (!(dtau > 0.0))
+        return;
+    const double t = (tau - tau_start) / dtau;
+    if (t < 1.0)
+    {
+        const double tc = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+        const double s = tc * tc * (3.0 - 2.0 * tc);
+        *lambda = (1.0 - s) * lambda_start + s * *lambda;
+        *curv = (1.0 - s) * curv_start + s * *curv;
+    }
+}
+
+double or_handoff_dtau(double n_periods, double lambda_main_tau_i,
+                       double lambda_start)
+{
+    const double pi = 3.14159265358979323846;
+    const double m_r = lambda_main_tau_i > 0.0 ? sqrt(lambda_main_tau_i)
+                                               : sqrt(lambda_start);
+    return n_periods * 2.0 * pi / m_r;
+}
+
 /* ================= PlaquetteWinding.hpp:24-51 ======================= */
 double or_wrap_angle_diff(double delta)
 {

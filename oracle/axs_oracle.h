@@ -23,6 +23,7 @@
 #ifndef AXS_ORACLE_H_
 #define AXS_ORACLE_H_
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -91,6 +92,35 @@ void or_rk4_step(double *state, int nx, int ny, int nz, double dx,
 /* Fills coeffs[12] from the background at tau (stage times above). */
 void or_stage_coeffs(const or_background *b, double tau, double dt,
                      double coeffs[12]);
+
+/* ------ pre-evolution background (PreEvolutionBackground.hpp:45-85) --- */
+double or_pre_R(double gamma_pre, double tau_pre);
+double or_pre_R_prime(double gamma_pre, double tau_pre);
+double or_pre_curvature_term_coeff(double gamma_pre, double tau_pre);
+double or_pre_lambda(double gamma_pre, double tau_pre);
+double or_pre_m_r_over_H_direct(double gamma_pre, double tau_pre);
+/* Stage scalars of the relaxing phase (AxionStringsLevel.cpp:869-876) at
+ * the stage times tau_pre, +dt/2, +dt/2, +dt. */
+void or_pre_stage_coeffs(double gamma_pre, double tau_pre, double dt,
+                         double coeffs[12]);
+
+/* -- pre-evolution -> main handoff (AxionStringsLevel.cpp:2027-2125) --- */
+/* out = {kappa, common_term, R_pre, R_main} at the end of relaxation. */
+void or_handoff_constants(const or_background *b, double gamma_pre,
+                          double tau_pre_end, double tau_i, double out[4]);
+/* psi <- kappa psi; Pi <- kappa^2 Pi + (psi / R_pre) common_term. */
+void or_handoff_rescale(double *state, size_t ncell, double kappa,
+                        double common_term, double R_pre);
+/* Smoothstep blend of lambda and the curvature coefficient after the
+ * handoff (AxionStringsLevel.cpp:889-907); *lambda, *curv hold the main
+ * schedule's values on entry. */
+void or_handoff_blend(double tau, double tau_start, double dtau,
+                      double lambda_start, double curv_start, double *lambda,
+                      double *curv);
+/* Length of the blend window: n_periods main-schedule core periods
+ * 2 pi / sqrt(lambda_main(tau_i)) (AxionStringsLevel.cpp:2070-2081). */
+double or_handoff_dtau(double n_periods, double lambda_main_tau_i,
+                       double lambda_start);
 
 /* ----------- windings / counts (StringFinder.hpp:45-122) ------------ */
 void or_windings_at(const double *state, int nx, int ny, int nz, int i, int j,

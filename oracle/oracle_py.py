@@ -102,6 +102,16 @@ def load():
     lib.or_fft3d.restype = _I
     lib.or_fft3d.argtypes = [_PD, _I, _I, _I, _I]
     lib.or_num_threads.restype = _I
+    for name in ["or_pre_R", "or_pre_R_prime", "or_pre_curvature_term_coeff",
+                 "or_pre_lambda", "or_pre_m_r_over_H_direct"]:
+        getattr(lib, name).restype = _D
+        getattr(lib, name).argtypes = [_D, _D]
+    lib.or_pre_stage_coeffs.argtypes = [_D, _D, _D, _PD]
+    lib.or_handoff_constants.argtypes = [pb, _D, _D, _D, _PD]
+    lib.or_handoff_rescale.argtypes = [_PD, C.c_size_t, _D, _D, _D]
+    lib.or_handoff_blend.argtypes = [_D, _D, _D, _D, _D, _PD, _PD]
+    lib.or_handoff_dtau.restype = _D
+    lib.or_handoff_dtau.argtypes = [_D, _D, _D]
     _lib = lib
     return lib
 
@@ -132,6 +142,43 @@ def stage_coeffs(b, tau, dt):
     c = np.zeros(12)
     load().or_stage_coeffs(C.byref(b), tau, dt, _d(c))
     return c
+
+
+def pre_stage_coeffs(gamma_pre, tau_pre, dt):
+    c = np.zeros(12)
+    load().or_pre_stage_coeffs(gamma_pre, tau_pre, dt, _d(c))
+    return c
+
+
+def pre_bg(gamma_pre, which, tau_pre):
+    """which: R, R_prime, curvature_term_coeff, lambda, m_r_over_H_direct."""
+    return getattr(load(), "or_pre_" + which)(gamma_pre, tau_pre)
+
+
+def handoff_constants(b, gamma_pre, tau_pre_end, tau_i):
+    out = np.zeros(4)
+    load().or_handoff_constants(C.byref(b), gamma_pre, tau_pre_end, tau_i,
+                                _d(out))
+    return tuple(out)  # kappa, common_term, R_pre, R_main
+
+
+def handoff_rescale(state, kappa, common_term, R_pre):
+    """In place on a (4, nz, ny, nx) float64 state."""
+    assert state.dtype == np.float64 and state.flags.c_contiguous
+    load().or_handoff_rescale(_d(state), state[0].size, kappa, common_term,
+                              R_pre)
+
+
+def handoff_blend(tau, tau_start, dtau, lambda_start, curv_start, lam, curv):
+    lv = C.c_double(lam)
+    cv = C.c_double(curv)
+    load().or_handoff_blend(tau, tau_start, dtau, lambda_start, curv_start,
+                            C.byref(lv), C.byref(cv))
+    return lv.value, cv.value
+
+
+def handoff_dtau(n_periods, lambda_main_tau_i, lambda_start):
+    return load().or_handoff_dtau(n_periods, lambda_main_tau_i, lambda_start)
 
 
 def rk4_step(state, dx, coeffs12, dt):
@@ -259,6 +306,8 @@ def ref_lib():
                  ("ref_radial_mass", 4)]:
         getattr(lib, n).restype = _D
         getattr(lib, n).argtypes = [_D] * k
+    lib.ref_pre_background.restype = _D
+    lib.ref_pre_background.argtypes = [_D, _I, _D]
     lib.ref_gamma_sq_v_sq_from_phi.restype = _D
     lib.ref_gamma_sq_v_sq_from_phi.argtypes = [_D] * 5
     lib.ref_xi_from_comoving_length.restype = _D

@@ -6,13 +6,14 @@
 // (axs_oracle.c) and the product's host Background against the reference
 // code itself.  Only headers that compile with nothing but <cmath> are used
 // (Background.hpp, PlaquetteWinding.hpp, Masking.hpp, Energy.hpp,
-// XiFormula.hpp, Velocity.hpp); everything else in the reference needs the absent
+// XiFormula.hpp, Velocity.hpp, PreEvolutionBackground.hpp); everything else in the reference needs the absent
 // GRTeclyn/AMReX submodules and is not built.  The output library goes to
 // oracle/_ref/ and is never shipped in git.
 #include "Background.hpp"
 #include "Energy.hpp"
 #include "Masking.hpp"
 #include "PlaquetteWinding.hpp"
+#include "PreEvolutionBackground.hpp"
 #include "Velocity.hpp"
 #include "XiFormula.hpp"
 
@@ -32,6 +33,25 @@ Background make_bg(double a_inv, double c0, double c1, double tau_switch,
 
 extern "C"
 {
+    // which: 0 R, 1 R', 2 curvature, 3 lambda, 4 m_r/H direct
+    double ref_pre_background(double gamma_pre, int which, double tau_pre)
+    {
+        const PreEvolutionBackground b(gamma_pre);
+        switch (which)
+        {
+        case 0:
+            return b.R(tau_pre);
+        case 1:
+            return b.R_prime(tau_pre);
+        case 2:
+            return b.curvature_term_coeff(tau_pre);
+        case 3:
+            return b.lambda(tau_pre);
+        default:
+            return b.m_r_over_H_direct(tau_pre);
+        }
+    }
+
     // which: 0 R, 1 R', 2 curvature, 3 lambda, 4 H, 5 H/m_r direct,
     //        6 H/m_r closed form, 7 tau_from_log_mr_over_h (tau = log value)
     double ref_background(double a_inv, double c0, double c1,

@@ -24,7 +24,8 @@ def run(args, cwd=None):
 
 
 @pytest.mark.parametrize("name", ["params_plain_64.txt", "params_scale_512.txt",
-                                  "params_precision_1024.txt"])
+                                  "params_precision_1024.txt",
+                                  "params_relaxed_64.txt"])
 def test_shipped_params_validate(name):
     r = run([os.path.join(PARAMS, name), "--check"])
     assert r.returncode == 0, r.stderr
@@ -59,6 +60,42 @@ def test_parameter_errors_name_the_key(extra, key):
     assert key in r.stderr
 
 
+# AxionStringsParams.hpp:135-261 (the fourier_relaxed readers)
+BAD_RELAXED = [
+    (["axion_strings.pre_evolution.gamma=0"], "gamma"),
+    (["axion_strings.pre_evolution.xi_target=-1"], "xi_target"),
+    (["axion_strings.pre_evolution.check_interval_fine=0"],
+     "check_interval_coarse"),
+    (["axion_strings.pre_evolution.check_threshold_fine=2.0"],
+     "check_threshold_medium"),
+    (["axion_strings.pre_evolution.handoff_transition_n_periods=-1"],
+     "handoff_transition_n_periods"),
+    (["axion_strings.pre_evolution.k_max_over_mr=0"], "k_max_over_mr"),
+    (["axion_strings.ic_mode=plane"], "ic_mode"),
+]
+
+
+@pytest.mark.parametrize("extra,key", BAD_RELAXED)
+def test_relaxed_parameter_errors_name_the_key(extra, key):
+    r = run([os.path.join(PARAMS, "params_relaxed_64.txt"), "--check", *extra])
+    assert r.returncode == 3, (r.stdout, r.stderr)
+    assert key in r.stderr
+
+
+def test_relaxed_requires_xi_target():
+    p = os.path.join(PARAMS, "params_relaxed_64.txt")
+    with open(p) as f:
+        text = "".join(l for l in f if "xi_target" not in l)
+    import tempfile
+    with tempfile.NamedTemporaryFile("w", suffix=".txt", delete=False) as f:
+        f.write(text)
+    try:
+        r = run([f.name, "--check"])
+    finally:
+        os.unlink(f.name)
+    assert r.returncode == 3 and "xi_target" in r.stderr
+
+
 @pytest.mark.gpu
 def test_driver_end_to_end_matches_capi(tmp_path, ctx, oracle):
     from axion_strings_adaptive_amd import capi
@@ -89,3 +126,109 @@ def test_driver_end_to_end_matches_capi(tmp_path, ctx, oracle):
     assert rows[1, 3] == pytest.approx(xi, rel=1e-9)
     spec = np.loadtxt(tmp_path / "axion_spectrum.dat")
     assert spec.shape[1] == 15
+
+
+def _pre_table(capi, oracle, gamma, t, dt, nsteps):
+    rows = []
+    for _ in range(nsteps):
+        c = oracle.pre_stage_coeffs(gamma, t, dt)
+        rows += [capi.RhsCoeffs(*c[3 * k:3 * k + 3]) for k in range(4)]
+        t += dt
+    return rows, t
+
+
+@pytest.mark.gpu
+def test_driver_relaxed_ic_matches_capi(tmp_path, ctx, oracle):
+    """ic_mode = fourier_relaxed end to end (AxionStringsLevel.cpp:784-822,
+    :869-876, :1044-1191, :2027-2125): the driver's xi checks, handoff step,
+    and first main-phase row agree with the same run assembled from the C
+    ABI and the oracle's pre-evolution, handoff and blend functions."""
+    from axion_strings_adaptive_amd import capi
+    n, dx, dt, gamma, kmax = 32, 0.5, 0.1, 1.0, 6.0
+    L = n * dx
+    coarse, medium, fine = 10, 5, 2
+    thr_med, thr_fine = 1.5, 1.1
+
+    def fresh():
+        lvl = capi.Level(ctx, n, dx, capi.AXS_FP64)
+        lvl.fourier_ic(0, kmax, 0.5, 1, 0)
+        lvl.fourier_ic(1, kmax, 0.5, 1, 1)
+        lvl.zero_components(2, 2)
+        return lvl
+
+    def xi_of(n_p):  # tau_i = 1, a_inv = 2
+        return oracle.load().or_xi_from_plaquette_count(n_p, dx, L, 2.0, 1.0)
+
+    # N_p after every relaxing step (the checks do not change the state)
+    lvl = fresh()
+    t, n_p = 0.0, [None]
+    for _ in range(150):
+        rows, t = _pre_table(capi, oracle, gamma, t, dt, 1)
+        lvl.advance_coeffs(rows, dt)
+        n_p.append(lvl.count_plaquettes()[0])
+    lvl.close()
+    # a target the network reaches after a few coarse checks
+    checks = [s for s in range(coarse, 151, coarse)]
+    m = next(i for i in range(3, len(checks))
+             if n_p[checks[i]] < min(n_p[c] for c in checks[:i]))
+    xi_target = xi_of(n_p[checks[m]])
+    # the reference's adaptive cadence on the recorded counts
+    step, interval, check_steps = 0, coarse, []
+    while True:
+        step += interval
+        check_steps.append(step)
+        ratio = xi_of(n_p[step]) / xi_target
+        if ratio <= 1.0:
+            break
+        interval = fine if ratio <= thr_fine else (
+            medium if ratio <= thr_med else coarse)
+    h = step
+    first_row = (h // 20 + 1) * 20
+
+    r = run([os.path.join(PARAMS, "params_relaxed_64.txt"),
+             f"amr.n_cell={n} {n} {n}", f"geometry.prob_extent={L} {L} {L}",
+             f"axion_strings.pre_evolution.xi_target={xi_target!r}",
+             f"axion_strings.pre_evolution.check_interval_medium={medium}",
+             f"evolution.max_steps={first_row}", "--out", str(tmp_path)])
+    assert r.returncode == 0, r.stderr
+    pre = np.atleast_2d(np.loadtxt(tmp_path / "pre_evolution_scalars.dat"))
+    assert pre.shape == (len(check_steps), 13)
+    assert [int(v) for v in pre[:, 4]] == [n_p[s] for s in check_steps]
+    tp = [0.0]
+    for _ in range(h):
+        tp.append(tp[-1] + dt)
+    assert list(pre[:, 0]) == pytest.approx([tp[s] for s in check_steps],
+                                            rel=1e-10)
+    assert list(pre[:, 5]) == pytest.approx([xi_of(n_p[s]) for s in check_steps],
+                                            rel=1e-9)
+    assert "transitioning in place" in r.stdout
+
+    # the same run through the C ABI: relax h steps, hand off, blend
+    lvl = fresh()
+    rows, t = _pre_table(capi, oracle, gamma, 0.0, dt, h)
+    lvl.advance_coeffs(rows, dt)
+    bg = capi.make_background(2.0)
+    ob = oracle.bg(2.0)
+    kappa, common, R_pre, _ = oracle.handoff_constants(ob, gamma, t, 1.0)
+    lvl.handoff_rescale(kappa, common, R_pre)
+    lam_s = oracle.pre_bg(gamma, "lambda", t)
+    dtau = oracle.handoff_dtau(3.0, capi.background_eval(bg, 3, 1.0), lam_s)
+    rows, tau = [], 1.0
+    for _ in range(first_row - h):
+        for ts in (tau, tau + 0.5 * dt, tau + 0.5 * dt, tau + dt):
+            c = capi.rhs_coeffs(bg, ts)
+            lam, cur = oracle.handoff_blend(ts, 1.0, dtau, lam_s, 1.0,
+                                            c.lambda_, c.curvature_coeff)
+            rows.append(capi.RhsCoeffs(cur, lam, c.R_squared))
+        tau += dt
+    lvl.advance_coeffs(rows, dt)
+    net = np.atleast_2d(np.loadtxt(tmp_path / "network_scalars.dat"))
+    assert net.shape == (1, 25)
+    assert net[0, 0] == pytest.approx(tau, rel=1e-12)
+    plain, weighted = lvl.count_plaquettes()
+    assert net[0, 1] == plain and net[0, 2] == weighted
+    R = capi.background_eval(bg, 0, tau)
+    lam = capi.background_eval(bg, 3, tau)
+    e = lvl.energy_sums(R, lam, 1.0 / tau, capi.AXS_MASK_B, 0.8)
+    assert net[0, 6] == pytest.approx(e[0] / e[13], rel=1e-9)
+    lvl.close()

@@ -265,8 +265,10 @@ def test_homogeneous_exact_solution_gpu(ctx, oracle):
     """psi = (R, 0), Pi = (R', 0) with R = tau (radiation era, lambda = 1).
     FP64 reproduces it to round-off.  In FP32 the stage states are rounded
     to float, and lambda (|psi|^2 - R^2) psi of a rounded psi is a force of
-    order lambda R^2 ulp: FP32C stays within a few float ulps of R, never
-    worse than plain FP32."""
+    order lambda R^2 ulp; the radial mode is massive, so these errors
+    oscillate instead of accumulating: FP32C and FP32 both stay within a few
+    float ulps of R.  (Where compensation pays, in the accumulated phase of
+    a random field, is checked by test_fp32c_tracks_fp64.)"""
     capi = _capi()
     bg = capi.make_background(2.0)
     err = {}
@@ -282,7 +284,7 @@ def test_homogeneous_exact_solution_gpu(ctx, oracle):
     assert err[capi.AXS_FP64] < 1e-13
     ulp = np.spacing(np.float32(11.0)) / 11.0
     assert err[capi.AXS_FP32C] < 4 * ulp
-    assert err[capi.AXS_FP32C] <= err[capi.AXS_FP32]
+    assert err[capi.AXS_FP32] < 4 * ulp
 
 
 def test_errors_are_reported(ctx):

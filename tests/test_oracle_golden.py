@@ -285,6 +285,114 @@ def test_background_matches_reference_header(oracle, ref):
                                                        tau)
 
 
+# ------------------------------- pre-evolution background and the handoff
+PRE_FNS = ["R", "R_prime", "curvature_term_coeff", "lambda",
+           "m_r_over_H_direct"]
+
+
+def test_pre_evolution_background_kats(oracle):
+    """tests/test_pre_evolution_background.cpp:12-78: m_r/H = gamma_pre for
+    every tau_pre, R m_r constant, R' the derivative of R, R'' = c R."""
+    for g in (0.5, 1.0, 3.0):
+        for t in (-2.0, -0.3, 0.0, 0.7, 4.0):
+            assert oracle.pre_bg(g, "m_r_over_H_direct", t) == \
+                pytest.approx(g, rel=1e-9)
+    rm0 = oracle.pre_bg(2.0, "R", 0.0) * math.sqrt(oracle.pre_bg(2.0, "lambda", 0.0))
+    for t in (-1.5, 0.0, 0.6, 2.2):
+        rm = oracle.pre_bg(2.0, "R", t) * math.sqrt(oracle.pre_bg(2.0, "lambda", t))
+        assert rm == pytest.approx(rm0, rel=1e-9)
+    h = 1e-6
+    for t in (-1.0, 0.0, 1.3):
+        num = (oracle.pre_bg(1.7, "R", t + h) - oracle.pre_bg(1.7, "R", t - h)) / (2 * h)
+        assert oracle.pre_bg(1.7, "R_prime", t) == pytest.approx(num, rel=1e-6)
+    h = 1e-5
+    for t in (-1.0, 0.0, 1.3):
+        R2 = (oracle.pre_bg(1.0, "R", t + h) - 2 * oracle.pre_bg(1.0, "R", t) +
+              oracle.pre_bg(1.0, "R", t - h)) / (h * h)
+        assert R2 == pytest.approx(oracle.pre_bg(1.0, "curvature_term_coeff", t) *
+                                   oracle.pre_bg(1.0, "R", t), rel=1e-4)
+    # stage scalars: {curvature, lambda, R^2} at tau, +dt/2, +dt/2, +dt
+    c = oracle.pre_stage_coeffs(1.3, 0.4, 0.1)
+    for s, t in enumerate((0.4, 0.45, 0.45, 0.5)):
+        assert c[3 * s] == 1.0
+        assert c[3 * s + 1] == pytest.approx(1.69 * math.exp(-2 * t), rel=1e-15)
+        assert c[3 * s + 2] == pytest.approx(math.exp(2 * t), rel=1e-15)
+
+
+def test_pre_evolution_background_matches_reference_header(oracle, ref):
+    for g in (0.5, 1.0, 1.7, 3.0):
+        for t in (-2.0, -0.3, 0.0, 0.7, 4.0, 21.6):
+            for w, name in enumerate(PRE_FNS):
+                assert oracle.pre_bg(g, name, t) == ref.ref_pre_background(g, w, t)
+
+
+def test_pre_evolution_homogeneous_solution(oracle):
+    """psi = (R_pre, 0), Pi = (R_pre', 0) solves the relaxing-phase EOM
+    (R'' = 1 * R, |psi| = R): RK4 tracks R = exp(tau_pre) with 4th-order
+    error in dt."""
+    errs = []
+    for dt, nsteps in ((0.1, 20), (0.05, 40)):
+        s = np.zeros((4, 4, 4, 4))
+        s[0] = 1.0
+        s[2] = 1.0
+        t = 0.0
+        for _ in range(nsteps):
+            oracle.rk4_step(s, 0.5, oracle.pre_stage_coeffs(1.0, t, dt), dt)
+            t += dt
+        assert np.max(np.abs(s[1])) == 0.0
+        errs.append(abs(s[0].max() - math.exp(t)) / math.exp(t))
+    assert errs[0] < 2e-6
+    assert 12.0 < errs[0] / errs[1] < 20.0
+
+
+def test_handoff_rescale_maps_vacuum_to_vacuum(oracle):
+    """The handoff (AxionStringsLevel.cpp:2027-2125) sends the pre-evolution
+    vacuum psi = R_pre e^{i theta}, Pi = R_pre' e^{i theta} to the main
+    one, R_main e^{i theta}, R_main' e^{i theta}: kappa matches the moduli
+    and common_term the velocities."""
+    rng = np.random.default_rng(3)
+    for (a_inv, c0, g, tp, ti) in [(2.0, 0.0, 1.0, 3.7, 1.0),
+                                   (3.0, 1.0, 2.5, 0.4, 2.2),
+                                   (2.0, 0.0, 0.5, 12.0, 0.7)]:
+        b = oracle.bg(a_inv, c0=c0)
+        kappa, common, R_pre, R_main = oracle.handoff_constants(b, g, tp, ti)
+        L = oracle.load()
+        assert R_pre == math.exp(tp)
+        assert R_main == L.or_bg_R(b, ti)
+        th = rng.uniform(-PI, PI, (2, 2, 3))
+        s = np.stack([R_pre * np.cos(th), R_pre * np.sin(th),
+                      math.exp(tp) * np.cos(th), math.exp(tp) * np.sin(th)])
+        s = np.ascontiguousarray(s)
+        oracle.handoff_rescale(s, kappa, common, R_pre)
+        Rp = L.or_bg_R_prime(b, ti)
+        np.testing.assert_allclose(s[0], R_main * np.cos(th), rtol=1e-14, atol=1e-14 * R_main)
+        np.testing.assert_allclose(s[1], R_main * np.sin(th), rtol=1e-14, atol=1e-14 * R_main)
+        np.testing.assert_allclose(s[2], Rp * np.cos(th), rtol=1e-12, atol=1e-12 * abs(Rp))
+        np.testing.assert_allclose(s[3], Rp * np.sin(th), rtol=1e-12, atol=1e-12 * abs(Rp))
+
+
+def test_handoff_blend_smoothstep(oracle):
+    """AxionStringsLevel.cpp:889-907: the start values at the handoff, the
+    main schedule once t >= 1, the midpoint at t = 1/2, zero slope at both
+    ends; n_periods main-schedule core periods long (:2070-2081)."""
+    lam_s, cur_s, lam_m, cur_m = 0.3, 1.0, 2.0, -0.25
+    t0, d = 1.0, 4.0
+    assert oracle.handoff_blend(t0, t0, d, lam_s, cur_s, lam_m, cur_m) == (lam_s, cur_s)
+    assert oracle.handoff_blend(t0 - 1, t0, d, lam_s, cur_s, lam_m, cur_m) == (lam_s, cur_s)
+    assert oracle.handoff_blend(t0 + d, t0, d, lam_s, cur_s, lam_m, cur_m) == (lam_m, cur_m)
+    assert oracle.handoff_blend(t0 + 9, t0, d, lam_s, cur_s, lam_m, cur_m) == (lam_m, cur_m)
+    assert oracle.handoff_blend(t0 + 5, t0, 0.0, lam_s, cur_s, lam_m, cur_m) == (lam_m, cur_m)
+    mid = oracle.handoff_blend(t0 + d / 2, t0, d, lam_s, cur_s, lam_m, cur_m)
+    assert mid == pytest.approx(((lam_s + lam_m) / 2, (cur_s + cur_m) / 2), rel=1e-15)
+    h = 1e-4
+    for te in (t0, t0 + d):
+        a = oracle.handoff_blend(te - h, t0, d, lam_s, cur_s, lam_m, cur_m)[0]
+        b = oracle.handoff_blend(te + h, t0, d, lam_s, cur_s, lam_m, cur_m)[0]
+        assert abs(b - a) / (2 * h) < 1e-3
+    assert oracle.handoff_dtau(3.0, 4.0, 9.0) == pytest.approx(3 * PI, rel=1e-15)
+    assert oracle.handoff_dtau(3.0, 0.0, 9.0) == pytest.approx(2 * PI, rel=1e-15)
+
+
 # ------------------------------------------- stencil / RHS / RK4 closed forms
 def test_laplacian_symbol_on_plane_waves(oracle):
     """diff2's symbol on e^{ikx}: (-2cos2t + 32 cos t - 30)/(12 dx^2), t=k dx
