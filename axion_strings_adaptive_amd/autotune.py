@@ -1,8 +1,10 @@
 """Autotuner for the fused RK-stage kernel's launch shape.
 
-The stage kernel is compiled in several tile variants (64 x TY column tiles,
-TY in {4, 8, 16}) and takes the z-chunk (planes streamed per workgroup) at
-run time.  tune() times every (variant, z_chunk) candidate on a real level
+The stage kernel is compiled in several tile variants ((64 VX) x TY column
+tiles with TY in {4, 8, 16} and VX in {1, 2, 4} cells per lane; each with or
+without non-temporal output stores and with or without the next plane's
+centre loads issued one plane ahead) and takes the z-chunk (planes streamed
+per workgroup) at run time.  tune() times every (variant, z_chunk) candidate on a real level
 with HIP events on the level's stream (two RK4 steps per candidate after one
 untimed step) and returns the fastest; the result is cached per
 (grid size, precision) in tune_cache.json next to this file so later runs
@@ -21,7 +23,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE = os.path.join(HERE, "tune_cache.json")
 
-Z_CHUNKS = (8, 16, 32, 64, 128)
+Z_CHUNKS = (16, 32, 64)
 
 
 def _key(n: int, precision: int) -> str:

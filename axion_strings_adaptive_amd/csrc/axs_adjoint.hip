@@ -290,7 +290,7 @@ static void build_table(const axs_background *b, double lambda0, double tau,
 struct Workspace
 {
     double *d = nullptr;
-    ~Workspace() { hipFree(d); }
+    ~Workspace() { (void)hipFree(d); }
 };
 
 // One forward RK4 step with the table row `row` (explicit stage scalars).
@@ -416,7 +416,7 @@ int axs_adjoint_grad_energy(axs_level *L, const axs_background *bg,
     struct Free
     {
         char *p;
-        ~Free() { hipFree(p); }
+        ~Free() { (void)hipFree(p); }
     } guard{base};
     char *cp = base;
     char *segs = cp + (size_t)ncp * st_bytes;

@@ -1012,15 +1012,15 @@ int axs_boxlevel_create(axs_ctx *ctx, int nboxes, const int *lo, const int *hi,
         ok = hipMalloc(&b->comp, bytes4) == hipSuccess;
     if (!ok)
     {
-        hipFree(b->y);
-        hipFree(b->acc);
-        hipFree(yab);
-        hipFree(b->dbox);
-        hipFree(b->downer);
-        hipFree(b->ditems);
-        hipFree(b->dcoeff);
-        hipFree(b->derr);
-        hipFree(b->comp);
+        (void)hipFree(b->y);
+        (void)hipFree(b->acc);
+        (void)hipFree(yab);
+        (void)hipFree(b->dbox);
+        (void)hipFree(b->downer);
+        (void)hipFree(b->ditems);
+        (void)hipFree(b->dcoeff);
+        (void)hipFree(b->derr);
+        (void)hipFree(b->comp);
         delete b;
         return set_error(AXS_ERR_NOMEM, "axs_boxlevel_create: out of memory");
     }
@@ -1047,22 +1047,22 @@ int axs_boxlevel_destroy(axs_boxlevel *b)
 {
     if (!b)
         return AXS_OK;
-    hipStreamSynchronize(b->ctx->stream);
+    (void)hipStreamSynchronize(b->ctx->stream);
     for (auto &e : b->graphs)
     {
-        hipGraphExecDestroy(e.ge);
-        hipGraphDestroy(e.g);
+        (void)hipGraphExecDestroy(e.ge);
+        (void)hipGraphDestroy(e.g);
     }
-    hipFree(b->y);
-    hipFree(b->acc);
-    hipFree(b->ya);
-    hipFree(b->dbox);
-    hipFree(b->downer);
-    hipFree(b->ditems);
-    hipFree(b->dcoeff);
-    hipFree(b->derr);
-    hipFree(b->comp);
-    hipFree(b->snap);
+    (void)hipFree(b->y);
+    (void)hipFree(b->acc);
+    (void)hipFree(b->ya);
+    (void)hipFree(b->dbox);
+    (void)hipFree(b->downer);
+    (void)hipFree(b->ditems);
+    (void)hipFree(b->dcoeff);
+    (void)hipFree(b->derr);
+    (void)hipFree(b->comp);
+    (void)hipFree(b->snap);
     delete b;
     return AXS_OK;
 }
@@ -1191,11 +1191,11 @@ int axs_boxlevel_snapshot_coarse(axs_boxlevel *b, const axs_boxlevel *c)
         AXS_HIP_TRY(hipStreamSynchronize(b->ctx->stream));
         for (auto &e : b->graphs)
         {
-            hipGraphExecDestroy(e.ge);
-            hipGraphDestroy(e.g);
+            (void)hipGraphExecDestroy(e.ge);
+            (void)hipGraphDestroy(e.g);
         }
         b->graphs.clear();
-        hipFree(b->snap);
+        (void)hipFree(b->snap);
         b->snap = nullptr;
         AXS_HIP_TRY(hipMalloc(&b->snap, sb + lb));
         b->snap_bytes = sb + lb;
@@ -1308,7 +1308,7 @@ int axs_boxlevel_rk4_step_graph(axs_boxlevel *b, const axs_rhs_coeffs coeffs[4],
         if (rc != AXS_OK || e != hipSuccess)
         {
             if (g)
-                hipGraphDestroy(g);
+                (void)hipGraphDestroy(g);
             return rc != AXS_OK
                        ? rc
                        : set_error(AXS_ERR_HIP, std::string("hipStreamEndCapture: ") +
@@ -1317,7 +1317,7 @@ int axs_boxlevel_rk4_step_graph(axs_boxlevel *b, const axs_rhs_coeffs coeffs[4],
         hipGraphExec_t ge = nullptr;
         if (hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess)
         {
-            hipGraphDestroy(g);
+            (void)hipGraphDestroy(g);
             return set_error(AXS_ERR_HIP, "hipGraphInstantiate failed");
         }
         b->graphs.push_back({dt, t0, df, coarse, g, ge});

@@ -225,14 +225,14 @@ int axs_ctx_destroy(axs_ctx *c)
 {
     if (!c)
         return AXS_OK;
-    hipSetDevice(c->device);
-    hipStreamSynchronize(c->stream);
-    hipFree(c->d_counts);
-    hipFree(c->d_partials);
-    hipFree(c->d_result);
-    hipHostFree(c->h_result);
-    hipHostFree(c->h_counts);
-    hipStreamDestroy(c->stream);
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(c->d_counts);
+    (void)hipFree(c->d_partials);
+    (void)hipFree(c->d_result);
+    (void)hipHostFree(c->h_result);
+    (void)hipHostFree(c->h_counts);
+    (void)hipStreamDestroy(c->stream);
     delete c;
     return AXS_OK;
 }
@@ -337,12 +337,12 @@ int axs_level_create(axs_ctx *ctx, int nx, int ny, int nz, double dx,
          hipMalloc(&L->d_step, sizeof(int)) == hipSuccess;
     if (!ok)
     {
-        hipFree(L->y);
-        hipFree(L->acc);
-        hipFree(yab);
-        hipFree(L->comp);
-        hipFree(L->d_coeff);
-        hipFree(L->d_step);
+        (void)hipFree(L->y);
+        (void)hipFree(L->acc);
+        (void)hipFree(yab);
+        (void)hipFree(L->comp);
+        (void)hipFree(L->d_coeff);
+        (void)hipFree(L->d_step);
         delete L;
         return set_error(AXS_ERR_NOMEM,
                          "axs_level_create: device allocation failed");
@@ -360,8 +360,8 @@ static void destroy_graph(axs_level *L)
 {
     if (L->graph_valid)
     {
-        hipGraphExecDestroy(L->graph_exec);
-        hipGraphDestroy(L->graph);
+        (void)hipGraphExecDestroy(L->graph_exec);
+        (void)hipGraphDestroy(L->graph);
         L->graph_valid = 0;
     }
 }
@@ -370,22 +370,22 @@ int axs_level_destroy(axs_level *L)
 {
     if (!L)
         return AXS_OK;
-    hipSetDevice(L->ctx->device);
-    hipStreamSynchronize(L->ctx->stream);
+    (void)hipSetDevice(L->ctx->device);
+    (void)hipStreamSynchronize(L->ctx->stream);
     destroy_graph(L);
     if (L->timing)
         for (int i = 0; i < 5; ++i)
-            hipEventDestroy(L->ev[i]);
+            (void)hipEventDestroy(L->ev[i]);
     if (L->fft_r2c)
         hipfftDestroy((hipfftHandle)L->fft_r2c);
     if (L->fft_c2r)
         hipfftDestroy((hipfftHandle)L->fft_c2r);
-    hipFree(L->y);
-    hipFree(L->acc);
-    hipFree(L->ya); // yb is part of this allocation
-    hipFree(L->comp);
-    hipFree(L->d_coeff);
-    hipFree(L->d_step);
+    (void)hipFree(L->y);
+    (void)hipFree(L->acc);
+    (void)hipFree(L->ya); // yb is part of this allocation
+    (void)hipFree(L->comp);
+    (void)hipFree(L->d_coeff);
+    (void)hipFree(L->d_step);
     delete L;
     return AXS_OK;
 }
@@ -671,7 +671,7 @@ static int capture_graph(axs_level *L, double dt)
     if (rc != AXS_OK)
     {
         if (g)
-            hipGraphDestroy(g);
+            (void)hipGraphDestroy(g);
         return rc;
     }
     if (e != hipSuccess)
@@ -681,7 +681,7 @@ static int capture_graph(axs_level *L, double dt)
     e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
     if (e != hipSuccess)
     {
-        hipGraphDestroy(g);
+        (void)hipGraphDestroy(g);
         return set_error(AXS_ERR_HIP, std::string("hipGraphInstantiate: ") +
                                           hipGetErrorString(e));
     }
@@ -797,7 +797,7 @@ int axs_level_enable_timing(axs_level *L, int on)
     {
         AXS_HIP_TRY(hipStreamSynchronize(L->ctx->stream));
         for (int i = 0; i < 5; ++i)
-            hipEventDestroy(L->ev[i]);
+            (void)hipEventDestroy(L->ev[i]);
         L->timing = 0;
     }
     return AXS_OK;

@@ -37,7 +37,11 @@ __device__ __forceinline__ double wrap_angle_diff_d(double delta)
 {
     const double pi = 3.14159265358979323846;
     const double two_pi = 2.0 * pi;
-    delta = fmod(delta, two_pi);
+    // fmod(delta, 2 pi) returns delta itself when |delta| < 2 pi, always the
+    // case for differences of two atan2 results except the +-2 pi corner:
+    // only then is the (slow, iterative) fmod evaluated.
+    if (!(fabs(delta) < two_pi))
+        delta = fmod(delta, two_pi);
     if (delta <= -pi)
         delta += two_pi;
     else if (delta > pi)
@@ -127,9 +131,24 @@ __device__ __forceinline__ double gamma_sq_v_sq_d(double phi1, double phi2,
     return term1 - term2;
 }
 
-constexpr int WTY = 4;
+constexpr int WTY = 8;
+// what a windings pass produces besides the counts (compile-time, so the
+// plain count pass carries none of the other paths' registers)
+enum
+{
+    WM_VEL = 1,   // string velocities at pierced corners
+    WM_TAG = 2,   // tagger output
+    WM_TAGD = 4,  // tagger derivative criteria (Laplacian, radial gradient)
+    WM_FIELD = 8, // per-cell windings
+    WM_ALL = 15
+};
 
-template <typename T>
+// Each workgroup owns a 64 x WTY column tile and streams a chunk of z: the
+// phases atan2(psi2, psi1) of a plane (tile + one halo column / row) go
+// into a 3-deep LDS ring, each computed once per vertex; the raw psi of the
+// plane after next is loaded into registers one plane ahead, so its memory
+// latency overlaps the windings of the current plane.
+template <typename T, int WM>
 __global__ void __launch_bounds__(64 * WTY)
     k_windings(const T *y, const float *lo, WindParams p)
 {
@@ -153,27 +172,44 @@ __global__ void __launch_bounds__(64 * WTY)
     const T *p1 = y;
     const T *p2 = y + n;
     const float *l1 = lo, *l2 = lo ? lo + n : nullptr;
+    const bool hx = tx == 0, hy = ty == 0;
 
-    auto phase = [&](int xx, int yy, int zz) -> double {
+    // raw psi of the points this thread stages: own cell, x halo, y halo,
+    // corner
+    double r[4][2];
+    auto raw = [&](int xx, int yy, int zz, double(&v)[2]) {
         const uint32_t i = (uint32_t)zz * plane + (uint32_t)yy * nx + xx;
-        return atan2(ld(p2, l2, i), ld(p1, l1, i));
+        v[0] = ld(p1, l1, i);
+        v[1] = ld(p2, l2, i);
     };
-    // fills ring slot s with the phases of plane zz (tile + 1 halo each way)
-    auto fill = [&](int s, int zz) {
-        th[s][ty][tx] = phase(xw, yw, zz);
-        if (tx == 0)
-            th[s][ty][TX] = phase(xh, yw, zz);
-        if (ty == 0)
+    auto load_plane = [&](int zz) {
+        raw(xw, yw, zz, r[0]);
+        if (hx)
+            raw(xh, yw, zz, r[1]);
+        if (hy)
         {
-            th[s][WTY][tx] = phase(xw, yh, zz);
-            if (tx == 0)
-                th[s][WTY][TX] = phase(xh, yh, zz);
+            raw(xw, yh, zz, r[2]);
+            if (hx)
+                raw(xh, yh, zz, r[3]);
+        }
+    };
+    auto store_plane = [&](int s) {
+        th[s][ty][tx] = atan2(r[0][1], r[0][0]);
+        if (hx)
+            th[s][ty][TX] = atan2(r[1][1], r[1][0]);
+        if (hy)
+        {
+            th[s][WTY][tx] = atan2(r[2][1], r[2][0]);
+            if (hx)
+                th[s][WTY][TX] = atan2(r[3][1], r[3][0]);
         }
     };
 
     const int z_begin = bz * p.z_chunk;
     const int z_end = min(z_begin + p.z_chunk, nz);
-    fill(z_begin % 3, z_begin);
+    load_plane(z_begin);
+    store_plane(z_begin % 3);
+    load_plane(wrapc(z_begin + 1, nz));
 
     unsigned long long plain_w = 0; // per wavefront (uniform)
     unsigned long long weighted = 0; // per lane
@@ -198,7 +234,9 @@ __global__ void __launch_bounds__(64 * WTY)
     for (int z = z_begin; z < z_end; ++z)
     {
         const int s0 = z % 3, s1 = (z + 1) % 3;
-        fill(s1, wrapc(z + 1, nz));
+        store_plane(s1);
+        if (z + 1 < z_end)
+            load_plane(wrapc(z + 2, nz));
         __syncthreads();
         int wxy = 0, wyz = 0, wzx = 0;
         bool in = active;
@@ -218,99 +256,111 @@ __global__ void __launch_bounds__(64 * WTY)
             wyz = plaquette_winding_d(t000, t010, t011, t001);
             wzx = plaquette_winding_d(t000, t001, t101, t100);
             weighted += (unsigned long long)(abs(wxy) + abs(wyz) + abs(wzx));
-            if (p.do_vel && (wxy != 0 || wyz != 0 || wzx != 0))
+            if constexpr ((WM & WM_VEL) != 0)
             {
-                const double c000 = corner(gx, gy, z);
-                if (wxy != 0)
+                if (p.do_vel && (wxy != 0 || wyz != 0 || wzx != 0))
                 {
-                    vsum += c000 + corner(gx + 1, gy, z) +
-                            corner(gx + 1, gy + 1, z) + corner(gx, gy + 1, z);
-                    vcount += 4;
-                }
-                if (wyz != 0)
-                {
-                    vsum += c000 + corner(gx, gy + 1, z) +
-                            corner(gx, gy + 1, z + 1) + corner(gx, gy, z + 1);
-                    vcount += 4;
-                }
-                if (wzx != 0)
-                {
-                    vsum += c000 + corner(gx, gy, z + 1) +
-                            corner(gx + 1, gy, z + 1) + corner(gx + 1, gy, z);
-                    vcount += 4;
+                    const double c000 = corner(gx, gy, z);
+                    if (wxy != 0)
+                    {
+                        vsum += c000 + corner(gx + 1, gy, z) +
+                                corner(gx + 1, gy + 1, z) + corner(gx, gy + 1, z);
+                        vcount += 4;
+                    }
+                    if (wyz != 0)
+                    {
+                        vsum += c000 + corner(gx, gy + 1, z) +
+                                corner(gx, gy + 1, z + 1) + corner(gx, gy, z + 1);
+                        vcount += 4;
+                    }
+                    if (wzx != 0)
+                    {
+                        vsum += c000 + corner(gx, gy, z + 1) +
+                                corner(gx + 1, gy, z + 1) + corner(gx + 1, gy, z);
+                        vcount += 4;
+                    }
                 }
             }
-            if (p.wfield)
+            if constexpr ((WM & WM_FIELD) != 0)
             {
+                if (p.wfield)
+                {
                 p.wfield[3 * (size_t)id] = (int8_t)wxy;
                 p.wfield[3 * (size_t)id + 1] = (int8_t)wyz;
                 p.wfield[3 * (size_t)id + 2] = (int8_t)wzx;
+                }
             }
-            if (p.tags)
+            if constexpr ((WM & WM_TAG) != 0)
             {
+                if (p.tags)
+                {
                 bool tag = (wxy != 0 || wyz != 0 || wzx != 0);
                 const double psi1 = ld(p1, l1, id), psi2 = ld(p2, l2, id);
                 if (!tag && p.amp_thr > 0.0 &&
                     sqrt(psi1 * psi1 + psi2 * psi2) / p.R < p.amp_thr)
                     tag = true;
-                if (!tag && p.use_lap)
+                if constexpr ((WM & WM_TAGD) != 0)
                 {
-                    const double dx2 = p.dx * p.dx;
-                    double lap[2];
-                    for (int c = 0; c < 2; ++c)
+                    if (!tag && p.use_lap)
                     {
-                        const T *q = c ? p2 : p1;
-                        const float *ql = c ? l2 : l1;
-                        const double f0 = ld(q, ql, id);
-                        double r = 0.0;
-                        r += d2_ref(S.at(q, ql, gx - 2, gy, z),
-                                    S.at(q, ql, gx - 1, gy, z), f0,
-                                    S.at(q, ql, gx + 1, gy, z),
-                                    S.at(q, ql, gx + 2, gy, z), i2);
-                        r += d2_ref(S.at(q, ql, gx, gy - 2, z),
-                                    S.at(q, ql, gx, gy - 1, z), f0,
-                                    S.at(q, ql, gx, gy + 1, z),
-                                    S.at(q, ql, gx, gy + 2, z), i2);
-                        r += d2_ref(S.at(q, ql, gx, gy, z - 2),
-                                    S.at(q, ql, gx, gy, z - 1), f0,
-                                    S.at(q, ql, gx, gy, z + 1),
-                                    S.at(q, ql, gx, gy, z + 2), i2);
-                        lap[c] = r;
+                        const double dx2 = p.dx * p.dx;
+                        double lap[2];
+                        for (int c = 0; c < 2; ++c)
+                        {
+                            const T *q = c ? p2 : p1;
+                            const float *ql = c ? l2 : l1;
+                            const double f0 = ld(q, ql, id);
+                            double rr = 0.0;
+                            rr += d2_ref(S.at(q, ql, gx - 2, gy, z),
+                                         S.at(q, ql, gx - 1, gy, z), f0,
+                                         S.at(q, ql, gx + 1, gy, z),
+                                         S.at(q, ql, gx + 2, gy, z), i2);
+                            rr += d2_ref(S.at(q, ql, gx, gy - 2, z),
+                                         S.at(q, ql, gx, gy - 1, z), f0,
+                                         S.at(q, ql, gx, gy + 1, z),
+                                         S.at(q, ql, gx, gy + 2, z), i2);
+                            rr += d2_ref(S.at(q, ql, gx, gy, z - 2),
+                                         S.at(q, ql, gx, gy, z - 1), f0,
+                                         S.at(q, ql, gx, gy, z + 1),
+                                         S.at(q, ql, gx, gy, z + 2), i2);
+                            lap[c] = rr;
+                        }
+                        if (dx2 * fabs(lap[0]) > p.grad_thr ||
+                            dx2 * fabs(lap[1]) > p.grad_thr)
+                            tag = true;
                     }
-                    if (dx2 * fabs(lap[0]) > p.grad_thr ||
-                        dx2 * fabs(lap[1]) > p.grad_thr)
-                        tag = true;
-                }
-                if (!tag && p.use_rgrad)
-                {
-                    const double psq = psi1 * psi1 + psi2 * psi2;
-                    double g1sq = 0.0, g2sq = 0.0, gtsq = 0.0;
-                    for (int d = 0; d < 3; ++d)
+                    if (!tag && p.use_rgrad)
                     {
-                        const int ex = d == 0, ey = d == 1, ez = d == 2;
-                        const double a =
-                            d1_ref(S.at(p1, l1, gx - 2 * ex, gy - 2 * ey, z - 2 * ez),
-                                   S.at(p1, l1, gx - ex, gy - ey, z - ez),
-                                   S.at(p1, l1, gx + ex, gy + ey, z + ez),
-                                   S.at(p1, l1, gx + 2 * ex, gy + 2 * ey, z + 2 * ez),
-                                   i1);
-                        const double b =
-                            d1_ref(S.at(p2, l2, gx - 2 * ex, gy - 2 * ey, z - 2 * ez),
-                                   S.at(p2, l2, gx - ex, gy - ey, z - ez),
-                                   S.at(p2, l2, gx + ex, gy + ey, z + ez),
-                                   S.at(p2, l2, gx + 2 * ex, gy + 2 * ey, z + 2 * ez),
-                                   i1);
-                        g1sq += a * a;
-                        g2sq += b * b;
-                        const double dth =
-                            psq > 0.0 ? (psi1 * b - psi2 * a) / psq : 0.0;
-                        gtsq += dth * dth;
+                        const double psq = psi1 * psi1 + psi2 * psi2;
+                        double g1sq = 0.0, g2sq = 0.0, gtsq = 0.0;
+                        for (int d = 0; d < 3; ++d)
+                        {
+                            const int ex = d == 0, ey = d == 1, ez = d == 2;
+                            const double a = d1_ref(
+                                S.at(p1, l1, gx - 2 * ex, gy - 2 * ey, z - 2 * ez),
+                                S.at(p1, l1, gx - ex, gy - ey, z - ez),
+                                S.at(p1, l1, gx + ex, gy + ey, z + ez),
+                                S.at(p1, l1, gx + 2 * ex, gy + 2 * ey, z + 2 * ez),
+                                i1);
+                            const double b = d1_ref(
+                                S.at(p2, l2, gx - 2 * ex, gy - 2 * ey, z - 2 * ez),
+                                S.at(p2, l2, gx - ex, gy - ey, z - ez),
+                                S.at(p2, l2, gx + ex, gy + ey, z + ez),
+                                S.at(p2, l2, gx + 2 * ex, gy + 2 * ey, z + 2 * ez),
+                                i1);
+                            g1sq += a * a;
+                            g2sq += b * b;
+                            const double dth =
+                                psq > 0.0 ? (psi1 * b - psi2 * a) / psq : 0.0;
+                            gtsq += dth * dth;
+                        }
+                        if (p.dx * sqrt(g1sq + g2sq - psq * gtsq) > p.rgrad_thr)
+                            tag = true;
                     }
-                    if (p.dx * sqrt(g1sq + g2sq - psq * gtsq) > p.rgrad_thr)
-                        tag = true;
                 }
                 if (tag)
                     p.tags[id] = 1;
+                }
             }
         }
         // pierced plaquettes of this wavefront, by ballot / popcount
@@ -321,16 +371,17 @@ __global__ void __launch_bounds__(64 * WTY)
     for (int o = 32; o > 0; o >>= 1)
     {
         weighted += __shfl_xor(weighted, o, 64);
-        vcount += __shfl_xor(vcount, o, 64);
+        if constexpr ((WM & WM_VEL) != 0)
+            vcount += __shfl_xor(vcount, o, 64);
     }
     if ((threadIdx.x & 63) == 0)
     {
         atomicAdd(&p.counts[0], plain_w);
         atomicAdd(&p.counts[1], weighted);
-        if (p.do_vel)
+        if constexpr ((WM & WM_VEL) != 0)
             atomicAdd(&p.counts[2], vcount);
     }
-    if (p.do_vel)
+    if constexpr ((WM & WM_VEL) != 0)
     {
         // fixed-order block reduction of the velocity sum -> one partial
         __shared__ double red[64 * WTY];
@@ -345,6 +396,35 @@ __global__ void __launch_bounds__(64 * WTY)
         }
         if (t == 0)
             p.vel_partials[blockIdx.x] = red[0];
+    }
+}
+
+template <typename T, int WM>
+static void launch_windings_t(int ntiles, hipStream_t s, const T *y,
+                              const float *lo, const WindParams &p)
+{
+    hipLaunchKernelGGL((k_windings<T, WM>), dim3(ntiles), dim3(64, WTY), 0, s,
+                       y, lo, p);
+}
+
+template <typename T>
+static void launch_windings(int wm, int ntiles, hipStream_t s, const T *y,
+                            const float *lo, const WindParams &p)
+{
+    switch (wm)
+    {
+    case 0:
+        return launch_windings_t<T, 0>(ntiles, s, y, lo, p);
+    case WM_VEL:
+        return launch_windings_t<T, WM_VEL>(ntiles, s, y, lo, p);
+    case WM_FIELD:
+        return launch_windings_t<T, WM_FIELD>(ntiles, s, y, lo, p);
+    case WM_TAG:
+        return launch_windings_t<T, WM_TAG>(ntiles, s, y, lo, p);
+    case WM_TAG | WM_TAGD:
+        return launch_windings_t<T, WM_TAG | WM_TAGD>(ntiles, s, y, lo, p);
+    default:
+        return launch_windings_t<T, WM_ALL>(ntiles, s, y, lo, p);
     }
 }
 
@@ -364,6 +444,7 @@ struct EnergyParams
     double thr;
     const uint8_t *mask;
     double *partials; // [gridDim.x][14]
+    int z_chunk, tiles_x, tiles_y, tiles_z;
 };
 
 __device__ __forceinline__ double masking_weight_d(int scheme, double thr,
@@ -399,103 +480,212 @@ __device__ __forceinline__ double masked_a_dot_d(int scheme, double thr,
 }
 
 constexpr int EB = 256;
+constexpr int ETY = 8; // energy tile: 64 x ETY cells, one wavefront per row
 
+// compute_total_energy_sums (EnergyKernel.hpp:42-339) per cell, Energy.hpp
+// pointwise forms.  Each workgroup owns a 64 x ETY column tile and streams
+// a chunk of z planes: psi1/psi2 (FP64, FP32C as hi + lo) of the current
+// plane sit in LDS with a 2-cell x/y halo, the z neighbours in a 5-deep
+// register queue, so each psi value is read from HBM once per plane
+// instead of once per stencil use.  Per cell the 14 terms are formed
+// exactly as the direct form (same d1 operand order); only the order of the
+// (reproducible, fixed-order) sums differs.
 template <typename T>
-__global__ void __launch_bounds__(EB)
+__global__ void __launch_bounds__(64 * ETY)
     k_energy(const T *y, const float *lo, EnergyParams p)
 {
+    __shared__ double sp[2][ETY + 4][64 + 4];
+    __shared__ double red[64 * ETY];
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int bid = blockIdx.x;
+    const int bx = bid % p.tiles_x;
+    const int rem = bid / p.tiles_x;
+    const int by = rem % p.tiles_y;
+    const int bz = rem / p.tiles_y;
     const uint32_t n = p.n;
     const int nx = p.nx, ny = p.ny, nz = p.nz;
     const uint32_t plane = (uint32_t)nx * ny;
-    const StarLoader S{nx, ny, nz, plane};
-    const T *p1 = y, *p2 = y + n, *q1 = y + 2 * n, *q2 = y + 3 * n;
-    const float *l1 = lo, *l2 = lo ? lo + n : nullptr,
-                *m1 = lo ? lo + 2 * n : nullptr, *m2 = lo ? lo + 3 * n : nullptr;
+    const int gx = bx * 64 + tx, gy = by * ETY + ty;
+    const bool active = gx < nx && gy < ny;
+    const int xw = gx % nx, yw = gy % ny;
+    const uint32_t rowc = (uint32_t)yw * nx + xw;
+    const T *p1 = y, *p2 = y + n, *q1 = y + 2 * (size_t)n, *q2 = y + 3 * (size_t)n;
+    const float *l1 = lo, *l2 = lo ? lo + n : nullptr;
+    const float *m1 = lo ? lo + 2 * (size_t)n : nullptr;
+    const float *m2 = lo ? lo + 3 * (size_t)n : nullptr;
+    // halo roles: lanes 0,1 / 62,63 load the x halo, rows 0,1 / ETY-2,ETY-1
+    // the y halo (ETY >= 4, so no row holds both y-halo roles)
+    const bool hxl = tx < 2, hxr = tx >= 62;
+    const bool hyl = ty < 2, hyr = ty >= ETY - 2;
+    uint32_t rowhx = 0, rowhy = 0;
+    int sx = 0, sy = 0;
+    if (hxl || hxr)
+    {
+        const int hx = hxl ? bx * 64 - 2 + tx : bx * 64 + 64 + (tx - 62);
+        rowhx = (uint32_t)yw * nx + (uint32_t)(((hx % nx) + nx) % nx);
+        sx = hxl ? tx : 64 + 2 + (tx - 62);
+    }
+    if (hyl || hyr)
+    {
+        const int hy = hyl ? by * ETY - 2 + ty : by * ETY + ETY + (ty - (ETY - 2));
+        rowhy = (uint32_t)(((hy % ny) + ny) % ny) * nx + xw;
+        sy = hyl ? ty : ETY + 2 + (ty - (ETY - 2));
+    }
     const double i1 = 1.0 / p.dx;
     const double R = p.R, R2 = R * R, R4 = R2 * R2;
     const double f_a = 1.4142135623730951;
+    const int z_begin = bz * p.z_chunk;
+    const int z_end = min(z_begin + p.z_chunk, nz);
+    auto zoff = [&](int z) -> uint32_t {
+        return (uint32_t)wrapc(wrapc(z, nz), nz) * plane;
+    };
+    double a1[5], a2[5];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+    {
+        const uint32_t o = zoff(z_begin - 2 + i) + rowc;
+        a1[i + 1] = ld(p1, l1, o);
+        a2[i + 1] = ld(p2, l2, o);
+    }
     double acc[14];
 #pragma unroll
     for (int q = 0; q < 14; ++q)
         acc[q] = 0.0;
-    for (uint32_t id = blockIdx.x * EB + threadIdx.x; id < n;
-         id += gridDim.x * EB)
+    // halo psi and centre Pi of the next plane, loaded one plane ahead
+    double hxv[2] = {0.0, 0.0}, hyv[2] = {0.0, 0.0}, piv[2] = {0.0, 0.0};
+    auto load_next = [&](int z) {
+        const uint32_t oz = zoff(z);
+        if (hxl || hxr)
+        {
+            hxv[0] = ld(p1, l1, oz + rowhx);
+            hxv[1] = ld(p2, l2, oz + rowhx);
+        }
+        if (hyl || hyr)
+        {
+            hyv[0] = ld(p1, l1, oz + rowhy);
+            hyv[1] = ld(p2, l2, oz + rowhy);
+        }
+        if (active)
+        {
+            piv[0] = ld(q1, m1, oz + rowc);
+            piv[1] = ld(q2, m2, oz + rowc);
+        }
+    };
+    load_next(z_begin);
+    for (int z = z_begin; z < z_end; ++z)
     {
-        if (p.mask && p.mask[id] == 0)
-            continue;
-        const int x = id % nx;
-        const int yy = (id / nx) % ny;
-        const int z = id / plane;
-        const double psi1 = ld(p1, l1, id), psi2 = ld(p2, l2, id);
-        const double Pi1 = ld(q1, m1, id), Pi2 = ld(q2, m2, id);
-        const double psi_sq = psi1 * psi1 + psi2 * psi2;
-        double g1sq = 0.0, g2sq = 0.0, gtsq = 0.0;
 #pragma unroll
-        for (int d = 0; d < 3; ++d)
+        for (int i = 0; i < 4; ++i)
         {
-            const int ex = d == 0, ey = d == 1, ez = d == 2;
-            const double a =
-                d1_ref(S.at(p1, l1, x - 2 * ex, yy - 2 * ey, z - 2 * ez),
-                       S.at(p1, l1, x - ex, yy - ey, z - ez),
-                       S.at(p1, l1, x + ex, yy + ey, z + ez),
-                       S.at(p1, l1, x + 2 * ex, yy + 2 * ey, z + 2 * ez), i1);
-            const double b =
-                d1_ref(S.at(p2, l2, x - 2 * ex, yy - 2 * ey, z - 2 * ez),
-                       S.at(p2, l2, x - ex, yy - ey, z - ez),
-                       S.at(p2, l2, x + ex, yy + ey, z + ez),
-                       S.at(p2, l2, x + 2 * ex, yy + 2 * ey, z + 2 * ez), i1);
-            g1sq += a * a;
-            g2sq += b * b;
-            const double dth =
-                (psi_sq > 0.0) ? (psi1 * b - psi2 * a) / psi_sq : 0.0;
-            gtsq += dth * dth;
+            a1[i] = a1[i + 1];
+            a2[i] = a2[i + 1];
         }
-        // Energy.hpp:200-325
-        const double c1 = Pi1 - p.RpR * psi1, c2 = Pi2 - p.RpR * psi2;
-        const double dpot = psi1 * psi1 + psi2 * psi2 - R * R;
-        const double pot = 0.25 * p.lambda * dpot * dpot;
-        const double rho = (c1 * c1 + c2 * c2 + (g1sq + g2sq) + pot) / R4;
-        const double tp =
-            (psi_sq > 0.0) ? (psi1 * Pi2 - Pi1 * psi2) / psi_sq : 0.0;
-        const double a_dot = f_a * tp / R;
-        const double rak = 0.5 * a_dot * a_dot;
-        const double rag = 0.5 * f_a * f_a * gtsq / R2;
-        double rrk = 0.0;
-        if (psi_sq > 0.0)
+        const uint32_t oz = zoff(z);
         {
-            const double rd = (psi1 * c1 + psi2 * c2) / sqrt(psi_sq);
-            rrk = (rd * rd) / R4;
+            const uint32_t o = zoff(z + 2) + rowc;
+            a1[4] = ld(p1, l1, o);
+            a2[4] = ld(p2, l2, o);
         }
-        const double rrg = (g1sq + g2sq - psi_sq * gtsq) / R4;
-        const double rrm = pot / R4;
-        const double w = masking_weight_d(p.scheme, p.thr, psi1, psi2, R);
-        acc[0] += rho;
-        acc[1] += rho * w;
-        acc[2] += rak;
-        acc[3] += rak * w;
-        acc[4] += rag;
-        acc[5] += rag * w;
-        acc[6] += rrk;
-        acc[7] += rrk * w;
-        acc[8] += rrg;
-        acc[9] += rrg * w;
-        acc[10] += rrm;
-        acc[11] += rrm * w;
-        acc[12] += w;
-        acc[13] += 1.0;
+        sp[0][ty + 2][tx + 2] = a1[2];
+        sp[1][ty + 2][tx + 2] = a2[2];
+        if (hxl || hxr)
+        {
+            sp[0][ty + 2][sx] = hxv[0];
+            sp[1][ty + 2][sx] = hxv[1];
+        }
+        if (hyl || hyr)
+        {
+            sp[0][sy][tx + 2] = hyv[0];
+            sp[1][sy][tx + 2] = hyv[1];
+        }
+        const double Pi1 = piv[0], Pi2 = piv[1];
+        if (z + 1 < z_end)
+            load_next(z + 1);
+        __syncthreads();
+        const uint32_t id = oz + rowc;
+        if (active && !(p.mask && p.mask[id] == 0))
+        {
+            const double psi1 = a1[2], psi2 = a2[2];
+            const double psi_sq = psi1 * psi1 + psi2 * psi2;
+            double g1sq = 0.0, g2sq = 0.0, gtsq = 0.0;
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+            {
+                double a, b;
+                if (d == 0)
+                {
+                    const double *r1 = &sp[0][ty + 2][tx + 2];
+                    const double *r2 = &sp[1][ty + 2][tx + 2];
+                    a = d1_ref(r1[-2], r1[-1], r1[1], r1[2], i1);
+                    b = d1_ref(r2[-2], r2[-1], r2[1], r2[2], i1);
+                }
+                else if (d == 1)
+                {
+                    a = d1_ref(sp[0][ty][tx + 2], sp[0][ty + 1][tx + 2],
+                               sp[0][ty + 3][tx + 2], sp[0][ty + 4][tx + 2], i1);
+                    b = d1_ref(sp[1][ty][tx + 2], sp[1][ty + 1][tx + 2],
+                               sp[1][ty + 3][tx + 2], sp[1][ty + 4][tx + 2], i1);
+                }
+                else
+                {
+                    a = d1_ref(a1[0], a1[1], a1[3], a1[4], i1);
+                    b = d1_ref(a2[0], a2[1], a2[3], a2[4], i1);
+                }
+                g1sq += a * a;
+                g2sq += b * b;
+                const double dth =
+                    (psi_sq > 0.0) ? (psi1 * b - psi2 * a) / psi_sq : 0.0;
+                gtsq += dth * dth;
+            }
+            // Energy.hpp:200-325
+            const double c1 = Pi1 - p.RpR * psi1, c2 = Pi2 - p.RpR * psi2;
+            const double dpot = psi1 * psi1 + psi2 * psi2 - R * R;
+            const double pot = 0.25 * p.lambda * dpot * dpot;
+            const double rho = (c1 * c1 + c2 * c2 + (g1sq + g2sq) + pot) / R4;
+            const double tp =
+                (psi_sq > 0.0) ? (psi1 * Pi2 - Pi1 * psi2) / psi_sq : 0.0;
+            const double a_dot = f_a * tp / R;
+            const double rak = 0.5 * a_dot * a_dot;
+            const double rag = 0.5 * f_a * f_a * gtsq / R2;
+            double rrk = 0.0;
+            if (psi_sq > 0.0)
+            {
+                const double rd = (psi1 * c1 + psi2 * c2) / sqrt(psi_sq);
+                rrk = (rd * rd) / R4;
+            }
+            const double rrg = (g1sq + g2sq - psi_sq * gtsq) / R4;
+            const double rrm = pot / R4;
+            const double w = masking_weight_d(p.scheme, p.thr, psi1, psi2, R);
+            acc[0] += rho;
+            acc[1] += rho * w;
+            acc[2] += rak;
+            acc[3] += rak * w;
+            acc[4] += rag;
+            acc[5] += rag * w;
+            acc[6] += rrk;
+            acc[7] += rrk * w;
+            acc[8] += rrg;
+            acc[9] += rrg * w;
+            acc[10] += rrm;
+            acc[11] += rrm * w;
+            acc[12] += w;
+            acc[13] += 1.0;
+        }
+        __syncthreads();
     }
-    __shared__ double red[EB];
+    // fixed-order block reduction of each sum -> one row of partials
+    const int t = ty * 64 + tx;
     for (int q = 0; q < 14; ++q)
     {
-        red[threadIdx.x] = acc[q];
+        red[t] = acc[q];
         __syncthreads();
-        for (int o = EB / 2; o > 0; o >>= 1)
+        for (int o = 32 * ETY; o > 0; o >>= 1)
         {
-            if ((int)threadIdx.x < o)
-                red[threadIdx.x] += red[threadIdx.x + o];
+            if (t < o)
+                red[t] += red[t + o];
             __syncthreads();
         }
-        if (threadIdx.x == 0)
+        if (t == 0)
             p.partials[(size_t)blockIdx.x * 14 + q] = red[0];
         __syncthreads();
     }
@@ -607,12 +797,18 @@ static int run_windings(axs_level *L, const uint8_t *mask, int8_t *wfield,
         p.vel_partials = L->ctx->d_partials;
     }
     hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, s, p.counts);
+    const int wm = (p.do_vel ? WM_VEL : 0) | (tags ? WM_TAG : 0) |
+                   ((tags && (p.use_lap || p.use_rgrad)) ? WM_TAGD : 0) |
+                   (wfield ? WM_FIELD : 0);
     if (L->precision == AXS_FP64)
-        hipLaunchKernelGGL(k_windings<double>, dim3(ntiles), dim3(64, WTY), 0,
-                           s, (const double *)L->y, (const float *)nullptr, p);
+        launch_windings<double>(wm, ntiles, s, (const double *)L->y,
+                                (const float *)nullptr, p);
     else
-        hipLaunchKernelGGL(k_windings<float>, dim3(ntiles), dim3(64, WTY), 0, s,
-                           (const float *)L->y, (const float *)L->comp, p);
+        launch_windings<float>(wm, ntiles, s, (const float *)L->y,
+                               L->precision == AXS_FP32C
+                                   ? (const float *)L->comp
+                                   : (const float *)nullptr,
+                               p);
     AXS_HIP_TRY(hipGetLastError());
     if (vel)
     {
@@ -712,13 +908,31 @@ int axs_energy_sums(axs_level *L, double R, double lambda, double RpR,
     p.thr = m->threshold;
     p.mask = mask;
     p.partials = L->ctx->d_partials;
-    const int nb = nblocks(p.n, EB, 4096);
+    p.tiles_x = (p.nx + 63) / 64;
+    p.tiles_y = (p.ny + ETY - 1) / ETY;
+    // z chunk: enough tiles to fill the device, at most kMaxPartials rows
+    int zc = 8;
+    while ((long)p.tiles_x * p.tiles_y * ((p.nz + zc - 1) / zc) > kMaxPartials)
+        zc *= 2;
+    while (zc < 64 &&
+           (long)p.tiles_x * p.tiles_y * ((p.nz + 2 * zc - 1) / (2 * zc)) >= 8192)
+        zc *= 2;
+    if (zc > p.nz)
+        zc = p.nz;
+    p.z_chunk = zc;
+    p.tiles_z = (p.nz + zc - 1) / zc;
+    const long nbl = (long)p.tiles_x * p.tiles_y * p.tiles_z;
+    AXS_CHECK_ARG(nbl <= kMaxPartials, "axs_energy_sums: grid too large");
+    const int nb = (int)nbl;
     if (L->precision == AXS_FP64)
-        hipLaunchKernelGGL(k_energy<double>, dim3(nb), dim3(EB), 0, s,
+        hipLaunchKernelGGL(k_energy<double>, dim3(nb), dim3(64, ETY), 0, s,
                            (const double *)L->y, (const float *)nullptr, p);
     else
-        hipLaunchKernelGGL(k_energy<float>, dim3(nb), dim3(EB), 0, s,
-                           (const float *)L->y, (const float *)L->comp, p);
+        hipLaunchKernelGGL(k_energy<float>, dim3(nb), dim3(64, ETY), 0, s,
+                           (const float *)L->y,
+                           L->precision == AXS_FP32C ? (const float *)L->comp
+                                                     : (const float *)nullptr,
+                           p);
     AXS_HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_sum_partials, dim3(14), dim3(256), 0, s, p.partials,
                        nb, 14, L->ctx->d_result);

@@ -103,7 +103,9 @@ __device__ __forceinline__ void vstore(T *p, const T (&x)[VX], bool nt)
 }
 
 // Each workgroup: a (64 VX) x TY column tile, streaming a chunk of z.
-template <typename T, int PREC, int MODE, int TY, int VX>
+// PF: the centre loads (Pi, y_n, acc, lo) of plane z+1 are issued before
+// plane z is computed, so they are in flight for a whole plane.
+template <typename T, int PREC, int MODE, int TY, int VX, bool PF>
 __global__ void __launch_bounds__(64 * TY)
     k_stage(StageParams<T> p)
 {
@@ -202,6 +204,48 @@ __global__ void __launch_bounds__(64 * TY)
     const double cc = cf[0], lam = cf[1], R2 = cf[2];
     const T i2 = (T)p.inv_dx2;
 
+    // centre values of the current plane (and, with PF, of the next one)
+    T pi[2][VX], yn[4][VX], ac[4][VX];
+    float lo[4][VX];
+    T pin[2][VX], ynn[4][VX], acn[4][VX];
+    float lon[4][VX];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int v = 0; v < VX; ++v)
+        {
+            yn[c][v] = ynn[c][v] = 0;
+            ac[c][v] = acn[c][v] = 0;
+            lo[c][v] = lon[c][v] = 0.0f;
+            if (c < 2)
+                pi[c][v] = pin[c][v] = 0;
+        }
+    auto load_centre = [&](int zz, T(&pi_)[2][VX], T(&yn_)[4][VX],
+                           T(&ac_)[4][VX], float(&lo_)[4][VX]) {
+        if (!active)
+            return;
+        const uint32_t o = (uint32_t)zz * plane + rowc;
+        vload<T, VX>(p.yin + 2 * n + o, pi_[0]);
+        vload<T, VX>(p.yin + 3 * n + o, pi_[1]);
+        if constexpr (MODE == MODE_S23 || MODE == MODE_S4)
+        {
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+            {
+                vload<T, VX>(p.yn + c * n + o, yn_[c]);
+                vload<T, VX>(p.acc + c * n + o, ac_[c]);
+            }
+        }
+        if constexpr (MODE == MODE_S4 && PREC == AXS_FP32C)
+        {
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                vload<float, VX>(p.comp + c * n + o, lo_[c]);
+        }
+    };
+    if constexpr (PF)
+        load_centre(z_begin, pi, yn, ac, lo);
+
     for (int z = z_begin; z < z_end; ++z)
     {
         const int b = (z - z_begin) & 1;
@@ -288,39 +332,15 @@ __global__ void __launch_bounds__(64 * TY)
             }
         }
 
-        // ---- centre loads of this plane ----
+        // ---- centre loads: this plane, or (PF) the next one ----
         const uint32_t oc = (uint32_t)z * plane + rowc;
-        T pi[2][VX], yn[4][VX], ac[4][VX];
-        float lo[4][VX];
-#pragma unroll
-        for (int c = 0; c < 4; ++c)
-#pragma unroll
-            for (int v = 0; v < VX; ++v)
-            {
-                yn[c][v] = 0;
-                ac[c][v] = 0;
-                lo[c][v] = 0.0f;
-            }
-        if (active)
+        if constexpr (PF)
         {
-            vload<T, VX>(p.yin + 2 * n + oc, pi[0]);
-            vload<T, VX>(p.yin + 3 * n + oc, pi[1]);
-            if (MODE == MODE_S23 || MODE == MODE_S4)
-            {
-#pragma unroll
-                for (int c = 0; c < 4; ++c)
-                {
-                    vload<T, VX>(p.yn + c * n + oc, yn[c]);
-                    vload<T, VX>(p.acc + c * n + oc, ac[c]);
-                }
-            }
-            if (MODE == MODE_S4 && PREC == AXS_FP32C)
-            {
-#pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    vload<float, VX>(p.comp + c * n + oc, lo[c]);
-            }
+            if (z + 1 < z_end)
+                load_centre(z + 1, pin, ynn, acn, lon);
         }
+        else
+            load_centre(z, pi, yn, ac, lo);
 
         __syncthreads();
 
@@ -386,6 +406,21 @@ __global__ void __launch_bounds__(64 * TY)
             }
         }
 
+        if constexpr (PF)
+        {
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+                for (int v = 0; v < VX; ++v)
+                {
+                    yn[c][v] = ynn[c][v];
+                    ac[c][v] = acn[c][v];
+                    lo[c][v] = lon[c][v];
+                    if (c < 2)
+                        pi[c][v] = pin[c][v];
+                }
+        }
+
         // ---- advance the queue ----
 #pragma unroll
         for (int i = 0; i < 4; ++i)
@@ -411,8 +446,9 @@ __global__ void k_step_advance(int *step, int reset)
 }
 
 // ------------------------------------------------------------ launchers --
-// Tile variants: (TY, VX) pairs; variants >= kBaseVariants use
-// non-temporal stores for the stage outputs.
+// Tile variants: (TY, VX) pairs.  variant = base + kBaseVariants * g with
+// g = 0 plain, 1 non-temporal stores of the stage outputs, 2 centre-load
+// prefetch (PF), 3 both.
 struct TileShape
 {
     int ty, vx;
@@ -421,57 +457,62 @@ static const TileShape kTiles[] = {{4, 1}, {8, 1}, {16, 1}, {4, 2},
                                    {8, 2}, {4, 4}, {8, 4}};
 constexpr int kBaseVariants = 7;
 
-int num_tile_variants() { return 2 * kBaseVariants; }
+int num_tile_variants() { return 4 * kBaseVariants; }
 
 template <typename T, int PREC, int MODE, int TY, int VX>
-static int launch_t(const StageParams<T> &p, int ntiles, hipStream_t s)
+static int launch_t(const StageParams<T> &p, int ntiles, hipStream_t s,
+                    bool pf)
 {
-    hipLaunchKernelGGL((k_stage<T, PREC, MODE, TY, VX>), dim3(ntiles),
-                       dim3(64, TY), 0, s, p);
+    if (pf)
+        hipLaunchKernelGGL((k_stage<T, PREC, MODE, TY, VX, true>),
+                           dim3(ntiles), dim3(64, TY), 0, s, p);
+    else
+        hipLaunchKernelGGL((k_stage<T, PREC, MODE, TY, VX, false>),
+                           dim3(ntiles), dim3(64, TY), 0, s, p);
     return AXS_OK;
 }
 
 template <typename T, int PREC, int MODE>
 static int launch_ty(int base, const StageParams<T> &p, int ntiles,
-                     hipStream_t s)
+                     hipStream_t s, bool pf)
 {
     switch (base)
     {
     case 0:
-        return launch_t<T, PREC, MODE, 4, 1>(p, ntiles, s);
+        return launch_t<T, PREC, MODE, 4, 1>(p, ntiles, s, pf);
     case 1:
-        return launch_t<T, PREC, MODE, 8, 1>(p, ntiles, s);
+        return launch_t<T, PREC, MODE, 8, 1>(p, ntiles, s, pf);
     case 2:
-        return launch_t<T, PREC, MODE, 16, 1>(p, ntiles, s);
+        return launch_t<T, PREC, MODE, 16, 1>(p, ntiles, s, pf);
     case 3:
-        return launch_t<T, PREC, MODE, 4, 2>(p, ntiles, s);
+        return launch_t<T, PREC, MODE, 4, 2>(p, ntiles, s, pf);
     case 4:
-        return launch_t<T, PREC, MODE, 8, 2>(p, ntiles, s);
+        return launch_t<T, PREC, MODE, 8, 2>(p, ntiles, s, pf);
     default:
         // FP64 x4 tiles would exceed 64 KiB of static LDS
         if constexpr (sizeof(T) == 8)
-            return launch_t<T, PREC, MODE, 8, 2>(p, ntiles, s);
+            return launch_t<T, PREC, MODE, 8, 2>(p, ntiles, s, pf);
         else if (base == 5)
-            return launch_t<T, PREC, MODE, 4, 4>(p, ntiles, s);
+            return launch_t<T, PREC, MODE, 4, 4>(p, ntiles, s, pf);
         else
-            return launch_t<T, PREC, MODE, 8, 4>(p, ntiles, s);
+            return launch_t<T, PREC, MODE, 8, 4>(p, ntiles, s, pf);
     }
 }
 
 template <typename T, int PREC>
 static int launch_mode(int mode, int base, const StageParams<T> &p, int ntiles,
-                       hipStream_t s)
+                       hipStream_t s, bool pf)
 {
     switch (mode)
     {
     case MODE_RHS:
-        return launch_ty<T, PREC, MODE_RHS>(base, p, ntiles, s);
+        return launch_ty<T, PREC, MODE_RHS>(base, p, ntiles, s, pf);
     case MODE_S1:
-        return launch_ty<T, PREC, MODE_S1>(base, p, ntiles, s);
+        return launch_ty<T, PREC, MODE_S1>(base, p, ntiles, s, pf);
     case MODE_S23:
-        return launch_ty<T, PREC, MODE_S23>(base, p, ntiles, s);
+        return launch_ty<T, PREC, MODE_S23>(base, p, ntiles, s, pf);
     default:
-        return launch_ty<T, PREC, MODE_S4>(base, p, ntiles, s);
+        return launch_ty<T, PREC, MODE_S4>(base, p, ntiles, s, pf);
     }
 }
 
@@ -500,7 +541,9 @@ static int fill_and_launch(axs_level *L, int mode, const void *yin, void *yout,
     int variant = (L->tile_variant >= 0 && L->tile_variant < num_tile_variants())
                       ? L->tile_variant
                       : 1;
-    p.nontemporal = variant >= kBaseVariants ? 1 : 0;
+    const int group = variant / kBaseVariants;
+    p.nontemporal = (group & 1) ? 1 : 0;
+    const bool pf = (group & 2) != 0;
     int base = variant % kBaseVariants;
     if (sizeof(T) == 8 && kTiles[base].vx == 4)
         base = 4; // FP64 x4 tiles would exceed 64 KiB of static LDS
@@ -527,13 +570,13 @@ static int fill_and_launch(axs_level *L, int mode, const void *yin, void *yout,
     int rc;
     if constexpr (sizeof(T) == 8)
         rc = launch_mode<T, AXS_FP64>(mode, base, p, (int)ntiles,
-                                      L->ctx->stream);
+                                      L->ctx->stream, pf);
     else if (L->precision == AXS_FP32C)
         rc = launch_mode<T, AXS_FP32C>(mode, base, p, (int)ntiles,
-                                       L->ctx->stream);
+                                       L->ctx->stream, pf);
     else
         rc = launch_mode<T, AXS_FP32>(mode, base, p, (int)ntiles,
-                                      L->ctx->stream);
+                                      L->ctx->stream, pf);
     if (rc != AXS_OK)
         return rc;
     AXS_HIP_TRY(hipGetLastError());

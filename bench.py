@@ -85,6 +85,50 @@ def diagnostics(capi, lvl, bg, tau, n):
     return {"rho_tot_unscreened": e[0] / e[13], "n_p": plain, "xi": xi, "R": R}
 
 
+def precision_run(capi, ctx, n, bg, dt, steps, every, tile32=None, tile64=None):
+    """FP32C and FP64 levels side by side from the same IC for `steps` RK4
+    steps, with the precision metrics every `every` steps: max |dphi|/eta,
+    relative energy error and delta xi."""
+    levels = []
+    for prec, tile in ((capi.AXS_FP32C, tile32), (capi.AXS_FP64, tile64)):
+        lvl = capi.Level(ctx, n, 0.5, prec)
+        lvl.fourier_ic(0, n / 8.0, 0.5, 42, 0)
+        lvl.fourier_ic(1, n / 8.0, 0.5, 42, 1)
+        lvl.zero_components(2, 2)
+        if tile is not None:
+            lvl.set_tile(*tile)
+        levels.append(lvl)
+    a, b = levels
+    tau, done, series = 1.0, 0, []
+    while done < steps:
+        m = min(every, steps - done)
+        for lvl in levels:
+            lvl.advance(bg, tau, dt, m, use_graph=True)
+        for _ in range(m):
+            tau += dt
+        done += m
+        d32 = diagnostics(capi, a, bg, tau, n)
+        d64 = diagnostics(capi, b, bg, tau, n)
+        dpsi, _ = a.compare(b)
+        row = {"step": done, "tau": tau,
+               "max_dphi_over_eta": dpsi / d64["R"],
+               "rel_energy_err": abs(d32["rho_tot_unscreened"] -
+                                     d64["rho_tot_unscreened"]) /
+               abs(d64["rho_tot_unscreened"]),
+               "delta_xi": d32["xi"] - d64["xi"], "xi_fp64": d64["xi"],
+               "n_p_fp32c": d32["n_p"], "n_p_fp64": d64["n_p"]}
+        series.append(row)
+        log(f"[precision] step {done} tau {tau:.2f}: max|dphi|/eta "
+            f"{row['max_dphi_over_eta']:.3e} rel energy err "
+            f"{row['rel_energy_err']:.3e} dxi {row['delta_xi']:.3e}")
+    for lvl in levels:
+        lvl.close()
+    worst = {k: max(abs(r[k]) for r in series)
+             for k in ("max_dphi_over_eta", "rel_energy_err", "delta_xi")}
+    return {"workload": f"{n}^3, {steps} RK4 steps, dtau={dt}, metrics every "
+                        f"{every} steps", "worst": worst, "series": series}
+
+
 def cpu_baseline(n_sample=512, steps=5):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
@@ -118,9 +162,14 @@ def main(argv=None):
                     help="CPU baseline sample: RK4 steps")
     ap.add_argument("--tile", type=str, default="",
                     help="variant,z_chunk (default: autotuned cache or built-in)")
+    ap.add_argument("--precision-steps", type=int, default=0,
+                    help="only run the FP32C-vs-FP64 precision comparison "
+                         "for this many steps")
+    ap.add_argument("--precision-every", type=int, default=100)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run 3 device copies of this many MiB (counter "
-                         "calibration: read N + write N bytes per copy)")
+                         "calibration: read N + write N bytes per copy) and 3 "
+                         "sums over it (read N bytes)")
     args = ap.parse_args(argv)
     if args.copy_probe_mib:
         import torch
@@ -129,8 +178,10 @@ def main(argv=None):
         b = torch.empty_like(a)
         for _ in range(3):
             b.copy_(a)
+        for _ in range(3):  # read-only: a reduction over N bytes
+            t = a.sum()
         torch.cuda.synchronize()
-        print(json.dumps({"copy_probe_bytes": nbytes}))
+        print(json.dumps({"copy_probe_bytes": nbytes, "sum": float(t)}))
         return 0
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -157,6 +208,16 @@ def main(argv=None):
     else:
         from axion_strings_adaptive_amd import autotune
         tile = autotune.cached_choice(n, capi.AXS_FP32C)
+
+    if args.precision_steps:
+        from axion_strings_adaptive_amd import autotune
+        res = precision_run(capi, ctx, n, bg, dt, args.precision_steps,
+                            args.precision_every, tile,
+                            autotune.cached_choice(n, capi.AXS_FP64)
+                            if not args.tile else tile)
+        print(json.dumps({"precision_run": res}), flush=True)
+        ctx.close()
+        return 0
 
     log(f"[bench] FP32C {n}^3: IC + {args.warmup} warmup + {args.steps} timed steps")
     a, wall32, ms32, ns32, tau_end = run_level(

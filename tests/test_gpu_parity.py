@@ -30,7 +30,7 @@ def seeded(oracle, n, seed=42):
 
 @pytest.mark.parametrize("shape", [(32, 32, 32), (64, 64, 64), (40, 24, 36),
                                    (4, 8, 12), (128, 16, 8), (36, 8, 8)])
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 8, 12])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 8, 12, 16, 20, 26])
 def test_rhs_fp64_bitexact(ctx, oracle, shape, variant):
     capi = _capi()
     s = random_state(shape, seed=sum(shape))
@@ -92,7 +92,8 @@ def test_rk4_step_explicit_coeffs_moore(ctx, oracle):
 
 @pytest.mark.parametrize("variant,zc", [(0, 4), (1, 0), (2, 16), (1, 64),
                                         (3, 8), (4, 0), (5, 32), (6, 0),
-                                        (9, 0), (13, 16)])
+                                        (9, 0), (13, 16), (14, 8), (15, 0),
+                                        (18, 4), (20, 32), (25, 0), (27, 16)])
 @pytest.mark.parametrize("precision", [0, 1])
 def test_tile_variants_identical(ctx, oracle, variant, zc, precision):
     capi = _capi()
@@ -188,6 +189,23 @@ def test_energy_sums_parity(ctx, oracle, scheme, thr, precision):
     R, lam, RpR = tau, 1.0, 1.0 / tau
     got = lvl.energy_sums(R, lam, RpR, scheme, thr)
     want = oracle.energy_sums(s, 0.5, R, lam, RpR, scheme, thr)
+    np.testing.assert_allclose(got, want, rtol=1e-11, atol=1e-300)
+
+
+@pytest.mark.parametrize("shape", [(40, 24, 36), (100, 12, 20), (64, 4, 8),
+                                   (130, 9, 7)])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_energy_sums_ragged_shapes(ctx, oracle, shape, precision):
+    """Tiles that overhang the grid in x and y, z chunks that do not divide
+    nz: periodic neighbours and partial tiles match the direct form."""
+    capi = _capi()
+    s = random_state(shape, seed=sum(shape), amp=0.7)
+    s[0] += 1.0
+    lvl = capi.Level(ctx, shape, 0.5, precision)
+    lvl.upload(s)
+    s = lvl.download()
+    got = lvl.energy_sums(1.3, 0.8, 0.2, 2, 0.8)
+    want = oracle.energy_sums(s, 0.5, 1.3, 0.8, 0.2, 2, 0.8)
     np.testing.assert_allclose(got, want, rtol=1e-11, atol=1e-300)
 
 
@@ -295,7 +313,7 @@ def test_errors_are_reported(ctx):
     with pytest.raises(capi.AxsError, match="comp"):
         lvl.fourier_ic(3, 1.0, 0.5, 1, 0)
     with pytest.raises(capi.AxsError):
-        lvl.set_tile(7, 0)
+        lvl.set_tile(capi.load().axs_num_tile_variants(), 0)
 
 
 @pytest.mark.parametrize("precision", [0, 1])
