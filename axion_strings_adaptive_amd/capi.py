@@ -290,6 +290,12 @@ class Level:
     def set_homogeneous(self, R: float, Rp: float):
         check(self.lib.axs_level_set_homogeneous(self.h, R, Rp))
 
+    def device_ptrs(self):
+        """(state, comp) device addresses; comp is 0 unless FP32C."""
+        st, cp = C.c_void_p(), C.c_void_p()
+        check(self.lib.axs_level_device_ptrs(self.h, C.byref(st), C.byref(cp)))
+        return st.value or 0, cp.value or 0
+
     def handoff_rescale(self, kappa: float, common_term: float, R_pre: float):
         check(self.lib.axs_level_handoff_rescale(self.h, kappa, common_term,
                                                  R_pre))

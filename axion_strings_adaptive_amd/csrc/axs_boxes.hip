@@ -60,6 +60,7 @@ struct axs_boxlevel
     int64_t nvalid;
     void *y, *acc, *ya, *yb;
     float *comp;
+    void *alloc[4]; // allocations behind y, acc, ya|yb, comp
     axs::WorkItem *ditems;
     int nitems;
     double *dcoeff;
@@ -998,10 +999,11 @@ int axs_boxlevel_create(axs_ctx *ctx, int nboxes, const int *lo, const int *hi,
     b->nvalid = nvalid;
     b->nitems = (int)items.size();
     const size_t bytes4 = (size_t)4 * off * b->word;
-    void *yab = nullptr;
-    bool ok = hipMalloc(&b->y, bytes4) == hipSuccess &&
-              hipMalloc(&b->acc, bytes4) == hipSuccess &&
-              hipMalloc(&yab, 2 * bytes4) == hipSuccess &&
+    // staggered array bases, as for uniform levels (axs_level_create)
+    const size_t S = kArrayStagger, pad = 8 * S;
+    bool ok = hipMalloc(&b->alloc[0], bytes4 + pad) == hipSuccess &&
+              hipMalloc(&b->alloc[1], bytes4 + pad) == hipSuccess &&
+              hipMalloc(&b->alloc[2], 2 * (bytes4 + pad)) == hipSuccess &&
               hipMalloc(&b->dbox, nboxes * sizeof(BoxD)) == hipSuccess &&
               hipMalloc(&b->downer, owner.size() * sizeof(int)) == hipSuccess &&
               hipMalloc(&b->ditems, items.size() * sizeof(WorkItem)) ==
@@ -1009,23 +1011,25 @@ int axs_boxlevel_create(axs_ctx *ctx, int nboxes, const int *lo, const int *hi,
               hipMalloc(&b->dcoeff, 12 * sizeof(double)) == hipSuccess &&
               hipMalloc(&b->derr, 4 * sizeof(unsigned long long)) == hipSuccess;
     if (ok && precision == AXS_FP32C)
-        ok = hipMalloc(&b->comp, bytes4) == hipSuccess;
+        ok = hipMalloc(&b->alloc[3], bytes4 + pad) == hipSuccess;
     if (!ok)
     {
-        (void)hipFree(b->y);
-        (void)hipFree(b->acc);
-        (void)hipFree(yab);
+        for (int i = 0; i < 4; ++i)
+            (void)hipFree(b->alloc[i]);
         (void)hipFree(b->dbox);
         (void)hipFree(b->downer);
         (void)hipFree(b->ditems);
         (void)hipFree(b->dcoeff);
         (void)hipFree(b->derr);
-        (void)hipFree(b->comp);
         delete b;
         return set_error(AXS_ERR_NOMEM, "axs_boxlevel_create: out of memory");
     }
-    b->ya = yab;
-    b->yb = (char *)yab + bytes4;
+    b->y = b->alloc[0];
+    b->acc = (char *)b->alloc[1] + S;
+    b->ya = (char *)b->alloc[2] + 2 * S;
+    b->yb = (char *)b->alloc[2] + bytes4 + 11 * S;
+    if (b->alloc[3])
+        b->comp = (float *)((char *)b->alloc[3] + 4 * S);
     hipStream_t s = ctx->stream;
     AXS_HIP_TRY(hipMemcpyAsync(b->dbox, b->hbox.data(), nboxes * sizeof(BoxD),
                                hipMemcpyHostToDevice, s));
@@ -1053,15 +1057,13 @@ int axs_boxlevel_destroy(axs_boxlevel *b)
         (void)hipGraphExecDestroy(e.ge);
         (void)hipGraphDestroy(e.g);
     }
-    (void)hipFree(b->y);
-    (void)hipFree(b->acc);
-    (void)hipFree(b->ya);
+    for (int i = 0; i < 4; ++i)
+        (void)hipFree(b->alloc[i]);
     (void)hipFree(b->dbox);
     (void)hipFree(b->downer);
     (void)hipFree(b->ditems);
     (void)hipFree(b->dcoeff);
     (void)hipFree(b->derr);
-    (void)hipFree(b->comp);
     (void)hipFree(b->snap);
     delete b;
     return AXS_OK;

@@ -318,29 +318,40 @@ int axs_level_create(axs_ctx *ctx, int nx, int ny, int nz, double dx,
     L->tile_variant = -1;
     L->z_chunk = 0;
     const size_t bytes4 = (size_t)4 * L->g.ncell * L->word;
-    bool ok = hipMalloc(&L->y, bytes4) == hipSuccess &&
-              hipMalloc(&L->acc, bytes4) == hipSuccess;
-    // ya and yb are one allocation so that together they can hold a
-    // 4-component double array (FP32 transfers, FFT scratch).
-    void *yab = nullptr;
-    ok = ok && hipMalloc(&yab, 2 * bytes4) == hipSuccess;
+    // The arrays start at staggered offsets from their (2 MiB-aligned)
+    // allocations.  A stage kernel streams up to 22 component arrays at the
+    // same cell offset; with every array on the same alignment those
+    // streams start on the same HBM channels, and the 1024^3 stage kernels
+    // ran 13% slower than with these offsets (measured on MI355X: FP32C
+    // 65.7 -> 56.9 ms per RK4 step).  Multiples of 256 B keep every vector
+    // access aligned.
+    const size_t S = kArrayStagger;
+    const size_t pad = 8 * S;
+    bool ok = hipMalloc(&L->alloc[0], bytes4 + pad) == hipSuccess &&
+              hipMalloc(&L->alloc[1], bytes4 + pad) == hipSuccess;
+    // ya and yb share one allocation: [ya, ya + 2 bytes4) lies inside it,
+    // so together they can hold a 4-component double array (FP32
+    // transfers, FFT scratch).
+    ok = ok && hipMalloc(&L->alloc[2], 2 * (bytes4 + pad)) == hipSuccess;
+    if (ok && precision == AXS_FP32C)
+        ok = hipMalloc(&L->alloc[3], bytes4 + pad) == hipSuccess;
     if (ok)
     {
-        L->ya = yab;
-        L->yb = (char *)yab + bytes4;
+        L->y = L->alloc[0];
+        L->acc = (char *)L->alloc[1] + S;
+        L->ya = (char *)L->alloc[2] + 2 * S;
+        L->yb = (char *)L->alloc[2] + bytes4 + 11 * S;
+        if (L->alloc[3])
+            L->comp = (float *)((char *)L->alloc[3] + 4 * S);
     }
-    if (ok && precision == AXS_FP32C)
-        ok = hipMalloc(&L->comp, bytes4) == hipSuccess;
     ok = ok &&
          hipMalloc(&L->d_coeff, (size_t)kCoeffTableSteps * 12 * sizeof(double)) ==
              hipSuccess &&
          hipMalloc(&L->d_step, sizeof(int)) == hipSuccess;
     if (!ok)
     {
-        (void)hipFree(L->y);
-        (void)hipFree(L->acc);
-        (void)hipFree(yab);
-        (void)hipFree(L->comp);
+        for (int i = 0; i < 4; ++i)
+            (void)hipFree(L->alloc[i]);
         (void)hipFree(L->d_coeff);
         (void)hipFree(L->d_step);
         delete L;
@@ -380,10 +391,8 @@ int axs_level_destroy(axs_level *L)
         hipfftDestroy((hipfftHandle)L->fft_r2c);
     if (L->fft_c2r)
         hipfftDestroy((hipfftHandle)L->fft_c2r);
-    (void)hipFree(L->y);
-    (void)hipFree(L->acc);
-    (void)hipFree(L->ya); // yb is part of this allocation
-    (void)hipFree(L->comp);
+    for (int i = 0; i < 4; ++i)
+        (void)hipFree(L->alloc[i]);
     (void)hipFree(L->d_coeff);
     (void)hipFree(L->d_step);
     delete L;

@@ -37,6 +37,8 @@ int set_error(int code, const std::string &msg);
 constexpr int kCoeffTableSteps = 4096;
 // Maximum partial-sum slots (blocks) a reduction pass may produce.
 constexpr int kMaxPartials = 65536;
+// Offset step between the state arrays of a level (see axs_level_create).
+constexpr size_t kArrayStagger = (size_t(1) << 20) + (size_t(36) << 10) + 768;
 
 struct Grid
 {
@@ -70,6 +72,7 @@ struct axs_level
     void *acc;          // RK accumulator (4 comps)
     void *ya;           // stage state A (4 comps)
     void *yb;           // stage state B (4 comps)
+    void *alloc[4];     // allocations behind y, acc, ya|yb, comp
     double *d_coeff;    // [kCoeffTableSteps][4][3]
     int *d_step;        // device step counter (graph replays)
     int tile_variant;   // stage-kernel tile variant

@@ -71,6 +71,65 @@ def run_level(capi, ctx, n, precision, bg, dt, warmup, steps, tile=None):
     return lvl, (t1 - t0), ms, nsteps, tau_end
 
 
+def timing_check(capi, ctx, n, precision, bg, dt, steps, tile):
+    """The same K steps timed three ways on one level: graph replay and
+    direct launches without per-stage events (wall clock), and direct
+    launches with HIP events between the stages."""
+    lvl = capi.Level(ctx, n, 0.5, precision)
+    lvl.fourier_ic(0, n / 8.0, 0.5, 42, 0)
+    lvl.fourier_ic(1, n / 8.0, 0.5, 42, 1)
+    lvl.zero_components(2, 2)
+    if tile is not None:
+        lvl.set_tile(*tile)
+    tau = 1.0
+    lvl.advance(bg, tau, dt, 2, use_graph=True)
+    tau += 2 * dt
+    out = {}
+    for mode in ("graph", "direct", "events", "graph", "events"):
+        lvl.enable_timing(mode == "events")
+        lvl.stage_times()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        lvl.advance(bg, tau, dt, steps, use_graph=(mode == "graph"))
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            tau += dt
+        rec = {"ms_per_step": 1e3 * (t1 - t0) / steps}
+        if mode == "events":
+            ms, ns = lvl.stage_times()
+            rec["stage_ms"] = [m / ns for m in ms]
+        out.setdefault(mode, []).append(rec)
+        log(f"[timing] {mode}: {rec}")
+    lvl.enable_timing(False)
+    lvl.close()
+    return out
+
+
+def alloc_check(capi, ctx, n, precision, bg, dt, steps, tile, rounds=4):
+    """K graph-replayed steps on `rounds` successive allocations of the
+    same level (each freed before the next): placement sensitivity."""
+    out = []
+    for r in range(rounds):
+        lvl = capi.Level(ctx, n, 0.5, precision)
+        lvl.fourier_ic(0, n / 8.0, 0.5, 42, 0)
+        lvl.fourier_ic(1, n / 8.0, 0.5, 42, 1)
+        lvl.zero_components(2, 2)
+        if tile is not None:
+            lvl.set_tile(*tile)
+        lvl.advance(bg, 1.0, dt, 2, use_graph=True)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        lvl.advance(bg, 1.2, dt, steps, use_graph=True)
+        ctx.synchronize()
+        ms = 1e3 * (time.perf_counter() - t0) / steps
+        ptrs = lvl.device_ptrs() if hasattr(lvl, "device_ptrs") else None
+        out.append({"round": r, "ms_per_step": ms, "ptrs": ptrs})
+        log(f"[alloc] round {r}: {ms:.2f} ms/step ptrs {ptrs}")
+        lvl.close()
+    return out
+
+
 def diagnostics(capi, lvl, bg, tau, n):
     R = capi.background_eval(bg, 0, tau)
     lam = capi.background_eval(bg, 3, tau)
@@ -162,6 +221,12 @@ def main(argv=None):
                     help="CPU baseline sample: RK4 steps")
     ap.add_argument("--tile", type=str, default="",
                     help="variant,z_chunk (default: autotuned cache or built-in)")
+    ap.add_argument("--alloc-check", action="store_true",
+                    help="only time the FP32C level on successive "
+                         "allocations")
+    ap.add_argument("--timing-check", action="store_true",
+                    help="only compare timing modes (graph / direct / "
+                         "per-stage events) on the FP32C level")
     ap.add_argument("--precision-steps", type=int, default=0,
                     help="only run the FP32C-vs-FP64 precision comparison "
                          "for this many steps")
@@ -209,6 +274,18 @@ def main(argv=None):
         from axion_strings_adaptive_amd import autotune
         tile = autotune.cached_choice(n, capi.AXS_FP32C)
 
+    if args.alloc_check:
+        res = alloc_check(capi, ctx, n, capi.AXS_FP32C, bg, dt, args.steps,
+                          tile)
+        print(json.dumps({"alloc_check": res}), flush=True)
+        ctx.close()
+        return 0
+    if args.timing_check:
+        res = timing_check(capi, ctx, n, capi.AXS_FP32C, bg, dt, args.steps,
+                           tile)
+        print(json.dumps({"timing_check": res}), flush=True)
+        ctx.close()
+        return 0
     if args.precision_steps:
         from axion_strings_adaptive_amd import autotune
         res = precision_run(capi, ctx, n, bg, dt, args.precision_steps,
