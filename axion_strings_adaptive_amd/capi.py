@@ -82,6 +82,7 @@ _SIGS = {
     "axs_rk4_step": (_I, [_P, C.POINTER(RhsCoeffs), _D]),
     "axs_rk4_advance": (_I, [_P, C.POINTER(Background), _D, _D, _I, _I]),
     "axs_rk4_advance_coeffs": (_I, [_P, C.POINTER(RhsCoeffs), _D, _I, _I]),
+    "axs_level_place": (_I, [_P, _I, C.POINTER(_D), C.POINTER(_I)]),
     "axs_level_enable_timing": (_I, [_P, _I]),
     "axs_level_stage_times": (_I, [_P, _PD, _PI64]),
     "axs_count_plaquettes": (_I, [_P, _P, _PI64]),
@@ -289,6 +290,14 @@ class Level:
 
     def set_homogeneous(self, R: float, Rp: float):
         check(self.lib.axs_level_set_homogeneous(self.h, R, Rp))
+
+    def place(self, trials: int = 4):
+        """Placement trials (state reset to zero): list of ms per step of
+        each placement tried; the level keeps the last one."""
+        ms = (_D * trials)()
+        used = C.c_int()
+        check(self.lib.axs_level_place(self.h, trials, ms, C.byref(used)))
+        return [ms[i] for i in range(used.value)]
 
     def device_ptrs(self):
         """(state, comp) device addresses; comp is 0 unless FP32C."""

@@ -40,20 +40,27 @@ HBM_PEAK_GBS = 8000.0
 #   FP32C (S = 16 B): stage1 3S, stage2 5S, stage3 5S, stage4 6S (y, lo rd/wr)
 #   FP64  (S = 32 B): stage1 3S, stage2 5S, stage3 5S, stage4 4S
 STAGE_BYTES = {1: [48, 80, 80, 96], 0: [96, 160, 160, 128]}
+# HBM bytes per cell of one FP32C stage-2/3 launch at 1024^3, from
+# rocprofv3 --pmc FETCH_SIZE (x 2.0) and WRITE_SIZE (x 1.0), the scales
+# measured on device copies and reads of known size; keyed by the tile
+# (variant, z_chunk) the launch used.
+MEASURED_S23_BYTES = {(20, 32): 86.03}
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_level(capi, ctx, n, precision, bg, dt, warmup, steps, tile=None):
+def run_level(capi, ctx, n, precision, bg, dt, warmup, steps, tile=None,
+              place_trials=0):
     lvl = capi.Level(ctx, n, 0.5, precision)
+    if tile is not None:
+        lvl.set_tile(*tile)
+    placement = lvl.place(place_trials) if place_trials > 0 else []
     kmax = n / 8.0
     lvl.fourier_ic(0, kmax, 0.5, 42, 0)
     lvl.fourier_ic(1, kmax, 0.5, 42, 1)
     lvl.zero_components(2, 2)
-    if tile is not None:
-        lvl.set_tile(*tile)
     tau = 1.0
     lvl.advance(bg, tau, dt, warmup, use_graph=True)
     tau += warmup * dt
@@ -68,7 +75,7 @@ def run_level(capi, ctx, n, precision, bg, dt, warmup, steps, tile=None):
     ms, nsteps = lvl.stage_times()
     lvl.enable_timing(False)
     tau_end = 1.0 + (warmup + steps) * dt
-    return lvl, (t1 - t0), ms, nsteps, tau_end
+    return lvl, (t1 - t0), ms, nsteps, tau_end, placement
 
 
 def timing_check(capi, ctx, n, precision, bg, dt, steps, tile):
@@ -215,6 +222,9 @@ def main(argv=None):
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--place-trials", type=int, default=6,
+                    help="placement trials per level before the IC "
+                         "(axs_level_place); 0 = keep the first placement")
     ap.add_argument("--cpu-n", type=int, default=512,
                     help="CPU baseline sample: level size")
     ap.add_argument("--cpu-steps", type=int, default=5,
@@ -297,8 +307,12 @@ def main(argv=None):
         return 0
 
     log(f"[bench] FP32C {n}^3: IC + {args.warmup} warmup + {args.steps} timed steps")
-    a, wall32, ms32, ns32, tau_end = run_level(
-        capi, ctx, n, capi.AXS_FP32C, bg, dt, args.warmup, args.steps, tile)
+    a, wall32, ms32, ns32, tau_end, place32 = run_level(
+        capi, ctx, n, capi.AXS_FP32C, bg, dt, args.warmup, args.steps, tile,
+        args.place_trials)
+    if place32:
+        log(f"[bench] FP32C placement trials (ms/step): "
+            f"{', '.join(f'{t:.2f}' for t in place32)}")
     ncell = n ** 3
     value = ncell * args.steps / wall32
     d32 = diagnostics(capi, a, bg, tau_end, n)
@@ -307,9 +321,13 @@ def main(argv=None):
     res64 = None
     if not args.no_fp64:
         log(f"[bench] FP64 {n}^3 same IC and steps")
-        b, wall64, ms64, ns64, _ = run_level(
+        b, wall64, ms64, ns64, _, place64 = run_level(
             capi, ctx, n, capi.AXS_FP64, bg, dt, args.warmup, args.steps,
-            autotune.cached_choice(n, capi.AXS_FP64) if not args.tile else tile)
+            autotune.cached_choice(n, capi.AXS_FP64) if not args.tile else tile,
+            args.place_trials)
+        if place64:
+            log(f"[bench] FP64 placement trials (ms/step): "
+                f"{', '.join(f'{t:.2f}' for t in place64)}")
         d64 = diagnostics(capi, b, bg, tau_end, n)
         dpsi, _ = a.compare(b)
         value64 = ncell * args.steps / wall64
@@ -322,7 +340,7 @@ def main(argv=None):
                                   d64["rho_tot_unscreened"]) /
             abs(d64["rho_tot_unscreened"]),
             "delta_xi": d32["xi"] - d64["xi"], "xi_fp64": d64["xi"],
-            "tau": tau_end,
+            "tau": tau_end, "placement_trials_ms": place64,
         }
         log(f"[bench] FP64 {value64:.4e} cell-updates/s; ratio {value / value64:.3f}")
         b.close()
@@ -331,9 +349,11 @@ def main(argv=None):
     st = [m / ns32 for m in ms32]  # ms per launch of each stage
     bytes23 = STAGE_BYTES[1][1] * ncell
     achieved = bytes23 / (0.5 * (st[1] + st[2]) * 1e-3) / 1e9
+    per_cell = MEASURED_S23_BYTES.get(tuple(tile) if tile else None)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                "traffic": None,
+                "traffic": per_cell * ncell if per_cell and n == 1024 else None,
+                "traffic_unit": "bytes per launch",
                 "kernel": "k_stage<float, FP32C, MODE_S23> (RK stages 2/3)",
                 "stage_ms": st,
                 "stage_gbs": [STAGE_BYTES[1][i] * ncell / (st[i] * 1e-3) / 1e9
@@ -358,7 +378,8 @@ def main(argv=None):
                                "Gaussian IC (seed 42, |p|<=N/8)",
                    "precision": "fp32 storage + Kahan-compensated y_n "
                                 "(fp32 hi + fp32 lo)",
-                   "tile": list(tile) if tile else None},
+                   "tile": list(tile) if tile else None,
+                   "placement_trials_ms": place32},
         "fp64": res64, "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

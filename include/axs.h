@@ -180,6 +180,18 @@ int axs_rk4_advance(axs_level *lvl, const axs_background *bg, double tau,
 int axs_rk4_advance_coeffs(axs_level *lvl, const axs_rhs_coeffs *coeffs,
                            double dt, int nsteps, int use_graph);
 
+/* Placement trials; no counterpart in the reference (an MI355X-specific
+ * set-up step).  The physical pages a level's arrays land on decide between
+ * two HBM bandwidth regimes of the stage kernels (1024^3 FP32C: about 57 or
+ * 65 ms per RK4 step for the same virtual layout).  Re-allocates the
+ * level's arrays up to `trials` times (1..16), timing two RK4 steps on each
+ * placement, and keeps the first placement after the first within 2% of
+ * the fastest seen, else the last.  Needs no more memory than the level.
+ * The state is reset to zero: call it before uploading data.
+ * ms_per_step (optional, `trials` entries) receives each trial's time and
+ * *used (optional) the number of placements tried. */
+int axs_level_place(axs_level *lvl, int trials, double *ms_per_step, int *used);
+
 /* Per-stage kernel timing with HIP events recorded on the context stream
  * around each stage launch of axs_rk4_advance (forces direct launches while
  * on).  ms[s] accumulates the milliseconds of stage s+1, *nsteps the steps

@@ -112,3 +112,26 @@ def test_handoff_rejects_zero_R_pre(ctx):
     with pytest.raises(capi.AxsError):
         lvl.handoff_rescale(1.0, 0.0, 0.0)
     lvl.close()
+
+
+def test_placement_trials_keep_a_working_level(ctx, oracle):
+    """axs_level_place re-allocates the arrays (state reset to zero); the
+    level it keeps computes the same bits as one that was never moved."""
+    capi = _capi()
+    s = oracle.seeded_ic(32, seed=8)
+    bg = capi.make_background(2.0)
+    a = capi.Level(ctx, 32, 0.5, capi.AXS_FP32C)
+    ms = a.place(3)
+    assert 1 <= len(ms) <= 3 and all(t > 0 for t in ms)
+    assert not np.any(a.download())
+    b = capi.Level(ctx, 32, 0.5, capi.AXS_FP32C)
+    for lvl in (a, b):
+        lvl.upload(s)
+        lvl.advance(bg, 1.0, 0.1, 3, use_graph=True)
+    ha, la = a.download_raw()
+    hb, lb = b.download_raw()
+    assert np.array_equal(ha, hb) and np.array_equal(la, lb)
+    with pytest.raises(capi.AxsError):
+        a.place(0)
+    a.close()
+    b.close()
