@@ -49,25 +49,58 @@ def blocks_from_tags(tags: np.ndarray, block: int) -> np.ndarray:
     return t.any(axis=(1, 3, 5))
 
 
-def boxes_from_blocks(blk: np.ndarray, fine_block: int, max_blocks_x: int):
-    """Fine-level boxes (ratio 2): one box per run of up to max_blocks_x
-    consecutive tagged coarse blocks along x."""
-    boxes = []
+def boxes_from_blocks(blk: np.ndarray, fine_block: int, max_blocks: int):
+    """Fine-level boxes (ratio 2) covering the tagged coarse blocks: runs of
+    up to max_blocks consecutive blocks along x, runs with the same x extent
+    merged along y, then along z (each again up to max_blocks), so that a
+    fully tagged region becomes boxes of max_blocks^3 blocks."""
     bz, by, bx = blk.shape
+    runs = {}  # (k, j) -> list of (i0, i1) block runs, i1 exclusive
     for k in range(bz):
         for j in range(by):
+            row = blk[k, j]
             i = 0
             while i < bx:
-                if not blk[k, j, i]:
+                if not row[i]:
                     i += 1
                     continue
                 i0 = i
-                while i < bx and blk[k, j, i] and i - i0 < max_blocks_x:
+                while i < bx and row[i] and i - i0 < max_blocks:
                     i += 1
-                lo = (i0 * fine_block, j * fine_block, k * fine_block)
-                hi = (i * fine_block - 1, (j + 1) * fine_block - 1,
-                      (k + 1) * fine_block - 1)
-                boxes.append((lo, hi))
+                runs.setdefault((k, j), []).append((i0, i))
+    # merge along y: same (k, i0, i1) in consecutive rows j
+    slabs = {}  # k -> list of (i0, i1, j0, j1)
+    for k in range(bz):
+        open_ = {}
+        out = []
+        for j in range(by + 1):
+            cur = set(runs.get((k, j), [])) if j < by else set()
+            for key in list(open_):
+                j0 = open_[key]
+                if key not in cur or j - j0 == max_blocks:
+                    out.append((key[0], key[1], j0, j))
+                    del open_[key]
+            for key in cur:
+                if key not in open_:
+                    open_[key] = j
+        slabs[k] = out
+    # merge along z: same (i0, i1, j0, j1) in consecutive planes k
+    boxes = []
+    open_ = {}
+    for k in range(bz + 1):
+        cur = set(slabs.get(k, [])) if k < bz else set()
+        for key in list(open_):
+            k0 = open_[key]
+            if key not in cur or k - k0 == max_blocks:
+                i0, i1, j0, j1 = key
+                boxes.append(((i0 * fine_block, j0 * fine_block, k0 * fine_block),
+                              (i1 * fine_block - 1, j1 * fine_block - 1,
+                               k * fine_block - 1)))
+                del open_[key]
+        for key in cur:
+            if key not in open_:
+                open_[key] = k
+    boxes.sort(key=lambda b: (b[0][2], b[0][1], b[0][0]))
     return boxes
 
 

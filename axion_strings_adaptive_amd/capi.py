@@ -136,6 +136,11 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     if not os.path.exists(path):
         raise AxsError(f"{path} not built: run __graft_entry__.build() or "
                        "python -m axion_strings_adaptive_amd.build")
+    # torch (which provides the device buffers of this binding) must bring
+    # up its HIP runtime before the library is loaded, so that both share
+    # one runtime instance in the process
+    import torch
+    torch.cuda.is_available()
     lib = C.CDLL(path)
     for name, (res, args) in _SIGS.items():
         fn = getattr(lib, name, None)

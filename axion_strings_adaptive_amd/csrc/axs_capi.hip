@@ -824,6 +824,7 @@ int axs_level_place(axs_level *L, int trials, double *ms_per_step, int *used)
     AXS_HIP_TRY(hipEventCreate(&e1));
     int rc = AXS_OK, k = 0;
     double best = 1e300;
+    const int min_trials = trials < 3 ? trials : 3;
     for (k = 0; k < trials; ++k)
     {
         if (k > 0)
@@ -858,8 +859,9 @@ int axs_level_place(axs_level *L, int trials, double *ms_per_step, int *used)
         if (ms_per_step)
             ms_per_step[k] = t;
         best = t < best ? t : best;
-        // keep the first placement within 2% of the fastest one seen
-        if (k >= 1 && t <= 1.02 * best)
+        // after min_trials placements, keep the first one within 2% of the
+        // fastest seen
+        if (k + 1 >= min_trials && t <= 1.02 * best)
         {
             ++k;
             break;

@@ -489,7 +489,7 @@ static int launch_ty(int base, const StageParams<T> &p, int ntiles,
     case 4:
         return launch_t<T, PREC, MODE, 8, 2>(p, ntiles, s, pf);
     default:
-        // FP64 x4 tiles would exceed 64 KiB of static LDS
+        // FP64 x4 tiles need more than 256 VGPRs (spills): run x2 instead
         if constexpr (sizeof(T) == 8)
             return launch_t<T, PREC, MODE, 8, 2>(p, ntiles, s, pf);
         else if (base == 5)
@@ -546,7 +546,7 @@ static int fill_and_launch(axs_level *L, int mode, const void *yin, void *yout,
     const bool pf = (group & 2) != 0;
     int base = variant % kBaseVariants;
     if (sizeof(T) == 8 && kTiles[base].vx == 4)
-        base = 4; // FP64 x4 tiles would exceed 64 KiB of static LDS
+        base = 4; // FP64 x4 tiles need more than 256 VGPRs (spills)
     // vector variants need whole vectors per row
     if (p.nx % kTiles[base].vx != 0)
         base = 1;

@@ -106,6 +106,22 @@ AxionStringsLevel::AxionStringsLevel(const ParamTable &params,
     axs_check(axs_level_create(m_ctx, m_n[0], m_n[1], m_n[2], m_dx,
                                m_precision, &m_level),
               "axs_level_create");
+    // extension: placement trials of the level's arrays before initData
+    int trials = 0;
+    as.queryAdd("placement_trials", trials);
+    if (trials < 0 || trials > 16)
+        as.error("placement_trials", "must be in 0..16");
+    if (trials > 0)
+    {
+        std::vector<double> ms(trials, 0.0);
+        int used = 0;
+        axs_check(axs_level_place(m_level, trials, ms.data(), &used),
+                  "axs_level_place");
+        std::cout << "  [AxionStrings] placement trials (ms/step):";
+        for (int i = 0; i < used; ++i)
+            std::cout << " " << ms[i];
+        std::cout << "\n";
+    }
 }
 
 AxionStringsLevel::~AxionStringsLevel()

@@ -12,6 +12,8 @@
 //   axion_strings.tagging.amplitude_threshold  tag where |psi|/R < value
 //   axion_strings.spectrum_field      a_dot | pi   (default a_dot)
 //   axion_strings.use_graph           replay a captured hipGraph per step
+//   axion_strings.placement_trials    re-allocate the level's arrays up to N
+//                                     times and keep a fast placement
 #pragma once
 
 #include "Background.hpp"

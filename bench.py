@@ -222,7 +222,7 @@ def main(argv=None):
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--place-trials", type=int, default=6,
+    ap.add_argument("--place-trials", type=int, default=8,
                     help="placement trials per level before the IC "
                          "(axs_level_place); 0 = keep the first placement")
     ap.add_argument("--cpu-n", type=int, default=512,

@@ -185,8 +185,9 @@ int axs_rk4_advance_coeffs(axs_level *lvl, const axs_rhs_coeffs *coeffs,
  * two HBM bandwidth regimes of the stage kernels (1024^3 FP32C: about 57 or
  * 65 ms per RK4 step for the same virtual layout).  Re-allocates the
  * level's arrays up to `trials` times (1..16), timing two RK4 steps on each
- * placement, and keeps the first placement after the first within 2% of
- * the fastest seen, else the last.  Needs no more memory than the level.
+ * placement, and keeps the first placement from the third on (or the
+ * last, with fewer trials) within 2% of the fastest seen, else the last.
+ * Needs no more memory than the level.
  * The state is reset to zero: call it before uploading data.
  * ms_per_step (optional, `trials` entries) receives each trial's time and
  * *used (optional) the number of placements tried. */

@@ -1,0 +1,43 @@
+"""Host-side box building of the static AMR hierarchy (amr.py): the boxes
+partition exactly the tagged blocks, respect the blocking factor and the
+maximum grid size, and a fully tagged region becomes max-size boxes."""
+
+import numpy as np
+
+from axion_strings_adaptive_amd import amr
+
+
+def _check_partition(blk, boxes, fb, max_blocks):
+    cov = np.zeros(tuple(s * fb for s in blk.shape), bool)
+    for lo, hi in boxes:
+        for d in range(3):
+            n = hi[d] - lo[d] + 1
+            assert lo[d] % fb == 0 and n % fb == 0 and fb <= n <= fb * max_blocks
+        sl = (slice(lo[2], hi[2] + 1), slice(lo[1], hi[1] + 1),
+              slice(lo[0], hi[0] + 1))
+        assert not cov[sl].any()
+        cov[sl] = True
+    assert np.array_equal(cov, np.kron(blk, np.ones((fb,) * 3, bool)))
+
+
+def test_boxes_partition_random_tag_patterns():
+    rng = np.random.default_rng(0)
+    for _ in range(200):
+        shape = tuple(int(v) for v in rng.integers(1, 9, 3))
+        blk = rng.random(shape) < rng.random()
+        _check_partition(blk, amr.boxes_from_blocks(blk, 16, 4), 16, 4)
+
+
+def test_fully_tagged_region_gives_max_size_boxes():
+    blk = np.ones((8, 8, 8), bool)
+    boxes = amr.boxes_from_blocks(blk, 16, 4)
+    assert len(boxes) == 8
+    assert all(hi[d] - lo[d] + 1 == 64 for lo, hi in boxes for d in range(3))
+
+
+def test_dilation_and_erosion_are_periodic():
+    m = np.zeros((8, 8, 8), bool)
+    m[0, 0, 0] = True
+    d = amr.dilate_periodic(m, 1)
+    assert d.sum() == 27 and d[7, 7, 7] and d[1, 1, 1]
+    assert np.array_equal(amr.erode_periodic(d, 1), m)
