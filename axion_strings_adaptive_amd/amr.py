@@ -9,6 +9,11 @@ merged along x into boxes of at most max_grid_size cells
 (amr.blocking_factor / amr.max_grid_size), and kept properly nested one
 block inside the coarser level's coverage.
 
+With regrid_int > 0 the finer levels are rebuilt every regrid_int coarse
+steps from fresh tags (the reference's amr.regrid_int): a new level takes
+the old level's data where the two overlap and quartic interpolation from
+the level below elsewhere.
+
 One coarse step advances level l once and level l+1 twice (subcycling,
 docs/conventions.md:239-241): the coarse state is snapshotted, the coarse
 level takes its RK4 step, the fine level takes two half steps whose
@@ -115,9 +120,18 @@ class Hierarchy:
                  blocking_factor: int = 16, max_grid_size: int = 64,
                  seed: int = 42, bg=None, tau0: float = 1.0,
                  dt_multiplier: float = 0.2, verbose: bool = True,
-                 force_full_refinement: bool = False, use_graph: bool = True):
+                 force_full_refinement: bool = False, use_graph: bool = True,
+                 regrid_int: int = 0):
         self.ctx = ctx
         self.use_graph = use_graph
+        self.max_levels = levels
+        self.regrid_int = regrid_int
+        self.force_full_refinement = force_full_refinement
+        self.n_error_buf = n_error_buf
+        self.blocking_factor = blocking_factor
+        self.max_grid_size = max_grid_size
+        self.nsteps = 0
+        self.nregrids = 0
         self.bg = bg if bg is not None else capi.make_background(
             2.0, c0=0.0, c1=2.0, tau_switch=1.0, has_switch=True)
         self.tau = tau0
@@ -144,25 +158,13 @@ class Hierarchy:
         self.levels.append(L0)
         self.domains.append((n0,) * 3)
         # ---- finer levels from the tagger ----
-        R = capi.background_eval(self.bg, 0, self.tau)
         covered = np.ones((n0 // 8,) * 3, bool)  # level-0 blocks of 8 cells
         for l in range(1, levels + 1):
             coarse = self.levels[-1]
-            cd = self.domains[-1]
-            tags_buf, _ = coarse.tag(amp_thr=amp_thr, R=R)
-            tags = tags_buf.to_numpy(np.uint8).reshape(cd[2], cd[1], cd[0]) > 0
-            if force_full_refinement:  # axion_strings.tagging.force_full_refinement
-                tags[:] = True
-            tags = dilate_periodic(tags, n_error_buf)
-            cblock = blocking_factor // 2          # coarse cells per fine box edge
-            blk = blocks_from_tags(tags, cblock)
-            if l > 1:  # proper nesting: one block inside the coarse coverage
-                blk &= erode_periodic(covered, 1)
-            boxes = boxes_from_blocks(blk, blocking_factor,
-                                      max_grid_size // blocking_factor)
+            boxes, blk = self._fine_boxes(l - 1, covered)
             if not boxes:
                 break
-            fd = tuple(2 * d for d in cd)
+            fd = tuple(2 * d for d in self.domains[-1])
             Lf = capi.BoxLevel(ctx, boxes, fd, self.dx[l], precision)
             Lf.fill_ghosts(coarse, 1.0, interior=True)
             self.levels.append(Lf)
@@ -173,6 +175,60 @@ class Hierarchy:
                 print(f"[amr] level {l}: {len(boxes)} boxes, "
                       f"{Lf.num_cells} cells", file=sys.stderr, flush=True)
         self.ncells = [L.num_cells for L in self.levels]
+
+    def _fine_boxes(self, l: int, covered: np.ndarray):
+        """Boxes of level l+1 from the tags of level l (ghosts filled):
+        tag, grow by n_error_buf, cut into blocking-factor blocks, keep them
+        one block inside the coverage of level l (l >= 1), merge."""
+        coarse = self.levels[l]
+        cd = self.domains[l]
+        R = capi.background_eval(self.bg, 0, self.tau)
+        tags_buf, _ = coarse.tag(amp_thr=self.amp_thr, R=R)
+        tags = tags_buf.to_numpy(np.uint8).reshape(cd[2], cd[1], cd[0]) > 0
+        if self.force_full_refinement:  # axion_strings.tagging.force_full_refinement
+            tags[:] = True
+        tags = dilate_periodic(tags, self.n_error_buf)
+        cblock = self.blocking_factor // 2   # coarse cells per fine box edge
+        blk = blocks_from_tags(tags, cblock)
+        if l >= 1:  # proper nesting: one block inside the coarse coverage
+            blk &= erode_periodic(covered, 1)
+        boxes = boxes_from_blocks(blk, self.blocking_factor,
+                                  self.max_grid_size // self.blocking_factor)
+        return boxes, blk
+
+    def regrid(self):
+        """Rebuilds levels 1.. from fresh tags at a synchronised time."""
+        covered = np.ones(tuple(d // 8 for d in self.domains[0]), bool)
+        for l in range(1, self.max_levels + 1):
+            coarse = self.levels[l - 1]
+            coarse.fill_ghosts(self.levels[l - 2] if l >= 2 else None)
+            boxes, blk = self._fine_boxes(l - 1, covered)
+            old = self.levels[l] if l < len(self.levels) else None
+            if not boxes:
+                for L in self.levels[l:]:
+                    L.close()
+                del self.levels[l:]
+                del self.domains[l:]
+                break
+            covered_next = np.kron(blk, np.ones((2, 2, 2), bool))
+            if old is not None and sorted(old.boxes) == sorted(
+                    tuple(map(tuple, b)) for b in boxes):
+                covered = covered_next
+                continue
+            fd = tuple(2 * d for d in self.domains[l - 1])
+            Lf = capi.BoxLevel(self.ctx, boxes, fd, self.dx[l], self.precision)
+            Lf.fill_ghosts(coarse, 1.0, interior=True)
+            if old is not None:
+                Lf.copy_valid_from(old)
+                old.close()
+                self.levels[l] = Lf
+            else:
+                self.levels.append(Lf)
+                self.domains.append(fd)
+            Lf.fill_ghosts(coarse, 1.0)
+            covered = covered_next
+        self.ncells = [L.num_cells for L in self.levels]
+        self.nregrids += 1
 
     def coeffs(self, tau, dt):
         return [capi.rhs_coeffs(self.bg, t) for t in
@@ -191,9 +247,13 @@ class Hierarchy:
             self.levels[l + 1].average_down_to(L)
 
     def step(self):
-        """One coarse step of the whole hierarchy."""
+        """One coarse step of the whole hierarchy (then a regrid every
+        regrid_int coarse steps)."""
         self._advance(0, self.tau, self.dt0, 0.0, 1.0)
         self.tau += self.dt0
+        self.nsteps += 1
+        if self.regrid_int > 0 and self.nsteps % self.regrid_int == 0:
+            self.regrid()
 
     def cell_updates_per_coarse_step(self) -> int:
         return sum(n * 2 ** l for l, n in enumerate(self.ncells))
@@ -228,9 +288,12 @@ def main(argv=None):
     ap.add_argument("--levels", type=int, default=2)
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--precision", type=int, default=capi.AXS_FP32C)
+    ap.add_argument("--regrid-int", type=int, default=0,
+                    help="rebuild the finer levels every N coarse steps")
     args = ap.parse_args(argv)
     ctx = capi.Context(0)
-    h = Hierarchy(ctx, args.n0, args.levels, precision=args.precision)
+    h = Hierarchy(ctx, args.n0, args.levels, precision=args.precision,
+                  regrid_int=args.regrid_int)
     h.step()  # warm-up
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -241,7 +304,7 @@ def main(argv=None):
     e = h.composite_energy()
     print(json.dumps({
         "levels": len(h.levels), "cells": h.ncells,
-        "boxes": [len(L.boxes) for L in h.levels],
+        "boxes": [len(L.boxes) for L in h.levels], "regrids": h.nregrids,
         "cell_updates_per_coarse_step": h.cell_updates_per_coarse_step(),
         "cell_updates_per_s": h.cell_updates_per_coarse_step() * args.steps / dt,
         "ms_per_coarse_step": 1e3 * dt / args.steps,

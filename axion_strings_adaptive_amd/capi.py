@@ -105,6 +105,7 @@ _SIGS = {
     "axs_boxlevel_download_dense": (_I, [_P, _PD]),
     "axs_boxlevel_copy_from_level": (_I, [_P, _P]),
     "axs_boxlevel_copy_to_level": (_I, [_P, _P]),
+    "axs_boxlevel_copy_valid": (_I, [_P, _P, C.POINTER(C.c_int64)]),
     "axs_boxlevel_snapshot_coarse": (_I, [_P, _P]),
     "axs_boxlevel_fill_ghosts": (_I, [_P, _P, _D, _I]),
     "axs_boxlevel_rk4_step": (_I, [_P, C.POINTER(RhsCoeffs), _D, _P, _D, _D]),
@@ -481,6 +482,12 @@ class BoxLevel:
 
     def copy_to_level(self, lvl: Level):
         check(self.lib.axs_boxlevel_copy_to_level(self.h, lvl.h))
+
+    def copy_valid_from(self, src: "BoxLevel") -> int:
+        """Copies the valid cells `src` also covers; returns their count."""
+        n = C.c_int64()
+        check(self.lib.axs_boxlevel_copy_valid(self.h, src.h, C.byref(n)))
+        return n.value
 
     def snapshot_coarse(self, coarse: "BoxLevel"):
         check(self.lib.axs_boxlevel_snapshot_coarse(self.h, coarse.h))

@@ -17,6 +17,7 @@
 #include "axs_stage_math.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -67,13 +68,17 @@ struct axs_boxlevel
     void *snap;      // coarse snapshot (y [+ comp]) for time interpolation
     float *snap_lo;
     size_t snap_bytes;
-    const axs_boxlevel *snap_of;
+    unsigned long long snap_uid; // uid of the level the snapshot is of
     unsigned long long *derr;
-    // captured step graphs, keyed by (dt, coarse, t0_frac, dt_frac)
+    // identity of this level for the graph caches of finer levels: unique
+    // over the process, unlike an address that a later level may reuse
+    unsigned long long uid;
+    // captured step graphs, keyed by (dt, coarse level uid, t0_frac,
+    // dt_frac); uid 0 = no coarse level
     struct GraphEntry
     {
         double dt, t0, df;
-        const axs_boxlevel *coarse;
+        unsigned long long coarse_uid;
         hipGraph_t g;
         hipGraphExec_t ge;
     };
@@ -167,7 +172,7 @@ __device__ double coarse_value(const CoarseSrc<T> &c, int comp, int cx, int cy,
         atomicAdd(err, 1ull);
         return 0.0;
     }
-    const uint32_t o = (uint32_t)comp * c.v.total + i;
+    const size_t o = (size_t)comp * c.v.total + i;
     double v = (double)c.cur[o] + (c.cur_lo ? (double)c.cur_lo[o] : 0.0);
     if (c.snap && c.t_frac < 1.0)
     {
@@ -229,9 +234,9 @@ __global__ void k_fill(LevelView f, T *buf, float *lo, int have_coarse,
         {
             for (int comp = 0; comp < 4; ++comp)
             {
-                buf[comp * f.total + dst] = buf[comp * f.total + src];
+                buf[(size_t)comp * f.total + dst] = buf[(size_t)comp * f.total + src];
                 if (lo)
-                    lo[comp * f.total + dst] = lo[comp * f.total + src];
+                    lo[(size_t)comp * f.total + dst] = lo[(size_t)comp * f.total + src];
             }
             continue;
         }
@@ -246,9 +251,9 @@ __global__ void k_fill(LevelView f, T *buf, float *lo, int have_coarse,
         {
             const double v = interp_quartic(c, comp, px, py, pz, err);
             const T h = (T)v;
-            buf[comp * f.total + dst] = h;
+            buf[(size_t)comp * f.total + dst] = h;
             if (lo)
-                lo[comp * f.total + dst] = (float)(v - (double)h);
+                lo[(size_t)comp * f.total + dst] = (float)(v - (double)h);
         }
     }
 }
@@ -290,7 +295,7 @@ __global__ void __launch_bounds__(256) k_box_stage(BoxStageParams<T> p)
     const double *cf = p.coeff + 3 * p.coeff_stage;
     const double cc = cf[0], lam = cf[1], R2 = cf[2];
     const T i2 = (T)p.inv_dx2;
-    const uint32_t tot = p.total;
+    const size_t tot = p.total;
     const int ng = p.ng;
     for (int it = blockIdx.x; it < p.nitems; it += gridDim.x)
     {
@@ -473,7 +478,7 @@ __global__ void k_box_tag(LevelView v, const T *y, const float *lo,
 {
     const BoxD B = v.box[blockIdx.y];
     const uint32_t cells = (uint32_t)B.n[0] * B.n[1] * B.n[2];
-    const uint32_t tot = v.total;
+    const size_t tot = v.total;
     unsigned long long plain = 0, weighted = 0;
     const int iters = (cells + gridDim.x * blockDim.x - 1) / (gridDim.x * blockDim.x);
     for (int r = 0; r < iters; ++r)
@@ -550,13 +555,13 @@ __global__ void k_box_energy(LevelView v, const T *y, const float *lo,
 {
     const BoxD B = v.box[blockIdx.y];
     const uint32_t cells = (uint32_t)B.n[0] * B.n[1] * B.n[2];
-    const uint32_t tot = v.total;
+    const size_t tot = v.total;
     const double i1 = 1.0 / dx, R2 = R * R, R4 = R2 * R2, f_a = 1.4142135623730951;
     double acc[14];
     for (int q = 0; q < 14; ++q)
         acc[q] = 0.0;
     auto val = [&](int comp, uint32_t i) -> double {
-        const uint32_t o = comp * tot + i;
+        const size_t o = (size_t)comp * tot + i;
         return (double)y[o] + (lo ? (double)lo[o] : 0.0);
     };
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < cells;
@@ -709,16 +714,16 @@ __global__ void k_average_down(LevelView f, const T *fy, const float *flo,
                 for (int j = 0; j < 2; ++j)
                     for (int i = 0; i < 2; ++i)
                     {
-                        const uint32_t o = comp * f.total +
+                        const size_t o = (size_t)comp * f.total +
                                            sidx(B, f.ng, 2 * ix + i, 2 * iy + j,
                                                 2 * iz + k);
                         s += (double)fy[o] + (flo ? (double)flo[o] : 0.0);
                     }
             const double v = 0.125 * s;
             const T h = (T)v;
-            cy_[comp * c.total + ci] = h;
+            cy_[(size_t)comp * c.total + ci] = h;
             if (clo)
-                clo[comp * c.total + ci] = (float)(v - (double)h);
+                clo[(size_t)comp * c.total + ci] = (float)(v - (double)h);
         }
     }
 }
@@ -743,18 +748,51 @@ __global__ void k_copy_level(LevelView v, T *by, float *blo, T *uy, float *ulo,
         {
             if (to_level)
             {
-                uy[comp * un + u] = by[comp * v.total + b];
+                uy[(size_t)comp * un + u] = by[(size_t)comp * v.total + b];
                 if (ulo && blo)
-                    ulo[comp * un + u] = blo[comp * v.total + b];
+                    ulo[(size_t)comp * un + u] = blo[(size_t)comp * v.total + b];
             }
             else
             {
-                by[comp * v.total + b] = uy[comp * un + u];
+                by[(size_t)comp * v.total + b] = uy[(size_t)comp * un + u];
                 if (ulo && blo)
-                    blo[comp * v.total + b] = ulo[comp * un + u];
+                    blo[(size_t)comp * v.total + b] = ulo[(size_t)comp * un + u];
             }
         }
     }
+}
+
+// Valid cells of every destination box that some source box also owns
+// (regridding: the new level keeps the old level's data where they
+// overlap).
+template <typename T>
+__global__ void k_copy_boxes(LevelView dv, T *dy, float *dlo, LevelView sv,
+                             const T *sy, const float *slo,
+                             unsigned long long *copied)
+{
+    const BoxD B = dv.box[blockIdx.y];
+    const uint32_t cells = (uint32_t)B.n[0] * B.n[1] * B.n[2];
+    unsigned long long cnt = 0;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < cells;
+         t += gridDim.x * blockDim.x)
+    {
+        const int lx = t % B.n[0], ly = (t / B.n[0]) % B.n[1],
+                  lz = t / ((uint32_t)B.n[0] * B.n[1]);
+        uint32_t si;
+        if (locate(sv, B.lo[0] + lx, B.lo[1] + ly, B.lo[2] + lz, si) < 0)
+            continue;
+        const uint32_t di = sidx(B, dv.ng, lx, ly, lz);
+        for (int comp = 0; comp < 4; ++comp)
+        {
+            dy[(size_t)comp * dv.total + di] = sy[(size_t)comp * sv.total + si];
+            if (dlo && slo)
+                dlo[(size_t)comp * dv.total + di] =
+                    slo[(size_t)comp * sv.total + si];
+        }
+        ++cnt;
+    }
+    if (cnt)
+        atomicAdd(copied, cnt);
 }
 
 __global__ void k_split(const double *src, float *hi, float *lo, size_t n)
@@ -837,7 +875,7 @@ static int fill_t(axs_boxlevel *bl, void *buf, float *lo,
         c.v = view_of(coarse);
         c.cur = (const T *)coarse->y;
         c.cur_lo = coarse->comp;
-        if (bl->snap && bl->snap_of == coarse)
+        if (bl->snap && bl->snap_uid == coarse->uid)
         {
             c.snap = (const T *)bl->snap;
             c.snap_lo = bl->snap_lo;
@@ -933,7 +971,9 @@ int axs_boxlevel_create(axs_ctx *ctx, int nboxes, const int *lo, const int *hi,
     for (int d = 0; d < 3; ++d)
         AXS_CHECK_ARG(domain[d] >= kBlock && domain[d] % kBlock == 0,
                       "axs_boxlevel_create: domain must be a multiple of 8");
+    static std::atomic<unsigned long long> next_uid{1};
     axs_boxlevel *b = new axs_boxlevel{};
+    b->uid = next_uid.fetch_add(1);
     b->ctx = ctx;
     b->nb = nboxes;
     b->ng = nghost;
@@ -990,7 +1030,9 @@ int axs_boxlevel_create(axs_ctx *ctx, int nboxes, const int *lo, const int *hi,
             items.push_back(WorkItem{i, y0, std::min(H, 256 / W), W});
         b->hbox.push_back(B);
     }
-    if (off * 4 >= (1ull << 32))
+    // offsets inside one component array are 32-bit; component strides
+    // are applied in 64-bit
+    if (off >= (1ull << 32))
     {
         delete b;
         return set_error(AXS_ERR_ARG, "axs_boxlevel_create: level too large");
@@ -1180,6 +1222,42 @@ int axs_boxlevel_copy_to_level(const axs_boxlevel *b, axs_level *dst)
     return copy_level(b, dst, 1);
 }
 
+int axs_boxlevel_copy_valid(axs_boxlevel *dst, const axs_boxlevel *src,
+                            int64_t *copied)
+{
+    AXS_CHECK_ARG(dst && src, "axs_boxlevel_copy_valid: NULL argument");
+    AXS_CHECK_ARG(dst != src, "axs_boxlevel_copy_valid: same level");
+    for (int d = 0; d < 3; ++d)
+        AXS_CHECK_ARG(dst->dom[d] == src->dom[d],
+                      "axs_boxlevel_copy_valid: domains differ");
+    AXS_CHECK_ARG(dst->precision == src->precision,
+                  "axs_boxlevel_copy_valid: precisions differ");
+    AXS_CHECK_ARG(dst->ctx == src->ctx,
+                  "axs_boxlevel_copy_valid: contexts differ");
+    hipStream_t s = dst->ctx->stream;
+    unsigned long long *cnt = dst->ctx->d_counts;
+    hipLaunchKernelGGL(k_zero_err, dim3(1), dim3(64), 0, s, cnt);
+    const dim3 g = box_grid(dst, max_valid(dst));
+    if (dst->precision == AXS_FP64)
+        hipLaunchKernelGGL(k_copy_boxes<double>, g, dim3(256), 0, s,
+                           view_of(dst), (double *)dst->y, (float *)nullptr,
+                           view_of(src), (const double *)src->y,
+                           (const float *)nullptr, cnt);
+    else
+        hipLaunchKernelGGL(k_copy_boxes<float>, g, dim3(256), 0, s,
+                           view_of(dst), (float *)dst->y, dst->comp,
+                           view_of(src), (const float *)src->y,
+                           (const float *)src->comp, cnt);
+    AXS_HIP_TRY(hipGetLastError());
+    AXS_HIP_TRY(hipMemcpyAsync(dst->ctx->h_counts, cnt,
+                               sizeof(unsigned long long),
+                               hipMemcpyDeviceToHost, s));
+    AXS_HIP_TRY(hipStreamSynchronize(s));
+    if (copied)
+        *copied = (int64_t)dst->ctx->h_counts[0];
+    return AXS_OK;
+}
+
 int axs_boxlevel_snapshot_coarse(axs_boxlevel *b, const axs_boxlevel *c)
 {
     AXS_CHECK_ARG(b && c, "axs_boxlevel_snapshot_coarse: NULL argument");
@@ -1203,7 +1281,7 @@ int axs_boxlevel_snapshot_coarse(axs_boxlevel *b, const axs_boxlevel *c)
         b->snap_bytes = sb + lb;
     }
     b->snap_lo = lb ? (float *)((char *)b->snap + sb) : nullptr;
-    b->snap_of = c;
+    b->snap_uid = c->uid;
     AXS_HIP_TRY(hipMemcpyAsync(b->snap, c->y, sb, hipMemcpyDeviceToDevice,
                                b->ctx->stream));
     if (lb)
@@ -1289,8 +1367,9 @@ int axs_boxlevel_rk4_step_graph(axs_boxlevel *b, const axs_rhs_coeffs coeffs[4],
     AXS_HIP_TRY(hipMemcpyAsync(b->dcoeff, h, sizeof(h), hipMemcpyHostToDevice, s));
     AXS_HIP_TRY(hipStreamSynchronize(s));
     axs_boxlevel::GraphEntry *hit = nullptr;
+    const unsigned long long cuid = coarse ? coarse->uid : 0ull;
     for (auto &e : b->graphs)
-        if (e.dt == dt && e.t0 == t0 && e.df == df && e.coarse == coarse)
+        if (e.dt == dt && e.t0 == t0 && e.df == df && e.coarse_uid == cuid)
             hit = &e;
     if (!hit)
     {
@@ -1322,7 +1401,7 @@ int axs_boxlevel_rk4_step_graph(axs_boxlevel *b, const axs_rhs_coeffs coeffs[4],
             (void)hipGraphDestroy(g);
             return set_error(AXS_ERR_HIP, "hipGraphInstantiate failed");
         }
-        b->graphs.push_back({dt, t0, df, coarse, g, ge});
+        b->graphs.push_back({dt, t0, df, cuid, g, ge});
         hit = &b->graphs.back();
     }
     AXS_HIP_TRY(hipGraphLaunch(hit->ge, s));
@@ -1368,10 +1447,15 @@ int axs_boxlevel_energy_sums(axs_boxlevel *b, double R, double lambda,
     AXS_CHECK_ARG(b && m && out, "axs_boxlevel_energy_sums: NULL argument");
     AXS_CHECK_ARG(R > 0.0, "axs_boxlevel_energy_sums: R must be > 0");
     hipStream_t s = b->ctx->stream;
-    const dim3 g = box_grid(b, max_valid(b));
+    dim3 g = box_grid(b, max_valid(b));
+    // one row of 14 partials per block: fewer blocks per box when there
+    // are many boxes (the kernel strides over a box's cells)
+    const size_t cap_rows = ((size_t)kMaxPartials * 16 - 14) / 14;
+    const size_t per_box = cap_rows / g.y;
+    AXS_CHECK_ARG(per_box >= 1, "axs_boxlevel_energy_sums: too many boxes");
+    if (g.x > per_box)
+        g.x = (unsigned)per_box;
     const size_t rows = (size_t)g.x * g.y;
-    AXS_CHECK_ARG(rows * 14 + 14 <= (size_t)kMaxPartials * 16,
-                  "axs_boxlevel_energy_sums: too many boxes");
     double *part = b->ctx->d_partials;
     if (b->precision == AXS_FP64)
         hipLaunchKernelGGL(k_box_energy<double>, g, dim3(256), 0, s, view_of(b),

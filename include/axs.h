@@ -183,14 +183,14 @@ int axs_rk4_advance_coeffs(axs_level *lvl, const axs_rhs_coeffs *coeffs,
 /* Placement trials; no counterpart in the reference (an MI355X-specific
  * set-up step).  The physical pages a level's arrays land on decide between
  * two HBM bandwidth regimes of the stage kernels (1024^3 FP32C: about 57 or
- * 65 ms per RK4 step for the same virtual layout).  Re-allocates the
- * level's arrays up to `trials` times (1..16), timing two RK4 steps on each
- * placement, and keeps the first placement from the third on (or the
- * last, with fewer trials) within 2% of the fastest seen, else the last.
- * Needs no more memory than the level.
- * The state is reset to zero: call it before uploading data.
- * ms_per_step (optional, `trials` entries) receives each trial's time and
- * *used (optional) the number of placements tried. */
+ * 65 ms per RK4 step for the same virtual layout).  Tries up to `trials`
+ * placements (1..16) of the level's arrays, timing two RK4 steps on each.
+ * While free memory holds a second copy of the arrays the fastest placement
+ * seen is kept; otherwise each placement is freed before the next is drawn
+ * and, from the third draw on, the first within 2% of the fastest seen is
+ * kept (else the last).  The state is reset to zero: call it before
+ * uploading data.  ms_per_step (optional, `trials` entries) receives each
+ * trial's time and *used (optional) the number of placements tried. */
 int axs_level_place(axs_level *lvl, int trials, double *ms_per_step, int *used);
 
 /* Per-stage kernel timing with HIP events recorded on the context stream
@@ -343,6 +343,13 @@ int axs_boxlevel_average_down(const axs_boxlevel *fine, axs_boxlevel *coarse);
  * and precision (cells outside the boxes are left as they are); with the
  * spectrum entry points this gives the level-0 FFT of the reference. */
 int axs_boxlevel_copy_to_level(const axs_boxlevel *bl, axs_level *dst);
+/* Regridding: copies the valid cells of `dst` that a valid box of `src`
+ * (same domain, precision and context) also covers, leaving the others
+ * as they are (AMReX's FillPatch from the old level inside AmrLevel::init
+ * after a regrid, AxionStringsLevel.cpp:1916-2005 tags the new grids).
+ * *copied (optional) receives the number of cells copied. */
+int axs_boxlevel_copy_valid(axs_boxlevel *dst, const axs_boxlevel *src,
+                            int64_t *copied);
 
 /* ------------------------------ adjoint ----------------------------- */
 /* Gradient of J = sum_cells (|grad psi1|^2 + |grad psi2|^2) dx^3 / R_f^4

@@ -67,3 +67,70 @@ def test_fully_refined_hierarchy_tracks_uniform_fine_grid(ctx, oracle):
         tau += dtf
     assert np.array_equal(fine.download_dense(), ref.download())
     h.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_copy_valid_between_box_levels(ctx, precision):
+    """axs_boxlevel_copy_valid copies exactly the valid cells both levels
+    cover and leaves the rest of the destination alone."""
+    from axion_strings_adaptive_amd import capi
+    n = 64
+    src_boxes = [((0, 0, 0), (31, 63, 63))]
+    dst_boxes = [((16, 0, 0), (47, 31, 63)), ((48, 32, 0), (63, 63, 31))]
+    src = capi.BoxLevel(ctx, src_boxes, (n, n, n), 0.5, precision)
+    dst = capi.BoxLevel(ctx, dst_boxes, (n, n, n), 0.5, precision)
+    rng = np.random.default_rng(3)
+    a = rng.normal(size=(4, n, n, n))
+    b = rng.normal(size=(4, n, n, n))
+    src.upload_dense(a)
+    dst.upload_dense(b)
+    a = src.download_dense(fill=0.0)   # as stored (FP32C: hi + lo)
+    b = dst.download_dense(fill=0.0)
+    copied = dst.copy_valid_from(src)
+    got = dst.download_dense(fill=0.0)
+    assert copied == 16 * 32 * 64
+    want = b.copy()
+    want[:, :, 0:32, 16:32] = a[:, :, 0:32, 16:32]
+    assert np.array_equal(got, want)
+    src.close()
+    dst.close()
+
+
+def test_regrid_with_unchanged_tags_keeps_levels(ctx):
+    from axion_strings_adaptive_amd import amr, capi
+    h = amr.Hierarchy(ctx, n0=32, levels=1, precision=capi.AXS_FP64,
+                      verbose=False, force_full_refinement=True)
+    before = h.levels[1]
+    s0 = before.download_dense(fill=0.0)
+    h.regrid()
+    assert h.levels[1] is before and h.nregrids == 1
+    assert np.array_equal(h.levels[1].download_dense(fill=0.0), s0)
+    h.close()
+
+
+def test_regridding_hierarchy_steps_and_keeps_volume(ctx):
+    """Regrid after coarse steps: cells the new level shares with the old
+    one keep their values (not re-interpolated), the composite volume stays
+    the domain's and the energies stay finite."""
+    from axion_strings_adaptive_amd import amr, capi
+    h = amr.Hierarchy(ctx, n0=32, levels=2, precision=capi.AXS_FP32C,
+                      verbose=False, amp_thr=0.45)
+    for _ in range(3):
+        h.step()
+        d_old = h.levels[1].download_dense(fill=np.nan)
+        h.regrid()
+        d_new = h.levels[1].download_dense(fill=np.nan)
+        both = ~np.isnan(d_old) & ~np.isnan(d_new)
+        assert both.any()
+        assert np.array_equal(d_old[both], d_new[both])
+        e = h.composite_energy()
+        assert e["n_total"] == pytest.approx((32 * 0.5) ** 3, rel=1e-12)
+        assert np.isfinite(e["rho_tot_unscreened"])
+    assert h.nregrids == 3
+    h2 = amr.Hierarchy(ctx, n0=32, levels=2, precision=capi.AXS_FP32C,
+                       verbose=False, amp_thr=0.45, regrid_int=1)
+    for _ in range(2):
+        h2.step()
+    assert h2.nregrids == 2
+    h2.close()
+    h.close()
