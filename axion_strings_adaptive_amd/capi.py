@@ -73,6 +73,7 @@ _SIGS = {
     "axs_level_upload": (_I, [_P, _PD]),
     "axs_level_download": (_I, [_P, _PD]),
     "axs_level_download_raw": (_I, [_P, _P, _P]),
+    "axs_level_upload_raw": (_I, [_P, _P, _P]),
     "axs_level_device_ptrs": (_I, [_P, C.POINTER(_P), C.POINTER(_P)]),
     "axs_level_set_homogeneous": (_I, [_P, _D, _D]),
     "axs_level_handoff_rescale": (_I, [_P, _D, _D, _D]),

@@ -502,6 +502,21 @@ int axs_level_download_raw(axs_level *L, void *hi, void *lo)
     return AXS_OK;
 }
 
+int axs_level_upload_raw(axs_level *L, const void *hi, const void *lo)
+{
+    AXS_CHECK_ARG(L && hi, "axs_level_upload_raw: NULL argument");
+    AXS_CHECK_ARG(L->precision != AXS_FP32C || lo,
+                  "axs_level_upload_raw: FP32C needs the low words");
+    const size_t bytes4 = (size_t)4 * L->g.ncell * L->word;
+    hipStream_t s = L->ctx->stream;
+    AXS_HIP_TRY(hipMemcpyAsync(L->y, hi, bytes4, hipMemcpyHostToDevice, s));
+    if (L->comp)
+        AXS_HIP_TRY(
+            hipMemcpyAsync(L->comp, lo, bytes4, hipMemcpyHostToDevice, s));
+    AXS_HIP_TRY(hipStreamSynchronize(s));
+    return AXS_OK;
+}
+
 int axs_level_set_homogeneous(axs_level *L, double R, double Rp)
 {
     AXS_CHECK_ARG(L, "axs_level_set_homogeneous: NULL level");

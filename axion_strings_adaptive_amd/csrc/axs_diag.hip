@@ -176,11 +176,27 @@ __global__ void __launch_bounds__(64 * WTY)
 
     // raw psi of the points this thread stages: own cell, x halo, y halo,
     // corner
-    double r[4][2];
-    auto raw = [&](int xx, int yy, int zz, double(&v)[2]) {
+    double r[4][4]; // psi1, psi2 (+ Pi1, Pi2 for the velocities)
+    // gamma^2 v^2 of every vertex of the ring planes (velocity passes)
+    __shared__ double gv[(WM & WM_VEL) ? 3 : 1][WTY + 1][TX + 1];
+    const T *q1 = y + 2 * (size_t)n, *q2 = y + 3 * (size_t)n;
+    const float *m1 = lo ? lo + 2 * (size_t)n : nullptr;
+    const float *m2 = lo ? lo + 3 * (size_t)n : nullptr;
+    auto raw = [&](int xx, int yy, int zz, double(&v)[4]) {
         const uint32_t i = (uint32_t)zz * plane + (uint32_t)yy * nx + xx;
         v[0] = ld(p1, l1, i);
         v[1] = ld(p2, l2, i);
+        if constexpr ((WM & WM_VEL) != 0)
+        {
+            v[2] = ld(q1, m1, i);
+            v[3] = ld(q2, m2, i);
+        }
+    };
+    // gamma^2 v^2 at one plaquette corner, VelocityKernel.hpp:82-96
+    auto corner_of = [&](const double(&v)[4]) -> double {
+        const double R = p.vel_R, RpR = p.vel_RpR;
+        return gamma_sq_v_sq_d(v[0] / R, v[1] / R, (v[2] - RpR * v[0]) / (R * R),
+                               (v[3] - RpR * v[1]) / (R * R), p.vel_mr);
     };
     auto load_plane = [&](int zz) {
         raw(xw, yw, zz, r[0]);
@@ -203,6 +219,21 @@ __global__ void __launch_bounds__(64 * WTY)
             if (hx)
                 th[s][WTY][TX] = atan2(r[3][1], r[3][0]);
         }
+        if constexpr ((WM & WM_VEL) != 0)
+        {
+            if (p.do_vel)
+            {
+                gv[s][ty][tx] = corner_of(r[0]);
+                if (hx)
+                    gv[s][ty][TX] = corner_of(r[1]);
+                if (hy)
+                {
+                    gv[s][WTY][tx] = corner_of(r[2]);
+                    if (hx)
+                        gv[s][WTY][TX] = corner_of(r[3]);
+                }
+            }
+        }
     };
 
     const int z_begin = bz * p.z_chunk;
@@ -217,19 +248,6 @@ __global__ void __launch_bounds__(64 * WTY)
     double vsum = 0.0;               // per lane
     const StarLoader S{nx, ny, nz, plane};
     const double i1 = 1.0 / p.dx, i2 = 1.0 / (p.dx * p.dx);
-    const T *q1 = y + 2 * (size_t)n, *q2 = y + 3 * (size_t)n;
-    const float *m1 = lo ? lo + 2 * (size_t)n : nullptr;
-    const float *m2 = lo ? lo + 3 * (size_t)n : nullptr;
-    // gamma^2 v^2 at one plaquette corner, VelocityKernel.hpp:82-96
-    auto corner = [&](int xx, int yy, int zz) -> double {
-        const double R = p.vel_R, RpR = p.vel_RpR;
-        const double psi1 = S.at(p1, l1, xx, yy, zz);
-        const double psi2 = S.at(p2, l2, xx, yy, zz);
-        const double Pi1 = S.at(q1, m1, xx, yy, zz);
-        const double Pi2 = S.at(q2, m2, xx, yy, zz);
-        return gamma_sq_v_sq_d(psi1 / R, psi2 / R, (Pi1 - RpR * psi1) / (R * R),
-                               (Pi2 - RpR * psi2) / (R * R), p.vel_mr);
-    };
 
     for (int z = z_begin; z < z_end; ++z)
     {
@@ -260,23 +278,24 @@ __global__ void __launch_bounds__(64 * WTY)
             {
                 if (p.do_vel && (wxy != 0 || wyz != 0 || wzx != 0))
                 {
-                    const double c000 = corner(gx, gy, z);
+                    // corners from the ring: s0 = plane z, s1 = plane z + 1
+                    const double c000 = gv[s0][ty][tx];
                     if (wxy != 0)
                     {
-                        vsum += c000 + corner(gx + 1, gy, z) +
-                                corner(gx + 1, gy + 1, z) + corner(gx, gy + 1, z);
+                        vsum += c000 + gv[s0][ty][tx + 1] +
+                                gv[s0][ty + 1][tx + 1] + gv[s0][ty + 1][tx];
                         vcount += 4;
                     }
                     if (wyz != 0)
                     {
-                        vsum += c000 + corner(gx, gy + 1, z) +
-                                corner(gx, gy + 1, z + 1) + corner(gx, gy, z + 1);
+                        vsum += c000 + gv[s0][ty + 1][tx] +
+                                gv[s1][ty + 1][tx] + gv[s1][ty][tx];
                         vcount += 4;
                     }
                     if (wzx != 0)
                     {
-                        vsum += c000 + corner(gx, gy, z + 1) +
-                                corner(gx + 1, gy, z + 1) + corner(gx + 1, gy, z);
+                        vsum += c000 + gv[s1][ty][tx] + gv[s1][ty][tx + 1] +
+                                gv[s0][ty][tx + 1];
                         vcount += 4;
                     }
                 }

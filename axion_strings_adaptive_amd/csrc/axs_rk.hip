@@ -538,9 +538,13 @@ static int fill_and_launch(axs_level *L, int mode, const void *yin, void *yout,
     p.nz = L->g.nz;
     p.n = L->g.ncell;
     p.inv_dx2 = 1.0 / (L->dx * L->dx);
+    // default: the 1024^3 autotuner choices (8-row tiles with centre-load
+    // prefetch; 4 cells per lane in FP32, 2 in FP64)
+    const int tuned = sizeof(T) == 8 ? 2 * kBaseVariants + 4
+                                     : 2 * kBaseVariants + 6;
     int variant = (L->tile_variant >= 0 && L->tile_variant < num_tile_variants())
                       ? L->tile_variant
-                      : 1;
+                      : tuned;
     const int group = variant / kBaseVariants;
     p.nontemporal = (group & 1) ? 1 : 0;
     const bool pf = (group & 2) != 0;
@@ -556,7 +560,7 @@ static int fill_and_launch(axs_level *L, int mode, const void *yin, void *yout,
     int zc = L->z_chunk;
     if (zc <= 0)
     {
-        zc = 64;
+        zc = 32;
         while (zc > 4 && (long)p.tiles_x * p.tiles_y * ((p.nz + zc - 1) / zc) <
                              4096)
             zc >>= 1;

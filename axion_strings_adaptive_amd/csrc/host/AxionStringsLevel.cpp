@@ -3,7 +3,12 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <ctime>
+#include <fstream>
 #include <iostream>
+#include <map>
+#include <sstream>
 #include <vector>
 
 namespace axs_host
@@ -94,6 +99,11 @@ AxionStringsLevel::AxionStringsLevel(const ParamTable &params,
         m_handoff_n_periods = Params::read_handoff_transition_n_periods(params);
         m_phase = Phase::Relaxing;
     }
+    amr.queryAdd("check_int", m_check_int);
+    amr.queryAdd("check_file", m_check_file);
+    amr.query("restart", m_restart);
+    if (!m_restart.empty() && !std::ifstream(m_restart).good())
+        amr.error("restart", "cannot open checkpoint " + m_restart);
     as.queryAdd("compute_spectrum", m_compute_spectrum);
     std::string field = "a_dot";
     as.queryAdd("spectrum_field", field);
@@ -139,6 +149,11 @@ AxionStringsLevel::~AxionStringsLevel()
 // AxionStringsLevel::initData, AxionStringsLevel.cpp:786-847
 void AxionStringsLevel::initData()
 {
+    if (!m_restart.empty())
+    {
+        read_checkpoint(m_restart);
+        return;
+    }
     std::string ic_mode = "homogeneous";
     ParmParse(m_params, "axion_strings").queryAdd("ic_mode", ic_mode);
     const bool relaxed = m_phase == Phase::Relaxing;
@@ -278,7 +293,7 @@ void AxionStringsLevel::evolve()
         evolve_relaxing();
     while (m_phase == Phase::Evolving)
     {
-        long k = steps_until_next_output();
+        long k = std::min(steps_until_next_output(), steps_to_checkpoint());
         k = clamp_steps(k, m_step, m_max_steps, m_a_time, m_stop_time, m_dt);
         if (k <= 0)
             break;
@@ -313,8 +328,17 @@ void AxionStringsLevel::evolve()
         m_step += k;
         if (output_due())
             diagnostics();
+        if (m_check_int > 0 && m_step % m_check_int == 0)
+            write_checkpoint();
     }
     axs_check(axs_ctx_synchronize(m_ctx), "synchronize");
+}
+
+long AxionStringsLevel::steps_to_checkpoint() const
+{
+    if (m_check_int <= 0)
+        return 100000000;
+    return m_check_int - m_step % m_check_int;
 }
 
 // The relaxing phase: RK4 steps on the pre-evolution background with its
@@ -324,9 +348,10 @@ void AxionStringsLevel::evolve_relaxing()
 {
     while (m_phase == Phase::Relaxing)
     {
-        const long want = m_xi_check_interval;
-        const long k = clamp_steps(want, m_step, m_max_steps, m_a_time,
-                                   m_stop_time, m_dt);
+        const long want = m_xi_check_interval - m_steps_since_check;
+        const long k = clamp_steps(std::min(want, steps_to_checkpoint()),
+                                   m_step, m_max_steps, m_a_time, m_stop_time,
+                                   m_dt);
         if (k <= 0)
             return;
         std::vector<axs_rhs_coeffs> table((size_t)k * 4);
@@ -350,9 +375,17 @@ void AxionStringsLevel::evolve_relaxing()
         for (long j = 0; j < k; ++j)
             m_a_time += m_dt;
         m_step += k;
-        if (k < want)
+        m_steps_since_check += k;
+        const bool checkpoint_due = m_check_int > 0 && m_step % m_check_int == 0;
+        if (m_steps_since_check >= m_xi_check_interval)
+        {
+            m_steps_since_check = 0;
+            relax_check();
+        }
+        if (checkpoint_due)
+            write_checkpoint();
+        if (m_steps_since_check > 0 && !checkpoint_due)
             return; // stopped by max_steps / stop_time
-        relax_check();
     }
 }
 
@@ -382,17 +415,13 @@ void AxionStringsLevel::relax_check()
               << "  radial/axion = " << rad_total / axion_total
               << "  rho_tot = " << e.rho_tot_unscreened << "\n";
     if (!m_pre_scalars)
-    {
-        m_pre_scalars = std::fopen(
-            (m_out_dir + "/pre_evolution_scalars.dat").c_str(), "w");
-        if (!m_pre_scalars)
-            throw AxsError("cannot write pre_evolution_scalars.dat");
         // column names of AxionStringsLevel.cpp:1122-1125 after the time
-        std::fprintf(m_pre_scalars,
-                     "# time tau_pre R_pre m_r_pre N_p xi xi_target "
-                     "rho_radial_kin rho_radial_grad rho_radial_mass "
-                     "rho_axion_kin rho_axion_grad rho_tot\n");
-    }
+        m_pre_scalars = open_dat("pre_evolution_scalars.dat",
+                                 "# time tau_pre R_pre m_r_pre N_p xi "
+                                 "xi_target rho_radial_kin rho_radial_grad "
+                                 "rho_radial_mass rho_axion_kin "
+                                 "rho_axion_grad rho_tot\n",
+                                 tau_pre);
     const double row[13] = {tau_pre,
                             tau_pre,
                             r_pre_now,
@@ -535,13 +564,8 @@ void AxionStringsLevel::diagnostics()
     const Spectrum s_uns =
         compute_spectrum(m_level, m_spectrum_kind, none, R, n_max);
     const double nsq = (double)n_cells * (double)n_cells;
-    const bool first = (m_spectrum == nullptr);
-    if (first)
-    {
-        m_spectrum = std::fopen((m_out_dir + "/axion_spectrum.dat").c_str(), "w");
-        if (!m_spectrum)
-            throw AxsError("cannot write axion_spectrum.dat");
-        std::fprintf(m_spectrum,
+    if (!m_spectrum)
+        m_spectrum = open_dat("axion_spectrum.dat",
                      "# tau mode_index k_over_H shell_average_screened "
                      "shell_average_unscreened full_cube_energy_screened "
                      "full_cube_energy_unscreened "
@@ -550,8 +574,7 @@ void AxionStringsLevel::diagnostics()
                      "real_space_mean_sq_screened real_space_mean_sq_unscreened "
                      "parseval_full_screened parseval_full_unscreened "
                      "parseval_inscribed_screened "
-                     "parseval_inscribed_unscreened\n");
-    }
+                     "parseval_inscribed_unscreened\n", tau);
     const double k_over_h_per_mode = H > 0.0 ? 2.0 * M_PI / (m_L * H) : 0.0;
     for (size_t i = 0; i < s_scr.shell_average.size(); ++i)
         std::fprintf(m_spectrum,
@@ -576,13 +599,8 @@ void AxionStringsLevel::write_network_scalars_row(
     double tension_tail)
 {
     if (!m_scalars)
-    {
-        m_scalars =
-            std::fopen((m_out_dir + "/network_scalars.dat").c_str(), "w");
-        if (!m_scalars)
-            throw AxsError("cannot write network_scalars.dat");
         // column names of AxionStringsLevel.cpp:1541-1551
-        std::fprintf(m_scalars,
+        m_scalars = open_dat("network_scalars.dat",
                      "# tau N_p N_p_weighted xi xi_weighted m_r_over_H "
                      "rho_tot_unscreened rho_tot_screened "
                      "rho_axion_kin_unscreened rho_axion_kin_screened "
@@ -592,8 +610,7 @@ void AxionStringsLevel::write_network_scalars_row(
                      "rho_radial_mass_unscreened rho_radial_mass_screened "
                      "n_total n_unmasked mean_gamma_sq_v_sq mean_gamma "
                      "n_velocity_corners tension_core_only "
-                     "tension_core_plus_tail\n");
-    }
+                     "tension_core_plus_tail\n", m_tau);
     const double wcount = (double)v.count * m_dx;
     const double mean_g2v2 = wcount > 0.0 ? v.sum_gamma_sq_v_sq / wcount : 0.0;
     const double row[25] = {m_tau,
@@ -625,6 +642,182 @@ void AxionStringsLevel::write_network_scalars_row(
         std::fprintf(m_scalars, i ? " %.10e" : "%.10e", row[i]);
     std::fprintf(m_scalars, "\n");
     std::fflush(m_scalars);
+}
+
+// A .dat file opened the way AxionStringsLevel.cpp:1510-1537 opens its
+// SmallDataIO files: a fresh run moves an existing file aside to
+// <name>.old.<seconds> and writes the header; a restarted run (amr.restart
+// set) keeps the file but drops its rows at or after the time t of the row
+// about to be written, i.e. the provisional rows of the segment being redone.
+FILE *AxionStringsLevel::open_dat(const std::string &name, const char *header,
+                                  double t)
+{
+    const std::string path = m_out_dir + "/" + name;
+    std::ifstream probe(path);
+    const bool exists = probe.good();
+    if (exists && !m_restart.empty())
+    {
+        std::vector<std::string> keep;
+        std::string line;
+        const double eps = 1e-8 * std::max(1.0, std::fabs(t));
+        while (std::getline(probe, line))
+        {
+            if (line.empty())
+                continue;
+            if (line[0] == '#')
+            {
+                keep.push_back(line);
+                continue;
+            }
+            const double ti = std::strtod(line.c_str(), nullptr);
+            if (ti < t - eps)
+                keep.push_back(line);
+        }
+        probe.close();
+        FILE *f = std::fopen(path.c_str(), "w");
+        if (!f)
+            throw AxsError("cannot write " + name);
+        for (const std::string &l : keep)
+            std::fprintf(f, "%s\n", l.c_str());
+        std::fflush(f);
+        return f;
+    }
+    probe.close();
+    if (exists)
+    {
+        const std::string aside =
+            path + ".old." + std::to_string((long long)std::time(nullptr));
+        std::rename(path.c_str(), aside.c_str());
+    }
+    FILE *f = std::fopen(path.c_str(), "w");
+    if (!f)
+        throw AxsError("cannot write " + name);
+    std::fputs(header, f);
+    return f;
+}
+
+// Checkpoint: a text header of key = value lines (doubles with 17
+// significant digits, so a restart continues bit for bit), then the raw
+// stored words of the state (FP32C: hi, then lo).
+void AxionStringsLevel::write_checkpoint()
+{
+    char num[32];
+    std::snprintf(num, sizeof num, "%05ld", m_step);
+    const std::string path = m_out_dir + "/" + m_check_file + num + ".axs";
+    FILE *f = std::fopen(path.c_str(), "wb");
+    if (!f)
+        throw AxsError("cannot write checkpoint " + path);
+    auto put = [&](const char *k, double v) {
+        std::fprintf(f, "%s = %.17g\n", k, v);
+    };
+    std::fprintf(f, "axion_strings checkpoint 1\n");
+    put("nx", m_n[0]);
+    put("ny", m_n[1]);
+    put("nz", m_n[2]);
+    put("dx", m_dx);
+    put("precision", m_precision);
+    put("step", (double)m_step);
+    put("tau", m_tau);
+    put("a_time", m_a_time);
+    put("tau_offset", m_tau_offset);
+    put("phase", m_phase == Phase::Relaxing ? 0 : 1);
+    put("next_log", m_next_log);
+    put("next_moore", m_next_moore_range);
+    put("moore_init", m_moore_init ? 1 : 0);
+    put("H_switch", m_moore_H_switch);
+    put("xi_check_interval", (double)m_xi_check_interval);
+    put("steps_since_check", (double)m_steps_since_check);
+    put("handoff_lambda_start", m_handoff_lambda_start);
+    put("handoff_curv_start", m_handoff_curv_start);
+    put("handoff_tau_start", m_handoff_tau_start);
+    put("handoff_dtau", m_handoff_dtau);
+    std::fprintf(f, "end\n");
+    const size_t n4 = (size_t)4 * m_n[0] * m_n[1] * m_n[2];
+    const size_t word = m_precision == AXS_FP64 ? 8 : 4;
+    std::vector<char> hi(n4 * word);
+    std::vector<float> lo(m_precision == AXS_FP32C ? n4 : 0);
+    axs_check(axs_level_download_raw(m_level, hi.data(),
+                                     lo.empty() ? nullptr : lo.data()),
+              "axs_level_download_raw");
+    bool ok = std::fwrite(hi.data(), 1, hi.size(), f) == hi.size();
+    if (!lo.empty())
+        ok = ok && std::fwrite(lo.data(), sizeof(float), lo.size(), f) ==
+                       lo.size();
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok)
+        throw AxsError("short write of checkpoint " + path);
+    std::cout << "  [AxionStrings] checkpoint " << path << "\n";
+}
+
+void AxionStringsLevel::read_checkpoint(const std::string &path)
+{
+    FILE *f = std::fopen(path.c_str(), "rb");
+    if (!f)
+        throw ParamError("amr.restart: cannot open " + path);
+    std::map<std::string, double> kv;
+    char line[256];
+    if (!std::fgets(line, sizeof line, f) ||
+        std::strncmp(line, "axion_strings checkpoint 1", 26) != 0)
+    {
+        std::fclose(f);
+        throw ParamError("amr.restart: " + path + " is not a checkpoint");
+    }
+    while (std::fgets(line, sizeof line, f))
+    {
+        if (std::strncmp(line, "end", 3) == 0)
+            break;
+        char key[128];
+        double v = 0.0;
+        if (std::sscanf(line, "%127s = %lf", key, &v) == 2)
+            kv[key] = v;
+    }
+    auto get = [&](const char *k) -> double {
+        auto it = kv.find(k);
+        if (it == kv.end())
+        {
+            std::fclose(f);
+            throw ParamError(std::string("amr.restart: checkpoint lacks ") + k);
+        }
+        return it->second;
+    };
+    if ((int)get("nx") != m_n[0] || (int)get("ny") != m_n[1] ||
+        (int)get("nz") != m_n[2] || (int)get("precision") != m_precision)
+    {
+        std::fclose(f);
+        throw ParamError("amr.restart: checkpoint grid or precision differs "
+                         "from amr.n_cell / axion_strings.precision");
+    }
+    m_step = (long)get("step");
+    m_tau = get("tau");
+    m_a_time = get("a_time");
+    m_tau_offset = get("tau_offset");
+    m_phase = get("phase") == 0 ? Phase::Relaxing : Phase::Evolving;
+    m_next_log = get("next_log");
+    m_next_moore_range = get("next_moore");
+    m_moore_init = get("moore_init") != 0;
+    m_moore_H_switch = get("H_switch");
+    m_xi_check_interval = (long)get("xi_check_interval");
+    m_steps_since_check = (long)get("steps_since_check");
+    m_handoff_lambda_start = get("handoff_lambda_start");
+    m_handoff_curv_start = get("handoff_curv_start");
+    m_handoff_tau_start = get("handoff_tau_start");
+    m_handoff_dtau = get("handoff_dtau");
+    const size_t n4 = (size_t)4 * m_n[0] * m_n[1] * m_n[2];
+    const size_t word = m_precision == AXS_FP64 ? 8 : 4;
+    std::vector<char> hi(n4 * word);
+    std::vector<float> lo(m_precision == AXS_FP32C ? n4 : 0);
+    bool ok = std::fread(hi.data(), 1, hi.size(), f) == hi.size();
+    if (!lo.empty())
+        ok = ok && std::fread(lo.data(), sizeof(float), lo.size(), f) ==
+                       lo.size();
+    std::fclose(f);
+    if (!ok)
+        throw ParamError("amr.restart: " + path + " is truncated");
+    axs_check(axs_level_upload_raw(m_level, hi.data(),
+                                   lo.empty() ? nullptr : lo.data()),
+              "axs_level_upload_raw");
+    std::cout << "  [AxionStrings] restart from " << path << " at step "
+              << m_step << ", tau = " << m_tau << "\n";
 }
 
 } // namespace axs_host

@@ -11,6 +11,11 @@
 //                                  blend; here one device table)
 //   relaxing-phase xi check :1044-1191 (pre_evolution_scalars.dat)
 //   apply_pre_evolution_to_main_rescale :2027-2125
+//   restart-safe .dat output :1510-1535 (SmallDataIO: a fresh run moves
+//                                  an existing file aside, a restart
+//                                  drops the rows of the redone segment)
+// plus checkpoints of the level state (amr.check_int / amr.check_file /
+// amr.restart, the keys of the reference's AMReX driver).
 //   specific_post_timestep :923-1914  (log(m_r/H) cadence :1198-1248,
 //                                  counts/xi/energy/velocity :1299-1361,
 //                                  network_scalars.dat :1536-1579,
@@ -50,7 +55,12 @@ class AxionStringsLevel
     };
     Phase phase() const { return m_phase; }
 
+    void write_checkpoint();
+
   private:
+    void read_checkpoint(const std::string &path);
+    FILE *open_dat(const std::string &name, const char *header, double t);
+    long steps_to_checkpoint() const;
     void evolve_relaxing();
     void relax_check();
     void apply_pre_evolution_to_main_rescale();
@@ -94,11 +104,16 @@ class AxionStringsLevel
     double m_xi_target{};
     XiCheckCadence m_xi_cadence;
     long m_xi_check_interval{50};
+    long m_steps_since_check{0};
     double m_handoff_n_periods{0.0};
     double m_a_time{0.0};     // the level clock: tau_pre, then on through
     double m_tau_offset{0.0}; // tau = m_tau_offset + m_a_time (main phase)
     double m_handoff_lambda_start{}, m_handoff_curv_start{};
     double m_handoff_tau_start{}, m_handoff_dtau{0.0};
+    // checkpoint / restart
+    long m_check_int{0};
+    std::string m_check_file{"chk"};
+    std::string m_restart;
 };
 
 } // namespace axs_host

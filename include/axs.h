@@ -137,6 +137,9 @@ int axs_level_download(axs_level *lvl, double *host_state);
 /* Download of the raw stored words: FP64 -> 4n doubles; FP32/FP32C -> 4n
  * floats of hi (and, for FP32C, 4n floats of lo into `lo` if non-NULL). */
 int axs_level_download_raw(axs_level *lvl, void *hi, void *lo);
+/* The inverse of axs_level_download_raw (exact restore of a checkpoint,
+ * the role of AmrLevel::restart for the state data): FP32C needs `lo`. */
+int axs_level_upload_raw(axs_level *lvl, const void *hi, const void *lo);
 /* Device pointers of the state (y_n) and compensation arrays. */
 int axs_level_device_ptrs(axs_level *lvl, void **state, void **comp);
 /* Homogeneous exact solution psi = (R, 0), Pi = (R', 0)
@@ -149,7 +152,10 @@ int axs_level_set_homogeneous(axs_level *lvl, double R, double R_prime);
  * and common_term = R_main' - kappa^2 R_pre' computed by the caller. */
 int axs_level_handoff_rescale(axs_level *lvl, double kappa,
                               double common_term, double R_pre);
-/* Selects the stage-kernel tile variant (autotuner output); -1 = default. */
+/* Selects the stage-kernel tile variant (autotuner output); -1 = default
+ * (the 1024^3 autotuner choice: 8-row tiles, 4 cells per lane in FP32 and
+ * 2 in FP64, centre loads one plane ahead; z-chunk 32, halved on small
+ * grids until there are 4096 tiles); z_chunk 0 = default. */
 int axs_level_set_tile(axs_level *lvl, int variant, int z_chunk);
 int axs_level_get_tile(axs_level *lvl, int *variant, int *z_chunk);
 
