@@ -54,6 +54,28 @@ def blocks_from_tags(tags: np.ndarray, block: int) -> np.ndarray:
     return t.any(axis=(1, 3, 5))
 
 
+def tagged_blocks(tags: np.ndarray, block: int, r: int) -> np.ndarray:
+    """blocks_from_tags(dilate_periodic(tags, r), block) without forming the
+    dilated cell mask: dilation by a cube and "any over a block" both act
+    axis by axis, so each axis is reduced to blocks at once, a block taking
+    its own cells plus the r cells on either side of it (periodic, r <=
+    block).  One pass over the cell mask, then passes over arrays that are
+    block times smaller."""
+    assert 0 <= r <= block
+    out = tags
+    for ax in (2, 1, 0):  # x, then y, then z ([z, y, x] arrays)
+        a = np.moveaxis(out, ax, -1)
+        nb = a.shape[-1] // block
+        a = a.reshape(a.shape[:-1] + (nb, block))
+        red = a.any(axis=-1)
+        if r > 0:
+            head = a[..., :r].any(axis=-1)   # first r cells of each block
+            tail = a[..., -r:].any(axis=-1)  # last r cells of each block
+            red = red | np.roll(head, -1, axis=-1) | np.roll(tail, 1, axis=-1)
+        out = np.moveaxis(red, -1, ax)
+    return np.ascontiguousarray(out)
+
+
 def boxes_from_blocks(blk: np.ndarray, fine_block: int, max_blocks: int):
     """Fine-level boxes (ratio 2) covering the tagged coarse blocks: runs of
     up to max_blocks consecutive blocks along x, runs with the same x extent
@@ -132,6 +154,7 @@ class Hierarchy:
         self.max_grid_size = max_grid_size
         self.nsteps = 0
         self.nregrids = 0
+        self.regrid_s = 0.0  # wall time spent in regrid() by step()
         self.bg = bg if bg is not None else capi.make_background(
             2.0, c0=0.0, c1=2.0, tau_switch=1.0, has_switch=True)
         self.tau = tau0
@@ -187,9 +210,8 @@ class Hierarchy:
         tags = tags_buf.to_numpy(np.uint8).reshape(cd[2], cd[1], cd[0]) > 0
         if self.force_full_refinement:  # axion_strings.tagging.force_full_refinement
             tags[:] = True
-        tags = dilate_periodic(tags, self.n_error_buf)
         cblock = self.blocking_factor // 2   # coarse cells per fine box edge
-        blk = blocks_from_tags(tags, cblock)
+        blk = tagged_blocks(tags, cblock, self.n_error_buf)
         if l >= 1:  # proper nesting: one block inside the coarse coverage
             blk &= erode_periodic(covered, 1)
         boxes = boxes_from_blocks(blk, self.blocking_factor,
@@ -253,7 +275,11 @@ class Hierarchy:
         self.tau += self.dt0
         self.nsteps += 1
         if self.regrid_int > 0 and self.nsteps % self.regrid_int == 0:
+            self.ctx.synchronize()
+            t0 = time.perf_counter()
             self.regrid()
+            self.ctx.synchronize()
+            self.regrid_s += time.perf_counter() - t0
 
     def cell_updates_per_coarse_step(self) -> int:
         return sum(n * 2 ** l for l, n in enumerate(self.ncells))
@@ -296,6 +322,7 @@ def main(argv=None):
                   regrid_int=args.regrid_int)
     h.step()  # warm-up
     ctx.synchronize()
+    h.regrid_s, n_regrid0 = 0.0, h.nregrids
     t0 = time.perf_counter()
     for _ in range(args.steps):
         h.step()
@@ -308,6 +335,10 @@ def main(argv=None):
         "cell_updates_per_coarse_step": h.cell_updates_per_coarse_step(),
         "cell_updates_per_s": h.cell_updates_per_coarse_step() * args.steps / dt,
         "ms_per_coarse_step": 1e3 * dt / args.steps,
+        "regrid_ms_mean": (1e3 * h.regrid_s / (h.nregrids - n_regrid0)
+                           if h.nregrids > n_regrid0 else None),
+        "ms_per_coarse_step_without_regrids":
+            1e3 * (dt - h.regrid_s) / args.steps,
         "n_total": e["n_total"], "L_tilde^3": (args.n0 * 0.5) ** 3,
         "rho_tot_unscreened": e["rho_tot_unscreened"]}))
     h.close()

@@ -1,11 +1,12 @@
 // axs_diag.hip -- string finding, tagging and energy reductions on a uniform
 // periodic level.
 //
-// Windings (StringFinder.hpp:45-122): every vertex phase atan2(psi2, psi1)
-// is evaluated once, in FP64 on the FP64-promoted state exactly like the
-// reference (StringFinder.hpp:49-53), into a 3-deep ring of LDS planes;
-// each cell then reads the 7 corner phases of its 3 low-corner plaquettes
-// from LDS.  Pierced-plaquette counts are formed per wavefront with
+// Windings (StringFinder.hpp:45-122): psi (FP64, FP32C promoted as hi +
+// lo) of a z plane sits in a 3-deep ring of LDS planes; each cell reads the
+// 7 corners of its 3 low-corner plaquettes from it and forms the windings
+// from sign tests that provably agree with the reference's FP64 atan2 phase
+// arithmetic (StringFinder.hpp:49-53), falling back to that arithmetic for
+// the near-antipodal edges where the two could differ.  Pierced-plaquette counts are formed per wavefront with
 // __ballot / __popcll and one 64-bit integer atomic per wavefront, so the
 // counts are exact and order independent.
 //
@@ -16,6 +17,7 @@
 
 #include <cfloat>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 namespace axs
@@ -143,17 +145,88 @@ enum
     WM_ALL = 15
 };
 
-// Each workgroup owns a 64 x WTY column tile and streams a chunk of z: the
-// phases atan2(psi2, psi1) of a plane (tile + one halo column / row) go
-// into a 3-deep LDS ring, each computed once per vertex; the raw psi of the
-// plane after next is loaded into registers one plane ahead, so its memory
-// latency overlaps the windings of the current plane.
-template <typename T, int WM>
+// Winding of an edge a -> b, without the phases.  With theta = atan2(y, x)
+// in [-pi, pi], the wrapped difference of StringFinder.hpp's
+// plaquette_winding is (theta_b - theta_a) - 2 pi k, k = +1 when
+// theta_b - theta_a > pi, -1 when it is <= -pi, else 0; round the sum over a
+// closed plaquette the raw differences cancel, so the winding is -sum(k).
+// k != 0 only when a and b lie on opposite sides of the x axis and the
+// short arc between them crosses the negative x axis: k = +1 for a below, b
+// above and a clockwise turn (cross(a, b) < 0), k = -1 for the mirror case.
+// These sign tests agree with the FP64 phase arithmetic unless the edge is
+// within ~1e-9 rad of antipodal: elsewhere the computed cross product has
+// the sign of the exact one (its rounding error is ~1e-16 |a||b|) and the
+// phase difference is >~1e-9 away from +-pi, far beyond the ~1e-16 rad
+// error of atan2.  Near-antipodal edges (dot < 0 and |cross| <= 1e-9
+// (|a.x| + |a.y|)(|b.x| + |b.y|)), vertices on the x axis (y == +-0, where
+// the sign of zero picks the phase +-pi) and non-finite or extreme values
+// are flagged, and the caller then evaluates the plaquettes with atan2
+// exactly as the reference does.
+struct Vtx
+{
+    double x, y;
+};
+
+__device__ __forceinline__ bool vtx_ok(const Vtx &v)
+{
+    const double m = fabs(v.x) + fabs(v.y);
+    return v.y != 0.0 && m >= 1e-140 && m <= 1e140;
+}
+
+__device__ __forceinline__ int edge_k(const Vtx &a, const Vtx &b, bool &amb)
+{
+#pragma clang fp contract(off)
+    // branch free: every lane evaluates the same instructions
+    const bool sa = a.y < 0.0, sb = b.y < 0.0;
+    const double d = a.x * b.x + a.y * b.y;
+    const double c = a.x * b.y - a.y * b.x;
+    const double m = (fabs(a.x) + fabs(a.y)) * (fabs(b.x) + fabs(b.y));
+    amb = amb || (d < 0.0 && !(fabs(c) > 1e-9 * m));
+    const int up = (sa && !sb && c < 0.0) ? 1 : 0;
+    const int down = (!sa && sb && c > 0.0) ? 1 : 0;
+    return up - down;
+}
+
+// The reference's phase arithmetic for the 3 plaquettes of a cell (flagged
+// cells only).  Inlined: as a call, the values live across it would be kept
+// in scratch memory on the common path.
+__device__ __forceinline__ void windings_by_phase(const Vtx *v, int *w)
+{
+    // v: 000, 100, 110, 010, 001, 101, 011
+    double t[7];
+    for (int i = 0; i < 7; ++i)
+        t[i] = atan2(v[i].y, v[i].x);
+    w[0] = plaquette_winding_d(t[0], t[1], t[2], t[3]);
+    w[1] = plaquette_winding_d(t[0], t[3], t[6], t[4]);
+    w[2] = plaquette_winding_d(t[0], t[4], t[5], t[1]);
+}
+
+// Each workgroup owns a 64 x WTY column tile and streams a chunk of z: psi
+// (and Pi for the velocity pass; FP64, FP32C as hi + lo) of a plane -- the
+// tile plus one halo column and row -- goes into a 3-deep LDS ring; each
+// cell reads the 7 corners of its 3 low-corner plaquettes from it and forms
+// the windings from the edge sign tests above (atan2 only for flagged
+// cells).  The stored words of the next WD planes are in flight in
+// registers while a plane is processed (each thread loads its own cell and
+// at most one halo point per plane), so enough bytes are outstanding to
+// stream at HBM rate; velocities read the Pi of pierced corners from the
+// ring.
+template <typename T, int WM, bool HASLO>
 __global__ void __launch_bounds__(64 * WTY)
+    __attribute__((amdgpu_waves_per_eu((WM & ~WM_FIELD) == 0 ? 4 : 1)))
     k_windings(const T *y, const float *lo, WindParams p)
 {
     constexpr int TX = 64;
-    __shared__ double th[3][WTY + 1][TX + 1];
+    constexpr bool VEL = (WM & WM_VEL) != 0;
+    constexpr int NC = VEL ? 4 : 2; // components staged per point
+    constexpr int WD = VEL ? 2 : 3; // planes of loads in flight
+    __shared__ double ring[NC][3][WTY + 1][TX + 1];
+    // per vertex: k + 1 of its +x edge (bits 0-1) and +y edge (bits 2-3) in
+    // the plane, flags of those edges (bits 4, 5) and of the vertex (bit 6);
+    // planes z and z + 1.  Per vertex of plane z: k + 1 of its +z edge
+    // (bits 0-1) and that edge's flag (bit 2).
+    __shared__ uint32_t pw[2][WTY + 1][TX + 1];
+    __shared__ uint32_t zw[WTY + 1][TX + 1];
 
     const int bid = blockIdx.x;
     const int bx = bid % p.tiles_x;
@@ -161,87 +234,85 @@ __global__ void __launch_bounds__(64 * WTY)
     const int by = rem % p.tiles_y;
     const int bz = rem / p.tiles_y;
     const int tx = threadIdx.x, ty = threadIdx.y;
+    const int t = ty * TX + tx;
     const int nx = p.nx, ny = p.ny, nz = p.nz;
     const uint32_t n = p.n;
     const uint32_t plane = (uint32_t)nx * ny;
     const int gx = bx * TX + tx, gy = by * WTY + ty;
     const bool active = gx < nx && gy < ny;
-    const int xw = gx % nx, yw = gy % ny;
-    const int xh = (bx * TX + TX) % nx;         // x halo column
-    const int yh = (by * WTY + WTY) % ny;       // y halo row
+    const int xh = (bx * TX + TX) % nx;   // x halo column
+    const int yh = (by * WTY + WTY) % ny; // y halo row
+    const uint32_t own = (uint32_t)(gy % ny) * nx + (uint32_t)(gx % nx);
+    // the one halo point this thread stages: the y halo row, then the x
+    // halo column, then the corner
+    const bool has_halo = t < TX + WTY + 1;
+    int hr = WTY, hc = TX;
+    uint32_t hoff = 0;
+    if (t < TX)
+    {
+        hc = t;
+        hoff = (uint32_t)yh * nx + (uint32_t)((bx * TX + t) % nx);
+    }
+    else if (t < TX + WTY)
+    {
+        hr = t - TX;
+        hoff = (uint32_t)((by * WTY + hr) % ny) * nx + (uint32_t)xh;
+    }
+    else if (has_halo)
+        hoff = (uint32_t)yh * nx + (uint32_t)xh;
+    else
+        hoff = own; // loaded but not stored: keeps the load unconditional
     const T *p1 = y;
     const T *p2 = y + n;
     const float *l1 = lo, *l2 = lo ? lo + n : nullptr;
-    const bool hx = tx == 0, hy = ty == 0;
 
-    // raw psi of the points this thread stages: own cell, x halo, y halo,
-    // corner
-    double r[4][4]; // psi1, psi2 (+ Pi1, Pi2 for the velocities)
-    // gamma^2 v^2 of every vertex of the ring planes (velocity passes)
-    __shared__ double gv[(WM & WM_VEL) ? 3 : 1][WTY + 1][TX + 1];
-    const T *q1 = y + 2 * (size_t)n, *q2 = y + 3 * (size_t)n;
-    const float *m1 = lo ? lo + 2 * (size_t)n : nullptr;
-    const float *m2 = lo ? lo + 3 * (size_t)n : nullptr;
-    auto raw = [&](int xx, int yy, int zz, double(&v)[4]) {
-        const uint32_t i = (uint32_t)zz * plane + (uint32_t)yy * nx + xx;
-        v[0] = ld(p1, l1, i);
-        v[1] = ld(p2, l2, i);
-        if constexpr ((WM & WM_VEL) != 0)
+    T oh[WD][NC], hh[WD][NC];
+    float ol[WD][NC], hl[WD][NC];
+    auto issue = [&](int q, int zz) {
+        const uint32_t oz = (uint32_t)zz * plane;
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
         {
-            v[2] = ld(q1, m1, i);
-            v[3] = ld(q2, m2, i);
+            // unconditional loads: a conditional one would send the
+            // register queue to scratch memory
+            oh[q][c] = y[(size_t)c * n + oz + own];
+            hh[q][c] = y[(size_t)c * n + oz + hoff];
+            if constexpr (HASLO)
+            {
+                ol[q][c] = lo[(size_t)c * n + oz + own];
+                hl[q][c] = lo[(size_t)c * n + oz + hoff];
+            }
+        }
+    };
+    auto store = [&](int q, int s) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+        {
+            double v = (double)oh[q][c], w = (double)hh[q][c];
+            if constexpr (HASLO)
+            {
+                v += (double)ol[q][c];
+                w += (double)hl[q][c];
+            }
+            ring[c][s][ty][tx] = v;
+            if (has_halo)
+                ring[c][s][hr][hc] = w;
         }
     };
     // gamma^2 v^2 at one plaquette corner, VelocityKernel.hpp:82-96
-    auto corner_of = [&](const double(&v)[4]) -> double {
+    auto corner = [&](int s, int oy, int ox) -> double {
+        const double v0 = ring[0][s][ty + oy][tx + ox];
+        const double v1 = ring[1][s][ty + oy][tx + ox];
+        const double v2 = ring[NC - 2][s][ty + oy][tx + ox];
+        const double v3 = ring[NC - 1][s][ty + oy][tx + ox];
         const double R = p.vel_R, RpR = p.vel_RpR;
-        return gamma_sq_v_sq_d(v[0] / R, v[1] / R, (v[2] - RpR * v[0]) / (R * R),
-                               (v[3] - RpR * v[1]) / (R * R), p.vel_mr);
-    };
-    auto load_plane = [&](int zz) {
-        raw(xw, yw, zz, r[0]);
-        if (hx)
-            raw(xh, yw, zz, r[1]);
-        if (hy)
-        {
-            raw(xw, yh, zz, r[2]);
-            if (hx)
-                raw(xh, yh, zz, r[3]);
-        }
-    };
-    auto store_plane = [&](int s) {
-        th[s][ty][tx] = atan2(r[0][1], r[0][0]);
-        if (hx)
-            th[s][ty][TX] = atan2(r[1][1], r[1][0]);
-        if (hy)
-        {
-            th[s][WTY][tx] = atan2(r[2][1], r[2][0]);
-            if (hx)
-                th[s][WTY][TX] = atan2(r[3][1], r[3][0]);
-        }
-        if constexpr ((WM & WM_VEL) != 0)
-        {
-            if (p.do_vel)
-            {
-                gv[s][ty][tx] = corner_of(r[0]);
-                if (hx)
-                    gv[s][ty][TX] = corner_of(r[1]);
-                if (hy)
-                {
-                    gv[s][WTY][tx] = corner_of(r[2]);
-                    if (hx)
-                        gv[s][WTY][TX] = corner_of(r[3]);
-                }
-            }
-        }
+        return gamma_sq_v_sq_d(v0 / R, v1 / R, (v2 - RpR * v0) / (R * R),
+                               (v3 - RpR * v1) / (R * R), p.vel_mr);
     };
 
     const int z_begin = bz * p.z_chunk;
     const int z_end = min(z_begin + p.z_chunk, nz);
-    load_plane(z_begin);
-    store_plane(z_begin % 3);
-    load_plane(wrapc(z_begin + 1, nz));
-
+    const int len = z_end - z_begin;
     unsigned long long plain_w = 0; // per wavefront (uniform)
     unsigned long long weighted = 0; // per lane
     unsigned long long vcount = 0;   // per lane
@@ -249,13 +320,42 @@ __global__ void __launch_bounds__(64 * WTY)
     const StarLoader S{nx, ny, nz, plane};
     const double i1 = 1.0 / p.dx, i2 = 1.0 / (p.dx * p.dx);
 
-    for (int z = z_begin; z < z_end; ++z)
-    {
+    auto vtx = [&](int s, int r, int c) -> Vtx {
+        return Vtx{ring[0][s][r][c], ring[1][s][r][c]};
+    };
+    // Edge words of plane zz (ring slot s1; its in-plane +x / +y edges and
+    // vertex flags) and, with s0 >= 0, the +z edges from plane slot s0 to
+    // s1.  Each edge is evaluated once, by the thread of its low vertex;
+    // the first WTY + TX + 1 threads also cover the halo column / row.
+    auto edges = [&](int zz, int s0, int s1) {
+        const int pq = zz & 1;
+        auto put = [&](int r, int c, bool wx, bool wy) {
+            const Vtx a = vtx(s1, r, c);
+            bool fx = false, fy = false, fz = false;
+            const int kx = wx ? edge_k(a, vtx(s1, r, c + 1), fx) : 0;
+            const int ky = wy ? edge_k(a, vtx(s1, r + 1, c), fy) : 0;
+            pw[pq][r][c] = (uint32_t)(kx + 1) | (uint32_t)(ky + 1) << 2 |
+                           (uint32_t)fx << 4 | (uint32_t)fy << 5 |
+                           (uint32_t)!vtx_ok(a) << 6;
+            if (s0 >= 0)
+            {
+                const int kz = edge_k(vtx(s0, r, c), a, fz);
+                zw[r][c] = (uint32_t)(kz + 1) | (uint32_t)fz << 2;
+            }
+        };
+        put(ty, tx, true, true);
+        if (t < TX)
+            put(WTY, t, true, false);
+        else if (t < TX + WTY)
+            put(t - TX, TX, false, true);
+        else if (t == TX + WTY)
+            put(WTY, TX, false, false);
+    };
+    auto kx_of = [](uint32_t w) { return (int)(w & 3u) - 1; };
+    auto ky_of = [](uint32_t w) { return (int)((w >> 2) & 3u) - 1; };
+
+    auto process = [&](int z) {
         const int s0 = z % 3, s1 = (z + 1) % 3;
-        store_plane(s1);
-        if (z + 1 < z_end)
-            load_plane(wrapc(z + 2, nz));
-        __syncthreads();
         int wxy = 0, wyz = 0, wzx = 0;
         bool in = active;
         const uint32_t id = (uint32_t)z * plane + (uint32_t)gy * nx + gx;
@@ -263,39 +363,64 @@ __global__ void __launch_bounds__(64 * WTY)
             in = false;
         if (in)
         {
-            const double t000 = th[s0][ty][tx];
-            const double t100 = th[s0][ty][tx + 1];
-            const double t110 = th[s0][ty + 1][tx + 1];
-            const double t010 = th[s0][ty + 1][tx];
-            const double t001 = th[s1][ty][tx];
-            const double t101 = th[s1][ty][tx + 1];
-            const double t011 = th[s1][ty + 1][tx];
-            wxy = plaquette_winding_d(t000, t100, t110, t010);
-            wyz = plaquette_winding_d(t000, t010, t011, t001);
-            wzx = plaquette_winding_d(t000, t001, t101, t100);
+            const int c0 = z & 1, c1 = c0 ^ 1;
+            const uint32_t P00 = pw[c0][ty][tx], P10 = pw[c0][ty][tx + 1];
+            const uint32_t P01 = pw[c0][ty + 1][tx], P11 = pw[c0][ty + 1][tx + 1];
+            const uint32_t Q00 = pw[c1][ty][tx], Q10 = pw[c1][ty][tx + 1];
+            const uint32_t Q01 = pw[c1][ty + 1][tx];
+            const uint32_t Z00 = zw[ty][tx], Z10 = zw[ty][tx + 1];
+            const uint32_t Z01 = zw[ty + 1][tx];
+            // the 9 edges and 7 vertices of the cell's 3 plaquettes
+            const uint32_t flags = (P00 & 0x70u) | (P10 & 0x60u) |
+                                   (P01 & 0x50u) | (P11 & 0x40u) |
+                                   (Q00 & 0x70u) | (Q10 & 0x40u) |
+                                   (Q01 & 0x40u) | ((Z00 | Z10 | Z01) & 4u);
+            if (flags == 0)
+            {
+                // plaquette_winding(t000, t100, t110, t010) etc. as -sum(k)
+                const int kz000 = kx_of(Z00), kz100 = kx_of(Z10);
+                const int kz010 = kx_of(Z01);
+                wxy = -(kx_of(P00) + ky_of(P10) - kx_of(P01) - ky_of(P00));
+                wyz = -(ky_of(P00) + kz010 - ky_of(Q00) - kz000);
+                wzx = -(kz000 + kx_of(Q00) - kz100 - kx_of(P00));
+            }
+            else
+            {
+                const Vtx v000 = vtx(s0, ty, tx), v100 = vtx(s0, ty, tx + 1);
+                const Vtx v110 = vtx(s0, ty + 1, tx + 1);
+                const Vtx v010 = vtx(s0, ty + 1, tx);
+                const Vtx v001 = vtx(s1, ty, tx), v101 = vtx(s1, ty, tx + 1);
+                const Vtx v011 = vtx(s1, ty + 1, tx);
+                const Vtx v[7] = {v000, v100, v110, v010, v001, v101, v011};
+                int w[3];
+                windings_by_phase(v, w);
+                wxy = w[0];
+                wyz = w[1];
+                wzx = w[2];
+            }
             weighted += (unsigned long long)(abs(wxy) + abs(wyz) + abs(wzx));
-            if constexpr ((WM & WM_VEL) != 0)
+            if constexpr (VEL)
             {
                 if (p.do_vel && (wxy != 0 || wyz != 0 || wzx != 0))
                 {
-                    // corners from the ring: s0 = plane z, s1 = plane z + 1
-                    const double c000 = gv[s0][ty][tx];
+                    // corners: s0 = plane z, s1 = plane z + 1
+                    const double c000 = corner(s0, 0, 0);
+                    const double c100 = corner(s0, 0, 1);
+                    const double c010 = corner(s0, 1, 0);
+                    const double c001 = corner(s1, 0, 0);
                     if (wxy != 0)
                     {
-                        vsum += c000 + gv[s0][ty][tx + 1] +
-                                gv[s0][ty + 1][tx + 1] + gv[s0][ty + 1][tx];
+                        vsum += c000 + c100 + corner(s0, 1, 1) + c010;
                         vcount += 4;
                     }
                     if (wyz != 0)
                     {
-                        vsum += c000 + gv[s0][ty + 1][tx] +
-                                gv[s1][ty + 1][tx] + gv[s1][ty][tx];
+                        vsum += c000 + c010 + corner(s1, 1, 0) + c001;
                         vcount += 4;
                     }
                     if (wzx != 0)
                     {
-                        vsum += c000 + gv[s1][ty][tx] + gv[s1][ty][tx + 1] +
-                                gv[s0][ty][tx + 1];
+                        vsum += c000 + c001 + corner(s1, 0, 1) + c100;
                         vcount += 4;
                     }
                 }
@@ -385,6 +510,36 @@ __global__ void __launch_bounds__(64 * WTY)
         // pierced plaquettes of this wavefront, by ballot / popcount
         plain_w += __popcll(__ballot(wxy != 0)) + __popcll(__ballot(wyz != 0)) +
                    __popcll(__ballot(wzx != 0));
+    };
+
+    // plane rel (0 .. len, the last one the upper neighbour of the chunk)
+    // sits in register slot rel % WD until it is stored to the ring
+    issue(0, z_begin);
+    store(0, z_begin % 3);
+#pragma unroll
+    for (int j = 1; j <= WD; ++j)
+        if (j <= len)
+            issue(j % WD, wrapc(z_begin + j, nz));
+    __syncthreads();
+    edges(z_begin, -1, z_begin % 3);
+    for (int k = 0; k < len; k += WD)
+    {
+#pragma unroll
+        for (int u = 0; u < WD; ++u)
+        {
+            const int rel = k + u;
+            if (rel < len)
+            {
+                const int z = z_begin + rel;
+                store((u + 1) % WD, (z + 1) % 3);
+                if (rel + 1 + WD <= len)
+                    issue((u + 1) % WD, wrapc(z + 1 + WD, nz));
+                __syncthreads();
+                edges(z + 1, z % 3, (z + 1) % 3);
+                __syncthreads();
+                process(z);
+            }
+        }
     }
     // wavefront reduction of the per-lane counts
     for (int o = 32; o > 0; o >>= 1)
@@ -393,18 +548,28 @@ __global__ void __launch_bounds__(64 * WTY)
         if constexpr ((WM & WM_VEL) != 0)
             vcount += __shfl_xor(vcount, o, 64);
     }
-    if ((threadIdx.x & 63) == 0)
+    // then across the block's wavefronts: one atomic per block and count
+    // (all blocks adding into one address serialise in L2)
+    __shared__ unsigned long long wcnt[WTY][3];
+    if (tx == 0)
     {
-        atomicAdd(&p.counts[0], plain_w);
-        atomicAdd(&p.counts[1], weighted);
-        if constexpr ((WM & WM_VEL) != 0)
-            atomicAdd(&p.counts[2], vcount);
+        wcnt[ty][0] = plain_w;
+        wcnt[ty][1] = weighted;
+        wcnt[ty][2] = vcount;
+    }
+    __syncthreads();
+    if (t < 3)
+    {
+        unsigned long long v = 0;
+        for (int w = 0; w < WTY; ++w)
+            v += wcnt[w][t];
+        if (t < 2 || (WM & WM_VEL) != 0)
+            atomicAdd(&p.counts[t], v);
     }
     if constexpr ((WM & WM_VEL) != 0)
     {
         // fixed-order block reduction of the velocity sum -> one partial
         __shared__ double red[64 * WTY];
-        const int t = threadIdx.y * 64 + threadIdx.x;
         red[t] = vsum;
         __syncthreads();
         for (int o = 32 * WTY; o > 0; o >>= 1)
@@ -422,8 +587,17 @@ template <typename T, int WM>
 static void launch_windings_t(int ntiles, hipStream_t s, const T *y,
                               const float *lo, const WindParams &p)
 {
-    hipLaunchKernelGGL((k_windings<T, WM>), dim3(ntiles), dim3(64, WTY), 0, s,
-                       y, lo, p);
+    if constexpr (std::is_same<T, float>::value)
+    {
+        if (lo)
+        {
+            hipLaunchKernelGGL((k_windings<T, WM, true>), dim3(ntiles),
+                               dim3(64, WTY), 0, s, y, lo, p);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_windings<T, WM, false>), dim3(ntiles), dim3(64, WTY),
+                       0, s, y, lo, p);
 }
 
 template <typename T>
@@ -509,13 +683,16 @@ constexpr int ETY = 8; // energy tile: 64 x ETY cells, one wavefront per row
 // instead of once per stencil use.  Per cell the 14 terms are formed
 // exactly as the direct form (same d1 operand order); only the order of the
 // (reproducible, fixed-order) sums differs.
-template <typename T>
+template <typename T, bool HASLO>
 __global__ void __launch_bounds__(64 * ETY)
+    __attribute__((amdgpu_waves_per_eu(4)))
     k_energy(const T *y, const float *lo, EnergyParams p)
 {
+    constexpr int ED = 2; // planes of loads in flight
     __shared__ double sp[2][ETY + 4][64 + 4];
     __shared__ double red[64 * ETY];
     const int tx = threadIdx.x, ty = threadIdx.y;
+    const int t = ty * 64 + tx;
     const int bid = blockIdx.x;
     const int bx = bid % p.tiles_x;
     const int rem = bid / p.tiles_x;
@@ -528,100 +705,89 @@ __global__ void __launch_bounds__(64 * ETY)
     const bool active = gx < nx && gy < ny;
     const int xw = gx % nx, yw = gy % ny;
     const uint32_t rowc = (uint32_t)yw * nx + xw;
-    const T *p1 = y, *p2 = y + n, *q1 = y + 2 * (size_t)n, *q2 = y + 3 * (size_t)n;
-    const float *l1 = lo, *l2 = lo ? lo + n : nullptr;
-    const float *m1 = lo ? lo + 2 * (size_t)n : nullptr;
-    const float *m2 = lo ? lo + 3 * (size_t)n : nullptr;
-    // halo roles: lanes 0,1 / 62,63 load the x halo, rows 0,1 / ETY-2,ETY-1
-    // the y halo (ETY >= 4, so no row holds both y-halo roles)
-    const bool hxl = tx < 2, hxr = tx >= 62;
-    const bool hyl = ty < 2, hyr = ty >= ETY - 2;
-    uint32_t rowhx = 0, rowhy = 0;
-    int sx = 0, sy = 0;
-    if (hxl || hxr)
+    // the one halo point this thread stages (2-cell halo, no corners):
+    // threads 0..255 the y halo rows, 256..287 the x halo columns; the rest
+    // load their own cell again (unconditional loads keep the register
+    // queue out of scratch memory) and store nothing
+    const bool has_halo = t < 4 * 64 + 4 * ETY;
+    int hr = 0, hc = 0;
+    uint32_t hrow = rowc;
+    if (t < 4 * 64)
     {
-        const int hx = hxl ? bx * 64 - 2 + tx : bx * 64 + 64 + (tx - 62);
-        rowhx = (uint32_t)yw * nx + (uint32_t)(((hx % nx) + nx) % nx);
-        sx = hxl ? tx : 64 + 2 + (tx - 62);
+        const int k = t / 64, c = t % 64;
+        hr = k < 2 ? k : ETY + k;            // LDS rows 0, 1, ETY+2, ETY+3
+        hc = c + 2;
+        const int hy = k < 2 ? by * ETY - 2 + k : by * ETY + ETY + (k - 2);
+        hrow = (uint32_t)(((hy % ny) + ny) % ny) * nx +
+               (uint32_t)((bx * 64 + c) % nx);
     }
-    if (hyl || hyr)
+    else if (has_halo)
     {
-        const int hy = hyl ? by * ETY - 2 + ty : by * ETY + ETY + (ty - (ETY - 2));
-        rowhy = (uint32_t)(((hy % ny) + ny) % ny) * nx + xw;
-        sy = hyl ? ty : ETY + 2 + (ty - (ETY - 2));
+        const int u = t - 4 * 64, r = u / 4, k = u % 4;
+        hr = r + 2;
+        hc = k < 2 ? k : 64 + k;             // LDS columns 0, 1, 66, 67
+        const int hx = k < 2 ? bx * 64 - 2 + k : bx * 64 + 64 + (k - 2);
+        hrow = (uint32_t)((by * ETY + r) % ny) * nx +
+               (uint32_t)(((hx % nx) + nx) % nx);
     }
     const double i1 = 1.0 / p.dx;
     const double R = p.R, R2 = R * R, R4 = R2 * R2;
     const double f_a = 1.4142135623730951;
     const int z_begin = bz * p.z_chunk;
     const int z_end = min(z_begin + p.z_chunk, nz);
+    const int len = z_end - z_begin;
     auto zoff = [&](int z) -> uint32_t {
         return (uint32_t)wrapc(wrapc(z, nz), nz) * plane;
+    };
+    // stored words in flight: psi of the own column WD planes ahead of
+    // its use as the z + 2 neighbour; halo psi and centre Pi of a plane
+    // ED planes ahead of the plane's turn
+    T cw[ED][2], hw[ED][2], pw[ED][2];
+    float cl[ED][2], hl[ED][2], pl[ED][2];
+    auto issue_c = [&](int q, int z) { // own psi of plane z
+        const uint32_t o = zoff(z) + rowc;
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+        {
+            cw[q][c] = y[(size_t)c * n + o];
+            if constexpr (HASLO)
+                cl[q][c] = lo[(size_t)c * n + o];
+        }
+    };
+    auto issue_h = [&](int q, int z) { // halo psi and own Pi of plane z
+        const uint32_t oz = zoff(z);
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+        {
+            hw[q][c] = y[(size_t)c * n + oz + hrow];
+            pw[q][c] = y[(size_t)(c + 2) * n + oz + rowc];
+            if constexpr (HASLO)
+            {
+                hl[q][c] = lo[(size_t)c * n + oz + hrow];
+                pl[q][c] = lo[(size_t)(c + 2) * n + oz + rowc];
+            }
+        }
+    };
+    auto val = [&](T w, float l) -> double {
+        if constexpr (HASLO)
+            return (double)w + (double)l;
+        else
+            return (double)w;
     };
     double a1[5], a2[5];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
     {
         const uint32_t o = zoff(z_begin - 2 + i) + rowc;
-        a1[i + 1] = ld(p1, l1, o);
-        a2[i + 1] = ld(p2, l2, o);
+        a1[i + 1] = val(y[o], HASLO ? lo[o] : 0.0f);
+        a2[i + 1] = val(y[(size_t)n + o], HASLO ? lo[(size_t)n + o] : 0.0f);
     }
     double acc[14];
 #pragma unroll
     for (int q = 0; q < 14; ++q)
         acc[q] = 0.0;
-    // halo psi and centre Pi of the next plane, loaded one plane ahead
-    double hxv[2] = {0.0, 0.0}, hyv[2] = {0.0, 0.0}, piv[2] = {0.0, 0.0};
-    auto load_next = [&](int z) {
-        const uint32_t oz = zoff(z);
-        if (hxl || hxr)
-        {
-            hxv[0] = ld(p1, l1, oz + rowhx);
-            hxv[1] = ld(p2, l2, oz + rowhx);
-        }
-        if (hyl || hyr)
-        {
-            hyv[0] = ld(p1, l1, oz + rowhy);
-            hyv[1] = ld(p2, l2, oz + rowhy);
-        }
-        if (active)
-        {
-            piv[0] = ld(q1, m1, oz + rowc);
-            piv[1] = ld(q2, m2, oz + rowc);
-        }
-    };
-    load_next(z_begin);
-    for (int z = z_begin; z < z_end; ++z)
-    {
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-        {
-            a1[i] = a1[i + 1];
-            a2[i] = a2[i + 1];
-        }
-        const uint32_t oz = zoff(z);
-        {
-            const uint32_t o = zoff(z + 2) + rowc;
-            a1[4] = ld(p1, l1, o);
-            a2[4] = ld(p2, l2, o);
-        }
-        sp[0][ty + 2][tx + 2] = a1[2];
-        sp[1][ty + 2][tx + 2] = a2[2];
-        if (hxl || hxr)
-        {
-            sp[0][ty + 2][sx] = hxv[0];
-            sp[1][ty + 2][sx] = hxv[1];
-        }
-        if (hyl || hyr)
-        {
-            sp[0][sy][tx + 2] = hyv[0];
-            sp[1][sy][tx + 2] = hyv[1];
-        }
-        const double Pi1 = piv[0], Pi2 = piv[1];
-        if (z + 1 < z_end)
-            load_next(z + 1);
-        __syncthreads();
-        const uint32_t id = oz + rowc;
+    auto cellwork = [&](int z, double Pi1, double Pi2) {
+        const uint32_t id = zoff(z) + rowc;
         if (active && !(p.mask && p.mask[id] == 0))
         {
             const double psi1 = a1[2], psi2 = a2[2];
@@ -690,10 +856,54 @@ __global__ void __launch_bounds__(64 * ETY)
             acc[12] += w;
             acc[13] += 1.0;
         }
-        __syncthreads();
+    };
+
+    // own psi of plane z + 2 + q sits in slot q % ED until it enters the
+    // z queue; halo / Pi of plane z in slot (z - z_begin) % ED
+#pragma unroll
+    for (int q = 0; q < ED; ++q)
+    {
+        issue_c(q, z_begin + 2 + q);
+        if (q < len)
+            issue_h(q, z_begin + q);
+    }
+    for (int k = 0; k < len; k += ED)
+    {
+#pragma unroll
+        for (int u = 0; u < ED; ++u)
+        {
+            const int rel = k + u;
+            if (rel < len)
+            {
+                const int z = z_begin + rel;
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                {
+                    a1[i] = a1[i + 1];
+                    a2[i] = a2[i + 1];
+                }
+                a1[4] = val(cw[u][0], cl[u][0]);
+                a2[4] = val(cw[u][1], cl[u][1]);
+                if (rel + ED < len)
+                    issue_c(u, z + 2 + ED);
+                sp[0][ty + 2][tx + 2] = a1[2];
+                sp[1][ty + 2][tx + 2] = a2[2];
+                if (has_halo)
+                {
+                    sp[0][hr][hc] = val(hw[u][0], hl[u][0]);
+                    sp[1][hr][hc] = val(hw[u][1], hl[u][1]);
+                }
+                const double Pi1 = val(pw[u][0], pl[u][0]);
+                const double Pi2 = val(pw[u][1], pl[u][1]);
+                if (rel + ED < len)
+                    issue_h(u, z + ED);
+                __syncthreads();
+                cellwork(z, Pi1, Pi2);
+                __syncthreads();
+            }
+        }
     }
     // fixed-order block reduction of each sum -> one row of partials
-    const int t = ty * 64 + tx;
     for (int q = 0; q < 14; ++q)
     {
         red[t] = acc[q];
@@ -795,6 +1005,9 @@ static int run_windings(axs_level *L, const uint8_t *mask, int8_t *wfield,
     p.R = tp ? tp->R : 1.0;
     p.use_lap = (tp && tp->gradient_threshold < DBL_MAX) ? 1 : 0;
     p.use_rgrad = (tp && tp->radial_gradient_threshold < DBL_MAX) ? 1 : 0;
+    const int wm = (vel ? WM_VEL : 0) | (tags ? WM_TAG : 0) |
+                   ((tags && (p.use_lap || p.use_rgrad)) ? WM_TAGD : 0) |
+                   (wfield ? WM_FIELD : 0);
     p.tiles_x = (p.nx + 63) / 64;
     p.tiles_y = (p.ny + WTY - 1) / WTY;
     int zc = 32;
@@ -816,18 +1029,12 @@ static int run_windings(axs_level *L, const uint8_t *mask, int8_t *wfield,
         p.vel_partials = L->ctx->d_partials;
     }
     hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, s, p.counts);
-    const int wm = (p.do_vel ? WM_VEL : 0) | (tags ? WM_TAG : 0) |
-                   ((tags && (p.use_lap || p.use_rgrad)) ? WM_TAGD : 0) |
-                   (wfield ? WM_FIELD : 0);
+    const float *lo = L->precision == AXS_FP32C ? (const float *)L->comp
+                                                : (const float *)nullptr;
     if (L->precision == AXS_FP64)
-        launch_windings<double>(wm, ntiles, s, (const double *)L->y,
-                                (const float *)nullptr, p);
+        launch_windings<double>(wm, ntiles, s, (const double *)L->y, nullptr, p);
     else
-        launch_windings<float>(wm, ntiles, s, (const float *)L->y,
-                               L->precision == AXS_FP32C
-                                   ? (const float *)L->comp
-                                   : (const float *)nullptr,
-                               p);
+        launch_windings<float>(wm, ntiles, s, (const float *)L->y, lo, p);
     AXS_HIP_TRY(hipGetLastError());
     if (vel)
     {
@@ -944,14 +1151,14 @@ int axs_energy_sums(axs_level *L, double R, double lambda, double RpR,
     AXS_CHECK_ARG(nbl <= kMaxPartials, "axs_energy_sums: grid too large");
     const int nb = (int)nbl;
     if (L->precision == AXS_FP64)
-        hipLaunchKernelGGL(k_energy<double>, dim3(nb), dim3(64, ETY), 0, s,
-                           (const double *)L->y, (const float *)nullptr, p);
+        hipLaunchKernelGGL((k_energy<double, false>), dim3(nb), dim3(64, ETY), 0,
+                           s, (const double *)L->y, (const float *)nullptr, p);
+    else if (L->precision == AXS_FP32C)
+        hipLaunchKernelGGL((k_energy<float, true>), dim3(nb), dim3(64, ETY), 0,
+                           s, (const float *)L->y, (const float *)L->comp, p);
     else
-        hipLaunchKernelGGL(k_energy<float>, dim3(nb), dim3(64, ETY), 0, s,
-                           (const float *)L->y,
-                           L->precision == AXS_FP32C ? (const float *)L->comp
-                                                     : (const float *)nullptr,
-                           p);
+        hipLaunchKernelGGL((k_energy<float, false>), dim3(nb), dim3(64, ETY), 0,
+                           s, (const float *)L->y, (const float *)nullptr, p);
     AXS_HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_sum_partials, dim3(14), dim3(256), 0, s, p.partials,
                        nb, 14, L->ctx->d_result);

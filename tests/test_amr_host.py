@@ -41,3 +41,14 @@ def test_dilation_and_erosion_are_periodic():
     d = amr.dilate_periodic(m, 1)
     assert d.sum() == 27 and d[7, 7, 7] and d[1, 1, 1]
     assert np.array_equal(amr.erode_periodic(d, 1), m)
+
+
+def test_tagged_blocks_equals_dilate_then_block():
+    rng = np.random.default_rng(1)
+    for _ in range(60):
+        block = int(rng.choice([2, 4, 8]))
+        r = int(rng.integers(0, block + 1))
+        shape = tuple(block * int(v) for v in rng.integers(1, 5, 3))
+        tags = rng.random(shape) < rng.choice([0.001, 0.01, 0.1])
+        want = amr.blocks_from_tags(amr.dilate_periodic(tags, r), block)
+        assert np.array_equal(amr.tagged_blocks(tags, block, r), want)

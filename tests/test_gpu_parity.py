@@ -5,6 +5,8 @@ Bars: FP64 RHS / RK4 bit-exact (same operation order, contraction off);
 windings, counts and tags bit-exact; energy sums, spectra and the IC within
 1e-12 .. 1e-10 relative (reduction order and FFT rounding differ)."""
 
+import math
+
 import numpy as np
 import pytest
 
@@ -146,6 +148,150 @@ def test_windings_and_counts_bitexact(ctx, oracle, precision):
     counts = lvl.count_plaquettes()
     assert counts == oracle.count_plaquettes(s)
     assert counts[0] > 0
+
+
+def _tricky_pool(extreme):
+    """Vertex values where phase-free winding tests are easiest to get wrong:
+    signed zeros on both axes, values a hair off the negative x axis,
+    subnormals, exact antipodes, antipodes rotated by 1e-16 .. 3e-10 rad
+    (inside the kernel's flagging bound) and by 1e-8 .. 1e-6 rad (outside
+    it), and (FP64 only) magnitudes beyond 1e+-140."""
+    rng = np.random.default_rng(11)
+    pool = [(1.0, 0.0), (1.0, -0.0), (-1.0, 0.0), (-1.0, -0.0), (0.0, 1.0),
+            (0.0, -1.0), (-0.0, 1.0), (-0.0, -1.0), (0.0, 0.0), (-0.0, -0.0),
+            (-1.0, 1e-17), (-1.0, -1e-17), (1e-17, 1.0), (-1e-17, 1.0),
+            (1e-17, -1.0), (-1e-17, -1.0), (-2.5, 1.5e-9), (-2.5, -1.5e-9)]
+    if extreme:
+        pool += [(-1.0, 1e-300), (-1.0, -1e-300), (-1.0, 5e-324),
+                 (-1.0, -5e-324), (1e-200, 1e-200), (-1e-200, 1e-201),
+                 (3e150, -1e150), (1e-320, -1.0), (-1e-320, 1.0)]
+    for _ in range(12):
+        t, r = rng.uniform(-np.pi, np.pi), rng.uniform(0.3, 2.0)
+        x, y = r * np.cos(t), r * np.sin(t)
+        pool += [(x, y), (-x, -y)]
+        for eps in (1e-12, -1e-12, 1e-16, -1e-16, 3e-10, -3e-10, 1e-8, -1e-8,
+                    1e-6, -1e-6):
+            c, s_ = np.cos(eps), np.sin(eps)
+            pool.append((-1.7 * (x * c - y * s_), -1.7 * (x * s_ + y * c)))
+    return np.array(pool)
+
+
+@pytest.mark.parametrize("shape", [(18, 12, 10), (20, 12, 10), (260, 8, 6),
+                                   (8, 4, 4), (36, 8, 8), (516, 5, 4)])
+@pytest.mark.parametrize("precision", [0, 1, 2])
+def test_windings_shapes(ctx, oracle, shape, precision):
+    """Both windings passes (the vector pass for nx % 4 == 0, the LDS-ring
+    pass otherwise) on ragged shapes, partial tiles and a level narrower
+    than one vector tile: windings field, counts and velocities."""
+    capi = _capi()
+    s = random_state(shape, seed=sum(shape) + precision)
+    lvl = capi.Level(ctx, shape, 0.5, precision)
+    lvl.upload(s)
+    s = lvl.download()
+    assert np.array_equal(lvl.windings_field(), oracle.windings_field(s))
+    assert lvl.count_plaquettes() == oracle.count_plaquettes(s)
+    total, count, _ = lvl.velocity(3.0, 1.0 / 3.0, 1.0)
+    wt, wc = oracle.velocity(s, 3.0, 1.0 / 3.0, 1.0)
+    assert count == wc and total == pytest.approx(wt, rel=1e-11)
+    nx, ny, nz = shape
+    mask = (np.random.default_rng(1).random((nz, ny, nx)) < 0.6).astype(np.uint8)
+    mb = capi.DeviceBuffer(mask.size)
+    mb.from_numpy(mask)
+    assert lvl.count_plaquettes(mask=mb) == oracle.count_plaquettes(s, mask)
+    total, count, _ = lvl.velocity(3.0, 1.0 / 3.0, 1.0, mask=mb)
+    wt, wc = oracle.velocity(s, 3.0, 1.0 / 3.0, 1.0, mask=mask)
+    assert count == wc and total == pytest.approx(wt, rel=1e-11)
+
+
+def _lround(x):
+    """C lround: nearest integer, ties away from zero."""
+    r = np.round(x)
+    tie = np.abs(x - np.trunc(x)) == 0.5
+    return np.where(tie, np.trunc(x) + np.sign(x), r)
+
+
+def _windings_from_phases(th):
+    """StringFinder.hpp plaquette_winding over a [z, y, x] phase array
+    (numpy restatement of the oracle's, for perturbed phases)."""
+    pi = np.pi
+
+    def wrap(d):
+        d = np.fmod(d, 2.0 * pi)
+        return np.where(d <= -pi, d + 2.0 * pi, np.where(d > pi, d - 2.0 * pi, d))
+
+    def sh(dx, dy, dz):
+        return np.roll(th, (-dz, -dy, -dx), axis=(0, 1, 2))
+
+    def pw(a, b, c, d):
+        tot = wrap(b - a) + wrap(c - b) + wrap(d - c) + wrap(a - d)
+        return _lround(tot / (2.0 * pi))
+
+    t000, t100, t110, t010 = th, sh(1, 0, 0), sh(1, 1, 0), sh(0, 1, 0)
+    t001, t101, t011 = sh(0, 0, 1), sh(1, 0, 1), sh(0, 1, 1)
+    return np.stack([pw(t000, t100, t110, t010), pw(t000, t010, t011, t001),
+                     pw(t000, t001, t101, t100)], -1).astype(np.int8)
+
+
+def _phases(s):
+    """C-library atan2 of every vertex (numpy's own arctan2 may take a SIMD
+    path that rounds differently from libm)."""
+    f = np.frompyfunc(math.atan2, 2, 1)
+    return f(s[1], s[0]).astype(np.float64)
+
+
+def _ulp_sensitivity(s, trials=32, seed=0):
+    """Range of each plaquette's winding when every vertex phase moves by up
+    to 2 ulp (the accuracy of a device atan2): (lo, hi) int8 arrays."""
+    th = _phases(s)
+    sp = np.abs(np.spacing(th))
+    rng = np.random.default_rng(seed)
+    ks = [np.full(th.shape, k) for k in (-2, -1, 1, 2)]
+    ks += [rng.integers(-2, 3, th.shape) for _ in range(trials)]
+    lo = hi = _windings_from_phases(th)
+    for k in ks:
+        w = _windings_from_phases(np.clip(th + k * sp, -np.pi, np.pi))
+        lo, hi = np.minimum(lo, w), np.maximum(hi, w)
+    return lo, hi
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_windings_adversarial(ctx, oracle, precision):
+    """Windings, counts and velocities on fields built from the tricky pool,
+    where near-antipodal edges and on-axis vertices are common: the HIP
+    sign-test path with its atan2 fallback against the oracle's atan2.
+    Bit-exact on every plaquette whose winding does not change when the
+    vertex phases move by 2 ulp; where it does (an edge exactly antipodal,
+    so the string passes through the edge and the result hangs on the last
+    bit of atan2, which the device and host libraries may round apart), the
+    device value lies in the range those phases give."""
+    capi = _capi()
+    pool = _tricky_pool(extreme=precision == 0)
+    rng = np.random.default_rng(5 + precision)
+    for n in (16, 20):
+        s = rng.normal(size=(4, n, n, n))
+        idx = rng.integers(0, len(pool), size=(n, n, n))
+        s[0], s[1] = pool[idx, 0], pool[idx, 1]
+        lvl = capi.Level(ctx, n, 0.5, precision)
+        lvl.upload(s)
+        s = lvl.download()
+        want = oracle.windings_field(s)
+        assert np.array_equal(_windings_from_phases(_phases(s)), want)
+        lo, hi = _ulp_sensitivity(s)
+        sens = lo != hi
+        got = lvl.windings_field()
+        bad = (got != want) & ~sens
+        assert not bad.any(), (
+            f"n={n}: {int(bad.sum())} stable plaquettes differ, first at "
+            f"{np.argwhere(bad)[:4].tolist()}")
+        assert ((got >= lo) & (got <= hi)).all()
+        assert sens.mean() < 0.05
+        plain, weighted = lvl.count_plaquettes()
+        assert plain == int((got != 0).sum())
+        assert weighted == int(np.abs(got.astype(np.int64)).sum())
+        if precision == 1 and np.array_equal(got, want):
+            total, count, _ = lvl.velocity(3.0, 1.0 / 3.0, 1.0)
+            wt, wc = oracle.velocity(s, 3.0, 1.0 / 3.0, 1.0)
+            assert count == wc and total == pytest.approx(wt, rel=1e-11)
 
 
 def test_counts_with_mask_and_string_lattice(ctx, oracle):
