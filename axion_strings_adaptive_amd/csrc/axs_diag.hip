@@ -621,6 +621,298 @@ static void launch_windings(int wm, int ntiles, hipStream_t s, const T *y,
     }
 }
 
+// ---------------------------------------- register-streaming windings pass --
+// Counts, velocities and the windings field without LDS or barriers.  A
+// wavefront owns 63 consecutive x cells of one y row (lane 63 is the x halo
+// of lane 62) and streams a chunk of z; each lane holds psi of its column
+// at rows y and y+1 for planes z and z+1 in registers (row y+1 is the next
+// wavefront's row y, read again through the cache) and takes the x+1
+// column from the next lane by a shuffle.  Per plane each lane evaluates
+// the 5 edges it owns -- +x at rows y and y+1, +y, and +z at rows y and y+1
+// -- with the sign tests above and packs their k values and flags into two
+// words, which the lane to its left reads by a shuffle; a cell's windings
+// are then sums of packed k values, with the atan2 path for flagged cells.
+// The stored words of the next RD planes are in flight while one is
+// processed.
+constexpr int RWY = 4;  // wavefronts (rows) per workgroup
+constexpr int RTX = 63; // cells per wavefront row
+
+// edge word of a plane, per lane: +x edge at row y (bits 0-1, k + 1),
+// +x at row y+1 (2-3), +y at row y (4-5); flags of those edges (6, 7, 8);
+// vertex flags of rows y and y+1 (9, 10).  z word: +z edges at rows y
+// (bits 0-1) and y+1 (2-3), their flags (4, 5).
+__device__ __forceinline__ int kfield(uint32_t w, int sh)
+{
+    return (int)((w >> sh) & 3u) - 1;
+}
+
+template <typename T, int WM, bool HASLO>
+__global__ void __launch_bounds__(64 * RWY)
+    k_windings_r(const T *y, const float *lo, WindParams p)
+{
+    constexpr bool VEL = (WM & WM_VEL) != 0;
+    constexpr int NC = VEL ? 4 : 2; // components per point
+    constexpr int RD = VEL ? 2 : 3; // planes of loads in flight
+    const int bid = blockIdx.x;
+    const int bx = bid % p.tiles_x;
+    const int rem = bid / p.tiles_x;
+    const int by = rem % p.tiles_y;
+    const int bz = rem / p.tiles_y;
+    const int lane = threadIdx.x, ty = threadIdx.y;
+    const int nx = p.nx, ny = p.ny, nz = p.nz;
+    const uint32_t n = p.n;
+    const uint32_t plane = (uint32_t)nx * ny;
+    const int gx = bx * RTX + lane; // lane 63: the halo column
+    const int gy = by * RWY + ty;
+    const bool active = lane < RTX && gx < nx && gy < ny;
+    const int xw = gx % nx;
+    const int yw = gy % ny, yw1 = (gy + 1) % ny;
+    const uint32_t ia = (uint32_t)yw * nx + xw, ib = (uint32_t)yw1 * nx + xw;
+
+    // stored words in flight: [slot][point a / b][component]
+    T qw[RD][2][NC];
+    float ql[RD][2][NC];
+    auto issue = [&](int q, int zz) {
+        const uint32_t oz = (uint32_t)zz * plane;
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+        {
+            qw[q][0][c] = y[(size_t)c * n + oz + ia];
+            qw[q][1][c] = y[(size_t)c * n + oz + ib];
+            if constexpr (HASLO)
+            {
+                ql[q][0][c] = lo[(size_t)c * n + oz + ia];
+                ql[q][1][c] = lo[(size_t)c * n + oz + ib];
+            }
+        }
+    };
+    auto val = [&](int q, int pt, int c) -> double {
+        if constexpr (HASLO)
+            return (double)qw[q][pt][c] + (double)ql[q][pt][c];
+        else
+            return (double)qw[q][pt][c];
+    };
+    // plane values: own column (a: row y, b: row y+1) and the x+1 column
+    double A[NC], B[NC], A1[NC], B1[NC];     // plane z + 1 (newest)
+    double Ac[NC], Bc[NC], A1c[NC], B1c[NC]; // plane z
+    uint32_t W = 0, W1 = 0, Wc = 0, W1c = 0; // edge words, planes z+1, z
+    auto land = [&](int q) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+        {
+            A[c] = val(q, 0, c);
+            B[c] = val(q, 1, c);
+            A1[c] = __shfl_down(A[c], 1, 64);
+            B1[c] = __shfl_down(B[c], 1, 64);
+        }
+        const Vtx a{A[0], A[1]}, b{B[0], B[1]};
+        const Vtx a1{A1[0], A1[1]}, b1{B1[0], B1[1]};
+        bool fx0 = false, fx1 = false, fy0 = false;
+        const int kx0 = edge_k(a, a1, fx0);
+        const int kx1 = edge_k(b, b1, fx1);
+        const int ky0 = edge_k(a, b, fy0);
+        W = (uint32_t)(kx0 + 1) | (uint32_t)(kx1 + 1) << 2 |
+            (uint32_t)(ky0 + 1) << 4 | (uint32_t)fx0 << 6 |
+            (uint32_t)fx1 << 7 | (uint32_t)fy0 << 8 |
+            (uint32_t)!vtx_ok(a) << 9 | (uint32_t)!vtx_ok(b) << 10;
+        W1 = __shfl_down(W, 1, 64);
+    };
+
+    const int z_begin = bz * p.z_chunk;
+    const int z_end = min(z_begin + p.z_chunk, nz);
+    const int len = z_end - z_begin;
+    uint32_t plain = 0, weighted = 0, vcount = 0; // per lane
+    double vsum = 0.0;
+    // gamma^2 v^2 at one plaquette corner, VelocityKernel.hpp:82-96
+    auto corner = [&](const double *v) -> double {
+        const double R = p.vel_R, RpR = p.vel_RpR;
+        return gamma_sq_v_sq_d(v[0] / R, v[1] / R,
+                               (v[NC - 2] - RpR * v[0]) / (R * R),
+                               (v[NC - 1] - RpR * v[1]) / (R * R), p.vel_mr);
+    };
+
+    auto process = [&](int z) {
+        // +z edges from plane z to z + 1 at rows y, y+1, and their x+1
+        bool fz0 = false, fz1 = false;
+        const int kz0 = edge_k(Vtx{Ac[0], Ac[1]}, Vtx{A[0], A[1]}, fz0);
+        const int kz1 = edge_k(Vtx{Bc[0], Bc[1]}, Vtx{B[0], B[1]}, fz1);
+        const uint32_t Z = (uint32_t)(kz0 + 1) | (uint32_t)(kz1 + 1) << 2 |
+                           (uint32_t)fz0 << 4 | (uint32_t)fz1 << 5;
+        const uint32_t Z1 = __shfl_down(Z, 1, 64);
+        if (!active)
+            return;
+        const uint32_t id = (uint32_t)z * plane + ia;
+        if (p.mask && p.mask[id] == 0)
+            return;
+        // flags of the 9 edges and 7 vertices of the cell's plaquettes
+        // Wc: +x (y, y+1), +y edges, vertices 000, 010; W1c: +y edge at
+        // x+1, vertices 100, 110; W: +x, +y edges, vertices 001, 011;
+        // W1: vertex 101; Z: +z edges at 000, 010; Z1: +z edge at 100
+        const uint32_t flags = (Wc & 0x7c0u) | (W1c & 0x700u) |
+                               (W & 0x740u) | (W1 & 0x200u) |
+                               (Z & 0x30u) | (Z1 & 0x10u);
+        int wxy, wyz, wzx;
+        if (flags == 0)
+        {
+            const int kx000 = kfield(Wc, 0), kx010 = kfield(Wc, 2);
+            const int ky000 = kfield(Wc, 4), ky100 = kfield(W1c, 4);
+            const int kx001 = kfield(W, 0), ky001 = kfield(W, 4);
+            const int kz000 = kz0, kz010 = kz1, kz100 = kfield(Z1, 0);
+            wxy = -(kx000 + ky100 - kx010 - ky000);
+            wyz = -(ky000 + kz010 - ky001 - kz000);
+            wzx = -(kz000 + kx001 - kz100 - kx000);
+        }
+        else
+        {
+            const Vtx v[7] = {Vtx{Ac[0], Ac[1]},   Vtx{A1c[0], A1c[1]},
+                              Vtx{B1c[0], B1c[1]}, Vtx{Bc[0], Bc[1]},
+                              Vtx{A[0], A[1]},     Vtx{A1[0], A1[1]},
+                              Vtx{B[0], B[1]}};
+            int w[3];
+            windings_by_phase(v, w);
+            wxy = w[0];
+            wyz = w[1];
+            wzx = w[2];
+        }
+        plain += (wxy != 0) + (wyz != 0) + (wzx != 0);
+        weighted += abs(wxy) + abs(wyz) + abs(wzx);
+        if constexpr (VEL)
+        {
+            if (p.do_vel && (wxy != 0 || wyz != 0 || wzx != 0))
+            {
+                const double c000 = corner(Ac), c100 = corner(A1c);
+                const double c010 = corner(Bc), c001 = corner(A);
+                if (wxy != 0)
+                {
+                    vsum += c000 + c100 + corner(B1c) + c010;
+                    vcount += 4;
+                }
+                if (wyz != 0)
+                {
+                    vsum += c000 + c010 + corner(B) + c001;
+                    vcount += 4;
+                }
+                if (wzx != 0)
+                {
+                    vsum += c000 + c001 + corner(A1) + c100;
+                    vcount += 4;
+                }
+            }
+        }
+        if constexpr ((WM & WM_FIELD) != 0)
+        {
+            if (p.wfield)
+            {
+                p.wfield[3 * (size_t)id] = (int8_t)wxy;
+                p.wfield[3 * (size_t)id + 1] = (int8_t)wyz;
+                p.wfield[3 * (size_t)id + 2] = (int8_t)wzx;
+            }
+        }
+    };
+
+    // plane rel (0 .. len) sits in slot rel % RD until it lands
+    issue(0, z_begin);
+#pragma unroll
+    for (int j = 1; j < RD; ++j)
+        issue(j, wrapc(z_begin + j, nz)); // j <= 2 < len + 1 or wraps
+    land(0);
+    issue(0, wrapc(z_begin + RD, nz));
+    for (int k = 0; k < len; k += RD)
+    {
+#pragma unroll
+        for (int u = 0; u < RD; ++u)
+        {
+            const int rel = k + u;
+            if (rel < len)
+            {
+                const int z = z_begin + rel;
+#pragma unroll
+                for (int c = 0; c < NC; ++c)
+                {
+                    Ac[c] = A[c];
+                    Bc[c] = B[c];
+                    A1c[c] = A1[c];
+                    B1c[c] = B1[c];
+                }
+                Wc = W;
+                W1c = W1;
+                land((u + 1) % RD); // plane z + 1
+                if (rel + 1 + RD <= len)
+                    issue((u + 1) % RD, wrapc(z + 1 + RD, nz));
+                process(z);
+            }
+        }
+    }
+    // wavefront, then block reduction of the counts: one atomic per block
+    unsigned long long cp = plain, cw = weighted, cv = vcount;
+    for (int o = 32; o > 0; o >>= 1)
+    {
+        cp += __shfl_xor(cp, o, 64);
+        cw += __shfl_xor(cw, o, 64);
+        cv += __shfl_xor(cv, o, 64);
+    }
+    __shared__ unsigned long long wcnt[RWY][3];
+    if (lane == 0)
+    {
+        wcnt[ty][0] = cp;
+        wcnt[ty][1] = cw;
+        wcnt[ty][2] = cv;
+    }
+    __shared__ double red[64 * RWY];
+    const int t = ty * 64 + lane;
+    red[t] = vsum;
+    __syncthreads();
+    if (t < 3)
+    {
+        unsigned long long v = 0;
+        for (int w = 0; w < RWY; ++w)
+            v += wcnt[w][t];
+        if (t < 2 || VEL)
+            atomicAdd(&p.counts[t], v);
+    }
+    if constexpr (VEL)
+    {
+        // fixed-order block reduction of the velocity sum -> one partial
+        for (int o = 32 * RWY; o > 0; o >>= 1)
+        {
+            if (t < o)
+                red[t] += red[t + o];
+            __syncthreads();
+        }
+        if (t == 0)
+            p.vel_partials[blockIdx.x] = red[0];
+    }
+}
+
+template <typename T, int WM>
+static void launch_windings_rt(int ntiles, hipStream_t s, const T *y,
+                               const float *lo, const WindParams &p)
+{
+    if constexpr (std::is_same<T, float>::value)
+    {
+        if (lo)
+        {
+            hipLaunchKernelGGL((k_windings_r<T, WM, true>), dim3(ntiles),
+                               dim3(64, RWY), 0, s, y, lo, p);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_windings_r<T, WM, false>), dim3(ntiles),
+                       dim3(64, RWY), 0, s, y, lo, p);
+}
+
+template <typename T>
+static void launch_windings_r(int wm, int ntiles, hipStream_t s, const T *y,
+                              const float *lo, const WindParams &p)
+{
+    if (wm == WM_VEL)
+        launch_windings_rt<T, WM_VEL>(ntiles, s, y, lo, p);
+    else if (wm == WM_FIELD)
+        launch_windings_rt<T, WM_FIELD>(ntiles, s, y, lo, p);
+    else
+        launch_windings_rt<T, 0>(ntiles, s, y, lo, p);
+}
+
 __global__ void k_zero_counts(unsigned long long *c)
 {
     if (threadIdx.x < 8)
@@ -1008,9 +1300,13 @@ static int run_windings(axs_level *L, const uint8_t *mask, int8_t *wfield,
     const int wm = (vel ? WM_VEL : 0) | (tags ? WM_TAG : 0) |
                    ((tags && (p.use_lap || p.use_rgrad)) ? WM_TAGD : 0) |
                    (wfield ? WM_FIELD : 0);
-    p.tiles_x = (p.nx + 63) / 64;
-    p.tiles_y = (p.ny + WTY - 1) / WTY;
-    int zc = 32;
+    // the register pass for counts / velocities / windings field, the LDS
+    // pass for the tagger outputs
+    const bool reg = (wm & (WM_TAG | WM_TAGD)) == 0 &&
+                     (wm & (WM_VEL | WM_FIELD)) != (WM_VEL | WM_FIELD);
+    p.tiles_x = reg ? (p.nx + RTX - 1) / RTX : (p.nx + 63) / 64;
+    p.tiles_y = reg ? (p.ny + RWY - 1) / RWY : (p.ny + WTY - 1) / WTY;
+    int zc = reg ? 64 : 32;
     while (zc > 2 && (long)p.tiles_x * p.tiles_y * ((p.nz + zc - 1) / zc) < 4096)
         zc >>= 1;
     if (zc > p.nz)
@@ -1032,7 +1328,16 @@ static int run_windings(axs_level *L, const uint8_t *mask, int8_t *wfield,
     const float *lo = L->precision == AXS_FP32C ? (const float *)L->comp
                                                 : (const float *)nullptr;
     if (L->precision == AXS_FP64)
-        launch_windings<double>(wm, ntiles, s, (const double *)L->y, nullptr, p);
+    {
+        if (reg)
+            launch_windings_r<double>(wm, ntiles, s, (const double *)L->y,
+                                      nullptr, p);
+        else
+            launch_windings<double>(wm, ntiles, s, (const double *)L->y,
+                                    nullptr, p);
+    }
+    else if (reg)
+        launch_windings_r<float>(wm, ntiles, s, (const float *)L->y, lo, p);
     else
         launch_windings<float>(wm, ntiles, s, (const float *)L->y, lo, p);
     AXS_HIP_TRY(hipGetLastError());
