@@ -63,7 +63,23 @@ def tagged_blocks(tags: np.ndarray, block: int, r: int) -> np.ndarray:
     block times smaller."""
     assert 0 <= r <= block
     out = tags
-    for ax in (2, 1, 0):  # x, then y, then z ([z, y, x] arrays)
+    axes = (2, 1, 0)  # x, then y, then z ([z, y, x] arrays)
+    if (block % 8 == 0 and r <= 8 and sys.byteorder == "little"
+            and tags.dtype == np.bool_ and tags.shape[2] % block == 0):
+        # x first, 8 cells per 64-bit word: a block is block / 8 words, its
+        # first r cells the low r bytes of its first word, its last r cells
+        # the high r bytes of its last word
+        nz, ny, nx = tags.shape
+        w = np.ascontiguousarray(tags).view(np.uint64).reshape(
+            nz, ny, nx // block, block // 8)
+        red = (w != 0).any(axis=-1)
+        if r > 0:
+            head = (w[..., 0] & np.uint64((1 << (8 * r)) - 1)) != 0
+            tail = (w[..., -1] >> np.uint64(8 * (8 - r))) != 0
+            red = red | np.roll(head, -1, axis=-1) | np.roll(tail, 1, axis=-1)
+        out = red
+        axes = (1, 0)
+    for ax in axes:
         a = np.moveaxis(out, ax, -1)
         nb = a.shape[-1] // block
         a = a.reshape(a.shape[:-1] + (nb, block))
@@ -155,6 +171,7 @@ class Hierarchy:
         self.nsteps = 0
         self.nregrids = 0
         self.regrid_s = 0.0  # wall time spent in regrid() by step()
+        self.regrid_parts = {}  # the same, by phase
         self.bg = bg if bg is not None else capi.make_background(
             2.0, c0=0.0, c1=2.0, tau_switch=1.0, has_switch=True)
         self.tau = tau0
@@ -218,13 +235,22 @@ class Hierarchy:
                                   self.max_grid_size // self.blocking_factor)
         return boxes, blk
 
+    def _lap(self, key, t0):
+        """Adds the device-synchronised time since t0 to regrid_parts[key]."""
+        self.ctx.synchronize()
+        t1 = time.perf_counter()
+        self.regrid_parts[key] = self.regrid_parts.get(key, 0.0) + t1 - t0
+        return t1
+
     def regrid(self):
         """Rebuilds levels 1.. from fresh tags at a synchronised time."""
         covered = np.ones(tuple(d // 8 for d in self.domains[0]), bool)
+        t = time.perf_counter()
         for l in range(1, self.max_levels + 1):
             coarse = self.levels[l - 1]
             coarse.fill_ghosts(self.levels[l - 2] if l >= 2 else None)
             boxes, blk = self._fine_boxes(l - 1, covered)
+            t = self._lap("tag_and_boxes", t)
             old = self.levels[l] if l < len(self.levels) else None
             if not boxes:
                 for L in self.levels[l:]:
@@ -239,15 +265,20 @@ class Hierarchy:
                 continue
             fd = tuple(2 * d for d in self.domains[l - 1])
             Lf = capi.BoxLevel(self.ctx, boxes, fd, self.dx[l], self.precision)
+            t = self._lap("create", t)
             Lf.fill_ghosts(coarse, 1.0, interior=True)
+            t = self._lap("interpolate", t)
             if old is not None:
                 Lf.copy_valid_from(old)
+                t = self._lap("copy_valid", t)
                 old.close()
+                t = self._lap("free", t)
                 self.levels[l] = Lf
             else:
                 self.levels.append(Lf)
                 self.domains.append(fd)
             Lf.fill_ghosts(coarse, 1.0)
+            t = self._lap("interpolate", t)
             covered = covered_next
         self.ncells = [L.num_cells for L in self.levels]
         self.nregrids += 1
@@ -323,6 +354,7 @@ def main(argv=None):
     h.step()  # warm-up
     ctx.synchronize()
     h.regrid_s, n_regrid0 = 0.0, h.nregrids
+    h.regrid_parts = {}
     t0 = time.perf_counter()
     for _ in range(args.steps):
         h.step()
@@ -339,6 +371,8 @@ def main(argv=None):
                            if h.nregrids > n_regrid0 else None),
         "ms_per_coarse_step_without_regrids":
             1e3 * (dt - h.regrid_s) / args.steps,
+        "regrid_ms_by_phase": {k: 1e3 * v / max(1, h.nregrids - n_regrid0)
+                               for k, v in h.regrid_parts.items()},
         "n_total": e["n_total"], "L_tilde^3": (args.n0 * 0.5) ** 3,
         "rho_tot_unscreened": e["rho_tot_unscreened"]}))
     h.close()

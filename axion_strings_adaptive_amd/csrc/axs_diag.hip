@@ -157,11 +157,11 @@ enum
 // within ~1e-9 rad of antipodal: elsewhere the computed cross product has
 // the sign of the exact one (its rounding error is ~1e-16 |a||b|) and the
 // phase difference is >~1e-9 away from +-pi, far beyond the ~1e-16 rad
-// error of atan2.  Near-antipodal edges (dot < 0 and |cross| <= 1e-9
-// (|a.x| + |a.y|)(|b.x| + |b.y|)), vertices on the x axis (y == +-0, where
-// the sign of zero picks the phase +-pi) and non-finite or extreme values
-// are flagged, and the caller then evaluates the plaquettes with atan2
-// exactly as the reference does.
+// error of atan2.  Near-antipodal edges (dot < 0 and |cross| <= 1e-9 |dot|,
+// where |dot| ~ |a||b|), vertices on the x axis (y == +-0, where the sign of
+// zero picks the phase +-pi) and non-finite or extreme values are flagged,
+// and the caller then evaluates the plaquettes with atan2 exactly as the
+// reference does.
 struct Vtx
 {
     double x, y;
@@ -180,8 +180,8 @@ __device__ __forceinline__ int edge_k(const Vtx &a, const Vtx &b, bool &amb)
     const bool sa = a.y < 0.0, sb = b.y < 0.0;
     const double d = a.x * b.x + a.y * b.y;
     const double c = a.x * b.y - a.y * b.x;
-    const double m = (fabs(a.x) + fabs(a.y)) * (fabs(b.x) + fabs(b.y));
-    amb = amb || (d < 0.0 && !(fabs(c) > 1e-9 * m));
+    // d < 0 and |c| <= 1e-9 |d|, i.e. |c| + 1e-9 d <= 0
+    amb = amb || fma(1e-9, d, fabs(c)) <= 0.0;
     const int up = (sa && !sb && c < 0.0) ? 1 : 0;
     const int down = (!sa && sb && c > 0.0) ? 1 : 0;
     return up - down;
@@ -648,11 +648,12 @@ __device__ __forceinline__ int kfield(uint32_t w, int sh)
 
 template <typename T, int WM, bool HASLO>
 __global__ void __launch_bounds__(64 * RWY)
+    __attribute__((amdgpu_waves_per_eu((WM & WM_VEL) != 0 ? 3 : 4)))
     k_windings_r(const T *y, const float *lo, WindParams p)
 {
     constexpr bool VEL = (WM & WM_VEL) != 0;
     constexpr int NC = VEL ? 4 : 2; // components per point
-    constexpr int RD = VEL ? 2 : 3; // planes of loads in flight
+    constexpr int RD = 2;           // planes of loads in flight
     const int bid = blockIdx.x;
     const int bx = bid % p.tiles_x;
     const int rem = bid / p.tiles_x;
@@ -692,9 +693,20 @@ __global__ void __launch_bounds__(64 * RWY)
         else
             return (double)qw[q][pt][c];
     };
-    // plane values: own column (a: row y, b: row y+1) and the x+1 column
-    double A[NC], B[NC], A1[NC], B1[NC];     // plane z + 1 (newest)
-    double Ac[NC], Bc[NC], A1c[NC], B1c[NC]; // plane z
+    // gamma^2 v^2 at a vertex, VelocityKernel.hpp:82-96 (velocity passes
+    // evaluate it once per vertex; a pierced plaquette sums its corners)
+    auto gamma_of = [&](const double *v) -> double {
+        const double R = p.vel_R, RpR = p.vel_RpR;
+        return gamma_sq_v_sq_d(v[0] / R, v[1] / R,
+                               (v[NC - 2] - RpR * v[0]) / (R * R),
+                               (v[NC - 1] - RpR * v[1]) / (R * R), p.vel_mr);
+    };
+    // plane values: own column (a: row y, b: row y+1) and psi of the x+1
+    // column; gamma^2 v^2 of the same four vertices
+    double A[NC], B[NC], A1[2], B1[2];     // plane z + 1 (newest)
+    double Ac[NC], Bc[NC], A1c[2], B1c[2]; // plane z
+    double gA = 0, gB = 0, gA1 = 0, gB1 = 0, gAc = 0, gBc = 0, gA1c = 0,
+           gB1c = 0;
     uint32_t W = 0, W1 = 0, Wc = 0, W1c = 0; // edge words, planes z+1, z
     auto land = [&](int q) {
 #pragma unroll
@@ -702,8 +714,19 @@ __global__ void __launch_bounds__(64 * RWY)
         {
             A[c] = val(q, 0, c);
             B[c] = val(q, 1, c);
+        }
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+        {
             A1[c] = __shfl_down(A[c], 1, 64);
             B1[c] = __shfl_down(B[c], 1, 64);
+        }
+        if constexpr (VEL)
+        {
+            gA = gamma_of(A);
+            gB = gamma_of(B);
+            gA1 = __shfl_down(gA, 1, 64);
+            gB1 = __shfl_down(gB, 1, 64);
         }
         const Vtx a{A[0], A[1]}, b{B[0], B[1]};
         const Vtx a1{A1[0], A1[1]}, b1{B1[0], B1[1]};
@@ -723,13 +746,6 @@ __global__ void __launch_bounds__(64 * RWY)
     const int len = z_end - z_begin;
     uint32_t plain = 0, weighted = 0, vcount = 0; // per lane
     double vsum = 0.0;
-    // gamma^2 v^2 at one plaquette corner, VelocityKernel.hpp:82-96
-    auto corner = [&](const double *v) -> double {
-        const double R = p.vel_R, RpR = p.vel_RpR;
-        return gamma_sq_v_sq_d(v[0] / R, v[1] / R,
-                               (v[NC - 2] - RpR * v[0]) / (R * R),
-                               (v[NC - 1] - RpR * v[1]) / (R * R), p.vel_mr);
-    };
 
     auto process = [&](int z) {
         // +z edges from plane z to z + 1 at rows y, y+1, and their x+1
@@ -778,23 +794,22 @@ __global__ void __launch_bounds__(64 * RWY)
         weighted += abs(wxy) + abs(wyz) + abs(wzx);
         if constexpr (VEL)
         {
-            if (p.do_vel && (wxy != 0 || wyz != 0 || wzx != 0))
+            if (p.do_vel)
             {
-                const double c000 = corner(Ac), c100 = corner(A1c);
-                const double c010 = corner(Bc), c001 = corner(A);
+                // corners 000, 100, 110, 010 (plane z), 001, 101, 011
                 if (wxy != 0)
                 {
-                    vsum += c000 + c100 + corner(B1c) + c010;
+                    vsum += gAc + gA1c + gB1c + gBc;
                     vcount += 4;
                 }
                 if (wyz != 0)
                 {
-                    vsum += c000 + c010 + corner(B) + c001;
+                    vsum += gAc + gBc + gB + gA;
                     vcount += 4;
                 }
                 if (wzx != 0)
                 {
-                    vsum += c000 + c001 + corner(A1) + c100;
+                    vsum += gAc + gA + gA1 + gA1c;
                     vcount += 4;
                 }
             }
@@ -831,9 +846,17 @@ __global__ void __launch_bounds__(64 * RWY)
                 {
                     Ac[c] = A[c];
                     Bc[c] = B[c];
+                }
+#pragma unroll
+                for (int c = 0; c < 2; ++c)
+                {
                     A1c[c] = A1[c];
                     B1c[c] = B1[c];
                 }
+                gAc = gA;
+                gBc = gB;
+                gA1c = gA1;
+                gB1c = gB1;
                 Wc = W;
                 W1c = W1;
                 land((u + 1) % RD); // plane z + 1
@@ -925,6 +948,8 @@ struct EnergyParams
     int nx, ny, nz;
     uint32_t n;
     double dx, R, lambda, RpR;
+    double iR, iR2, iR4; // 1 / R, 1 / R^2, 1 / R^4 (kernel arguments, so
+                         // they sit in scalar registers)
     int scheme;
     double thr;
     const uint8_t *mask;
@@ -1023,7 +1048,8 @@ __global__ void __launch_bounds__(64 * ETY)
                (uint32_t)(((hx % nx) + nx) % nx);
     }
     const double i1 = 1.0 / p.dx;
-    const double R = p.R, R2 = R * R, R4 = R2 * R2;
+    const double R = p.R;
+    const double iR = p.iR, iR2 = p.iR2, iR4 = p.iR4;
     const double f_a = 1.4142135623730951;
     const int z_begin = bz * p.z_chunk;
     const int z_end = min(z_begin + p.z_chunk, nz);
@@ -1118,20 +1144,23 @@ __global__ void __launch_bounds__(64 * ETY)
             const double c1 = Pi1 - p.RpR * psi1, c2 = Pi2 - p.RpR * psi2;
             const double dpot = psi1 * psi1 + psi2 * psi2 - R * R;
             const double pot = 0.25 * p.lambda * dpot * dpot;
-            const double rho = (c1 * c1 + c2 * c2 + (g1sq + g2sq) + pot) / R4;
+            // the divisions by powers of R as products with their
+            // reciprocals (one rounding apart; positive terms, so the sums
+            // move by ~1e-16 relative)
+            const double rho = (c1 * c1 + c2 * c2 + (g1sq + g2sq) + pot) * iR4;
             const double tp =
                 (psi_sq > 0.0) ? (psi1 * Pi2 - Pi1 * psi2) / psi_sq : 0.0;
-            const double a_dot = f_a * tp / R;
+            const double a_dot = f_a * tp * iR;
             const double rak = 0.5 * a_dot * a_dot;
-            const double rag = 0.5 * f_a * f_a * gtsq / R2;
+            const double rag = 0.5 * f_a * f_a * gtsq * iR2;
             double rrk = 0.0;
             if (psi_sq > 0.0)
             {
                 const double rd = (psi1 * c1 + psi2 * c2) / sqrt(psi_sq);
-                rrk = (rd * rd) / R4;
+                rrk = (rd * rd) * iR4;
             }
-            const double rrg = (g1sq + g2sq - psi_sq * gtsq) / R4;
-            const double rrm = pot / R4;
+            const double rrg = (g1sq + g2sq - psi_sq * gtsq) * iR4;
+            const double rrm = pot * iR4;
             const double w = masking_weight_d(p.scheme, p.thr, psi1, psi2, R);
             acc[0] += rho;
             acc[1] += rho * w;
@@ -1435,6 +1464,9 @@ int axs_energy_sums(axs_level *L, double R, double lambda, double RpR,
     p.R = R;
     p.lambda = lambda;
     p.RpR = RpR;
+    p.iR = 1.0 / R;
+    p.iR2 = 1.0 / (R * R);
+    p.iR4 = 1.0 / ((R * R) * (R * R));
     p.scheme = m->scheme;
     p.thr = m->threshold;
     p.mask = mask;

@@ -151,6 +151,40 @@ def diagnostics(capi, lvl, bg, tau, n):
     return {"rho_tot_unscreened": e[0] / e[13], "n_p": plain, "xi": xi, "R": R}
 
 
+def diag_timing(capi, ctx, lvl, bg, tau, n, reps=3):
+    """Wall time (best of `reps`, device-synchronised) of the per-output
+    diagnostics on the FP32C level after the timed steps, with their
+    algorithmic HBM bytes per cell (FP32C words are read as hi + lo):
+    plaquette counts 16 B (psi), velocities 32 B (psi, Pi), energy sums
+    32 B (psi, Pi), masked a_dot spectrum (fill 32 B + R2C FFT + binning)."""
+    R = capi.background_eval(bg, 0, tau)
+    lam = capi.background_eval(bg, 3, tau)
+    RpR = capi.background_eval(bg, 8, tau)
+    m_r = math.sqrt(lam)
+    calls = {
+        "count_plaquettes": (16, lambda: lvl.count_plaquettes()),
+        "velocity": (32, lambda: lvl.velocity(R, RpR, m_r)),
+        "energy_sums": (32, lambda: lvl.energy_sums(R, lam, RpR,
+                                                     capi.AXS_MASK_B, 0.8)),
+        "spectrum_masked_a_dot": (None, lambda: lvl.spectrum(
+            capi.AXS_FIELD_MASKED_A_DOT, capi.AXS_MASK_B, 0.8, R)),
+    }
+    out = {}
+    for name, (bpc, f) in calls.items():
+        f()
+        ctx.synchronize()
+        best = float("inf")
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            f()
+            ctx.synchronize()
+            best = min(best, time.perf_counter() - t0)
+        out[name] = {"ms": 1e3 * best}
+        if bpc:
+            out[name]["alg_gbs"] = bpc * n ** 3 / best / 1e9
+    return out
+
+
 def precision_run(capi, ctx, n, bg, dt, steps, every, tile32=None, tile64=None):
     """FP32C and FP64 levels side by side from the same IC for `steps` RK4
     steps, with the precision metrics every `every` steps: max |dphi|/eta,
@@ -222,6 +256,8 @@ def main(argv=None):
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-diag", action="store_true",
+                    help="skip the diagnostics timing after the FP32C run")
     ap.add_argument("--place-trials", type=int, default=8,
                     help="placement trials per level before the IC "
                          "(axs_level_place); 0 = keep the first placement")
@@ -317,6 +353,11 @@ def main(argv=None):
     value = ncell * args.steps / wall32
     d32 = diagnostics(capi, a, bg, tau_end, n)
     log(f"[bench] FP32C {value:.4e} cell-updates/s, {1e3 * wall32 / args.steps:.2f} ms/step")
+    diag_ms = None
+    if not args.no_diag:
+        diag_ms = diag_timing(capi, ctx, a, bg, tau_end, n)
+        log("[bench] FP32C diagnostics (ms): " + ", ".join(
+            f"{k} {v['ms']:.2f}" for k, v in diag_ms.items()))
 
     res64 = None
     if not args.no_fp64:
@@ -381,6 +422,7 @@ def main(argv=None):
                    "tile": list(tile) if tile else None,
                    "placement_trials_ms": place32},
         "fp64": res64, "roofline": roofline, "cpu_baseline": cpu,
+        "diagnostics_fp32c": diag_ms,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

@@ -46,7 +46,7 @@ def test_dilation_and_erosion_are_periodic():
 def test_tagged_blocks_equals_dilate_then_block():
     rng = np.random.default_rng(1)
     for _ in range(60):
-        block = int(rng.choice([2, 4, 8]))
+        block = int(rng.choice([2, 4, 8, 16]))
         r = int(rng.integers(0, block + 1))
         shape = tuple(block * int(v) for v in rng.integers(1, 5, 3))
         tags = rng.random(shape) < rng.choice([0.001, 0.01, 0.1])
