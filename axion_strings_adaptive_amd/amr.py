@@ -334,6 +334,42 @@ class Hierarchy:
                 "rho_tot_screened": acc[1] / n_unmasked if n_unmasked else 0.0,
                 "n_total": n_total, "n_unmasked": n_unmasked}
 
+    def composite_level0(self) -> capi.Level:
+        """collapse_to_level0 (AxionStringsLevel.cpp:110-134): copies of the
+        levels averaged down from the finest to level 0, which then holds
+        level 0's own data where nothing refines it and the finer data's
+        average where something does; returned as a uniform level-0
+        resolution field (the caller closes it).  The hierarchy itself is
+        left unchanged."""
+        src = self.levels[-1]
+        tmp = []
+        for l in range(len(self.levels) - 2, -1, -1):
+            L = self.levels[l]
+            c = capi.BoxLevel(self.ctx, L.boxes, self.domains[l], self.dx[l],
+                              self.precision)
+            c.copy_valid_from(L)
+            src.average_down_to(c)
+            tmp.append(c)
+            src = c
+        out = capi.Level(self.ctx, self.domains[0], self.dx[0], self.precision)
+        src.copy_to_level(out)
+        for c in tmp:
+            c.close()
+        return out
+
+    def composite_spectrum(self, kind=capi.AXS_FIELD_MASKED_A_DOT,
+                           scheme=capi.AXS_MASK_B, thr=0.8):
+        """The spectrum of the collapsed level-0 field
+        (AxionStringsLevel.cpp:1617-1619 on collapse_to_level0's result):
+        (shell sums, shell counts, [full cube, inscribed sphere], sum of
+        squares)."""
+        lvl = self.composite_level0()
+        try:
+            R = capi.background_eval(self.bg, 0, self.tau)
+            return lvl.spectrum(kind, scheme, thr, R)
+        finally:
+            lvl.close()
+
     def close(self):
         for L in self.levels:
             L.close()
@@ -361,6 +397,11 @@ def main(argv=None):
     ctx.synchronize()
     dt = time.perf_counter() - t0
     e = h.composite_energy()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ss, sc, tot, _ = h.composite_spectrum()
+    ctx.synchronize()
+    t_spec = time.perf_counter() - t0
     print(json.dumps({
         "levels": len(h.levels), "cells": h.ncells,
         "boxes": [len(L.boxes) for L in h.levels], "regrids": h.nregrids,
@@ -374,7 +415,9 @@ def main(argv=None):
         "regrid_ms_by_phase": {k: 1e3 * v / max(1, h.nregrids - n_regrid0)
                                for k, v in h.regrid_parts.items()},
         "n_total": e["n_total"], "L_tilde^3": (args.n0 * 0.5) ** 3,
-        "rho_tot_unscreened": e["rho_tot_unscreened"]}))
+        "rho_tot_unscreened": e["rho_tot_unscreened"],
+        "composite_spectrum_ms": 1e3 * t_spec,
+        "composite_spectrum_nonempty_shells": int((sc > 0).sum())}))
     h.close()
     ctx.close()
 

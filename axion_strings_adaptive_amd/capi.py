@@ -62,6 +62,7 @@ _SIGS = {
     "axs_ctx_create": (_I, [_I, C.POINTER(_P)]),
     "axs_ctx_destroy": (_I, [_P]),
     "axs_ctx_synchronize": (_I, [_P]),
+    "axs_ctx_trim": (_I, [_P]),
     "axs_ctx_stream": (_P, [_P]),
     "axs_last_error": (C.c_char_p, []),
     "axs_num_tile_variants": (_I, []),
@@ -208,6 +209,10 @@ class Context:
 
     def synchronize(self):
         check(self.lib.axs_ctx_synchronize(self.h))
+
+    def trim(self):
+        """Frees the device arrays kept from destroyed box levels."""
+        check(self.lib.axs_ctx_trim(self.h))
 
     def close(self):
         """Destroys every level still alive on this context, then itself."""

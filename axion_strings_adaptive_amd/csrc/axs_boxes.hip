@@ -61,13 +61,16 @@ struct axs_boxlevel
     int64_t nvalid;
     void *y, *acc, *ya, *yb;
     float *comp;
-    void *alloc[4]; // allocations behind y, acc, ya|yb, comp
+    void *alloc[4]; // allocations behind y, acc, ya|yb, comp (from the
+                    // context's pool)
+    size_t alloc_bytes[4];
     axs::WorkItem *ditems;
     int nitems;
     double *dcoeff;
     void *snap;      // coarse snapshot (y [+ comp]) for time interpolation
     float *snap_lo;
-    size_t snap_bytes;
+    size_t snap_bytes; // bytes in use
+    size_t snap_cap;   // bytes of the pooled block
     unsigned long long snap_uid; // uid of the level the snapshot is of
     unsigned long long *derr;
     // identity of this level for the graph caches of finer levels: unique
@@ -1148,9 +1151,10 @@ int axs_boxlevel_create(axs_ctx *ctx, int nboxes, const int *lo, const int *hi,
     const size_t bytes4 = (size_t)4 * off * b->word;
     // staggered array bases, as for uniform levels (axs_level_create)
     const size_t S = kArrayStagger, pad = 8 * S;
-    bool ok = hipMalloc(&b->alloc[0], bytes4 + pad) == hipSuccess &&
-              hipMalloc(&b->alloc[1], bytes4 + pad) == hipSuccess &&
-              hipMalloc(&b->alloc[2], 2 * (bytes4 + pad)) == hipSuccess &&
+    b->alloc[0] = pool_alloc(ctx, bytes4 + pad, &b->alloc_bytes[0]);
+    b->alloc[1] = pool_alloc(ctx, bytes4 + pad, &b->alloc_bytes[1]);
+    b->alloc[2] = pool_alloc(ctx, 2 * (bytes4 + pad), &b->alloc_bytes[2]);
+    bool ok = b->alloc[0] && b->alloc[1] && b->alloc[2] &&
               hipMalloc(&b->dbox, nboxes * sizeof(BoxD)) == hipSuccess &&
               hipMalloc(&b->downer, owner.size() * sizeof(int)) == hipSuccess &&
               hipMalloc(&b->ditems, items.size() * sizeof(WorkItem)) ==
@@ -1158,11 +1162,14 @@ int axs_boxlevel_create(axs_ctx *ctx, int nboxes, const int *lo, const int *hi,
               hipMalloc(&b->dcoeff, 12 * sizeof(double)) == hipSuccess &&
               hipMalloc(&b->derr, 4 * sizeof(unsigned long long)) == hipSuccess;
     if (ok && precision == AXS_FP32C)
-        ok = hipMalloc(&b->alloc[3], bytes4 + pad) == hipSuccess;
+    {
+        b->alloc[3] = pool_alloc(ctx, bytes4 + pad, &b->alloc_bytes[3]);
+        ok = b->alloc[3] != nullptr;
+    }
     if (!ok)
     {
         for (int i = 0; i < 4; ++i)
-            (void)hipFree(b->alloc[i]);
+            pool_release(ctx, b->alloc[i], b->alloc_bytes[i]);
         (void)hipFree(b->dbox);
         (void)hipFree(b->downer);
         (void)hipFree(b->ditems);
@@ -1205,13 +1212,13 @@ int axs_boxlevel_destroy(axs_boxlevel *b)
         (void)hipGraphDestroy(e.g);
     }
     for (int i = 0; i < 4; ++i)
-        (void)hipFree(b->alloc[i]);
+        pool_release(b->ctx, b->alloc[i], b->alloc_bytes[i]);
     (void)hipFree(b->dbox);
     (void)hipFree(b->downer);
     (void)hipFree(b->ditems);
     (void)hipFree(b->dcoeff);
     (void)hipFree(b->derr);
-    (void)hipFree(b->snap);
+    pool_release(b->ctx, b->snap, b->snap_cap);
     delete b;
     return AXS_OK;
 }
@@ -1380,9 +1387,14 @@ int axs_boxlevel_snapshot_coarse(axs_boxlevel *b, const axs_boxlevel *c)
             (void)hipGraphDestroy(e.g);
         }
         b->graphs.clear();
-        (void)hipFree(b->snap);
-        b->snap = nullptr;
-        AXS_HIP_TRY(hipMalloc(&b->snap, sb + lb));
+        pool_release(b->ctx, b->snap, b->snap_cap);
+        b->snap = pool_alloc(b->ctx, sb + lb, &b->snap_cap);
+        if (!b->snap)
+        {
+            b->snap_bytes = b->snap_cap = 0;
+            return set_error(AXS_ERR_NOMEM,
+                             "axs_boxlevel_snapshot_coarse: out of memory");
+        }
         b->snap_bytes = sb + lb;
     }
     b->snap_lo = lb ? (float *)((char *)b->snap + sb) : nullptr;

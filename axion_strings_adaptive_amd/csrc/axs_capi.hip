@@ -223,6 +223,57 @@ static void free_arrays(axs_level *L)
     L->comp = nullptr;
 }
 
+
+void *pool_alloc(axs_ctx *c, size_t bytes, size_t *got)
+{
+    int best = -1;
+    for (int i = 0; i < (int)c->pool.size(); ++i)
+    {
+        const size_t b = c->pool[i].second;
+        if (b >= bytes && b / 2 <= bytes &&
+            (best < 0 || b < c->pool[best].second))
+            best = i;
+    }
+    if (best >= 0)
+    {
+        void *p = c->pool[best].first;
+        *got = c->pool[best].second;
+        c->pool.erase(c->pool.begin() + best);
+        return p;
+    }
+    const size_t want = bytes + bytes / 8;
+    void *p = nullptr;
+    if (hipMalloc(&p, want) != hipSuccess)
+    {
+        (void)hipGetLastError();
+        pool_trim(c);
+        if (hipMalloc(&p, want) != hipSuccess)
+        {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+    }
+    *got = want;
+    return p;
+}
+
+void pool_release(axs_ctx *c, void *p, size_t bytes)
+{
+    if (!p)
+        return;
+    if (bytes >= (size_t(64) << 20) && c->pool.size() < 12)
+        c->pool.emplace_back(p, bytes);
+    else
+        (void)hipFree(p);
+}
+
+void pool_trim(axs_ctx *c)
+{
+    for (auto &e : c->pool)
+        (void)hipFree(e.first);
+    c->pool.clear();
+}
+
 } // namespace axs
 
 using namespace axs;
@@ -268,6 +319,7 @@ int axs_ctx_destroy(axs_ctx *c)
         return AXS_OK;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    pool_trim(c);
     (void)hipFree(c->d_counts);
     (void)hipFree(c->d_partials);
     (void)hipFree(c->d_result);
@@ -275,6 +327,14 @@ int axs_ctx_destroy(axs_ctx *c)
     (void)hipHostFree(c->h_counts);
     (void)hipStreamDestroy(c->stream);
     delete c;
+    return AXS_OK;
+}
+
+int axs_ctx_trim(axs_ctx *c)
+{
+    AXS_CHECK_ARG(c, "axs_ctx_trim: NULL context");
+    AXS_HIP_TRY(hipStreamSynchronize(c->stream));
+    pool_trim(c);
     return AXS_OK;
 }
 

@@ -7,6 +7,8 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "../../include/axs.h"
 
@@ -40,6 +42,16 @@ constexpr int kMaxPartials = 65536;
 // Offset step between the state arrays of a level (see axs_level_create).
 constexpr size_t kArrayStagger = (size_t(1) << 20) + (size_t(36) << 10) + 768;
 
+// Device allocation for box-level arrays: the smallest pooled block of at
+// least `bytes` (and at most twice that), else hipMalloc of `bytes` plus an
+// eighth (room for the next level to grow into), releasing the pool and
+// retrying once if that fails.  *got receives the block's size.
+void *pool_alloc(axs_ctx *c, size_t bytes, size_t *got);
+// Keeps a block of >= 64 MiB for reuse (at most 12 blocks), else frees it.
+void pool_release(axs_ctx *c, void *p, size_t bytes);
+// Frees every pooled block.
+void pool_trim(axs_ctx *c);
+
 struct Grid
 {
     int nx, ny, nz;
@@ -58,6 +70,9 @@ struct axs_ctx
     double *d_result;             // [64]
     double *h_result;             // pinned [64]
     unsigned long long *h_counts; // pinned [8]
+    // large device arrays of destroyed box levels, kept for the next box
+    // level (regrids create a level while the one it replaces still lives)
+    std::vector<std::pair<void *, size_t>> pool;
 };
 
 struct axs_level

@@ -107,6 +107,11 @@ typedef struct
 int axs_ctx_create(int device, axs_ctx **out);
 int axs_ctx_destroy(axs_ctx *ctx);
 int axs_ctx_synchronize(axs_ctx *ctx);
+/* Frees the device arrays the context keeps from destroyed box levels for
+ * reuse by the next box level (a regrid creates the new level while the one
+ * it replaces still exists).  No counterpart in the reference (AMReX's arena
+ * plays this role there); destroying the context frees them too. */
+int axs_ctx_trim(axs_ctx *ctx);
 /* The HIP stream the context launches on (a hipStream_t). */
 void *axs_ctx_stream(axs_ctx *ctx);
 const char *axs_last_error(void);

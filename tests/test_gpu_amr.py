@@ -134,3 +134,31 @@ def test_regridding_hierarchy_steps_and_keeps_volume(ctx):
     assert h2.nregrids == 2
     h2.close()
     h.close()
+
+
+def test_composite_level0_collapses_by_average_down(ctx):
+    """collapse_to_level0: level 0 where nothing refines it, the 2x2x2
+    average of the (recursively collapsed) finer level where something
+    does; the hierarchy itself unchanged."""
+    from axion_strings_adaptive_amd import amr, capi
+    h = amr.Hierarchy(ctx, n0=32, levels=2, precision=capi.AXS_FP64,
+                      verbose=False, regrid_int=1)
+    h.step()  # regrid: fine data partly interpolated, not yet averaged down
+    dense = [L.download_dense() for L in h.levels]
+    want = dense[-1]
+    for l in range(len(dense) - 2, -1, -1):
+        d = dense[l].copy()
+        nz, ny, nx = d.shape[1:]
+        avg = want.reshape(4, nz, 2, ny, 2, nx, 2).mean(axis=(2, 4, 6))
+        covered = ~np.isnan(avg[0])
+        d[:, covered] = avg[:, covered]
+        want = d
+    got = h.composite_level0()
+    g = got.download()
+    got.close()
+    np.testing.assert_allclose(g, want, rtol=1e-13, atol=1e-13)
+    for L, d in zip(h.levels, dense):
+        assert np.array_equal(L.download_dense(), d, equal_nan=True)
+    ss, sc, tot, sumsq = h.composite_spectrum()
+    assert (sc > 0).sum() > 0 and np.all(np.isfinite(ss))
+    h.close()

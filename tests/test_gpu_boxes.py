@@ -179,3 +179,31 @@ def test_two_level_subcycled_step_tracks_uniform_fine(ctx, oracle):
         ref.rk4_step(c, dtf)
         tau += dtf
     assert np.array_equal(fine.download_dense(), ref.download())
+
+
+def test_recycled_arrays_start_clean(ctx, oracle):
+    """A box level built on arrays the context kept from a destroyed one
+    starts from zero state and steps exactly like one on fresh arrays; trim
+    releases the kept arrays.  192^3 in 64^3 boxes: arrays of ~150 MB,
+    above the size the context keeps."""
+    capi = _capi()
+    n = 192
+    boxes = tiling(n, [64])
+    s0 = np.random.default_rng(7).normal(size=(4, n, n, n))
+    coeffs = [capi.RhsCoeffs(0.0, 1.0, 1.0)] * 4
+    first = capi.BoxLevel(ctx, boxes, (n, n, n), 0.5, capi.AXS_FP32C)
+    first.upload_dense(s0)
+    first.fill_ghosts()
+    first.rk4_step(coeffs, 0.1)
+    want = first.download_dense()
+    first.close()
+    for trim in (False, True):
+        if trim:
+            ctx.trim()
+        b = capi.BoxLevel(ctx, boxes, (n, n, n), 0.5, capi.AXS_FP32C)
+        assert np.array_equal(b.download_dense(), np.zeros_like(want))
+        b.upload_dense(s0)
+        b.fill_ghosts()
+        b.rk4_step(coeffs, 0.1)
+        assert np.array_equal(b.download_dense(), want)
+        b.close()
