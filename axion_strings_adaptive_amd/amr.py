@@ -5,7 +5,7 @@ Level 0 is a periodic base grid (tiled into boxes); each finer level
 the plaquette winding is nonzero (StringTagger.hpp:240-274) or, as an
 extension, where |psi|/R < amplitude_threshold -- grown by n_error_buf cells
 (the reference's amr.n_error_buf), cut into blocking-factor blocks and
-merged along x into boxes of at most max_grid_size cells
+merged along x, then y, then z into boxes of at most max_grid_size cells
 (amr.blocking_factor / amr.max_grid_size), and kept properly nested one
 block inside the coarser level's coverage.
 

@@ -186,37 +186,11 @@ __device__ double coarse_value(const CoarseSrc<T> &c, int comp, int cx, int cy,
     return v;
 }
 
-// Tensor-product quartic interpolation of coarse component `comp` at the
-// fine cell (fx, fy, fz) (ratio 2, cell centred).
-template <typename T>
-__device__ double interp_quartic(const CoarseSrc<T> &c, int comp, int fx,
-                                 int fy, int fz, unsigned long long *err)
-{
-    const int cx = fx >> 1, cy = fy >> 1, cz = fz >> 1; // floor for fx >= 0
-    const double *wx = kQuartW[fx & 1], *wy = kQuartW[fy & 1],
-                 *wz = kQuartW[fz & 1];
-    double acc = 0.0;
-    for (int k = 0; k < 5; ++k)
-    {
-        double ay = 0.0;
-        for (int j = 0; j < 5; ++j)
-        {
-            double ax = 0.0;
-            for (int i = 0; i < 5; ++i)
-                ax += wx[i] * coarse_value(c, comp, cx + i - 2, cy + j - 2,
-                                           cz + k - 2, err);
-            ay += wy[j] * ax;
-        }
-        acc += wz[k] * ay;
-    }
-    return acc;
-}
-
 // Interior prolongation: one thread per coarse cell under a fine box's valid
 // region (box bounds are even) computes its 8 fine children at once, so each
 // coarse neighbour is located once for the 4 components and read once for
 // the 8 children.  Per child the tensor-product sums run in the same order
-// as interp_quartic's (x innermost, then y, then z), so the values are the
+// as interp_quartic4's (x innermost, then y, then z), so the values are the
 // same.
 template <typename T>
 __global__ void __launch_bounds__(256)
@@ -309,28 +283,115 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-// ghost fill (and optional interior prolongation) of one buffer
+// Ghost index t of a grown box -> local cell (lx, ly, lz): the ghost shell
+// enumerated as the two z slabs (full grown planes), then the two y slabs of
+// each valid plane (full grown rows), then the two x runs of each valid
+// row, so consecutive threads walk along x wherever the shell allows.
+__device__ __forceinline__ void ghost_cell(uint32_t t, int nx, int ny, int nz,
+                                           int ng, int &lx, int &ly, int &lz)
+{
+    const uint32_t gx = nx + 2 * ng, gy = ny + 2 * ng;
+    const uint32_t A = (uint32_t)ng * gy * gx; // one z slab
+    if (t < 2 * A)
+    {
+        const uint32_t r = t % A;
+        lz = (t < A ? -ng : nz) + (int)(r / (gy * gx));
+        ly = (int)((r % (gy * gx)) / gx) - ng;
+        lx = (int)(r % gx) - ng;
+        return;
+    }
+    t -= 2 * A;
+    const uint32_t Bp = 2u * ng * gx; // y slabs of one valid plane
+    if (t < Bp * (uint32_t)nz)
+    {
+        lz = (int)(t / Bp);
+        const uint32_t r = t % Bp, half = (uint32_t)ng * gx;
+        ly = (r < half ? -ng : ny) + (int)((r % half) / gx);
+        lx = (int)(r % gx) - ng;
+        return;
+    }
+    t -= Bp * (uint32_t)nz;
+    const uint32_t row = t / (2u * ng), k = t % (2u * ng);
+    lz = (int)(row / (uint32_t)ny);
+    ly = (int)(row % (uint32_t)ny);
+    lx = k < (uint32_t)ng ? (int)k - ng : nx + (int)k - ng;
+}
+
+// Tensor-product quartic interpolation of the 4 coarse components at the
+// fine cell (fx, fy, fz) (ratio 2, cell centred): each coarse neighbour is
+// located once; per component the sums run x innermost, then y, then z.
+template <typename T>
+__device__ void interp_quartic4(const CoarseSrc<T> &c, int fx, int fy, int fz,
+                                double (&out)[4], unsigned long long *err)
+{
+    const int cx = fx >> 1, cy = fy >> 1, cz = fz >> 1;
+    const double *wx = kQuartW[fx & 1], *wy = kQuartW[fy & 1],
+                 *wz = kQuartW[fz & 1];
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < 5; ++k)
+    {
+        double ay[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int j = 0; j < 5; ++j)
+        {
+            double ax[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int i = 0; i < 5; ++i)
+            {
+                uint32_t ci;
+                const bool ok =
+                    locate(c.v, cx + i - 2, cy + j - 2, cz + k - 2, ci) >= 0;
+                if (!ok)
+                    atomicAdd(err, 1ull);
+                for (int q = 0; q < 4; ++q)
+                {
+                    double v = 0.0;
+                    if (ok)
+                    {
+                        const size_t o = (size_t)q * c.v.total + ci;
+                        v = (double)c.cur[o] +
+                            (c.cur_lo ? (double)c.cur_lo[o] : 0.0);
+                        if (c.snap && c.t_frac < 1.0)
+                        {
+                            const double sv =
+                                (double)c.snap[o] +
+                                (c.snap_lo ? (double)c.snap_lo[o] : 0.0);
+                            v = (1.0 - c.t_frac) * sv + c.t_frac * v;
+                        }
+                    }
+                    ax[q] += wx[i] * v;
+                }
+            }
+            for (int q = 0; q < 4; ++q)
+                ay[q] += wy[j] * ax[q];
+        }
+        for (int q = 0; q < 4; ++q)
+            acc[q] += wz[k] * ay[q];
+    }
+    for (int q = 0; q < 4; ++q)
+        out[q] = acc[q];
+}
+
+// ghost fill of one buffer: each ghost cell from the box that owns it on
+// this level, else (coarse-fine boundary) by quartic interpolation from
+// the coarse level
 template <typename T>
 __global__ void k_fill(LevelView f, T *buf, float *lo, int have_coarse,
-                       CoarseSrc<T> c, int interior, unsigned long long *err)
+                       CoarseSrc<T> c, unsigned long long *err)
 {
     const BoxD B = f.box[blockIdx.y];
     const int ng = f.ng;
-    const int gx = B.n[0] + 2 * ng, gy = B.n[1] + 2 * ng, gz = B.n[2] + 2 * ng;
-    const uint32_t cells = (uint32_t)gx * gy * gz;
+    const uint32_t gx = B.n[0] + 2 * ng, gy = B.n[1] + 2 * ng,
+                   gz = B.n[2] + 2 * ng;
+    const uint32_t cells =
+        gx * gy * gz - (uint32_t)B.n[0] * B.n[1] * B.n[2];
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < cells;
          t += gridDim.x * blockDim.x)
     {
-        const int lx = (int)(t % gx) - ng, ly = (int)((t / gx) % gy) - ng,
-                  lz = (int)(t / ((uint32_t)gx * gy)) - ng;
-        const bool valid = lx >= 0 && lx < B.n[0] && ly >= 0 && ly < B.n[1] &&
-                           lz >= 0 && lz < B.n[2];
-        if (valid && !interior)
-            continue;
-        const uint32_t dst = B.off + t;
+        int lx, ly, lz;
+        ghost_cell(t, B.n[0], B.n[1], B.n[2], ng, lx, ly, lz);
+        const uint32_t dst = sidx(B, ng, lx, ly, lz);
         const int X = B.lo[0] + lx, Y = B.lo[1] + ly, Z = B.lo[2] + lz;
         uint32_t src;
-        if (!interior && locate(f, X, Y, Z, src) >= 0)
+        if (locate(f, X, Y, Z, src) >= 0)
         {
             for (int comp = 0; comp < 4; ++comp)
             {
@@ -347,9 +408,11 @@ __global__ void k_fill(LevelView f, T *buf, float *lo, int have_coarse,
         }
         const int px = pmod(X, f.dom[0]), py = pmod(Y, f.dom[1]),
                   pz = pmod(Z, f.dom[2]);
+        double v4[4];
+        interp_quartic4(c, px, py, pz, v4, err);
         for (int comp = 0; comp < 4; ++comp)
         {
-            const double v = interp_quartic(c, comp, px, py, pz, err);
+            const double v = v4[comp];
             const T h = (T)v;
             buf[(size_t)comp * f.total + dst] = h;
             if (lo)
@@ -990,9 +1053,9 @@ static int fill_t(axs_boxlevel *bl, void *buf, float *lo,
                            (T *)buf, lo, c, bl->derr);
         AXS_HIP_TRY(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_fill<T>, box_grid(bl, max_grown(bl)), dim3(256), 0,
-                       bl->ctx->stream, view_of(bl), (T *)buf, lo,
-                       coarse ? 1 : 0, c, 0, bl->derr);
+    hipLaunchKernelGGL(k_fill<T>, box_grid(bl, max_grown(bl) - max_valid(bl)),
+                       dim3(256), 0, bl->ctx->stream, view_of(bl), (T *)buf,
+                       lo, coarse ? 1 : 0, c, bl->derr);
     AXS_HIP_TRY(hipGetLastError());
     return AXS_OK;
 }
