@@ -142,6 +142,8 @@ AxionStringsLevel::~AxionStringsLevel()
         std::fclose(m_spectrum);
     if (m_pre_scalars)
         std::fclose(m_pre_scalars);
+    if (m_loop_scalars)
+        std::fclose(m_loop_scalars);
     axs_level_destroy(m_level);
     axs_ctx_destroy(m_ctx);
 }
@@ -525,6 +527,11 @@ void AxionStringsLevel::diagnostics()
     const PlaquetteCounts counts = count_plaquettes(m_level);
     const double ell = (2.0 / 3.0) * (double)counts.n_p_plain * m_dx;
     const double ell_w = (2.0 / 3.0) * (double)counts.n_p_weighted * m_dx;
+    if (m_flat)
+    {
+        flat_diagnostics(counts, ell, m_r);
+        return;
+    }
     const double xi = m_flat ? 0.0 : xi_from_comoving_length(ell, m_L, m_bg.a_inv, tau);
     const double xi_w =
         m_flat ? 0.0 : xi_from_comoving_length(ell_w, m_L, m_bg.a_inv, tau);
@@ -591,6 +598,53 @@ void AxionStringsLevel::diagnostics()
                      s_scr.inscribed_sphere_energy / nsq,
                      s_uns.inscribed_sphere_energy / nsq);
     std::fflush(m_spectrum);
+}
+
+// The flat-space branch of specific_post_timestep (AxionStringsLevel.cpp:
+// 932-1040): string length, energy with R = 1 (n_total * rho_tot is the
+// conserved total energy), mean gamma^2 v^2 at pierced corners, one
+// loop_scalars.dat row per output; no xi, tension or spectrum.
+void AxionStringsLevel::flat_diagnostics(const PlaquetteCounts &counts,
+                                         double ell, double m_r)
+{
+    const double t = m_tau;
+    const VelocityResult vel =
+        compute_velocity_at_pierced_corners(m_level, 1.0, 0.0, m_r);
+    const double vw_sum = vel.sum_gamma_sq_v_sq * m_dx;
+    const double vw_cnt = (double)vel.count * m_dx;
+    const TotalEnergyResult e = compute_composite_total_energy(
+        {EnergyLevelInput{m_level, m_dx, nullptr}}, 1.0, m_r * m_r, 0.0,
+        m_masking);
+    const double total_energy = e.n_total * e.rho_tot_unscreened;
+    const double mean_g2v2 = vw_cnt > 0.0 ? vw_sum / vw_cnt : 0.0;
+    const double mean_gamma = std::sqrt(1.0 + mean_g2v2);
+    std::cout << "  [AxionStrings flat-space] t = " << t
+              << "  N_p = " << counts.n_p_plain << "  string length = " << ell
+              << "  rho_tot(unscreened) = " << e.rho_tot_unscreened
+              << "  total energy = " << total_energy
+              << "  <gamma^2 v^2> = " << mean_g2v2
+              << "  <gamma> = " << mean_gamma
+              << "  N_corners = " << vel.count << "\n";
+    if (!m_loop_scalars)
+        // column names of AxionStringsLevel.cpp:1027-1030 after the time
+        m_loop_scalars = open_dat("loop_scalars.dat",
+                                  "# time N_p string_length "
+                                  "rho_tot_unscreened total_energy "
+                                  "mean_gamma_sq_v_sq mean_gamma "
+                                  "n_velocity_corners\n",
+                                  t);
+    const double row[8] = {t,
+                           (double)counts.n_p_plain,
+                           ell,
+                           e.rho_tot_unscreened,
+                           total_energy,
+                           mean_g2v2,
+                           mean_gamma,
+                           (double)vel.count};
+    for (int i = 0; i < 8; ++i)
+        std::fprintf(m_loop_scalars, i ? " %.10e" : "%.10e", row[i]);
+    std::fprintf(m_loop_scalars, "\n");
+    std::fflush(m_loop_scalars);
 }
 
 void AxionStringsLevel::write_network_scalars_row(

@@ -67,6 +67,8 @@ class AxionStringsLevel
     void stage_coeffs_at(double a_time, axs_rhs_coeffs *c) const;
     bool output_due();
     long steps_until_next_output();
+    void flat_diagnostics(const PlaquetteCounts &counts, double ell,
+                          double m_r);
     void write_network_scalars_row(const PlaquetteCounts &c, double xi,
                                    double xi_w, double m_r_over_h,
                                    const TotalEnergyResult &e,
@@ -98,6 +100,7 @@ class AxionStringsLevel
     FILE *m_scalars{nullptr};
     FILE *m_spectrum{nullptr};
     FILE *m_pre_scalars{nullptr};
+    FILE *m_loop_scalars{nullptr};
     // pre-evolution (ic_mode = fourier_relaxed)
     Phase m_phase{Phase::Evolving};
     PreEvolutionBackground m_pre;

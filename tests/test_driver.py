@@ -278,3 +278,39 @@ def test_driver_restart_continues_bit_for_bit(tmp_path, precision):
     assert any(f.name.startswith("network_scalars.dat.old.")
                for f in seg.iterdir())
     assert len(np.atleast_2d(np.loadtxt(seg / "network_scalars.dat"))) == 1
+
+
+@pytest.mark.gpu
+def test_driver_flat_space_loop_scalars(tmp_path, ctx):
+    """Flat space (axion_strings.flat_space = 1): R = 1, lambda = m_r^2, a
+    step-count cadence, and loop_scalars.dat rows (AxionStringsLevel.cpp:
+    932-1040) equal to the same quantities through the C ABI."""
+    from axion_strings_adaptive_amd import capi
+    p = os.path.join(PARAMS, "params_plain_64.txt")
+    r = run([p, "amr.n_cell=32 32 32", "geometry.prob_extent=16 16 16",
+             "axion_strings.flat_space=1", "axion_strings.flat.m_r=1.0",
+             "axion_strings.flat.output_cadence_steps=10",
+             "axion_strings.k_max_over_mr=8", "evolution.max_steps=20",
+             "--out", str(tmp_path)])
+    assert r.returncode == 0, r.stderr
+    assert not (tmp_path / "network_scalars.dat").exists()
+    rows = np.loadtxt(tmp_path / "loop_scalars.dat")
+    assert rows.shape == (2, 8)
+    lvl = capi.Level(ctx, 32, 0.5, capi.AXS_FP64)
+    lvl.fourier_ic(0, 4.0, 0.5, 42, 0)
+    lvl.fourier_ic(1, 4.0, 0.5, 42, 1)
+    lvl.zero_components(2, 2)
+    bg = capi.make_background(flat_space=True, flat_m_r=1.0)
+    lvl.advance(bg, 0.0, 0.1, 20)
+    assert rows[1, 0] == pytest.approx(2.0, rel=1e-12)
+    plain, _ = lvl.count_plaquettes()
+    assert rows[1, 1] == plain
+    assert rows[1, 2] == pytest.approx((2.0 / 3.0) * plain * 0.5, rel=1e-12)
+    e = lvl.energy_sums(1.0, 1.0, 0.0, capi.AXS_MASK_B, 0.8)
+    assert rows[1, 3] == pytest.approx(e[0] / e[13], rel=1e-9)
+    assert rows[1, 4] == pytest.approx(e[0] * 0.5 ** 3, rel=1e-9)
+    total, count, _ = lvl.velocity(1.0, 0.0, 1.0)
+    assert rows[1, 7] == count
+    g2v2 = total / count if count else 0.0
+    assert rows[1, 5] == pytest.approx(g2v2, rel=1e-9)
+    assert rows[1, 6] == pytest.approx(np.sqrt(1.0 + g2v2), rel=1e-9)
