@@ -111,26 +111,25 @@ struct WindParams
     int use_lap, use_rgrad;
     // string velocities at pierced corners (VelocityKernel.hpp:37-141)
     int do_vel;
-    double vel_R, vel_RpR, vel_mr;
+    double vel_RpR;
+    // host-side reciprocals for gamma_sq_v_sq_k: 1/R, 1/R^2,
+    // 1/(2 m_r^2 c1^2), e1/(c1 c1^2), e1/(4 m_r^2 c1^4 c1)
+    double vel_iR, vel_iR2, vel_k1, vel_kc, vel_k3;
     double *vel_partials; // [gridDim.x]
 };
 
-// gamma_sq_v_sq_from_phi, Velocity.hpp:62-78
-__device__ __forceinline__ double gamma_sq_v_sq_d(double phi1, double phi2,
+// gamma_sq_v_sq_from_phi (Velocity.hpp:62-78) with its divisions by
+// uniform constants taken as products with host-computed reciprocals (one
+// rounding apart per term; the sums over corners move by ~1e-16 relative)
+__device__ __forceinline__ double gamma_sq_v_sq_k(double phi1, double phi2,
                                                   double pd1, double pd2,
-                                                  double m_r)
+                                                  double k1, double kc,
+                                                  double k3)
 {
-    const double c1 = 0.41222;
-    const double e1 = -0.025763;
     const double phi_sq = phi1 * phi1 + phi2 * phi2;
     const double phi_dot_sq = pd1 * pd1 + pd2 * pd2;
     const double re = phi1 * pd1 + phi2 * pd2;
-    const double m_r_sq = m_r * m_r;
-    const double c1_sq = c1 * c1;
-    const double term1 = (phi_dot_sq / (2.0 * m_r_sq * c1_sq)) *
-                         (1.0 - e1 * phi_sq / (c1 * c1_sq));
-    const double term2 = (e1 / (4.0 * m_r_sq * c1_sq * c1_sq * c1)) * re * re;
-    return term1 - term2;
+    return (phi_dot_sq * k1) * (1.0 - kc * phi_sq) - k3 * re * re;
 }
 
 constexpr int WTY = 8;
@@ -305,9 +304,10 @@ __global__ void __launch_bounds__(64 * WTY)
         const double v1 = ring[1][s][ty + oy][tx + ox];
         const double v2 = ring[NC - 2][s][ty + oy][tx + ox];
         const double v3 = ring[NC - 1][s][ty + oy][tx + ox];
-        const double R = p.vel_R, RpR = p.vel_RpR;
-        return gamma_sq_v_sq_d(v0 / R, v1 / R, (v2 - RpR * v0) / (R * R),
-                               (v3 - RpR * v1) / (R * R), p.vel_mr);
+        const double RpR = p.vel_RpR, iR = p.vel_iR, iR2 = p.vel_iR2;
+        return gamma_sq_v_sq_k(v0 * iR, v1 * iR, (v2 - RpR * v0) * iR2,
+                               (v3 - RpR * v1) * iR2, p.vel_k1, p.vel_kc,
+                               p.vel_k3);
     };
 
     const int z_begin = bz * p.z_chunk;
@@ -696,10 +696,11 @@ __global__ void __launch_bounds__(64 * RWY)
     // gamma^2 v^2 at a vertex, VelocityKernel.hpp:82-96 (velocity passes
     // evaluate it once per vertex; a pierced plaquette sums its corners)
     auto gamma_of = [&](const double *v) -> double {
-        const double R = p.vel_R, RpR = p.vel_RpR;
-        return gamma_sq_v_sq_d(v[0] / R, v[1] / R,
-                               (v[NC - 2] - RpR * v[0]) / (R * R),
-                               (v[NC - 1] - RpR * v[1]) / (R * R), p.vel_mr);
+        const double RpR = p.vel_RpR, iR = p.vel_iR, iR2 = p.vel_iR2;
+        return gamma_sq_v_sq_k(v[0] * iR, v[1] * iR,
+                               (v[NC - 2] - RpR * v[0]) * iR2,
+                               (v[NC - 1] - RpR * v[1]) * iR2, p.vel_k1,
+                               p.vel_kc, p.vel_k3);
     };
     // plane values: own column (a: row y, b: row y+1) and psi of the x+1
     // column; gamma^2 v^2 of the same four vertices
@@ -1348,9 +1349,15 @@ static int run_windings(axs_level *L, const uint8_t *mask, int8_t *wfield,
         AXS_CHECK_ARG(ntiles <= kMaxPartials * 16,
                       "axs_velocity_at_pierced_corners: grid too large");
         p.do_vel = 1;
-        p.vel_R = vel->R;
         p.vel_RpR = vel->RpR;
-        p.vel_mr = vel->m_r;
+        // the constants of gamma_sq_v_sq_from_phi (Velocity.hpp:62-78)
+        const double c1 = 0.41222, e1 = -0.025763;
+        const double m_r_sq = vel->m_r * vel->m_r, c1_sq = c1 * c1;
+        p.vel_iR = 1.0 / vel->R;
+        p.vel_iR2 = 1.0 / (vel->R * vel->R);
+        p.vel_k1 = 1.0 / (2.0 * m_r_sq * c1_sq);
+        p.vel_kc = e1 / (c1 * c1_sq);
+        p.vel_k3 = e1 / (4.0 * m_r_sq * c1_sq * c1_sq * c1);
         p.vel_partials = L->ctx->d_partials;
     }
     hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, s, p.counts);
