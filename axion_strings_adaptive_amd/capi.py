@@ -513,12 +513,13 @@ class BoxLevel:
                  t0_frac, dt_frac))
 
     def tag(self, amp_thr=0.0, R=1.0, tags: Optional[DeviceBuffer] = None,
-            cover: Optional[DeviceBuffer] = None):
+            cover: Optional[DeviceBuffer] = None, grad_thr=DBL_MAX,
+            rgrad_thr=DBL_MAX):
         nx, ny, nz = self.domain
         if tags is None:
             tags = DeviceBuffer(nx * ny * nz)
             tags.zero()
-        p = TaggerParams(DBL_MAX, DBL_MAX, amp_thr, R)
+        p = TaggerParams(grad_thr, rgrad_thr, amp_thr, R)
         counts = (C.c_int64 * 2)()
         check(self.lib.axs_boxlevel_tag(
             self.h, C.byref(p), _P(tags.ptr),

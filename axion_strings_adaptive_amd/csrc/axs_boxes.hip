@@ -634,11 +634,41 @@ __device__ __forceinline__ int pwind(double t1, double t2, double t3, double t4)
     return (int)lround(total / two_pi);
 }
 
+// FourthOrderDerivatives diff1 / diff2 (published 4th-order forms), the
+// same arithmetic as the uniform-level tagger
+__device__ __forceinline__ double tag_d1(double fm2, double fm1, double fp1,
+                                         double fp2, double i1)
+{
+#pragma clang fp contract(off)
+    const double w_far = 8.33333333333333333333e-2;
+    const double w_near = 6.66666666666666666667e-1;
+    return (w_far * fm2 - w_near * fm1 + w_near * fp1 - w_far * fp2) * i1;
+}
+__device__ __forceinline__ double tag_d2(double fm2, double fm1, double f0,
+                                         double fp1, double fp2, double i2)
+{
+#pragma clang fp contract(off)
+    const double w_far = 8.33333333333333333333e-2;
+    const double w_near = 1.33333333333333333333e+0;
+    const double w_local = 2.50000000000000000000e+0;
+    return (-w_far * fm2 + w_near * fm1 - w_local * f0 + w_near * fp1 -
+            w_far * fp2) *
+           i2;
+}
+
+struct BoxTagParams
+{
+    double amp_thr, R;
+    // StringTagger.hpp:83-218 derivative criteria (DBL_MAX = off)
+    double grad_thr, rgrad_thr, dx;
+};
+
 template <typename T>
 __global__ void k_box_tag(LevelView v, const T *y, const float *lo,
-                          uint8_t *tags, const uint8_t *cover, double amp_thr,
-                          double R, unsigned long long *counts)
+                          uint8_t *tags, const uint8_t *cover, BoxTagParams tp,
+                          unsigned long long *counts)
 {
+    const double amp_thr = tp.amp_thr, R = tp.R;
     const BoxD B = v.box[blockIdx.y];
     const uint32_t cells = (uint32_t)B.n[0] * B.n[1] * B.n[2];
     const size_t tot = v.total;
@@ -675,13 +705,63 @@ __global__ void k_box_tag(LevelView v, const T *y, const float *lo,
                 if (tags)
                 {
                     bool tag = wxy != 0 || wyz != 0 || wzx != 0;
+                    // component c at offset (a, b, d) from the cell
+                    auto at = [&](int c, int a, int b, int d) -> double {
+                        const uint32_t i =
+                            sidx(B, v.ng, lx + a, ly + b, lz + d);
+                        const size_t o = (size_t)c * tot + i;
+                        return (double)y[o] + (lo ? (double)lo[o] : 0.0);
+                    };
+                    const double psi1 = at(0, 0, 0, 0), psi2 = at(1, 0, 0, 0);
                     if (!tag && amp_thr > 0.0)
+                        tag = sqrt(psi1 * psi1 + psi2 * psi2) / R < amp_thr;
+                    const double i1 = 1.0 / tp.dx, i2 = 1.0 / (tp.dx * tp.dx);
+                    if (!tag && tp.grad_thr < DBL_MAX)
                     {
-                        const uint32_t i = sidx(B, v.ng, lx, ly, lz);
-                        const double p1 = (double)y[i] + (lo ? (double)lo[i] : 0.0);
-                        const double p2 = (double)y[tot + i] +
-                                          (lo ? (double)lo[tot + i] : 0.0);
-                        tag = sqrt(p1 * p1 + p2 * p2) / R < amp_thr;
+                        // dx^2 |laplacian psi_c| > threshold
+                        const double dx2 = tp.dx * tp.dx;
+                        double lap[2];
+                        for (int c = 0; c < 2; ++c)
+                        {
+                            const double f0 = at(c, 0, 0, 0);
+                            double rr = 0.0;
+                            rr += tag_d2(at(c, -2, 0, 0), at(c, -1, 0, 0), f0,
+                                         at(c, 1, 0, 0), at(c, 2, 0, 0), i2);
+                            rr += tag_d2(at(c, 0, -2, 0), at(c, 0, -1, 0), f0,
+                                         at(c, 0, 1, 0), at(c, 0, 2, 0), i2);
+                            rr += tag_d2(at(c, 0, 0, -2), at(c, 0, 0, -1), f0,
+                                         at(c, 0, 0, 1), at(c, 0, 0, 2), i2);
+                            lap[c] = rr;
+                        }
+                        if (dx2 * fabs(lap[0]) > tp.grad_thr ||
+                            dx2 * fabs(lap[1]) > tp.grad_thr)
+                            tag = true;
+                    }
+                    if (!tag && tp.rgrad_thr < DBL_MAX)
+                    {
+                        // dx sqrt(|grad psi|^2 - |psi|^2 |grad theta|^2)
+                        const double psq = psi1 * psi1 + psi2 * psi2;
+                        double g1sq = 0.0, g2sq = 0.0, gtsq = 0.0;
+                        for (int d = 0; d < 3; ++d)
+                        {
+                            const int ex = d == 0, ey = d == 1, ez = d == 2;
+                            const double a = tag_d1(
+                                at(0, -2 * ex, -2 * ey, -2 * ez),
+                                at(0, -ex, -ey, -ez), at(0, ex, ey, ez),
+                                at(0, 2 * ex, 2 * ey, 2 * ez), i1);
+                            const double b = tag_d1(
+                                at(1, -2 * ex, -2 * ey, -2 * ez),
+                                at(1, -ex, -ey, -ez), at(1, ex, ey, ez),
+                                at(1, 2 * ex, 2 * ey, 2 * ez), i1);
+                            g1sq += a * a;
+                            g2sq += b * b;
+                            const double dth =
+                                psq > 0.0 ? (psi1 * b - psi2 * a) / psq : 0.0;
+                            gtsq += dth * dth;
+                        }
+                        if (tp.dx * sqrt(g1sq + g2sq - psq * gtsq) >
+                            tp.rgrad_thr)
+                            tag = true;
                     }
                     if (tag)
                         tags[dense] = 1;
@@ -1593,23 +1673,24 @@ int axs_boxlevel_tag(axs_boxlevel *b, const axs_tagger_params *p,
                      int64_t counts[2])
 {
     AXS_CHECK_ARG(b && counts, "axs_boxlevel_tag: NULL argument");
-    AXS_CHECK_ARG(!p || (p->gradient_threshold == DBL_MAX &&
-                         p->radial_gradient_threshold == DBL_MAX),
-                  "axs_boxlevel_tag: the Laplacian / radial-gradient "
-                  "criteria are served on uniform levels only");
+    AXS_CHECK_ARG(b->ng >= 2, "axs_boxlevel_tag: needs >= 2 ghost cells");
     hipStream_t s = b->ctx->stream;
     hipLaunchKernelGGL(k_zero_err, dim3(1), dim3(64), 0, s, b->ctx->d_counts);
     const dim3 g = box_grid(b, max_valid(b));
-    const double amp = p ? p->amplitude_threshold : 0.0;
-    const double R = p ? p->R : 1.0;
+    BoxTagParams tp{};
+    tp.amp_thr = p ? p->amplitude_threshold : 0.0;
+    tp.R = p ? p->R : 1.0;
+    tp.grad_thr = p ? p->gradient_threshold : DBL_MAX;
+    tp.rgrad_thr = p ? p->radial_gradient_threshold : DBL_MAX;
+    tp.dx = b->dx;
     if (b->precision == AXS_FP64)
         hipLaunchKernelGGL(k_box_tag<double>, g, dim3(256), 0, s, view_of(b),
                            (const double *)b->y, (const float *)nullptr,
-                           dense_tags, cover, amp, R, b->ctx->d_counts);
+                           dense_tags, cover, tp, b->ctx->d_counts);
     else
         hipLaunchKernelGGL(k_box_tag<float>, g, dim3(256), 0, s, view_of(b),
                            (const float *)b->y, (const float *)b->comp,
-                           dense_tags, cover, amp, R, b->ctx->d_counts);
+                           dense_tags, cover, tp, b->ctx->d_counts);
     AXS_HIP_TRY(hipGetLastError());
     AXS_HIP_TRY(hipMemcpyAsync(b->ctx->h_counts, b->ctx->d_counts,
                                2 * sizeof(unsigned long long),

@@ -330,9 +330,11 @@ int axs_boxlevel_rk4_step_graph(axs_boxlevel *bl,
                                 const axs_boxlevel *coarse, double t0_frac,
                                 double dt_frac);
 /* StringTagger over every valid box cell into a dense device uint8 array
- * of the level's domain (sets only); plaquette counts of the same pass.
- * Ghosts must be filled.  cover: optional dense device uint8 coverage mask
- * (0 = covered by a finer level, excluded from the counts). */
+ * of the level's domain (sets only), all criteria of axs_tag_cells
+ * (windings, Laplacian, radial gradient, amplitude); plaquette counts of
+ * the same pass.  Ghosts must be filled.  cover: optional dense device
+ * uint8 coverage mask (0 = covered by a finer level, excluded from the
+ * counts and the tags). */
 int axs_boxlevel_tag(axs_boxlevel *bl, const axs_tagger_params *p,
                      uint8_t *dense_tags, const uint8_t *cover,
                      int64_t counts[2]);

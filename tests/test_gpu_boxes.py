@@ -207,3 +207,25 @@ def test_recycled_arrays_start_clean(ctx, oracle):
         b.rk4_step(coeffs, 0.1)
         assert np.array_equal(b.download_dense(), want)
         b.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_box_tag_derivative_criteria(ctx, oracle, precision):
+    """StringTagger's Laplacian and radial-gradient criteria on a box level
+    (StringTagger.hpp:83-218) equal the oracle's tags on the same state."""
+    capi = _capi()
+    n = 32
+    lvl = capi.Level(ctx, n, 0.5, precision)
+    lvl.upload(oracle.seeded_ic(n, seed=11))
+    lvl.advance(capi.make_background(2.0), 1.0, 0.1, 10)
+    s = lvl.download()
+    bl = capi.BoxLevel(ctx, tiling(n, [16, 8, 8]), (n, n, n), 0.5, precision)
+    bl.copy_from_level(lvl)
+    bl.fill_ghosts()
+    for g, r in ((0.05, capi.DBL_MAX), (capi.DBL_MAX, 0.3), (0.1, 0.4)):
+        tags, counts = bl.tag(grad_thr=g, rgrad_thr=r)
+        t = tags.to_numpy(np.uint8).reshape(n, n, n)
+        want = oracle.tag_cells(s, 0.5, g, r)
+        assert np.array_equal(t, want), (g, r)
+        assert 0 < t.sum() < t.size
+        assert counts == oracle.count_plaquettes(s)
