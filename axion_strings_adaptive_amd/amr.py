@@ -1,9 +1,11 @@
-"""Static AMR hierarchy on box levels: the batching form of the hot path.
+"""AMR hierarchy on box levels: the batching form of the hot path.
 
 Level 0 is a periodic base grid (tiled into boxes); each finer level
 (ratio 2) is built from the string tagger on the level below -- tag where
-the plaquette winding is nonzero (StringTagger.hpp:240-274) or, as an
-extension, where |psi|/R < amplitude_threshold -- grown by n_error_buf cells
+the plaquette winding is nonzero, where dx^2 |laplacian psi_i| exceeds
+gradient_threshold or dx |grad |psi|| exceeds radial_gradient_threshold
+(StringTagger.hpp:83-274; both off by default, as in the reference) or, as
+an extension, where |psi|/R < amplitude_threshold -- grown by n_error_buf cells
 (the reference's amr.n_error_buf), cut into blocking-factor blocks and
 merged along x, then y, then z into boxes of at most max_grid_size cells
 (amr.blocking_factor / amr.max_grid_size), and kept properly nested one
@@ -159,7 +161,8 @@ class Hierarchy:
                  seed: int = 42, bg=None, tau0: float = 1.0,
                  dt_multiplier: float = 0.2, verbose: bool = True,
                  force_full_refinement: bool = False, use_graph: bool = True,
-                 regrid_int: int = 0):
+                 regrid_int: int = 0, gradient_threshold: float = capi.DBL_MAX,
+                 radial_gradient_threshold: float = capi.DBL_MAX):
         self.ctx = ctx
         self.use_graph = use_graph
         self.max_levels = levels
@@ -179,6 +182,9 @@ class Hierarchy:
         self.dx = [dx0 / 2 ** l for l in range(levels + 1)]
         self.dt0 = dt_multiplier * dx0
         self.amp_thr = amp_thr
+        # axion_strings.tagging.{gradient,radial_gradient}_threshold
+        self.grad_thr = gradient_threshold
+        self.rgrad_thr = radial_gradient_threshold
         self.verbose = verbose
         self.levels: list[capi.BoxLevel] = []
         self.domains = []
@@ -223,7 +229,9 @@ class Hierarchy:
         coarse = self.levels[l]
         cd = self.domains[l]
         R = capi.background_eval(self.bg, 0, self.tau)
-        tags_buf, _ = coarse.tag(amp_thr=self.amp_thr, R=R)
+        tags_buf, _ = coarse.tag(amp_thr=self.amp_thr, R=R,
+                                 grad_thr=self.grad_thr,
+                                 rgrad_thr=self.rgrad_thr)
         tags = tags_buf.to_numpy(np.uint8).reshape(cd[2], cd[1], cd[0]) > 0
         if self.force_full_refinement:  # axion_strings.tagging.force_full_refinement
             tags[:] = True
@@ -381,12 +389,19 @@ def main(argv=None):
     ap.add_argument("--levels", type=int, default=2)
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--precision", type=int, default=capi.AXS_FP32C)
+    ap.add_argument("--gradient-threshold", type=float, default=capi.DBL_MAX,
+                    help="tag where dx^2 |laplacian psi_i| exceeds this")
+    ap.add_argument("--radial-gradient-threshold", type=float,
+                    default=capi.DBL_MAX,
+                    help="tag where dx |grad |psi|| exceeds this")
     ap.add_argument("--regrid-int", type=int, default=0,
                     help="rebuild the finer levels every N coarse steps")
     args = ap.parse_args(argv)
     ctx = capi.Context(0)
     h = Hierarchy(ctx, args.n0, args.levels, precision=args.precision,
-                  regrid_int=args.regrid_int)
+                  regrid_int=args.regrid_int,
+                  gradient_threshold=args.gradient_threshold,
+                  radial_gradient_threshold=args.radial_gradient_threshold)
     h.step()  # warm-up
     ctx.synchronize()
     h.regrid_s, n_regrid0 = 0.0, h.nregrids
