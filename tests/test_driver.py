@@ -314,3 +314,30 @@ def test_driver_flat_space_loop_scalars(tmp_path, ctx):
     g2v2 = total / count if count else 0.0
     assert rows[1, 5] == pytest.approx(g2v2, rel=1e-9)
     assert rows[1, 6] == pytest.approx(np.sqrt(1.0 + g2v2), rel=1e-9)
+
+
+@pytest.mark.gpu
+def test_driver_relaxed_restart_continues_bit_for_bit(tmp_path):
+    """Checkpoint / restart across the pre-evolution relaxation: the relaxing
+    clock, the adaptive xi-check cadence and (after the handoff) the blend
+    window round-trip through the checkpoint, so the .dat files of a run
+    restarted from an early checkpoint equal an uninterrupted run's."""
+    p = os.path.join(PARAMS, "params_relaxed_64.txt")
+    common = [p, "amr.n_cell=32 32 32", "geometry.prob_extent=16 16 16",
+              "evolution.max_steps=120",
+              "axion_strings.output_cadence_steps=20"]
+    full, seg = tmp_path / "full", tmp_path / "seg"
+    full.mkdir()
+    seg.mkdir()
+    r = run([*common, "--out", str(full)])
+    assert r.returncode == 0, r.stderr
+    r = run([*common, "amr.check_int=10", "evolution.max_steps=40",
+             "--out", str(seg)])
+    assert r.returncode == 0, r.stderr
+    r = run([*common, "amr.restart=" + str(seg / "chk00010.axs"),
+             "--out", str(seg)])
+    assert r.returncode == 0, r.stderr
+    names = sorted(f.name for f in full.iterdir() if f.suffix == ".dat")
+    assert "pre_evolution_scalars.dat" in names
+    for name in names:
+        assert (full / name).read_text() == (seg / name).read_text(), name
