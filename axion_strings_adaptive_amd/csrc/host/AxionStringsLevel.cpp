@@ -3,7 +3,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <chrono>
 #include <cstring>
+#include <functional>
 #include <ctime>
 #include <fstream>
 #include <iostream>
@@ -329,11 +331,22 @@ void AxionStringsLevel::evolve()
         }
         m_step += k;
         if (output_due())
-            diagnostics();
+            timed_diagnostics([this] { diagnostics(); });
         if (m_check_int > 0 && m_step % m_check_int == 0)
             write_checkpoint();
     }
     axs_check(axs_ctx_synchronize(m_ctx), "synchronize");
+}
+
+void AxionStringsLevel::timed_diagnostics(const std::function<void()> &f)
+{
+    axs_check(axs_ctx_synchronize(m_ctx), "synchronize");
+    const auto t0 = std::chrono::steady_clock::now();
+    f();
+    axs_check(axs_ctx_synchronize(m_ctx), "synchronize");
+    m_diag_seconds += std::chrono::duration<double>(
+                          std::chrono::steady_clock::now() - t0)
+                          .count();
 }
 
 long AxionStringsLevel::steps_to_checkpoint() const
@@ -382,7 +395,7 @@ void AxionStringsLevel::evolve_relaxing()
         if (m_steps_since_check >= m_xi_check_interval)
         {
             m_steps_since_check = 0;
-            relax_check();
+            timed_diagnostics([this] { relax_check(); });
         }
         if (checkpoint_due)
             write_checkpoint();

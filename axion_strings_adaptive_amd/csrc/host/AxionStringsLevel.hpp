@@ -27,6 +27,7 @@
 #include "AxionStringsParams.hpp"
 
 #include <cstdio>
+#include <functional>
 #include <string>
 
 namespace axs_host
@@ -46,6 +47,9 @@ class AxionStringsLevel
 
     double tau() const { return m_tau; }
     long step() const { return m_step; }
+    // wall seconds spent in the output diagnostics and the relaxing phase's
+    // xi checks (each measured from a synchronised device)
+    double diagnostics_seconds() const { return m_diag_seconds; }
     axs_level *level() { return m_level; }
 
     enum class Phase
@@ -69,6 +73,7 @@ class AxionStringsLevel
     long steps_until_next_output();
     void flat_diagnostics(const PlaquetteCounts &counts, double ell,
                           double m_r);
+    void timed_diagnostics(const std::function<void()> &f);
     void write_network_scalars_row(const PlaquetteCounts &c, double xi,
                                    double xi_w, double m_r_over_h,
                                    const TotalEnergyResult &e,
@@ -101,6 +106,7 @@ class AxionStringsLevel
     FILE *m_spectrum{nullptr};
     FILE *m_pre_scalars{nullptr};
     FILE *m_loop_scalars{nullptr};
+    double m_diag_seconds{0.0}; // wall time in diagnostics and xi checks
     // pre-evolution (ic_mode = fourier_relaxed)
     Phase m_phase{Phase::Evolving};
     PreEvolutionBackground m_pre;

@@ -7,6 +7,7 @@
 // just_check_params), without touching a GPU.
 #include "AxionStringsLevel.hpp"
 
+#include <chrono>
 #include <cstring>
 #include <iostream>
 
@@ -40,11 +41,23 @@ int main(int argc, char **argv)
             std::cout << "parameters ok\n";
             return 0;
         }
+        using clock = std::chrono::steady_clock;
+        const auto t0 = clock::now();
         AxionStringsLevel level(table, out_dir);
+        const auto t1 = clock::now();
         level.initData();
+        const auto t2 = clock::now();
         level.evolve();
+        const auto t3 = clock::now();
+        auto sec = [](clock::duration d) {
+            return std::chrono::duration<double>(d).count();
+        };
         std::cout << "done: tau = " << level.tau() << "  steps = "
                   << level.step() << "\n";
+        std::cout << "wall (s): set-up " << sec(t1 - t0) << "  initial data "
+                  << sec(t2 - t1) << "  evolution " << sec(t3 - t2)
+                  << " (of which diagnostics " << level.diagnostics_seconds()
+                  << ")\n";
     }
     catch (const ParamError &e)
     {
