@@ -185,6 +185,43 @@ def diag_timing(capi, ctx, lvl, bg, tau, n, reps=3):
     return out
 
 
+def pcie_check(capi, ctx, n, bg, dt, steps, tile):
+    """The host-buffer boundary: wall time of axs_level_upload and
+    axs_level_download (FP64 host state, 32 B per cell, across PCIe) of an
+    n^3 FP32C level, and the step rate a caller would see if the state went
+    down and up around every `steps` RK4 steps."""
+    import numpy as np
+    lvl = capi.Level(ctx, n, 0.5, capi.AXS_FP32C)
+    if tile is not None:
+        lvl.set_tile(*tile)
+    host = np.random.default_rng(1).normal(size=(4, n, n, n))
+    lvl.upload(host)  # first touch of the pages
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    lvl.upload(host)
+    ctx.synchronize()
+    t_up = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    out = lvl.download()
+    t_down = time.perf_counter() - t0
+    lvl.advance(bg, 1.0, dt, 1)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    lvl.advance(bg, 1.1, dt, steps)
+    ctx.synchronize()
+    t_steps = time.perf_counter() - t0
+    lvl.close()
+    nbytes = host.nbytes
+    del host, out
+    ncell = n ** 3
+    return {"n": n, "state_bytes": nbytes, "upload_s": t_up,
+            "download_s": t_down, "upload_gbs": nbytes / t_up / 1e9,
+            "download_gbs": nbytes / t_down / 1e9, "steps": steps,
+            "device_cell_updates_per_s": ncell * steps / t_steps,
+            "with_round_trip_cell_updates_per_s":
+                ncell * steps / (t_steps + t_up + t_down)}
+
+
 def precision_run(capi, ctx, n, bg, dt, steps, every, tile32=None, tile64=None):
     """FP32C and FP64 levels side by side from the same IC for `steps` RK4
     steps, with the precision metrics every `every` steps: max |dphi|/eta,
@@ -256,6 +293,10 @@ def main(argv=None):
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--pcie-check", action="store_true",
+                    help="only time the host-buffer upload / download of a "
+                         "--pcie-n^3 FP32C level around --steps RK4 steps")
+    ap.add_argument("--pcie-n", type=int, default=512)
     ap.add_argument("--no-diag", action="store_true",
                     help="skip the diagnostics timing after the FP32C run")
     ap.add_argument("--place-trials", type=int, default=8,
@@ -324,6 +365,11 @@ def main(argv=None):
         res = alloc_check(capi, ctx, n, capi.AXS_FP32C, bg, dt, args.steps,
                           tile)
         print(json.dumps({"alloc_check": res}), flush=True)
+        ctx.close()
+        return 0
+    if args.pcie_check:
+        res = pcie_check(capi, ctx, args.pcie_n, bg, dt, args.steps, tile)
+        print(json.dumps({"pcie_check": res}), flush=True)
         ctx.close()
         return 0
     if args.timing_check:

@@ -199,7 +199,10 @@ int axs_rk4_advance_coeffs(axs_level *lvl, const axs_rhs_coeffs *coeffs,
  * While free memory holds a second copy of the arrays the fastest placement
  * seen is kept; otherwise each placement is freed before the next is drawn
  * and, from the third draw on, the first within 2% of the fastest seen is
- * kept (else the last).  The state is reset to zero: call it before
+ * kept (else the last).  Either way the trials stop as soon as the kept
+ * placement streams the RK4 step's algorithmic bytes (304 B per cell
+ * FP32C, 544 FP64, 272 FP32) at >= 69% of the 8 TB/s HBM peak, the fast
+ * regime.  The state is reset to zero: call it before
  * uploading data.  ms_per_step (optional, `trials` entries) receives each
  * trial's time and *used (optional) the number of placements tried. */
 int axs_level_place(axs_level *lvl, int trials, double *ms_per_step, int *used);
