@@ -250,6 +250,69 @@ int or_plaquette_winding(double t1, double t2, double t3, double t4)
     return (int)lround(total / two_pi);
 }
 
+/* ===================== BoxPlan.hpp:16-196 =========================== */
+/* compute_general_box_plan, BoxPlan.hpp:36-50 */
+void or_general_box_plan(int N, double N1, double N2, double a_inv, double c,
+                         double out[4])
+{
+    /* tau_f = (N/(N1 N2))^((a_inv-1)/(a_inv-c)) tau0, L = (N1/R0) * same,
+     * R0 = 1/(a_inv-1) (lambda0 = 1, tau0 = 1) */
+    const double power = pow((double)N / (N1 * N2), (a_inv - 1.0) / (a_inv - c));
+    const double R0 = 1.0 / (a_inv - 1.0);
+    out[0] = power;
+    out[1] = (N1 / R0) * power;
+    out[2] = out[1] / N;
+    out[3] = out[2] / 3.0;
+}
+
+/* compute_amr_box_plan, BoxPlan.hpp:96-125 */
+void or_amr_box_plan(int N_effective, double N1, double N2, double a_inv,
+                     double c, int max_level, double out[4], int *N_base,
+                     double *log_add)
+{
+    double g[4];
+    or_general_box_plan(N_effective, N1, N2, a_inv, c, g);
+    const int ratio = 1 << max_level;
+    out[0] = g[1];
+    out[1] = g[2];
+    out[2] = g[2] * (double)ratio;
+    out[3] = g[0];
+    *N_base = (N_effective % ratio == 0) ? N_effective / ratio : -1;
+    /* level ell comes online max_level - ell + 1 level spacings of
+     * ln 2 (a_inv - c)/(1 - c) in log(m_r/H) before tau_f */
+    const double b_inv = a_inv - 1.0;
+    const double at_f = (a_inv - c) / b_inv * log(g[0]);
+    const double step = log(2.0) * (a_inv - c) / (1.0 - c);
+    for (int ell = 1; ell <= max_level; ++ell)
+        log_add[ell - 1] = at_f - (double)(max_level - ell + 1) * step;
+}
+
+/* pre_evolution_L_tilde, BoxPlan.hpp:130-137 */
+double or_pre_evolution_L_tilde(double L_tilde_main, double a_inv, double c,
+                                double tau_i)
+{
+    return L_tilde_main * (1.0 / (a_inv - 1.0)) *
+           pow(tau_i / 1.0, (1.0 - c) / (a_inv - 1.0));
+}
+
+/* moore_max_log_dynamic_range, BoxPlan.hpp:144-149 */
+double or_moore_max_log_dynamic_range(int N, double N1, double N2,
+                                      double gamma)
+{
+    return 2.0 * log((double)N / (N2 * N1 * gamma));
+}
+
+/* compute_moore_box_plan, BoxPlan.hpp:183-194 */
+void or_moore_box_plan(int N, double N1, double N2, double gamma, double a_inv,
+                       double b_inv, double R_switch, double m_r_switch,
+                       double tau_switch, double out[4])
+{
+    out[0] = 1.0 / (N2 * R_switch * m_r_switch);
+    out[1] = N * out[0];
+    out[2] = or_moore_max_log_dynamic_range(N, N1, N2, gamma);
+    out[3] = tau_switch * exp(out[2] * b_inv / a_inv);
+}
+
 /* ===================== XiFormula.hpp:32-57 ========================== */
 double or_xi_from_comoving_length(double ell, double L, double a_inv,
                                   double tau)

@@ -167,6 +167,24 @@ int or_spectrum(const double *field, int nx, int ny, int nz,
 
 /* ---------------- Fourier IC (FourierIC.hpp:28-130) ----------------- */
 uint64_t or_hash_u64(uint64_t x);
+/* BoxPlan.hpp:16-196: box planning from the resolution targets N1, N2.
+ * out = {tau_f, L_tilde, delta_x, delta_tau_leapfrog_bound}. */
+void or_general_box_plan(int N, double N1, double N2, double a_inv, double c,
+                         double out[4]);
+/* out = {L_tilde, dx_finest, dx_base, tau_f}; *N_base (-1 if N_effective is
+ * not a multiple of 2^max_level); log_add[max_level]. */
+void or_amr_box_plan(int N_effective, double N1, double N2, double a_inv,
+                     double c, int max_level, double out[4], int *N_base,
+                     double *log_add);
+double or_pre_evolution_L_tilde(double L_tilde_main, double a_inv, double c,
+                                double tau_i);
+double or_moore_max_log_dynamic_range(int N, double N1, double N2,
+                                      double gamma);
+/* out = {dx, L_tilde, D, tau_end}. */
+void or_moore_box_plan(int N, double N1, double N2, double gamma, double a_inv,
+                       double b_inv, double R_switch, double m_r_switch,
+                       double tau_switch, double out[4]);
+
 double or_uniform_from_hash(uint64_t h);
 void or_gaussian_mode(uint64_t seed, int comp, int i, int j, int k,
                       double *re, double *im);

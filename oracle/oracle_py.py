@@ -113,6 +113,7 @@ def load():
     lib.or_handoff_dtau.restype = _D
     lib.or_handoff_dtau.argtypes = [_D, _D, _D]
     _lib = lib
+    _bind_box_plan(lib, "or_")
     return lib
 
 
@@ -316,7 +317,62 @@ def ref_lib():
     lib.ref_xi_from_plaquette_count.argtypes = [_D] * 5
     lib.ref_plaquette_count_for_target_xi.restype = _D
     lib.ref_plaquette_count_for_target_xi.argtypes = [_D, _I, _D, _D, _D]
+    _bind_box_plan(lib, "ref_")
     return lib
+
+
+def _bind_box_plan(lib, pre):
+    _PD = C.POINTER(C.c_double)
+    getattr(lib, pre + "general_box_plan").restype = None
+    getattr(lib, pre + "general_box_plan").argtypes = [_I, _D, _D, _D, _D, _PD]
+    getattr(lib, pre + "amr_box_plan").restype = None
+    getattr(lib, pre + "amr_box_plan").argtypes = [
+        _I, _D, _D, _D, _D, _I, _PD, C.POINTER(C.c_int), _PD]
+    getattr(lib, pre + "pre_evolution_L_tilde").restype = _D
+    getattr(lib, pre + "pre_evolution_L_tilde").argtypes = [_D] * 4
+    getattr(lib, pre + "moore_max_log_dynamic_range").restype = _D
+    getattr(lib, pre + "moore_max_log_dynamic_range").argtypes = [_I, _D, _D, _D]
+    getattr(lib, pre + "moore_box_plan").restype = None
+    getattr(lib, pre + "moore_box_plan").argtypes = [_I] + [_D] * 8 + [_PD]
+
+
+def general_box_plan(N, N1, N2, a_inv, c, lib=None):
+    """(tau_f, L_tilde, delta_x, delta_tau_leapfrog_bound), BoxPlan.hpp."""
+    lib, pre = (load(), "or_") if lib is None else (lib, "ref_")
+    out = (C.c_double * 4)()
+    getattr(lib, pre + "general_box_plan")(N, N1, N2, a_inv, c, out)
+    return tuple(out)
+
+
+def amr_box_plan(N_eff, N1, N2, a_inv, c, max_level, lib=None):
+    """(L_tilde, dx_finest, dx_base, tau_f, N_base, [log_add per level])."""
+    lib, pre = (load(), "or_") if lib is None else (lib, "ref_")
+    out = (C.c_double * 4)()
+    nb = C.c_int()
+    la = (C.c_double * max(1, max_level))()
+    getattr(lib, pre + "amr_box_plan")(N_eff, N1, N2, a_inv, c, max_level,
+                                       out, C.byref(nb), la)
+    return tuple(out) + (nb.value, list(la)[:max_level])
+
+
+def pre_evolution_L_tilde(L, a_inv, c, tau_i, lib=None):
+    lib, pre = (load(), "or_") if lib is None else (lib, "ref_")
+    return getattr(lib, pre + "pre_evolution_L_tilde")(L, a_inv, c, tau_i)
+
+
+def moore_max_log_dynamic_range(N, N1, N2, gamma, lib=None):
+    lib, pre = (load(), "or_") if lib is None else (lib, "ref_")
+    return getattr(lib, pre + "moore_max_log_dynamic_range")(N, N1, N2, gamma)
+
+
+def moore_box_plan(N, N1, N2, gamma, a_inv, b_inv, R_sw, mr_sw, tau_sw,
+                   lib=None):
+    """(dx, L_tilde, D, tau_end), BoxPlan.hpp."""
+    lib, pre = (load(), "or_") if lib is None else (lib, "ref_")
+    out = (C.c_double * 4)()
+    getattr(lib, pre + "moore_box_plan")(N, N1, N2, gamma, a_inv, b_inv, R_sw,
+                                         mr_sw, tau_sw, out)
+    return tuple(out)
 
 
 def seeded_ic(n, seed=42, dx=0.5, target_ms=0.5, k_max_cells=None):

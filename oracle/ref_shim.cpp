@@ -5,11 +5,13 @@
 // oracle/Makefile), behind a C ABI, so tests/ can check the CPU oracle
 // (axs_oracle.c) and the product's host Background against the reference
 // code itself.  Only headers that compile with nothing but <cmath> are used
-// (Background.hpp, PlaquetteWinding.hpp, Masking.hpp, Energy.hpp,
-// XiFormula.hpp, Velocity.hpp, PreEvolutionBackground.hpp); everything else in the reference needs the absent
+// (Background.hpp, BoxPlan.hpp, PlaquetteWinding.hpp, Masking.hpp,
+// Energy.hpp, XiFormula.hpp, Velocity.hpp, PreEvolutionBackground.hpp);
+// everything else in the reference needs the absent
 // GRTeclyn/AMReX submodules and is not built.  The output library goes to
 // oracle/_ref/ and is never shipped in git.
 #include "Background.hpp"
+#include "BoxPlan.hpp"
 #include "Energy.hpp"
 #include "Masking.hpp"
 #include "PlaquetteWinding.hpp"
@@ -33,6 +35,57 @@ Background make_bg(double a_inv, double c0, double c1, double tau_switch,
 
 extern "C"
 {
+    void ref_general_box_plan(int N, double N1, double N2, double a_inv,
+                              double c, double out[4])
+    {
+        const GeneralBoxPlan p = compute_general_box_plan(N, N1, N2, a_inv, c);
+        out[0] = p.tau_f;
+        out[1] = p.L_tilde;
+        out[2] = p.delta_x;
+        out[3] = p.delta_tau_leapfrog_bound;
+    }
+
+    void ref_amr_box_plan(int N_effective, double N1, double N2, double a_inv,
+                          double c, int max_level, double out[4], int *N_base,
+                          double *log_add)
+    {
+        const AmrBoxPlan p =
+            compute_amr_box_plan(N_effective, N1, N2, a_inv, c, max_level);
+        out[0] = p.L_tilde;
+        out[1] = p.dx_finest;
+        out[2] = p.dx_base;
+        out[3] = p.tau_f;
+        *N_base = p.N_base;
+        for (int i = 0; i < max_level; ++i)
+            log_add[i] = p.log_add[(size_t)i];
+    }
+
+    double ref_pre_evolution_L_tilde(double L, double a_inv, double c,
+                                     double tau_i)
+    {
+        return pre_evolution_L_tilde(L, a_inv, c, tau_i);
+    }
+
+    double ref_moore_max_log_dynamic_range(int N, double N1, double N2,
+                                           double gamma)
+    {
+        return moore_max_log_dynamic_range(N, N1, N2, gamma);
+    }
+
+    void ref_moore_box_plan(int N, double N1, double N2, double gamma,
+                            double a_inv, double b_inv, double R_switch,
+                            double m_r_switch, double tau_switch,
+                            double out[4])
+    {
+        const MooreBoxPlan p =
+            compute_moore_box_plan(N, N1, N2, gamma, a_inv, b_inv, R_switch,
+                                   m_r_switch, tau_switch);
+        out[0] = p.dx;
+        out[1] = p.L_tilde;
+        out[2] = p.D;
+        out[3] = p.tau_end;
+    }
+
     // which: 0 R, 1 R', 2 curvature, 3 lambda, 4 m_r/H direct
     double ref_pre_background(double gamma_pre, int which, double tau_pre)
     {

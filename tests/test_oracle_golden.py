@@ -285,6 +285,113 @@ def test_background_matches_reference_header(oracle, ref):
                                                        tau)
 
 
+# ------------------------------------------------------------ box planning
+# tests/test_box_plan.cpp:8-228, checked against the background formulas (not
+# against the plan's own algebra)
+def _phys(L, b, tau):
+    R = L.or_bg_R(b, tau)
+    H = L.or_bg_R_prime(b, tau) / (R * R)
+    return R, H, math.sqrt(L.or_bg_lambda(b, tau))
+
+
+@pytest.mark.parametrize("N,N1,N2,c", [(512, 1.0, 4.0, 0.0), (800, 1.5, 6.0, 1.0),
+                                       (1024, 1.0, 4.0, 1.5), (300, 0.7, 3.0, 0.5)])
+def test_general_box_plan_delivers_N1_N2_at_tau_f(oracle, N, N1, N2, c):
+    L = oracle.load()
+    tau_f, Lt, dx, dt_bound = oracle.general_box_plan(N, N1, N2, 2.0, c)
+    R, H, mr = _phys(L, oracle.bg(2.0, c0=c), tau_f)
+    assert H * Lt * R == pytest.approx(N1, rel=1e-9)
+    assert 1.0 / (dx * R * mr) == pytest.approx(N2, rel=1e-9)
+    assert dx * N == pytest.approx(Lt, rel=1e-15)
+    assert dt_bound == pytest.approx(dx / 3.0, rel=1e-15)
+
+
+def test_pre_evolution_L_tilde_kats(oracle):
+    assert oracle.pre_evolution_L_tilde(10.0, 2.0, 1.0, 4.0) == \
+        pytest.approx(10.0, rel=1e-12)
+    assert oracle.pre_evolution_L_tilde(10.0, 2.0, 0.0, 4.0) == \
+        pytest.approx(40.0, rel=1e-12)
+    # a_inv = 3: L (1/2) tau_i^((1-c)/2)
+    assert oracle.pre_evolution_L_tilde(8.0, 3.0, 0.0, 9.0) == \
+        pytest.approx(12.0, rel=1e-12)
+
+
+def test_moore_dynamic_range_inverse(oracle):
+    for (N1, N2, g, D) in [(1.0, 5.0, 3.0, 6.0), (1.2, 8.0, 2.0, 4.0)]:
+        N = int(round(N2 * N1 * g * math.exp(D / 2.0)))
+        assert oracle.moore_max_log_dynamic_range(N, N1, N2, g) == \
+            pytest.approx(D, rel=1e-3)
+
+
+def test_amr_box_plan_base_grid_and_divisibility(oracle):
+    Lt, dxf, dxb, tau_f, nb, la = oracle.amr_box_plan(4096, 1.2, 8.0, 2.0, 0.0, 4)
+    assert nb == 4096 // 16
+    assert dxb == pytest.approx(dxf * 16, rel=1e-15)
+    assert len(la) == 4
+    assert oracle.amr_box_plan(4097, 1.2, 8.0, 2.0, 0.0, 4)[4] == -1
+    g = oracle.general_box_plan(4096, 1.2, 8.0, 2.0, 0.0)
+    assert (tau_f, Lt, dxf) == (g[0], g[1], g[2])
+
+
+def test_amr_box_plan_level_spacing(oracle):
+    L = oracle.load()
+    Lt, dxf, dxb, tau_f, nb, la = oracle.amr_box_plan(4096, 1.0, 4.0, 2.0, 0.0, 5)
+    d = math.log(4.0)
+    for i in range(1, len(la)):
+        assert la[i] - la[i - 1] == pytest.approx(d, rel=1e-9)
+    at_f = -math.log(L.or_bg_H_over_mr_closed_form(oracle.bg(2.0), tau_f))
+    assert la[-1] == pytest.approx(at_f - d, rel=1e-9)
+
+
+@pytest.mark.parametrize("N,lev,N2,a_inv", [(8192, 3, 6.0, 2.5), (2048, 2, 4.0, 2.0),
+                                             (4096, 4, 5.0, 3.0)])
+def test_amr_box_plan_level_thresholds_hit_N2(oracle, N, lev, N2, a_inv):
+    """At each level's switch-on time, the level below it alone gives exactly
+    N2 (level 1: the base grid)."""
+    L = oracle.load()
+    Lt, dxf, dxb, tau_f, nb, la = oracle.amr_box_plan(N, 1.0, N2, a_inv, 0.0, lev)
+    b = oracle.bg(a_inv)
+    for ell in range(1, lev + 1):
+        tau = L.or_bg_tau_from_log_mr_over_h(b, la[ell - 1])
+        R, _, mr = _phys(L, b, tau)
+        dx_below = dxb / 2 ** (ell - 1)
+        assert 1.0 / (R * dx_below * mr) == pytest.approx(N2, rel=1e-6)
+
+
+def test_moore_box_plan_N2_at_switch_and_dynamic_range(oracle):
+    L = oracle.load()
+    b = oracle.bg(2.0, c0=1.0)
+    tau_sw = 30.0
+    R_sw, _, mr_sw = _phys(L, b, tau_sw)
+    gamma = 1.0 / L.or_bg_H_over_mr_direct(b, tau_sw)
+    dx, Lt, D, tau_end = oracle.moore_box_plan(4000, 1.2, 8.0, gamma, 2.0, 1.0,
+                                              R_sw, mr_sw, tau_sw)
+    assert 1.0 / (8.0 * dx * R_sw * mr_sw) == pytest.approx(1.0, rel=1e-12)
+    R12, _, mr12 = _phys(L, b, 12.0)
+    assert R12 * mr12 == pytest.approx(R_sw * mr_sw, rel=1e-9)
+    assert math.log(L.or_bg_H(b, tau_sw) / L.or_bg_H(b, tau_end)) == \
+        pytest.approx(D, rel=1e-9)
+    assert Lt == pytest.approx(4000 * dx, rel=1e-15)
+
+
+def test_box_plan_matches_reference_header(oracle, ref):
+    for args in [(512, 1.0, 4.0, 2.0, 0.0), (800, 1.5, 6.0, 2.0, 1.0),
+                 (1024, 1.0, 4.0, 3.0, 2.0), (4097, 0.9, 5.0, 2.5, 0.3)]:
+        assert oracle.general_box_plan(*args) == \
+            oracle.general_box_plan(*args, lib=ref)
+        for lev in (0, 1, 3, 5):
+            assert oracle.amr_box_plan(*args, lev) == \
+                oracle.amr_box_plan(*args, lev, lib=ref)
+    for a in [(10.0, 2.0, 1.0, 4.0), (10.0, 2.0, 0.0, 4.0), (7.5, 3.0, 0.5, 2.5)]:
+        assert oracle.pre_evolution_L_tilde(*a) == \
+            oracle.pre_evolution_L_tilde(*a, lib=ref)
+    for a in [(1024, 1.0, 4.0, 3.0), (4000, 1.2, 8.0, 17.3)]:
+        assert oracle.moore_max_log_dynamic_range(*a) == \
+            oracle.moore_max_log_dynamic_range(*a, lib=ref)
+    a = (4000, 1.2, 8.0, 17.3, 2.0, 1.0, 0.5, 2.0, 30.0)
+    assert oracle.moore_box_plan(*a) == oracle.moore_box_plan(*a, lib=ref)
+
+
 # ------------------------------- pre-evolution background and the handoff
 PRE_FNS = ["R", "R_prime", "curvature_term_coeff", "lambda",
            "m_r_over_H_direct"]
