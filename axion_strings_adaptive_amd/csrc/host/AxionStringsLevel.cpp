@@ -31,9 +31,9 @@ AxionStringsLevel::AxionStringsLevel(const ParamTable &params,
                                      const std::string &out_dir)
     : m_params(params), m_out_dir(out_dir)
 {
-    Params::check_params(params);
-    ParmParse amr(params, "amr"), geom(params, "geometry"),
-        evo(params, "evolution"), as(params, "axion_strings");
+    Params::check_params(m_params);
+    ParmParse amr(m_params, "amr"), geom(m_params, "geometry"),
+        evo(m_params, "evolution"), as(m_params, "axion_strings");
     std::vector<int> n_cell;
     amr.getarr("n_cell", n_cell);
     if (n_cell.size() != 3)
@@ -65,13 +65,16 @@ AxionStringsLevel::AxionStringsLevel(const ParamTable &params,
     m_precision = precision_from(prec, as);
     as.queryAdd("use_graph", m_use_graph);
     as.queryAdd("flat_space", m_flat);
-    m_masking = Params::read_masking_params(params, "axion_strings.masking");
-    m_tagging = Params::read_tagging_params(params);
+    // the box plan's stop time (AxionStringsLevel.cpp:168-175); flat-space
+    // runs stop on evolution.stop_time / max_steps only
+    m_has_tau_f = !m_flat && as.query("derived_tau_f", m_tau_f);
+    m_masking = Params::read_masking_params(m_params, "axion_strings.masking");
+    m_tagging = Params::read_tagging_params(m_params);
     if (m_flat)
     {
-        ParmParse(params, "axion_strings.flat").queryAdd("m_r", m_flat_mr);
+        ParmParse(m_params, "axion_strings.flat").queryAdd("m_r", m_flat_mr);
         long every = 0;
-        ParmParse(params, "axion_strings.flat")
+        ParmParse(m_params, "axion_strings.flat")
             .queryAdd("output_cadence_steps", every);
         m_cadence.every_steps = every > 0 ? every : 1;
         m_tau_i = 0.0;
@@ -79,9 +82,9 @@ AxionStringsLevel::AxionStringsLevel(const ParamTable &params,
     }
     else
     {
-        m_bg = Params::read_background(params);
-        m_tau_i = Params::read_tau_i(params, m_bg);
-        m_cadence = Params::read_output_cadence(params);
+        m_bg = Params::read_background(m_params);
+        m_tau_i = Params::read_tau_i(m_params, m_bg);
+        m_cadence = Params::read_output_cadence(m_params);
         m_next_log = m_cadence.first_log_mr_over_h;
         m_cbg = axs_background{m_bg.a_inv,           m_bg.c_sched.c0,
                                m_bg.c_sched.c1,      m_bg.c_sched.tau_switch,
@@ -94,11 +97,11 @@ AxionStringsLevel::AxionStringsLevel(const ParamTable &params,
     if (!m_flat && ic_mode == "fourier_relaxed")
     {
         // AxionStringsLevel.cpp:192-203
-        m_pre = Params::read_pre_evolution_background(params);
-        m_xi_target = Params::read_xi_target(params);
-        m_xi_cadence = Params::read_xi_check_cadence(params);
+        m_pre = Params::read_pre_evolution_background(m_params);
+        m_xi_target = Params::read_xi_target(m_params);
+        m_xi_cadence = Params::read_xi_check_cadence(m_params);
         m_xi_check_interval = m_xi_cadence.coarse_interval;
-        m_handoff_n_periods = Params::read_handoff_transition_n_periods(params);
+        m_handoff_n_periods = Params::read_handoff_transition_n_periods(m_params);
         m_phase = Phase::Relaxing;
     }
     amr.queryAdd("check_int", m_check_int);
@@ -299,6 +302,15 @@ void AxionStringsLevel::evolve()
     {
         long k = std::min(steps_until_next_output(), steps_to_checkpoint());
         k = clamp_steps(k, m_step, m_max_steps, m_a_time, m_stop_time, m_dt);
+        if (m_has_tau_f)
+        {
+            // okToContinue (AxionStringsLevel.cpp:2007-2025): another step
+            // while tau < tau_f
+            long left = 0;
+            for (double t = m_tau; t < m_tau_f && left < k; t += m_dt)
+                ++left;
+            k = std::min(k, left);
+        }
         if (k <= 0)
             break;
         const bool blending =
@@ -567,8 +579,15 @@ void AxionStringsLevel::diagnostics()
     (void)vw_cnt;
     write_network_scalars_row(counts, xi, xi_w, m_r_over_h, e, vr, t_core,
                               t_tail);
-    std::cout << "  [AxionStrings] tau = " << tau << "  step = " << m_step
-              << "  N_p = " << counts.n_p_plain
+    // progress against the box plan's tau_f (AxionStringsLevel.cpp:1444-1468)
+    if (m_has_tau_f)
+        std::cout << "  ************ tau/tau_f = " << tau / m_tau_f
+                  << " ************\n";
+    std::cout << "  [AxionStrings] tau = " << tau << "  step = " << m_step;
+    if (m_has_tau_f)
+        std::cout << "  (" << 100.0 * (tau - m_tau_i) / (m_tau_f - m_tau_i)
+                  << "% of conformal time elapsed)";
+    std::cout << "  N_p = " << counts.n_p_plain
               << "  N_p_W = " << counts.n_p_weighted << "  xi = " << xi
               << "  m_r/H = " << m_r_over_h
               << "  rho_tot unscreened = " << e.rho_tot_unscreened

@@ -80,7 +80,7 @@ class AxionStringsLevel
                                    const VelocityResult &v, double tension_core,
                                    double tension_tail);
 
-    const ParamTable &m_params;
+    ParamTable m_params; // the caller's table with the box plan applied
     std::string m_out_dir;
     axs_ctx *m_ctx{nullptr};
     axs_level *m_level{nullptr};
@@ -92,6 +92,8 @@ class AxionStringsLevel
     Background m_bg;
     axs_background m_cbg{};
     double m_tau_i{}, m_tau{}, m_stop_time{-1.0};
+    bool m_has_tau_f{false}; // axion_strings.derived_tau_f of the box plan
+    double m_tau_f{0.0};
     long m_step{0}, m_max_steps{-1};
     bool m_use_graph{true};
     MaskingParams m_masking;

@@ -17,11 +17,14 @@
 #pragma once
 
 #include "Background.hpp"
+#include "BoxPlan.hpp"
 #include "ParmParse.hpp"
 
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
+#include <ostream>
 #include <string>
 #include <vector>
 
@@ -268,11 +271,247 @@ inline MaskingParams read_masking_params(const ParamTable &t,
     return p;
 }
 
-// check_params, AxionStringsParams.hpp:1142-1208 (the parts that apply to
-// a single-level run without box planning), plus the masking / tagging
-// reads that AxionStringsLevel::variableSetUp performs at level set-up
-// (AxionStringsLevel.cpp:186-190), so that every key error surfaces here.
-inline void check_params(const ParamTable &t)
+// A double written so that reading it back gives the same value.
+inline std::string exact_str(double v)
+{
+    char buf[32];
+    std::snprintf(buf, sizeof buf, "%.17g", v);
+    return buf;
+}
+
+// Derive-if-absent, cross-check-if-present for geometry.prob_extent
+// (AxionStringsParams.hpp:808-830 and the two later copies).
+inline void plan_prob_extent(ParamTable &t, double L_tilde, std::ostream *log)
+{
+    ParmParse geom(t, "geometry");
+    if (geom.contains("prob_extent"))
+    {
+        std::vector<double> ext;
+        geom.getarr("prob_extent", ext);
+        for (double L : ext)
+            if (std::abs(L - L_tilde) > 1.0e-6 * L_tilde)
+                geom.error("prob_extent",
+                           "does not match the box-planning-derived L_tilde "
+                           "-- either remove geometry.prob_extent to let it "
+                           "be derived, or fix axion_strings.N/N1/N2 to match");
+        return;
+    }
+    const std::string v = exact_str(L_tilde);
+    t.set("geometry.prob_extent", v + " " + v + " " + v);
+    if (log)
+        *log << "  -> geometry.prob_extent set to " << v << " " << v << " "
+             << v << "\n";
+}
+
+// The same for amr.n_cell (AxionStringsParams.hpp:957-978, :1038-1058).
+inline void plan_n_cell(ParamTable &t, int n, const char *what,
+                        std::ostream *log)
+{
+    ParmParse amr(t, "amr");
+    if (amr.contains("n_cell"))
+    {
+        std::vector<long> nc;
+        amr.getarr("n_cell", nc);
+        for (long v : nc)
+            if (v != n)
+                amr.error("n_cell", std::string("does not match ") + what +
+                                        " -- either remove amr.n_cell to let "
+                                        "it be derived, or fix "
+                                        "axion_strings.N/amr.max_level to "
+                                        "match");
+        return;
+    }
+    const std::string v = std::to_string(n);
+    t.set("amr.n_cell", v + " " + v + " " + v);
+    if (log)
+        *log << "  -> amr.n_cell set to " << v << " " << v << " " << v
+             << "\n";
+}
+
+// The derived stop time, and evolution.stop_time = -1 (no limit) when it is
+// absent, so that tau >= derived_tau_f is what ends the run
+// (AxionStringsParams.hpp:1117-1129).
+inline void plan_stop(ParamTable &t, double tau_f, std::ostream *log)
+{
+    t.set("axion_strings.derived_tau_f", exact_str(tau_f));
+    if (!t.contains("evolution.stop_time"))
+    {
+        t.set("evolution.stop_time", "-1");
+        if (log)
+            *log << "  -> evolution.stop_time set to -1: the run stops once "
+                    "tau >= axion_strings.derived_tau_f = "
+                 << exact_str(tau_f) << "\n";
+    }
+}
+
+// apply_pre_evolution_gamma_default, AxionStringsParams.hpp:638-688: a
+// relaxing run relaxes at 1/dx of the level it starts on unless
+// axion_strings.pre_evolution.gamma says otherwise (a differing value only
+// warns).
+inline void apply_pre_evolution_gamma_default(ParamTable &t,
+                                              const std::string &ic_mode,
+                                              double dx_base, std::ostream *log)
+{
+    if (ic_mode != "fourier_relaxed")
+        return;
+    const double expect = 1.0 / dx_base;
+    ParmParse pre(t, "axion_strings.pre_evolution");
+    if (pre.contains("gamma"))
+    {
+        double g = 0.0;
+        pre.get("gamma", g);
+        if (std::abs(g - expect) > 1.0e-6 * expect && log)
+            *log << "  WARNING: axion_strings.pre_evolution.gamma = " << g
+                 << " differs from 1/dx_base = " << expect
+                 << " of the box plan; relaxing at the value given\n";
+        return;
+    }
+    t.set("axion_strings.pre_evolution.gamma", exact_str(expect));
+    if (log)
+        *log << "  -> axion_strings.pre_evolution.gamma set to "
+             << exact_str(expect) << " (= 1/dx_base)\n";
+}
+
+// apply_box_plan, AxionStringsParams.hpp:690-1130.  Nothing happens unless
+// axion_strings.N is set; then N, N1, N2 fix amr.n_cell, geometry.prob_extent
+// and the stop time axion_strings.derived_tau_f (and for refined runs the
+// level schedule axion_strings.derived_level_add_log_mr_over_h), each
+// derived when absent and checked when present.  Runs that reach Moore
+// (c = a_inv) through a switch get the Moore plan (tau_f = its tau_end); a run
+// with c0 = a_inv from the start only gets the dynamic-range check.  Safe to
+// apply twice: the second pass finds its own values and accepts them.
+inline void apply_box_plan(ParamTable &t, const Background &b, double tau_i,
+                           std::ostream *log)
+{
+    ParmParse pp(t, "axion_strings");
+    if (!pp.contains("N"))
+        return;
+    int N = 0;
+    double N1 = 0.0, N2 = 0.0;
+    pp.get("N", N);
+    pp.get("N1", N1);
+    pp.get("N2", N2);
+    if (N <= 0 || N1 <= 0.0 || N2 <= 0.0)
+        pp.error("N", "N, N1 and N2 must all be > 0");
+    const double a_inv = b.a_inv;
+    const bool moore = b.c_sched.has_switch ||
+                       std::abs(b.c_sched.c0 - a_inv) < 1.0e-9;
+    int max_level = 0;
+    ParmParse(t, "amr").queryAdd("max_level", max_level);
+    if (log)
+        *log << "box plan: a_inv = " << a_inv << ", N = " << N
+             << ", N1 = " << N1 << ", N2 = " << N2
+             << ", amr.max_level = " << max_level << "\n";
+    if (moore && max_level > 0)
+        pp.error("N", "amr.max_level > 0 is not supported reaching Moore -- "
+                      "set amr.max_level = 0 for a fat->Moore run");
+
+    if (moore && !b.c_sched.has_switch)
+    {
+        double gamma = 0.0, target = 0.0;
+        pp.get("gamma", gamma);
+        pp.get("target_log_range", target);
+        if (gamma <= 0.0)
+            pp.error("gamma", "must be > 0");
+        const double D = moore_max_log_dynamic_range(N, N1, N2, gamma);
+        if (log)
+            *log << "  Moore from tau_i: achievable log range " << D
+                 << ", requested " << target
+                 << " (geometry and stop time are set by hand)\n";
+        if (D < target)
+            pp.error("target_log_range",
+                     "cannot be reached with the requested N, N1, N2, gamma");
+        return;
+    }
+
+    if (moore)
+    {
+        const double tau_sw = b.c_sched.tau_switch;
+        const double gamma = 1.0 / b.H_over_mr_direct(tau_sw);
+        const MooreBoxPlan plan = compute_moore_box_plan(
+            N, N1, N2, gamma, a_inv, b.b_inv, b.R(tau_sw),
+            std::sqrt(b.lambda(tau_sw)), tau_sw);
+        if (log)
+            *log << "  Moore through a switch at tau = " << tau_sw
+                 << " (m_r/H = " << gamma << "): L_tilde = " << plan.L_tilde
+                 << ", dx = " << plan.dx << ", D = " << plan.D
+                 << ", tau_end = " << plan.tau_end << "\n";
+        double target = 0.0;
+        if (pp.query("target_log_range", target) && plan.D < target)
+            pp.error("target_log_range",
+                     "cannot be reached with the requested N, N1, N2 -- "
+                     "increase N or reduce N1/N2/the switch's log(m_r/H)");
+        plan_prob_extent(t, plan.L_tilde, log);
+        plan_stop(t, plan.tau_end, log);
+        return;
+    }
+
+    const double c = b.c_sched.c0;
+    std::string ic_mode = "homogeneous";
+    pp.queryAdd("ic_mode", ic_mode);
+
+    if (max_level > 0)
+    {
+        const AmrBoxPlan plan =
+            compute_amr_box_plan(N, N1, N2, a_inv, c, max_level);
+        if (log)
+        {
+            *log << "  refined: tau_f = " << plan.tau_f << " (tau_i = "
+                 << tau_i << "), L_tilde = " << plan.L_tilde
+                 << ", dx_finest = " << plan.dx_finest
+                 << ", dx_base = " << plan.dx_base << "\n";
+            for (int ell = 1; ell <= max_level; ++ell)
+                *log << "    level " << ell << " from log(m_r/H) = "
+                     << plan.log_add[size_t(ell - 1)] << "\n";
+        }
+        if (plan.N_base <= 0)
+            pp.error("N", "N is not evenly divisible by 2^amr.max_level -- "
+                          "choose an effective resolution N that is a clean "
+                          "multiple of 2^max_level");
+        if (plan.tau_f <= tau_i)
+            pp.error("N", "the requested N, N1, N2, c configuration gives "
+                          "tau_f <= tau_i; cannot evolve forward from tau_i");
+        if (ic_mode == "fourier_relaxed" &&
+            !t.contains("axion_strings.pre_evolution.gamma") &&
+            -std::log(b.H_over_mr_direct(tau_i)) >= plan.log_add[0] && log)
+            *log << "  WARNING: log_mr_over_h_i is at or past the level-1 "
+                    "threshold; relaxing at 1/dx_base all the same\n";
+        apply_pre_evolution_gamma_default(t, ic_mode, plan.dx_base, log);
+        plan_n_cell(t, plan.N_base,
+                    "the box-planning-derived base-level grid size "
+                    "(N/2^max_level)",
+                    log);
+        plan_prob_extent(t, plan.L_tilde, log);
+        std::string sched;
+        for (double v : plan.log_add)
+            sched += (sched.empty() ? "" : " ") + exact_str(v);
+        t.set("axion_strings.derived_level_add_log_mr_over_h", sched);
+        plan_stop(t, plan.tau_f, log);
+        return;
+    }
+
+    plan_n_cell(t, N, "axion_strings.N", log);
+    const GeneralBoxPlan plan = compute_general_box_plan(N, N1, N2, a_inv, c);
+    if (log)
+        *log << "  tau_f = " << plan.tau_f << " (tau_i = " << tau_i
+             << "), L_tilde = " << plan.L_tilde << ", dx = " << plan.delta_x
+             << ", dx/3 = " << plan.delta_tau_leapfrog_bound << "\n"
+             << "  (pre-evolution box for this plan, reported only: "
+             << pre_evolution_L_tilde(plan.L_tilde, a_inv, c, tau_i)
+             << ")\n";
+    if (plan.tau_f <= tau_i)
+        pp.error("N", "the requested N, N1, N2, c configuration gives tau_f "
+                      "<= tau_i; cannot evolve forward from tau_i");
+    apply_pre_evolution_gamma_default(t, ic_mode, plan.delta_x, log);
+    plan_prob_extent(t, plan.L_tilde, log);
+    plan_stop(t, plan.tau_f, log);
+}
+
+// The rest of check_params, AxionStringsParams.hpp:1142-1208, plus the
+// masking / tagging reads that AxionStringsLevel::variableSetUp performs at
+// level set-up (AxionStringsLevel.cpp:186-190), so that every key error
+// surfaces here.  Reads only; the box plan is applied before it.
+inline void validate_params(const ParamTable &t)
 {
     read_masking_params(t, "axion_strings.masking");
     read_tagging_params(t);
@@ -313,6 +552,21 @@ inline void check_params(const ParamTable &t)
                  "requires axion_strings.masking.scheme = B -- the spectrum "
                  "saved to axion_spectrum.dat must be the genuinely screened "
                  "field");
+}
+
+// check_params, AxionStringsParams.hpp:1142-1208: the box plan (which adds
+// the derived keys to the table; log = where to print it, or nullptr), then
+// every read and check.
+inline void check_params(ParamTable &t, std::ostream *log = nullptr)
+{
+    bool flat_space = false;
+    ParmParse(t, "axion_strings").queryAdd("flat_space", flat_space);
+    if (!flat_space)
+    {
+        const Background b = read_background(t);
+        apply_box_plan(t, b, read_tau_i(t, b), log);
+    }
+    validate_params(t);
 }
 
 } // namespace Params

@@ -4,7 +4,8 @@
 //   axion_strings params.txt [key=value ...] [--check] [--out DIR]
 //
 // --check only parses and validates the parameters (the reference's
-// just_check_params), without touching a GPU.
+// just_check_params), without touching a GPU, and lists the keys the box
+// plan derived ("derived key = values").
 #include "AxionStringsLevel.hpp"
 
 #include <chrono>
@@ -35,9 +36,19 @@ int main(int argc, char **argv)
             else
                 table.add_line(argv[i]);
         }
-        Params::check_params(table);
+        const ParamTable given = table;
+        Params::check_params(table, &std::cout);
         if (check_only)
         {
+            // the keys the box plan added, as the run would read them
+            for (const auto &kv : table.all())
+                if (!given.contains(kv.first))
+                {
+                    std::cout << "derived " << kv.first << " =";
+                    for (const auto &v : kv.second)
+                        std::cout << " " << v;
+                    std::cout << "\n";
+                }
             std::cout << "parameters ok\n";
             return 0;
         }

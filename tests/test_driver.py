@@ -25,7 +25,8 @@ def run(args, cwd=None):
 
 @pytest.mark.parametrize("name", ["params_plain_64.txt", "params_scale_512.txt",
                                   "params_precision_1024.txt",
-                                  "params_relaxed_64.txt"])
+                                  "params_relaxed_64.txt",
+                                  "params_planned_256.txt"])
 def test_shipped_params_validate(name):
     r = run([os.path.join(PARAMS, name), "--check"])
     assert r.returncode == 0, r.stderr
