@@ -23,7 +23,14 @@ coarse-fine ghosts interpolate the coarse state in time, then the fine
 valid cells are averaged down onto the coarse level.  Each RK stage of each
 level is one persistent launch over that level's box work-list.
 
+With a box plan (the reference's axion_strings.N, N1, N2 with
+amr.max_level > 0) the base grid, its spacing and the level schedule come
+from amr_box_plan, and a level is only created once log(m_r/H) has reached
+its entry of the schedule (the reference's schedule-gated tagger).
+
     python -m axion_strings_adaptive_amd.amr --n0 256 --levels 2 --steps 4
+    python -m axion_strings_adaptive_amd.amr --plan 1024 1 4 --levels 2 \
+        --log-mr-over-h-i 1.0 --regrid-int 4 --steps 100
 """
 
 from __future__ import annotations
@@ -36,6 +43,28 @@ import time
 import numpy as np
 
 from . import capi
+
+
+def amr_box_plan(N_effective: int, N1: float, N2: float, a_inv: float,
+                 c: float, max_level: int) -> dict:
+    """compute_amr_box_plan (BoxPlan.hpp:96-125; the same arithmetic as the
+    driver's csrc/host/BoxPlan.hpp): N is the finest level's effective
+    resolution, reached at tau_f with H L = N1 and 1/(dx m_r) = N2 (tau0 = 1,
+    lambda0 = 1, R0 = 1/(a_inv - 1)); level l (entry l-1 of log_add) is
+    needed from log(m_r/H) = log_f - (max_level - l + 1) ln2 (a_inv-c)/(1-c).
+    N_base is -1 when 2^max_level does not divide N."""
+    grow = (N_effective / (N1 * N2)) ** ((a_inv - 1.0) / (a_inv - c))
+    R0 = 1.0 / (a_inv - 1.0)
+    L_tilde = (N1 / R0) * grow
+    dx_finest = L_tilde / N_effective
+    refine = 1 << max_level
+    log_f = (a_inv - c) / (a_inv - 1.0) * np.log(grow)
+    spacing = np.log(2.0) * (a_inv - c) / (1.0 - c)
+    return {"L_tilde": L_tilde, "dx_finest": dx_finest,
+            "dx_base": dx_finest * refine, "tau_f": grow,
+            "N_base": N_effective // refine if N_effective % refine == 0 else -1,
+            "log_add": [float(log_f - (max_level - l + 1) * spacing)
+                        for l in range(1, max_level + 1)]}
 
 
 def dilate_periodic(mask: np.ndarray, r: int) -> np.ndarray:
@@ -162,8 +191,13 @@ class Hierarchy:
                  dt_multiplier: float = 0.2, verbose: bool = True,
                  force_full_refinement: bool = False, use_graph: bool = True,
                  regrid_int: int = 0, gradient_threshold: float = capi.DBL_MAX,
-                 radial_gradient_threshold: float = capi.DBL_MAX):
+                 radial_gradient_threshold: float = capi.DBL_MAX,
+                 level_add_log_mr_over_h=None):
         self.ctx = ctx
+        # the box plan's level schedule (entry l-1: log(m_r/H) from which
+        # level l may exist), or None for no gating
+        self.level_add = (None if level_add_log_mr_over_h is None
+                          else [float(v) for v in level_add_log_mr_over_h])
         self.use_graph = use_graph
         self.max_levels = levels
         self.regrid_int = regrid_int
@@ -228,6 +262,8 @@ class Hierarchy:
         one block inside the coverage of level l (l >= 1), merge."""
         coarse = self.levels[l]
         cd = self.domains[l]
+        if not self.level_allowed(l + 1):
+            return [], None
         R = capi.background_eval(self.bg, 0, self.tau)
         tags_buf, _ = coarse.tag(amp_thr=self.amp_thr, R=R,
                                  grad_thr=self.grad_thr,
@@ -242,6 +278,18 @@ class Hierarchy:
         boxes = boxes_from_blocks(blk, self.blocking_factor,
                                   self.max_grid_size // self.blocking_factor)
         return boxes, blk
+
+    def level_allowed(self, l: int) -> bool:
+        """Schedule gating of the tagger (AxionStringsLevel.cpp:1957-1983):
+        level l may be created once log(m_r/H) at the current tau has
+        reached the schedule's entry for it; with force_full_refinement,
+        or without a schedule, always."""
+        if self.force_full_refinement or self.level_add is None:
+            return True
+        if l > len(self.level_add):
+            return False
+        log_mr_over_h = -np.log(capi.background_eval(self.bg, 5, self.tau))
+        return bool(log_mr_over_h >= self.level_add[l - 1])
 
     def _lap(self, key, t0):
         """Adds the device-synchronised time since t0 to regrid_parts[key]."""
@@ -396,21 +444,49 @@ def main(argv=None):
                     help="tag where dx |grad |psi|| exceeds this")
     ap.add_argument("--regrid-int", type=int, default=0,
                     help="rebuild the finer levels every N coarse steps")
+    ap.add_argument("--plan", type=float, nargs=3, metavar=("N", "N1", "N2"),
+                    help="box plan (axion_strings.N, N1, N2): the base grid "
+                         "N/2^levels, its spacing, and the level schedule; "
+                         "constant c = --c0 from log(m_r/H) = "
+                         "--log-mr-over-h-i, stopping at tau_f")
+    ap.add_argument("--c0", type=float, default=0.0)
+    ap.add_argument("--log-mr-over-h-i", type=float, default=0.0)
     args = ap.parse_args(argv)
     ctx = capi.Context(0)
-    h = Hierarchy(ctx, args.n0, args.levels, precision=args.precision,
+    kw = {}
+    n0 = args.n0
+    tau_f = None
+    if args.plan:
+        N, N1, N2 = int(args.plan[0]), args.plan[1], args.plan[2]
+        plan = amr_box_plan(N, N1, N2, 2.0, args.c0, args.levels)
+        if plan["N_base"] <= 0:
+            ap.error("--plan: N is not divisible by 2^levels")
+        bg = capi.make_background(2.0, c0=args.c0)
+        tau0 = capi.background_eval(bg, 7, args.log_mr_over_h_i)
+        if plan["tau_f"] <= tau0:
+            ap.error("--plan: tau_f <= tau_i")
+        n0, tau_f = plan["N_base"], plan["tau_f"]
+        kw = dict(bg=bg, tau0=tau0, dx0=plan["dx_base"],
+                  level_add_log_mr_over_h=plan["log_add"])
+    h = Hierarchy(ctx, n0, args.levels, precision=args.precision,
                   regrid_int=args.regrid_int,
                   gradient_threshold=args.gradient_threshold,
-                  radial_gradient_threshold=args.radial_gradient_threshold)
+                  radial_gradient_threshold=args.radial_gradient_threshold,
+                  **kw)
     h.step()  # warm-up
     ctx.synchronize()
     h.regrid_s, n_regrid0 = 0.0, h.nregrids
     h.regrid_parts = {}
     t0 = time.perf_counter()
+    steps = 0
     for _ in range(args.steps):
+        if tau_f is not None and h.tau >= tau_f:
+            break
         h.step()
+        steps += 1
     ctx.synchronize()
     dt = time.perf_counter() - t0
+    args.steps = max(1, steps)
     e = h.composite_energy()
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -429,7 +505,9 @@ def main(argv=None):
             1e3 * (dt - h.regrid_s) / args.steps,
         "regrid_ms_by_phase": {k: 1e3 * v / max(1, h.nregrids - n_regrid0)
                                for k, v in h.regrid_parts.items()},
-        "n_total": e["n_total"], "L_tilde^3": (args.n0 * 0.5) ** 3,
+        "tau": h.tau, "tau_f": tau_f,
+        "n_total": e["n_total"],
+        "L_tilde^3": (n0 * h.dx[0]) ** 3,
         "rho_tot_unscreened": e["rho_tot_unscreened"],
         "composite_spectrum_ms": 1e3 * t_spec,
         "composite_spectrum_nonempty_shells": int((sc > 0).sum())}))

@@ -1,8 +1,10 @@
 """Host-side box building of the static AMR hierarchy (amr.py): the boxes
 partition exactly the tagged blocks, respect the blocking factor and the
-maximum grid size, and a fully tagged region becomes max-size boxes."""
+maximum grid size, and a fully tagged region becomes max-size boxes; the
+hierarchy's box plan equals the oracle's."""
 
 import numpy as np
+import pytest
 
 from axion_strings_adaptive_amd import amr
 
@@ -52,3 +54,17 @@ def test_tagged_blocks_equals_dilate_then_block():
         tags = rng.random(shape) < rng.choice([0.001, 0.01, 0.1])
         want = amr.blocks_from_tags(amr.dilate_periodic(tags, r), block)
         assert np.array_equal(amr.tagged_blocks(tags, block, r), want)
+
+
+@pytest.mark.parametrize("args", [(1024, 1.0, 4.0, 2.0, 0.0, 2),
+                                  (4096, 1.2, 8.0, 2.0, 0.0, 4),
+                                  (4097, 1.2, 8.0, 2.0, 0.0, 4),
+                                  (8192, 1.0, 6.0, 2.5, 0.0, 3),
+                                  (512, 1.0, 4.0, 2.0, 0.5, 1)])
+def test_amr_box_plan_equals_oracle(oracle, args):
+    """The hierarchy's box plan (base grid, spacings, tau_f, level schedule)
+    is the oracle's compute_amr_box_plan exactly."""
+    p = amr.amr_box_plan(*args)
+    L, dxf, dxb, tau_f, nb, la = oracle.amr_box_plan(*args)
+    assert (p["L_tilde"], p["dx_finest"], p["dx_base"], p["tau_f"],
+            p["N_base"], p["log_add"]) == (L, dxf, dxb, tau_f, nb, la)

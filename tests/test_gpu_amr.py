@@ -162,3 +162,38 @@ def test_composite_level0_collapses_by_average_down(ctx):
     ss, sc, tot, sumsq = h.composite_spectrum()
     assert (sc > 0).sum() > 0 and np.all(np.isfinite(ss))
     h.close()
+
+
+@pytest.mark.gpu
+def test_schedule_gates_level_creation(ctx):
+    """With a level schedule (the box plan's log(m_r/H) per level), a level
+    appears at the first regrid after log(m_r/H) = 2 ln tau (c = 0) reaches
+    its entry, not before; force_full_refinement bypasses the gate
+    (AxionStringsLevel.cpp:1939-1983)."""
+    from axion_strings_adaptive_amd import amr, capi
+    bg = capi.make_background(2.0, c0=0.0)
+    sched = [2 * np.log(1.15), 2 * np.log(1.35)]
+    h = amr.Hierarchy(ctx, n0=32, levels=2, precision=capi.AXS_FP64,
+                      verbose=False, bg=bg, tau0=1.0, regrid_int=1,
+                      level_add_log_mr_over_h=sched)
+    seen = [len(h.levels)]
+    for _ in range(5):
+        h.step()
+        seen.append(len(h.levels))
+    assert seen == [1, 1, 2, 2, 3, 3], seen
+    e = h.composite_energy()
+    assert np.isfinite(e["rho_tot_unscreened"])
+    h.close()
+    h = amr.Hierarchy(ctx, n0=32, levels=2, precision=capi.AXS_FP64,
+                      verbose=False, bg=bg, tau0=1.0, regrid_int=1,
+                      level_add_log_mr_over_h=sched[:1])
+    for _ in range(5):
+        h.step()
+    assert len(h.levels) == 2  # no schedule entry for level 2
+    h.close()
+    h = amr.Hierarchy(ctx, n0=32, levels=1, precision=capi.AXS_FP64,
+                      verbose=False, bg=bg, tau0=1.0,
+                      level_add_log_mr_over_h=sched,
+                      force_full_refinement=True)
+    assert len(h.levels) == 2
+    h.close()
