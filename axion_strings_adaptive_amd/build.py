@@ -27,6 +27,7 @@ DRIVER = os.path.join(BUILD, "axion_strings")
 HIP_SOURCES = [
     "axs_capi.hip",
     "axs_rk.hip",
+    "axs_pair.hip",
     "axs_diag.hip",
     "axs_fft.hip",
     "axs_boxes.hip",

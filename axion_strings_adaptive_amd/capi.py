@@ -79,6 +79,9 @@ _SIGS = {
     "axs_level_set_homogeneous": (_I, [_P, _D, _D]),
     "axs_level_handoff_rescale": (_I, [_P, _D, _D, _D]),
     "axs_level_set_tile": (_I, [_P, _I, _I]),
+    "axs_num_pair_variants": (_I, []),
+    "axs_level_set_stage_fusion": (_I, [_P, _I]),
+    "axs_level_set_pair_tile": (_I, [_P, _I, _I]),
     "axs_level_get_tile": (_I, [_P, C.POINTER(_I), C.POINTER(_I)]),
     "axs_rhs_eval": (_I, [_P, C.POINTER(RhsCoeffs), _P]),
     "axs_rk4_step": (_I, [_P, C.POINTER(RhsCoeffs), _D]),
@@ -323,6 +326,14 @@ class Level:
 
     def set_tile(self, variant: int, z_chunk: int = 0):
         check(self.lib.axs_level_set_tile(self.h, variant, z_chunk))
+
+    def set_fusion(self, on: bool):
+        """RK4 stages in two fused launches (True, the default) or one
+        launch per stage; the same bits either way."""
+        check(self.lib.axs_level_set_stage_fusion(self.h, int(bool(on))))
+
+    def set_pair_tile(self, variant: int, z_chunk: int = 0):
+        check(self.lib.axs_level_set_pair_tile(self.h, variant, z_chunk))
 
     def rhs(self, coeffs: RhsCoeffs) -> np.ndarray:
         buf = DeviceBuffer(4 * self.ncell * self.word)

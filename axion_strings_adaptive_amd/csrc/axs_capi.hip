@@ -418,6 +418,9 @@ int axs_level_create(axs_ctx *ctx, int nx, int ny, int nz, double dx,
     L->word = precision == AXS_FP64 ? 8 : 4;
     L->tile_variant = -1;
     L->z_chunk = 0;
+    L->fused = 1;
+    L->pair_variant = -1;
+    L->pair_z_chunk = 0;
     bool ok = alloc_arrays(L);
     ok = ok &&
          hipMalloc(&L->d_coeff, (size_t)kCoeffTableSteps * 12 * sizeof(double)) ==
@@ -483,6 +486,31 @@ int axs_level_set_tile(axs_level *L, int variant, int z_chunk)
         destroy_graph(L);
     L->tile_variant = variant;
     L->z_chunk = z_chunk;
+    return AXS_OK;
+}
+
+int axs_num_pair_variants(void) { return axs::num_pair_variants(); }
+
+int axs_level_set_stage_fusion(axs_level *L, int on)
+{
+    AXS_CHECK_ARG(L, "axs_level_set_stage_fusion: NULL level");
+    AXS_CHECK_ARG(on == 0 || on == 1, "axs_level_set_stage_fusion: on is 0 or 1");
+    if (on != L->fused)
+        destroy_graph(L);
+    L->fused = on;
+    return AXS_OK;
+}
+
+int axs_level_set_pair_tile(axs_level *L, int variant, int z_chunk)
+{
+    AXS_CHECK_ARG(L, "axs_level_set_pair_tile: NULL level");
+    AXS_CHECK_ARG(variant >= -1 && variant < axs::num_pair_variants(),
+                  "axs_level_set_pair_tile: bad variant");
+    AXS_CHECK_ARG(z_chunk >= 0, "axs_level_set_pair_tile: bad z_chunk");
+    if (variant != L->pair_variant || z_chunk != L->pair_z_chunk)
+        destroy_graph(L);
+    L->pair_variant = variant;
+    L->pair_z_chunk = z_chunk;
     return AXS_OK;
 }
 
@@ -719,7 +747,8 @@ static int run_step_timed(axs_level *L, const int *step_ptr, double dt)
     {
         float ms = 0.0f;
         AXS_HIP_TRY(hipEventElapsedTime(&ms, L->ev[st], L->ev[st + 1]));
-        L->stage_ms[st] += ms;
+        // with fusion on, entries 0 and 1 hold stages 1+2 and 3+4
+        L->stage_ms[L->fused ? st / 2 : st] += ms;
     }
     L->timed_steps += 1;
     return AXS_OK;
@@ -737,6 +766,68 @@ static int run_step(axs_level *L, const int *step_ptr, double dt)
     if ((rc = launch_rk_stage(L, 3, L->yb, L->ya, L->d_coeff, step_ptr, 2, dt)))
         return rc;
     return launch_rk_stage(L, 4, L->ya, L->y, L->d_coeff, step_ptr, 3, dt);
+}
+
+// One RK4 step as two fused launches (stages 1+2, stages 3+4) from the state
+// in `from` into `to` (to != from: pair 2 reads psi_n around every cell);
+// FP32C lo is updated in place.
+static int run_step_fused(axs_level *L, const int *step_ptr, double dt,
+                          const void *from, void *to)
+{
+    hipStream_t s = L->ctx->stream;
+    int rc;
+    if (L->timing)
+        AXS_HIP_TRY(hipEventRecord(L->ev[0], s));
+    if ((rc = launch_rk_pair(L, 1, from, from, L->yb, L->d_coeff, step_ptr,
+                             dt)))
+        return rc;
+    if (L->timing)
+        AXS_HIP_TRY(hipEventRecord(L->ev[1], s));
+    if ((rc = launch_rk_pair(L, 2, L->yb, from, to, L->d_coeff, step_ptr, dt)))
+        return rc;
+    if (!L->timing)
+        return AXS_OK;
+    AXS_HIP_TRY(hipEventRecord(L->ev[2], s));
+    AXS_HIP_TRY(hipEventSynchronize(L->ev[2]));
+    for (int i = 0; i < 2; ++i)
+    {
+        float ms = 0.0f;
+        AXS_HIP_TRY(hipEventElapsedTime(&ms, L->ev[i], L->ev[i + 1]));
+        L->stage_ms[i] += ms;
+    }
+    L->timed_steps += 1;
+    return AXS_OK;
+}
+
+// count RK4 steps; the state ends in L->y.  With fusion on, steps go in twos
+// (L->y -> L->ya -> L->y) and an odd last step runs stage by stage (in
+// place); both give the same bits.  With advance, the device step counter
+// moves on after every step.
+static int run_steps(axs_level *L, const int *step_ptr, double dt, int count,
+                     bool advance)
+{
+    int rc;
+    int i = 0;
+    if (L->fused)
+        for (; i + 2 <= count; i += 2)
+        {
+            if ((rc = run_step_fused(L, step_ptr, dt, L->y, L->ya)))
+                return rc;
+            if (advance && (rc = launch_step_advance(L, 0)))
+                return rc;
+            if ((rc = run_step_fused(L, step_ptr, dt, L->ya, L->y)))
+                return rc;
+            if (advance && (rc = launch_step_advance(L, 0)))
+                return rc;
+        }
+    for (; i < count; ++i)
+    {
+        if ((rc = run_step(L, step_ptr, dt)))
+            return rc;
+        if (advance && (rc = launch_step_advance(L, 0)))
+            return rc;
+    }
+    return AXS_OK;
 }
 
 int axs_rk4_step(axs_level *L, const axs_rhs_coeffs coeffs[4], double dt)
@@ -760,10 +851,9 @@ static int capture_graph(axs_level *L, double dt)
 {
     destroy_graph(L);
     hipStream_t s = L->ctx->stream;
+    const int per = L->fused ? 2 : 1; // steps per replay
     AXS_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    int rc = run_step(L, L->d_step, dt);
-    if (rc == AXS_OK)
-        rc = launch_step_advance(L, 0);
+    int rc = run_steps(L, L->d_step, dt, per, true);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(s, &g);
     if (rc != AXS_OK)
@@ -787,6 +877,7 @@ static int capture_graph(axs_level *L, double dt)
     L->graph_exec = ge;
     L->graph_valid = 1;
     L->graph_key[0] = dt;
+    L->graph_key[1] = per;
     return AXS_OK;
 }
 
@@ -820,22 +911,19 @@ static int advance_table(axs_level *L, double dt, int nsteps, int use_graph,
             return rc;
         if (use_graph && !L->timing)
         {
-            if (!L->graph_valid || L->graph_key[0] != dt)
+            const int per = L->fused ? 2 : 1;
+            if (!L->graph_valid || L->graph_key[0] != dt ||
+                L->graph_key[1] != per)
                 if ((rc = capture_graph(L, dt)))
                     return rc;
-            for (int i = 0; i < chunk; ++i)
+            int i = 0;
+            for (; i + per <= chunk; i += per)
                 AXS_HIP_TRY(hipGraphLaunch(L->graph_exec, L->ctx->stream));
+            if (i < chunk && (rc = run_steps(L, L->d_step, dt, chunk - i, true)))
+                return rc;
         }
-        else
-        {
-            for (int i = 0; i < chunk; ++i)
-            {
-                if ((rc = run_step(L, L->d_step, dt)))
-                    return rc;
-                if ((rc = launch_step_advance(L, 0)))
-                    return rc;
-            }
-        }
+        else if ((rc = run_steps(L, L->d_step, dt, chunk, true)))
+            return rc;
         done += chunk;
     }
     return AXS_OK;
@@ -899,9 +987,8 @@ static int time_placement(axs_level *L, hipEvent_t e0, hipEvent_t e1,
     if (rc || (rc = run_step(L, nullptr, 0.1)))
         return rc;
     AXS_HIP_TRY(hipEventRecord(e0, s));
-    for (int i = 0; i < 2; ++i)
-        if ((rc = run_step(L, nullptr, 0.1)))
-            return rc;
+    if ((rc = run_steps(L, nullptr, 0.1, 2, false)))
+        return rc;
     AXS_HIP_TRY(hipEventRecord(e1, s));
     AXS_HIP_TRY(hipEventSynchronize(e1));
     float ms = 0.0f;
@@ -990,11 +1077,18 @@ int axs_level_place(axs_level *L, int trials, double *ms_per_step, int *used)
     // 2 GiB of headroom beyond a second copy of the arrays
     const size_t need = level_array_bytes(L) + ((size_t)2 << 30);
     // the fast regime: the step's algorithmic bytes at >= 69% of 8 TB/s
-    const double step_bytes = L->precision == AXS_FP32C  ? 304.0
-                              : L->precision == AXS_FP64 ? 544.0
-                                                         : 272.0;
+    // the RK4 step's algorithmic bytes per cell: per-stage launches
+    // 19 / 17 / 17 S, fused pairs 9 / 7 / 7 S (S = 4 words)
+    const double S = 4.0 * (double)L->word;
+    const double step_bytes =
+        L->fused ? (L->precision == AXS_FP32C ? 9.0 : 7.0) * S
+                 : (L->precision == AXS_FP32C ? 19.0 : 17.0) * S;
+    // the fast regime's floor (1024^3: FP32C fused pairs 31.3 ms against
+    // 33.8 ms per step, FP64 54.7 ms)
+    const double fast_frac =
+        L->fused ? (L->precision == AXS_FP64 ? 0.54 : 0.60) : 0.69;
     const double fast_ms =
-        step_bytes * (double)L->g.ncell / (0.69 * 8.0e12) * 1e3;
+        step_bytes * (double)L->g.ncell / (fast_frac * 8.0e12) * 1e3;
     bool sequential = false;
     while (rc == AXS_OK && n < trials && !(best <= fast_ms))
     {

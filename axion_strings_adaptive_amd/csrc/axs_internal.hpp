@@ -92,6 +92,9 @@ struct axs_level
     int *d_step;        // device step counter (graph replays)
     int tile_variant;   // stage-kernel tile variant
     int z_chunk;        // z planes per block
+    int fused;          // RK4 stages in two fused pairs (else one per stage)
+    int pair_variant;   // fused-pair kernel tile variant (-1 = default)
+    int pair_z_chunk;   // z planes per block of the pair kernels (0 = auto)
     // captured one-step graph
     hipGraph_t graph;
     hipGraphExec_t graph_exec;
@@ -114,6 +117,10 @@ namespace axs
 int launch_rk_stage(axs_level *L, int stage, const void *yin, void *yout,
                     const double *coeff_table, const int *step_ptr,
                     int coeff_stage, double dt);
+int launch_rk_pair(axs_level *L, int pair, const void *yin, const void *yn,
+                   void *yout, const double *coeff_table, const int *step_ptr,
+                   double dt);
+int num_pair_variants();
 int launch_rhs_eval(axs_level *L, const double *coeff3_dev, void *rhs);
 int launch_step_advance(axs_level *L, int reset);
 int num_tile_variants();

@@ -18,6 +18,7 @@
 // dispatched to the same XCD and its L2.
 #include "axs_internal.hpp"
 #include "axs_stage_math.hpp"
+#include "axs_vec.hpp"
 
 namespace axs
 {
@@ -43,64 +44,6 @@ template <typename T> struct StageParams
     int xcd_remap;
     int nontemporal; // stream the stage outputs past the caches
 };
-
-__device__ __forceinline__ int wrap_lo(int i, int n) { return i < 0 ? i + n : i; }
-__device__ __forceinline__ int wrap_hi(int i, int n) { return i >= n ? i - n : i; }
-
-template <typename T> struct StarOf;
-template <> struct StarOf<double>
-{
-    using type = Star;
-};
-template <> struct StarOf<float>
-{
-    using type = StarF;
-};
-
-// VX consecutive x cells per lane: vector loads/stores of VX words
-template <typename T, int VX> struct VecOf
-{
-    typedef T type __attribute__((ext_vector_type(VX)));
-};
-
-template <typename T, int VX>
-__device__ __forceinline__ void vload(const T *p, T (&o)[VX])
-{
-    if constexpr (VX == 1)
-        o[0] = p[0];
-    else
-    {
-        const typename VecOf<T, VX>::type v =
-            *reinterpret_cast<const typename VecOf<T, VX>::type *>(p);
-#pragma unroll
-        for (int i = 0; i < VX; ++i)
-            o[i] = v[i];
-    }
-}
-
-template <typename T, int VX>
-__device__ __forceinline__ void vstore(T *p, const T (&x)[VX], bool nt)
-{
-    if constexpr (VX == 1)
-    {
-        if (nt)
-            __builtin_nontemporal_store(x[0], p);
-        else
-            p[0] = x[0];
-    }
-    else
-    {
-        typename VecOf<T, VX>::type v;
-#pragma unroll
-        for (int i = 0; i < VX; ++i)
-            v[i] = x[i];
-        auto *q = reinterpret_cast<typename VecOf<T, VX>::type *>(p);
-        if (nt)
-            __builtin_nontemporal_store(v, q);
-        else
-            *q = v;
-    }
-}
 
 // Each workgroup: a (64 VX) x TY column tile, streaming a chunk of z.
 // PF: the centre loads (Pi, y_n, acc, lo) of plane z+1 are issued before

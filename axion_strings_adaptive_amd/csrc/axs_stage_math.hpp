@@ -114,6 +114,19 @@ __device__ __forceinline__ double final_f64(double y, double h6, double acc,
     return y + h6 * (acc + k);
 }
 
+// y + a k, as stages 1-3 form their outputs (the fused stage pairs also
+// form the next stage's psi at stencil points with it, so both agree to the
+// bit).
+template <typename T, int PREC>
+__device__ __forceinline__ T stage_axpy(T y, double a, T k)
+{
+#pragma clang fp contract(off)
+    if constexpr (PREC == AXS_FP64)
+        return axpy_f64(y, a, k);
+    else
+        return fmaf((float)a, k, y);
+}
+
 // Stage update of one cell.  k[4] = (Pi1, Pi2, F1, F2) of the stage input;
 // yn[4] = y_n (stage 1: the stage input itself), ac[4] the accumulator,
 // lo[4] the FP32C low part of y_n.  Outputs: acc_out (stages 1-3),
@@ -135,10 +148,7 @@ __device__ __forceinline__ void stage_update(const T k[4], const T yn[4],
         for (int c = 0; c < 4; ++c)
         {
             acc_out[c] = k[c];
-            if constexpr (PREC == AXS_FP64)
-                y_out[c] = axpy_f64(yn[c], a, k[c]);
-            else
-                y_out[c] = fmaf((float)a, k[c], yn[c]);
+            y_out[c] = stage_axpy<T, PREC>(yn[c], a, k[c]);
         }
     }
     else if constexpr (MODE == MODE_S23)
@@ -146,15 +156,10 @@ __device__ __forceinline__ void stage_update(const T k[4], const T yn[4],
         for (int c = 0; c < 4; ++c)
         {
             if constexpr (PREC == AXS_FP64)
-            {
                 acc_out[c] = acc2_f64(ac[c], k[c]);
-                y_out[c] = axpy_f64(yn[c], a, k[c]);
-            }
             else
-            {
                 acc_out[c] = fmaf(2.0f, k[c], ac[c]);
-                y_out[c] = fmaf((float)a, k[c], yn[c]);
-            }
+            y_out[c] = stage_axpy<T, PREC>(yn[c], a, k[c]);
         }
     }
     else

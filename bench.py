@@ -7,7 +7,9 @@ radiation era a = tau from tau = 1, lambda = eta = 1 with the c = 2 (Moore)
 schedule reached through a switch at tau = 1 (c0 = 0, c1 = 2), dtau = 0.1;
 psi1, psi2 hash-seeded Gaussian fields (seed 42, modes 0 < |p| <= N/8,
 <psi_i^2> = 0.5), Pi = 0, generated on the device.  One step = one RK4 step
-of every cell (four fused RHS + RK-stage launches).  The FP32C run is the
+of every cell: two launches, stages 1+2 and stages 3+4, each fusing the RHS
+of two stages with their RK updates (--no-fusion: one launch per stage, the
+same bits).  The FP32C run is the
 headline `value`; the FP64 run on the same IC is timed the same way and
 reported beside it together with the precision metrics (max |dphi|/eta,
 relative energy error, delta xi) at the end of the timed steps.
@@ -40,11 +42,16 @@ HBM_PEAK_GBS = 8000.0
 #   FP32C (S = 16 B): stage1 3S, stage2 5S, stage3 5S, stage4 6S (y, lo rd/wr)
 #   FP64  (S = 32 B): stage1 3S, stage2 5S, stage3 5S, stage4 4S
 STAGE_BYTES = {1: [48, 80, 80, 96], 0: [96, 160, 160, 128]}
+# ... of each fused-pair launch: pair 1 reads y_n, writes acc and y_2 (3S);
+# pair 2 reads y_2, y_n, acc, writes y_{n+1} (4S) + FP32C lo read/write (2S)
+PAIR_BYTES = {1: [48, 96], 0: [96, 128]}
 # HBM bytes per cell of one FP32C stage-2/3 launch at 1024^3, from
 # rocprofv3 --pmc FETCH_SIZE (x 2.0) and WRITE_SIZE (x 1.0), the scales
 # measured on device copies and reads of known size; keyed by the tile
 # (variant, z_chunk) the launch used.
 MEASURED_S23_BYTES = {(20, 32): 86.03}
+# the same for the fused FP32C pair-2 launch, keyed by (pair variant, z_chunk)
+MEASURED_PAIR2_BYTES = {}
 
 
 def log(msg):
@@ -52,10 +59,13 @@ def log(msg):
 
 
 def run_level(capi, ctx, n, precision, bg, dt, warmup, steps, tile=None,
-              place_trials=0):
+              place_trials=0, fused=True, pair_tile=None):
     lvl = capi.Level(ctx, n, 0.5, precision)
     if tile is not None:
         lvl.set_tile(*tile)
+    lvl.set_fusion(fused)
+    if pair_tile is not None:
+        lvl.set_pair_tile(*pair_tile)
     placement = lvl.place(place_trials) if place_trials > 0 else []
     kmax = n / 8.0
     lvl.fourier_ic(0, kmax, 0.5, 42, 0)
@@ -308,6 +318,10 @@ def main(argv=None):
                     help="CPU baseline sample: RK4 steps")
     ap.add_argument("--tile", type=str, default="",
                     help="variant,z_chunk (default: autotuned cache or built-in)")
+    ap.add_argument("--no-fusion", action="store_true",
+                    help="one launch per RK stage instead of two fused pairs")
+    ap.add_argument("--pair-tile", type=str, default="",
+                    help="fused-pair variant,z_chunk (default: built-in)")
     ap.add_argument("--alloc-check", action="store_true",
                     help="only time the FP32C level on successive "
                          "allocations")
@@ -361,6 +375,10 @@ def main(argv=None):
         from axion_strings_adaptive_amd import autotune
         tile = autotune.cached_choice(n, capi.AXS_FP32C)
 
+    fused = not args.no_fusion
+    pair_tile = (tuple(int(t) for t in args.pair_tile.split(","))
+                 if args.pair_tile else None)
+
     if args.alloc_check:
         res = alloc_check(capi, ctx, n, capi.AXS_FP32C, bg, dt, args.steps,
                           tile)
@@ -391,7 +409,7 @@ def main(argv=None):
     log(f"[bench] FP32C {n}^3: IC + {args.warmup} warmup + {args.steps} timed steps")
     a, wall32, ms32, ns32, tau_end, place32 = run_level(
         capi, ctx, n, capi.AXS_FP32C, bg, dt, args.warmup, args.steps, tile,
-        args.place_trials)
+        args.place_trials, fused, pair_tile)
     if place32:
         log(f"[bench] FP32C placement trials (ms/step): "
             f"{', '.join(f'{t:.2f}' for t in place32)}")
@@ -411,7 +429,7 @@ def main(argv=None):
         b, wall64, ms64, ns64, _, place64 = run_level(
             capi, ctx, n, capi.AXS_FP64, bg, dt, args.warmup, args.steps,
             autotune.cached_choice(n, capi.AXS_FP64) if not args.tile else tile,
-            args.place_trials)
+            args.place_trials, fused, pair_tile)
         if place64:
             log(f"[bench] FP64 placement trials (ms/step): "
                 f"{', '.join(f'{t:.2f}' for t in place64)}")
@@ -432,21 +450,43 @@ def main(argv=None):
         log(f"[bench] FP64 {value64:.4e} cell-updates/s; ratio {value / value64:.3f}")
         b.close()
 
-    # roofline of the dominant kernel: the stage-2/3 launch (two per step)
-    st = [m / ns32 for m in ms32]  # ms per launch of each stage
-    bytes23 = STAGE_BYTES[1][1] * ncell
-    achieved = bytes23 / (0.5 * (st[1] + st[2]) * 1e-3) / 1e9
-    per_cell = MEASURED_S23_BYTES.get(tuple(tile) if tile else None)
-    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                "traffic": per_cell * ncell if per_cell and n == 1024 else None,
-                "traffic_unit": "bytes per launch",
-                "kernel": "k_stage<float, FP32C, MODE_S23> (RK stages 2/3)",
-                "stage_ms": st,
-                "stage_gbs": [STAGE_BYTES[1][i] * ncell / (st[i] * 1e-3) / 1e9
-                              for i in range(4)],
-                "step_gbs": sum(STAGE_BYTES[1]) * ncell /
-                (sum(st) * 1e-3) / 1e9}
+    st = [m / ns32 for m in ms32]  # ms per launch of each stage (pair)
+    if fused:
+        # roofline of the dominant kernel: the longer of the two pair
+        # launches (one each per step)
+        k = 0 if st[0] > st[1] else 1
+        achieved = PAIR_BYTES[1][k] * ncell / (st[k] * 1e-3) / 1e9
+        per_cell = (MEASURED_PAIR2_BYTES.get(tuple(pair_tile) if pair_tile
+                                             else (-1, 0))
+                    if k == 1 else None)
+        roofline = {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": per_cell * ncell if per_cell and n == 1024 else None,
+            "traffic_unit": "bytes per launch",
+            "kernel": f"k_pair<float, FP32C, {k + 1}> (RK stages "
+                      f"{'1+2' if k == 0 else '3+4'})",
+            "algorithmic_bytes_per_cell": PAIR_BYTES[1][k],
+            "pair_ms": st[:2],
+            "pair_gbs": [PAIR_BYTES[1][i] * ncell / (st[i] * 1e-3) / 1e9
+                         for i in range(2)],
+            "step_gbs": sum(PAIR_BYTES[1]) * ncell / (sum(st[:2]) * 1e-3) / 1e9}
+    else:
+        # roofline of the dominant kernel: the stage-2/3 launch (two per step)
+        bytes23 = STAGE_BYTES[1][1] * ncell
+        achieved = bytes23 / (0.5 * (st[1] + st[2]) * 1e-3) / 1e9
+        per_cell = MEASURED_S23_BYTES.get(tuple(tile) if tile else None)
+        roofline = {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": per_cell * ncell if per_cell and n == 1024 else None,
+            "traffic_unit": "bytes per launch",
+            "kernel": "k_stage<float, FP32C, MODE_S23> (RK stages 2/3)",
+            "algorithmic_bytes_per_cell": STAGE_BYTES[1][1],
+            "stage_ms": st,
+            "stage_gbs": [STAGE_BYTES[1][i] * ncell / (st[i] * 1e-3) / 1e9
+                          for i in range(4)],
+            "step_gbs": sum(STAGE_BYTES[1]) * ncell / (sum(st) * 1e-3) / 1e9}
     a.close()
 
     cpu = None
@@ -465,7 +505,9 @@ def main(argv=None):
                                "Gaussian IC (seed 42, |p|<=N/8)",
                    "precision": "fp32 storage + Kahan-compensated y_n "
                                 "(fp32 hi + fp32 lo)",
+                   "stage_fusion": fused,
                    "tile": list(tile) if tile else None,
+                   "pair_tile": list(pair_tile) if pair_tile else None,
                    "placement_trials_ms": place32},
         "fp64": res64, "roofline": roofline, "cpu_baseline": cpu,
         "diagnostics_fp32c": diag_ms,

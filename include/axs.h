@@ -163,6 +163,23 @@ int axs_level_handoff_rescale(axs_level *lvl, double kappa,
  * grids until there are 4096 tiles); z_chunk 0 = default. */
 int axs_level_set_tile(axs_level *lvl, int variant, int z_chunk);
 int axs_level_get_tile(axs_level *lvl, int *variant, int *z_chunk);
+/* Stage fusion; no counterpart in the reference (how the RK4 step of
+ * AxionStringsLevel.cpp:850-921 is launched).  on = 1 (the default): each
+ * RK4 step is two launches, stages 1+2 and stages 3+4, which form the second
+ * stage's psi at every stencil point from the first stage's inputs
+ * (psi_n + (dt/2) Pi_n, psi_n + dt Pi_2) and so never store y_1 or y_3: 9 S
+ * instead of 19 S (FP32C) and 7 S instead of 17 S (FP64, FP32) of HBM
+ * traffic per cell, S = 4 words.  Steps run in twos through a second state
+ * buffer, and an odd last step of a call runs stage by stage; the bits are
+ * the same either way.  on = 0: one launch per stage. */
+int axs_level_set_stage_fusion(axs_level *lvl, int on);
+/* Tile variant of the fused-pair kernels (-1 = default, the 1024^3 sweep's
+ * fastest: 8-row tiles of 2 cells per lane with z chunk 32 in FP32, 4-row
+ * tiles of 2 cells per lane with z chunk 64 in FP64) and their z chunk
+ * (0 = default; on small grids halved until there are 4096 tiles);
+ * axs_num_pair_variants() variants. */
+int axs_num_pair_variants(void);
+int axs_level_set_pair_tile(axs_level *lvl, int variant, int z_chunk);
 
 /* ------------------- RHS and RK4 (the hot path) --------------------- */
 /* AxionStringsRHS<>::operator() over every cell of a periodic level
@@ -178,7 +195,8 @@ int axs_rhs_eval(axs_level *lvl, const axs_rhs_coeffs *c, void *rhs);
 int axs_rk4_step(axs_level *lvl, const axs_rhs_coeffs coeffs[4], double dt);
 /* nsteps RK4 steps from conformal time tau with the background's stage
  * scalars (host FP64), uploaded once as a table.  use_graph = 1 replays a
- * captured hipGraph of one step (4 stages + step-counter advance). */
+ * captured hipGraph of two steps (fused pairs) or one step (per-stage
+ * launches), each followed by the step-counter advance. */
 int axs_rk4_advance(axs_level *lvl, const axs_background *bg, double tau,
                     double dt, int nsteps, int use_graph);
 
@@ -194,23 +212,28 @@ int axs_rk4_advance_coeffs(axs_level *lvl, const axs_rhs_coeffs *coeffs,
 /* Placement trials; no counterpart in the reference (an MI355X-specific
  * set-up step).  The physical pages a level's arrays land on decide between
  * two HBM bandwidth regimes of the stage kernels (1024^3 FP32C: about 57 or
- * 65 ms per RK4 step for the same virtual layout).  Tries up to `trials`
+ * 65 ms per RK4 step with per-stage launches, 31.3 or 33.8 ms with fused
+ * pairs, for the same virtual layout).  Tries up to `trials`
  * placements (1..16) of the level's arrays, timing two RK4 steps on each.
  * While free memory holds a second copy of the arrays the fastest placement
  * seen is kept; otherwise each placement is freed before the next is drawn
  * and, from the third draw on, the first within 2% of the fastest seen is
  * kept (else the last).  Either way the trials stop as soon as the kept
- * placement streams the RK4 step's algorithmic bytes (304 B per cell
- * FP32C, 544 FP64, 272 FP32) at >= 69% of the 8 TB/s HBM peak, the fast
- * regime.  The state is reset to zero: call it before
+ * placement streams the RK4 step's algorithmic bytes at the fast regime's
+ * rate: per-stage launches 304 B per cell (FP32C), 544 (FP64), 272 (FP32)
+ * at >= 69% of the 8 TB/s HBM peak; fused pairs 144, 224, 112 B at >= 60%
+ * (FP64: 54%).  The state is reset to zero: call it before
  * uploading data.  ms_per_step (optional, `trials` entries) receives each
  * trial's time and *used (optional) the number of placements tried. */
 int axs_level_place(axs_level *lvl, int trials, double *ms_per_step, int *used);
 
 /* Per-stage kernel timing with HIP events recorded on the context stream
  * around each stage launch of axs_rk4_advance (forces direct launches while
- * on).  ms[s] accumulates the milliseconds of stage s+1, *nsteps the steps
- * timed; axs_level_stage_times resets the accumulators. */
+ * on).  ms[s] accumulates the milliseconds of stage s+1; with stage fusion
+ * on, ms[0] and ms[1] accumulate those of stages 1+2 and 3+4 (the fused
+ * launches, or the per-stage launches of an odd last step) and ms[2], ms[3]
+ * stay 0.  *nsteps counts the steps timed; axs_level_stage_times resets the
+ * accumulators. */
 int axs_level_enable_timing(axs_level *lvl, int on);
 int axs_level_stage_times(axs_level *lvl, double ms[4], int64_t *nsteps);
 

@@ -1,0 +1,114 @@
+"""Stage fusion: the two fused launches per RK4 step (stages 1+2 and 3+4,
+the second stage's psi formed at every stencil point from the first stage's
+inputs) give the same bits as one launch per stage -- every precision, pair
+tile variant and z chunk, ragged shapes, odd and even step counts, direct
+launches and graph replays, stage scalars that change between steps and
+stages -- and FP64 matches the oracle."""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _capi():
+    from axion_strings_adaptive_amd import capi
+    return capi
+
+
+def random_state(shape, seed=0):
+    nx, ny, nz = shape
+    rng = np.random.default_rng(seed)
+    return rng.normal(size=(4, nz, ny, nx))
+
+
+def _bg(capi):
+    # c0 = 0 -> c1 = 2 at tau = 1.15: lambda and the curvature coefficient
+    # change between steps and across the stages of a step
+    return capi.make_background(2.0, c0=0.0, c1=2.0, tau_switch=1.15,
+                                has_switch=True)
+
+
+def _run(ctx, shape, precision, s0, nsteps, graph, fused, variant=-1, zc=0):
+    capi = _capi()
+    lvl = capi.Level(ctx, shape, 0.5, precision)
+    lvl.set_fusion(fused)
+    lvl.set_pair_tile(variant, zc)
+    lvl.upload(s0)
+    lvl.advance(_bg(capi), 1.0, 0.1, nsteps, use_graph=graph)
+    hi, lo = lvl.download_raw()
+    lvl.close()
+    return hi, lo
+
+
+def _same(a, b):
+    assert np.array_equal(a[0], b[0])
+    assert (a[1] is None and b[1] is None) or np.array_equal(a[1], b[1])
+
+
+@pytest.mark.parametrize("shape", [(64, 64, 64), (40, 24, 36), (4, 8, 12),
+                                   (128, 16, 8), (36, 8, 8), (130, 10, 7)])
+@pytest.mark.parametrize("precision", [0, 1, 2])
+def test_fused_equals_per_stage_shapes(ctx, shape, precision):
+    s0 = random_state(shape, seed=sum(shape) + precision)
+    for nsteps in (1, 2, 5):
+        ref = _run(ctx, shape, precision, s0, nsteps, False, False)
+        for graph in (False, True):
+            got = _run(ctx, shape, precision, s0, nsteps, graph, True)
+            _same(got, ref)
+
+
+@pytest.mark.parametrize("variant", list(range(10)))
+@pytest.mark.parametrize("zc", [0, 4, 16])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_pair_variants_identical(ctx, variant, zc, precision):
+    shape = (256, 24, 20)  # several x tiles at every cells-per-lane width
+    s0 = random_state(shape, seed=5)
+    ref = _run(ctx, shape, precision, s0, 4, False, False)
+    got = _run(ctx, shape, precision, s0, 4, True, True, variant, zc)
+    _same(got, ref)
+
+
+def test_fused_fp64_matches_oracle(ctx, oracle):
+    """7 steps (three fused pairs and a per-stage last step) across the
+    fat -> Moore switch, graph replay: bit-exact with the oracle."""
+    capi = _capi()
+    n, dx, dt = 32, 0.5, 0.1
+    s0 = oracle.seeded_ic(n, seed=3)
+    ob = oracle.bg(2.0, c0=0.0, c1=2.0, tau_switch=1.15, has_switch=True)
+    want = s0.copy()
+    oracle.advance(want, dx, ob, 1.0, dt, 7)
+    lvl = capi.Level(ctx, n, dx, capi.AXS_FP64)
+    lvl.upload(s0)
+    lvl.advance(_bg(capi), 1.0, dt, 7, use_graph=True)
+    assert np.array_equal(lvl.download(), want)
+
+
+def test_fused_timing_reports_pairs(ctx):
+    capi = _capi()
+    lvl = capi.Level(ctx, 64, 0.5, capi.AXS_FP32C)
+    lvl.upload(random_state((64, 64, 64), seed=2))
+    lvl.enable_timing(True)
+    lvl.stage_times()
+    for k in (4, 3):  # an odd last step runs per stage, into the same slots
+        lvl.advance(_bg(capi), 1.0, 0.1, k)
+        ms, nsteps = lvl.stage_times()
+        assert nsteps == k
+        assert ms[0] > 0 and ms[1] > 0 and ms[2] == 0 and ms[3] == 0
+    lvl.set_fusion(False)
+    lvl.advance(_bg(capi), 1.0, 0.1, 2)
+    ms, nsteps = lvl.stage_times()
+    assert nsteps == 2 and min(ms) > 0
+    lvl.close()
+
+
+def test_fusion_setters_reject_bad_arguments(ctx):
+    capi = _capi()
+    lvl = capi.Level(ctx, 16, 0.5, capi.AXS_FP64)
+    with pytest.raises(capi.AxsError):
+        capi.check(capi.load().axs_level_set_stage_fusion(lvl.h, 2))
+    with pytest.raises(capi.AxsError):
+        lvl.set_pair_tile(capi.load().axs_num_pair_variants(), 0)
+    with pytest.raises(capi.AxsError):
+        lvl.set_pair_tile(0, -1)
+    lvl.close()
