@@ -1083,10 +1083,10 @@ int axs_level_place(axs_level *L, int trials, double *ms_per_step, int *used)
     const double step_bytes =
         L->fused ? (L->precision == AXS_FP32C ? 9.0 : 7.0) * S
                  : (L->precision == AXS_FP32C ? 19.0 : 17.0) * S;
-    // the fast regime's floor (1024^3: FP32C fused pairs 31.3 ms against
-    // 33.8 ms per step, FP64 54.7 ms)
+    // the fast regime's floor (1024^3 FP32C: per-stage 57 ms against 65 ms
+    // per step; fused pairs 30.1 ms against 32.0-33.8 ms; FP64 fused 53.8-55.5)
     const double fast_frac =
-        L->fused ? (L->precision == AXS_FP64 ? 0.54 : 0.60) : 0.69;
+        L->fused ? (L->precision == AXS_FP64 ? 0.54 : 0.63) : 0.69;
     const double fast_ms =
         step_bytes * (double)L->g.ncell / (fast_frac * 8.0e12) * 1e3;
     bool sequential = false;

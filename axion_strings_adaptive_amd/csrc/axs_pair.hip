@@ -471,16 +471,17 @@ __global__ void __launch_bounds__(64 * TY) k_pair(PairParams<T> p)
 }
 
 // ------------------------------------------------------------ launchers --
-// Pair tile variants: (TY, VX).  variant = base + kPairBase * g, g = 0
-// plain, 1 centre-load prefetch (pair 2).
+// Pair tile variants: (TY, VX).  variant = base + kPairBase * g, g bit 0:
+// centre-load prefetch (pair 2), bit 1: non-temporal stores of the outputs.
 struct PairShape
 {
     int ty, vx;
 };
-static const PairShape kPairTiles[] = {{4, 1}, {8, 1}, {4, 2}, {8, 2}, {4, 4}};
-constexpr int kPairBase = 5;
+static const PairShape kPairTiles[] = {{4, 1}, {8, 1}, {4, 2},
+                                       {8, 2}, {4, 4}, {16, 1}};
+constexpr int kPairBase = 6;
 
-int num_pair_variants() { return 2 * kPairBase; }
+int num_pair_variants() { return 4 * kPairBase; }
 
 template <typename T, int PREC, int PAIR, int TY, int VX>
 static void launch_pt(const PairParams<T> &p, int ntiles, hipStream_t s,
@@ -516,11 +517,19 @@ static void launch_pshape(int base, const PairParams<T> &p, int ntiles,
         else
             launch_pt<T, PREC, PAIR, 8, 2>(p, ntiles, s, pf);
         break;
-    default:
+    case 4:
         if constexpr (sizeof(T) == 8)
             launch_pt<T, PREC, PAIR, 4, 2>(p, ntiles, s, pf);
         else
             launch_pt<T, PREC, PAIR, 4, 4>(p, ntiles, s, pf);
+        break;
+    default:
+        // 16-row tiles: a quarter of the rows are y-halo reads, not half
+        // (FP64 would spill at 1024 threads: 8 rows)
+        if constexpr (sizeof(T) == 8)
+            launch_pt<T, PREC, PAIR, 8, 1>(p, ntiles, s, pf);
+        else
+            launch_pt<T, PREC, PAIR, 16, 1>(p, ntiles, s, pf);
         break;
     }
 }
@@ -538,10 +547,16 @@ int launch_rk_pair(axs_level *L, int pair, const void *yin, const void *yn,
     const bool dflt =
         !(L->pair_variant >= 0 && L->pair_variant < num_pair_variants());
     const int variant = dflt ? (f64 ? 2 : 3) : L->pair_variant;
-    const bool pf = variant >= kPairBase;
+    const int group = variant / kPairBase;
+    const bool pf = (group & 1) != 0;
     int base = variant % kPairBase;
+    // FP64 tiles that would spill run smaller: 8 x 2 and 4 x 4 as 4 x 2,
+    // 16 x 1 as 8 x 1 (the same mapping as launch_pshape, so that the grid
+    // below is sized for the kernel that runs)
     if (f64 && (base == 3 || base == 4))
-        base = 2; // FP64 8 x 2 and 4 x 4 tiles spill: 4 x 2
+        base = 2;
+    if (f64 && base == 5)
+        base = 1;
     if (nx % kPairTiles[base].vx != 0)
         base = 1;
     const int TY = kPairTiles[base].ty, VX = kPairTiles[base].vx;
@@ -586,7 +601,7 @@ int launch_rk_pair(axs_level *L, int pair, const void *yin, const void *yn,
         p.tiles_y = tiles_y;
         p.tiles_z = tiles_z;
         p.xcd_remap = (ntiles % 8 == 0) ? 1 : 0;
-        p.nontemporal = 0;
+        p.nontemporal = (group & 2) ? 1 : 0;
     };
     if (f64)
     {

@@ -212,7 +212,7 @@ int axs_rk4_advance_coeffs(axs_level *lvl, const axs_rhs_coeffs *coeffs,
 /* Placement trials; no counterpart in the reference (an MI355X-specific
  * set-up step).  The physical pages a level's arrays land on decide between
  * two HBM bandwidth regimes of the stage kernels (1024^3 FP32C: about 57 or
- * 65 ms per RK4 step with per-stage launches, 31.3 or 33.8 ms with fused
+ * 65 ms per RK4 step with per-stage launches, 30.1 to 33.8 ms with fused
  * pairs, for the same virtual layout).  Tries up to `trials`
  * placements (1..16) of the level's arrays, timing two RK4 steps on each.
  * While free memory holds a second copy of the arrays the fastest placement
@@ -221,7 +221,7 @@ int axs_rk4_advance_coeffs(axs_level *lvl, const axs_rhs_coeffs *coeffs,
  * kept (else the last).  Either way the trials stop as soon as the kept
  * placement streams the RK4 step's algorithmic bytes at the fast regime's
  * rate: per-stage launches 304 B per cell (FP32C), 544 (FP64), 272 (FP32)
- * at >= 69% of the 8 TB/s HBM peak; fused pairs 144, 224, 112 B at >= 60%
+ * at >= 69% of the 8 TB/s HBM peak; fused pairs 144, 224, 112 B at >= 63%
  * (FP64: 54%).  The state is reset to zero: call it before
  * uploading data.  ms_per_step (optional, `trials` entries) receives each
  * trial's time and *used (optional) the number of placements tried. */

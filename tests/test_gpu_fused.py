@@ -58,7 +58,7 @@ def test_fused_equals_per_stage_shapes(ctx, shape, precision):
             _same(got, ref)
 
 
-@pytest.mark.parametrize("variant", list(range(10)))
+@pytest.mark.parametrize("variant", list(range(24)))
 @pytest.mark.parametrize("zc", [0, 4, 16])
 @pytest.mark.parametrize("precision", [0, 1])
 def test_pair_variants_identical(ctx, variant, zc, precision):
