@@ -327,10 +327,11 @@ class Level:
     def set_tile(self, variant: int, z_chunk: int = 0):
         check(self.lib.axs_level_set_tile(self.h, variant, z_chunk))
 
-    def set_fusion(self, on: bool):
-        """RK4 stages in two fused launches (True, the default) or one
-        launch per stage; the same bits either way."""
-        check(self.lib.axs_level_set_stage_fusion(self.h, int(bool(on))))
+    def set_fusion(self, on):
+        """RK4 stages in two fused launches (1 / True, the default), the same
+        without a stored accumulator (2), or one launch per stage (0 /
+        False); the same bits every way."""
+        check(self.lib.axs_level_set_stage_fusion(self.h, int(on)))
 
     def set_pair_tile(self, variant: int, z_chunk: int = 0):
         check(self.lib.axs_level_set_pair_tile(self.h, variant, z_chunk))

@@ -418,7 +418,9 @@ int axs_level_create(axs_ctx *ctx, int nx, int ny, int nz, double dx,
     L->word = precision == AXS_FP64 ? 8 : 4;
     L->tile_variant = -1;
     L->z_chunk = 0;
-    L->fused = 1;
+    // the 1024^3 sweep's fastest: FP32 / FP32C pairs without a stored
+    // accumulator, FP64 pairs with one
+    L->fused = precision == AXS_FP64 ? 1 : 2;
     L->pair_variant = -1;
     L->pair_z_chunk = 0;
     bool ok = alloc_arrays(L);
@@ -494,7 +496,8 @@ int axs_num_pair_variants(void) { return axs::num_pair_variants(); }
 int axs_level_set_stage_fusion(axs_level *L, int on)
 {
     AXS_CHECK_ARG(L, "axs_level_set_stage_fusion: NULL level");
-    AXS_CHECK_ARG(on == 0 || on == 1, "axs_level_set_stage_fusion: on is 0 or 1");
+    AXS_CHECK_ARG(on >= 0 && on <= 2,
+                  "axs_level_set_stage_fusion: on is 0, 1 or 2");
     if (on != L->fused)
         destroy_graph(L);
     L->fused = on;
@@ -778,12 +781,14 @@ static int run_step_fused(axs_level *L, const int *step_ptr, double dt,
     int rc;
     if (L->timing)
         AXS_HIP_TRY(hipEventRecord(L->ev[0], s));
-    if ((rc = launch_rk_pair(L, 1, from, from, L->yb, L->d_coeff, step_ptr,
+    const int p1 = L->fused == 2 ? 3 : 1; // 2: accumulator-free pairs
+    if ((rc = launch_rk_pair(L, p1, from, from, L->yb, L->d_coeff, step_ptr,
                              dt)))
         return rc;
     if (L->timing)
         AXS_HIP_TRY(hipEventRecord(L->ev[1], s));
-    if ((rc = launch_rk_pair(L, 2, L->yb, from, to, L->d_coeff, step_ptr, dt)))
+    if ((rc = launch_rk_pair(L, p1 + 1, L->yb, from, to, L->d_coeff, step_ptr,
+                             dt)))
         return rc;
     if (!L->timing)
         return AXS_OK;
@@ -1080,13 +1085,16 @@ int axs_level_place(axs_level *L, int trials, double *ms_per_step, int *used)
     // the RK4 step's algorithmic bytes per cell: per-stage launches
     // 19 / 17 / 17 S, fused pairs 9 / 7 / 7 S (S = 4 words)
     const double S = 4.0 * (double)L->word;
+    const double fp32c = L->precision == AXS_FP32C ? 2.0 : 0.0;
     const double step_bytes =
-        L->fused ? (L->precision == AXS_FP32C ? 9.0 : 7.0) * S
-                 : (L->precision == AXS_FP32C ? 19.0 : 17.0) * S;
+        (L->fused == 2 ? 5.0 : L->fused == 1 ? 7.0 : 17.0) * S + fp32c * S;
     // the fast regime's floor (1024^3 FP32C: per-stage 57 ms against 65 ms
-    // per step; fused pairs 30.1 ms against 32.0-33.8 ms; FP64 fused 53.8-55.5)
-    const double fast_frac =
-        L->fused ? (L->precision == AXS_FP64 ? 0.54 : 0.63) : 0.69;
+    // per step; fused pairs 30.1 ms against 32.0-33.8 ms; pairs without an
+    // accumulator 27.9 ms against 29-31 ms; FP64 fused 53.8-55.5 ms)
+    const double fast_frac = L->fused == 0               ? 0.69
+                             : L->precision == AXS_FP64 ? 0.54
+                             : L->fused == 2            ? 0.53
+                                                        : 0.63;
     const double fast_ms =
         step_bytes * (double)L->g.ncell / (fast_frac * 8.0e12) * 1e3;
     bool sequential = false;

@@ -92,7 +92,8 @@ struct axs_level
     int *d_step;        // device step counter (graph replays)
     int tile_variant;   // stage-kernel tile variant
     int z_chunk;        // z planes per block
-    int fused;          // RK4 stages in two fused pairs (else one per stage)
+    int fused;          // RK4 stages: 0 per stage, 1 fused pairs, 2 pairs
+                        // without a stored accumulator
     int pair_variant;   // fused-pair kernel tile variant (-1 = default)
     int pair_z_chunk;   // z planes per block of the pair kernels (0 = auto)
     // captured one-step graph

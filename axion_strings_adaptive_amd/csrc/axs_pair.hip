@@ -20,6 +20,13 @@
 //
 // Pair 2 reads psi_n on the star while other workgroups store y_{n+1}, so its
 // output must not be y_n's buffer: callers alternate two state buffers.
+//
+// Modes 3 and 4 (accumulator-free pairs): mode 3 is pair 1 without the
+// accumulator store; mode 4 is pair 2 reading Pi_n on the star as well and
+// recomputing acc = k1 + 2 k2 at the centre (k1 from the star of psi_n,
+// k2 from that of psi_1 = psi_n + (dt/2) Pi_n, formed at every stencil point)
+// instead of loading it: 2 S + 3 S per step (FP64, FP32; FP32C + 2 S), with
+// the same expressions and so the same bits.
 #include "axs_internal.hpp"
 #include "axs_stage_math.hpp"
 #include "axs_vec.hpp"
@@ -38,8 +45,9 @@ template <typename T> struct PairParams
     float *comp;    // FP32C low part of y_n (pair 2; centre only, in place)
     const double *coeff; // stage-scalar table [step][4][3]
     const int *step;     // device step index or nullptr
-    int coeff_stage0;    // the first stage's row: 0 (pair 1) or 2 (pair 2)
-    double a0, a1;       // stage axpy factors: dt/2, dt/2 (pair 1); dt (pair 2)
+    int coeff_stage0;    // the first stage's row: 0 (pairs 1, 3) or 2 (2, 4)
+    double a0;           // first stage's axpy factor: dt/2 (pairs 1, 3), dt (2, 4)
+    double a1;           // dt/2 (stages 1 and 2)
     double h6;           // dt/6
     float h6_lo;         // dt/6 - (float)(dt/6)
     int nx, ny, nz;
@@ -61,8 +69,11 @@ __global__ void __launch_bounds__(64 * TY) k_pair(PairParams<T> p)
     constexpr int TX = 64 * VX;
     constexpr int HX = TX + 4;
     constexpr int HY = TY + 4;
-    constexpr int NB = PAIR == 2 ? 6 : 4; // words loaded per stencil point
-    __shared__ T sm[2][4][HY][HX];         // [buffer][A1 A2 B1 B2][y][x]
+    // words loaded per stencil point and star fields per plane
+    constexpr int NB = PAIR == 2 ? 6 : (PAIR == 4 ? 8 : 4);
+    constexpr int NF = PAIR == 4 ? 8 : 4;
+    constexpr bool FIRST = PAIR == 1 || PAIR == 3; // stages 1 + 2
+    __shared__ T sm[2][NF][HY][HX]; // [buffer][A1 A2 B1 B2 (C1 C2 D1 D2)][y][x]
 
     int bid = blockIdx.x;
     if (p.xcd_remap)
@@ -114,33 +125,41 @@ __global__ void __launch_bounds__(64 * TY) k_pair(PairParams<T> p)
     const int z_end = min(z_begin + p.z_chunk, nz);
 
     // the words of one stencil point: psi1, psi2, Pi1, Pi2 of the first
-    // stage's input, then (pair 2) psi_n 1, 2
-    const T *src[6] = {p.yin,         p.yin + n, p.yin + 2 * n, p.yin + 3 * n,
-                       p.yn,          p.yn + n};
+    // stage's input, then (pairs 2, 4) psi_n 1, 2, then (pair 4) Pi_n 1, 2
+    const T *src[8] = {p.yin,         p.yin + n,     p.yin + 2 * n,
+                       p.yin + 3 * n, p.yn,          p.yn + n,
+                       p.yn + 2 * n,  p.yn + 3 * n};
     auto zoff = [&](int z) -> uint32_t {
         z = wrap_lo(wrap_hi(z, nz), nz);
         return (uint32_t)z * plane;
     };
     // B (the second stage's psi) at a point from its loaded words
     auto form_b = [&](const T *w, int c) -> T {
-        const T base = PAIR == 1 ? w[c] : w[4 + c];
+        const T base = FIRST ? w[c] : w[4 + c];
         return stage_axpy<T, PREC>(base, p.a0, w[2 + c]);
+    };
+    // D (pair 4: stage 2's psi = psi_n + (dt/2) Pi_n) at a point
+    auto form_d = [&](const T *w, int c) -> T {
+        return stage_axpy<T, PREC>(w[4 + c], p.a1, w[6 + c]);
     };
 
     T qa1[5][VX], qa2[5][VX], qb1[5][VX], qb2[5][VX];
     T qp1[3][VX], qp2[3][VX], qn1[3][VX], qn2[3][VX];
+    // pair 4: C = psi_n and D = psi_1 at z-2..z+2, Pi_n at z..z+2
+    T qc1[5][VX], qc2[5][VX], qd1[5][VX], qd2[5][VX];
+    T qm1[3][VX], qm2[3][VX];
 #pragma unroll
     for (int i = 0; i < 5; ++i)
     {
         const uint32_t o = zoff(z_begin - 2 + i) + rowc;
-        T w[6][VX];
+        T w[8][VX];
 #pragma unroll
         for (int c = 0; c < NB; ++c)
             vload<T, VX>(src[c] + o, w[c]);
 #pragma unroll
         for (int v = 0; v < VX; ++v)
         {
-            T pt[6];
+            T pt[8];
 #pragma unroll
             for (int c = 0; c < NB; ++c)
                 pt[c] = w[c][v];
@@ -148,10 +167,30 @@ __global__ void __launch_bounds__(64 * TY) k_pair(PairParams<T> p)
             qa2[i][v] = pt[1];
             qb1[i][v] = form_b(pt, 0);
             qb2[i][v] = form_b(pt, 1);
+            if constexpr (PAIR == 4)
+            {
+                qc1[i][v] = pt[4];
+                qc2[i][v] = pt[5];
+                qd1[i][v] = form_d(pt, 0);
+                qd2[i][v] = form_d(pt, 1);
+            }
+            else
+            {
+                qc1[i][v] = qc2[i][v] = qd1[i][v] = qd2[i][v] = 0;
+            }
             if (i >= 2)
             {
                 qp1[i - 2][v] = pt[2];
                 qp2[i - 2][v] = pt[3];
+                if constexpr (PAIR == 4)
+                {
+                    qm1[i - 2][v] = pt[6];
+                    qm2[i - 2][v] = pt[7];
+                }
+                else
+                {
+                    qm1[i - 2][v] = qm2[i - 2][v] = 0;
+                }
                 if constexpr (PAIR == 2)
                 {
                     qn1[i - 2][v] = pt[4];
@@ -166,7 +205,7 @@ __global__ void __launch_bounds__(64 * TY) k_pair(PairParams<T> p)
         }
     }
     // halo words of the current plane
-    T hx[6][HW], hy[6][VX];
+    T hx[8][HW], hy[8][VX];
     auto load_halo = [&](int z) {
         const uint32_t o = zoff(z);
         if (hxl || hxr)
@@ -183,7 +222,7 @@ __global__ void __launch_bounds__(64 * TY) k_pair(PairParams<T> p)
         }
     };
 #pragma unroll
-    for (int c = 0; c < 6; ++c)
+    for (int c = 0; c < 8; ++c)
     {
 #pragma unroll
         for (int v = 0; v < HW; ++v)
@@ -194,10 +233,13 @@ __global__ void __launch_bounds__(64 * TY) k_pair(PairParams<T> p)
     }
     load_halo(z_begin);
 
-    const double *cf0 =
-        p.coeff + 12 * (p.step ? *p.step : 0) + 3 * p.coeff_stage0;
+    const double *cfs = p.coeff + 12 * (p.step ? *p.step : 0);
+    const double *cf0 = cfs + 3 * p.coeff_stage0;
     const double cc0 = cf0[0], lam0 = cf0[1], R20 = cf0[2];
     const double cc1 = cf0[3], lam1 = cf0[4], R21 = cf0[5];
+    // pair 4: stages 1 and 2 as well
+    const double ccs1 = cfs[0], lams1 = cfs[1], R2s1 = cfs[2];
+    const double ccs2 = cfs[3], lams2 = cfs[4], R2s2 = cfs[5];
     const T i2 = (T)p.inv_dx2;
 
     // pair 2 centre values: Pi_n, acc, lo (and with PF those of plane z+1)
@@ -215,6 +257,15 @@ __global__ void __launch_bounds__(64 * TY) k_pair(PairParams<T> p)
         }
     auto load_centre = [&](int zz, T(&pn_)[2][VX], T(&ac_)[4][VX],
                            float(&lo_)[4][VX]) {
+        if constexpr (PAIR == 4 && PREC == AXS_FP32C)
+        {
+            if (!active)
+                return;
+            const uint32_t o = (uint32_t)zz * plane + rowc;
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                vload<float, VX>(p.comp + c * n + o, lo_[c]);
+        }
         if constexpr (PAIR == 2)
         {
             if (!active)
@@ -248,6 +299,13 @@ __global__ void __launch_bounds__(64 * TY) k_pair(PairParams<T> p)
             sm[b][1][ty + 2][cx0 + v] = qa2[2][v];
             sm[b][2][ty + 2][cx0 + v] = qb1[2][v];
             sm[b][3][ty + 2][cx0 + v] = qb2[2][v];
+            if constexpr (PAIR == 4)
+            {
+                sm[b][4 % NF][ty + 2][cx0 + v] = qc1[2][v];
+                sm[b][5 % NF][ty + 2][cx0 + v] = qc2[2][v];
+                sm[b][6 % NF][ty + 2][cx0 + v] = qd1[2][v];
+                sm[b][7 % NF][ty + 2][cx0 + v] = qd2[2][v];
+            }
         }
         if (hxl || hxr)
         {
@@ -261,7 +319,7 @@ __global__ void __launch_bounds__(64 * TY) k_pair(PairParams<T> p)
 #pragma unroll
             for (int v = 0; v < HW; ++v)
             {
-                T pt[6];
+                T pt[8];
 #pragma unroll
                 for (int c = 0; c < NB; ++c)
                     pt[c] = hx[c][v];
@@ -269,6 +327,13 @@ __global__ void __launch_bounds__(64 * TY) k_pair(PairParams<T> p)
                 sm[b][1][ty + 2][col0 + v] = pt[1];
                 sm[b][2][ty + 2][col0 + v] = form_b(pt, 0);
                 sm[b][3][ty + 2][col0 + v] = form_b(pt, 1);
+                if constexpr (PAIR == 4)
+                {
+                    sm[b][4 % NF][ty + 2][col0 + v] = pt[4];
+                    sm[b][5 % NF][ty + 2][col0 + v] = pt[5];
+                    sm[b][6 % NF][ty + 2][col0 + v] = form_d(pt, 0);
+                    sm[b][7 % NF][ty + 2][col0 + v] = form_d(pt, 1);
+                }
             }
         }
         if (hyl || hyr)
@@ -277,7 +342,7 @@ __global__ void __launch_bounds__(64 * TY) k_pair(PairParams<T> p)
 #pragma unroll
             for (int v = 0; v < VX; ++v)
             {
-                T pt[6];
+                T pt[8];
 #pragma unroll
                 for (int c = 0; c < NB; ++c)
                     pt[c] = hy[c][v];
@@ -285,13 +350,20 @@ __global__ void __launch_bounds__(64 * TY) k_pair(PairParams<T> p)
                 sm[b][1][row][cx0 + v] = pt[1];
                 sm[b][2][row][cx0 + v] = form_b(pt, 0);
                 sm[b][3][row][cx0 + v] = form_b(pt, 1);
+                if constexpr (PAIR == 4)
+                {
+                    sm[b][4 % NF][row][cx0 + v] = pt[4];
+                    sm[b][5 % NF][row][cx0 + v] = pt[5];
+                    sm[b][6 % NF][row][cx0 + v] = form_d(pt, 0);
+                    sm[b][7 % NF][row][cx0 + v] = form_d(pt, 1);
+                }
             }
         }
 
         // ---- prefetch: queue heads at z+3, halos of plane z+1 ----
-        T nw[6][VX];
+        T nw[8][VX];
 #pragma unroll
-        for (int c = 0; c < 6; ++c)
+        for (int c = 0; c < 8; ++c)
 #pragma unroll
             for (int v = 0; v < VX; ++v)
                 nw[c][v] = 0;
@@ -358,7 +430,7 @@ __global__ void __launch_bounds__(64 * TY) k_pair(PairParams<T> p)
                     lov[c] = lo[c][v];
                     acv[c] = ac[c][v];
                 }
-                if constexpr (PAIR == 1)
+                if constexpr (FIRST)
                 {
                     // stage 1 on y_n, stage 2 on y_1 (y_n is the input)
                     ynv[0] = qa1[2][v];
@@ -383,11 +455,56 @@ __global__ void __launch_bounds__(64 * TY) k_pair(PairParams<T> p)
                 }
                 else
                 {
+                    if constexpr (PAIR == 4)
+                    {
+                        // acc = k1 + 2 k2 recomputed: stage 1 on y_n (star
+                        // C), stage 2 on y_1 (star D), as pair 1 forms it
+                        ynv[0] = qc1[2][v];
+                        ynv[1] = qc2[2][v];
+                        ynv[2] = qm1[0][v];
+                        ynv[3] = qm2[0][v];
+                        const S sC1{sm[b][4 % NF][cy][cx - 2], sm[b][4 % NF][cy][cx - 1],
+                                    sm[b][4 % NF][cy][cx + 1], sm[b][4 % NF][cy][cx + 2],
+                                    sm[b][4 % NF][cy - 2][cx], sm[b][4 % NF][cy - 1][cx],
+                                    sm[b][4 % NF][cy + 1][cx], sm[b][4 % NF][cy + 2][cx],
+                                    qc1[0][v], qc1[1][v], qc1[3][v], qc1[4][v], qc1[2][v]};
+                        const S sC2{sm[b][5 % NF][cy][cx - 2], sm[b][5 % NF][cy][cx - 1],
+                                    sm[b][5 % NF][cy][cx + 1], sm[b][5 % NF][cy][cx + 2],
+                                    sm[b][5 % NF][cy - 2][cx], sm[b][5 % NF][cy - 1][cx],
+                                    sm[b][5 % NF][cy + 1][cx], sm[b][5 % NF][cy + 2][cx],
+                                    qc2[0][v], qc2[1][v], qc2[3][v], qc2[4][v], qc2[2][v]};
+                        const S sD1{sm[b][6 % NF][cy][cx - 2], sm[b][6 % NF][cy][cx - 1],
+                                    sm[b][6 % NF][cy][cx + 1], sm[b][6 % NF][cy][cx + 2],
+                                    sm[b][6 % NF][cy - 2][cx], sm[b][6 % NF][cy - 1][cx],
+                                    sm[b][6 % NF][cy + 1][cx], sm[b][6 % NF][cy + 2][cx],
+                                    qd1[0][v], qd1[1][v], qd1[3][v], qd1[4][v], qd1[2][v]};
+                        const S sD2{sm[b][7 % NF][cy][cx - 2], sm[b][7 % NF][cy][cx - 1],
+                                    sm[b][7 % NF][cy][cx + 1], sm[b][7 % NF][cy][cx + 2],
+                                    sm[b][7 % NF][cy - 2][cx], sm[b][7 % NF][cy - 1][cx],
+                                    sm[b][7 % NF][cy + 1][cx], sm[b][7 % NF][cy + 2][cx],
+                                    qd2[0][v], qd2[1][v], qd2[3][v], qd2[4][v], qd2[2][v]};
+                        T FC1, FC2, FD1, FD2;
+                        force(star_lap(sC1, i2), star_lap(sC2, i2), qc1[2][v],
+                              qc2[2][v], ccs1, lams1, R2s1, FC1, FC2);
+                        force(star_lap(sD1, i2), star_lap(sD2, i2), qd1[2][v],
+                              qd2[2][v], ccs2, lams2, R2s2, FD1, FD2);
+                        const T k1[4] = {qm1[0][v], qm2[0][v], FC1, FC2};
+                        T acc1[4], y1[4];
+                        stage_update<T, PREC, MODE_S1>(k1, ynv, acv, lov, p.a1,
+                                                       p.h6, p.h6_lo, acc1, y1);
+                        const T k2[4] = {y1[2], y1[3], FD1, FD2};
+                        T y2u[4];
+                        stage_update<T, PREC, MODE_S23>(k2, ynv, acc1, lov, p.a1,
+                                                        p.h6, p.h6_lo, acv, y2u);
+                    }
+                    else
+                    {
+                        ynv[0] = qn1[0][v];
+                        ynv[1] = qn2[0][v];
+                        ynv[2] = pn[0][v];
+                        ynv[3] = pn[1][v];
+                    }
                     // stage 3 on y_2, stage 4 on y_3
-                    ynv[0] = qn1[0][v];
-                    ynv[1] = qn2[0][v];
-                    ynv[2] = pn[0][v];
-                    ynv[3] = pn[1][v];
                     const T k3[4] = {qp1[0][v], qp2[0][v], FA1, FA2};
                     T acc3[4], y3[4];
                     stage_update<T, PREC, MODE_S23>(k3, ynv, acv, lov, p.a0,
@@ -411,7 +528,7 @@ __global__ void __launch_bounds__(64 * TY) k_pair(PairParams<T> p)
                 vstore<T, VX>(p.yout + c * n + oc, out_y[c], nt);
                 if constexpr (PAIR == 1)
                     vstore<T, VX>(p.acc + c * n + oc, out_a[c], nt);
-                if constexpr (PAIR == 2 && PREC == AXS_FP32C)
+                if constexpr (!FIRST && PREC == AXS_FP32C)
                     vstore<float, VX>(p.comp + c * n + oc, out_lo[c], nt);
             }
         }
@@ -440,6 +557,10 @@ __global__ void __launch_bounds__(64 * TY) k_pair(PairParams<T> p)
                 qa2[i][v] = qa2[i + 1][v];
                 qb1[i][v] = qb1[i + 1][v];
                 qb2[i][v] = qb2[i + 1][v];
+                qc1[i][v] = qc1[i + 1][v];
+                qc2[i][v] = qc2[i + 1][v];
+                qd1[i][v] = qd1[i + 1][v];
+                qd2[i][v] = qd2[i + 1][v];
             }
 #pragma unroll
         for (int i = 0; i < 2; ++i)
@@ -450,13 +571,15 @@ __global__ void __launch_bounds__(64 * TY) k_pair(PairParams<T> p)
                 qp2[i][v] = qp2[i + 1][v];
                 qn1[i][v] = qn1[i + 1][v];
                 qn2[i][v] = qn2[i + 1][v];
+                qm1[i][v] = qm1[i + 1][v];
+                qm2[i][v] = qm2[i + 1][v];
             }
 #pragma unroll
         for (int v = 0; v < VX; ++v)
         {
-            T pt[6];
+            T pt[8];
 #pragma unroll
-            for (int c = 0; c < 6; ++c)
+            for (int c = 0; c < 8; ++c)
                 pt[c] = nw[c][v];
             qa1[4][v] = pt[0];
             qa2[4][v] = pt[1];
@@ -466,6 +589,15 @@ __global__ void __launch_bounds__(64 * TY) k_pair(PairParams<T> p)
             qp2[2][v] = pt[3];
             qn1[2][v] = pt[4];
             qn2[2][v] = pt[5];
+            if constexpr (PAIR == 4)
+            {
+                qc1[4][v] = pt[4];
+                qc2[4][v] = pt[5];
+                qd1[4][v] = form_d(pt, 0);
+                qd2[4][v] = form_d(pt, 1);
+                qm1[2][v] = pt[6];
+                qm2[2][v] = pt[7];
+            }
         }
     }
 }
@@ -499,8 +631,27 @@ template <typename T, int PREC, int PAIR>
 static void launch_pshape(int base, const PairParams<T> &p, int ntiles,
                           hipStream_t s, bool pf)
 {
-    switch (base)
+    if constexpr (PAIR >= 3)
     {
+        // the tiles launch_rk_pair maps these modes to (no spilling shapes)
+        if constexpr (PAIR == 3 && sizeof(T) == 4)
+        {
+            if (base == 3)
+            {
+                launch_pt<T, PREC, PAIR, 8, 2>(p, ntiles, s, pf);
+                return;
+            }
+        }
+        if (base == 2)
+            launch_pt<T, PREC, PAIR, 4, 2>(p, ntiles, s, pf);
+        else if (base == 1)
+            launch_pt<T, PREC, PAIR, 8, 1>(p, ntiles, s, pf);
+        else
+            launch_pt<T, PREC, PAIR, 4, 1>(p, ntiles, s, pf);
+    }
+    else
+        switch (base)
+        {
     case 0:
         launch_pt<T, PREC, PAIR, 4, 1>(p, ntiles, s, pf);
         break;
@@ -531,11 +682,33 @@ static void launch_pshape(int base, const PairParams<T> &p, int ntiles,
         else
             launch_pt<T, PREC, PAIR, 16, 1>(p, ntiles, s, pf);
         break;
+        }
+}
+
+template <typename T, int PREC>
+static void launch_pmode(int pair, int base, const PairParams<T> &p,
+                         int ntiles, hipStream_t s, bool pf)
+{
+    switch (pair)
+    {
+    case 1:
+        launch_pshape<T, PREC, 1>(base, p, ntiles, s, pf);
+        break;
+    case 2:
+        launch_pshape<T, PREC, 2>(base, p, ntiles, s, pf);
+        break;
+    case 3:
+        launch_pshape<T, PREC, 3>(base, p, ntiles, s, pf);
+        break;
+    default:
+        launch_pshape<T, PREC, 4>(base, p, ntiles, s, pf);
+        break;
     }
 }
 
 // Pair 1 (stages 1, 2): y_n = yin -> acc, y_2 = yout.  Pair 2 (stages 3, 4):
-// y_2 = yin, y_n = yn, acc, lo -> y_{n+1} = yout (yout != yn).
+// y_2 = yin, y_n = yn, acc, lo -> y_{n+1} = yout (yout != yn).  Pairs 3
+// and 4: the same without the accumulator (pair 4 recomputes it).
 int launch_rk_pair(axs_level *L, int pair, const void *yin, const void *yn,
                    void *yout, const double *coeff_table, const int *step_ptr,
                    double dt)
@@ -546,7 +719,9 @@ int launch_rk_pair(axs_level *L, int pair, const void *yin, const void *yn,
     // FP64: 4 x 2 tiles, z chunk 64)
     const bool dflt =
         !(L->pair_variant >= 0 && L->pair_variant < num_pair_variants());
-    const int variant = dflt ? (f64 ? 2 : 3) : L->pair_variant;
+    // (accumulator-free stages 3+4, more registers: FP32 8 x 1)
+    const int variant =
+        dflt ? ((pair == 4 && !f64) ? 1 : (f64 ? 2 : 3)) : L->pair_variant;
     const int group = variant / kPairBase;
     const bool pf = (group & 1) != 0;
     int base = variant % kPairBase;
@@ -556,6 +731,16 @@ int launch_rk_pair(axs_level *L, int pair, const void *yin, const void *yn,
     if (f64 && (base == 3 || base == 4))
         base = 2;
     if (f64 && base == 5)
+        base = 1;
+    // accumulator-free pairs: the stages 3+4 kernel holds four star fields,
+    // its 8 x 2, 4 x 4 and 16 x 1 tiles spill (and mode 2 in 4 x 4 tiles with
+    // prefetch gave wrong results in a GPU test, cause not found), so those
+    // run as 4 x 2 / 8 x 1, and the stages 1+2 kernel's 4 x 4 and 16 x 1 too
+    if (pair == 4 && base == 3)
+        base = 2;
+    if (pair >= 3 && base == 4)
+        base = 2;
+    if (pair >= 3 && base == 5)
         base = 1;
     if (nx % kPairTiles[base].vx != 0)
         base = 1;
@@ -586,8 +771,9 @@ int launch_rk_pair(axs_level *L, int pair, const void *yin, const void *yn,
         p.comp = L->comp;
         p.coeff = coeff_table;
         p.step = step_ptr;
-        p.coeff_stage0 = pair == 1 ? 0 : 2;
-        p.a0 = pair == 1 ? 0.5 * dt : dt;
+        const bool first = pair == 1 || pair == 3;
+        p.coeff_stage0 = first ? 0 : 2;
+        p.a0 = first ? 0.5 * dt : dt;
         p.a1 = 0.5 * dt;
         p.h6 = dt / 6.0;
         p.h6_lo = (float)(p.h6 - (double)(float)p.h6);
@@ -607,29 +793,16 @@ int launch_rk_pair(axs_level *L, int pair, const void *yin, const void *yn,
     {
         PairParams<double> p{};
         fill(p);
-        if (pair == 1)
-            launch_pshape<double, AXS_FP64, 1>(base, p, (int)ntiles, s, pf);
-        else
-            launch_pshape<double, AXS_FP64, 2>(base, p, (int)ntiles, s, pf);
+        launch_pmode<double, AXS_FP64>(pair, base, p, (int)ntiles, s, pf);
     }
     else
     {
         PairParams<float> p{};
         fill(p);
         if (L->precision == AXS_FP32C)
-        {
-            if (pair == 1)
-                launch_pshape<float, AXS_FP32C, 1>(base, p, (int)ntiles, s, pf);
-            else
-                launch_pshape<float, AXS_FP32C, 2>(base, p, (int)ntiles, s, pf);
-        }
+            launch_pmode<float, AXS_FP32C>(pair, base, p, (int)ntiles, s, pf);
         else
-        {
-            if (pair == 1)
-                launch_pshape<float, AXS_FP32, 1>(base, p, (int)ntiles, s, pf);
-            else
-                launch_pshape<float, AXS_FP32, 2>(base, p, (int)ntiles, s, pf);
-        }
+            launch_pmode<float, AXS_FP32>(pair, base, p, (int)ntiles, s, pf);
     }
     AXS_HIP_TRY(hipGetLastError());
     return AXS_OK;

@@ -45,6 +45,9 @@ STAGE_BYTES = {1: [48, 80, 80, 96], 0: [96, 160, 160, 128]}
 # ... of each fused-pair launch: pair 1 reads y_n, writes acc and y_2 (3S);
 # pair 2 reads y_2, y_n, acc, writes y_{n+1} (4S) + FP32C lo read/write (2S)
 PAIR_BYTES = {1: [48, 96], 0: [96, 128]}
+# ... without a stored accumulator (fusion mode 2): pair 1 reads y_n and
+# writes y_2 (2S); pair 2 reads y_2, y_n, writes y_{n+1} (3S) (+ lo 2S)
+PAIR_BYTES_NOACC = {1: [32, 80], 0: [64, 96]}
 # HBM bytes per cell of one FP32C stage-2/3 launch at 1024^3, from
 # rocprofv3 --pmc FETCH_SIZE (x 2.0) and WRITE_SIZE (x 1.0), the scales
 # measured on device copies and reads of known size; keyed by the tile
@@ -320,6 +323,10 @@ def main(argv=None):
                     help="variant,z_chunk (default: autotuned cache or built-in)")
     ap.add_argument("--no-fusion", action="store_true",
                     help="one launch per RK stage instead of two fused pairs")
+    ap.add_argument("--fusion", type=int, default=0, choices=(0, 1, 2),
+                    help="fused pairs (1) or fused pairs without a stored "
+                         "accumulator (2); 0 = the library's default per "
+                         "precision (FP32C 2, FP64 1)")
     ap.add_argument("--pair-tile", type=str, default="",
                     help="fused-pair variant,z_chunk (default: built-in)")
     ap.add_argument("--alloc-check", action="store_true",
@@ -375,7 +382,9 @@ def main(argv=None):
         from axion_strings_adaptive_amd import autotune
         tile = autotune.cached_choice(n, capi.AXS_FP32C)
 
-    fused = not args.no_fusion
+    # stage fusion mode per precision (0 = one launch per stage)
+    fused = 0 if args.no_fusion else (args.fusion or 2)
+    fused64 = 0 if args.no_fusion else (args.fusion or 1)
     pair_tile = (tuple(int(t) for t in args.pair_tile.split(","))
                  if args.pair_tile else None)
 
@@ -429,7 +438,7 @@ def main(argv=None):
         b, wall64, ms64, ns64, _, place64 = run_level(
             capi, ctx, n, capi.AXS_FP64, bg, dt, args.warmup, args.steps,
             autotune.cached_choice(n, capi.AXS_FP64) if not args.tile else tile,
-            args.place_trials, fused, pair_tile)
+            args.place_trials, fused64, pair_tile)
         if place64:
             log(f"[bench] FP64 placement trials (ms/step): "
                 f"{', '.join(f'{t:.2f}' for t in place64)}")
@@ -438,7 +447,7 @@ def main(argv=None):
         value64 = ncell * args.steps / wall64
         res64 = {
             "value": value64, "ms_per_step": 1e3 * wall64 / args.steps,
-            "stage_ms": [m / ns64 for m in ms64],
+            "stage_ms": [m / ns64 for m in ms64], "stage_fusion": fused64,
             "speedup_fp32c_over_fp64": value / value64,
             "max_dphi_over_eta": dpsi / d64["R"],
             "rel_energy_err": abs(d32["rho_tot_unscreened"] -
@@ -455,22 +464,23 @@ def main(argv=None):
         # roofline of the dominant kernel: the longer of the two pair
         # launches (one each per step)
         k = 0 if st[0] > st[1] else 1
-        achieved = PAIR_BYTES[1][k] * ncell / (st[k] * 1e-3) / 1e9
+        pb = PAIR_BYTES if fused == 1 else PAIR_BYTES_NOACC
+        achieved = pb[1][k] * ncell / (st[k] * 1e-3) / 1e9
         per_cell = (MEASURED_PAIR2_BYTES.get(tuple(pair_tile) if pair_tile
                                              else (-1, 0))
-                    if k == 1 else None)
+                    if k == 1 and fused == 1 else None)
         roofline = {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
             "traffic": per_cell * ncell if per_cell and n == 1024 else None,
             "traffic_unit": "bytes per launch",
-            "kernel": f"k_pair<float, FP32C, {k + 1}> (RK stages "
-                      f"{'1+2' if k == 0 else '3+4'})",
-            "algorithmic_bytes_per_cell": PAIR_BYTES[1][k],
+            "kernel": f"k_pair<float, FP32C, {k + 1 + 2 * (fused - 1)}> "
+                      f"(RK stages {'1+2' if k == 0 else '3+4'})",
+            "algorithmic_bytes_per_cell": pb[1][k],
             "pair_ms": st[:2],
-            "pair_gbs": [PAIR_BYTES[1][i] * ncell / (st[i] * 1e-3) / 1e9
+            "pair_gbs": [pb[1][i] * ncell / (st[i] * 1e-3) / 1e9
                          for i in range(2)],
-            "step_gbs": sum(PAIR_BYTES[1]) * ncell / (sum(st[:2]) * 1e-3) / 1e9}
+            "step_gbs": sum(pb[1]) * ncell / (sum(st[:2]) * 1e-3) / 1e9}
     else:
         # roofline of the dominant kernel: the stage-2/3 launch (two per step)
         bytes23 = STAGE_BYTES[1][1] * ncell

@@ -171,7 +171,11 @@ int axs_level_get_tile(axs_level *lvl, int *variant, int *z_chunk);
  * instead of 19 S (FP32C) and 7 S instead of 17 S (FP64, FP32) of HBM
  * traffic per cell, S = 4 words.  Steps run in twos through a second state
  * buffer, and an odd last step of a call runs stage by stage; the bits are
- * the same either way.  on = 0: one launch per stage. */
+ * the same either way.  on = 2: the pairs without a stored accumulator --
+ * the stages 3+4 launch reads Pi_n on the stencil too and recomputes
+ * acc = k1 + 2 k2 at each cell from the stars of psi_n and psi_1 =
+ * psi_n + (dt/2) Pi_n: 7 S (FP32C), 5 S (FP64, FP32) per cell, for twice the
+ * RHS work in that launch; same bits.  on = 0: one launch per stage. */
 int axs_level_set_stage_fusion(axs_level *lvl, int on);
 /* Tile variant of the fused-pair kernels (-1 = default, the 1024^3 sweep's
  * fastest: 8-row tiles of 2 cells per lane with z chunk 32 in FP32, 4-row
@@ -222,7 +226,7 @@ int axs_rk4_advance_coeffs(axs_level *lvl, const axs_rhs_coeffs *coeffs,
  * placement streams the RK4 step's algorithmic bytes at the fast regime's
  * rate: per-stage launches 304 B per cell (FP32C), 544 (FP64), 272 (FP32)
  * at >= 69% of the 8 TB/s HBM peak; fused pairs 144, 224, 112 B at >= 63%
- * (FP64: 54%).  The state is reset to zero: call it before
+ * (FP64: 54%); pairs without a stored accumulator 112, 160, 80 B at >= 53%.  The state is reset to zero: call it before
  * uploading data.  ms_per_step (optional, `trials` entries) receives each
  * trial's time and *used (optional) the number of placements tried. */
 int axs_level_place(axs_level *lvl, int trials, double *ms_per_step, int *used);

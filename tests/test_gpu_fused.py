@@ -52,20 +52,22 @@ def _same(a, b):
 def test_fused_equals_per_stage_shapes(ctx, shape, precision):
     s0 = random_state(shape, seed=sum(shape) + precision)
     for nsteps in (1, 2, 5):
-        ref = _run(ctx, shape, precision, s0, nsteps, False, False)
+        ref = _run(ctx, shape, precision, s0, nsteps, False, 0)
         for graph in (False, True):
-            got = _run(ctx, shape, precision, s0, nsteps, graph, True)
-            _same(got, ref)
+            for mode in (1, 2):
+                got = _run(ctx, shape, precision, s0, nsteps, graph, mode)
+                _same(got, ref)
 
 
 @pytest.mark.parametrize("variant", list(range(24)))
 @pytest.mark.parametrize("zc", [0, 4, 16])
 @pytest.mark.parametrize("precision", [0, 1])
-def test_pair_variants_identical(ctx, variant, zc, precision):
+@pytest.mark.parametrize("mode", [1, 2])
+def test_pair_variants_identical(ctx, variant, zc, precision, mode):
     shape = (256, 24, 20)  # several x tiles at every cells-per-lane width
     s0 = random_state(shape, seed=5)
-    ref = _run(ctx, shape, precision, s0, 4, False, False)
-    got = _run(ctx, shape, precision, s0, 4, True, True, variant, zc)
+    ref = _run(ctx, shape, precision, s0, 4, False, 0)
+    got = _run(ctx, shape, precision, s0, 4, True, mode, variant, zc)
     _same(got, ref)
 
 
@@ -79,6 +81,10 @@ def test_fused_fp64_matches_oracle(ctx, oracle):
     want = s0.copy()
     oracle.advance(want, dx, ob, 1.0, dt, 7)
     lvl = capi.Level(ctx, n, dx, capi.AXS_FP64)
+    lvl.upload(s0)
+    lvl.advance(_bg(capi), 1.0, dt, 7, use_graph=True)
+    assert np.array_equal(lvl.download(), want)
+    lvl.set_fusion(2)
     lvl.upload(s0)
     lvl.advance(_bg(capi), 1.0, dt, 7, use_graph=True)
     assert np.array_equal(lvl.download(), want)
@@ -106,7 +112,7 @@ def test_fusion_setters_reject_bad_arguments(ctx):
     capi = _capi()
     lvl = capi.Level(ctx, 16, 0.5, capi.AXS_FP64)
     with pytest.raises(capi.AxsError):
-        capi.check(capi.load().axs_level_set_stage_fusion(lvl.h, 2))
+        capi.check(capi.load().axs_level_set_stage_fusion(lvl.h, 3))
     with pytest.raises(capi.AxsError):
         lvl.set_pair_tile(capi.load().axs_num_pair_variants(), 0)
     with pytest.raises(capi.AxsError):
