@@ -719,9 +719,9 @@ int launch_rk_pair(axs_level *L, int pair, const void *yin, const void *yn,
     // FP64: 4 x 2 tiles, z chunk 64)
     const bool dflt =
         !(L->pair_variant >= 0 && L->pair_variant < num_pair_variants());
-    // (accumulator-free stages 3+4, more registers: FP32 8 x 1)
+    // (accumulator-free pairs: FP32 8 x 1 for both launches)
     const int variant =
-        dflt ? ((pair == 4 && !f64) ? 1 : (f64 ? 2 : 3)) : L->pair_variant;
+        dflt ? ((pair >= 3 && !f64) ? 1 : (f64 ? 2 : 3)) : L->pair_variant;
     const int group = variant / kPairBase;
     const bool pf = (group & 1) != 0;
     int base = variant % kPairBase;

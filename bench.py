@@ -53,8 +53,9 @@ PAIR_BYTES_NOACC = {1: [32, 80], 0: [64, 96]}
 # measured on device copies and reads of known size; keyed by the tile
 # (variant, z_chunk) the launch used.
 MEASURED_S23_BYTES = {(20, 32): 86.03}
-# the same for the fused FP32C pair-2 launch, keyed by (pair variant, z_chunk)
-MEASURED_PAIR2_BYTES = {(-1, 0): 117.40}
+# the same for the fused FP32C stages 3+4 launch, keyed by (pair variant,
+# z_chunk): pairs with a stored accumulator (mode 1) and without (mode 2)
+MEASURED_PAIR2_BYTES = {1: {(-1, 0): 117.40}, 2: {(-1, 0): 103.48}}
 
 
 def log(msg):
@@ -466,9 +467,8 @@ def main(argv=None):
         k = 0 if st[0] > st[1] else 1
         pb = PAIR_BYTES if fused == 1 else PAIR_BYTES_NOACC
         achieved = pb[1][k] * ncell / (st[k] * 1e-3) / 1e9
-        per_cell = (MEASURED_PAIR2_BYTES.get(tuple(pair_tile) if pair_tile
-                                             else (-1, 0))
-                    if k == 1 and fused == 1 else None)
+        per_cell = (MEASURED_PAIR2_BYTES[fused].get(
+            tuple(pair_tile) if pair_tile else (-1, 0)) if k == 1 else None)
         roofline = {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,

@@ -178,8 +178,9 @@ int axs_level_get_tile(axs_level *lvl, int *variant, int *z_chunk);
  * RHS work in that launch; same bits.  on = 0: one launch per stage. */
 int axs_level_set_stage_fusion(axs_level *lvl, int on);
 /* Tile variant of the fused-pair kernels (-1 = default, the 1024^3 sweep's
- * fastest: 8-row tiles of 2 cells per lane with z chunk 32 in FP32, 4-row
- * tiles of 2 cells per lane with z chunk 64 in FP64) and their z chunk
+ * fastest: 8-row tiles of 2 cells per lane with z chunk 32 in FP32 (of 1
+ * cell per lane for the pairs without an accumulator), 4-row tiles of 2
+ * cells per lane with z chunk 64 in FP64) and their z chunk
  * (0 = default; on small grids halved until there are 4096 tiles);
  * axs_num_pair_variants() variants. */
 int axs_num_pair_variants(void);
