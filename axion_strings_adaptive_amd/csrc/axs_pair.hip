@@ -57,6 +57,8 @@ template <typename T> struct PairParams
     int tiles_x, tiles_y, tiles_z;
     int xcd_remap;
     int nontemporal;
+    int y_major; // tile ids run along y first (tiles sharing y-halo rows
+                 // run together on one XCD)
 };
 
 // Each workgroup: a (64 VX) x TY column tile, streaming a chunk of z planes.
@@ -81,10 +83,21 @@ __global__ void __launch_bounds__(64 * TY) k_pair(PairParams<T> p)
         const int ntiles = p.tiles_x * p.tiles_y * p.tiles_z;
         bid = (bid & 7) * (ntiles >> 3) + (bid >> 3);
     }
-    const int bx = bid % p.tiles_x;
-    const int rem = bid / p.tiles_x;
-    const int by = rem % p.tiles_y;
-    const int bz = rem / p.tiles_y;
+    int bx, by, bz;
+    if (p.y_major)
+    {
+        by = bid % p.tiles_y;
+        const int rem = bid / p.tiles_y;
+        bx = rem % p.tiles_x;
+        bz = rem / p.tiles_x;
+    }
+    else
+    {
+        bx = bid % p.tiles_x;
+        const int rem = bid / p.tiles_x;
+        by = rem % p.tiles_y;
+        bz = rem / p.tiles_y;
+    }
 
     const int tx = threadIdx.x;
     const int ty = threadIdx.y;
@@ -604,7 +617,8 @@ __global__ void __launch_bounds__(64 * TY) k_pair(PairParams<T> p)
 
 // ------------------------------------------------------------ launchers --
 // Pair tile variants: (TY, VX).  variant = base + kPairBase * g, g bit 0:
-// centre-load prefetch (pair 2), bit 1: non-temporal stores of the outputs.
+// centre-load prefetch (pair 2), bit 1: non-temporal stores of the outputs,
+// bit 2: y-major tile order.
 struct PairShape
 {
     int ty, vx;
@@ -613,7 +627,7 @@ static const PairShape kPairTiles[] = {{4, 1}, {8, 1}, {4, 2},
                                        {8, 2}, {4, 4}, {16, 1}};
 constexpr int kPairBase = 6;
 
-int num_pair_variants() { return 4 * kPairBase; }
+int num_pair_variants() { return 8 * kPairBase; }
 
 template <typename T, int PREC, int PAIR, int TY, int VX>
 static void launch_pt(const PairParams<T> &p, int ntiles, hipStream_t s,
@@ -788,6 +802,7 @@ int launch_rk_pair(axs_level *L, int pair, const void *yin, const void *yn,
         p.tiles_z = tiles_z;
         p.xcd_remap = (ntiles % 8 == 0) ? 1 : 0;
         p.nontemporal = (group & 2) ? 1 : 0;
+        p.y_major = (group & 4) ? 1 : 0;
     };
     if (f64)
     {

@@ -59,7 +59,7 @@ def test_fused_equals_per_stage_shapes(ctx, shape, precision):
                 _same(got, ref)
 
 
-@pytest.mark.parametrize("variant", list(range(24)))
+@pytest.mark.parametrize("variant", list(range(48)))
 @pytest.mark.parametrize("zc", [0, 4, 16])
 @pytest.mark.parametrize("precision", [0, 1])
 @pytest.mark.parametrize("mode", [1, 2])
