@@ -28,6 +28,7 @@ HIP_SOURCES = [
     "axs_capi.hip",
     "axs_rk.hip",
     "axs_pair.hip",
+    "axs_step.hip",
     "axs_diag.hip",
     "axs_fft.hip",
     "axs_boxes.hip",

@@ -418,9 +418,9 @@ int axs_level_create(axs_ctx *ctx, int nx, int ny, int nz, double dx,
     L->word = precision == AXS_FP64 ? 8 : 4;
     L->tile_variant = -1;
     L->z_chunk = 0;
-    // the 1024^3 sweep's fastest: FP32 / FP32C pairs without a stored
-    // accumulator, FP64 pairs with one
-    L->fused = precision == AXS_FP64 ? 1 : 2;
+    // the 1024^3 sweep's fastest: FP32 / FP32C the whole step in one
+    // launch, FP64 fused pairs with a stored accumulator
+    L->fused = precision == AXS_FP64 ? 1 : 3;
     L->pair_variant = -1;
     L->pair_z_chunk = 0;
     bool ok = alloc_arrays(L);
@@ -496,8 +496,11 @@ int axs_num_pair_variants(void) { return axs::num_pair_variants(); }
 int axs_level_set_stage_fusion(axs_level *L, int on)
 {
     AXS_CHECK_ARG(L, "axs_level_set_stage_fusion: NULL level");
-    AXS_CHECK_ARG(on >= 0 && on <= 2,
-                  "axs_level_set_stage_fusion: on is 0, 1 or 2");
+    AXS_CHECK_ARG(on >= 0 && on <= 3,
+                  "axs_level_set_stage_fusion: on is 0, 1, 2 or 3");
+    AXS_CHECK_ARG(on != 3 || L->precision != AXS_FP64,
+                  "axs_level_set_stage_fusion: one launch per step (3) is "
+                  "FP32 / FP32C only");
     if (on != L->fused)
         destroy_graph(L);
     L->fused = on;
@@ -750,8 +753,9 @@ static int run_step_timed(axs_level *L, const int *step_ptr, double dt)
     {
         float ms = 0.0f;
         AXS_HIP_TRY(hipEventElapsedTime(&ms, L->ev[st], L->ev[st + 1]));
-        // with fusion on, entries 0 and 1 hold stages 1+2 and 3+4
-        L->stage_ms[L->fused ? st / 2 : st] += ms;
+        // with fused pairs, entries 0 and 1 hold stages 1+2 and 3+4; with
+        // one launch per step, entry 0 the whole step
+        L->stage_ms[L->fused == 3 ? 0 : (L->fused ? st / 2 : st)] += ms;
     }
     L->timed_steps += 1;
     return AXS_OK;
@@ -781,6 +785,21 @@ static int run_step_fused(axs_level *L, const int *step_ptr, double dt,
     int rc;
     if (L->timing)
         AXS_HIP_TRY(hipEventRecord(L->ev[0], s));
+    if (L->fused == 3)
+    {
+        // the whole step in one launch
+        if ((rc = launch_rk_step(L, from, to, L->d_coeff, step_ptr, dt)))
+            return rc;
+        if (!L->timing)
+            return AXS_OK;
+        AXS_HIP_TRY(hipEventRecord(L->ev[1], s));
+        AXS_HIP_TRY(hipEventSynchronize(L->ev[1]));
+        float ms = 0.0f;
+        AXS_HIP_TRY(hipEventElapsedTime(&ms, L->ev[0], L->ev[1]));
+        L->stage_ms[0] += ms;
+        L->timed_steps += 1;
+        return AXS_OK;
+    }
     const int p1 = L->fused == 2 ? 3 : 1; // 2: accumulator-free pairs
     if ((rc = launch_rk_pair(L, p1, from, from, L->yb, L->d_coeff, step_ptr,
                              dt)))
@@ -1087,12 +1106,19 @@ int axs_level_place(axs_level *L, int trials, double *ms_per_step, int *used)
     const double S = 4.0 * (double)L->word;
     const double fp32c = L->precision == AXS_FP32C ? 2.0 : 0.0;
     const double step_bytes =
-        (L->fused == 2 ? 5.0 : L->fused == 1 ? 7.0 : 17.0) * S + fp32c * S;
+        (L->fused == 3   ? 2.0
+         : L->fused == 2 ? 5.0
+         : L->fused == 1 ? 7.0
+                         : 17.0) *
+            S +
+        fp32c * S;
     // the fast regime's floor (1024^3 FP32C: per-stage 57 ms against 65 ms
     // per step; fused pairs 30.1 ms against 32.0-33.8 ms; pairs without an
-    // accumulator 27.9 ms against 29-31 ms; FP64 fused 53.8-55.5 ms)
+    // accumulator 27.9 ms against 29-31 ms; one launch per step 24.0 ms;
+    // FP64 fused 53.8-55.5 ms)
     const double fast_frac = L->fused == 0               ? 0.69
                              : L->precision == AXS_FP64 ? 0.54
+                             : L->fused == 3            ? 0.35
                              : L->fused == 2            ? 0.53
                                                         : 0.63;
     const double fast_ms =

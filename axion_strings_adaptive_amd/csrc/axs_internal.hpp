@@ -93,7 +93,8 @@ struct axs_level
     int tile_variant;   // stage-kernel tile variant
     int z_chunk;        // z planes per block
     int fused;          // RK4 stages: 0 per stage, 1 fused pairs, 2 pairs
-                        // without a stored accumulator
+                        // without a stored accumulator, 3 one launch
+                        // per step (FP32 / FP32C)
     int pair_variant;   // fused-pair kernel tile variant (-1 = default)
     int pair_z_chunk;   // z planes per block of the pair kernels (0 = auto)
     // captured one-step graph
@@ -122,6 +123,8 @@ int launch_rk_pair(axs_level *L, int pair, const void *yin, const void *yn,
                    void *yout, const double *coeff_table, const int *step_ptr,
                    double dt);
 int num_pair_variants();
+int launch_rk_step(axs_level *L, const void *yin, void *yout,
+                   const double *coeff_table, const int *step_ptr, double dt);
 int launch_rhs_eval(axs_level *L, const double *coeff3_dev, void *rhs);
 int launch_step_advance(axs_level *L, int reset);
 int num_tile_variants();

@@ -48,6 +48,9 @@ PAIR_BYTES = {1: [48, 96], 0: [96, 128]}
 # ... without a stored accumulator (fusion mode 2): pair 1 reads y_n and
 # writes y_2 (2S); pair 2 reads y_2, y_n, writes y_{n+1} (3S) (+ lo 2S)
 PAIR_BYTES_NOACC = {1: [32, 80], 0: [64, 96]}
+# ... of the one launch per step (fusion mode 3, FP32C): read y_n and lo,
+# write y_{n+1} and lo (4S)
+STEP_BYTES_ONE = 64
 # HBM bytes per cell of one FP32C stage-2/3 launch at 1024^3, from
 # rocprofv3 --pmc FETCH_SIZE (x 2.0) and WRITE_SIZE (x 1.0), the scales
 # measured on device copies and reads of known size; keyed by the tile
@@ -56,6 +59,8 @@ MEASURED_S23_BYTES = {(20, 32): 86.03}
 # the same for the fused FP32C stages 3+4 launch, keyed by (pair variant,
 # z_chunk): pairs with a stored accumulator (mode 1) and without (mode 2)
 MEASURED_PAIR2_BYTES = {1: {(-1, 0): 117.40}, 2: {(-1, 0): 103.48}}
+# ... and of the one-launch step kernel (mode 3)
+MEASURED_STEP_BYTES = {}
 
 
 def log(msg):
@@ -324,10 +329,11 @@ def main(argv=None):
                     help="variant,z_chunk (default: autotuned cache or built-in)")
     ap.add_argument("--no-fusion", action="store_true",
                     help="one launch per RK stage instead of two fused pairs")
-    ap.add_argument("--fusion", type=int, default=0, choices=(0, 1, 2),
-                    help="fused pairs (1) or fused pairs without a stored "
-                         "accumulator (2); 0 = the library's default per "
-                         "precision (FP32C 2, FP64 1)")
+    ap.add_argument("--fusion", type=int, default=0, choices=(0, 1, 2, 3),
+                    help="fused pairs (1), fused pairs without a stored "
+                         "accumulator (2), the whole step in one launch (3, "
+                         "FP32C only); 0 = the library's default per "
+                         "precision (FP32C 3, FP64 1)")
     ap.add_argument("--pair-tile", type=str, default="",
                     help="fused-pair variant,z_chunk (default: built-in)")
     ap.add_argument("--alloc-check", action="store_true",
@@ -384,8 +390,8 @@ def main(argv=None):
         tile = autotune.cached_choice(n, capi.AXS_FP32C)
 
     # stage fusion mode per precision (0 = one launch per stage)
-    fused = 0 if args.no_fusion else (args.fusion or 2)
-    fused64 = 0 if args.no_fusion else (args.fusion or 1)
+    fused = 0 if args.no_fusion else (args.fusion or 3)
+    fused64 = 0 if args.no_fusion else (min(args.fusion, 2) or 1)
     pair_tile = (tuple(int(t) for t in args.pair_tile.split(","))
                  if args.pair_tile else None)
 
@@ -461,7 +467,20 @@ def main(argv=None):
         b.close()
 
     st = [m / ns32 for m in ms32]  # ms per launch of each stage (pair)
-    if fused:
+    if fused == 3:
+        # one launch per step: the step kernel is the only one
+        achieved = STEP_BYTES_ONE * ncell / (st[0] * 1e-3) / 1e9
+        per_cell = MEASURED_STEP_BYTES.get(tuple(pair_tile) if pair_tile
+                                           else (-1, 0))
+        roofline = {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": per_cell * ncell if per_cell and n == 1024 else None,
+            "traffic_unit": "bytes per launch",
+            "kernel": "k_step<FP32C, 12> (the whole RK4 step)",
+            "algorithmic_bytes_per_cell": STEP_BYTES_ONE,
+            "step_launch_ms": st[0], "step_gbs": achieved}
+    elif fused:
         # roofline of the dominant kernel: the longer of the two pair
         # launches (one each per step)
         k = 0 if st[0] > st[1] else 1

@@ -175,12 +175,18 @@ int axs_level_get_tile(axs_level *lvl, int *variant, int *z_chunk);
  * the stages 3+4 launch reads Pi_n on the stencil too and recomputes
  * acc = k1 + 2 k2 at each cell from the stars of psi_n and psi_1 =
  * psi_n + (dt/2) Pi_n: 7 S (FP32C), 5 S (FP64, FP32) per cell, for twice the
- * RHS work in that launch; same bits.  on = 0: one launch per stage. */
+ * RHS work in that launch; same bits.  on = 3 (FP32 / FP32C only, their
+ * default): the whole step in one launch, which forms every stage input
+ * from y_n on the fly -- the stage-1 and stage-2 RHS on the tile grown by 2
+ * cells from an LDS plane of psi_n grown by 4 -- and stores only y_{n+1}:
+ * 4 S (FP32C), 2 S (FP32) per cell; same bits.  on = 0: one launch per
+ * stage.  FP64's default is 1. */
 int axs_level_set_stage_fusion(axs_level *lvl, int on);
 /* Tile variant of the fused-pair kernels (-1 = default, the 1024^3 sweep's
  * fastest: 8-row tiles of 2 cells per lane with z chunk 32 in FP32 (of 1
  * cell per lane for the pairs without an accumulator), 4-row tiles of 2
- * cells per lane with z chunk 64 in FP64) and their z chunk
+ * cells per lane with z chunk 64 in FP64; for the one launch per step the
+ * tile height: -1 = 12 rows, 0 = 8, 1 = 4) and their z chunk
  * (0 = default; on small grids halved until there are 4096 tiles);
  * axs_num_pair_variants() variants. */
 int axs_num_pair_variants(void);
@@ -227,17 +233,19 @@ int axs_rk4_advance_coeffs(axs_level *lvl, const axs_rhs_coeffs *coeffs,
  * placement streams the RK4 step's algorithmic bytes at the fast regime's
  * rate: per-stage launches 304 B per cell (FP32C), 544 (FP64), 272 (FP32)
  * at >= 69% of the 8 TB/s HBM peak; fused pairs 144, 224, 112 B at >= 63%
- * (FP64: 54%); pairs without a stored accumulator 112, 160, 80 B at >= 53%.  The state is reset to zero: call it before
+ * (FP64: 54%); pairs without a stored accumulator 112, 160, 80 B at >= 53%;
+ * one launch per step 64 (FP32C), 32 (FP32) B at >= 35%.  The state is reset to zero: call it before
  * uploading data.  ms_per_step (optional, `trials` entries) receives each
  * trial's time and *used (optional) the number of placements tried. */
 int axs_level_place(axs_level *lvl, int trials, double *ms_per_step, int *used);
 
 /* Per-stage kernel timing with HIP events recorded on the context stream
  * around each stage launch of axs_rk4_advance (forces direct launches while
- * on).  ms[s] accumulates the milliseconds of stage s+1; with stage fusion
- * on, ms[0] and ms[1] accumulate those of stages 1+2 and 3+4 (the fused
- * launches, or the per-stage launches of an odd last step) and ms[2], ms[3]
- * stay 0.  *nsteps counts the steps timed; axs_level_stage_times resets the
+ * on).  ms[s] accumulates the milliseconds of stage s+1; with fused pairs
+ * (stage fusion 1, 2), ms[0] and ms[1] accumulate those of stages 1+2 and
+ * 3+4 (the fused launches, or the per-stage launches of an odd last step)
+ * and ms[2], ms[3] stay 0; with one launch per step (3), ms[0] accumulates
+ * the whole step.  *nsteps counts the steps timed; axs_level_stage_times resets the
  * accumulators. */
 int axs_level_enable_timing(axs_level *lvl, int on);
 int axs_level_stage_times(axs_level *lvl, double ms[4], int64_t *nsteps);

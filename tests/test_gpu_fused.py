@@ -90,6 +90,24 @@ def test_fused_fp64_matches_oracle(ctx, oracle):
     assert np.array_equal(lvl.download(), want)
 
 
+@pytest.mark.parametrize("shape", [(64, 64, 64), (40, 24, 36), (4, 8, 12),
+                                   (128, 16, 8), (130, 10, 7), (61, 9, 5)])
+@pytest.mark.parametrize("precision", [1, 2])
+def test_one_launch_step_equals_per_stage(ctx, shape, precision):
+    """Fusion mode 3 (the whole RK4 step in one launch, FP32 / FP32C):
+    the same bits as one launch per stage, odd and even step counts, graph
+    replay, z chunks down to 4 planes."""
+    s0 = random_state(shape, seed=sum(shape) + 7 * precision)
+    for nsteps in (1, 2, 5):
+        ref = _run(ctx, shape, precision, s0, nsteps, False, 0)
+        for graph in (False, True):
+            for zc in (0, 4, 16):
+                for v in (-1, 1, 2):  # 8-, 4- and 12-row tiles
+                    got = _run(ctx, shape, precision, s0, nsteps, graph, 3, v,
+                               zc)
+                    _same(got, ref)
+
+
 def test_fused_timing_reports_pairs(ctx):
     capi = _capi()
     lvl = capi.Level(ctx, 64, 0.5, capi.AXS_FP32C)
@@ -97,6 +115,12 @@ def test_fused_timing_reports_pairs(ctx):
     lvl.enable_timing(True)
     lvl.stage_times()
     for k in (4, 3):  # an odd last step runs per stage, into the same slots
+        lvl.advance(_bg(capi), 1.0, 0.1, k)  # default: one launch per step
+        ms, nsteps = lvl.stage_times()
+        assert nsteps == k
+        assert ms[0] > 0 and ms[1] == 0 and ms[2] == 0 and ms[3] == 0
+    lvl.set_fusion(2)
+    for k in (4, 3):
         lvl.advance(_bg(capi), 1.0, 0.1, k)
         ms, nsteps = lvl.stage_times()
         assert nsteps == k
@@ -112,7 +136,9 @@ def test_fusion_setters_reject_bad_arguments(ctx):
     capi = _capi()
     lvl = capi.Level(ctx, 16, 0.5, capi.AXS_FP64)
     with pytest.raises(capi.AxsError):
-        capi.check(capi.load().axs_level_set_stage_fusion(lvl.h, 3))
+        capi.check(capi.load().axs_level_set_stage_fusion(lvl.h, 4))
+    with pytest.raises(capi.AxsError):  # one launch per step: FP32 only
+        lvl.set_fusion(3)
     with pytest.raises(capi.AxsError):
         lvl.set_pair_tile(capi.load().axs_num_pair_variants(), 0)
     with pytest.raises(capi.AxsError):
