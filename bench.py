@@ -60,7 +60,7 @@ MEASURED_S23_BYTES = {(20, 32): 86.03}
 # z_chunk): pairs with a stored accumulator (mode 1) and without (mode 2)
 MEASURED_PAIR2_BYTES = {1: {(-1, 0): 117.40}, 2: {(-1, 0): 103.48}}
 # ... and of the one-launch step kernel (mode 3)
-MEASURED_STEP_BYTES = {}
+MEASURED_STEP_BYTES = {(-1, 0): 97.07}
 
 
 def log(msg):
