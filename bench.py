@@ -7,9 +7,10 @@ radiation era a = tau from tau = 1, lambda = eta = 1 with the c = 2 (Moore)
 schedule reached through a switch at tau = 1 (c0 = 0, c1 = 2), dtau = 0.1;
 psi1, psi2 hash-seeded Gaussian fields (seed 42, modes 0 < |p| <= N/8,
 <psi_i^2> = 0.5), Pi = 0, generated on the device.  One step = one RK4 step
-of every cell: two launches, stages 1+2 and stages 3+4, each fusing the RHS
-of two stages with their RK updates (--no-fusion: one launch per stage, the
-same bits).  The FP32C run is the
+of every cell: for FP32C one launch that forms every stage input from y_n
+and stores only y_{n+1}; for FP64 two launches, stages 1+2 and 3+4, each
+fusing the RHS of two stages with their RK updates (--fusion picks a mode,
+--no-fusion one launch per stage; the same bits every way).  The FP32C run is the
 headline `value`; the FP64 run on the same IC is timed the same way and
 reported beside it together with the precision metrics (max |dphi|/eta,
 relative energy error, delta xi) at the end of the timed steps.
