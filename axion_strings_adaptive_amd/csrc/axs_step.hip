@@ -44,17 +44,22 @@ struct StepParams
 
 constexpr int kStepTX = 60; // interior columns per tile (64 lanes - 2 x 2)
 
-// Block (64, TY + 4): thread (tx, ty) owns the point (X0 + tx - 2,
-// Y0 + ty - 2) of the tile grown by 2; the interior is 2 <= tx < 62,
-// 2 <= ty < TY + 2.  Loop variable q: the plane where F(psi_n), F(psi_1)
+// Block of 64 (TY + 4) threads (AL: 68 (TY + 4)): each owns one point of
+// the tile grown by 2.  Loop variable q: the plane where F(psi_n), F(psi_1)
 // and psi_2, psi_3 are formed; the final stage runs at plane q - 2.
-template <int PREC, int TY>
-__global__ void __launch_bounds__(64 * (TY + 4)) k_step(StepParams p)
+template <int PREC, int TY, bool AL>
+__global__ void __launch_bounds__((AL ? 68 : 64) * (TY + 4)) k_step(StepParams p)
 {
-    constexpr int HYA = TY + 8, HXA = 68; // psi_n, psi_1 plane grown by 4
-    constexpr int HYB = TY + 4, HXB = 64; // psi_2, psi_3 plane grown by 2
-    __shared__ float sa[2][4][HYA][HXA];  // [buffer][psn1 psn2 ps11 ps12]
-    __shared__ float sb[3][4][HYB][HXB];  // [q % 3][ps21 ps22 ps31 ps32]
+    // AL = false: 64 x (TY + 4) threads, interior 60 x TY (rows of the tile
+    // grown by 2 are the 64 lanes of a wave); AL = true: interior 64 x TY
+    // (rows start on cache-line boundaries), the 4 x-halo columns of the
+    // grown tile taken by 4 (TY + 4) more threads
+    constexpr int TXI = AL ? 64 : kStepTX; // interior columns
+    constexpr int EXB = TXI + 4;           // grown by 2
+    constexpr int EXA = TXI + 8;           // grown by 4
+    constexpr int HYA = TY + 8, HYB = TY + 4;
+    __shared__ float sa[2][4][HYA][EXA]; // [buffer][psn1 psn2 ps11 ps12]
+    __shared__ float sb[3][4][HYB][EXB]; // [q % 3][ps21 ps22 ps31 ps32]
 
     int bid = blockIdx.x;
     if (p.xcd_remap)
@@ -67,30 +72,44 @@ __global__ void __launch_bounds__(64 * (TY + 4)) k_step(StepParams p)
     const int by = rem % p.tiles_y;
     const int bz = rem / p.tiles_y;
 
-    const int tx = threadIdx.x, ty = threadIdx.y;
+    // the own point (ec, er) of the tile grown by 2: x = X0 - 2 + ec,
+    // y = Y0 - 2 + er
+    const int t = threadIdx.x;
+    int ec, er;
+    if (!AL || t < 64 * (TY + 4))
+    {
+        ec = t % 64 + (AL ? 2 : 0);
+        er = t / 64;
+    }
+    else
+    {
+        const int u = t - 64 * (TY + 4);
+        ec = u % 4 < 2 ? u % 4 : u % 4 + 64;
+        er = u / 4;
+    }
     const int nx = p.nx, ny = p.ny, nz = p.nz;
     const uint32_t n = p.n;
     const uint32_t plane = (uint32_t)nx * (uint32_t)ny;
-    const int X0 = bx * kStepTX, Y0 = by * TY;
+    const int X0 = bx * TXI, Y0 = by * TY;
     auto wrapx = [&](int x) { return ((x % nx) + nx) % nx; };
     auto wrapy = [&](int y) { return ((y % ny) + ny) % ny; };
-    const int gx = X0 + tx - 2, gy = Y0 + ty - 2;
+    const int gx = X0 - 2 + ec, gy = Y0 - 2 + er;
     const uint32_t row = (uint32_t)wrapy(gy) * nx + wrapx(gx);
-    const bool interior = tx >= 2 && tx < 62 && ty >= 2 && ty < TY + 2;
+    const bool interior = ec >= 2 && ec < TXI + 2 && er >= 2 && er < TY + 2;
     const bool active = interior && gx < nx && gy < ny;
     // halo points of the plane grown by 4 (a plus shape: no corners)
-    const bool hyl = ty < 2, hyr = ty >= TY + 2;
-    const bool hxl = tx < 2, hxr = tx >= 62;
+    const bool hyl = er < 2, hyr = er >= TY + 2;
+    const bool hxl = ec < 2, hxr = ec >= TXI + 2;
     uint32_t rowhy = 0, rowhx = 0;
     int yah = 0, xah = 0;
     if (hyl || hyr)
     {
-        yah = hyl ? ty : ty + 4;
+        yah = hyl ? er : er + 4;
         rowhy = (uint32_t)wrapy(Y0 - 4 + yah) * nx + wrapx(gx);
     }
     if (hxl || hxr)
     {
-        xah = hxl ? tx : tx + 4;
+        xah = hxl ? ec : ec + 4;
         rowhx = (uint32_t)wrapy(gy) * nx + wrapx(X0 - 4 + xah);
     }
 
@@ -159,7 +178,7 @@ __global__ void __launch_bounds__(64 * (TY + 4)) k_step(StepParams p)
         const int sbz = (q - z_begin + 3) % 3;  // slot of plane q - 2
         // ---- psi_n, psi_1 of plane q into LDS (grown by 4) ----
         {
-            const int ya = ty + 2, xa = tx + 2;
+            const int ya = er + 2, xa = ec + 2;
 #pragma unroll
             for (int c = 0; c < 2; ++c)
             {
@@ -208,7 +227,7 @@ __global__ void __launch_bounds__(64 * (TY + 4)) k_step(StepParams p)
         // ---- F(psi_n), F(psi_1) at the own point of plane q; Pi_1, Pi_2,
         //      psi_2, psi_3 there ----
         {
-            const int ya = ty + 2, xa = tx + 2;
+            const int ya = er + 2, xa = ec + 2;
             // the Laplacian of field k of the LDS plane, z neighbours zq
             auto lap_a = [&](int k, const float(&zq)[5][2], int c) {
                 const StarF s{sa[ba][k][ya][xa - 2], sa[ba][k][ya][xa - 1],
@@ -231,8 +250,8 @@ __global__ void __launch_bounds__(64 * (TY + 4)) k_step(StepParams p)
                 p2[c] = stage_axpy<float, PREC>(qpn[2][c], ah, F[1][c]);
                 s2[c] = stage_axpy<float, PREC>(qn[2][c], ah, p1[c]);
                 s3[c] = stage_axpy<float, PREC>(qn[2][c], p.h, p2[c]);
-                sb[sbq][c][ty][tx] = s2[c];
-                sb[sbq][2 + c][ty][tx] = s3[c];
+                sb[sbq][c][er][ec] = s2[c];
+                sb[sbq][2 + c][er][ec] = s3[c];
             }
             // queues of plane-q values (index 2 = plane q)
 #pragma unroll
@@ -263,10 +282,10 @@ __global__ void __launch_bounds__(64 * (TY + 4)) k_step(StepParams p)
         if (z >= z_begin && active)
         {
             auto lap_b = [&](int k, const float(&zq)[5][2], int c) {
-                const StarF s{sb[sbz][k][ty][tx - 2], sb[sbz][k][ty][tx - 1],
-                              sb[sbz][k][ty][tx + 1], sb[sbz][k][ty][tx + 2],
-                              sb[sbz][k][ty - 2][tx], sb[sbz][k][ty - 1][tx],
-                              sb[sbz][k][ty + 1][tx], sb[sbz][k][ty + 2][tx],
+                const StarF s{sb[sbz][k][er][ec - 2], sb[sbz][k][er][ec - 1],
+                              sb[sbz][k][er][ec + 1], sb[sbz][k][er][ec + 2],
+                              sb[sbz][k][er - 2][ec], sb[sbz][k][er - 1][ec],
+                              sb[sbz][k][er + 1][ec], sb[sbz][k][er + 2][ec],
                               zq[0][c], zq[1][c], zq[3][c], zq[4][c], zq[2][c]};
                 return star_lap(s, i2);
             };
@@ -323,30 +342,36 @@ __global__ void __launch_bounds__(64 * (TY + 4)) k_step(StepParams p)
     }
 }
 
-template <int TY>
+template <int TY, bool AL>
 static void launch_step_ty(const StepParams &p, int prec, long ntiles,
                            hipStream_t s)
 {
+    const dim3 block((AL ? 68 : 64) * (TY + 4));
     if (prec == AXS_FP32C)
-        hipLaunchKernelGGL((k_step<AXS_FP32C, TY>), dim3(ntiles),
-                           dim3(64, TY + 4), 0, s, p);
+        hipLaunchKernelGGL((k_step<AXS_FP32C, TY, AL>), dim3(ntiles), block, 0,
+                           s, p);
     else
-        hipLaunchKernelGGL((k_step<AXS_FP32, TY>), dim3(ntiles),
-                           dim3(64, TY + 4), 0, s, p);
+        hipLaunchKernelGGL((k_step<AXS_FP32, TY, AL>), dim3(ntiles), block, 0,
+                           s, p);
 }
 
 // One RK4 step in one launch: y_n = yin -> y_{n+1} = yout (yout != yin),
 // FP32C lo in place.  Interior tile 60 x TY: the level's pair variant
-// selects TY (-1, the default, and 2 and above: 12 -- 1024 threads, 4 waves
-// per SIMD, the 1024^3 sweep's fastest; 0: 8; 1: 4).
+// selects the tile: -1 (default) and 3: 64 x 11 with rows aligned to cache
+// lines (1020 threads, 4 waves per SIMD; z chunk 128), the 1024^3 sweep's
+// fastest; 2: 60 x 12 (1024 threads); 0: 60 x 8; 1: 60 x 4.
 int launch_rk_step(axs_level *L, const void *yin, void *yout,
                    const double *coeff_table, const int *step_ptr, double dt)
 {
     if (L->precision == AXS_FP64)
         return set_error(AXS_ERR_UNSUPPORTED,
                          "one launch per RK4 step is FP32 / FP32C only");
+    // default (-1) and variant 3: 64-column interior (rows aligned to cache
+    // lines), 11 rows, 1020 threads -- the 1024^3 sweep's fastest
+    const bool al = L->pair_variant == 3 || L->pair_variant < 0;
     const int TY =
-        L->pair_variant == 1 ? 4 : (L->pair_variant == 0 ? 8 : 12);
+        al ? 11 : (L->pair_variant == 1 ? 4 : (L->pair_variant == 0 ? 8 : 12));
+    const int txi = al ? 64 : kStepTX;
     StepParams p{};
     p.yn = static_cast<const float *>(yin);
     p.yout = static_cast<float *>(yout);
@@ -361,9 +386,9 @@ int launch_rk_step(axs_level *L, const void *yin, void *yout,
     p.nz = L->g.nz;
     p.n = L->g.ncell;
     p.inv_dx2 = 1.0 / (L->dx * L->dx);
-    p.tiles_x = (p.nx + kStepTX - 1) / kStepTX;
+    p.tiles_x = (p.nx + txi - 1) / txi;
     p.tiles_y = (p.ny + TY - 1) / TY;
-    int zc = L->pair_z_chunk > 0 ? L->pair_z_chunk : 64;
+    int zc = L->pair_z_chunk > 0 ? L->pair_z_chunk : (al ? 128 : 64);
     while (zc > 4 && (long)p.tiles_x * p.tiles_y * ((p.nz + zc - 1) / zc) < 4096)
         zc >>= 1;
     if (zc > p.nz)
@@ -372,12 +397,14 @@ int launch_rk_step(axs_level *L, const void *yin, void *yout,
     p.tiles_z = (p.nz + zc - 1) / zc;
     const long ntiles = (long)p.tiles_x * p.tiles_y * p.tiles_z;
     p.xcd_remap = (ntiles % 8 == 0) ? 1 : 0;
-    if (TY == 4)
-        launch_step_ty<4>(p, L->precision, ntiles, L->ctx->stream);
+    if (al)
+        launch_step_ty<11, true>(p, L->precision, ntiles, L->ctx->stream);
+    else if (TY == 4)
+        launch_step_ty<4, false>(p, L->precision, ntiles, L->ctx->stream);
     else if (TY == 12)
-        launch_step_ty<12>(p, L->precision, ntiles, L->ctx->stream);
+        launch_step_ty<12, false>(p, L->precision, ntiles, L->ctx->stream);
     else
-        launch_step_ty<8>(p, L->precision, ntiles, L->ctx->stream);
+        launch_step_ty<8, false>(p, L->precision, ntiles, L->ctx->stream);
     AXS_HIP_TRY(hipGetLastError());
     return AXS_OK;
 }

@@ -61,7 +61,7 @@ MEASURED_S23_BYTES = {(20, 32): 86.03}
 # z_chunk): pairs with a stored accumulator (mode 1) and without (mode 2)
 MEASURED_PAIR2_BYTES = {1: {(-1, 0): 117.40}, 2: {(-1, 0): 103.48}}
 # ... and of the one-launch step kernel (mode 3)
-MEASURED_STEP_BYTES = {(-1, 0): 97.07}
+MEASURED_STEP_BYTES = {(2, 0): 97.07}
 
 
 def log(msg):
@@ -478,7 +478,7 @@ def main(argv=None):
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
             "traffic": per_cell * ncell if per_cell and n == 1024 else None,
             "traffic_unit": "bytes per launch",
-            "kernel": "k_step<FP32C, 12> (the whole RK4 step)",
+            "kernel": "k_step<FP32C, 11, aligned> (the whole RK4 step)",
             "algorithmic_bytes_per_cell": STEP_BYTES_ONE,
             "step_launch_ms": st[0], "step_gbs": achieved}
     elif fused:

@@ -186,7 +186,8 @@ int axs_level_set_stage_fusion(axs_level *lvl, int on);
  * fastest: 8-row tiles of 2 cells per lane with z chunk 32 in FP32 (of 1
  * cell per lane for the pairs without an accumulator), 4-row tiles of 2
  * cells per lane with z chunk 64 in FP64; for the one launch per step the
- * tile height: -1 = 12 rows, 0 = 8, 1 = 4) and their z chunk
+ * tile: -1 and 3 = 64 x 11 cells with rows aligned to cache lines, z chunk
+ * 128; 2 = 60 x 12; 0 = 60 x 8; 1 = 60 x 4) and their z chunk
  * (0 = default; on small grids halved until there are 4096 tiles);
  * axs_num_pair_variants() variants. */
 int axs_num_pair_variants(void);
