@@ -11,9 +11,10 @@
 // F(psi_n) and F(psi_1) on the star as well: each workgroup evaluates those
 // on its tile grown by 2 cells in x and y (one thread per point of the grown
 // tile) from an LDS plane of psi_n, psi_1 grown by 4, streams z with
-// register queues, and stores only y_{n+1}: 4 S + 2 S (FP32C lo) = 64 B per
-// cell and step (112 B for the fused pairs without an accumulator, 304 B for
-// one launch per stage).  Every value is formed by the expressions of the
+// register queues, and stores only y_{n+1}: y_n read and y_{n+1} written,
+// 2 S, plus 2 S for the FP32C low words = 64 B per cell and step (112 B for
+// the fused pairs without an accumulator, 304 B for one launch per stage;
+// S = 4 words).  Every value is formed by the expressions of the
 // per-stage kernels (stage_update, stage_axpy, force, star_lap), so the bits
 // are the same.  The output must not be y_n's buffer (callers alternate two
 // state buffers).
