@@ -47,7 +47,8 @@ def _same(a, b):
 
 
 @pytest.mark.parametrize("shape", [(64, 64, 64), (40, 24, 36), (4, 8, 12),
-                                   (128, 16, 8), (36, 8, 8), (130, 10, 7)])
+                                   (128, 16, 8), (36, 8, 8), (130, 10, 7),
+                                   (9, 4, 4), (67, 13, 5)])
 @pytest.mark.parametrize("precision", [0, 1, 2])
 def test_fused_equals_per_stage_shapes(ctx, shape, precision):
     s0 = random_state(shape, seed=sum(shape) + precision)
@@ -91,7 +92,8 @@ def test_fused_fp64_matches_oracle(ctx, oracle):
 
 
 @pytest.mark.parametrize("shape", [(64, 64, 64), (40, 24, 36), (4, 8, 12),
-                                   (128, 16, 8), (130, 10, 7), (61, 9, 5)])
+                                   (128, 16, 8), (130, 10, 7), (61, 9, 5),
+                                   (9, 4, 4), (67, 13, 5), (200, 5, 4)])
 @pytest.mark.parametrize("precision", [1, 2])
 def test_one_launch_step_equals_per_stage(ctx, shape, precision):
     """Fusion mode 3 (the whole RK4 step in one launch, FP32 / FP32C):
