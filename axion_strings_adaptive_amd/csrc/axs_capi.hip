@@ -1114,12 +1114,12 @@ int axs_level_place(axs_level *L, int trials, double *ms_per_step, int *used)
         fp32c * S;
     // the fast regime's floor (1024^3 FP32C: per-stage 57 ms against 65 ms
     // per step; fused pairs 30.1 ms against 32.0-33.8 ms; pairs without an
-    // accumulator 27.9 ms against 29-31 ms; one launch per step 21.9-22.3
-    // ms against 24.0 ms;
+    // accumulator 27.9 ms against 29-31 ms; one launch per step 18.5-19.6
+    // ms;
     // FP64 fused 53.8-55.5 ms)
     const double fast_frac = L->fused == 0               ? 0.69
                              : L->precision == AXS_FP64 ? 0.54
-                             : L->fused == 3            ? 0.38
+                             : L->fused == 3            ? 0.42
                              : L->fused == 2            ? 0.53
                                                         : 0.63;
     const double fast_ms =

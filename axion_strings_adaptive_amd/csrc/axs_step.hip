@@ -126,8 +126,12 @@ __global__ void __launch_bounds__((AL ? 68 : 64) * (TY + 4)) k_step(StepParams p
         return stage_axpy<float, PREC>(ps, ah, pi);
     };
 
-    // stage scalars (c, lambda, R^2) of the four stages
-    const double *cf = p.coeff + 12 * (p.step ? *p.step : 0);
+    // stage scalars (c, lambda, R^2) of the four stages, read once
+    const double *cfp = p.coeff + 12 * (p.step ? *p.step : 0);
+    double cf[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i)
+        cf[i] = cfp[i];
     const float i2 = (float)p.inv_dx2;
 
     // register queues of the own point: psi_n, psi_1, Pi_n at q-2..q+2;

@@ -235,7 +235,7 @@ int axs_rk4_advance_coeffs(axs_level *lvl, const axs_rhs_coeffs *coeffs,
  * rate: per-stage launches 304 B per cell (FP32C), 544 (FP64), 272 (FP32)
  * at >= 69% of the 8 TB/s HBM peak; fused pairs 144, 224, 112 B at >= 63%
  * (FP64: 54%); pairs without a stored accumulator 112, 160, 80 B at >= 53%;
- * one launch per step 64 (FP32C), 32 (FP32) B at >= 38%.  The state is reset to zero: call it before
+ * one launch per step 64 (FP32C), 32 (FP32) B at >= 42%.  The state is reset to zero: call it before
  * uploading data.  ms_per_step (optional, `trials` entries) receives each
  * trial's time and *used (optional) the number of placements tried. */
 int axs_level_place(axs_level *lvl, int trials, double *ms_per_step, int *used);
