@@ -85,16 +85,25 @@ __device__ __forceinline__ void force(double lap1, double lap2, double psi1,
     F2 = lap2 + cc * psi2 - pc * psi2;
 }
 
+// FP32 force with the stage scalars already rounded: ccf = (float)c,
+// hlam = 0.5f * (float)lambda, r2f = (float)R^2.
+__device__ __forceinline__ void force_f(float lap1, float lap2, float psi1,
+                                        float psi2, float ccf, float hlam,
+                                        float r2f, float &F1, float &F2)
+{
+#pragma clang fp contract(off)
+    const float d = fmaf(psi1, psi1, fmaf(psi2, psi2, -r2f));
+    const float pc = hlam * d;
+    F1 = fmaf(-pc, psi1, fmaf(ccf, psi1, lap1));
+    F2 = fmaf(-pc, psi2, fmaf(ccf, psi2, lap2));
+}
+
 __device__ __forceinline__ void force(float lap1, float lap2, float psi1,
                                       float psi2, double cc, double lam,
                                       double R2, float &F1, float &F2)
 {
-#pragma clang fp contract(off)
-    const float d = fmaf(psi1, psi1, fmaf(psi2, psi2, -(float)R2));
-    const float pc = (0.5f * (float)lam) * d;
-    const float ccf = (float)cc;
-    F1 = fmaf(-pc, psi1, fmaf(ccf, psi1, lap1));
-    F2 = fmaf(-pc, psi2, fmaf(ccf, psi2, lap2));
+    force_f(lap1, lap2, psi1, psi2, (float)cc, 0.5f * (float)lam, (float)R2,
+            F1, F2);
 }
 
 __device__ __forceinline__ double axpy_f64(double y, double a, double k)
@@ -127,15 +136,15 @@ __device__ __forceinline__ T stage_axpy(T y, double a, T k)
         return fmaf((float)a, k, y);
 }
 
-// Stage update of one cell.  k[4] = (Pi1, Pi2, F1, F2) of the stage input;
-// yn[4] = y_n (stage 1: the stage input itself), ac[4] the accumulator,
-// lo[4] the FP32C low part of y_n.  Outputs: acc_out (stages 1-3),
-// y_out (the next stage input, or y_{n+1} at stage 4), lo (stage 4 FP32C).
-template <typename T, int PREC, int MODE>
-__device__ __forceinline__ void stage_update(const T k[4], const T yn[4],
-                                             const T ac[4], float lo[4],
-                                             double a, double h6, float h6_lo,
-                                             T acc_out[4], T y_out[4])
+// FP32 / FP32C stage update with the scalars already rounded to float:
+// a_f = (float)a, h6_f = (float)(dt/6), h6_lo the low part of dt/6.
+template <int PREC, int MODE>
+__device__ __forceinline__ void stage_update_f(const float k[4],
+                                               const float yn[4],
+                                               const float ac[4], float lo[4],
+                                               float a_f, float h6_f,
+                                               float h6_lo, float acc_out[4],
+                                               float y_out[4])
 {
 #pragma clang fp contract(off)
     if constexpr (MODE == MODE_RHS)
@@ -148,40 +157,79 @@ __device__ __forceinline__ void stage_update(const T k[4], const T yn[4],
         for (int c = 0; c < 4; ++c)
         {
             acc_out[c] = k[c];
-            y_out[c] = stage_axpy<T, PREC>(yn[c], a, k[c]);
+            y_out[c] = fmaf(a_f, k[c], yn[c]);
         }
     }
     else if constexpr (MODE == MODE_S23)
     {
         for (int c = 0; c < 4; ++c)
         {
-            if constexpr (PREC == AXS_FP64)
-                acc_out[c] = acc2_f64(ac[c], k[c]);
-            else
-                acc_out[c] = fmaf(2.0f, k[c], ac[c]);
-            y_out[c] = stage_axpy<T, PREC>(yn[c], a, k[c]);
+            acc_out[c] = fmaf(2.0f, k[c], ac[c]);
+            y_out[c] = fmaf(a_f, k[c], yn[c]);
         }
     }
     else
     {
         for (int c = 0; c < 4; ++c)
         {
-            if constexpr (PREC == AXS_FP64)
-                y_out[c] = final_f64(yn[c], h6, ac[c], k[c]);
-            else if constexpr (PREC == AXS_FP32)
-                y_out[c] = fmaf((float)h6, ac[c] + k[c], yn[c]);
+            if constexpr (PREC == AXS_FP32)
+                y_out[c] = fmaf(h6_f, ac[c] + k[c], yn[c]);
             else
             {
                 // Kahan: true y_n = hi + lo; add delta = (dt/6)(acc + k)
                 // with dt/6 carried as hi + lo floats.
                 const float sv = ac[c] + k[c];
-                const float delta = fmaf((float)h6, sv, h6_lo * sv);
+                const float delta = fmaf(h6_f, sv, h6_lo * sv);
                 const float yk = delta + lo[c];
                 const float t = yn[c] + yk;
                 lo[c] = yk - (t - yn[c]);
                 y_out[c] = t;
             }
         }
+    }
+}
+
+// Stage update of one cell.  k[4] = (Pi1, Pi2, F1, F2) of the stage input;
+// yn[4] = y_n (stage 1: the stage input itself), ac[4] the accumulator,
+// lo[4] the FP32C low part of y_n.  Outputs: acc_out (stages 1-3),
+// y_out (the next stage input, or y_{n+1} at stage 4), lo (stage 4 FP32C).
+template <typename T, int PREC, int MODE>
+__device__ __forceinline__ void stage_update(const T k[4], const T yn[4],
+                                             const T ac[4], float lo[4],
+                                             double a, double h6, float h6_lo,
+                                             T acc_out[4], T y_out[4])
+{
+#pragma clang fp contract(off)
+    if constexpr (PREC != AXS_FP64)
+    {
+        stage_update_f<PREC, MODE>(k, yn, ac, lo, (float)a, (float)h6, h6_lo,
+                                   acc_out, y_out);
+    }
+    else if constexpr (MODE == MODE_RHS)
+    {
+        for (int c = 0; c < 4; ++c)
+            y_out[c] = k[c];
+    }
+    else if constexpr (MODE == MODE_S1)
+    {
+        for (int c = 0; c < 4; ++c)
+        {
+            acc_out[c] = k[c];
+            y_out[c] = axpy_f64(yn[c], a, k[c]);
+        }
+    }
+    else if constexpr (MODE == MODE_S23)
+    {
+        for (int c = 0; c < 4; ++c)
+        {
+            acc_out[c] = acc2_f64(ac[c], k[c]);
+            y_out[c] = axpy_f64(yn[c], a, k[c]);
+        }
+    }
+    else
+    {
+        for (int c = 0; c < 4; ++c)
+            y_out[c] = final_f64(yn[c], h6, ac[c], k[c]);
     }
 }
 

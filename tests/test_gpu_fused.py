@@ -104,10 +104,26 @@ def test_one_launch_step_equals_per_stage(ctx, shape, precision):
         ref = _run(ctx, shape, precision, s0, nsteps, False, 0)
         for graph in (False, True):
             for zc in (0, 4, 16):
-                for v in (-1, 0, 1, 2):  # aligned 64 x 11; 60 x 8, 4, 12
+                # -1: ring kernel, 3: register-queue kernel (both 64 x 11
+                # aligned); 0, 1, 2: 60 x 8, 4, 12
+                for v in (-1, 0, 1, 2, 3):
                     got = _run(ctx, shape, precision, s0, nsteps, graph, 3, v,
                                zc)
                     _same(got, ref)
+
+
+@pytest.mark.parametrize("precision", [1, 2])
+def test_one_launch_step_long_z_chunks(ctx, precision):
+    """The production z chunks (128 and 64 planes, several chunks along z,
+    explicit and automatic) give the per-stage bits."""
+    shape = (64, 22, 256)
+    s0 = random_state(shape, seed=911 + precision)
+    for nsteps in (1, 3):
+        ref = _run(ctx, shape, precision, s0, nsteps, False, 0)
+        for zc in (0, 64, 128, 96):
+            for v in (-1, 3):
+                got = _run(ctx, shape, precision, s0, nsteps, True, 3, v, zc)
+                _same(got, ref)
 
 
 def test_fused_timing_reports_pairs(ctx):
