@@ -449,12 +449,14 @@ int axs_level_create(axs_ctx *ctx, int nx, int ny, int nz, double dx,
 
 static void destroy_graph(axs_level *L)
 {
-    if (L->graph_valid)
-    {
-        (void)hipGraphExecDestroy(L->graph_exec);
-        (void)hipGraphDestroy(L->graph);
-        L->graph_valid = 0;
-    }
+    for (int i = 0; i < 2; ++i)
+        if (L->graph_valid[i])
+        {
+            (void)hipGraphExecDestroy(L->graph_exec[i]);
+            (void)hipGraphDestroy(L->graph[i]);
+            L->graph_valid[i] = 0;
+            L->graph_y[i] = nullptr;
+        }
 }
 
 int axs_level_destroy(axs_level *L)
@@ -753,9 +755,7 @@ static int run_step_timed(axs_level *L, const int *step_ptr, double dt)
     {
         float ms = 0.0f;
         AXS_HIP_TRY(hipEventElapsedTime(&ms, L->ev[st], L->ev[st + 1]));
-        // with fused pairs, entries 0 and 1 hold stages 1+2 and 3+4; with
-        // one launch per step, entry 0 the whole step
-        L->stage_ms[L->fused == 3 ? 0 : (L->fused ? st / 2 : st)] += ms;
+        L->stage_ms[st] += ms;
     }
     L->timed_steps += 1;
     return AXS_OK;
@@ -775,21 +775,26 @@ static int run_step(axs_level *L, const int *step_ptr, double dt)
     return launch_rk_stage(L, 4, L->ya, L->y, L->d_coeff, step_ptr, 3, dt);
 }
 
-// One RK4 step as two fused launches (stages 1+2, stages 3+4) from the state
-// in `from` into `to` (to != from: pair 2 reads psi_n around every cell);
-// FP32C lo is updated in place.
-static int run_step_fused(axs_level *L, const int *step_ptr, double dt,
-                          const void *from, void *to)
+// One fused RK4 step: the whole step in one launch (fusion 3) or two
+// launches, stages 1+2 and stages 3+4 (fusions 1, 2; y_2 in yb, the
+// accumulator of fusion 1 in ya).  Both read y_n around every cell while
+// other workgroups store, so y_{n+1} goes to a second array: the
+// accumulator array acc, which these paths do not use, and the two
+// pointers then trade places, so that L->y names the new state.  FP32C lo
+// is updated in place.
+static int run_step_fused(axs_level *L, const int *step_ptr, double dt)
 {
     hipStream_t s = L->ctx->stream;
+    const void *from = L->y;
+    void *to = L->acc;
     int rc;
     if (L->timing)
         AXS_HIP_TRY(hipEventRecord(L->ev[0], s));
     if (L->fused == 3)
     {
-        // the whole step in one launch
         if ((rc = launch_rk_step(L, from, to, L->d_coeff, step_ptr, dt)))
             return rc;
+        std::swap(L->y, L->acc);
         if (!L->timing)
             return AXS_OK;
         AXS_HIP_TRY(hipEventRecord(L->ev[1], s));
@@ -801,14 +806,15 @@ static int run_step_fused(axs_level *L, const int *step_ptr, double dt,
         return AXS_OK;
     }
     const int p1 = L->fused == 2 ? 3 : 1; // 2: accumulator-free pairs
-    if ((rc = launch_rk_pair(L, p1, from, from, L->yb, L->d_coeff, step_ptr,
-                             dt)))
+    if ((rc = launch_rk_pair(L, p1, from, from, L->yb, L->ya, L->d_coeff,
+                             step_ptr, dt)))
         return rc;
     if (L->timing)
         AXS_HIP_TRY(hipEventRecord(L->ev[1], s));
-    if ((rc = launch_rk_pair(L, p1 + 1, L->yb, from, to, L->d_coeff, step_ptr,
-                             dt)))
+    if ((rc = launch_rk_pair(L, p1 + 1, L->yb, from, to, L->ya, L->d_coeff,
+                             step_ptr, dt)))
         return rc;
+    std::swap(L->y, L->acc);
     if (!L->timing)
         return AXS_OK;
     AXS_HIP_TRY(hipEventRecord(L->ev[2], s));
@@ -823,30 +829,18 @@ static int run_step_fused(axs_level *L, const int *step_ptr, double dt,
     return AXS_OK;
 }
 
-// count RK4 steps; the state ends in L->y.  With fusion on, steps go in twos
-// (L->y -> L->ya -> L->y) and an odd last step runs stage by stage (in
-// place); both give the same bits.  With advance, the device step counter
-// moves on after every step.
+// count RK4 steps in the level's stage-fusion mode, each fused step one
+// launch (or one pair of launches) whatever count is; L->y names the
+// state after every step.  With advance, the device step counter moves on
+// after every step.
 static int run_steps(axs_level *L, const int *step_ptr, double dt, int count,
                      bool advance)
 {
     int rc;
-    int i = 0;
-    if (L->fused)
-        for (; i + 2 <= count; i += 2)
-        {
-            if ((rc = run_step_fused(L, step_ptr, dt, L->y, L->ya)))
-                return rc;
-            if (advance && (rc = launch_step_advance(L, 0)))
-                return rc;
-            if ((rc = run_step_fused(L, step_ptr, dt, L->ya, L->y)))
-                return rc;
-            if (advance && (rc = launch_step_advance(L, 0)))
-                return rc;
-        }
-    for (; i < count; ++i)
+    for (int i = 0; i < count; ++i)
     {
-        if ((rc = run_step(L, step_ptr, dt)))
+        if ((rc = L->fused ? run_step_fused(L, step_ptr, dt)
+                           : run_step(L, step_ptr, dt)))
             return rc;
         if (advance && (rc = launch_step_advance(L, 0)))
             return rc;
@@ -868,14 +862,24 @@ int axs_rk4_step(axs_level *L, const axs_rhs_coeffs coeffs[4], double dt)
     int rc = upload_coeffs(L, h, 12, L->d_coeff);
     if (rc)
         return rc;
-    return run_step(L, nullptr, dt);
+    return run_steps(L, nullptr, dt, 1, false);
 }
 
-static int capture_graph(axs_level *L, double dt)
+// Captures `per` steps (2 with fused steps, which then end in the buffer
+// they start from; 1 per stage) from the current state buffer into a graph
+// slot; returns the slot.
+static int capture_graph(axs_level *L, double dt, int per, int *slot_out)
 {
-    destroy_graph(L);
+    const int slot = L->graph_next;
+    L->graph_next ^= 1;
+    if (L->graph_valid[slot])
+    {
+        (void)hipGraphExecDestroy(L->graph_exec[slot]);
+        (void)hipGraphDestroy(L->graph[slot]);
+        L->graph_valid[slot] = 0;
+    }
     hipStream_t s = L->ctx->stream;
-    const int per = L->fused ? 2 : 1; // steps per replay
+    const void *y0 = L->y;
     AXS_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     int rc = run_steps(L, L->d_step, dt, per, true);
     hipGraph_t g = nullptr;
@@ -889,6 +893,12 @@ static int capture_graph(axs_level *L, double dt)
     if (e != hipSuccess)
         return set_error(AXS_ERR_HIP, std::string("hipStreamEndCapture: ") +
                                           hipGetErrorString(e));
+    if (L->y != y0)
+    {
+        (void)hipGraphDestroy(g);
+        return set_error(AXS_ERR_ARG, "capture_graph: a replay must end in "
+                                      "the buffer it starts from");
+    }
     hipGraphExec_t ge = nullptr;
     e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
     if (e != hipSuccess)
@@ -897,12 +907,28 @@ static int capture_graph(axs_level *L, double dt)
         return set_error(AXS_ERR_HIP, std::string("hipGraphInstantiate: ") +
                                           hipGetErrorString(e));
     }
-    L->graph = g;
-    L->graph_exec = ge;
-    L->graph_valid = 1;
-    L->graph_key[0] = dt;
-    L->graph_key[1] = per;
+    L->graph[slot] = g;
+    L->graph_exec[slot] = ge;
+    L->graph_valid[slot] = 1;
+    L->graph_key[slot][0] = dt;
+    L->graph_key[slot][1] = per;
+    L->graph_y[slot] = y0;
+    *slot_out = slot;
     return AXS_OK;
+}
+
+// The graph slot replaying `per` steps of dt from the current state
+// buffer, captured on first use.
+static int graph_for(axs_level *L, double dt, int per, int *slot)
+{
+    for (int i = 0; i < 2; ++i)
+        if (L->graph_valid[i] && L->graph_y[i] == L->y &&
+            L->graph_key[i][0] == dt && L->graph_key[i][1] == per)
+        {
+            *slot = i;
+            return AXS_OK;
+        }
+    return capture_graph(L, dt, per, slot);
 }
 
 } // extern "C"
@@ -936,13 +962,16 @@ static int advance_table(axs_level *L, double dt, int nsteps, int use_graph,
         if (use_graph && !L->timing)
         {
             const int per = L->fused ? 2 : 1;
-            if (!L->graph_valid || L->graph_key[0] != dt ||
-                L->graph_key[1] != per)
-                if ((rc = capture_graph(L, dt)))
-                    return rc;
             int i = 0;
-            for (; i + per <= chunk; i += per)
-                AXS_HIP_TRY(hipGraphLaunch(L->graph_exec, L->ctx->stream));
+            if (chunk >= per)
+            {
+                int slot = 0;
+                if ((rc = graph_for(L, dt, per, &slot)))
+                    return rc;
+                for (; i + per <= chunk; i += per)
+                    AXS_HIP_TRY(
+                        hipGraphLaunch(L->graph_exec[slot], L->ctx->stream));
+            }
             if (i < chunk && (rc = run_steps(L, L->d_step, dt, chunk - i, true)))
                 return rc;
         }
@@ -1008,7 +1037,7 @@ static int time_placement(axs_level *L, hipEvent_t e0, hipEvent_t e1,
     if (L->comp)
         AXS_HIP_TRY(hipMemsetAsync(L->comp, 0, bytes4, s));
     int rc = upload_coeffs(L, row, 12, L->d_coeff);
-    if (rc || (rc = run_step(L, nullptr, 0.1)))
+    if (rc || (rc = run_steps(L, nullptr, 0.1, 1, false)))
         return rc;
     AXS_HIP_TRY(hipEventRecord(e0, s));
     if ((rc = run_steps(L, nullptr, 0.1, 2, false)))

@@ -82,9 +82,11 @@ struct axs_level
     double dx;
     int precision;
     size_t word;        // bytes per stored word (8 or 4)
-    void *y;            // y_n: 4 comps
+    void *y;            // y_n: 4 comps (the current state)
     float *comp;        // FP32C: low part of y_n (4 comps), else nullptr
-    void *acc;          // RK accumulator (4 comps)
+    void *acc;          // RK accumulator of the per-stage launches (4
+                        // comps); the fused steps write y_{n+1} here and
+                        // then swap it with y
     void *ya;           // stage state A (4 comps)
     void *yb;           // stage state B (4 comps)
     void *alloc[4];     // allocations behind y, acc, ya|yb, comp
@@ -97,11 +99,14 @@ struct axs_level
                         // per step (FP32 / FP32C)
     int pair_variant;   // fused-pair kernel tile variant (-1 = default)
     int pair_z_chunk;   // z planes per block of the pair kernels (0 = auto)
-    // captured one-step graph
-    hipGraph_t graph;
-    hipGraphExec_t graph_exec;
-    int graph_valid;
-    double graph_key[2]; // dt and table identity at capture
+    // captured step graphs, one per state buffer the replay starts from
+    // (fused steps alternate y between two arrays)
+    hipGraph_t graph[2];
+    hipGraphExec_t graph_exec[2];
+    int graph_valid[2];
+    double graph_key[2][2]; // dt and steps per replay at capture
+    const void *graph_y[2]; // the state buffer at capture
+    int graph_next;         // slot the next capture replaces
     // FFT plans (lazily created)
     void *fft_r2c;  // hipfftHandle stored as void*
     void *fft_c2r;
@@ -120,8 +125,8 @@ int launch_rk_stage(axs_level *L, int stage, const void *yin, void *yout,
                     const double *coeff_table, const int *step_ptr,
                     int coeff_stage, double dt);
 int launch_rk_pair(axs_level *L, int pair, const void *yin, const void *yn,
-                   void *yout, const double *coeff_table, const int *step_ptr,
-                   double dt);
+                   void *yout, void *acc, const double *coeff_table,
+                   const int *step_ptr, double dt);
 int num_pair_variants();
 int launch_rk_step(axs_level *L, const void *yin, void *yout,
                    const double *coeff_table, const int *step_ptr, double dt);

@@ -722,10 +722,11 @@ static void launch_pmode(int pair, int base, const PairParams<T> &p,
 
 // Pair 1 (stages 1, 2): y_n = yin -> acc, y_2 = yout.  Pair 2 (stages 3, 4):
 // y_2 = yin, y_n = yn, acc, lo -> y_{n+1} = yout (yout != yn).  Pairs 3
-// and 4: the same without the accumulator (pair 4 recomputes it).
+// and 4: the same without the accumulator (pair 4 recomputes it; acc is
+// not used).
 int launch_rk_pair(axs_level *L, int pair, const void *yin, const void *yn,
-                   void *yout, const double *coeff_table, const int *step_ptr,
-                   double dt)
+                   void *yout, void *acc, const double *coeff_table,
+                   const int *step_ptr, double dt)
 {
     const bool f64 = L->precision == AXS_FP64;
     const int nx = L->g.nx, ny = L->g.ny, nz = L->g.nz;
@@ -780,7 +781,7 @@ int launch_rk_pair(axs_level *L, int pair, const void *yin, const void *yn,
             typename std::remove_pointer<decltype(p.yn)>::type>::type;
         p.yin = static_cast<const TT *>(yin);
         p.yn = static_cast<const TT *>(yn);
-        p.acc = static_cast<TT *>(L->acc);
+        p.acc = static_cast<TT *>(acc);
         p.yout = static_cast<TT *>(yout);
         p.comp = L->comp;
         p.coeff = coeff_table;

@@ -145,7 +145,10 @@ int axs_level_download_raw(axs_level *lvl, void *hi, void *lo);
 /* The inverse of axs_level_download_raw (exact restore of a checkpoint,
  * the role of AmrLevel::restart for the state data): FP32C needs `lo`. */
 int axs_level_upload_raw(axs_level *lvl, const void *hi, const void *lo);
-/* Device pointers of the state (y_n) and compensation arrays. */
+/* Device pointers of the state (y_n) and compensation arrays.  With stage
+ * fusion on, every RK4 step writes y_{n+1} into a second array of the level
+ * and the level's state then names that array: the state pointer holds
+ * until the next step call. */
 int axs_level_device_ptrs(axs_level *lvl, void **state, void **comp);
 /* Homogeneous exact solution psi = (R, 0), Pi = (R', 0)
  * (AxionStringsLevel.cpp:824-847). */
@@ -164,23 +167,23 @@ int axs_level_handoff_rescale(axs_level *lvl, double kappa,
 int axs_level_set_tile(axs_level *lvl, int variant, int z_chunk);
 int axs_level_get_tile(axs_level *lvl, int *variant, int *z_chunk);
 /* Stage fusion; no counterpart in the reference (how the RK4 step of
- * AxionStringsLevel.cpp:850-921 is launched).  on = 1 (the default): each
- * RK4 step is two launches, stages 1+2 and stages 3+4, which form the second
- * stage's psi at every stencil point from the first stage's inputs
- * (psi_n + (dt/2) Pi_n, psi_n + dt Pi_2) and so never store y_1 or y_3: 9 S
- * instead of 19 S (FP32C) and 7 S instead of 17 S (FP64, FP32) of HBM
- * traffic per cell, S = 4 words.  Steps run in twos through a second state
- * buffer, and an odd last step of a call runs stage by stage; the bits are
- * the same either way.  on = 2: the pairs without a stored accumulator --
- * the stages 3+4 launch reads Pi_n on the stencil too and recomputes
- * acc = k1 + 2 k2 at each cell from the stars of psi_n and psi_1 =
- * psi_n + (dt/2) Pi_n: 7 S (FP32C), 5 S (FP64, FP32) per cell, for twice the
- * RHS work in that launch; same bits.  on = 3 (FP32 / FP32C only, their
+ * AxionStringsLevel.cpp:850-921 is launched).  on = 1 (FP64's default):
+ * each RK4 step is two launches, stages 1+2 and stages 3+4, which form the
+ * second stage's psi at every stencil point from the first stage's inputs
+ * (psi_n + (dt/2) Pi_n, psi_n + dt Pi_2) and so never store y_1 or y_3:
+ * 9 S instead of 19 S (FP32C) and 7 S instead of 17 S (FP64, FP32) of HBM
+ * traffic per cell, S = 4 words.  on = 2: the pairs without a stored
+ * accumulator -- the stages 3+4 launch reads Pi_n on the stencil too and
+ * recomputes acc = k1 + 2 k2 at each cell from the stars of psi_n and
+ * psi_1 = psi_n + (dt/2) Pi_n: 7 S (FP32C), 5 S (FP64, FP32) per cell, for
+ * twice the RHS work in that launch.  on = 3 (FP32 / FP32C only, their
  * default): the whole step in one launch, which forms every stage input
- * from y_n on the fly -- the stage-1 and stage-2 RHS on the tile grown by 2
- * cells from an LDS plane of psi_n grown by 4 -- and stores only y_{n+1}:
- * 4 S (FP32C), 2 S (FP32) per cell; same bits.  on = 0: one launch per
- * stage.  FP64's default is 1. */
+ * from y_n on the fly -- the stage-1 and stage-2 RHS on the tile grown by
+ * 2 cells from an LDS plane of psi_n grown by 4 -- and stores only
+ * y_{n+1}: 4 S (FP32C), 2 S (FP32) per cell.  on = 0: one launch per
+ * stage.  Every mode gives the same bits.  The fused modes write y_{n+1}
+ * into a second array of the level, which then becomes the state (see
+ * axs_level_device_ptrs); every step, also a single one, runs fused. */
 int axs_level_set_stage_fusion(axs_level *lvl, int on);
 /* Tile variant of the fused-pair kernels (-1 = default, the 1024^3 sweep's
  * fastest: 8-row tiles of 2 cells per lane with z chunk 32 in FP32 (of 1
@@ -201,14 +204,15 @@ int axs_level_set_pair_tile(axs_level *lvl, int variant, int z_chunk);
  * 4*nx*ny*nz words of the level's precision (float for FP32C/FP32). */
 int axs_rhs_eval(axs_level *lvl, const axs_rhs_coeffs *c, void *rhs);
 /* One classic RK4 step with explicit stage scalars (stage s at tau,
- * tau+dt/2, tau+dt/2, tau+dt): four fused RHS + RK-stage launches.  This
- * replaces GRTeclyn's RK4 stage driver around specific_eval_rhs
- * (AxionStringsLevel.cpp:850-921). */
+ * tau+dt/2, tau+dt/2, tau+dt), launched in the level's stage-fusion mode
+ * (FP32 / FP32C: one launch; FP64: two).  This replaces GRTeclyn's RK4
+ * stage driver around specific_eval_rhs (AxionStringsLevel.cpp:850-921). */
 int axs_rk4_step(axs_level *lvl, const axs_rhs_coeffs coeffs[4], double dt);
 /* nsteps RK4 steps from conformal time tau with the background's stage
  * scalars (host FP64), uploaded once as a table.  use_graph = 1 replays a
- * captured hipGraph of two steps (fused pairs) or one step (per-stage
- * launches), each followed by the step-counter advance. */
+ * captured hipGraph of two steps (fused modes; one graph per state array a
+ * replay starts from) or one step (per-stage launches), each followed by
+ * the step-counter advance; an odd last step runs as a direct launch. */
 int axs_rk4_advance(axs_level *lvl, const axs_background *bg, double tau,
                     double dt, int nsteps, int use_graph);
 
@@ -243,10 +247,9 @@ int axs_level_place(axs_level *lvl, int trials, double *ms_per_step, int *used);
 /* Per-stage kernel timing with HIP events recorded on the context stream
  * around each stage launch of axs_rk4_advance (forces direct launches while
  * on).  ms[s] accumulates the milliseconds of stage s+1; with fused pairs
- * (stage fusion 1, 2), ms[0] and ms[1] accumulate those of stages 1+2 and
- * 3+4 (the fused launches, or the per-stage launches of an odd last step)
- * and ms[2], ms[3] stay 0; with one launch per step (3), ms[0] accumulates
- * the whole step.  *nsteps counts the steps timed; axs_level_stage_times resets the
+ * (stage fusion 1, 2), ms[0] and ms[1] accumulate those of the stages 1+2
+ * and 3+4 launches and ms[2], ms[3] stay 0; with one launch per step (3),
+ * ms[0] accumulates the step launch.  *nsteps counts the steps timed; axs_level_stage_times resets the
  * accumulators. */
 int axs_level_enable_timing(axs_level *lvl, int on);
 int axs_level_stage_times(axs_level *lvl, double ms[4], int64_t *nsteps);

@@ -132,7 +132,7 @@ def test_fused_timing_reports_pairs(ctx):
     lvl.upload(random_state((64, 64, 64), seed=2))
     lvl.enable_timing(True)
     lvl.stage_times()
-    for k in (4, 3):  # an odd last step runs per stage, into the same slots
+    for k in (4, 3):  # every step, an odd last one too, is one launch
         lvl.advance(_bg(capi), 1.0, 0.1, k)  # default: one launch per step
         ms, nsteps = lvl.stage_times()
         assert nsteps == k
@@ -162,3 +162,51 @@ def test_fusion_setters_reject_bad_arguments(ctx):
     with pytest.raises(capi.AxsError):
         lvl.set_pair_tile(0, -1)
     lvl.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1, 2])
+@pytest.mark.parametrize("graph", [False, True])
+def test_single_step_calls_equal_one_call(ctx, precision, graph):
+    """N calls of one step each (advance and rk4_step) give the bits of one
+    N-step call in every precision's default fusion mode; each single step
+    is one fused launch (FP32 / FP32C: the step kernel alone in the timing
+    slots; FP64: the two pair launches), and the state pointer follows the
+    array the step wrote."""
+    capi = _capi()
+    shape = (40, 24, 36)
+    s0 = random_state(shape, seed=55 + precision)
+    bg = _bg(capi)
+    dt, nsteps = 0.1, 5
+    ref = capi.Level(ctx, shape, 0.5, precision)
+    ref.upload(s0)
+    ref.advance(bg, 1.0, dt, nsteps, use_graph=graph)
+    want = ref.download_raw()
+    ref.close()
+    one = capi.Level(ctx, shape, 0.5, precision)
+    one.upload(s0)
+    ptrs = set()
+    tau = 1.0  # accumulated as axs_rk4_advance accumulates it
+    for i in range(nsteps):
+        one.advance(bg, tau, dt, 1, use_graph=graph)
+        ptrs.add(one.device_ptrs()[0])
+        tau += dt
+    _same(one.download_raw(), want)
+    assert len(ptrs) == 2  # the state alternates between two arrays
+    # rk4_step with the same stage scalars
+    stp = capi.Level(ctx, shape, 0.5, precision)
+    stp.upload(s0)
+    tau = 1.0
+    for i in range(nsteps):
+        coeffs = [capi.rhs_coeffs(bg, t)
+                  for t in (tau, tau + 0.5 * dt, tau + 0.5 * dt, tau + dt)]
+        stp.rk4_step(coeffs, dt)
+        tau += dt
+    _same(stp.download_raw(), want)
+    stp.enable_timing(True)
+    stp.stage_times()
+    stp.advance(bg, 1.5, dt, 1)
+    ms, n = stp.stage_times()
+    assert n == 1 and ms[0] > 0 and ms[2] == 0 and ms[3] == 0
+    assert (ms[1] > 0) == (precision == capi.AXS_FP64)
+    stp.close()
+    one.close()
