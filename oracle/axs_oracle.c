@@ -18,6 +18,12 @@
  *     4th-order convergence, homogeneous exact solution).
  *   FFT-based spectrum / IC: pinned by Parseval, single-mode and brute-force
  *     DFT checks; the C2R of non-Hermitian input is projected explicitly.
+ *   Grid-level energy sums (all 14 slots, every masking scheme) and the
+ *     tagger's Laplacian and radial-gradient criteria: pinned by closed
+ *     forms on fields whose discrete derivatives are known exactly (a
+ *     rotating plane wave; a modulus varying along x with a phase along y),
+ *     each with a check that a plausible slip (scaling, sqrt, dropped
+ *     term, slot order) gives a different answer.
  */
 #include "axs_oracle.h"
 

@@ -685,6 +685,134 @@ def test_energy_sums_plane_wave_gradient(oracle):
     assert pot == 0.0
 
 
+def _rotating_wave(n, A, m, omega):
+    """psi = A (cos kx, sin kx), Pi = i omega psi = omega A (-sin, cos), k =
+    2 pi m / n per cell, constant along y and z."""
+    t = 2 * np.pi * m / n
+    ph = t * np.arange(n)
+    s = np.zeros((4, n, n, n))
+    s[0] = (A * np.cos(ph))[None, None, :]
+    s[1] = (A * np.sin(ph))[None, None, :]
+    s[2] = -omega * s[1]
+    s[3] = omega * s[0]
+    return s, t
+
+
+def test_energy_sums_rotating_plane_wave_every_slot(oracle):
+    """All 14 sums in TotalEnergySums order (EnergyKernel.hpp:85-101), in
+    closed form (Energy.hpp:64-189) on psi = A e^{ikx}, Pi = i omega psi:
+    theta' = omega, grad theta = (g, 0, 0) with the 4th-order first-
+    derivative symbol g = (8 sin t - sin 2t)/(6 dx), |grad psi|^2 = A^2 g^2,
+    psi . Pi = 0.  Per cell, with f_a^2 / 2 = 1:
+      rho       = (omega^2 A^2 + (R'/R)^2 A^2 + A^2 g^2
+                   + lambda/4 (A^2 - R^2)^2) / R^4
+      axion kin = omega^2 / R^2,       axion grad = g^2 / R^2
+      radial kin = (R'/R)^2 A^2 / R^4, radial grad = 0,
+      radial mass = lambda/4 (A^2 - R^2)^2 / R^4;
+    the masked slot of each is the unmasked one times the masking weight
+    (scheme B top-hat on |psi|/R = A/R, scheme A (1 + (A/R - 1)/sqrt 2)^2),
+    slot 12 sums the weight and slot 13 counts the cells."""
+    n, dx, A, m, omega = 16, 0.5, 1.3, 2, 0.7
+    R, lam, RpR = 1.1, 0.9, 0.35
+    s, t = _rotating_wave(n, A, m, omega)
+    g = (8 * np.sin(t) - np.sin(2 * t)) / (6 * dx)
+    N = n ** 3
+    R4 = R ** 4
+    pot = 0.25 * lam * (A * A - R * R) ** 2
+    vals = [(omega ** 2 * A ** 2 + RpR ** 2 * A ** 2 + A * A * g * g + pot) / R4,
+            omega ** 2 / R ** 2,
+            g * g / R ** 2,
+            RpR ** 2 * A * A / R4,
+            0.0,
+            pot / R4]
+    wA = (1.0 + (A / R - 1.0) / math.sqrt(2.0)) ** 2
+    for scheme, thr, w in [(0, 0.8, 1.0), (2, 0.8, 1.0), (2, 1.5, 0.0),
+                           (1, 0.8, wA)]:
+        e = oracle.energy_sums(s, dx, R, lam, RpR, scheme=scheme, thr=thr)
+        for k, v in enumerate(vals):
+            scale = N * max(abs(x) for x in vals)
+            assert e[2 * k] == pytest.approx(N * v, rel=1e-12,
+                                             abs=1e-12 * scale), k
+            assert e[2 * k + 1] == pytest.approx(N * v * w, rel=1e-12,
+                                                 abs=1e-12 * scale), k
+        assert e[12] == pytest.approx(N * w, rel=1e-13)
+        assert e[13] == N
+
+
+def _distinct_midpoint(v, rank=1):
+    """A threshold halfway between the rank-th and (rank+1)-th distinct
+    values of v (so that no cell sits on it)."""
+    u = np.unique(np.round(v, 12))
+    return 0.5 * (u[rank] + u[rank + 1])
+
+
+def test_tagger_laplacian_threshold_closed_form(oracle):
+    """StringTagger.hpp:130-141: a cell is tagged where dx^2 |lap psi_i| >
+    threshold.  psi1 = B + A cos(kx) > 0, psi2 = 0 (no windings): dx^2 |lap
+    psi1| = dx^2 A |s(k)| |cos(k x)| with the diff2 symbol s(k) = (-2 cos 2t
+    + 32 cos t - 30)/(12 dx^2); the tags follow that closed form at
+    thresholds between its distinct values, and a dx (not dx^2) scaling or a
+    missing |.| would tag other cells."""
+    n, dx, A, B, m = 16, 0.5, 0.4, 1.0, 3
+    t = 2 * np.pi * m / n
+    x = np.arange(n)
+    s = np.zeros((4, n, n, n))
+    s[0] = (B + A * np.cos(t * x))[None, None, :]
+    sym = (-2 * np.cos(2 * t) + 32 * np.cos(t) - 30) / (12 * dx * dx)
+    v = dx * dx * A * abs(sym) * np.abs(np.cos(t * x))
+    big = np.finfo(float).max
+    caught = {"dx": False, "signed": False}
+    for rank in (0, 1, 2):
+        thr = _distinct_midpoint(v, rank)
+        want = np.broadcast_to((v > thr)[None, None, :], (n, n, n))
+        got = oracle.tag_cells(s, dx, thr, big)
+        np.testing.assert_array_equal(got, want.astype(np.uint8))
+        assert 0 < want.sum() < n ** 3
+        wrong = dx * A * abs(sym) * np.abs(np.cos(t * x)) > thr
+        caught["dx"] |= not np.array_equal(wrong, v > thr)
+        signed = dx * dx * A * sym * np.cos(t * x) > thr
+        caught["signed"] |= not np.array_equal(signed, v > thr)
+    assert all(caught.values()), caught
+
+
+def test_tagger_radial_gradient_closed_form(oracle):
+    """StringTagger.hpp:143-150, 182-216: a cell is tagged where
+    dx sqrt(|grad psi1|^2 + |grad psi2|^2 - |psi|^2 |grad theta|^2) >
+    threshold.  psi = rho(x) (cos qy, sin qy), rho = B + A cos(kx) > 0: the
+    modulus varies along x, the phase along y (no windings).  With the
+    discrete first-derivative symbols g(k), g(q) the gradient terms are
+    A^2 g(k)^2 sin^2(kx) + rho^2 g(q)^2 and rho^2 |grad theta|^2 =
+    rho^2 g(q)^2, so the criterion is dx A g(k) |sin(kx)| > threshold; a
+    missing sqrt, a dx^2 scaling or a dropped phase term would tag other
+    cells."""
+    n, dx, A, B, m, mq = 16, 0.5, 0.4, 1.0, 3, 1
+    t, tq = 2 * np.pi * m / n, 2 * np.pi * mq / n
+    i = np.arange(n)
+    rho = B + A * np.cos(t * i)
+    s = np.zeros((4, n, n, n))
+    s[0] = rho[None, None, :] * np.cos(tq * i)[None, :, None]
+    s[1] = rho[None, None, :] * np.sin(tq * i)[None, :, None]
+    assert oracle.count_plaquettes(s) == (0, 0)
+    gk = (8 * np.sin(t) - np.sin(2 * t)) / (6 * dx)
+    gq = (8 * np.sin(tq) - np.sin(2 * tq)) / (6 * dx)
+    v = dx * A * abs(gk) * np.abs(np.sin(t * i))
+    big = np.finfo(float).max
+    caught = {"sqrt": False, "dx2": False, "phase": False}
+    for rank in (0, 1, 2):
+        thr = _distinct_midpoint(v, rank)
+        want = np.broadcast_to((v > thr)[None, None, :], (n, n, n))
+        got = oracle.tag_cells(s, dx, big, thr)
+        np.testing.assert_array_equal(got, want.astype(np.uint8))
+        assert 0 < want.sum() < n ** 3
+        no_sqrt = dx * (A * gk * np.sin(t * i)) ** 2 > thr
+        caught["sqrt"] |= not np.array_equal(no_sqrt, v > thr)
+        caught["dx2"] |= not np.array_equal(dx * v > thr, v > thr)
+        no_phase = dx * np.sqrt((A * gk * np.sin(t * i)) ** 2
+                                + (rho * gq) ** 2) > thr
+        caught["phase"] |= not np.array_equal(no_phase, v > thr)
+    assert all(caught.values()), caught
+
+
 # ----------------------------------------------------------------- spectrum
 def test_fft_matches_brute_force_dft(oracle):
     rng = np.random.default_rng(8)
