@@ -67,6 +67,7 @@ struct axs_boxlevel
     axs::WorkItem *ditems;
     int nitems;
     double *dcoeff;
+    double *h_coeff; // pinned staging of the stage scalars (12)
     void *snap;      // coarse snapshot (y [+ comp]) for time interpolation
     float *snap_lo;
     size_t snap_bytes; // bytes in use
@@ -1303,6 +1304,7 @@ int axs_boxlevel_create(axs_ctx *ctx, int nboxes, const int *lo, const int *hi,
               hipMalloc(&b->ditems, items.size() * sizeof(WorkItem)) ==
                   hipSuccess &&
               hipMalloc(&b->dcoeff, 12 * sizeof(double)) == hipSuccess &&
+              hipHostMalloc(&b->h_coeff, 12 * sizeof(double)) == hipSuccess &&
               hipMalloc(&b->derr, 4 * sizeof(unsigned long long)) == hipSuccess;
     if (ok && precision == AXS_FP32C)
     {
@@ -1317,6 +1319,7 @@ int axs_boxlevel_create(axs_ctx *ctx, int nboxes, const int *lo, const int *hi,
         (void)hipFree(b->downer);
         (void)hipFree(b->ditems);
         (void)hipFree(b->dcoeff);
+        (void)hipHostFree(b->h_coeff);
         (void)hipFree(b->derr);
         delete b;
         return set_error(AXS_ERR_NOMEM, "axs_boxlevel_create: out of memory");
@@ -1360,6 +1363,7 @@ int axs_boxlevel_destroy(axs_boxlevel *b)
     (void)hipFree(b->downer);
     (void)hipFree(b->ditems);
     (void)hipFree(b->dcoeff);
+    (void)hipHostFree(b->h_coeff);
     (void)hipFree(b->derr);
     pool_release(b->ctx, b->snap, b->snap_cap);
     delete b;
@@ -1571,6 +1575,27 @@ int axs_boxlevel_fill_ghosts(axs_boxlevel *b, const axs_boxlevel *coarse,
     return check_err(b, "axs_boxlevel_fill_ghosts");
 }
 
+} // extern "C"
+
+// Stage scalars into the level's device table through its pinned staging
+// buffer: an asynchronous copy, ordered before the stage launches on the
+// stream.  Every step call ends with a stream synchronisation (the error
+// count read-back), so the staging buffer is free again by the next call.
+static int upload_stage_scalars(axs_boxlevel *b, const axs_rhs_coeffs coeffs[4])
+{
+    for (int s = 0; s < 4; ++s)
+    {
+        b->h_coeff[3 * s] = coeffs[s].curvature_coeff;
+        b->h_coeff[3 * s + 1] = coeffs[s].lambda;
+        b->h_coeff[3 * s + 2] = coeffs[s].R_squared;
+    }
+    AXS_HIP_TRY(hipMemcpyAsync(b->dcoeff, b->h_coeff, 12 * sizeof(double),
+                               hipMemcpyHostToDevice, b->ctx->stream));
+    return AXS_OK;
+}
+
+extern "C" {
+
 int axs_boxlevel_rk4_step(axs_boxlevel *b, const axs_rhs_coeffs coeffs[4],
                           double dt, const axs_boxlevel *coarse, double t0,
                           double df)
@@ -1581,16 +1606,10 @@ int axs_boxlevel_rk4_step(axs_boxlevel *b, const axs_rhs_coeffs coeffs[4],
             AXS_CHECK_ARG(coarse->dom[d] * 2 == b->dom[d] &&
                               coarse->precision == b->precision,
                           "axs_boxlevel_rk4_step: coarse level mismatch");
-    double h[12];
-    for (int s = 0; s < 4; ++s)
-    {
-        h[3 * s] = coeffs[s].curvature_coeff;
-        h[3 * s + 1] = coeffs[s].lambda;
-        h[3 * s + 2] = coeffs[s].R_squared;
-    }
     hipStream_t s = b->ctx->stream;
-    AXS_HIP_TRY(hipMemcpyAsync(b->dcoeff, h, sizeof(h), hipMemcpyHostToDevice, s));
-    AXS_HIP_TRY(hipStreamSynchronize(s));
+    int rc0 = upload_stage_scalars(b, coeffs);
+    if (rc0)
+        return rc0;
     hipLaunchKernelGGL(k_zero_err, dim3(1), dim3(64), 0, s, b->derr);
     const double tf[4] = {t0, t0 + 0.5 * df, t0 + 0.5 * df, t0 + df};
     void *in[4] = {b->y, b->ya, b->yb, b->ya};
@@ -1616,16 +1635,10 @@ int axs_boxlevel_rk4_step_graph(axs_boxlevel *b, const axs_rhs_coeffs coeffs[4],
             AXS_CHECK_ARG(coarse->dom[d] * 2 == b->dom[d] &&
                               coarse->precision == b->precision,
                           "axs_boxlevel_rk4_step_graph: coarse level mismatch");
-    double h[12];
-    for (int s = 0; s < 4; ++s)
-    {
-        h[3 * s] = coeffs[s].curvature_coeff;
-        h[3 * s + 1] = coeffs[s].lambda;
-        h[3 * s + 2] = coeffs[s].R_squared;
-    }
     hipStream_t s = b->ctx->stream;
-    AXS_HIP_TRY(hipMemcpyAsync(b->dcoeff, h, sizeof(h), hipMemcpyHostToDevice, s));
-    AXS_HIP_TRY(hipStreamSynchronize(s));
+    int rc0 = upload_stage_scalars(b, coeffs);
+    if (rc0)
+        return rc0;
     axs_boxlevel::GraphEntry *hit = nullptr;
     const unsigned long long cuid = coarse ? coarse->uid : 0ull;
     for (auto &e : b->graphs)
@@ -1664,8 +1677,13 @@ int axs_boxlevel_rk4_step_graph(axs_boxlevel *b, const axs_rhs_coeffs coeffs[4],
         b->graphs.push_back({dt, t0, df, cuid, g, ge});
         hit = &b->graphs.back();
     }
+    // the ghost fills of the replay count cells without an owning box in
+    // derr, cleared by a kernel before the replay (not a memset node) and
+    // read back after it, as the direct path does
+    hipLaunchKernelGGL(k_zero_err, dim3(1), dim3(64), 0, s, b->derr);
+    AXS_HIP_TRY(hipGetLastError());
     AXS_HIP_TRY(hipGraphLaunch(hit->ge, s));
-    return AXS_OK;
+    return check_err(b, "axs_boxlevel_rk4_step_graph");
 }
 
 int axs_boxlevel_tag(axs_boxlevel *b, const axs_tagger_params *p,

@@ -747,13 +747,17 @@ int launch_rk_pair(axs_level *L, int pair, const void *yin, const void *yn,
         base = 2;
     if (f64 && base == 5)
         base = 1;
-    // accumulator-free pairs: the stages 3+4 kernel holds four star fields,
-    // its 8 x 2, 4 x 4 and 16 x 1 tiles spill (and mode 2 in 4 x 4 tiles with
-    // prefetch gave wrong results in a GPU test, cause not found), so those
-    // run as 4 x 2 / 8 x 1, and the stages 1+2 kernel's 4 x 4 and 16 x 1 too
+    // accumulator-free pairs in FP32 / FP32C 4 x 4 tiles are not launched:
+    // one GPU test of them (with the centre-load prefetch) gave wrong
+    // results and the cause has not been shown
+    if (pair >= 3 && base == 4 && !f64)
+        return set_error(AXS_ERR_UNSUPPORTED,
+                         "fused pairs without an accumulator (stage fusion 2) "
+                         "do not run in 4 x 4 tiles (pair variants 4 + 6 g)");
+    // the stages 3+4 kernel of those pairs holds four star fields: its 8 x 2
+    // and 16 x 1 tiles spill, so they run as 4 x 2 / 8 x 1, and the stages
+    // 1+2 kernel's 16 x 1 too (FP64 4 x 4 runs as 4 x 2, above)
     if (pair == 4 && base == 3)
-        base = 2;
-    if (pair >= 3 && base == 4)
         base = 2;
     if (pair >= 3 && base == 5)
         base = 1;

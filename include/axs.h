@@ -367,7 +367,8 @@ int axs_boxlevel_rk4_step(axs_boxlevel *bl, const axs_rhs_coeffs coeffs[4],
 /* The same step replayed from a hipGraph captured on first use for this
  * (dt, coarse, t0_frac, dt_frac): the 4 ghost fills and 4 work-list stage
  * launches are one graph launch; the stage scalars are copied to the device
- * before it.  Cells without an owner are not reported in this mode. */
+ * before it, and cells without an owning box are reported after it, as by
+ * axs_boxlevel_rk4_step (AXS_ERR_ARG). */
 int axs_boxlevel_rk4_step_graph(axs_boxlevel *bl,
                                 const axs_rhs_coeffs coeffs[4], double dt,
                                 const axs_boxlevel *coarse, double t0_frac,

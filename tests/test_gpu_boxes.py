@@ -109,6 +109,27 @@ def test_unnested_fill_is_reported(ctx):
         fine.fill_ghosts()
 
 
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_unnested_step_is_reported(ctx, use_graph):
+    """A box level that does not tile its domain, stepped without a coarse
+    level: the ghost fills find cells with no owning box, and both the
+    direct and the graph-replayed step return AXS_ERR_ARG (replayed twice:
+    the count is cleared before every replay), while a properly tiled level
+    steps cleanly on the same path."""
+    capi = _capi()
+    coeffs = [capi.rhs_coeffs(capi.make_background(2.0), t)
+              for t in (1.0, 1.05, 1.05, 1.1)]
+    fine = capi.BoxLevel(ctx, [((0, 0, 0), (15, 15, 15))], (32, 32, 32), 0.5)
+    for _ in range(2):
+        with pytest.raises(capi.AxsError, match="no owning box"):
+            fine.rk4_step(coeffs, 0.1, use_graph=use_graph)
+    fine.close()
+    ok = capi.BoxLevel(ctx, tiling(32, [16]), (32, 32, 32), 0.5)
+    for _ in range(2):
+        ok.rk4_step(coeffs, 0.1, use_graph=use_graph)
+    ok.close()
+
+
 def test_box_tag_and_energy_with_coverage(ctx, oracle):
     capi = _capi()
     n = 32

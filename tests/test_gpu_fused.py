@@ -67,6 +67,13 @@ def test_fused_equals_per_stage_shapes(ctx, shape, precision):
 def test_pair_variants_identical(ctx, variant, zc, precision, mode):
     shape = (256, 24, 20)  # several x tiles at every cells-per-lane width
     s0 = random_state(shape, seed=5)
+    if mode == 2 and precision != 0 and variant % 6 == 4:
+        # FP32 / FP32C pairs without an accumulator in 4 x 4 tiles are
+        # refused: a GPU test of that tile gave wrong results, cause not
+        # shown, so it is not launched
+        with pytest.raises(_capi().AxsError, match="4 x 4"):
+            _run(ctx, shape, precision, s0, 4, True, mode, variant, zc)
+        return
     ref = _run(ctx, shape, precision, s0, 4, False, 0)
     got = _run(ctx, shape, precision, s0, 4, True, mode, variant, zc)
     _same(got, ref)
