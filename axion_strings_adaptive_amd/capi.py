@@ -20,6 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "libaxs_hip.so")
 
 AXS_OK = 0
+AXS_ERR_ARG, AXS_ERR_HIP, AXS_ERR_NOMEM, AXS_ERR_UNSUPPORTED = 1, 2, 3, 4
 AXS_FP64, AXS_FP32C, AXS_FP32 = 0, 1, 2
 AXS_MASK_NONE, AXS_MASK_A, AXS_MASK_B = 0, 1, 2
 AXS_FIELD_MASKED_A_DOT, AXS_FIELD_PI1, AXS_FIELD_PI2 = 0, 1, 2
@@ -96,6 +97,8 @@ _SIGS = {
     "axs_windings_field": (_I, [_P, _P]),
     "axs_tag_cells": (_I, [_P, C.POINTER(TaggerParams), _P, _PI64]),
     "axs_energy_sums": (_I, [_P, _D, _D, _D, C.POINTER(Masking), _P, _PD]),
+    "axs_energy_sums_noalloc": (_I, [_P, _P, _I, _I, _I, _I, _D, _D, _D, _D,
+                                     C.POINTER(Masking), _P, _P, _P]),
     "axs_fill_field": (_I, [_P, _I, C.POINTER(Masking), _D, _P]),
     "axs_level_spectrum": (_I, [_P, _I, C.POINTER(Masking), _D, _PD, _PI64,
                                 _PD, _PD]),

@@ -17,6 +17,8 @@
 
 #include <cfloat>
 #include <cmath>
+#include <mutex>
+#include <string>
 #include <type_traits>
 #include <vector>
 
@@ -1454,20 +1456,21 @@ int axs_tag_cells(axs_level *L, const axs_tagger_params *tp, uint8_t *tags,
     return rc;
 }
 
-int axs_energy_sums(axs_level *L, double R, double lambda, double RpR,
-                    const axs_masking *m, const uint8_t *mask, double out[14])
+} // extern "C"
+
+// The energy pass's launch parameters for an nx x ny x nz periodic grid;
+// *nblocks receives the grid size (one partial row of 14 per block).
+static EnergyParams energy_params(int nx, int ny, int nz, double dx, double R,
+                                  double lambda, double RpR,
+                                  const axs_masking *m, const uint8_t *mask,
+                                  long *nblocks)
 {
-    AXS_CHECK_ARG(L && m && out, "axs_energy_sums: NULL argument");
-    AXS_CHECK_ARG(R > 0.0, "axs_energy_sums: R must be > 0");
-    AXS_CHECK_ARG(m->scheme >= 0 && m->scheme <= 2,
-                  "axs_energy_sums: unknown masking scheme");
-    hipStream_t s = L->ctx->stream;
     EnergyParams p{};
-    p.nx = L->g.nx;
-    p.ny = L->g.ny;
-    p.nz = L->g.nz;
-    p.n = L->g.ncell;
-    p.dx = L->dx;
+    p.nx = nx;
+    p.ny = ny;
+    p.nz = nz;
+    p.n = (uint32_t)((long long)nx * ny * nz);
+    p.dx = dx;
     p.R = R;
     p.lambda = lambda;
     p.RpR = RpR;
@@ -1477,7 +1480,6 @@ int axs_energy_sums(axs_level *L, double R, double lambda, double RpR,
     p.scheme = m->scheme;
     p.thr = m->threshold;
     p.mask = mask;
-    p.partials = L->ctx->d_partials;
     p.tiles_x = (p.nx + 63) / 64;
     p.tiles_y = (p.ny + ETY - 1) / ETY;
     // z chunk: enough tiles to fill the device, at most kMaxPartials rows
@@ -1491,28 +1493,133 @@ int axs_energy_sums(axs_level *L, double R, double lambda, double RpR,
         zc = p.nz;
     p.z_chunk = zc;
     p.tiles_z = (p.nz + zc - 1) / zc;
-    const long nbl = (long)p.tiles_x * p.tiles_y * p.tiles_z;
-    AXS_CHECK_ARG(nbl <= kMaxPartials, "axs_energy_sums: grid too large");
-    const int nb = (int)nbl;
-    if (L->precision == AXS_FP64)
+    *nblocks = (long)p.tiles_x * p.tiles_y * p.tiles_z;
+    return p;
+}
+
+// The energy pass and the fixed-order reduction of its partials into
+// result[14] (device), on stream s.
+static int launch_energy(EnergyParams p, int precision, const void *y,
+                         const float *comp, double *partials, int nb,
+                         double *result, hipStream_t s)
+{
+    p.partials = partials;
+    if (precision == AXS_FP64)
         hipLaunchKernelGGL((k_energy<double, false>), dim3(nb), dim3(64, ETY), 0,
-                           s, (const double *)L->y, (const float *)nullptr, p);
-    else if (L->precision == AXS_FP32C)
+                           s, (const double *)y, (const float *)nullptr, p);
+    else if (precision == AXS_FP32C)
         hipLaunchKernelGGL((k_energy<float, true>), dim3(nb), dim3(64, ETY), 0,
-                           s, (const float *)L->y, (const float *)L->comp, p);
+                           s, (const float *)y, comp, p);
     else
         hipLaunchKernelGGL((k_energy<float, false>), dim3(nb), dim3(64, ETY), 0,
-                           s, (const float *)L->y, (const float *)nullptr, p);
+                           s, (const float *)y, (const float *)nullptr, p);
     AXS_HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_sum_partials, dim3(14), dim3(256), 0, s, p.partials,
-                       nb, 14, L->ctx->d_result);
+    hipLaunchKernelGGL(k_sum_partials, dim3(14), dim3(256), 0, s, partials, nb,
+                       14, result);
     AXS_HIP_TRY(hipGetLastError());
+    return AXS_OK;
+}
+
+// Reduction partials of axs_energy_sums_noalloc: one row of 14 per block of
+// the energy pass, at most kNoAllocEnergyBlocks blocks (32768: a 1024^3
+// grid).  Every block writes its whole row before the reduction reads it,
+// so the array needs no clearing between calls.  Host code never uses this
+// name as an address: it takes the device address from
+// hipGetSymbolAddress, once per device (scratch_on_device).
+constexpr long kNoAllocEnergyBlocks = 32768;
+static __device__ __align__(256) double scratch[kNoAllocEnergyBlocks * 14];
+
+static int scratch_on_device(double **out)
+{
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static double *ptr[kMaxDevices] = {};
+    int dev = -1;
+    AXS_HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices)
+        return set_error(AXS_ERR_UNSUPPORTED,
+                         "axs_energy_sums_noalloc: device index above 63");
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptr[dev])
+    {
+        void *p = nullptr;
+        // with device dev current: its own copy of the array
+        AXS_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p)
+            return set_error(AXS_ERR_HIP,
+                             "axs_energy_sums_noalloc: no address for the "
+                             "scratch array");
+        ptr[dev] = static_cast<double *>(p);
+    }
+    *out = ptr[dev];
+    return AXS_OK;
+}
+
+extern "C" {
+
+int axs_energy_sums(axs_level *L, double R, double lambda, double RpR,
+                    const axs_masking *m, const uint8_t *mask, double out[14])
+{
+    AXS_CHECK_ARG(L && m && out, "axs_energy_sums: NULL argument");
+    AXS_CHECK_ARG(R > 0.0, "axs_energy_sums: R must be > 0");
+    AXS_CHECK_ARG(m->scheme >= 0 && m->scheme <= 2,
+                  "axs_energy_sums: unknown masking scheme");
+    hipStream_t s = L->ctx->stream;
+    long nbl = 0;
+    const EnergyParams p = energy_params(L->g.nx, L->g.ny, L->g.nz, L->dx, R,
+                                         lambda, RpR, m, mask, &nbl);
+    AXS_CHECK_ARG(nbl <= kMaxPartials, "axs_energy_sums: grid too large");
+    int rc = launch_energy(p, L->precision, L->y, L->comp, L->ctx->d_partials,
+                           (int)nbl, L->ctx->d_result, s);
+    if (rc)
+        return rc;
     AXS_HIP_TRY(hipMemcpyAsync(L->ctx->h_result, L->ctx->d_result,
                                14 * sizeof(double), hipMemcpyDeviceToHost, s));
     AXS_HIP_TRY(hipStreamSynchronize(s));
     for (int q = 0; q < 14; ++q)
         out[q] = L->ctx->h_result[q];
     return AXS_OK;
+}
+
+int axs_energy_sums_noalloc(const void *state, const float *comp, int nx,
+                            int ny, int nz, int precision, double dx, double R,
+                            double lambda, double R_prime_over_R,
+                            const axs_masking *m, const uint8_t *mask,
+                            double *out_dev, void *stream)
+{
+    // every argument and the size limit are checked before any HIP call
+    AXS_CHECK_ARG(m, "axs_energy_sums_noalloc: NULL masking");
+    AXS_CHECK_ARG(precision == AXS_FP64 || precision == AXS_FP32C ||
+                      precision == AXS_FP32,
+                  "axs_energy_sums_noalloc: unknown precision");
+    AXS_CHECK_ARG(nx >= 4 && ny >= 4 && nz >= 4,
+                  "axs_energy_sums_noalloc: every dimension must be >= 4");
+    AXS_CHECK_ARG((long long)nx * ny * nz * 4 <= (1LL << 32),
+                  "axs_energy_sums_noalloc: 4*nx*ny*nz must be <= 2^32");
+    AXS_CHECK_ARG(dx > 0.0 && R > 0.0,
+                  "axs_energy_sums_noalloc: dx and R must be > 0");
+    AXS_CHECK_ARG(m->scheme >= 0 && m->scheme <= 2,
+                  "axs_energy_sums_noalloc: unknown masking scheme");
+    long nbl = 0;
+    const EnergyParams p = energy_params(nx, ny, nz, dx, R, lambda,
+                                         R_prime_over_R, m, mask, &nbl);
+    if (nbl > kNoAllocEnergyBlocks)
+        return set_error(AXS_ERR_ARG,
+                         "axs_energy_sums_noalloc: the grid needs " +
+                             std::to_string(nbl) +
+                             " energy blocks, above this entry point's fixed "
+                             "scratch of 32768 (a 1024^3 grid); use "
+                             "axs_energy_sums");
+    AXS_CHECK_ARG(state && out_dev, "axs_energy_sums_noalloc: NULL pointer");
+    AXS_CHECK_ARG((precision == AXS_FP32C) == (comp != nullptr),
+                  "axs_energy_sums_noalloc: comp is required for FP32C and "
+                  "only for it");
+    double *part = nullptr;
+    int rc = scratch_on_device(&part);
+    if (rc)
+        return rc;
+    return launch_energy(p, precision, state, comp, part, (int)nbl, out_dev,
+                         static_cast<hipStream_t>(stream));
 }
 
 int axs_fill_field(axs_level *L, int kind, const axs_masking *m, double R,

@@ -3,9 +3,13 @@
  * axion-string simulator's per-step hot path.
  *
  * Every entry point returns an int status (AXS_OK = 0) and never throws.
- * The text of the last error is available from axs_last_error().  No entry
- * point allocates device memory except the *_create calls and
- * axs_level_spectrum's one-time FFT plan creation.
+ * The text of the last error is available from axs_last_error().  Device
+ * memory is allocated by the *_create calls, by axs_level_spectrum and
+ * axs_level_fourier_ic on their first use (the FFT plans), by
+ * axs_level_place (trial placements of the level's arrays), by
+ * axs_adjoint_grad_energy and axs_forward_grad_energy (their checkpoint and
+ * adjoint arrays, per call), and by box levels (their arrays, from a
+ * context pool); axs_energy_sums_noalloc allocates nothing.
  *
  * Data model.  A level (axs_level) is one uniform, periodic grid of
  * nx*ny*nz cells with spacing dx that owns its state in HBM, structure of
@@ -284,6 +288,26 @@ int axs_tag_cells(axs_level *lvl, const axs_tagger_params *p, uint8_t *tags,
 int axs_energy_sums(axs_level *lvl, double R, double lambda,
                     double R_prime_over_R, const axs_masking *m,
                     const uint8_t *mask, double out[14]);
+
+/* compute_total_energy_sums over caller-owned device state, allocating
+ * nothing.  No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under hipGraph capture, tight serving loops,
+ * harnesses that forbid it).  Same kernels, sums and order as
+ * axs_energy_sums, hence the same bits: state is 4*nx*ny*nz words of the
+ * precision's type in the level layout, comp the FP32C low words (NULL
+ * otherwise), mask an optional device uint8 per cell.  The 14 sums are
+ * written to out_dev (device, 14 doubles) on `stream` (a hipStream_t; NULL
+ * = the default stream); the call returns without synchronising.  The
+ * reduction partials live in a fixed scratch array of the library (3.7 MB
+ * per GPU, held until the process ends): grids needing more than 32768
+ * energy blocks (beyond 1024^3) are refused with AXS_ERR_ARG before any GPU
+ * work.  Calls that overlap on one GPU share that array and race: order
+ * them on one stream. */
+int axs_energy_sums_noalloc(const void *state, const float *comp, int nx,
+                            int ny, int nz, int precision, double dx, double R,
+                            double lambda, double R_prime_over_R,
+                            const axs_masking *m, const uint8_t *mask,
+                            double *out_dev, void *stream);
 
 /* ------------------------- spectrum / masking ----------------------- */
 /* fill_masked_a_dot_buffer (MaskedFieldBuffer.hpp:15-35) and the Pi1/Pi2
