@@ -350,21 +350,25 @@ __global__ void __launch_bounds__((AL ? 68 : 64) * (TY + 4)) k_step(StepParams p
 // ---------------------------------------------------------------------
 // k_step_r: the same step, the same tile (64 x TY cells, rows aligned to
 // cache lines, one thread per point of the tile grown by 2, the 4 x-halo
-// columns on an extra wave) and the same expressions, with 27% fewer vector
-// instructions than k_step (6.3e9 against 8.6e9 per 1024^3 launch):
+// columns on an extra wave) and the same expressions, with 23% fewer vector
+// instructions than k_step (6.65e9 against 8.62e9 per 1024^3 launch):
 //  * every per-thread queue along z is a ring whose length divides 6 and
 //    the z loop is unrolled by 6, so that a ring index is a compile-time
 //    constant in every plane and no queue moves data;
 //  * psi_2, psi_3 of six planes stay in an LDS ring (their z neighbours are
 //    the thread's own earlier writes, so they need no barrier), psi_1 at
-//    the z neighbours is formed again from psi_n and Pi_n: registers go to
-//    loads in flight instead, two planes ahead for psi_n, Pi_n and the
-//    halo words, one plane ahead for the FP32C low words;
+//    the z neighbours is formed again from psi_n and Pi_n, and the grown
+//    tile's corner points load no y-halo (nothing formed there is read):
+//    117 registers, no spills;
 //  * loads and stores are buffer instructions on per-plane descriptors
 //    built in scalar registers: a thread's address is its 32-bit row
 //    offset, fixed for the whole launch, so no vector address arithmetic;
 //  * psi pairs live in LDS as float2 (one 64-bit access per star point),
 //    and the stage scalars are rounded to float once, in scalar registers.
+// Measured at 1024^3 on MI355X it runs at the speed of k_step (19 ms per
+// step, within the placement spread), with more HBM traffic (62-70 against
+// 42 B per cell read): its time per plane is not set by the vector
+// instructions it saved.
 // ---------------------------------------------------------------------
 // A uniform address kept in scalar registers (readfirstlane stops the
 // compiler from building it, or anything derived from it, in vector
@@ -403,9 +407,9 @@ __device__ __forceinline__ float uniform_f(float v)
         float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
 }
 
-// Loads run two planes ahead of their first use (psi_n, Pi_n of the own
-// point and the halo words) or one plane ahead (the FP32C low words).
-template <int PREC, int TY>
+// D: psi_n, Pi_n of the own point and the halo words are loaded D planes
+// before their first use (D = 1 or 2); the FP32C low words one plane ahead.
+template <int PREC, int TY, int D>
 __global__ void __launch_bounds__(68 * (TY + 4)) k_step_r(StepParams p)
 {
     constexpr int EXB = 68, EXA = 72;
@@ -452,22 +456,29 @@ __global__ void __launch_bounds__(68 * (TY + 4)) k_step_r(StepParams p)
     const uint32_t rowb = 4u * ((uint32_t)wrapy(gy) * nx + wrapx(gx));
     const bool interior = ec >= 2 && ec < 66 && er >= 2 && er < TY + 2;
     const bool active = interior && gx < nx && gy < ny;
-    // halo points of the plane grown by 4 (a plus shape: no corners); no
-    // interior thread has one
-    const bool hyl = er < 2, hyr = er >= TY + 2;
+    // halo points of the plane grown by 4 (a plus shape): the y-halo rows
+    // above and below the grown tile's columns 2..65, the x-halo columns
+    // beside its rows.  The grown tile's corner points (x-halo wave, top
+    // and bottom two rows) need no y-halo: what is formed there is never
+    // read, since every later star is a plus.  So each thread has at most
+    // one halo point (none for interior threads) at LDS (yh, xh), global
+    // byte offset rowhb.
     const bool hxl = ec < 2, hxr = ec >= 66;
-    const bool hy_on = hyl || hyr, hx_on = hxl || hxr;
-    uint32_t rowhyb = 0, rowhxb = 0;
-    int yah = 0, xah = 0;
+    const bool hx_on = hxl || hxr;
+    const bool hyl = er < 2, hyr = er >= TY + 2;
+    const bool hy_on = (hyl || hyr) && !hx_on;
+    const bool h_on = hx_on || hy_on;
+    uint32_t rowhb = 0;
+    int yh = er + 2, xh = ec + 2;
     if (hy_on)
     {
-        yah = hyl ? er : er + 4;
-        rowhyb = 4u * ((uint32_t)wrapy(Y0 - 4 + yah) * nx + wrapx(gx));
+        yh = hyl ? er : er + 4;
+        rowhb = 4u * ((uint32_t)wrapy(Y0 - 4 + yh) * nx + wrapx(gx));
     }
     if (hx_on)
     {
-        xah = hxl ? ec : ec + 4;
-        rowhxb = 4u * ((uint32_t)wrapy(gy) * nx + wrapx(X0 - 4 + xah));
+        xh = hxl ? ec : ec + 4;
+        rowhb = 4u * ((uint32_t)wrapy(gy) * nx + wrapx(X0 - 4 + xh));
     }
     const int ya = er + 2, xa = ec + 2;
 
@@ -485,9 +496,20 @@ __global__ void __launch_bounds__(68 * (TY + 4)) k_step_r(StepParams p)
         bo[c] = uniform_addr(p.yout + (size_t)c * n);
         bl[c] = PREC == AXS_FP32C ? uniform_addr(p.comp + (size_t)c * n) : 0;
     }
-    auto rs_n = [&](int c, uint32_t o) { return plane_rsrc(bn[c], o); };
-    auto rs_o = [&](int c, uint32_t o) { return plane_rsrc(bo[c], o); };
-    auto rs_l = [&](int c, uint32_t o) { return plane_rsrc(bl[c], o); };
+    // loads / stores of component c at plane offset o (cells), byte offset
+    // off within the plane
+    auto ld_n = [&](int c, uint32_t o, uint32_t off) {
+        return bload(plane_rsrc(bn[c], o), off);
+    };
+    auto ld_l = [&](int c, uint32_t o, uint32_t off) {
+        return bload(plane_rsrc(bl[c], o), off);
+    };
+    auto st_o = [&](int c, uint32_t o, uint32_t off, float v) {
+        bstore(plane_rsrc(bo[c], o), off, v);
+    };
+    auto st_l = [&](int c, uint32_t o, uint32_t off, float v) {
+        bstore(plane_rsrc(bl[c], o), off, v);
+    };
 
     // stage scalars of the four stages, rounded as force() rounds them
     const double *cfp = p.coeff + 12 * (p.step ? *p.step : 0);
@@ -510,41 +532,41 @@ __global__ void __launch_bounds__(68 * (TY + 4)) k_step_r(StepParams p)
     // Rings: plane q in slot (q - q0) % 6 (q0 = z_begin - 2) of N, P
     // (psi_n, Pi_n at q-2..q+2) and of the LDS ring sb; in slot
     // (q - q0) % 3 of FN, F1 (F(psi_n), F(psi_1) at q-2..q).  In flight:
-    // psi_n, Pi_n of the next two planes in staging slot T[(plane - q0) % 2],
-    // halo words of the next two planes in HY, HX[(plane - q0) % 2], the
+    // psi_n, Pi_n of the next D planes in staging slot T[(plane - q0) % D],
+    // halo words of the next D planes in HW[(plane - q0) % D], the
     // FP32C low words of plane z in LO[(z - z_begin) % 2]
+    static_assert(D == 1 || D == 2, "prefetch distance 1 or 2");
     float N[6][2], P[6][2], FN[3][2], F1[3][2];
-    float T[2][4], HY[2][4], HX[2][4], LO[2][4];
+    float T[D][4], HW[D][4], LO[2][4];
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            T[i][c] = HW[i][c] = 0.0f;
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int c = 0; c < 4; ++c)
-            T[i][c] = HY[i][c] = HX[i][c] = LO[i][c] = 0.0f;
+            LO[i][c] = 0.0f;
     const int q0 = z_begin - 2;
     auto load_own = [&](int z, float (&w)[4]) {
         const uint32_t o = zoff(z);
 #pragma unroll
         for (int c = 0; c < 4; ++c)
-            w[c] = bload(rs_n(c, o), rowb);
+            w[c] = ld_n(c, o, rowb);
     };
-    auto load_halo = [&](int z, float (&wy)[4], float (&wx)[4]) {
+    auto load_halo = [&](int z, float (&w)[4]) {
         const uint32_t o = zoff(z);
-        if (hy_on)
+        if (h_on)
         {
 #pragma unroll
             for (int c = 0; c < 4; ++c)
-                wy[c] = bload(rs_n(c, o), rowhyb);
-        }
-        if (hx_on)
-        {
-#pragma unroll
-            for (int c = 0; c < 4; ++c)
-                wx[c] = bload(rs_n(c, o), rowhxb);
+                w[c] = ld_n(c, o, rowhb);
         }
     };
-    // prologue: planes q0-2..q0+1 into N, P; q0+2, q0+3 in flight (staging
-    // slot (plane - q0) % 2); halos of q0 and q0+1 (slots 0, 1).  The low
-    // words of plane z_begin are loaded in the loop, one plane ahead.
+    // prologue: planes q0-2..q0+1 into N, P; q0+2..q0+1+D in flight
+    // (staging slot (plane - q0) % D); halos of q0..q0+D-1.  The low words
+    // of plane z_begin are loaded in the loop, one plane ahead.
 #pragma unroll
     for (int k = -2; k <= 1; ++k)
     {
@@ -556,10 +578,12 @@ __global__ void __launch_bounds__(68 * (TY + 4)) k_step_r(StepParams p)
         P[s][0] = w[2];
         P[s][1] = w[3];
     }
-    load_own(q0 + 2, T[0]);
-    load_own(q0 + 3, T[1]);
-    load_halo(q0, HY[0], HX[0]);
-    load_halo(q0 + 1, HY[1], HX[1]);
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+    {
+        load_own(q0 + 2 + i, T[i]);
+        load_halo(q0 + i, HW[i]);
+    }
 
     // one plane q, ring phase PH = (q - q0) % 6
     auto plane_step = [&]<int PH>(int q) {
@@ -573,8 +597,8 @@ __global__ void __launch_bounds__(68 * (TY + 4)) k_step_r(StepParams p)
         constexpr int t0 = PH % 3;        // plane q in FN, F1
         constexpr int tm2 = (PH + 1) % 3; // q - 2
         constexpr int ba = PH & 1;        // LDS buffer of plane q
-        constexpr int tq = PH & 1;        // T slot of planes q + 2 and q + 4
-        constexpr int hq = PH & 1;        // HY, HX slot of plane q (and q + 2)
+        constexpr int tq = PH % D;        // T slot of planes q + 2, q + 2 + D
+        constexpr int hq = PH % D;        // HW slot of planes q, q + D
         constexpr int lz = PH & 1;        // LO slot of plane q - 2
         constexpr int lz1 = (PH + 1) & 1; // LO slot of plane q - 1
         // the own point's LDS indices, taken through an empty register
@@ -600,23 +624,18 @@ __global__ void __launch_bounds__(68 * (TY + 4)) k_step_r(StepParams p)
         SA(ba, 0, 0, 0) = make_float2(N[s0][0], N[s0][1]);
         SA(ba, 1, 0, 0) = make_float2(psi1_of(N[s0][0], P[s0][0]),
                                       psi1_of(N[s0][1], P[s0][1]));
-        if (hy_on)
+        if (h_on)
         {
-            const float *h = HY[hq];
-            sa[((ba * 2 + 0) * HYA + yah) * EXA + xa] = make_float2(h[0], h[1]);
-            sa[((ba * 2 + 1) * HYA + yah) * EXA + xa] =
+            const float *h = HW[hq];
+            sa[((ba * 2 + 0) * HYA + yh) * EXA + xh] = make_float2(h[0], h[1]);
+            sa[((ba * 2 + 1) * HYA + yh) * EXA + xh] =
                 make_float2(psi1_of(h[0], h[2]), psi1_of(h[1], h[3]));
         }
-        if (hx_on)
-        {
-            const float *h = HX[hq];
-            sa[((ba * 2 + 0) * HYA + ya) * EXA + xah] = make_float2(h[0], h[1]);
-            sa[((ba * 2 + 1) * HYA + ya) * EXA + xah] =
-                make_float2(psi1_of(h[0], h[2]), psi1_of(h[1], h[3]));
-        }
-        // loads: own point of plane q + 4 into the slot just emptied, lo of
-        // plane q - 1
-        load_own(q + 4, T[tq]);
+        // loads: own point of plane q + 2 + D into the slot just emptied (D
+        // = 1; with D = 2 at the end of the plane, see there), lo of plane
+        // q - 1
+        if constexpr (D == 1)
+            load_own(q + 2 + D, T[tq]);
         const int z = q - 2; // plane of the final stage
         if constexpr (PREC == AXS_FP32C)
         {
@@ -626,7 +645,7 @@ __global__ void __launch_bounds__(68 * (TY + 4)) k_step_r(StepParams p)
                 const uint32_t ol = (uint32_t)zl * plane;
 #pragma unroll
                 for (int c = 0; c < 4; ++c)
-                    LO[lz1][c] = bload(rs_l(c, ol), rowb);
+                    LO[lz1][c] = ld_l(c, ol, rowb);
             }
         }
 
@@ -693,13 +712,15 @@ __global__ void __launch_bounds__(68 * (TY + 4)) k_step_r(StepParams p)
         {
             const uint32_t oz = (uint32_t)z * plane;
             float lov[4] = {LO[lz][0], LO[lz][1], LO[lz][2], LO[lz][3]};
-            // halo words are never used by an interior thread: mark them
-            // dead on this path so that their registers serve the stage
-#pragma unroll
-            for (int c = 0; c < 4; ++c)
+            if constexpr (D == 1)
             {
-                HY[hq][c] = __builtin_nondeterministic_value(HY[hq][c]);
-                HX[hq][c] = __builtin_nondeterministic_value(HX[hq][c]);
+                // halo words are never used by an interior thread: mark
+                // them dead on this path so that their registers serve the
+                // stage (with D = 2 they stay put: registers that may still
+                // be the targets of loads in flight would need a drain)
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    HW[hq][c] = __builtin_nondeterministic_value(HW[hq][c]);
             }
             float Fs2[2], Fs3[2];
             {
@@ -755,16 +776,25 @@ __global__ void __launch_bounds__(68 * (TY + 4)) k_step_r(StepParams p)
 #pragma unroll
             for (int c = 0; c < 4; ++c)
             {
-                bstore(rs_o(c, oz), rowb, y4[c]);
+                st_o(c, oz, rowb, y4[c]);
                 if constexpr (PREC == AXS_FP32C)
-                    bstore(rs_l(c, oz), rowb, lov[c]);
+                    st_l(c, oz, rowb, lov[c]);
             }
         }
         else
         {
-            // halos of plane q + 2 into the slot plane q used
-            load_halo(q + 2, HY[hq], HX[hq]);
+            // halos of plane q + D into the slot plane q used
+            load_halo(q + D, HW[hq]);
         }
+        // D = 2: the own point's loads are the last memory instructions of
+        // every plane on every path.  The wait before a loaded value is used
+        // counts the memory instructions issued after it, the least over
+        // all paths (stores, the low words and the halo words are issued on
+        // some paths only); with these loads last, two planes ahead, the
+        // waits at the top of a plane leave the newest loads in flight
+        // instead of draining every store and load
+        if constexpr (D == 2)
+            load_own(q + 2 + D, T[tq]);
     };
 
     const int q_end = z_end + 2;
@@ -816,7 +846,7 @@ int launch_rk_step(axs_level *L, const void *yin, void *yout,
     if (L->precision == AXS_FP64)
         return set_error(AXS_ERR_UNSUPPORTED,
                          "one launch per RK4 step is FP32 / FP32C only");
-    const bool ring = L->pair_variant < 0 || L->pair_variant >= 4;
+    const bool ring = L->pair_variant < 0 || L->pair_variant == 4;
     const bool al = ring || L->pair_variant == 3;
     const int TY =
         al ? 11 : (L->pair_variant == 1 ? 4 : (L->pair_variant == 0 ? 8 : 12));
@@ -861,12 +891,20 @@ int launch_rk_step(axs_level *L, const void *yin, void *yout,
     if (ring)
     {
         const dim3 block(68 * (11 + 4));
-        if (L->precision == AXS_FP32C)
-            hipLaunchKernelGGL((k_step_r<AXS_FP32C, 11>), dim3(ntiles), block,
-                               0, s, p);
+        // variant 4: loads two planes ahead; default: one
+        const bool far = L->pair_variant == 4;
+        if (L->precision == AXS_FP32C && far)
+            hipLaunchKernelGGL((k_step_r<AXS_FP32C, 11, 2>), dim3(ntiles),
+                               block, 0, s, p);
+        else if (L->precision == AXS_FP32C)
+            hipLaunchKernelGGL((k_step_r<AXS_FP32C, 11, 1>), dim3(ntiles),
+                               block, 0, s, p);
+        else if (far)
+            hipLaunchKernelGGL((k_step_r<AXS_FP32, 11, 2>), dim3(ntiles),
+                               block, 0, s, p);
         else
-            hipLaunchKernelGGL((k_step_r<AXS_FP32, 11>), dim3(ntiles), block,
-                               0, s, p);
+            hipLaunchKernelGGL((k_step_r<AXS_FP32, 11, 1>), dim3(ntiles),
+                               block, 0, s, p);
     }
     else if (al)
         launch_step_ty<11, true>(p, L->precision, ntiles, s);
