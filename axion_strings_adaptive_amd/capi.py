@@ -4,6 +4,11 @@ This is test/bench plumbing: it loads the in-tree HIP library, declares the
 C signatures and wraps a level in a small Python class.  It never computes
 anything itself and raises if the library is missing -- there is no CPU
 fallback.
+
+Loading the library also loads the autotuner's launch-shape table
+(tune_table.txt next to this file, or the file AXS_TUNING_TABLE names; set
+it to the empty string for the built-in shapes), so that levels created
+through this binding run the tuned tiles.
 """
 
 from __future__ import annotations
@@ -18,6 +23,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "libaxs_hip.so")
+TUNING_PATH = os.path.join(HERE, "tune_table.txt")
 
 AXS_OK = 0
 AXS_ERR_ARG, AXS_ERR_HIP, AXS_ERR_NOMEM, AXS_ERR_UNSUPPORTED = 1, 2, 3, 4
@@ -67,6 +73,7 @@ _SIGS = {
     "axs_ctx_stream": (_P, [_P]),
     "axs_last_error": (C.c_char_p, []),
     "axs_num_tile_variants": (_I, []),
+    "axs_tuning_load": (_I, [C.c_char_p]),
     "axs_background_eval": (_I, [C.POINTER(Background), _I, _D, _PD]),
     "axs_background_rhs_coeffs": (_I, [C.POINTER(Background), _D,
                                        C.POINTER(RhsCoeffs)]),
@@ -84,6 +91,8 @@ _SIGS = {
     "axs_level_set_stage_fusion": (_I, [_P, _I]),
     "axs_level_set_pair_tile": (_I, [_P, _I, _I]),
     "axs_level_get_tile": (_I, [_P, C.POINTER(_I), C.POINTER(_I)]),
+    "axs_level_get_launch": (_I, [_P, C.POINTER(_I), C.POINTER(_I),
+                                  C.POINTER(_I)]),
     "axs_rhs_eval": (_I, [_P, C.POINTER(RhsCoeffs), _P]),
     "axs_rk4_step": (_I, [_P, C.POINTER(RhsCoeffs), _D]),
     "axs_rk4_advance": (_I, [_P, C.POINTER(Background), _D, _D, _I, _I]),
@@ -109,6 +118,9 @@ _SIGS = {
                                  C.POINTER(_I), _I, _D, _I, C.POINTER(_P)]),
     "axs_boxlevel_destroy": (_I, [_P]),
     "axs_boxlevel_num_cells": (C.c_int64, [_P]),
+    "axs_boxlevel_set_work_shape": (_I, [_P, _I, _I]),
+    "axs_boxlevel_get_work_shape": (_I, [_P, C.POINTER(_I), C.POINTER(_I),
+                                         C.POINTER(_I)]),
     "axs_boxlevel_upload_dense": (_I, [_P, _PD]),
     "axs_boxlevel_download_dense": (_I, [_P, _PD]),
     "axs_boxlevel_copy_from_level": (_I, [_P, _P]),
@@ -157,8 +169,18 @@ def load(path: str = LIB_PATH) -> C.CDLL:
             continue
         fn.restype = res
         fn.argtypes = args
+    table = os.environ.get("AXS_TUNING_TABLE", TUNING_PATH)
+    if table and (table != TUNING_PATH or os.path.exists(table)):
+        check(lib.axs_tuning_load(table.encode()))
     _lib = lib
     return lib
+
+
+def load_tuning(path: str, lib: C.CDLL | None = None) -> None:
+    """Loads a tuning table (autotune.py's output) into the library; levels
+    and box levels created afterwards take their launch shapes from it."""
+    lib = lib or load()
+    check(lib.axs_tuning_load(str(path).encode()))
 
 
 def check(rc: int) -> None:
@@ -339,6 +361,17 @@ class Level:
     def set_pair_tile(self, variant: int, z_chunk: int = 0):
         check(self.lib.axs_level_set_pair_tile(self.h, variant, z_chunk))
 
+    def launch_shape(self) -> dict:
+        """The level's launch settings: stage fusion mode, stage tile variant
+        and z chunk, pair variant and pair z chunk."""
+        v, z, f, pv, pz = (C.c_int() for _ in range(5))
+        check(self.lib.axs_level_get_tile(self.h, C.byref(v), C.byref(z)))
+        check(self.lib.axs_level_get_launch(self.h, C.byref(f), C.byref(pv),
+                                            C.byref(pz)))
+        return {"fusion": f.value, "tile_variant": v.value,
+                "tile_z_chunk": z.value, "pair_variant": pv.value,
+                "pair_z_chunk": pz.value}
+
     def rhs(self, coeffs: RhsCoeffs) -> np.ndarray:
         buf = DeviceBuffer(4 * self.ncell * self.word)
         check(self.lib.axs_rhs_eval(self.h, C.byref(coeffs), _P(buf.ptr)))
@@ -487,6 +520,17 @@ class BoxLevel:
     @property
     def num_cells(self) -> int:
         return int(self.lib.axs_boxlevel_num_cells(self.h))
+
+    def set_work_shape(self, item_rows: int, grid_blocks: int):
+        check(self.lib.axs_boxlevel_set_work_shape(self.h, item_rows,
+                                                   grid_blocks))
+
+    def work_shape(self) -> tuple:
+        """(item_rows, grid_blocks, number of work items)."""
+        r, g, n = C.c_int(), C.c_int(), C.c_int()
+        check(self.lib.axs_boxlevel_get_work_shape(self.h, C.byref(r),
+                                                   C.byref(g), C.byref(n)))
+        return r.value, g.value, n.value
 
     def upload_dense(self, dense: np.ndarray):
         d = np.ascontiguousarray(dense, dtype=np.float64)

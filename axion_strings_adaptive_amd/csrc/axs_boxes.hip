@@ -66,6 +66,8 @@ struct axs_boxlevel
     size_t alloc_bytes[4];
     axs::WorkItem *ditems;
     int nitems;
+    int item_rows;   // rows per work item asked for (clamped per box)
+    int grid_blocks; // cap on the persistent grid of the stage launches
     double *dcoeff;
     double *h_coeff; // pinned staging of the stage scalars (12)
     void *snap;      // coarse snapshot (y [+ comp]) for time interpolation
@@ -423,6 +425,23 @@ __global__ void k_fill(LevelView f, T *buf, float *lo, int have_coarse,
 }
 
 // --------------------------------------------------- work-list stage ----
+// Items of box b: rows y0, y0 + H, ... of H = clamp(rows, 4, 256 / W)
+// rows each (W = the box's x edge, 8..64); a lane row of the item loads the
+// y halo below (its first two rows) or above (its last two), so H >= 4.
+static std::vector<WorkItem> make_items(const std::vector<BoxD> &boxes,
+                                        int rows)
+{
+    std::vector<WorkItem> items;
+    for (size_t i = 0; i < boxes.size(); ++i)
+    {
+        const int W = boxes[i].n[0];
+        const int H = std::max(4, std::min(rows, 256 / W));
+        for (int y0 = 0; y0 < boxes[i].n[1]; y0 += H)
+            items.push_back(WorkItem{(int)i, y0, H, W});
+    }
+    return items;
+}
+
 template <typename T> struct BoxStageParams
 {
     const BoxD *box;
@@ -1170,7 +1189,7 @@ static int stage_t(axs_boxlevel *bl, int stage, const void *yin, void *yout,
     p.h6 = dt / 6.0;
     p.h6_lo = (float)(p.h6 - (double)(float)p.h6);
     p.inv_dx2 = 1.0 / (bl->dx * bl->dx);
-    const int grid = std::min(bl->nitems, 2048);
+    const int grid = std::min(bl->nitems, bl->grid_blocks);
     hipStream_t s = bl->ctx->stream;
     switch (stage)
     {
@@ -1240,7 +1259,6 @@ int axs_boxlevel_create(axs_ctx *ctx, int nboxes, const int *lo, const int *hi,
     std::vector<int> owner((size_t)b->bdim[0] * b->bdim[1] * b->bdim[2], -1);
     uint64_t off = 0;
     int64_t nvalid = 0;
-    std::vector<WorkItem> items;
     for (int i = 0; i < nboxes; ++i)
     {
         BoxD B{};
@@ -1276,10 +1294,6 @@ int axs_boxlevel_create(axs_ctx *ctx, int nboxes, const int *lo, const int *hi,
                     }
                     o = i;
                 }
-        const int W = B.n[0];
-        const int H = std::max(1, 256 / W);
-        for (int y0 = 0; y0 < B.n[1]; y0 += H)
-            items.push_back(WorkItem{i, y0, std::min(H, 256 / W), W});
         b->hbox.push_back(B);
     }
     // offsets inside one component array are 32-bit; component strides
@@ -1291,6 +1305,12 @@ int axs_boxlevel_create(axs_ctx *ctx, int nboxes, const int *lo, const int *hi,
     }
     b->total = (uint32_t)off;
     b->nvalid = nvalid;
+    // work shape: whole-width items of up to 256 lanes and a grid of up to
+    // 2048 blocks unless the tuning table has one for this precision
+    b->item_rows = 256;
+    b->grid_blocks = 2048;
+    (void)tuned_box(precision, &b->item_rows, &b->grid_blocks);
+    const std::vector<WorkItem> items = make_items(b->hbox, b->item_rows);
     b->nitems = (int)items.size();
     const size_t bytes4 = (size_t)4 * off * b->word;
     // staggered array bases, as for uniform levels (axs_level_create)
@@ -1371,6 +1391,57 @@ int axs_boxlevel_destroy(axs_boxlevel *b)
 }
 
 int64_t axs_boxlevel_num_cells(axs_boxlevel *b) { return b ? b->nvalid : 0; }
+
+int axs_boxlevel_set_work_shape(axs_boxlevel *b, int item_rows, int grid_blocks)
+{
+    AXS_CHECK_ARG(b, "axs_boxlevel_set_work_shape: NULL level");
+    AXS_CHECK_ARG(item_rows >= 1 && item_rows <= 256,
+                  "axs_boxlevel_set_work_shape: 1 <= item_rows <= 256");
+    AXS_CHECK_ARG(grid_blocks >= 1 && grid_blocks <= 65536,
+                  "axs_boxlevel_set_work_shape: 1 <= grid_blocks <= 65536");
+    if (item_rows == b->item_rows && grid_blocks == b->grid_blocks)
+        return AXS_OK;
+    hipStream_t s = b->ctx->stream;
+    AXS_HIP_TRY(hipStreamSynchronize(s));
+    // captured step graphs hold the old item list and grid
+    for (auto &e : b->graphs)
+    {
+        (void)hipGraphExecDestroy(e.ge);
+        (void)hipGraphDestroy(e.g);
+    }
+    b->graphs.clear();
+    const std::vector<WorkItem> items = make_items(b->hbox, item_rows);
+    if ((int)items.size() != b->nitems)
+    {
+        WorkItem *d = nullptr;
+        if (hipMalloc(&d, items.size() * sizeof(WorkItem)) != hipSuccess)
+            return set_error(AXS_ERR_NOMEM,
+                             "axs_boxlevel_set_work_shape: out of memory");
+        (void)hipFree(b->ditems);
+        b->ditems = d;
+    }
+    AXS_HIP_TRY(hipMemcpyAsync(b->ditems, items.data(),
+                               items.size() * sizeof(WorkItem),
+                               hipMemcpyHostToDevice, s));
+    AXS_HIP_TRY(hipStreamSynchronize(s));
+    b->nitems = (int)items.size();
+    b->item_rows = item_rows;
+    b->grid_blocks = grid_blocks;
+    return AXS_OK;
+}
+
+int axs_boxlevel_get_work_shape(const axs_boxlevel *b, int *item_rows,
+                                int *grid_blocks, int *nitems)
+{
+    AXS_CHECK_ARG(b, "axs_boxlevel_get_work_shape: NULL level");
+    if (item_rows)
+        *item_rows = b->item_rows;
+    if (grid_blocks)
+        *grid_blocks = b->grid_blocks;
+    if (nitems)
+        *nitems = b->nitems;
+    return AXS_OK;
+}
 
 int axs_boxlevel_upload_dense(axs_boxlevel *b, const double *dense)
 {

@@ -8,6 +8,9 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <fstream>
+#include <mutex>
+#include <sstream>
 #include <vector>
 
 namespace axs
@@ -224,6 +227,54 @@ static void free_arrays(axs_level *L)
 }
 
 
+// ------------------------------------------------------ tuning table --
+// Lines of the table (autotune.py writes it; '#' starts a comment):
+//   level <nx> <ny> <nz> <precision> <fusion> <tile_variant> <tile_z_chunk>
+//         <pair_variant> <pair_z_chunk>
+//   box <precision> <item_rows> <grid_blocks>
+struct LevelTune
+{
+    int nx, ny, nz, precision, fused, tile_v, tile_zc, pair_v, pair_zc;
+};
+struct BoxTune
+{
+    int precision, rows, grid;
+};
+static std::mutex g_tune_mu;
+static std::vector<LevelTune> g_level_tune;
+static std::vector<BoxTune> g_box_tune;
+
+bool tuned_level(int nx, int ny, int nz, int precision, int fused,
+                 int *tile_variant, int *tile_z_chunk, int *pair_variant,
+                 int *pair_z_chunk)
+{
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    for (const LevelTune &t : g_level_tune)
+        if (t.nx == nx && t.ny == ny && t.nz == nz &&
+            t.precision == precision && t.fused == fused)
+        {
+            *tile_variant = t.tile_v;
+            *tile_z_chunk = t.tile_zc;
+            *pair_variant = t.pair_v;
+            *pair_z_chunk = t.pair_zc;
+            return true;
+        }
+    return false;
+}
+
+bool tuned_box(int precision, int *item_rows, int *grid_blocks)
+{
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    for (const BoxTune &t : g_box_tune)
+        if (t.precision == precision)
+        {
+            *item_rows = t.rows;
+            *grid_blocks = t.grid;
+            return true;
+        }
+    return false;
+}
+
 void *pool_alloc(axs_ctx *c, size_t bytes, size_t *got)
 {
     int best = -1;
@@ -284,6 +335,69 @@ extern "C" {
 const char *axs_last_error(void) { return g_last_error.c_str(); }
 
 int axs_num_tile_variants(void) { return axs::num_tile_variants(); }
+
+int axs_tuning_load(const char *path)
+{
+    AXS_CHECK_ARG(path, "axs_tuning_load: NULL path");
+    std::ifstream f(path);
+    if (!f)
+        return set_error(AXS_ERR_ARG,
+                         std::string("axs_tuning_load: cannot read ") + path);
+    std::vector<LevelTune> lv;
+    std::vector<BoxTune> bx;
+    std::string line;
+    int ln = 0;
+    while (std::getline(f, line))
+    {
+        ++ln;
+        const size_t hash = line.find('#');
+        if (hash != std::string::npos)
+            line.resize(hash);
+        std::istringstream in(line);
+        std::string kind;
+        if (!(in >> kind))
+            continue;
+        std::string extra;
+        if (kind == "level")
+        {
+            LevelTune t{};
+            if (!(in >> t.nx >> t.ny >> t.nz >> t.precision >> t.fused >>
+                  t.tile_v >> t.tile_zc >> t.pair_v >> t.pair_zc) ||
+                (in >> extra) || t.nx < 4 || t.ny < 4 || t.nz < 4 ||
+                t.precision < 0 || t.precision > 2 || t.fused < 0 ||
+                t.fused > 3 || t.tile_v < -1 ||
+                t.tile_v >= axs::num_tile_variants() || t.tile_zc < 0 ||
+                t.pair_v < -1 || t.pair_v >= axs::num_pair_variants() ||
+                t.pair_zc < 0)
+                return set_error(AXS_ERR_ARG,
+                                 std::string("axs_tuning_load: ") + path +
+                                     ":" + std::to_string(ln) +
+                                     ": bad level line");
+            lv.push_back(t);
+        }
+        else if (kind == "box")
+        {
+            BoxTune t{};
+            if (!(in >> t.precision >> t.rows >> t.grid) || (in >> extra) ||
+                t.precision < 0 || t.precision > 2 || t.rows < 1 ||
+                t.rows > 256 || t.grid < 1 || t.grid > 65536)
+                return set_error(AXS_ERR_ARG,
+                                 std::string("axs_tuning_load: ") + path +
+                                     ":" + std::to_string(ln) +
+                                     ": bad box line");
+            bx.push_back(t);
+        }
+        else
+            return set_error(AXS_ERR_ARG,
+                             std::string("axs_tuning_load: ") + path + ":" +
+                                 std::to_string(ln) + ": unknown entry '" +
+                                 kind + "'");
+    }
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_level_tune.swap(lv);
+    g_box_tune.swap(bx);
+    return AXS_OK;
+}
 
 int axs_ctx_create(int device, axs_ctx **out)
 {
@@ -423,6 +537,9 @@ int axs_level_create(axs_ctx *ctx, int nx, int ny, int nz, double dx,
     L->fused = precision == AXS_FP64 ? 1 : 3;
     L->pair_variant = -1;
     L->pair_z_chunk = 0;
+    // launch shapes of this shape and precision from the tuning table
+    (void)tuned_level(nx, ny, nz, precision, L->fused, &L->tile_variant,
+                      &L->z_chunk, &L->pair_variant, &L->pair_z_chunk);
     bool ok = alloc_arrays(L);
     ok = ok &&
          hipMalloc(&L->d_coeff, (size_t)kCoeffTableSteps * 12 * sizeof(double)) ==
@@ -504,7 +621,13 @@ int axs_level_set_stage_fusion(axs_level *L, int on)
                   "axs_level_set_stage_fusion: one launch per step (3) is "
                   "FP32 / FP32C only");
     if (on != L->fused)
+    {
         destroy_graph(L);
+        // the tuning table's launch shapes for the new mode, if it has them
+        (void)tuned_level(L->g.nx, L->g.ny, L->g.nz, L->precision, on,
+                          &L->tile_variant, &L->z_chunk, &L->pair_variant,
+                          &L->pair_z_chunk);
+    }
     L->fused = on;
     return AXS_OK;
 }
@@ -527,6 +650,19 @@ int axs_level_get_tile(axs_level *L, int *variant, int *z_chunk)
     AXS_CHECK_ARG(L && variant && z_chunk, "axs_level_get_tile: NULL");
     *variant = L->tile_variant;
     *z_chunk = L->z_chunk;
+    return AXS_OK;
+}
+
+int axs_level_get_launch(axs_level *L, int *fusion, int *pair_variant,
+                         int *pair_z_chunk)
+{
+    AXS_CHECK_ARG(L, "axs_level_get_launch: NULL level");
+    if (fusion)
+        *fusion = L->fused;
+    if (pair_variant)
+        *pair_variant = L->pair_variant;
+    if (pair_z_chunk)
+        *pair_z_chunk = L->pair_z_chunk;
     return AXS_OK;
 }
 

@@ -52,6 +52,15 @@ void pool_release(axs_ctx *c, void *p, size_t bytes);
 // Frees every pooled block.
 void pool_trim(axs_ctx *c);
 
+// Launch shapes from the tuning table (axs_tuning_load).  tuned_level
+// fills the four values for a level of this shape, precision and stage
+// fusion mode and returns true when the table has them; tuned_box likewise
+// for box levels of a precision.
+bool tuned_level(int nx, int ny, int nz, int precision, int fused,
+                 int *tile_variant, int *tile_z_chunk, int *pair_variant,
+                 int *pair_z_chunk);
+bool tuned_box(int precision, int *item_rows, int *grid_blocks);
+
 struct Grid
 {
     int nx, ny, nz;

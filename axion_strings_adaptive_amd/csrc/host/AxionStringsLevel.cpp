@@ -117,6 +117,12 @@ AxionStringsLevel::AxionStringsLevel(const ParamTable &params,
     else if (field != "a_dot")
         as.error("spectrum_field", "must be a_dot or pi");
 
+    // extension: the autotuner's launch-shape table, read before the level
+    // is created so that the level takes its tuned tiles
+    std::string tuning;
+    as.queryAdd("tuning_table", tuning);
+    if (!tuning.empty())
+        axs_check(axs_tuning_load(tuning.c_str()), "axs_tuning_load");
     axs_check(axs_ctx_create(0, &m_ctx), "axs_ctx_create");
     axs_check(axs_level_create(m_ctx, m_n[0], m_n[1], m_n[2], m_dx,
                                m_precision, &m_level),

@@ -14,6 +14,9 @@
 //   axion_strings.use_graph           replay a captured hipGraph per step
 //   axion_strings.placement_trials    re-allocate the level's arrays up to N
 //                                     times and keep a fast placement
+//   axion_strings.tuning_table        the autotuner's launch-shape table
+//                                     (axs_tuning_load), relative to the
+//                                     parameter file
 #pragma once
 
 #include "Background.hpp"

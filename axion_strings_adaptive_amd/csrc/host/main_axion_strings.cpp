@@ -27,6 +27,17 @@ int main(int argc, char **argv)
     try
     {
         table.read_file(argv[1]);
+        // a relative tuning table named in the file is relative to the file
+        // (one given on the command line, to the working directory)
+        const std::string tkey = "axion_strings.tuning_table";
+        if (table.contains(tkey))
+        {
+            const std::string tp = table.values(tkey).front();
+            const std::string pf = argv[1];
+            const size_t slash = pf.find_last_of('/');
+            if (!tp.empty() && tp[0] != '/' && slash != std::string::npos)
+                table.set(tkey, pf.substr(0, slash + 1) + tp);
+        }
         for (int i = 2; i < argc; ++i)
         {
             if (!std::strcmp(argv[i], "--check"))

@@ -64,6 +64,14 @@ MEASURED_PAIR2_BYTES = {1: {(-1, 0): 117.40}, 2: {(-1, 0): 103.48}}
 MEASURED_STEP_BYTES = {(-1, 0): 74.58, (2, 0): 97.07}
 
 
+def step_kernel_name(pair_variant):
+    """The one-launch step kernel a pair variant selects."""
+    if pair_variant in (-1, 4):
+        return f"k_step_r<FP32C, 11, {2 if pair_variant == 4 else 1}>"
+    ty = {0: 8, 1: 4, 3: 11}.get(pair_variant, 12)
+    return f"k_step<FP32C, {ty}, {'aligned' if ty == 11 else 'unaligned'}>"
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -327,7 +335,8 @@ def main(argv=None):
     ap.add_argument("--cpu-steps", type=int, default=5,
                     help="CPU baseline sample: RK4 steps")
     ap.add_argument("--tile", type=str, default="",
-                    help="variant,z_chunk (default: autotuned cache or built-in)")
+                    help="stage tile variant,z_chunk (default: the tuning "
+                         "table's, else built-in)")
     ap.add_argument("--no-fusion", action="store_true",
                     help="one launch per RK stage instead of two fused pairs")
     ap.add_argument("--fusion", type=int, default=0, choices=(0, 1, 2, 3),
@@ -336,7 +345,8 @@ def main(argv=None):
                          "FP32C only); 0 = the library's default per "
                          "precision (FP32C 3, FP64 1)")
     ap.add_argument("--pair-tile", type=str, default="",
-                    help="fused-pair variant,z_chunk (default: built-in)")
+                    help="fused-pair / step variant,z_chunk (default: the "
+                         "tuning table's, else built-in)")
     ap.add_argument("--alloc-check", action="store_true",
                     help="only time the FP32C level on successive "
                          "allocations")
@@ -382,13 +392,11 @@ def main(argv=None):
     n, dt = args.n, 0.1
     bg = capi.make_background(2.0, c0=0.0, c1=2.0, tau_switch=1.0,
                               has_switch=True)
+    # launch shapes: the tuning table's (capi.load applies it) unless given
     tile = None
     if args.tile:
         v, z = (int(t) for t in args.tile.split(","))
         tile = (v, z)
-    else:
-        from axion_strings_adaptive_amd import autotune
-        tile = autotune.cached_choice(n, capi.AXS_FP32C)
 
     # stage fusion mode per precision (0 = one launch per stage)
     fused = 0 if args.no_fusion else (args.fusion or 3)
@@ -414,11 +422,8 @@ def main(argv=None):
         ctx.close()
         return 0
     if args.precision_steps:
-        from axion_strings_adaptive_amd import autotune
         res = precision_run(capi, ctx, n, bg, dt, args.precision_steps,
-                            args.precision_every, tile,
-                            autotune.cached_choice(n, capi.AXS_FP64)
-                            if not args.tile else tile)
+                            args.precision_every, tile, tile)
         print(json.dumps({"precision_run": res}), flush=True)
         ctx.close()
         return 0
@@ -445,8 +450,7 @@ def main(argv=None):
         log(f"[bench] FP64 {n}^3 same IC and steps")
         b, wall64, ms64, ns64, _, place64 = run_level(
             capi, ctx, n, capi.AXS_FP64, bg, dt, args.warmup, args.steps,
-            autotune.cached_choice(n, capi.AXS_FP64) if not args.tile else tile,
-            args.place_trials, fused64, pair_tile)
+            tile, args.place_trials, fused64, pair_tile)
         if place64:
             log(f"[bench] FP64 placement trials (ms/step): "
                 f"{', '.join(f'{t:.2f}' for t in place64)}")
@@ -456,6 +460,7 @@ def main(argv=None):
         res64 = {
             "value": value64, "ms_per_step": 1e3 * wall64 / args.steps,
             "stage_ms": [m / ns64 for m in ms64], "stage_fusion": fused64,
+            "launch_shape": b.launch_shape(),
             "speedup_fp32c_over_fp64": value / value64,
             "max_dphi_over_eta": dpsi / d64["R"],
             "rel_energy_err": abs(d32["rho_tot_unscreened"] -
@@ -468,17 +473,18 @@ def main(argv=None):
         b.close()
 
     st = [m / ns32 for m in ms32]  # ms per launch of each stage (pair)
+    shape32 = a.launch_shape()  # the tuning table's or the given shapes
+    pkey = (shape32["pair_variant"], shape32["pair_z_chunk"])
     if fused == 3:
         # one launch per step: the step kernel is the only one
         achieved = STEP_BYTES_ONE * ncell / (st[0] * 1e-3) / 1e9
-        per_cell = MEASURED_STEP_BYTES.get(tuple(pair_tile) if pair_tile
-                                           else (-1, 0))
+        per_cell = MEASURED_STEP_BYTES.get(pkey)
         roofline = {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
             "traffic": per_cell * ncell if per_cell and n == 1024 else None,
             "traffic_unit": "bytes per launch",
-            "kernel": "k_step<FP32C, 11, aligned> (the whole RK4 step)",
+            "kernel": step_kernel_name(pkey[0]) + " (the whole RK4 step)",
             "algorithmic_bytes_per_cell": STEP_BYTES_ONE,
             "step_launch_ms": st[0], "step_gbs": achieved}
     elif fused:
@@ -487,8 +493,8 @@ def main(argv=None):
         k = 0 if st[0] > st[1] else 1
         pb = PAIR_BYTES if fused == 1 else PAIR_BYTES_NOACC
         achieved = pb[1][k] * ncell / (st[k] * 1e-3) / 1e9
-        per_cell = (MEASURED_PAIR2_BYTES[fused].get(
-            tuple(pair_tile) if pair_tile else (-1, 0)) if k == 1 else None)
+        per_cell = (MEASURED_PAIR2_BYTES[fused].get(pkey) if k == 1
+                    else None)
         roofline = {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
@@ -505,7 +511,8 @@ def main(argv=None):
         # roofline of the dominant kernel: the stage-2/3 launch (two per step)
         bytes23 = STAGE_BYTES[1][1] * ncell
         achieved = bytes23 / (0.5 * (st[1] + st[2]) * 1e-3) / 1e9
-        per_cell = MEASURED_S23_BYTES.get(tuple(tile) if tile else None)
+        per_cell = MEASURED_S23_BYTES.get(
+            (shape32["tile_variant"], shape32["tile_z_chunk"]))
         roofline = {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
@@ -536,8 +543,7 @@ def main(argv=None):
                    "precision": "fp32 storage + Kahan-compensated y_n "
                                 "(fp32 hi + fp32 lo)",
                    "stage_fusion": fused,
-                   "tile": list(tile) if tile else None,
-                   "pair_tile": list(pair_tile) if pair_tile else None,
+                   "launch_shape": shape32,
                    "placement_trials_ms": place32},
         "fp64": res64, "roofline": roofline, "cpu_baseline": cpu,
         "diagnostics_fp32c": diag_ms,

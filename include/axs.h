@@ -121,6 +121,20 @@ void *axs_ctx_stream(axs_ctx *ctx);
 const char *axs_last_error(void);
 /* Number of compiled stage-kernel tile variants (for the autotuner). */
 int axs_num_tile_variants(void);
+/* Loads a tuning table (the autotuner's output, a text file) that replaces
+ * the one loaded before.  No counterpart in the reference (which picks
+ * launch shapes inside AMReX's ParallelFor).  Lines, '#' to end of line a
+ * comment:
+ *   level <nx> <ny> <nz> <precision> <fusion> <tile_variant> <tile_z_chunk>
+ *         <pair_variant> <pair_z_chunk>
+ *   box <precision> <item_rows> <grid_blocks>
+ * A level created afterwards, and a level switched to another stage fusion
+ * mode, takes the tile, pair variant and z chunks of the `level` line of
+ * its shape, precision and fusion mode; a box level created afterwards
+ * takes the work shape of the `box` line of its precision.  Levels without
+ * a line keep the built-in defaults.  A malformed line fails the call with
+ * AXS_ERR_ARG naming it and leaves the table as it was. */
+int axs_tuning_load(const char *path);
 
 /* ------------------------- host background -------------------------- */
 /* Background.hpp:53-133.  which: 0 R, 1 R', 2 curvature_term_coeff,
@@ -199,6 +213,10 @@ int axs_level_set_stage_fusion(axs_level *lvl, int on);
  * axs_num_pair_variants() variants. */
 int axs_num_pair_variants(void);
 int axs_level_set_pair_tile(axs_level *lvl, int variant, int z_chunk);
+/* The level's stage fusion mode, pair variant and pair z chunk (any
+ * pointer may be NULL). */
+int axs_level_get_launch(axs_level *lvl, int *fusion, int *pair_variant,
+                         int *pair_z_chunk);
 
 /* ------------------- RHS and RK4 (the hot path) --------------------- */
 /* AxionStringsRHS<>::operator() over every cell of a periodic level
@@ -359,6 +377,19 @@ int axs_boxlevel_create(axs_ctx *ctx, int nboxes, const int *lo,
 int axs_boxlevel_destroy(axs_boxlevel *bl);
 /* Number of valid cells (without ghosts) over all boxes. */
 int64_t axs_boxlevel_num_cells(axs_boxlevel *bl);
+/* Work shape of the box level's RK4 stage launches; no counterpart in the
+ * reference (AMReX's ParallelFor picks it there).  Every box is cut into
+ * items of whole box rows, H = clamp(item_rows, 4, 256 / box x edge) rows
+ * each, and a grid of min(items, grid_blocks) blocks strides over them.
+ * Defaults: item_rows 256 (the widest items), grid_blocks 2048, or the
+ * tuning table's `box` line.  Every shape gives the same bits.  Setting a
+ * new shape waits for the level's stream, drops its captured graphs and
+ * may reallocate the item list. */
+int axs_boxlevel_set_work_shape(axs_boxlevel *bl, int item_rows,
+                                int grid_blocks);
+/* Current item_rows, grid_blocks and number of items (NULL skips). */
+int axs_boxlevel_get_work_shape(const axs_boxlevel *bl, int *item_rows,
+                                int *grid_blocks, int *nitems);
 /* Copies values from / to a dense periodic host array of the whole domain
  * (4 * domain cells, doubles, x fastest); cells outside every box are not
  * read / not written. */

@@ -86,3 +86,42 @@ def test_null_arguments_are_rejected_without_a_gpu(lib):
     assert b"NULL" in lib.axs_last_error()
     assert lib.axs_rk4_step(None, None, 0.1) == 1
     assert lib.axs_energy_sums(None, 1.0, 1.0, 0.0, None, None, None) == 1
+
+
+def test_tuning_table_parser(lib, tmp_path):
+    """axs_tuning_load accepts the autotuner's format (comments, blank
+    lines, both entry kinds) and names the file and line of a malformed
+    entry: wrong field count, out-of-range variant, fusion, precision or
+    box shape, or an unknown kind."""
+    from axion_strings_adaptive_amd import capi
+    good = tmp_path / "good.txt"
+    good.write_text("# launch shapes\n\n"
+                    "level 1024 1024 1024 1 3 -1 0 4 128  # one launch\n"
+                    "level 256 256 256 0 1 2 32 5 64\n"
+                    "box 1 8 1024\n")
+    capi.load_tuning(good, lib)
+    nt, npv = lib.axs_num_tile_variants(), lib.axs_num_pair_variants()
+    bad_lines = [
+        "level 1024 1024 1024 1 3 -1 0 4",          # too few fields
+        "level 1024 1024 1024 1 3 -1 0 4 128 7",    # too many
+        f"level 64 64 64 1 1 {nt} 0 -1 0",           # tile variant
+        f"level 64 64 64 1 1 -1 0 {npv} 0",          # pair variant
+        "level 64 64 64 1 4 -1 0 -1 0",              # fusion mode
+        "level 64 64 64 3 1 -1 0 -1 0",              # precision
+        "level 2 64 64 1 1 -1 0 -1 0",               # grid edge
+        "level 64 64 64 1 1 -1 -5 -1 0",             # z chunk
+        "box 1 0 1024", "box 1 8 0", "box 1 257 64", "box 1 8",
+        "tile 1 2 3",
+    ]
+    for i, line in enumerate(bad_lines):
+        f = tmp_path / f"bad{i}.txt"
+        f.write_text("# ok\nbox 0 16 512\n" + line + "\n")
+        with pytest.raises(capi.AxsError, match=rf"bad{i}\.txt:3"):
+            capi.load_tuning(f, lib)
+    with pytest.raises(capi.AxsError, match="cannot read"):
+        capi.load_tuning(tmp_path / "missing.txt", lib)
+    empty = tmp_path / "empty.txt"
+    empty.write_text("")
+    capi.load_tuning(empty, lib)  # an empty table is valid
+    if os.path.exists(capi.TUNING_PATH):  # the table load() applied
+        capi.load_tuning(capi.TUNING_PATH, lib)
