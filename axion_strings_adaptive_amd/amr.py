@@ -496,6 +496,12 @@ def main(argv=None):
     print(json.dumps({
         "levels": len(h.levels), "cells": h.ncells,
         "boxes": [len(L.boxes) for L in h.levels], "regrids": h.nregrids,
+        # ghost cells the RK stage fills write per level (2-cell shells)
+        "stage_fill_ghost_cells": [
+            sum((b[1][0] - b[0][0] + 5) * (b[1][1] - b[0][1] + 5) *
+                (b[1][2] - b[0][2] + 5) - (b[1][0] - b[0][0] + 1) *
+                (b[1][1] - b[0][1] + 1) * (b[1][2] - b[0][2] + 1)
+                for b in L.boxes) for L in h.levels],
         "cell_updates_per_coarse_step": h.cell_updates_per_coarse_step(),
         "cell_updates_per_s": h.cell_updates_per_coarse_step() * args.steps / dt,
         "ms_per_coarse_step": 1e3 * dt / args.steps,

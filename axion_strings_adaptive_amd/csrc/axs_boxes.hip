@@ -193,7 +193,7 @@ __device__ double coarse_value(const CoarseSrc<T> &c, int comp, int cx, int cy,
 // region (box bounds are even) computes its 8 fine children at once, so each
 // coarse neighbour is located once for the 4 components and read once for
 // the 8 children.  Per child the tensor-product sums run in the same order
-// as interp_quartic4's (x innermost, then y, then z), so the values are the
+// as interp_quartic's (x innermost, then y, then z), so the values are the
 // same.
 template <typename T>
 __global__ void __launch_bounds__(256)
@@ -323,20 +323,20 @@ __device__ __forceinline__ void ghost_cell(uint32_t t, int nx, int ny, int nz,
 // Tensor-product quartic interpolation of the 4 coarse components at the
 // fine cell (fx, fy, fz) (ratio 2, cell centred): each coarse neighbour is
 // located once; per component the sums run x innermost, then y, then z.
-template <typename T>
-__device__ void interp_quartic4(const CoarseSrc<T> &c, int fx, int fy, int fz,
-                                double (&out)[4], unsigned long long *err)
+template <typename T, int NC>
+__device__ void interp_quartic(const CoarseSrc<T> &c, int fx, int fy, int fz,
+                               double (&out)[NC], unsigned long long *err)
 {
     const int cx = fx >> 1, cy = fy >> 1, cz = fz >> 1;
     const double *wx = kQuartW[fx & 1], *wy = kQuartW[fy & 1],
                  *wz = kQuartW[fz & 1];
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    double acc[NC] = {};
     for (int k = 0; k < 5; ++k)
     {
-        double ay[4] = {0.0, 0.0, 0.0, 0.0};
+        double ay[NC] = {};
         for (int j = 0; j < 5; ++j)
         {
-            double ax[4] = {0.0, 0.0, 0.0, 0.0};
+            double ax[NC] = {};
             for (int i = 0; i < 5; ++i)
             {
                 uint32_t ci;
@@ -344,7 +344,7 @@ __device__ void interp_quartic4(const CoarseSrc<T> &c, int fx, int fy, int fz,
                     locate(c.v, cx + i - 2, cy + j - 2, cz + k - 2, ci) >= 0;
                 if (!ok)
                     atomicAdd(err, 1ull);
-                for (int q = 0; q < 4; ++q)
+                for (int q = 0; q < NC; ++q)
                 {
                     double v = 0.0;
                     if (ok)
@@ -363,40 +363,42 @@ __device__ void interp_quartic4(const CoarseSrc<T> &c, int fx, int fy, int fz,
                     ax[q] += wx[i] * v;
                 }
             }
-            for (int q = 0; q < 4; ++q)
+            for (int q = 0; q < NC; ++q)
                 ay[q] += wy[j] * ax[q];
         }
-        for (int q = 0; q < 4; ++q)
+        for (int q = 0; q < NC; ++q)
             acc[q] += wz[k] * ay[q];
     }
-    for (int q = 0; q < 4; ++q)
+    for (int q = 0; q < NC; ++q)
         out[q] = acc[q];
 }
 
-// ghost fill of one buffer: each ghost cell from the box that owns it on
-// this level, else (coarse-fine boundary) by quartic interpolation from
-// the coarse level
-template <typename T>
+// ghost fill of one buffer: each ghost cell of the shell of width w (<= the
+// stored ghost width) from the box that owns it on this level, else
+// (coarse-fine boundary) by quartic interpolation from the coarse level;
+// components 0 .. NC-1.  The RK stages fill only what their stencil reads,
+// psi (NC = 2) in a shell of 2; the full fill is NC = 4, w = ng.
+template <typename T, int NC>
 __global__ void k_fill(LevelView f, T *buf, float *lo, int have_coarse,
-                       CoarseSrc<T> c, unsigned long long *err)
+                       CoarseSrc<T> c, int w, unsigned long long *err)
 {
     const BoxD B = f.box[blockIdx.y];
     const int ng = f.ng;
-    const uint32_t gx = B.n[0] + 2 * ng, gy = B.n[1] + 2 * ng,
-                   gz = B.n[2] + 2 * ng;
+    const uint32_t gx = B.n[0] + 2 * w, gy = B.n[1] + 2 * w,
+                   gz = B.n[2] + 2 * w;
     const uint32_t cells =
         gx * gy * gz - (uint32_t)B.n[0] * B.n[1] * B.n[2];
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < cells;
          t += gridDim.x * blockDim.x)
     {
         int lx, ly, lz;
-        ghost_cell(t, B.n[0], B.n[1], B.n[2], ng, lx, ly, lz);
+        ghost_cell(t, B.n[0], B.n[1], B.n[2], w, lx, ly, lz);
         const uint32_t dst = sidx(B, ng, lx, ly, lz);
         const int X = B.lo[0] + lx, Y = B.lo[1] + ly, Z = B.lo[2] + lz;
         uint32_t src;
         if (locate(f, X, Y, Z, src) >= 0)
         {
-            for (int comp = 0; comp < 4; ++comp)
+            for (int comp = 0; comp < NC; ++comp)
             {
                 buf[(size_t)comp * f.total + dst] = buf[(size_t)comp * f.total + src];
                 if (lo)
@@ -411,9 +413,9 @@ __global__ void k_fill(LevelView f, T *buf, float *lo, int have_coarse,
         }
         const int px = pmod(X, f.dom[0]), py = pmod(Y, f.dom[1]),
                   pz = pmod(Z, f.dom[2]);
-        double v4[4];
-        interp_quartic4(c, px, py, pz, v4, err);
-        for (int comp = 0; comp < 4; ++comp)
+        double v4[NC];
+        interp_quartic<T, NC>(c, px, py, pz, v4, err);
+        for (int comp = 0; comp < NC; ++comp)
         {
             const double v = v4[comp];
             const T h = (T)v;
@@ -1097,12 +1099,14 @@ static dim3 box_grid(const axs_boxlevel *b, size_t max_cells)
     return dim3((unsigned)std::max<size_t>(1, chunks), (unsigned)b->nb);
 }
 
-static size_t max_grown(const axs_boxlevel *b)
+// ghost cells of the widest shell of width w over the boxes
+static size_t max_shell(const axs_boxlevel *b, int w)
 {
     size_t m = 0;
     for (const BoxD &B : b->hbox)
-        m = std::max(m, (size_t)(B.n[0] + 2 * b->ng) * (B.n[1] + 2 * b->ng) *
-                            (B.n[2] + 2 * b->ng));
+        m = std::max(m, (size_t)(B.n[0] + 2 * w) * (B.n[1] + 2 * w) *
+                                (B.n[2] + 2 * w) -
+                            (size_t)B.n[0] * B.n[1] * B.n[2]);
     return m;
 }
 
@@ -1128,9 +1132,11 @@ static int check_err(axs_boxlevel *bl, const char *what)
     return AXS_OK;
 }
 
+// psi_only: the stage fills (components 0, 1 in a shell of 2, no lo)
 template <typename T>
 static int fill_t(axs_boxlevel *bl, void *buf, float *lo,
-                  const axs_boxlevel *coarse, double t_frac, int interior)
+                  const axs_boxlevel *coarse, double t_frac, int interior,
+                  bool psi_only)
 {
     CoarseSrc<T> c{};
     if (coarse)
@@ -1153,19 +1159,26 @@ static int fill_t(axs_boxlevel *bl, void *buf, float *lo,
                            (T *)buf, lo, c, bl->derr);
         AXS_HIP_TRY(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_fill<T>, box_grid(bl, max_grown(bl) - max_valid(bl)),
-                       dim3(256), 0, bl->ctx->stream, view_of(bl), (T *)buf,
-                       lo, coarse ? 1 : 0, c, bl->derr);
+    if (psi_only)
+        hipLaunchKernelGGL((k_fill<T, 2>), box_grid(bl, max_shell(bl, 2)),
+                           dim3(256), 0, bl->ctx->stream, view_of(bl),
+                           (T *)buf, (float *)nullptr, coarse ? 1 : 0, c, 2,
+                           bl->derr);
+    else
+        hipLaunchKernelGGL((k_fill<T, 4>), box_grid(bl, max_shell(bl, bl->ng)),
+                           dim3(256), 0, bl->ctx->stream, view_of(bl),
+                           (T *)buf, lo, coarse ? 1 : 0, c, bl->ng, bl->derr);
     AXS_HIP_TRY(hipGetLastError());
     return AXS_OK;
 }
 
 static int fill(axs_boxlevel *bl, void *buf, float *lo,
-                const axs_boxlevel *coarse, double t_frac, int interior)
+                const axs_boxlevel *coarse, double t_frac, int interior,
+                bool psi_only = false)
 {
     if (bl->precision == AXS_FP64)
-        return fill_t<double>(bl, buf, lo, coarse, t_frac, interior);
-    return fill_t<float>(bl, buf, lo, coarse, t_frac, interior);
+        return fill_t<double>(bl, buf, lo, coarse, t_frac, interior, psi_only);
+    return fill_t<float>(bl, buf, lo, coarse, t_frac, interior, psi_only);
 }
 
 template <typename T, int PREC>
@@ -1687,7 +1700,7 @@ int axs_boxlevel_rk4_step(axs_boxlevel *b, const axs_rhs_coeffs coeffs[4],
     void *out[4] = {b->ya, b->yb, b->ya, b->y};
     for (int st = 0; st < 4; ++st)
     {
-        int rc = fill(b, in[st], st == 0 ? b->comp : nullptr, coarse, tf[st], 0);
+        int rc = fill(b, in[st], nullptr, coarse, tf[st], 0, true);
         if (!rc)
             rc = stage(b, st + 1, in[st], out[st], dt);
         if (rc)
@@ -1724,7 +1737,7 @@ int axs_boxlevel_rk4_step_graph(axs_boxlevel *b, const axs_rhs_coeffs coeffs[4],
         int rc = AXS_OK;
         for (int st = 0; st < 4 && rc == AXS_OK; ++st)
         {
-            rc = fill(b, in[st], st == 0 ? b->comp : nullptr, coarse, tf[st], 0);
+            rc = fill(b, in[st], nullptr, coarse, tf[st], 0, true);
             if (rc == AXS_OK)
                 rc = stage(b, st + 1, in[st], out[st], dt);
         }

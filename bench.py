@@ -60,8 +60,9 @@ MEASURED_S23_BYTES = {(20, 32): 86.03}
 # the same for the fused FP32C stages 3+4 launch, keyed by (pair variant,
 # z_chunk): pairs with a stored accumulator (mode 1) and without (mode 2)
 MEASURED_PAIR2_BYTES = {1: {(-1, 0): 117.40}, 2: {(-1, 0): 103.48}}
-# ... and of the one-launch step kernel (mode 3)
-MEASURED_STEP_BYTES = {(-1, 0): 74.58, (2, 0): 97.07}
+# ... and of the one-launch step kernel (mode 3), keyed by (pair variant,
+# z_chunk) as the level reports them (profiles/k_step_1024cubed_*.md)
+MEASURED_STEP_BYTES = {(4, 64): 88.99, (3, 0): 74.58, (2, 0): 97.07}
 
 
 def step_kernel_name(pair_variant):

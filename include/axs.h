@@ -415,7 +415,11 @@ int axs_boxlevel_fill_ghosts(axs_boxlevel *bl, const axs_boxlevel *coarse,
 /* One RK4 step of every box, each stage one persistent launch over the
  * box work-list, preceded by the ghost fill of its input at the stage time
  * (t0_frac, t0_frac + dt_frac/2, t0_frac + dt_frac/2, t0_frac + dt_frac of
- * the coarse interval; a ratio-2 subcycled fine step has dt_frac = 0.5). */
+ * the coarse interval; a ratio-2 subcycled fine step has dt_frac = 0.5).
+ * These fills write what the stage stencil reads, psi in the 2 ghost layers
+ * next to each box; the state's other ghost values are left as they were
+ * (axs_boxlevel_fill_ghosts fills all of them, as tagging and the energy
+ * sums need). */
 int axs_boxlevel_rk4_step(axs_boxlevel *bl, const axs_rhs_coeffs coeffs[4],
                           double dt, const axs_boxlevel *coarse,
                           double t0_frac, double dt_frac);
