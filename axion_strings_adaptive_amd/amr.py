@@ -16,6 +16,11 @@ steps from fresh tags (the reference's amr.regrid_int): a new level takes
 the old level's data where the two overlap and quartic interpolation from
 the level below elsewhere.
 
+The hierarchy's host logic is native (axs_amr.hip: the axs_hierarchy_*
+entry points that the C++ driver uses too); Hierarchy below drives it and
+views its levels, and the numpy block / box functions restate the native
+host steps for the CPU tests.
+
 One coarse step advances level l once and level l+1 twice (subcycling,
 docs/conventions.md:239-241): the coarse state is snapshotted, the coarse
 level takes its RK4 step, the fine level takes two half steps whose
@@ -36,6 +41,7 @@ its entry of the schedule (the reference's schedule-gated tagger).
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import sys
 import time
@@ -183,6 +189,13 @@ def erode_periodic(mask: np.ndarray, r: int) -> np.ndarray:
 
 
 class Hierarchy:
+    """The AMR hierarchy (the native axs_hierarchy_* of libaxs_hip.so: tag
+    -> blocks -> boxes, subcycled steps, regrids, composite sums) built on a
+    seeded-IC base level; its levels are BoxLevel views of the native ones.
+    The block and box functions above restate the native host steps
+    (axs_amr_tagged_blocks / axs_amr_erode / axs_amr_boxes_from_blocks) for
+    testing."""
+
     def __init__(self, ctx: capi.Context, n0: int = 256, levels: int = 2,
                  dx0: float = 0.5, precision: int = capi.AXS_FP32C,
                  amp_thr: float = 0.5, n_error_buf: int = 2,
@@ -194,6 +207,8 @@ class Hierarchy:
                  radial_gradient_threshold: float = capi.DBL_MAX,
                  level_add_log_mr_over_h=None):
         self.ctx = ctx
+        self.lib = ctx.lib
+        self.h = None
         # the box plan's level schedule (entry l-1: log(m_r/H) from which
         # level l may exist), or None for no gating
         self.level_add = (None if level_add_log_mr_over_h is None
@@ -205,13 +220,8 @@ class Hierarchy:
         self.n_error_buf = n_error_buf
         self.blocking_factor = blocking_factor
         self.max_grid_size = max_grid_size
-        self.nsteps = 0
-        self.nregrids = 0
-        self.regrid_s = 0.0  # wall time spent in regrid() by step()
-        self.regrid_parts = {}  # the same, by phase
         self.bg = bg if bg is not None else capi.make_background(
             2.0, c0=0.0, c1=2.0, tau_switch=1.0, has_switch=True)
-        self.tau = tau0
         self.precision = precision
         self.dx = [dx0 / 2 ** l for l in range(levels + 1)]
         self.dt0 = dt_multiplier * dx0
@@ -220,64 +230,86 @@ class Hierarchy:
         self.grad_thr = gradient_threshold
         self.rgrad_thr = radial_gradient_threshold
         self.verbose = verbose
+        self._views = {}
         self.levels: list[capi.BoxLevel] = []
-        self.domains = []
-        # ---- level 0: seeded IC on a uniform level, copied into boxes ----
+        p = capi.AmrParams()
+        p.max_level = levels
+        p.n_error_buf = n_error_buf
+        p.blocking_factor = blocking_factor
+        p.max_grid_size = max_grid_size
+        p.regrid_int = regrid_int
+        p.force_full_refinement = int(force_full_refinement)
+        p.gradient_threshold = gradient_threshold
+        p.radial_gradient_threshold = radial_gradient_threshold
+        p.amplitude_threshold = amp_thr
+        sched = self.level_add or []
+        if len(sched) > capi.AXS_MAX_LEVELS:
+            raise ValueError("at most 8 schedule entries")
+        p.n_schedule = len(sched)
+        for i, v in enumerate(sched):
+            p.level_add_log_mr_over_h[i] = v
+        p.use_graph = int(use_graph)
+        self.params = p
+        # ---- level 0: seeded IC on a uniform level ----
         base = capi.Level(ctx, n0, dx0, precision)
-        base.fourier_ic(0, n0 / 8.0, 0.5, seed, 0)
-        base.fourier_ic(1, n0 / 8.0, 0.5, seed, 1)
-        base.zero_components(2, 2)
-        e0 = min(max_grid_size, n0)
-        cuts = [(p, min(p + e0, n0) - 1) for p in range(0, n0, e0)]
-        boxes0 = [((x0, y0, z0), (x1, y1, z1)) for (z0, z1) in cuts
-                  for (y0, y1) in cuts for (x0, x1) in cuts]
-        L0 = capi.BoxLevel(ctx, boxes0, (n0,) * 3, dx0, precision)
-        L0.copy_from_level(base)
-        base.close()
-        L0.fill_ghosts()
-        self.levels.append(L0)
-        self.domains.append((n0,) * 3)
-        # ---- finer levels from the tagger ----
-        covered = np.ones((n0 // 8,) * 3, bool)  # level-0 blocks of 8 cells
-        for l in range(1, levels + 1):
-            coarse = self.levels[-1]
-            boxes, blk = self._fine_boxes(l - 1, covered)
-            if not boxes:
-                break
-            fd = tuple(2 * d for d in self.domains[-1])
-            Lf = capi.BoxLevel(ctx, boxes, fd, self.dx[l], precision)
-            Lf.fill_ghosts(coarse, 1.0, interior=True)
-            self.levels.append(Lf)
-            self.domains.append(fd)
-            # coverage of the new level in blocks of 8 of its own cells
-            covered = np.kron(blk, np.ones((2, 2, 2), bool))
-            if verbose:
-                print(f"[amr] level {l}: {len(boxes)} boxes, "
-                      f"{Lf.num_cells} cells", file=sys.stderr, flush=True)
-        self.ncells = [L.num_cells for L in self.levels]
+        try:
+            base.fourier_ic(0, n0 / 8.0, 0.5, seed, 0)
+            base.fourier_ic(1, n0 / 8.0, 0.5, seed, 1)
+            base.zero_components(2, 2)
+            h = capi._P()
+            capi.check(self.lib.axs_hierarchy_create(
+                ctx.h, base.h, C.byref(p), C.byref(self.bg), None, tau0,
+                self.dt0, C.byref(h)))
+        finally:
+            base.close()
+        self.h = h
+        ctx.adopt(self)
+        self._sync()
+        if verbose:
+            for l in range(1, len(self.levels)):
+                print(f"[amr] level {l}: {len(self.levels[l].boxes)} boxes, "
+                      f"{self.levels[l].num_cells} cells", file=sys.stderr,
+                      flush=True)
 
-    def _fine_boxes(self, l: int, covered: np.ndarray):
-        """Boxes of level l+1 from the tags of level l (ghosts filled):
-        tag, grow by n_error_buf, cut into blocking-factor blocks, keep them
-        one block inside the coverage of level l (l >= 1), merge."""
-        coarse = self.levels[l]
-        cd = self.domains[l]
-        if not self.level_allowed(l + 1):
-            return [], None
-        R = capi.background_eval(self.bg, 0, self.tau)
-        tags_buf, _ = coarse.tag(amp_thr=self.amp_thr, R=R,
-                                 grad_thr=self.grad_thr,
-                                 rgrad_thr=self.rgrad_thr)
-        tags = tags_buf.to_numpy(np.uint8).reshape(cd[2], cd[1], cd[0]) > 0
-        if self.force_full_refinement:  # axion_strings.tagging.force_full_refinement
-            tags[:] = True
-        cblock = self.blocking_factor // 2   # coarse cells per fine box edge
-        blk = tagged_blocks(tags, cblock, self.n_error_buf)
-        if l >= 1:  # proper nesting: one block inside the coarse coverage
-            blk &= erode_periodic(covered, 1)
-        boxes = boxes_from_blocks(blk, self.blocking_factor,
-                                  self.max_grid_size // self.blocking_factor)
-        return boxes, blk
+    def _sync(self):
+        """Refreshes the level views, their boxes and the counters."""
+        n, tau = C.c_int(), C.c_double()
+        ns, nr = C.c_int64(), C.c_int64()
+        secs = (C.c_double * 6)()
+        capi.check(self.lib.axs_hierarchy_info(self.h, C.byref(n),
+                                               C.byref(tau), C.byref(ns),
+                                               C.byref(nr), secs))
+        self.tau, self.nsteps, self.nregrids = tau.value, ns.value, nr.value
+        self.regrid_s = secs[0]
+        self.regrid_parts = {k: secs[i + 1] for i, k in enumerate(
+            ("tag_and_boxes", "create", "interpolate", "copy_valid", "free"))
+            if secs[i + 1] > 0.0}
+        levels, domains, views = [], [], {}
+        for l in range(n.value):
+            hl, dom = capi._P(), (C.c_int * 3)()
+            dxl, nb = C.c_double(), C.c_int()
+            capi.check(self.lib.axs_hierarchy_level(
+                self.h, l, C.byref(hl), dom, C.byref(dxl), C.byref(nb)))
+            lo = (C.c_int * (3 * nb.value))()
+            hi = (C.c_int * (3 * nb.value))()
+            capi.check(self.lib.axs_hierarchy_boxes(self.h, l, lo, hi))
+            boxes = [(tuple(lo[3 * b:3 * b + 3]), tuple(hi[3 * b:3 * b + 3]))
+                     for b in range(nb.value)]
+            v = self._views.get(hl.value)
+            if v is None or v.boxes != boxes or v.h is None:
+                v = capi.BoxLevel.borrowed(self.ctx, hl.value, boxes,
+                                           tuple(dom), dxl.value,
+                                           self.precision)
+            views[hl.value] = v
+            levels.append(v)
+            domains.append(tuple(dom))
+        for k, v in self._views.items():
+            if views.get(k) is not v:
+                v.h = None  # its native level is gone
+        self._views = views
+        self.levels = levels
+        self.domains = domains
+        self.ncells = [L.num_cells for L in levels]
 
     def level_allowed(self, l: int) -> bool:
         """Schedule gating of the tagger (AxionStringsLevel.cpp:1957-1983):
@@ -291,82 +323,20 @@ class Hierarchy:
         log_mr_over_h = -np.log(capi.background_eval(self.bg, 5, self.tau))
         return bool(log_mr_over_h >= self.level_add[l - 1])
 
-    def _lap(self, key, t0):
-        """Adds the device-synchronised time since t0 to regrid_parts[key]."""
-        self.ctx.synchronize()
-        t1 = time.perf_counter()
-        self.regrid_parts[key] = self.regrid_parts.get(key, 0.0) + t1 - t0
-        return t1
-
     def regrid(self):
         """Rebuilds levels 1.. from fresh tags at a synchronised time."""
-        covered = np.ones(tuple(d // 8 for d in self.domains[0]), bool)
-        t = time.perf_counter()
-        for l in range(1, self.max_levels + 1):
-            coarse = self.levels[l - 1]
-            coarse.fill_ghosts(self.levels[l - 2] if l >= 2 else None)
-            boxes, blk = self._fine_boxes(l - 1, covered)
-            t = self._lap("tag_and_boxes", t)
-            old = self.levels[l] if l < len(self.levels) else None
-            if not boxes:
-                for L in self.levels[l:]:
-                    L.close()
-                del self.levels[l:]
-                del self.domains[l:]
-                break
-            covered_next = np.kron(blk, np.ones((2, 2, 2), bool))
-            if old is not None and sorted(old.boxes) == sorted(
-                    tuple(map(tuple, b)) for b in boxes):
-                covered = covered_next
-                continue
-            fd = tuple(2 * d for d in self.domains[l - 1])
-            Lf = capi.BoxLevel(self.ctx, boxes, fd, self.dx[l], self.precision)
-            t = self._lap("create", t)
-            Lf.fill_ghosts(coarse, 1.0, interior=True)
-            t = self._lap("interpolate", t)
-            if old is not None:
-                Lf.copy_valid_from(old)
-                t = self._lap("copy_valid", t)
-                old.close()
-                t = self._lap("free", t)
-                self.levels[l] = Lf
-            else:
-                self.levels.append(Lf)
-                self.domains.append(fd)
-            Lf.fill_ghosts(coarse, 1.0)
-            t = self._lap("interpolate", t)
-            covered = covered_next
-        self.ncells = [L.num_cells for L in self.levels]
-        self.nregrids += 1
+        capi.check(self.lib.axs_hierarchy_regrid(self.h))
+        self._sync()
 
     def coeffs(self, tau, dt):
         return [capi.rhs_coeffs(self.bg, t) for t in
                 (tau, tau + 0.5 * dt, tau + 0.5 * dt, tau + dt)]
 
-    def _advance(self, l: int, tau: float, dt: float, t0: float, df: float):
-        L = self.levels[l]
-        coarse = self.levels[l - 1] if l > 0 else None
-        if l + 1 < len(self.levels):
-            self.levels[l + 1].snapshot_coarse(L)
-        L.rk4_step(self.coeffs(tau, dt), dt, coarse, t0, df,
-                   use_graph=self.use_graph)
-        if l + 1 < len(self.levels):
-            self._advance(l + 1, tau, 0.5 * dt, 0.0, 0.5)
-            self._advance(l + 1, tau + 0.5 * dt, 0.5 * dt, 0.5, 0.5)
-            self.levels[l + 1].average_down_to(L)
-
     def step(self):
         """One coarse step of the whole hierarchy (then a regrid every
         regrid_int coarse steps)."""
-        self._advance(0, self.tau, self.dt0, 0.0, 1.0)
-        self.tau += self.dt0
-        self.nsteps += 1
-        if self.regrid_int > 0 and self.nsteps % self.regrid_int == 0:
-            self.ctx.synchronize()
-            t0 = time.perf_counter()
-            self.regrid()
-            self.ctx.synchronize()
-            self.regrid_s += time.perf_counter() - t0
+        capi.check(self.lib.axs_hierarchy_step(self.h))
+        self._sync()
 
     def cell_updates_per_coarse_step(self) -> int:
         return sum(n * 2 ** l for l, n in enumerate(self.ncells))
@@ -378,39 +348,33 @@ class Hierarchy:
         lam = capi.background_eval(self.bg, 3, self.tau)
         RpR = capi.background_eval(self.bg, 8, self.tau)
         acc = np.zeros(14)
-        for l, L in enumerate(self.levels):
-            L.fill_ghosts(self.levels[l - 1] if l > 0 else None)
-            cover = None
-            if l + 1 < len(self.levels):
-                cover = self.levels[l + 1].coverage_mask(self.domains[l])
-            e = L.energy_sums(R, lam, RpR, scheme, thr, cover=cover)
-            acc += e * self.dx[l] ** 3
+        m = capi.Masking(scheme, thr)
+        capi.check(self.lib.axs_hierarchy_energy_sums(
+            self.h, R, lam, RpR, C.byref(m), capi.dptr(acc)))
         n_total, n_unmasked = acc[13], acc[12]
         return {"rho_tot_unscreened": acc[0] / n_total,
                 "rho_tot_screened": acc[1] / n_unmasked if n_unmasked else 0.0,
                 "n_total": n_total, "n_unmasked": n_unmasked}
 
+    def plaquette_counts(self) -> np.ndarray:
+        """[level, (plain, weighted)] pierced plaquettes over the cells no
+        finer level covers."""
+        out = np.zeros((len(self.levels), 2), np.int64)
+        capi.check(self.lib.axs_hierarchy_plaquette_counts(
+            self.h, out.ctypes.data_as(C.POINTER(C.c_int64))))
+        return out
+
     def composite_level0(self) -> capi.Level:
-        """collapse_to_level0 (AxionStringsLevel.cpp:110-134): copies of the
-        levels averaged down from the finest to level 0, which then holds
-        level 0's own data where nothing refines it and the finer data's
-        average where something does; returned as a uniform level-0
-        resolution field (the caller closes it).  The hierarchy itself is
-        left unchanged."""
-        src = self.levels[-1]
-        tmp = []
-        for l in range(len(self.levels) - 2, -1, -1):
-            L = self.levels[l]
-            c = capi.BoxLevel(self.ctx, L.boxes, self.domains[l], self.dx[l],
-                              self.precision)
-            c.copy_valid_from(L)
-            src.average_down_to(c)
-            tmp.append(c)
-            src = c
+        """collapse_to_level0 (AxionStringsLevel.cpp:110-134): level 0's own
+        data where nothing refines it and the finer data's recursive average
+        where something does, as a uniform level-0 resolution field (the
+        caller closes it).  The hierarchy itself is left unchanged."""
         out = capi.Level(self.ctx, self.domains[0], self.dx[0], self.precision)
-        src.copy_to_level(out)
-        for c in tmp:
-            c.close()
+        try:
+            capi.check(self.lib.axs_hierarchy_collapse(self.h, out.h))
+        except Exception:
+            out.close()
+            raise
         return out
 
     def composite_spectrum(self, kind=capi.AXS_FIELD_MASKED_A_DOT,
@@ -427,8 +391,18 @@ class Hierarchy:
             lvl.close()
 
     def close(self):
-        for L in self.levels:
-            L.close()
+        if self.h:
+            for v in self._views.values():
+                v.h = None
+            self.levels = []
+            self.lib.axs_hierarchy_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def main(argv=None):
@@ -475,8 +449,7 @@ def main(argv=None):
                   **kw)
     h.step()  # warm-up
     ctx.synchronize()
-    h.regrid_s, n_regrid0 = 0.0, h.nregrids
-    h.regrid_parts = {}
+    rs0, n_regrid0, parts0 = h.regrid_s, h.nregrids, dict(h.regrid_parts)
     t0 = time.perf_counter()
     steps = 0
     for _ in range(args.steps):
@@ -505,11 +478,12 @@ def main(argv=None):
         "cell_updates_per_coarse_step": h.cell_updates_per_coarse_step(),
         "cell_updates_per_s": h.cell_updates_per_coarse_step() * args.steps / dt,
         "ms_per_coarse_step": 1e3 * dt / args.steps,
-        "regrid_ms_mean": (1e3 * h.regrid_s / (h.nregrids - n_regrid0)
+        "regrid_ms_mean": (1e3 * (h.regrid_s - rs0) / (h.nregrids - n_regrid0)
                            if h.nregrids > n_regrid0 else None),
         "ms_per_coarse_step_without_regrids":
-            1e3 * (dt - h.regrid_s) / args.steps,
-        "regrid_ms_by_phase": {k: 1e3 * v / max(1, h.nregrids - n_regrid0)
+            1e3 * (dt - (h.regrid_s - rs0)) / args.steps,
+        "regrid_ms_by_phase": {k: 1e3 * (v - parts0.get(k, 0.0)) /
+                               max(1, h.nregrids - n_regrid0)
                                for k, v in h.regrid_parts.items()},
         "tau": h.tau, "tau_f": tau_f,
         "n_total": e["n_total"],

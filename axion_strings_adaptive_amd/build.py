@@ -33,6 +33,7 @@ HIP_SOURCES = [
     "axs_fft.hip",
     "axs_boxes.hip",
     "axs_adjoint.hip",
+    "axs_amr.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -54,6 +54,27 @@ class TaggerParams(C.Structure):
                 ("amplitude_threshold", C.c_double), ("R", C.c_double)]
 
 
+AXS_MAX_LEVELS = 8
+
+
+class AmrParams(C.Structure):
+    _fields_ = [("max_level", C.c_int), ("n_error_buf", C.c_int),
+                ("blocking_factor", C.c_int), ("max_grid_size", C.c_int),
+                ("regrid_int", C.c_int), ("force_full_refinement", C.c_int),
+                ("gradient_threshold", C.c_double),
+                ("radial_gradient_threshold", C.c_double),
+                ("amplitude_threshold", C.c_double),
+                ("n_schedule", C.c_int),
+                ("level_add_log_mr_over_h", C.c_double * AXS_MAX_LEVELS),
+                ("use_graph", C.c_int)]
+
+
+class CoeffBlend(C.Structure):
+    _fields_ = [("tau_start", C.c_double), ("dtau", C.c_double),
+                ("lambda_start", C.c_double),
+                ("curvature_start", C.c_double)]
+
+
 class Masking(C.Structure):
     _fields_ = [("scheme", C.c_int), ("threshold", C.c_double)]
 
@@ -71,6 +92,10 @@ _SIGS = {
     "axs_ctx_synchronize": (_I, [_P]),
     "axs_ctx_trim": (_I, [_P]),
     "axs_ctx_stream": (_P, [_P]),
+    "axs_device_alloc": (_I, [_P, C.c_size_t, C.POINTER(_P)]),
+    "axs_device_free": (_I, [_P, _P]),
+    "axs_device_memset": (_I, [_P, _P, _I, C.c_size_t]),
+    "axs_device_download": (_I, [_P, _P, _P, C.c_size_t]),
     "axs_last_error": (C.c_char_p, []),
     "axs_num_tile_variants": (_I, []),
     "axs_tuning_load": (_I, [C.c_char_p]),
@@ -134,6 +159,25 @@ _SIGS = {
     "axs_boxlevel_tag": (_I, [_P, C.POINTER(TaggerParams), _P, _P, _PI64]),
     "axs_boxlevel_energy_sums": (_I, [_P, _D, _D, _D, C.POINTER(Masking), _P,
                                       _PD]),
+    "axs_hierarchy_create": (_I, [_P, _P, C.POINTER(AmrParams),
+                                  C.POINTER(Background), C.POINTER(CoeffBlend),
+                                  _D, _D, C.POINTER(_P)]),
+    "axs_hierarchy_destroy": (_I, [_P]),
+    "axs_hierarchy_step": (_I, [_P]),
+    "axs_hierarchy_regrid": (_I, [_P]),
+    "axs_hierarchy_info": (_I, [_P, C.POINTER(_I), _PD, _PI64, _PI64, _PD]),
+    "axs_hierarchy_level": (_I, [_P, _I, C.POINTER(_P), C.POINTER(_I), _PD,
+                                 C.POINTER(_I)]),
+    "axs_hierarchy_boxes": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_I)]),
+    "axs_hierarchy_energy_sums": (_I, [_P, _D, _D, _D, C.POINTER(Masking),
+                                       _PD]),
+    "axs_hierarchy_plaquette_counts": (_I, [_P, _PI64]),
+    "axs_hierarchy_collapse": (_I, [_P, _P]),
+    "axs_amr_tagged_blocks": (_I, [_P, C.POINTER(_I), _I, _I, _P]),
+    "axs_amr_erode": (_I, [_P, C.POINTER(_I), _I, _P]),
+    "axs_amr_boxes_from_blocks": (_I, [_P, C.POINTER(_I), _I, _I, _I,
+                                       C.POINTER(_I), C.POINTER(_I),
+                                       C.POINTER(_I)]),
     "axs_boxlevel_coverage_mask": (_I, [_P, C.POINTER(_I), _P]),
     "axs_boxlevel_average_down": (_I, [_P, _P]),
     "axs_adjoint_grad_energy": (_I, [_P, C.POINTER(Background), _D, _D, _D,
@@ -515,7 +559,21 @@ class BoxLevel:
         check(self.lib.axs_boxlevel_create(ctx.h, len(boxes), lo, hi, dom, 3,
                                            dx, precision, C.byref(h)))
         self.h = h
+        self.owned = True
         ctx.adopt(self)
+
+    @classmethod
+    def borrowed(cls, ctx: Context, handle: int, boxes, domain, dx: float,
+                 precision: int) -> "BoxLevel":
+        """A view of a box level that another object (an AMR hierarchy)
+        owns: close() forgets the handle without destroying the level."""
+        self = cls.__new__(cls)
+        self.ctx, self.lib = ctx, ctx.lib
+        self.domain, self.dx, self.precision = tuple(domain), dx, precision
+        self.boxes = [tuple(map(tuple, b)) for b in boxes]
+        self.h = _P(handle)
+        self.owned = False
+        return self
 
     @property
     def num_cells(self) -> int:
@@ -607,7 +665,8 @@ class BoxLevel:
 
     def close(self):
         if self.h:
-            self.lib.axs_boxlevel_destroy(self.h)
+            if self.owned:
+                self.lib.axs_boxlevel_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -461,6 +461,49 @@ int axs_ctx_synchronize(axs_ctx *c)
 
 void *axs_ctx_stream(axs_ctx *c) { return c ? (void *)c->stream : nullptr; }
 
+int axs_device_alloc(axs_ctx *c, size_t bytes, void **out)
+{
+    AXS_CHECK_ARG(c && out, "axs_device_alloc: NULL argument");
+    *out = nullptr;
+    AXS_HIP_TRY(hipSetDevice(c->device));
+    if (hipMalloc(out, bytes ? bytes : 1) != hipSuccess)
+    {
+        *out = nullptr;
+        return set_error(AXS_ERR_NOMEM, "axs_device_alloc: out of memory (" +
+                                            std::to_string(bytes) + " bytes)");
+    }
+    return AXS_OK;
+}
+
+int axs_device_free(axs_ctx *c, void *ptr)
+{
+    AXS_CHECK_ARG(c, "axs_device_free: NULL context");
+    if (!ptr)
+        return AXS_OK;
+    AXS_HIP_TRY(hipStreamSynchronize(c->stream));
+    AXS_HIP_TRY(hipFree(ptr));
+    return AXS_OK;
+}
+
+int axs_device_memset(axs_ctx *c, void *ptr, int value, size_t bytes)
+{
+    AXS_CHECK_ARG(c && (ptr || !bytes), "axs_device_memset: NULL argument");
+    if (bytes)
+        AXS_HIP_TRY(hipMemsetAsync(ptr, value, bytes, c->stream));
+    return AXS_OK;
+}
+
+int axs_device_download(axs_ctx *c, void *host, const void *dev, size_t bytes)
+{
+    AXS_CHECK_ARG(c && ((host && dev) || !bytes),
+                  "axs_device_download: NULL argument");
+    if (bytes)
+        AXS_HIP_TRY(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost,
+                                   c->stream));
+    AXS_HIP_TRY(hipStreamSynchronize(c->stream));
+    return AXS_OK;
+}
+
 int axs_background_eval(const axs_background *bg, int which, double x,
                         double *out)
 {

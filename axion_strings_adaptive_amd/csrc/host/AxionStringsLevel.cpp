@@ -5,6 +5,7 @@
 #include <cmath>
 #include <chrono>
 #include <cstring>
+#include <filesystem>
 #include <functional>
 #include <ctime>
 #include <fstream>
@@ -31,6 +32,9 @@ AxionStringsLevel::AxionStringsLevel(const ParamTable &params,
                                      const std::string &out_dir)
     : m_params(params), m_out_dir(out_dir)
 {
+    // the output folder is made if it does not exist
+    std::error_code ec;
+    std::filesystem::create_directories(m_out_dir, ec);
     Params::check_params(m_params);
     ParmParse amr(m_params, "amr"), geom(m_params, "geometry"),
         evo(m_params, "evolution"), as(m_params, "axion_strings");
@@ -38,11 +42,7 @@ AxionStringsLevel::AxionStringsLevel(const ParamTable &params,
     amr.getarr("n_cell", n_cell);
     if (n_cell.size() != 3)
         amr.error("n_cell", "needs 3 values");
-    int max_level = 0;
-    amr.queryAdd("max_level", max_level);
-    if (max_level != 0)
-        amr.error("max_level", "the uniform-level driver runs max_level = 0; "
-                               "refined levels use the box-level API");
+    m_amr = Params::read_amr_keys(m_params);
     std::vector<double> extent;
     geom.getarr("prob_extent", extent);
     if (extent.size() != 3)
@@ -109,6 +109,19 @@ AxionStringsLevel::AxionStringsLevel(const ParamTable &params,
     amr.query("restart", m_restart);
     if (!m_restart.empty() && !std::ifstream(m_restart).good())
         amr.error("restart", "cannot open checkpoint " + m_restart);
+    if (m_amr.max_level > 0)
+    {
+        if (m_check_int > 0 || !m_restart.empty())
+            amr.error("max_level", "checkpoints of refined runs are not "
+                                   "supported here (amr.check_int, "
+                                   "amr.restart)");
+        if (m_n[0] != m_n[1] || m_n[0] != m_n[2])
+            amr.error("n_cell", "refined runs need a cubic base grid here");
+        // the box plan's level schedule (AxionStringsLevel.cpp:178-184)
+        if (!m_flat &&
+            as.contains("derived_level_add_log_mr_over_h"))
+            as.getarr("derived_level_add_log_mr_over_h", m_level_add);
+    }
     as.queryAdd("compute_spectrum", m_compute_spectrum);
     std::string field = "a_dot";
     as.queryAdd("spectrum_field", field);
@@ -147,6 +160,7 @@ AxionStringsLevel::AxionStringsLevel(const ParamTable &params,
 
 AxionStringsLevel::~AxionStringsLevel()
 {
+    axs_hierarchy_destroy(m_hier);
     if (m_scalars)
         std::fclose(m_scalars);
     if (m_spectrum)
@@ -304,6 +318,8 @@ void AxionStringsLevel::evolve()
 {
     if (m_phase == Phase::Relaxing)
         evolve_relaxing();
+    if (m_phase == Phase::Evolving && m_amr.max_level > 0 && !m_hier)
+        build_hierarchy();
     while (m_phase == Phase::Evolving)
     {
         long k = std::min(steps_until_next_output(), steps_to_checkpoint());
@@ -322,7 +338,14 @@ void AxionStringsLevel::evolve()
         const bool blending =
             m_handoff_dtau > 0.0 &&
             m_tau < m_handoff_tau_start + m_handoff_dtau + m_dt;
-        if (blending)
+        if (m_hier)
+        {
+            // coarse steps of the hierarchy (its stage scalars carry the
+            // handoff blend themselves)
+            for (long j = 0; j < k; ++j)
+                axs_check(axs_hierarchy_step(m_hier), "axs_hierarchy_step");
+        }
+        else if (blending)
         {
             std::vector<axs_rhs_coeffs> table((size_t)k * 4);
             double tau = m_tau;
@@ -555,12 +578,27 @@ void AxionStringsLevel::diagnostics()
     }
     const double m_r = std::sqrt(lambda);
 
-    const PlaquetteCounts counts = count_plaquettes(m_level);
-    const double ell = (2.0 / 3.0) * (double)counts.n_p_plain * m_dx;
-    const double ell_w = (2.0 / 3.0) * (double)counts.n_p_weighted * m_dx;
+    PlaquetteCounts counts;
+    double ell = 0.0, ell_w = 0.0;
+    TotalEnergyResult composite{};
+    if (m_hier)
+    {
+        // composite counts and energies over the hierarchy; the level-0
+        // collapse into m_level for the velocities and the spectrum
+        composite_sums(R, m_flat ? m_r * m_r : lambda, RpR, &counts, &ell,
+                       &ell_w, &composite);
+        axs_check(axs_hierarchy_collapse(m_hier, m_level),
+                  "axs_hierarchy_collapse");
+    }
+    else
+    {
+        counts = count_plaquettes(m_level);
+        ell = (2.0 / 3.0) * (double)counts.n_p_plain * m_dx;
+        ell_w = (2.0 / 3.0) * (double)counts.n_p_weighted * m_dx;
+    }
     if (m_flat)
     {
-        flat_diagnostics(counts, ell, m_r);
+        flat_diagnostics(counts, ell, m_r, m_hier ? &composite : nullptr);
         return;
     }
     const double xi = m_flat ? 0.0 : xi_from_comoving_length(ell, m_L, m_bg.a_inv, tau);
@@ -572,8 +610,11 @@ void AxionStringsLevel::diagnostics()
     const double vw_sum = vel.sum_gamma_sq_v_sq * m_dx;
     const double vw_cnt = (double)vel.count * m_dx;
 
-    const TotalEnergyResult e = compute_composite_total_energy(
-        {EnergyLevelInput{m_level, m_dx, nullptr}}, R, lambda, RpR, m_masking);
+    const TotalEnergyResult e =
+        m_hier ? composite
+               : compute_composite_total_energy(
+                     {EnergyLevelInput{m_level, m_dx, nullptr}}, R, lambda,
+                     RpR, m_masking);
 
     const double sum_core = e.n_total * e.rho_tot_unscreened -
                             e.n_unmasked * e.rho_tot_screened;
@@ -598,6 +639,27 @@ void AxionStringsLevel::diagnostics()
               << "  m_r/H = " << m_r_over_h
               << "  rho_tot unscreened = " << e.rho_tot_unscreened
               << "  screened = " << e.rho_tot_screened << "\n";
+    if (m_hier)
+    {
+        int nl = 0;
+        int64_t nreg = 0;
+        axs_check(axs_hierarchy_info(m_hier, &nl, nullptr, nullptr, &nreg,
+                                     nullptr),
+                  "axs_hierarchy_info");
+        std::cout << "  [AxionStrings AMR] levels = " << nl
+                  << "  regrids = " << nreg << "  cells:";
+        for (int l = 0; l < nl; ++l)
+        {
+            axs_boxlevel *bl = nullptr;
+            int nb = 0;
+            axs_check(axs_hierarchy_level(m_hier, l, &bl, nullptr, nullptr,
+                                          &nb),
+                      "axs_hierarchy_level");
+            std::cout << " " << axs_boxlevel_num_cells(bl) << " (" << nb
+                      << " boxes)";
+        }
+        std::cout << "\n";
+    }
 
     if (!m_compute_spectrum)
         return;
@@ -643,16 +705,19 @@ void AxionStringsLevel::diagnostics()
 // conserved total energy), mean gamma^2 v^2 at pierced corners, one
 // loop_scalars.dat row per output; no xi, tension or spectrum.
 void AxionStringsLevel::flat_diagnostics(const PlaquetteCounts &counts,
-                                         double ell, double m_r)
+                                         double ell, double m_r,
+                                         const TotalEnergyResult *composite)
 {
     const double t = m_tau;
     const VelocityResult vel =
         compute_velocity_at_pierced_corners(m_level, 1.0, 0.0, m_r);
     const double vw_sum = vel.sum_gamma_sq_v_sq * m_dx;
     const double vw_cnt = (double)vel.count * m_dx;
-    const TotalEnergyResult e = compute_composite_total_energy(
-        {EnergyLevelInput{m_level, m_dx, nullptr}}, 1.0, m_r * m_r, 0.0,
-        m_masking);
+    const TotalEnergyResult e =
+        composite ? *composite
+                  : compute_composite_total_energy(
+                        {EnergyLevelInput{m_level, m_dx, nullptr}}, 1.0,
+                        m_r * m_r, 0.0, m_masking);
     const double total_energy = e.n_total * e.rho_tot_unscreened;
     const double mean_g2v2 = vw_cnt > 0.0 ? vw_sum / vw_cnt : 0.0;
     const double mean_gamma = std::sqrt(1.0 + mean_g2v2);
@@ -683,6 +748,92 @@ void AxionStringsLevel::flat_diagnostics(const PlaquetteCounts &counts,
         std::fprintf(m_loop_scalars, i ? " %.10e" : "%.10e", row[i]);
     std::fprintf(m_loop_scalars, "\n");
     std::fflush(m_loop_scalars);
+}
+
+// The hierarchy from the uniform level at the start of the main phase
+// (tag_cells never refines while relaxing, AxionStringsLevel.cpp:1924-1934).
+void AxionStringsLevel::build_hierarchy()
+{
+    axs_amr_params p{};
+    p.max_level = m_amr.max_level;
+    p.n_error_buf = m_amr.n_error_buf;
+    p.blocking_factor = m_amr.blocking_factor;
+    p.max_grid_size = m_amr.max_grid_size;
+    p.regrid_int = m_amr.regrid_int;
+    p.force_full_refinement = m_tagging.force_full_refinement ? 1 : 0;
+    p.gradient_threshold = m_tagging.gradient_threshold;
+    p.radial_gradient_threshold = m_tagging.radial_gradient_threshold;
+    p.amplitude_threshold = m_tagging.amplitude_threshold;
+    p.n_schedule = (int)std::min<size_t>(m_level_add.size(), AXS_MAX_LEVELS);
+    for (int i = 0; i < p.n_schedule; ++i)
+        p.level_add_log_mr_over_h[i] = m_level_add[(size_t)i];
+    p.use_graph = m_use_graph ? 1 : 0;
+    const axs_coeff_blend blend{m_handoff_tau_start, m_handoff_dtau,
+                                m_handoff_lambda_start, m_handoff_curv_start};
+    axs_check(axs_hierarchy_create(m_ctx, m_level, &p, &m_cbg, &blend, m_tau,
+                                   m_dt, &m_hier),
+              "axs_hierarchy_create");
+    int nl = 0;
+    axs_check(axs_hierarchy_info(m_hier, &nl, nullptr, nullptr, nullptr,
+                                 nullptr),
+              "axs_hierarchy_info");
+    std::cout << "  [AxionStrings AMR] hierarchy at tau = " << m_tau << ": "
+              << nl << " level(s), max_level " << p.max_level
+              << ", blocking_factor " << p.blocking_factor
+              << ", max_grid_size " << p.max_grid_size << ", n_error_buf "
+              << p.n_error_buf << ", regrid_int " << p.regrid_int << "\n";
+}
+
+// The composite xi counts and energies of AxionStringsLevel.cpp:1299-1361:
+// each level over the cells no finer level covers, lengths weighted by the
+// level's dx, energies by dx^3.
+void AxionStringsLevel::composite_sums(double R, double lambda, double RpR,
+                                       PlaquetteCounts *counts, double *ell,
+                                       double *ell_w, TotalEnergyResult *e)
+{
+    int nl = 0;
+    axs_check(axs_hierarchy_info(m_hier, &nl, nullptr, nullptr, nullptr,
+                                 nullptr),
+              "axs_hierarchy_info");
+    std::vector<int64_t> c((size_t)2 * nl, 0);
+    axs_check(axs_hierarchy_plaquette_counts(m_hier, c.data()),
+              "axs_hierarchy_plaquette_counts");
+    *counts = PlaquetteCounts{};
+    *ell = *ell_w = 0.0;
+    for (int l = 0; l < nl; ++l)
+    {
+        axs_boxlevel *bl = nullptr;
+        double dxl = 0.0;
+        axs_check(axs_hierarchy_level(m_hier, l, &bl, nullptr, &dxl, nullptr),
+                  "axs_hierarchy_level");
+        counts->n_p_plain += (long)c[(size_t)2 * l];
+        counts->n_p_weighted += (long)c[(size_t)2 * l + 1];
+        *ell += (2.0 / 3.0) * (double)c[(size_t)2 * l] * dxl;
+        *ell_w += (2.0 / 3.0) * (double)c[(size_t)2 * l + 1] * dxl;
+    }
+    const axs_masking m = to_c(m_masking);
+    double acc[14];
+    axs_check(axs_hierarchy_energy_sums(m_hier, R, lambda, RpR, &m, acc),
+              "axs_hierarchy_energy_sums");
+    TotalEnergyResult o{};
+    o.n_total = acc[13];
+    o.n_unmasked = acc[12];
+    o.rho_tot_unscreened = acc[0] / o.n_total;
+    o.rho_axion_kinetic_unscreened = acc[2] / o.n_total;
+    o.rho_axion_gradient_unscreened = acc[4] / o.n_total;
+    o.rho_radial_kinetic_unscreened = acc[6] / o.n_total;
+    o.rho_radial_gradient_unscreened = acc[8] / o.n_total;
+    o.rho_radial_mass_unscreened = acc[10] / o.n_total;
+    if (o.n_unmasked > 0.0)
+    {
+        o.rho_tot_screened = acc[1] / o.n_unmasked;
+        o.rho_axion_kinetic_screened = acc[3] / o.n_unmasked;
+        o.rho_axion_gradient_screened = acc[5] / o.n_unmasked;
+        o.rho_radial_kinetic_screened = acc[7] / o.n_unmasked;
+        o.rho_radial_gradient_screened = acc[9] / o.n_unmasked;
+        o.rho_radial_mass_screened = acc[11] / o.n_unmasked;
+    }
+    *e = o;
 }
 
 void AxionStringsLevel::write_network_scalars_row(

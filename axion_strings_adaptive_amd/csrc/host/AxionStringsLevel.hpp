@@ -20,7 +20,13 @@
 //                                  counts/xi/energy/velocity :1299-1361,
 //                                  network_scalars.dat :1536-1579,
 //                                  spectrum + axion_spectrum.dat :1593-1764)
-// on one uniform periodic level, the level-0 path of the reference.
+// on one uniform periodic level, the level-0 path of the reference, or, with
+// amr.max_level > 0, on the AMR hierarchy (axs_hierarchy_*) built from that
+// level once the main phase starts: subcycled steps, regrids every
+// amr.regrid_int coarse steps, and the composite diagnostics of
+// AxionStringsLevel.cpp:1256-1361 (counts and energies over every level,
+// each masked by the coverage of the next finer; velocities and the
+// spectrum on the level-0 collapse, collapse_to_level0 :110-134).
 #pragma once
 
 #include "AxionStringsOps.hpp"
@@ -29,6 +35,7 @@
 #include <cstdio>
 #include <functional>
 #include <string>
+#include <vector>
 
 namespace axs_host
 {
@@ -51,6 +58,7 @@ class AxionStringsLevel
     // xi checks (each measured from a synchronised device)
     double diagnostics_seconds() const { return m_diag_seconds; }
     axs_level *level() { return m_level; }
+    axs_hierarchy *hierarchy() { return m_hier; }
 
     enum class Phase
     {
@@ -72,7 +80,11 @@ class AxionStringsLevel
     bool output_due();
     long steps_until_next_output();
     void flat_diagnostics(const PlaquetteCounts &counts, double ell,
-                          double m_r);
+                          double m_r, const TotalEnergyResult *composite);
+    void build_hierarchy();
+    void composite_sums(double R, double lambda, double RpR,
+                        PlaquetteCounts *counts, double *ell, double *ell_w,
+                        TotalEnergyResult *e);
     void timed_diagnostics(const std::function<void()> &f);
     void write_network_scalars_row(const PlaquetteCounts &c, double xi,
                                    double xi_w, double m_r_over_h,
@@ -84,6 +96,10 @@ class AxionStringsLevel
     std::string m_out_dir;
     axs_ctx *m_ctx{nullptr};
     axs_level *m_level{nullptr};
+    // refined runs: the hierarchy (from the main phase on) and its keys
+    axs_hierarchy *m_hier{nullptr};
+    Params::AmrKeys m_amr;
+    std::vector<double> m_level_add; // the box plan's level schedule
     int m_n[3]{};
     double m_dx{}, m_L{}, m_dt{};
     int m_precision{AXS_FP64};

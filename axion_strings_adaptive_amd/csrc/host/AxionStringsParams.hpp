@@ -23,6 +23,7 @@
 #include "BoxPlan.hpp"
 #include "ParmParse.hpp"
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
@@ -510,6 +511,113 @@ inline void apply_box_plan(ParamTable &t, const Background &b, double tau_i,
     plan_stop(t, plan.tau_f, log);
 }
 
+// The amr.* keys of a refined run, read by AMReX's AmrMesh / Amr in the
+// reference; absent keys take AMReX's defaults (n_error_buf 1,
+// blocking_factor 8, max_grid_size 32 in 3-D, regrid_int 1).  The box
+// levels here hold boxes of 8..64 cells per edge on corners that are
+// multiples of 8, so blocking_factor is 8, 16, 32 or 64 and max_grid_size a
+// multiple of it up to 64.
+struct AmrKeys
+{
+    int max_level{0};
+    int n_error_buf{1};
+    int blocking_factor{8};
+    int max_grid_size{32};
+    int regrid_int{1};
+};
+
+inline AmrKeys read_amr_keys(const ParamTable &t)
+{
+    ParmParse pp(t, "amr");
+    AmrKeys k{};
+    pp.queryAdd("max_level", k.max_level);
+    if (k.max_level < 0 || k.max_level > 8)
+        pp.error("max_level", "must be in 0..8");
+    pp.queryAdd("n_error_buf", k.n_error_buf);
+    pp.queryAdd("blocking_factor", k.blocking_factor);
+    pp.queryAdd("max_grid_size", k.max_grid_size);
+    pp.queryAdd("regrid_int", k.regrid_int);
+    if (k.max_level == 0)
+        return k;
+    const int bf = k.blocking_factor;
+    if (bf != 8 && bf != 16 && bf != 32 && bf != 64)
+        pp.error("blocking_factor", "must be 8, 16, 32 or 64 here");
+    if (k.max_grid_size < bf || k.max_grid_size > 64 ||
+        k.max_grid_size % bf != 0)
+        pp.error("max_grid_size", "must be a multiple of amr.blocking_factor "
+                                  "and at most 64 here");
+    if (k.n_error_buf < 0 || k.n_error_buf > bf / 2)
+        pp.error("n_error_buf", "must be in 0..amr.blocking_factor/2 here");
+    return k;
+}
+
+// apply_regrid_buffer_policy, AxionStringsParams.hpp:403-494: asked for by
+// axion_strings.tagging.regrid_interval_steps = N > 0 on a refined run.
+// Between two regrids N steps apart a string moving at v = 1 crosses
+// N * evolution.dt_multiplier cells, so the tag buffer must be at least
+// ceil(buffer_safety_factor * N * dt_multiplier) cells (and 1): amr.n_error_buf
+// is set to that when absent and must not be smaller when given, and
+// amr.regrid_int is set to N when absent and must equal N when given.
+inline void apply_regrid_buffer_policy(ParamTable &t, std::ostream *log)
+{
+    ParmParse amr(t, "amr");
+    int max_level = 0;
+    amr.queryAdd("max_level", max_level);
+    if (max_level <= 0)
+        return;
+    ParmParse tag(t, "axion_strings.tagging");
+    int every = 0;
+    tag.queryAdd("regrid_interval_steps", every);
+    if (every == 0)
+        return;
+    if (every < 0)
+        tag.error("regrid_interval_steps", "must be > 0");
+    double safety = 1.0;
+    tag.queryAdd("buffer_safety_factor", safety);
+    if (safety < 1.0)
+        tag.error("buffer_safety_factor", "must be >= 1.0");
+    double dtm = 0.0;
+    ParmParse(t, "evolution").get("dt_multiplier", dtm);
+    const int need =
+        std::max(1, static_cast<int>(std::ceil(safety * every * dtm)));
+    if (log)
+        *log << "regrid buffer: " << every << " steps between regrids, "
+             << "safety factor " << safety << ", dt_multiplier " << dtm
+             << " -> at least " << need << " buffer cells\n";
+    if (amr.contains("n_error_buf"))
+    {
+        int given = 0;
+        amr.get("n_error_buf", given);
+        if (given < need)
+            amr.error("n_error_buf",
+                      "is below the minimum buffer for axion_strings.tagging."
+                      "regrid_interval_steps / buffer_safety_factor (" +
+                          std::to_string(need) +
+                          " cells): remove it or raise it");
+    }
+    else
+    {
+        t.set("amr.n_error_buf", std::to_string(need));
+        if (log)
+            *log << "  amr.n_error_buf = " << need << "\n";
+    }
+    if (amr.contains("regrid_int"))
+    {
+        int given = 0;
+        amr.get("regrid_int", given);
+        if (given != every)
+            amr.error("regrid_int",
+                      "differs from axion_strings.tagging."
+                      "regrid_interval_steps: remove one or make them equal");
+    }
+    else
+    {
+        t.set("amr.regrid_int", std::to_string(every));
+        if (log)
+            *log << "  amr.regrid_int = " << every << "\n";
+    }
+}
+
 // The rest of check_params, AxionStringsParams.hpp:1142-1208, plus the
 // masking / tagging reads that AxionStringsLevel::variableSetUp performs at
 // level set-up (AxionStringsLevel.cpp:186-190), so that every key error
@@ -518,6 +626,7 @@ inline void validate_params(const ParamTable &t)
 {
     read_masking_params(t, "axion_strings.masking");
     read_tagging_params(t);
+    read_amr_keys(t);
     bool flat_space = false;
     ParmParse(t, "axion_strings").queryAdd("flat_space", flat_space);
     if (flat_space)
@@ -569,6 +678,7 @@ inline void check_params(ParamTable &t, std::ostream *log = nullptr)
         const Background b = read_background(t);
         apply_box_plan(t, b, read_tau_i(t, b), log);
     }
+    apply_regrid_buffer_policy(t, log);
     validate_params(t);
 }
 

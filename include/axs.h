@@ -118,6 +118,17 @@ int axs_ctx_synchronize(axs_ctx *ctx);
 int axs_ctx_trim(axs_ctx *ctx);
 /* The HIP stream the context launches on (a hipStream_t). */
 void *axs_ctx_stream(axs_ctx *ctx);
+/* Device memory for callers built without HIP (the C++ driver's tag and
+ * coverage buffers, FFI bindings); no counterpart in the reference (AMReX's
+ * arenas play this role there).  alloc: uninitialised bytes on the
+ * context's device; free waits for the context's stream first; memset is
+ * ordered on the stream; download copies after the stream's earlier work
+ * and returns when the bytes are on the host. */
+int axs_device_alloc(axs_ctx *ctx, size_t bytes, void **out);
+int axs_device_free(axs_ctx *ctx, void *ptr);
+int axs_device_memset(axs_ctx *ctx, void *ptr, int value, size_t bytes);
+int axs_device_download(axs_ctx *ctx, void *host, const void *dev,
+                        size_t bytes);
 const char *axs_last_error(void);
 /* Number of compiled stage-kernel tile variants (for the autotuner). */
 int axs_num_tile_variants(void);
@@ -438,6 +449,7 @@ int axs_boxlevel_rk4_step_graph(axs_boxlevel *bl,
  * the same pass.  Ghosts must be filled.  cover: optional dense device
  * uint8 coverage mask (0 = covered by a finer level, excluded from the
  * counts and the tags). */
+/* dense_tags may be NULL: the plaquette counts only. */
 int axs_boxlevel_tag(axs_boxlevel *bl, const axs_tagger_params *p,
                      uint8_t *dense_tags, const uint8_t *cover,
                      int64_t counts[2]);
@@ -466,6 +478,114 @@ int axs_boxlevel_copy_to_level(const axs_boxlevel *bl, axs_level *dst);
  * *copied (optional) receives the number of cells copied. */
 int axs_boxlevel_copy_valid(axs_boxlevel *dst, const axs_boxlevel *src,
                             int64_t *copied);
+
+/* -------------------------- AMR hierarchy --------------------------- */
+/* The reference's AMR driver around the hot path (AMReX's Amr with
+ * AxionStringsLevel as its AmrLevel): a base level and up to max_level
+ * ratio-2 levels of boxes, built by tagging (tag_cells,
+ * AxionStringsLevel.cpp:1916-2005), buffering by amr.n_error_buf, cutting
+ * into amr.blocking_factor blocks and merging them into boxes of at most
+ * amr.max_grid_size cells; advanced by subcycling (level l+1 takes two
+ * half steps per step of level l, its coarse-fine ghosts interpolated in
+ * time, then averaged down onto level l); rebuilt from fresh tags every
+ * amr.regrid_int coarse steps.  Box generation is block merging (runs of
+ * blocks along x, merged along y, then z), not AMReX's Berger-Rigoutsos
+ * clustering: the boxes differ from the reference's, the nesting,
+ * buffering and coverage rules are the same. */
+#define AXS_MAX_LEVELS 8
+typedef struct axs_hierarchy axs_hierarchy;
+typedef struct axs_amr_params
+{
+    int max_level;       /* amr.max_level, 1..AXS_MAX_LEVELS */
+    int n_error_buf;     /* amr.n_error_buf: tags grown by this many cells */
+    int blocking_factor; /* amr.blocking_factor: 8, 16, 32 or 64 fine cells */
+    int max_grid_size;   /* amr.max_grid_size: box edges <= this (<= 64) */
+    int regrid_int;      /* amr.regrid_int: rebuild every N coarse steps
+                            (<= 0: never) */
+    int force_full_refinement; /* axion_strings.tagging.force_full_refinement */
+    double gradient_threshold;         /* axion_strings.tagging.* (DBL_MAX =  */
+    double radial_gradient_threshold;  /* off), amplitude_threshold 0 = off  */
+    double amplitude_threshold;
+    int n_schedule; /* entries of level_add_log_mr_over_h; 0 = no gating */
+    double level_add_log_mr_over_h[AXS_MAX_LEVELS]; /* entry l-1: level l may
+                       exist from this log(m_r/H) on (the box plan's
+                       axion_strings.derived_level_add_log_mr_over_h) */
+    int use_graph; /* replay each level's step from a captured graph */
+} axs_amr_params;
+/* The stage scalars' blend after the pre-evolution handoff
+ * (AxionStringsLevel.cpp:889-907): lambda and the curvature coefficient go
+ * from the *_start values to the background's over [tau_start, tau_start +
+ * dtau] by smoothstep.  dtau = 0: no blend. */
+typedef struct axs_coeff_blend
+{
+    double tau_start, dtau, lambda_start, curvature_start;
+} axs_coeff_blend;
+/* Builds the hierarchy at conformal time tau: level 0 takes the valid
+ * cells of `base` (a uniform periodic level, cube cells) in boxes of
+ * min(max_grid_size, n) cells per edge; each finer level is created from
+ * the tags of the level below while the schedule allows it.  dt0 is the
+ * base level's step (evolution.dt_multiplier * dx).  `base` is only read;
+ * the caller keeps it. */
+int axs_hierarchy_create(axs_ctx *ctx, const axs_level *base,
+                         const axs_amr_params *p, const axs_background *bg,
+                         const axs_coeff_blend *blend, double tau, double dt0,
+                         axs_hierarchy **out);
+int axs_hierarchy_destroy(axs_hierarchy *h);
+/* One coarse step of the whole hierarchy, then a regrid when the step
+ * count reaches a multiple of regrid_int. */
+int axs_hierarchy_step(axs_hierarchy *h);
+/* Rebuilds levels 1.. from fresh tags: a level whose boxes do not change
+ * is kept as it is; a new level takes quartic interpolation from the level
+ * below, then the old level's values where both cover a cell. */
+int axs_hierarchy_regrid(axs_hierarchy *h);
+/* nlevels (levels in use), tau, coarse steps taken, regrids done, and the
+ * regrid wall seconds (device-synchronised): [0] total, [1] tagging and
+ * boxes, [2] level creation, [3] interpolation, [4] copies from the old
+ * level, [5] freeing it.  Any pointer may be NULL. */
+int axs_hierarchy_info(const axs_hierarchy *h, int *nlevels, double *tau,
+                       int64_t *nsteps, int64_t *nregrids,
+                       double regrid_seconds[6]);
+/* Level l (borrowed: valid until the next step, regrid or destroy), its
+ * domain, cell size and box count.  Any output but lvl may be NULL. */
+int axs_hierarchy_level(const axs_hierarchy *h, int l, axs_boxlevel **lvl,
+                        int domain[3], double *dx, int *nboxes);
+/* Box corners of level l (inclusive), 3 ints per box, in the level's order
+ * (sorted by z, y, x of the low corner; level 0 tiles in that order). */
+int axs_hierarchy_boxes(const axs_hierarchy *h, int l, int *lo, int *hi);
+/* compute_composite_total_energy's raw sums (EnergyKernel.hpp:287-339):
+ * every level's 14 energy sums over the cells no finer level covers,
+ * weighted by dx_l^3 and added; out[13] is the comoving volume counted,
+ * out[12] its unmasked part.  Fills every level's ghosts first. */
+int axs_hierarchy_energy_sums(axs_hierarchy *h, double R, double lambda,
+                              double R_prime_over_R, const axs_masking *m,
+                              double out[14]);
+/* Pierced-plaquette counts (plain, winding-weighted) of every level over
+ * the cells no finer level covers (the composite xi of
+ * AxionStringsLevel.cpp:1299-1337 weights level l's by dx_l):
+ * counts[2 l], counts[2 l + 1].  Fills every level's ghosts first. */
+int axs_hierarchy_plaquette_counts(axs_hierarchy *h, int64_t *counts);
+/* collapse_to_level0 (AxionStringsLevel.cpp:110-134) into the uniform
+ * level dst (level 0's domain and precision): level 0's values where
+ * nothing refines it, the recursive 2x2x2 average of the finer levels
+ * where something does.  The hierarchy is not changed. */
+int axs_hierarchy_collapse(axs_hierarchy *h, axs_level *dst);
+/* The host steps of box generation, exposed for testing:
+ * tagged_blocks: blocks of `block` cells of a dense [z][y][x] tag array of
+ * size dom that hold a tag after growing the tags by r cells (periodic,
+ * r <= block) -> out [dom/block] bytes;
+ * erode: a [z][y][x] block mask kept where its whole periodic
+ * (2r+1)^3 neighbourhood is set;
+ * boxes_from_blocks: boxes of the next finer level (ratio 2, fine_block
+ * fine cells per block) over the set blocks: runs of up to max_blocks along
+ * x, merged along y, then z (each up to max_blocks); *nboxes boxes written
+ * to lo / hi (3 ints each) if at most max_boxes, else AXS_ERR_ARG. */
+int axs_amr_tagged_blocks(const uint8_t *tags, const int dom[3], int block,
+                          int r, uint8_t *out);
+int axs_amr_erode(const uint8_t *mask, const int bdim[3], int r,
+                  uint8_t *out);
+int axs_amr_boxes_from_blocks(const uint8_t *blk, const int bdim[3],
+                              int fine_block, int max_blocks, int max_boxes,
+                              int *nboxes, int *lo, int *hi);
 
 /* ------------------------------ adjoint ----------------------------- */
 /* Gradient of J = sum_cells (|grad psi1|^2 + |grad psi2|^2) dx^3 / R_f^4

@@ -68,3 +68,77 @@ def test_amr_box_plan_equals_oracle(oracle, args):
     L, dxf, dxb, tau_f, nb, la = oracle.amr_box_plan(*args)
     assert (p["L_tilde"], p["dx_finest"], p["dx_base"], p["tau_f"],
             p["N_base"], p["log_add"]) == (L, dxf, dxb, tau_f, nb, la)
+
+
+# ---- the native host steps (axs_amr_*) against the numpy restatements ----
+def _native():
+    import ctypes as C
+    from axion_strings_adaptive_amd import capi
+    lib = capi.load()
+    u8 = C.POINTER(C.c_uint8)
+
+    def tagged_blocks(tags, block, r):
+        t = np.ascontiguousarray(tags, dtype=np.uint8)
+        nz, ny, nx = t.shape
+        out = np.zeros((nz // block, ny // block, nx // block), np.uint8)
+        capi.check(lib.axs_amr_tagged_blocks(
+            t.ctypes.data_as(u8), (C.c_int * 3)(nx, ny, nz), block, r,
+            out.ctypes.data_as(u8)))
+        return out.astype(bool)
+
+    def erode(mask, r):
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        bz, by, bx = m.shape
+        out = np.zeros_like(m)
+        capi.check(lib.axs_amr_erode(m.ctypes.data_as(u8),
+                                     (C.c_int * 3)(bx, by, bz), r,
+                                     out.ctypes.data_as(u8)))
+        return out.astype(bool)
+
+    def boxes(blk, fb, max_blocks):
+        b = np.ascontiguousarray(blk, dtype=np.uint8)
+        bz, by, bx = b.shape
+        cap = b.size
+        lo, hi = (C.c_int * (3 * cap + 3))(), (C.c_int * (3 * cap + 3))()
+        n = C.c_int()
+        capi.check(lib.axs_amr_boxes_from_blocks(
+            b.ctypes.data_as(u8), (C.c_int * 3)(bx, by, bz), fb, max_blocks,
+            cap, C.byref(n), lo, hi))
+        return [(tuple(lo[3 * i:3 * i + 3]), tuple(hi[3 * i:3 * i + 3]))
+                for i in range(n.value)]
+
+    return tagged_blocks, erode, boxes
+
+
+def test_native_tagged_blocks_equal_numpy():
+    tagged_blocks, _, _ = _native()
+    rng = np.random.default_rng(11)
+    for _ in range(60):
+        block = int(rng.choice([2, 4, 8, 16]))
+        r = int(rng.integers(0, block + 1))
+        shape = tuple(block * int(v) for v in rng.integers(1, 5, 3))
+        tags = rng.random(shape) < rng.choice([0.001, 0.01, 0.1])
+        assert np.array_equal(tagged_blocks(tags, block, r),
+                              amr.tagged_blocks(tags, block, r))
+
+
+def test_native_erode_equals_numpy():
+    _, erode, _ = _native()
+    rng = np.random.default_rng(12)
+    for _ in range(40):
+        shape = tuple(int(v) for v in rng.integers(1, 9, 3))
+        m = rng.random(shape) < rng.random()
+        for r in (0, 1, 2):
+            assert np.array_equal(erode(m, r), amr.erode_periodic(m, r))
+
+
+def test_native_boxes_equal_numpy():
+    _, _, boxes = _native()
+    rng = np.random.default_rng(13)
+    for _ in range(200):
+        shape = tuple(int(v) for v in rng.integers(1, 9, 3))
+        blk = rng.random(shape) < rng.random()
+        fb, mb = int(rng.choice([8, 16, 32])), int(rng.integers(1, 5))
+        got = boxes(blk, fb, mb)
+        assert got == amr.boxes_from_blocks(blk, fb, mb)
+        _check_partition(blk, got, fb, mb)
