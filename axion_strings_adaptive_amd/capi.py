@@ -122,6 +122,9 @@ _SIGS = {
     "axs_rk4_step": (_I, [_P, C.POINTER(RhsCoeffs), _D]),
     "axs_rk4_advance": (_I, [_P, C.POINTER(Background), _D, _D, _I, _I]),
     "axs_rk4_advance_coeffs": (_I, [_P, C.POINTER(RhsCoeffs), _D, _I, _I]),
+    "axs_rk4_advance_windings": (_I, [_P, C.POINTER(Background), _D, _D, _I,
+                                      _I, C.POINTER(TaggerParams), _P, _PI64,
+                                      _I]),
     "axs_level_place": (_I, [_P, _I, C.POINTER(_D), C.POINTER(_I)]),
     "axs_level_enable_timing": (_I, [_P, _I]),
     "axs_level_stage_times": (_I, [_P, _PD, _PI64]),
@@ -431,6 +434,22 @@ class Level:
                 use_graph: bool = False):
         check(self.lib.axs_rk4_advance(self.h, C.byref(bg), tau, dt, nsteps,
                                        int(use_graph)))
+
+    def advance_windings(self, bg: Background, tau: float, dt: float,
+                         nsteps: int, every: int, tags=None, grad_thr=DBL_MAX,
+                         rgrad_thr=DBL_MAX, amp_thr=0.0,
+                         use_graph: bool = False) -> np.ndarray:
+        """nsteps steps with the windings (and, given a tags DeviceBuffer,
+        the tagger) after every `every` steps; returns the [segment,
+        (plain, weighted)] counts."""
+        nseg = nsteps // every
+        out = np.zeros((max(nseg, 1), 2), np.int64)
+        p = TaggerParams(grad_thr, rgrad_thr, amp_thr, 1.0)
+        check(self.lib.axs_rk4_advance_windings(
+            self.h, C.byref(bg), tau, dt, nsteps, every, C.byref(p),
+            _P(tags.ptr) if tags is not None else None,
+            out.ctypes.data_as(C.POINTER(C.c_int64)), int(use_graph)))
+        return out[:nseg]
 
     def advance_coeffs(self, coeffs, dt: float, use_graph: bool = False):
         """len(coeffs) // 4 RK4 steps; coeffs[4 i + s] = stage s of step i."""

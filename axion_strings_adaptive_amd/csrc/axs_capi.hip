@@ -5,6 +5,7 @@
 
 #include <hipfft/hipfft.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -617,6 +618,13 @@ static void destroy_graph(axs_level *L)
             L->graph_valid[i] = 0;
             L->graph_y[i] = nullptr;
         }
+    if (L->dgraph_valid)
+    {
+        (void)hipGraphExecDestroy(L->dgraph_exec);
+        (void)hipGraphDestroy(L->dgraph);
+        L->dgraph_valid = 0;
+        L->dgraph_y = L->dgraph_tags = nullptr;
+    }
 }
 
 int axs_level_destroy(axs_level *L)
@@ -636,6 +644,9 @@ int axs_level_destroy(axs_level *L)
     free_arrays(L);
     (void)hipFree(L->d_coeff);
     (void)hipFree(L->d_step);
+    (void)hipFree(L->d_seg);
+    (void)hipFree(L->d_dcounts);
+    (void)hipFree(L->d_dR);
     delete L;
     return AXS_OK;
 }
@@ -1200,6 +1211,212 @@ int axs_rk4_advance_coeffs(axs_level *L, const axs_rhs_coeffs *coeffs,
                          [&](int i, int s, axs_rhs_coeffs *c) {
                              *c = coeffs[(size_t)i * 4 + s];
                          });
+}
+
+} // extern "C"
+
+// Room for nseg cadence segments (counts and R per segment); a new
+// allocation drops the cadence graph, which holds the old addresses.
+static int diag_buffers(axs_level *L, int nseg)
+{
+    if (!L->d_seg)
+        AXS_HIP_TRY(hipMalloc(&L->d_seg, sizeof(int)));
+    if (L->diag_cap >= nseg)
+        return AXS_OK;
+    destroy_graph(L);
+    (void)hipFree(L->d_dcounts);
+    (void)hipFree(L->d_dR);
+    L->d_dcounts = nullptr;
+    L->d_dR = nullptr;
+    L->diag_cap = 0;
+    const int cap = std::max(nseg, 64);
+    if (hipMalloc(&L->d_dcounts, (size_t)3 * cap * sizeof(unsigned long long)) !=
+            hipSuccess ||
+        hipMalloc(&L->d_dR, (size_t)cap * sizeof(double)) != hipSuccess)
+        return set_error(AXS_ERR_NOMEM,
+                         "axs_rk4_advance_windings: device allocation failed");
+    L->diag_cap = cap;
+    return AXS_OK;
+}
+
+// `every` steps of dt from the device coefficient table, then the cadence
+// pass (the direct form of one segment, and the body of its graph)
+static int run_segment(axs_level *L, double dt, int every,
+                       const axs_tagger_params *tp, uint8_t *tags)
+{
+    int rc = run_steps(L, L->d_step, dt, every, true);
+    if (rc)
+        return rc;
+    return launch_windings_pass(L, tp, tags, L->d_dcounts, L->d_seg, L->d_dR);
+}
+
+// The cadence graph for `units` segments from the current state buffer,
+// captured on first use; a replay must end in the buffer it starts from.
+static int dgraph_for(axs_level *L, double dt, int every, int units,
+                      const axs_tagger_params *tp, uint8_t *tags)
+{
+    const double key[6] = {dt, (double)every, (double)units,
+                           tags ? tp->gradient_threshold : 0.0,
+                           tags ? tp->radial_gradient_threshold : 0.0,
+                           tags ? tp->amplitude_threshold : 0.0};
+    if (L->dgraph_valid && L->dgraph_y == L->y && L->dgraph_tags == tags &&
+        std::equal(key, key + 6, L->dgraph_key))
+        return AXS_OK;
+    if (L->dgraph_valid)
+    {
+        (void)hipGraphExecDestroy(L->dgraph_exec);
+        (void)hipGraphDestroy(L->dgraph);
+        L->dgraph_valid = 0;
+    }
+    hipStream_t s = L->ctx->stream;
+    const void *y0 = L->y;
+    AXS_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int rc = AXS_OK;
+    for (int u = 0; u < units && rc == AXS_OK; ++u)
+        rc = run_segment(L, dt, every, tp, tags);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc != AXS_OK || e != hipSuccess || L->y != y0)
+    {
+        if (g)
+            (void)hipGraphDestroy(g);
+        if (rc != AXS_OK)
+            return rc;
+        return e != hipSuccess
+                   ? set_error(AXS_ERR_HIP, std::string("hipStreamEndCapture: ") +
+                                                hipGetErrorString(e))
+                   : set_error(AXS_ERR_ARG, "axs_rk4_advance_windings: a "
+                                            "replay must end in the buffer it "
+                                            "starts from");
+    }
+    hipGraphExec_t ge = nullptr;
+    if (hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess)
+    {
+        (void)hipGraphDestroy(g);
+        return set_error(AXS_ERR_HIP, "hipGraphInstantiate failed");
+    }
+    L->dgraph = g;
+    L->dgraph_exec = ge;
+    L->dgraph_valid = 1;
+    std::copy(key, key + 6, L->dgraph_key);
+    L->dgraph_y = y0;
+    L->dgraph_tags = tags;
+    return AXS_OK;
+}
+
+extern "C" {
+
+int axs_rk4_advance_windings(axs_level *L, const axs_background *bg,
+                             double tau, double dt, int nsteps, int every,
+                             const axs_tagger_params *tp, uint8_t *tags,
+                             int64_t *counts, int use_graph)
+{
+    AXS_CHECK_ARG(L && bg, "axs_rk4_advance_windings: NULL argument");
+    AXS_CHECK_ARG(every >= 1 && every <= kCoeffTableSteps,
+                  "axs_rk4_advance_windings: every must be 1..4096");
+    AXS_CHECK_ARG(nsteps >= 0 && nsteps % every == 0,
+                  "axs_rk4_advance_windings: nsteps must be a multiple of "
+                  "every");
+    AXS_CHECK_ARG(counts || nsteps == 0,
+                  "axs_rk4_advance_windings: NULL counts");
+    AXS_CHECK_ARG(dt > 0.0, "axs_rk4_advance_windings: dt must be > 0");
+    AXS_CHECK_ARG(bg->flat_space || bg->a_inv > 1.0,
+                  "axs_rk4_advance_windings: a_inv must be > 1");
+    const axs_tagger_params dflt{DBL_MAX, DBL_MAX, 0.0, 1.0};
+    if (tags && !tp)
+        tp = &dflt;
+    if (tags)
+        AXS_CHECK_ARG(tp->gradient_threshold > 0.0 &&
+                          tp->radial_gradient_threshold > 0.0,
+                      "axs_rk4_advance_windings: thresholds must be > 0");
+    const int nseg = nsteps / every;
+    if (nseg == 0)
+        return AXS_OK;
+    int rc = diag_buffers(L, nseg);
+    if (rc)
+        return rc;
+    hipStream_t s = L->ctx->stream;
+    // R at the end of every segment (the amplitude criterion), on the same
+    // tau sequence as the stage scalars: tau += dt per step
+    std::vector<double> Rh(nseg);
+    {
+        double t = tau;
+        for (int j = 0; j < nseg; ++j)
+        {
+            for (int k = 0; k < every; ++k)
+                t += dt;
+            Rh[j] = 1.0;
+            if ((rc = axs_background_eval(bg, 0, t, &Rh[j])))
+                return rc;
+        }
+    }
+    AXS_HIP_TRY(hipMemcpyAsync(L->d_dR, Rh.data(), nseg * sizeof(double),
+                               hipMemcpyHostToDevice, s));
+    AXS_HIP_TRY(hipMemsetAsync(L->d_dcounts, 0,
+                               (size_t)3 * nseg * sizeof(unsigned long long),
+                               s));
+    if ((rc = launch_seg_reset(L, L->d_seg)))
+        return rc;
+    // coefficient table chunks of whole segments
+    const int seg_per_chunk = kCoeffTableSteps / every;
+    std::vector<double> table;
+    double ts[4] = {0, 0, 0, 0};
+    int done = 0; // segments
+    double t = tau;
+    while (done < nseg)
+    {
+        const int cseg = std::min(nseg - done, seg_per_chunk);
+        const int csteps = cseg * every;
+        table.assign((size_t)csteps * 12, 0.0);
+        for (int i = 0; i < csteps; ++i)
+        {
+            ts[0] = t;
+            ts[1] = ts[2] = t + 0.5 * dt;
+            ts[3] = t + dt;
+            for (int st = 0; st < 4; ++st)
+            {
+                axs_rhs_coeffs c;
+                rhs_coeffs_at(bg, ts[st], &c);
+                table[(size_t)i * 12 + 3 * st] = c.curvature_coeff;
+                table[(size_t)i * 12 + 3 * st + 1] = c.lambda;
+                table[(size_t)i * 12 + 3 * st + 2] = c.R_squared;
+            }
+            t += dt;
+        }
+        if ((rc = upload_coeffs(L, table.data(), table.size(), L->d_coeff)))
+            return rc;
+        if ((rc = launch_step_advance(L, 1)))
+            return rc;
+        int j = 0;
+        if (use_graph && !L->timing)
+        {
+            // fused steps swap the state buffers: an odd segment length
+            // needs two segments per replay to end where it started
+            const int units = (L->fused && every % 2) ? 2 : 1;
+            if (cseg >= units)
+            {
+                if ((rc = dgraph_for(L, dt, every, units, tp, tags)))
+                    return rc;
+                for (; j + units <= cseg; j += units)
+                    AXS_HIP_TRY(hipGraphLaunch(L->dgraph_exec, s));
+            }
+        }
+        for (; j < cseg; ++j)
+            if ((rc = run_segment(L, dt, every, tp, tags)))
+                return rc;
+        done += cseg;
+    }
+    std::vector<unsigned long long> c((size_t)3 * nseg);
+    AXS_HIP_TRY(hipMemcpyAsync(c.data(), L->d_dcounts,
+                               c.size() * sizeof(unsigned long long),
+                               hipMemcpyDeviceToHost, s));
+    AXS_HIP_TRY(hipStreamSynchronize(s));
+    for (int j = 0; j < nseg; ++j)
+    {
+        counts[2 * j] = (int64_t)c[(size_t)3 * j];
+        counts[2 * j + 1] = (int64_t)c[(size_t)3 * j + 1];
+    }
+    return AXS_OK;
 }
 
 } // extern "C"

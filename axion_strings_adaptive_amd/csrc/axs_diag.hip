@@ -108,6 +108,10 @@ struct WindParams
     int8_t *wfield;         // optional: per-cell windings
     uint8_t *tags;          // optional: tagger output (only sets)
     unsigned long long *counts; // [0] plain, [1] weighted
+    // cadence passes (axs_rk4_advance_windings): counts go to
+    // counts + 3 * (*seg), and the amplitude criterion takes R_tab[*seg]
+    const int *seg;
+    const double *R_tab;
     // tagger extras
     double dx, grad_thr, rgrad_thr, amp_thr, R;
     int use_lap, use_rgrad;
@@ -442,8 +446,9 @@ __global__ void __launch_bounds__(64 * WTY)
                 {
                 bool tag = (wxy != 0 || wyz != 0 || wzx != 0);
                 const double psi1 = ld(p1, l1, id), psi2 = ld(p2, l2, id);
+                const double Ramp = p.R_tab ? p.R_tab[*p.seg] : p.R;
                 if (!tag && p.amp_thr > 0.0 &&
-                    sqrt(psi1 * psi1 + psi2 * psi2) / p.R < p.amp_thr)
+                    sqrt(psi1 * psi1 + psi2 * psi2) / Ramp < p.amp_thr)
                     tag = true;
                 if constexpr ((WM & WM_TAGD) != 0)
                 {
@@ -565,8 +570,9 @@ __global__ void __launch_bounds__(64 * WTY)
         unsigned long long v = 0;
         for (int w = 0; w < WTY; ++w)
             v += wcnt[w][t];
+        unsigned long long *cnt = p.seg ? p.counts + 3 * *p.seg : p.counts;
         if (t < 2 || (WM & WM_VEL) != 0)
-            atomicAdd(&p.counts[t], v);
+            atomicAdd(&cnt[t], v);
     }
     if constexpr ((WM & WM_VEL) != 0)
     {
@@ -893,8 +899,9 @@ __global__ void __launch_bounds__(64 * RWY)
         unsigned long long v = 0;
         for (int w = 0; w < RWY; ++w)
             v += wcnt[w][t];
+        unsigned long long *cnt = p.seg ? p.counts + 3 * *p.seg : p.counts;
         if (t < 2 || VEL)
-            atomicAdd(&p.counts[t], v);
+            atomicAdd(&cnt[t], v);
     }
     if constexpr (VEL)
     {
@@ -1308,9 +1315,18 @@ struct VelArgs
     int64_t *count;
 };
 
+// enq: a cadence pass -- enqueue only, counts into enq->counts + 3 * seg
+struct CadenceArgs
+{
+    unsigned long long *counts;
+    const int *seg;
+    const double *R_tab;
+};
+
 static int run_windings(axs_level *L, const uint8_t *mask, int8_t *wfield,
                         uint8_t *tags, const axs_tagger_params *tp,
-                        int64_t out[2], const VelArgs *vel = nullptr)
+                        int64_t out[2], const VelArgs *vel = nullptr,
+                        const CadenceArgs *enq = nullptr)
 {
     hipStream_t s = L->ctx->stream;
     WindParams p{};
@@ -1362,7 +1378,14 @@ static int run_windings(axs_level *L, const uint8_t *mask, int8_t *wfield,
         p.vel_k3 = e1 / (4.0 * m_r_sq * c1_sq * c1_sq * c1);
         p.vel_partials = L->ctx->d_partials;
     }
-    hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, s, p.counts);
+    if (enq)
+    {
+        p.counts = enq->counts;
+        p.seg = enq->seg;
+        p.R_tab = enq->R_tab;
+    }
+    else
+        hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, s, p.counts);
     const float *lo = L->precision == AXS_FP32C ? (const float *)L->comp
                                                 : (const float *)nullptr;
     if (L->precision == AXS_FP64)
@@ -1379,6 +1402,8 @@ static int run_windings(axs_level *L, const uint8_t *mask, int8_t *wfield,
     else
         launch_windings<float>(wm, ntiles, s, (const float *)L->y, lo, p);
     AXS_HIP_TRY(hipGetLastError());
+    if (enq)
+        return AXS_OK;
     if (vel)
     {
         hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s,
@@ -1457,6 +1482,60 @@ int axs_tag_cells(axs_level *L, const axs_tagger_params *tp, uint8_t *tags,
 }
 
 } // extern "C"
+
+namespace axs
+{
+
+__global__ void k_fill_u8(uint8_t *p, size_t n, uint8_t v)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_seg_advance(int *seg, int reset)
+{
+    if (threadIdx.x == 0)
+        *seg = reset ? 0 : *seg + 1;
+}
+
+// One cadence pass on the level's current state: clear the tags (a fill
+// kernel, so that the pass can be captured into a graph), the windings /
+// tagger pass adding its counts at counts + 3 * (*seg), then *seg += 1.
+int launch_windings_pass(axs_level *L, const axs_tagger_params *tp,
+                         uint8_t *tags, unsigned long long *counts,
+                         int *seg, const double *R_tab)
+{
+    hipStream_t s = L->ctx->stream;
+    if (tags)
+    {
+        const size_t n = L->g.ncell;
+        const unsigned blocks =
+            (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+        hipLaunchKernelGGL(k_fill_u8, dim3(blocks), dim3(256), 0, s, tags, n,
+                           (uint8_t)0);
+        AXS_HIP_TRY(hipGetLastError());
+    }
+    const CadenceArgs enq{counts, seg, R_tab};
+    int rc = run_windings(L, nullptr, nullptr, tags, tags ? tp : nullptr,
+                          nullptr, nullptr, &enq);
+    if (rc)
+        return rc;
+    hipLaunchKernelGGL(k_seg_advance, dim3(1), dim3(64), 0, s, seg, 0);
+    AXS_HIP_TRY(hipGetLastError());
+    return AXS_OK;
+}
+
+int launch_seg_reset(axs_level *L, int *seg)
+{
+    hipLaunchKernelGGL(k_seg_advance, dim3(1), dim3(64), 0, L->ctx->stream,
+                       seg, 1);
+    AXS_HIP_TRY(hipGetLastError());
+    return AXS_OK;
+}
+
+} // namespace axs
+
 
 // The energy pass's launch parameters for an nx x ny x nz periodic grid;
 // *nblocks receives the grid size (one partial row of 14 per block).

@@ -116,6 +116,20 @@ struct axs_level
     double graph_key[2][2]; // dt and steps per replay at capture
     const void *graph_y[2]; // the state buffer at capture
     int graph_next;         // slot the next capture replaces
+    // windings / tagger cadence of axs_rk4_advance_windings: the device
+    // segment counter, counts (3 per segment) and R per segment, with room
+    // for diag_cap segments; its graph of one or two segments
+    int *d_seg;
+    unsigned long long *d_dcounts;
+    double *d_dR;
+    int diag_cap;
+    hipGraph_t dgraph;
+    hipGraphExec_t dgraph_exec;
+    int dgraph_valid;
+    double dgraph_key[6]; // dt, every, segments per replay, the tagger's
+                          // three thresholds
+    const void *dgraph_y;    // the state buffer at capture
+    const void *dgraph_tags; // the tags buffer at capture
     // FFT plans (lazily created)
     void *fft_r2c;  // hipfftHandle stored as void*
     void *fft_c2r;
@@ -142,4 +156,11 @@ int launch_rk_step(axs_level *L, const void *yin, void *yout,
 int launch_rhs_eval(axs_level *L, const double *coeff3_dev, void *rhs);
 int launch_step_advance(axs_level *L, int reset);
 int num_tile_variants();
+// cadence passes (axs_diag.hip): tags cleared, windings / tagger counts
+// added at counts + 3 * (*seg) with R_tab[*seg] for the amplitude
+// criterion, then *seg += 1; launch_seg_reset sets *seg = 0
+int launch_windings_pass(axs_level *L, const axs_tagger_params *tp,
+                         uint8_t *tags, unsigned long long *counts, int *seg,
+                         const double *R_tab);
+int launch_seg_reset(axs_level *L, int *seg);
 } // namespace axs

@@ -257,6 +257,24 @@ int axs_rk4_advance(axs_level *lvl, const axs_background *bg, double tau,
  * coefficient (:889-907).  Same launches as axs_rk4_advance. */
 int axs_rk4_advance_coeffs(axs_level *lvl, const axs_rhs_coeffs *coeffs,
                            double dt, int nsteps, int use_graph);
+/* nsteps RK4 steps as axs_rk4_advance (nsteps a multiple of `every`, at
+ * most 4096), each run of `every` steps followed by the plaquette windings
+ * pass of axs_count_plaquettes -- and, when tags != NULL, by the
+ * StringTagger pass of axs_tag_cells into tags (cleared first), with R(tau)
+ * of that point for the amplitude criterion (tp->R is not used; tp NULL =
+ * the reference tagger) -- the post-step sequence of the reference
+ * (AxionStringsLevel.cpp:1316 counts, :1999-2003 tags) at a step cadence.
+ * counts[2 j], counts[2 j + 1]: plain and weighted pierced plaquettes after
+ * segment j (nsteps / every segments); tags hold the last segment's tags.
+ * With use_graph, one graph replays a segment -- its steps, the tag clear
+ * and the windings / tagger pass (two segments when `every` is odd and the
+ * steps are fused, so that a replay ends in the state buffer it starts
+ * from) -- captured on first use; the counts are read back once, at the
+ * end.  The same bits as the direct launches. */
+int axs_rk4_advance_windings(axs_level *lvl, const axs_background *bg,
+                             double tau, double dt, int nsteps, int every,
+                             const axs_tagger_params *tp, uint8_t *tags,
+                             int64_t *counts, int use_graph);
 
 /* Placement trials; no counterpart in the reference (an MI355X-specific
  * set-up step).  The physical pages a level's arrays land on decide between
