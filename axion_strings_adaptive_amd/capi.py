@@ -54,6 +54,13 @@ class TaggerParams(C.Structure):
                 ("amplitude_threshold", C.c_double), ("R", C.c_double)]
 
 
+class Array4(C.Structure):
+    _fields_ = [("p", C.c_void_p), ("jstride", C.c_int64),
+                ("kstride", C.c_int64), ("nstride", C.c_int64),
+                ("begin", C.c_int * 3), ("end", C.c_int * 3),
+                ("ncomp", C.c_int)]
+
+
 AXS_MAX_LEVELS = 8
 
 
@@ -162,6 +169,13 @@ _SIGS = {
     "axs_boxlevel_tag": (_I, [_P, C.POINTER(TaggerParams), _P, _P, _PI64]),
     "axs_boxlevel_energy_sums": (_I, [_P, _D, _D, _D, C.POINTER(Masking), _P,
                                       _PD]),
+    "axs_rhs_array4": (_I, [C.POINTER(Array4), C.POINTER(Array4),
+                            C.POINTER(_I), C.POINTER(_I),
+                            C.POINTER(RhsCoeffs), _D, _I, _P]),
+    "axs_tag_array4": (_I, [C.POINTER(Array4), C.POINTER(Array4),
+                            C.POINTER(_I), C.POINTER(_I),
+                            C.POINTER(TaggerParams), _D, _I, C.POINTER(Array4),
+                            _P, _P]),
     "axs_hierarchy_create": (_I, [_P, _P, C.POINTER(AmrParams),
                                   C.POINTER(Background), C.POINTER(CoeffBlend),
                                   _D, _D, C.POINTER(_P)]),

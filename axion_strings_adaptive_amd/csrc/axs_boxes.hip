@@ -21,6 +21,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 namespace axs
@@ -685,12 +686,96 @@ struct BoxTagParams
     double grad_thr, rgrad_thr, dx;
 };
 
+// The windings of the three plaquettes at a cell's low corner
+// (StringFinder.hpp:76-122) and, when `criteria`, the tagger's criteria at
+// the cell (StringTagger.hpp:83-274, plus the amplitude extension).
+// at(c, a, b, d): component c at offset (a, b, d) from the cell, as hi + lo
+// in double.
+template <typename At>
+__device__ __forceinline__ bool tag_cell(const At &at, const BoxTagParams &tp,
+                                         bool criteria, int &wxy, int &wyz,
+                                         int &wzx)
+{
+    auto ph = [&](int a, int b, int c) -> double {
+        return atan2(at(1, a, b, c), at(0, a, b, c));
+    };
+    const double t000 = ph(0, 0, 0), t100 = ph(1, 0, 0), t110 = ph(1, 1, 0),
+                 t010 = ph(0, 1, 0), t001 = ph(0, 0, 1), t101 = ph(1, 0, 1),
+                 t011 = ph(0, 1, 1);
+    wxy = pwind(t000, t100, t110, t010);
+    wyz = pwind(t000, t010, t011, t001);
+    wzx = pwind(t000, t001, t101, t100);
+    if (!criteria)
+        return false;
+    bool tag = wxy != 0 || wyz != 0 || wzx != 0;
+    const double psi1 = at(0, 0, 0, 0), psi2 = at(1, 0, 0, 0);
+    if (!tag && tp.amp_thr > 0.0)
+        tag = sqrt(psi1 * psi1 + psi2 * psi2) / tp.R < tp.amp_thr;
+    const double i1 = 1.0 / tp.dx, i2 = 1.0 / (tp.dx * tp.dx);
+    if (!tag && tp.grad_thr < DBL_MAX)
+    {
+        // dx^2 |laplacian psi_c| > threshold
+        const double dx2 = tp.dx * tp.dx;
+        double lap[2];
+        for (int c = 0; c < 2; ++c)
+        {
+            const double f0 = at(c, 0, 0, 0);
+            double rr = 0.0;
+            rr += tag_d2(at(c, -2, 0, 0), at(c, -1, 0, 0), f0, at(c, 1, 0, 0),
+                         at(c, 2, 0, 0), i2);
+            rr += tag_d2(at(c, 0, -2, 0), at(c, 0, -1, 0), f0, at(c, 0, 1, 0),
+                         at(c, 0, 2, 0), i2);
+            rr += tag_d2(at(c, 0, 0, -2), at(c, 0, 0, -1), f0, at(c, 0, 0, 1),
+                         at(c, 0, 0, 2), i2);
+            lap[c] = rr;
+        }
+        if (dx2 * fabs(lap[0]) > tp.grad_thr || dx2 * fabs(lap[1]) > tp.grad_thr)
+            tag = true;
+    }
+    if (!tag && tp.rgrad_thr < DBL_MAX)
+    {
+        // dx sqrt(|grad psi|^2 - |psi|^2 |grad theta|^2)
+        const double psq = psi1 * psi1 + psi2 * psi2;
+        double g1sq = 0.0, g2sq = 0.0, gtsq = 0.0;
+        for (int d = 0; d < 3; ++d)
+        {
+            const int ex = d == 0, ey = d == 1, ez = d == 2;
+            const double a =
+                tag_d1(at(0, -2 * ex, -2 * ey, -2 * ez), at(0, -ex, -ey, -ez),
+                       at(0, ex, ey, ez), at(0, 2 * ex, 2 * ey, 2 * ez), i1);
+            const double b =
+                tag_d1(at(1, -2 * ex, -2 * ey, -2 * ez), at(1, -ex, -ey, -ez),
+                       at(1, ex, ey, ez), at(1, 2 * ex, 2 * ey, 2 * ez), i1);
+            g1sq += a * a;
+            g2sq += b * b;
+            const double dth = psq > 0.0 ? (psi1 * b - psi2 * a) / psq : 0.0;
+            gtsq += dth * dth;
+        }
+        if (tp.dx * sqrt(g1sq + g2sq - psq * gtsq) > tp.rgrad_thr)
+            tag = true;
+    }
+    return tag;
+}
+
+// per-wave plaquette counts -> one atomic pair per wave
+__device__ __forceinline__ void add_counts(unsigned long long plain,
+                                           unsigned long long weighted,
+                                           unsigned long long *counts)
+{
+    for (int o = 32; o > 0; o >>= 1)
+        weighted += __shfl_xor(weighted, o, 64);
+    if ((threadIdx.x & 63) == 0)
+    {
+        atomicAdd(&counts[0], plain);
+        atomicAdd(&counts[1], weighted);
+    }
+}
+
 template <typename T>
 __global__ void k_box_tag(LevelView v, const T *y, const float *lo,
                           uint8_t *tags, const uint8_t *cover, BoxTagParams tp,
                           unsigned long long *counts)
 {
-    const double amp_thr = tp.amp_thr, R = tp.R;
     const BoxD B = v.box[blockIdx.y];
     const uint32_t cells = (uint32_t)B.n[0] * B.n[1] * B.n[2];
     const size_t tot = v.total;
@@ -709,97 +794,134 @@ __global__ void k_box_tag(LevelView v, const T *y, const float *lo,
                 ((uint32_t)Z * v.dom[1] + Y) * v.dom[0] + X;
             if (!cover || cover[dense])
             {
-                auto ph = [&](int a, int b, int c) -> double {
-                    const uint32_t i = sidx(B, v.ng, lx + a, ly + b, lz + c);
-                    const double p1 = (double)y[i] + (lo ? (double)lo[i] : 0.0);
-                    const double p2 = (double)y[tot + i] +
-                                      (lo ? (double)lo[tot + i] : 0.0);
-                    return atan2(p2, p1);
+                // component c at offset (a, b, d) from the cell
+                auto at = [&](int c, int a, int b, int d) -> double {
+                    const uint32_t i = sidx(B, v.ng, lx + a, ly + b, lz + d);
+                    const size_t o = (size_t)c * tot + i;
+                    return (double)y[o] + (lo ? (double)lo[o] : 0.0);
                 };
-                const double t000 = ph(0, 0, 0), t100 = ph(1, 0, 0),
-                             t110 = ph(1, 1, 0), t010 = ph(0, 1, 0),
-                             t001 = ph(0, 0, 1), t101 = ph(1, 0, 1),
-                             t011 = ph(0, 1, 1);
-                wxy = pwind(t000, t100, t110, t010);
-                wyz = pwind(t000, t010, t011, t001);
-                wzx = pwind(t000, t001, t101, t100);
+                const bool tag = tag_cell(at, tp, tags != nullptr, wxy, wyz, wzx);
                 weighted += abs(wxy) + abs(wyz) + abs(wzx);
-                if (tags)
-                {
-                    bool tag = wxy != 0 || wyz != 0 || wzx != 0;
-                    // component c at offset (a, b, d) from the cell
-                    auto at = [&](int c, int a, int b, int d) -> double {
-                        const uint32_t i =
-                            sidx(B, v.ng, lx + a, ly + b, lz + d);
-                        const size_t o = (size_t)c * tot + i;
-                        return (double)y[o] + (lo ? (double)lo[o] : 0.0);
-                    };
-                    const double psi1 = at(0, 0, 0, 0), psi2 = at(1, 0, 0, 0);
-                    if (!tag && amp_thr > 0.0)
-                        tag = sqrt(psi1 * psi1 + psi2 * psi2) / R < amp_thr;
-                    const double i1 = 1.0 / tp.dx, i2 = 1.0 / (tp.dx * tp.dx);
-                    if (!tag && tp.grad_thr < DBL_MAX)
-                    {
-                        // dx^2 |laplacian psi_c| > threshold
-                        const double dx2 = tp.dx * tp.dx;
-                        double lap[2];
-                        for (int c = 0; c < 2; ++c)
-                        {
-                            const double f0 = at(c, 0, 0, 0);
-                            double rr = 0.0;
-                            rr += tag_d2(at(c, -2, 0, 0), at(c, -1, 0, 0), f0,
-                                         at(c, 1, 0, 0), at(c, 2, 0, 0), i2);
-                            rr += tag_d2(at(c, 0, -2, 0), at(c, 0, -1, 0), f0,
-                                         at(c, 0, 1, 0), at(c, 0, 2, 0), i2);
-                            rr += tag_d2(at(c, 0, 0, -2), at(c, 0, 0, -1), f0,
-                                         at(c, 0, 0, 1), at(c, 0, 0, 2), i2);
-                            lap[c] = rr;
-                        }
-                        if (dx2 * fabs(lap[0]) > tp.grad_thr ||
-                            dx2 * fabs(lap[1]) > tp.grad_thr)
-                            tag = true;
-                    }
-                    if (!tag && tp.rgrad_thr < DBL_MAX)
-                    {
-                        // dx sqrt(|grad psi|^2 - |psi|^2 |grad theta|^2)
-                        const double psq = psi1 * psi1 + psi2 * psi2;
-                        double g1sq = 0.0, g2sq = 0.0, gtsq = 0.0;
-                        for (int d = 0; d < 3; ++d)
-                        {
-                            const int ex = d == 0, ey = d == 1, ez = d == 2;
-                            const double a = tag_d1(
-                                at(0, -2 * ex, -2 * ey, -2 * ez),
-                                at(0, -ex, -ey, -ez), at(0, ex, ey, ez),
-                                at(0, 2 * ex, 2 * ey, 2 * ez), i1);
-                            const double b = tag_d1(
-                                at(1, -2 * ex, -2 * ey, -2 * ez),
-                                at(1, -ex, -ey, -ez), at(1, ex, ey, ez),
-                                at(1, 2 * ex, 2 * ey, 2 * ez), i1);
-                            g1sq += a * a;
-                            g2sq += b * b;
-                            const double dth =
-                                psq > 0.0 ? (psi1 * b - psi2 * a) / psq : 0.0;
-                            gtsq += dth * dth;
-                        }
-                        if (tp.dx * sqrt(g1sq + g2sq - psq * gtsq) >
-                            tp.rgrad_thr)
-                            tag = true;
-                    }
-                    if (tag)
-                        tags[dense] = 1;
-                }
+                if (tag)
+                    tags[dense] = 1;
             }
         }
         plain += __popcll(__ballot(wxy != 0)) + __popcll(__ballot(wyz != 0)) +
                  __popcll(__ballot(wzx != 0));
     }
-    for (int o = 32; o > 0; o >>= 1)
-        weighted += __shfl_xor(weighted, o, 64);
-    if ((threadIdx.x & 63) == 0)
+    add_counts(plain, weighted, counts);
+}
+
+// ------------------------------------------------ borrowed box views ----
+// An axs_array4 as the kernels see it: element (i, j, k, n) of a view at
+// p[(i - b0) + (j - b1) js + (k - b2) ks + n ns].
+template <typename T> struct View4
+{
+    T *p;
+    int64_t js, ks, ns;
+    int b0, b1, b2;
+    __device__ __forceinline__ T &operator()(int i, int j, int k, int n) const
     {
-        atomicAdd(&counts[0], plain);
-        atomicAdd(&counts[1], weighted);
+        return p[(int64_t)(i - b0) + (int64_t)(j - b1) * js +
+                 (int64_t)(k - b2) * ks + (int64_t)n * ns];
     }
+};
+
+template <typename T> static View4<T> view4(const axs_array4 &a)
+{
+    return View4<T>{(T *)a.p, a.jstride, a.kstride, a.nstride,
+                    a.begin[0],  a.begin[1], a.begin[2]};
+}
+
+// AxionStringsRHS over the cells of one box: one thread per (i, j) column
+// of a 64 x 4 tile, streaming z with psi's z neighbours in a register queue
+// and the x / y neighbours read from the view (the L1 / L2 serve them).
+// The per-cell arithmetic is that of the level kernels, so the bits are.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_rhs_array4(View4<const T> in, View4<T> out, int lo0, int lo1, int lo2,
+                 int nx, int ny, int nz, int zchunk, double cc, double lam,
+                 double R2, double inv_dx2)
+{
+    const int x = blockIdx.x * 64 + (int)threadIdx.x;
+    const int y = blockIdx.y * 4 + (int)threadIdx.y;
+    if (x >= nx || y >= ny)
+        return;
+    const int i = lo0 + x, j = lo1 + y;
+    const int k0 = lo2 + (int)blockIdx.z * zchunk;
+    const int k1 = min(k0 + zchunk, lo2 + nz);
+    const T i2 = (T)inv_dx2;
+    using S = typename std::conditional<std::is_same<T, double>::value, Star,
+                                        StarF>::type;
+    T q1[5], q2[5];
+    for (int d = 0; d < 4; ++d)
+    {
+        q1[d] = in(i, j, k0 - 2 + d, 0);
+        q2[d] = in(i, j, k0 - 2 + d, 1);
+    }
+    for (int k = k0; k < k1; ++k)
+    {
+        q1[4] = in(i, j, k + 2, 0);
+        q2[4] = in(i, j, k + 2, 1);
+        const S a1{in(i - 2, j, k, 0), in(i - 1, j, k, 0), in(i + 1, j, k, 0),
+                   in(i + 2, j, k, 0), in(i, j - 2, k, 0), in(i, j - 1, k, 0),
+                   in(i, j + 1, k, 0), in(i, j + 2, k, 0), q1[0],
+                   q1[1],              q1[3],              q1[4],
+                   q1[2]};
+        const S a2{in(i - 2, j, k, 1), in(i - 1, j, k, 1), in(i + 1, j, k, 1),
+                   in(i + 2, j, k, 1), in(i, j - 2, k, 1), in(i, j - 1, k, 1),
+                   in(i, j + 1, k, 1), in(i, j + 2, k, 1), q2[0],
+                   q2[1],              q2[3],              q2[4],
+                   q2[2]};
+        T F1, F2;
+        force(star_lap(a1, i2), star_lap(a2, i2), q1[2], q2[2], cc, lam, R2, F1,
+              F2);
+        out(i, j, k, 0) = in(i, j, k, 2);
+        out(i, j, k, 1) = in(i, j, k, 3);
+        out(i, j, k, 2) = F1;
+        out(i, j, k, 3) = F2;
+        for (int d = 0; d < 4; ++d)
+        {
+            q1[d] = q1[d + 1];
+            q2[d] = q2[d + 1];
+        }
+    }
+}
+
+// The tagger / windings pass over the cells of one box of a borrowed view
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_tag_array4(View4<const T> in, View4<const float> lo, int has_lo,
+                 View4<uint8_t> tags, int has_tags, int lo0, int lo1, int lo2,
+                 int nx, int ny, int nz, BoxTagParams tp,
+                 unsigned long long *counts)
+{
+    const uint32_t cells = (uint32_t)nx * ny * nz;
+    unsigned long long plain = 0, weighted = 0;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const int iters = (int)((cells + stride - 1) / stride);
+    for (int r = 0; r < iters; ++r)
+    {
+        const uint32_t t = (uint32_t)r * stride + blockIdx.x * blockDim.x +
+                           threadIdx.x;
+        int wxy = 0, wyz = 0, wzx = 0;
+        if (t < cells)
+        {
+            const int i = lo0 + (int)(t % nx), j = lo1 + (int)((t / nx) % ny),
+                      k = lo2 + (int)(t / ((uint32_t)nx * ny));
+            auto at = [&](int c, int a, int b, int d) -> double {
+                return (double)in(i + a, j + b, k + d, c) +
+                       (has_lo ? (double)lo(i + a, j + b, k + d, c) : 0.0);
+            };
+            const bool tag = tag_cell(at, tp, has_tags != 0, wxy, wyz, wzx);
+            weighted += abs(wxy) + abs(wyz) + abs(wzx);
+            if (tag)
+                tags(i, j, k, 0) = 1;
+        }
+        plain += __popcll(__ballot(wxy != 0)) + __popcll(__ballot(wyz != 0)) +
+                 __popcll(__ballot(wzx != 0));
+    }
+    add_counts(plain, weighted, counts);
 }
 
 // ------------------------------------------------------------- energy ----
@@ -1377,6 +1499,142 @@ int axs_boxlevel_create(axs_ctx *ctx, int nboxes, const int *lo, const int *hi,
         AXS_HIP_TRY(hipMemsetAsync(b->comp, 0, bytes4, s));
     AXS_HIP_TRY(hipStreamSynchronize(s));
     *out = b;
+    return AXS_OK;
+}
+
+// does view a hold cells lo - glo .. hi + ghi with at least nc components?
+static bool view_holds(const axs_array4 *a, const int lo[3], const int hi[3],
+                       int glo, int ghi, int nc)
+{
+    if (!a || !a->p || a->ncomp < nc)
+        return false;
+    for (int d = 0; d < 3; ++d)
+        if (lo[d] > hi[d] || lo[d] - glo < a->begin[d] ||
+            hi[d] + ghi >= a->end[d])
+            return false;
+    return true;
+}
+
+int axs_rhs_array4(const axs_array4 *state, axs_array4 *rhs, const int lo[3],
+                   const int hi[3], const axs_rhs_coeffs *c, double dx,
+                   int precision, void *stream)
+{
+    AXS_CHECK_ARG(state && rhs && lo && hi && c,
+                  "axs_rhs_array4: NULL argument");
+    AXS_CHECK_ARG(precision == AXS_FP64 || precision == AXS_FP32,
+                  "axs_rhs_array4: precision must be AXS_FP64 (double views) "
+                  "or AXS_FP32 (float views)");
+    AXS_CHECK_ARG(dx > 0.0, "axs_rhs_array4: dx must be > 0");
+    AXS_CHECK_ARG(view_holds(state, lo, hi, 2, 2, 4),
+                  "axs_rhs_array4: the state view must hold lo-2 .. hi+2 with "
+                  "4 components");
+    AXS_CHECK_ARG(view_holds(rhs, lo, hi, 0, 0, 4),
+                  "axs_rhs_array4: the rhs view must hold lo .. hi with 4 "
+                  "components");
+    const int nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1,
+              nz = hi[2] - lo[2] + 1;
+    const long cols = (long)((nx + 63) / 64) * ((ny + 3) / 4);
+    int zc = 64;
+    while (zc > 4 && cols * ((nz + zc - 1) / zc) < 4096)
+        zc >>= 1;
+    const dim3 grid((nx + 63) / 64, (ny + 3) / 4, (nz + zc - 1) / zc);
+    hipStream_t s = (hipStream_t)stream;
+    const double i2 = 1.0 / (dx * dx);
+    if (precision == AXS_FP64)
+    {
+        const View4<double> in = view4<double>(*state);
+        hipLaunchKernelGGL(k_rhs_array4<double>, grid, dim3(64, 4), 0, s,
+                           View4<const double>{in.p, in.js, in.ks, in.ns, in.b0,
+                                               in.b1, in.b2},
+                           view4<double>(*rhs), lo[0], lo[1], lo[2], nx, ny, nz,
+                           zc, c->curvature_coeff, c->lambda, c->R_squared, i2);
+    }
+    else
+    {
+        const View4<float> in = view4<float>(*state);
+        hipLaunchKernelGGL(k_rhs_array4<float>, grid, dim3(64, 4), 0, s,
+                           View4<const float>{in.p, in.js, in.ks, in.ns, in.b0,
+                                              in.b1, in.b2},
+                           view4<float>(*rhs), lo[0], lo[1], lo[2], nx, ny, nz,
+                           zc, c->curvature_coeff, c->lambda, c->R_squared, i2);
+    }
+    AXS_HIP_TRY(hipGetLastError());
+    return AXS_OK;
+}
+
+int axs_tag_array4(const axs_array4 *state, const axs_array4 *state_lo,
+                   const int lo[3], const int hi[3],
+                   const axs_tagger_params *p, double dx, int precision,
+                   axs_array4 *tags, unsigned long long *counts_dev,
+                   void *stream)
+{
+    AXS_CHECK_ARG(state && lo && hi && counts_dev,
+                  "axs_tag_array4: NULL argument");
+    AXS_CHECK_ARG(precision == AXS_FP64 || precision == AXS_FP32,
+                  "axs_tag_array4: precision must be AXS_FP64 (double views) "
+                  "or AXS_FP32 (float views, FP32C with state_lo)");
+    AXS_CHECK_ARG(dx > 0.0, "axs_tag_array4: dx must be > 0");
+    AXS_CHECK_ARG(!state_lo || precision == AXS_FP32,
+                  "axs_tag_array4: state_lo goes with float views");
+    const bool deriv = tags && p &&
+                       (p->gradient_threshold < DBL_MAX ||
+                        p->radial_gradient_threshold < DBL_MAX);
+    // windings read the cell and its +1 neighbours; the derivative criteria
+    // a radius-2 star
+    AXS_CHECK_ARG(view_holds(state, lo, hi, deriv ? 2 : 0, deriv ? 2 : 1, 2),
+                  "axs_tag_array4: the state view must hold lo .. hi+1 (lo-2 "
+                  ".. hi+2 with the derivative criteria) with 2 components");
+    AXS_CHECK_ARG(!state_lo || (state_lo->p && state_lo->jstride == state->jstride &&
+                                state_lo->kstride == state->kstride &&
+                                state_lo->nstride == state->nstride &&
+                                state_lo->begin[0] == state->begin[0] &&
+                                state_lo->begin[1] == state->begin[1] &&
+                                state_lo->begin[2] == state->begin[2]),
+                  "axs_tag_array4: state_lo must have the state view's shape");
+    AXS_CHECK_ARG(!tags || view_holds(tags, lo, hi, 0, 0, 1),
+                  "axs_tag_array4: the tags view must hold lo .. hi");
+    if (p && tags)
+    {
+        AXS_CHECK_ARG(p->gradient_threshold > 0.0 &&
+                          p->radial_gradient_threshold > 0.0,
+                      "axs_tag_array4: thresholds must be > 0");
+        AXS_CHECK_ARG(p->amplitude_threshold <= 0.0 || p->R > 0.0,
+                      "axs_tag_array4: amplitude criterion needs R > 0");
+    }
+    BoxTagParams tp{};
+    tp.amp_thr = p ? p->amplitude_threshold : 0.0;
+    tp.R = p ? p->R : 1.0;
+    tp.grad_thr = p ? p->gradient_threshold : DBL_MAX;
+    tp.rgrad_thr = p ? p->radial_gradient_threshold : DBL_MAX;
+    tp.dx = dx;
+    const int nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1,
+              nz = hi[2] - lo[2] + 1;
+    const size_t cells = (size_t)nx * ny * nz;
+    AXS_CHECK_ARG(cells < (size_t(1) << 31), "axs_tag_array4: box too large");
+    const unsigned blocks = (unsigned)std::min<size_t>((cells + 255) / 256, 4096);
+    hipStream_t s = (hipStream_t)stream;
+    const View4<uint8_t> tv = tags ? view4<uint8_t>(*tags) : View4<uint8_t>{};
+    const View4<float> lv = state_lo ? view4<float>(*state_lo) : View4<float>{};
+    const View4<const float> lc{lv.p, lv.js, lv.ks, lv.ns, lv.b0, lv.b1, lv.b2};
+    if (precision == AXS_FP64)
+    {
+        const View4<double> in = view4<double>(*state);
+        hipLaunchKernelGGL(k_tag_array4<double>, dim3(blocks), dim3(256), 0, s,
+                           View4<const double>{in.p, in.js, in.ks, in.ns, in.b0,
+                                               in.b1, in.b2},
+                           lc, 0, tv, tags ? 1 : 0, lo[0], lo[1], lo[2], nx, ny,
+                           nz, tp, counts_dev);
+    }
+    else
+    {
+        const View4<float> in = view4<float>(*state);
+        hipLaunchKernelGGL(k_tag_array4<float>, dim3(blocks), dim3(256), 0, s,
+                           View4<const float>{in.p, in.js, in.ks, in.ns, in.b0,
+                                              in.b1, in.b2},
+                           lc, state_lo ? 1 : 0, tv, tags ? 1 : 0, lo[0], lo[1],
+                           lo[2], nx, ny, nz, tp, counts_dev);
+    }
+    AXS_HIP_TRY(hipGetLastError());
     return AXS_OK;
 }
 

@@ -497,6 +497,50 @@ int axs_boxlevel_copy_to_level(const axs_boxlevel *bl, axs_level *dst);
 int axs_boxlevel_copy_valid(axs_boxlevel *dst, const axs_boxlevel *src,
                             int64_t *copied);
 
+/* ---------------------- borrowed box data (Array4) ------------------- */
+/* A view of one box's data that the caller owns in device memory, as
+ * AMReX's Array4 (AMReX_Array4.H): element (i, j, k, n) at
+ * p[(i - begin[0]) + (j - begin[1]) jstride + (k - begin[2]) kstride +
+ *   n nstride] for begin <= (i, j, k) < end, 0 <= n < ncomp; strides in
+ * elements.  With these entry points the kernels run inside the reference's
+ * per-box loops on its own MultiFab data, with no copy in or out. */
+typedef struct
+{
+    void *p;
+    int64_t jstride, kstride, nstride;
+    int begin[3];
+    int end[3];
+    int ncomp;
+} axs_array4;
+/* AxionStringsRHS::operator() over the cells lo..hi (inclusive) of one box,
+ * the ParallelFor of AxionStringsLevel.cpp:913-918
+ * (AxionStringsRHS.impl.hpp:25-60): rhs(.., 0..3) = (Pi1, Pi2,
+ * lap psi1 + c psi1 - (lambda/2)(|psi|^2 - R^2) psi1, ...).  The state view
+ * holds lo-2 .. hi+2 (ghosts filled by the caller, as FillPatch does) with 4
+ * components; the rhs view holds lo .. hi.  precision AXS_FP64 (double
+ * views) or AXS_FP32 (float views): the arithmetic and bits of
+ * axs_rhs_eval.  Asynchronous on `stream` (a hipStream_t, NULL = the null
+ * stream); allocates nothing. */
+int axs_rhs_array4(const axs_array4 *state, axs_array4 *rhs, const int lo[3],
+                   const int hi[3], const axs_rhs_coeffs *c, double dx,
+                   int precision, void *stream);
+/* StringTagger / count_plaquettes over the cells lo..hi of one box
+ * (StringTagger.hpp:83-274, StringFinder.hpp:76-122; the ParallelFor of
+ * AxionStringsLevel.cpp:1999-2003): the windings of the plaquettes at each
+ * cell's low corner, added to counts_dev[0] (plain) and counts_dev[1]
+ * (weighted), unsigned 64-bit device counters the caller clears; with a
+ * tags view (uint8, 1 component) tags are set to 1 where a criterion of p
+ * holds (never cleared; p NULL = the reference tagger).  The state view
+ * holds lo .. hi+1 (lo-2 .. hi+2 with the derivative criteria); FP32C
+ * levels pass their compensation words as state_lo (same shape, float),
+ * else NULL.  The bits of axs_tag_cells / axs_boxlevel_tag.  Asynchronous
+ * on `stream`; allocates nothing. */
+int axs_tag_array4(const axs_array4 *state, const axs_array4 *state_lo,
+                   const int lo[3], const int hi[3],
+                   const axs_tagger_params *p, double dx, int precision,
+                   axs_array4 *tags, unsigned long long *counts_dev,
+                   void *stream);
+
 /* -------------------------- AMR hierarchy --------------------------- */
 /* The reference's AMR driver around the hot path (AMReX's Amr with
  * AxionStringsLevel as its AmrLevel): a base level and up to max_level
