@@ -99,6 +99,18 @@ struct StarLoader
 };
 
 // --------------------------------------------------- windings / tagging --
+// Tile of a workgroup: with 8 | ntiles, the consecutive tiles k (n/8) ..
+// (k+1)(n/8) - 1 go to XCD k (workgroups are dealt to the 8 XCDs round
+// robin), so tiles that share halo rows and columns run on one XCD at about
+// the same time and meet in its L2; else the identity.
+__device__ __forceinline__ int xcd_tile(int ntiles)
+{
+    const int b = (int)blockIdx.x;
+    if (ntiles & 7)
+        return b;
+    return (b & 7) * (ntiles >> 3) + (b >> 3);
+}
+
 struct WindParams
 {
     int nx, ny, nz;
@@ -233,7 +245,7 @@ __global__ void __launch_bounds__(64 * WTY)
     __shared__ uint32_t pw[2][WTY + 1][TX + 1];
     __shared__ uint32_t zw[WTY + 1][TX + 1];
 
-    const int bid = blockIdx.x;
+    const int bid = xcd_tile(p.tiles_x * p.tiles_y * p.tiles_z);
     const int bx = bid % p.tiles_x;
     const int rem = bid / p.tiles_x;
     const int by = rem % p.tiles_y;
@@ -587,7 +599,7 @@ __global__ void __launch_bounds__(64 * WTY)
             __syncthreads();
         }
         if (t == 0)
-            p.vel_partials[blockIdx.x] = red[0];
+            p.vel_partials[bid] = red[0];
     }
 }
 
@@ -662,7 +674,7 @@ __global__ void __launch_bounds__(64 * RWY)
     constexpr bool VEL = (WM & WM_VEL) != 0;
     constexpr int NC = VEL ? 4 : 2; // components per point
     constexpr int RD = 2;           // planes of loads in flight
-    const int bid = blockIdx.x;
+    const int bid = xcd_tile(p.tiles_x * p.tiles_y * p.tiles_z);
     const int bx = bid % p.tiles_x;
     const int rem = bid / p.tiles_x;
     const int by = rem % p.tiles_y;
@@ -913,7 +925,7 @@ __global__ void __launch_bounds__(64 * RWY)
             __syncthreads();
         }
         if (t == 0)
-            p.vel_partials[blockIdx.x] = red[0];
+            p.vel_partials[bid] = red[0];
     }
 }
 
@@ -1020,7 +1032,7 @@ __global__ void __launch_bounds__(64 * ETY)
     __shared__ double red[64 * ETY];
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int t = ty * 64 + tx;
-    const int bid = blockIdx.x;
+    const int bid = xcd_tile(p.tiles_x * p.tiles_y * p.tiles_z);
     const int bx = bid % p.tiles_x;
     const int rem = bid / p.tiles_x;
     const int by = rem % p.tiles_y;
@@ -1246,7 +1258,7 @@ __global__ void __launch_bounds__(64 * ETY)
             __syncthreads();
         }
         if (t == 0)
-            p.partials[(size_t)blockIdx.x * 14 + q] = red[0];
+            p.partials[(size_t)bid * 14 + q] = red[0];
         __syncthreads();
     }
 }
