@@ -5,8 +5,9 @@ bits; only the time differs):
 
 * stage fusion 3 (FP32 / FP32C, the whole step in one launch): the step
   kernel's variant (-1 = the 64 x 11 ring kernel loading one plane ahead,
-  4 = the same loading two planes ahead, 3 = 64 x 11 with rows aligned to
-  cache lines, 2 = 60 x 12, 0 = 60 x 8, 1 = 60 x 4) and its z chunk;
+  4 = the same loading two planes ahead, 6 / 5 = those two with the tiles
+  walked in 4 x 8 patches, 3 = 64 x 11 with rows aligned to cache lines,
+  2 = 60 x 12, 0 = 60 x 8, 1 = 60 x 4) and its z chunk;
 * stage fusions 1 and 2 (the fused pairs, with and without a stored
   accumulator): the pair tile (TY x VX cells per lane, centre-load prefetch,
   non-temporal stores) and z chunk;
@@ -41,7 +42,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tune_table.txt")
 RESULTS = os.path.join(HERE, "tune_results.json")
 
-STEP_VARIANTS = (-1, 4, 3, 2, 0, 1)
+STEP_VARIANTS = (-1, 4, 6, 5, 3, 2, 0, 1)
 STEP_Z_CHUNKS = (32, 64, 128, 256)
 PAIR_Z_CHUNKS = (16, 32, 64)
 STAGE_Z_CHUNKS = (16, 32, 64)
@@ -174,33 +175,52 @@ def tune_boxes(ctx, precision: int, steps: int = 4):
     return t, r, g, res
 
 
-def write_table(level_best: dict, box_best: dict, path: str = TABLE,
-                results_path: str = RESULTS):
+def results_of(level_best: dict, box_best: dict) -> dict:
     """level_best: {(n, precision): {fusion: (ms, v, z, cands)}};
-    box_best: {precision: (ms, rows, grid, cands)}."""
+    box_best: {precision: (ms, rows, grid, cands)} -> the results record,
+    keyed "level <n> <precision> <fusion>" / "box <precision>"."""
+    results = {}
+    for (n, p), modes in level_best.items():
+        for fusion, (t, v, z, cands) in modes.items():
+            results[f"level {n} {p} {fusion}"] = {
+                "best": [v, z], "ms_per_step": t,
+                "candidates": [[c[1], c[2], round(c[0], 4)] for c in cands]}
+    for p, (t, r, g, cands) in box_best.items():
+        results[f"box {p}"] = {
+            "best": [r, g], "ms_sum": t,
+            "candidates": [[c[1], c[2], round(c[0], 4)] for c in cands]}
+    return results
+
+
+def write_table(results: dict, path: str = TABLE,
+                results_path: str = RESULTS):
+    """The tuning table (axs_tuning_load format) and the results record."""
     lines = ["# Launch shapes timed by axion_strings_adaptive_amd.autotune "
              "(axs_tuning_load format).",
              "# level <nx> <ny> <nz> <precision> <fusion> <tile_variant> "
              "<tile_z_chunk> <pair_variant> <pair_z_chunk>",
              "# box <precision> <item_rows> <grid_blocks>"]
-    results = {}
-    for (n, p), modes in sorted(level_best.items()):
-        for fusion, (t, v, z, cands) in sorted(modes.items()):
+
+    def order(k):
+        f = k.split()
+        return (0, int(f[1]), int(f[2]), int(f[3])) if f[0] == "level" \
+            else (1, int(f[1]), 0, 0)
+    for k in sorted(results, key=order):
+        f, e = k.split(), results[k]
+        if f[0] == "level":
+            n, p, fusion = int(f[1]), int(f[2]), int(f[3])
+            v, z = e["best"]
             tv, tz, pv, pz = (v, z, -1, 0) if fusion == 0 else (-1, 0, v, z)
             lines.append(f"level {n} {n} {n} {p} {fusion} {tv} {tz} {pv} {pz}"
-                         f"  # {t:.3f} ms/step")
-            results[f"level {n} {p} {fusion}"] = {
-                "best": [v, z], "ms_per_step": t,
-                "candidates": [[c[1], c[2], round(c[0], 4)] for c in cands]}
-    for p, (t, r, g, cands) in sorted(box_best.items()):
-        lines.append(f"box {p} {r} {g}  # {t:.3f} ms over 3 box sizes")
-        results[f"box {p}"] = {
-            "best": [r, g], "ms_sum": t,
-            "candidates": [[c[1], c[2], round(c[0], 4)] for c in cands]}
-    with open(path, "w") as f:
-        f.write("\n".join(lines) + "\n")
-    with open(results_path, "w") as f:
-        json.dump(results, f, indent=1, sort_keys=True)
+                         f"  # {e['ms_per_step']:.3f} ms/step")
+        else:
+            r, g = e["best"]
+            lines.append(f"box {f[1]} {r} {g}  # {e['ms_sum']:.3f} ms over 3 "
+                         "box sizes")
+    with open(path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    with open(results_path, "w") as fh:
+        json.dump(results, fh, indent=1, sort_keys=True)
 
 
 def main(argv=None):
@@ -214,6 +234,9 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--table", default=TABLE)
     ap.add_argument("--results", default=RESULTS)
+    ap.add_argument("--merge", default=None,
+                    help="a results record (tune_results.json) whose entries "
+                         "this run does not retune are kept")
     args = ap.parse_args(argv)
     from . import capi
     ctx = capi.Context(0)
@@ -232,7 +255,12 @@ def main(argv=None):
         box_best[p] = tune_boxes(ctx, p)
         _log(f"best boxes prec={p}: rows={box_best[p][1]} "
              f"grid={box_best[p][2]} {box_best[p][0]:.3f} ms")
-    write_table(level_best, box_best, args.table, args.results)
+    results = {}
+    if args.merge:
+        with open(args.merge) as fh:
+            results = json.load(fh)
+    results.update(results_of(level_best, box_best))
+    write_table(results, args.table, args.results)
     print(json.dumps({"table": args.table, "results": args.results}),
           flush=True)
     ctx.close()

@@ -41,7 +41,34 @@ struct StepParams
     int z_chunk;
     int tiles_x, tiles_y, tiles_z;
     int xcd_remap;
+    int patch; // tile order: 0 x-major; 1 patches of 4 x 8 tiles (x, y)
 };
+
+// Tile (bx, by, bz) of position s in the launch order.  x-major: x fastest,
+// then y, then z.  Patches: the tile rows are cut into bands of 8 (the last
+// band may be shorter), and a band is walked in groups of 4 columns, each
+// group row by row -- so 32 consecutive positions (one XCD's workgroups at
+// a time, with the XCD remap) cover a 4 x 8 patch of tiles whose halo
+// rows and columns are each other's interior.  Needs tiles_x % 4 == 0.
+__device__ __forceinline__ void step_tile(const StepParams &p, int s, int &bx,
+                                          int &by, int &bz)
+{
+    const int per_z = p.tiles_x * p.tiles_y;
+    bz = s / per_z;
+    int r = s - bz * per_z;
+    if (!p.patch)
+    {
+        bx = r % p.tiles_x;
+        by = r / p.tiles_x;
+        return;
+    }
+    const int band = r / (p.tiles_x * 8);
+    const int h = min(8, p.tiles_y - band * 8);
+    r -= band * p.tiles_x * 8;
+    const int g = r / (4 * h), w = r % (4 * h);
+    by = band * 8 + w / 4;
+    bx = g * 4 + (w & 3);
+}
 
 constexpr int kStepTX = 60; // interior columns per tile (64 lanes - 2 x 2)
 
@@ -426,10 +453,8 @@ __global__ void __launch_bounds__(68 * (TY + 4)) k_step_r(StepParams p)
         const int ntiles = p.tiles_x * p.tiles_y * p.tiles_z;
         bid = (bid & 7) * (ntiles >> 3) + (bid >> 3);
     }
-    const int bx = bid % p.tiles_x;
-    const int rem = bid / p.tiles_x;
-    const int by = rem % p.tiles_y;
-    const int bz = rem / p.tiles_y;
+    int bx, by, bz;
+    step_tile(p, bid, bx, by, bz);
 
     // own point (ec, er) of the tile grown by 2: x = X0 - 2 + ec,
     // y = Y0 - 2 + er
@@ -846,7 +871,10 @@ int launch_rk_step(axs_level *L, const void *yin, void *yout,
     if (L->precision == AXS_FP64)
         return set_error(AXS_ERR_UNSUPPORTED,
                          "one launch per RK4 step is FP32 / FP32C only");
-    const bool ring = L->pair_variant < 0 || L->pair_variant == 4;
+    // ring kernel: -1 (loads one plane ahead), 4 (two), 5 / 6 (4 / -1 with
+    // the tiles in 4 x 8 patches)
+    const int pv = L->pair_variant;
+    const bool ring = pv < 0 || pv == 4 || pv == 5 || pv == 6;
     const bool al = ring || L->pair_variant == 3;
     const int TY =
         al ? 11 : (L->pair_variant == 1 ? 4 : (L->pair_variant == 0 ? 8 : 12));
@@ -887,12 +915,13 @@ int launch_rk_step(axs_level *L, const void *yin, void *yout,
     p.tiles_z = (p.nz + zc - 1) / zc;
     const long ntiles = (long)p.tiles_x * p.tiles_y * p.tiles_z;
     p.xcd_remap = (ntiles % 8 == 0) ? 1 : 0;
+    p.patch = (ring && (pv == 5 || pv == 6) && p.tiles_x % 4 == 0) ? 1 : 0;
     hipStream_t s = L->ctx->stream;
     if (ring)
     {
         const dim3 block(68 * (11 + 4));
-        // variant 4: loads two planes ahead; default: one
-        const bool far = L->pair_variant == 4;
+        // variants 4, 5: loads two planes ahead; else one
+        const bool far = pv == 4 || pv == 5;
         if (L->precision == AXS_FP32C && far)
             hipLaunchKernelGGL((k_step_r<AXS_FP32C, 11, 2>), dim3(ntiles),
                                block, 0, s, p);

@@ -100,7 +100,8 @@ def test_fused_fp64_matches_oracle(ctx, oracle):
 
 @pytest.mark.parametrize("shape", [(64, 64, 64), (40, 24, 36), (4, 8, 12),
                                    (128, 16, 8), (130, 10, 7), (61, 9, 5),
-                                   (9, 4, 4), (67, 13, 5), (200, 5, 4)])
+                                   (9, 4, 4), (67, 13, 5), (200, 5, 4),
+                                   (256, 90, 9), (256, 30, 6)])
 @pytest.mark.parametrize("precision", [1, 2])
 def test_one_launch_step_equals_per_stage(ctx, shape, precision):
     """Fusion mode 3 (the whole RK4 step in one launch, FP32 / FP32C):
@@ -111,10 +112,10 @@ def test_one_launch_step_equals_per_stage(ctx, shape, precision):
         ref = _run(ctx, shape, precision, s0, nsteps, False, 0)
         for graph in (False, True):
             for zc in (0, 4, 16):
-                # -1, 4: ring kernel, loads one / two planes ahead; 3:
-                # register-queue kernel (all 64 x 11 aligned); 0, 1, 2:
-                # 60 x 8, 4, 12
-                for v in (-1, 0, 1, 2, 3, 4):
+                # -1, 4: ring kernel, loads one / two planes ahead (6, 5:
+                # the same with tiles in 4 x 8 patches); 3: register-queue
+                # kernel (all 64 x 11 aligned); 0, 1, 2: 60 x 8, 4, 12
+                for v in (-1, 0, 1, 2, 3, 4, 5, 6):
                     got = _run(ctx, shape, precision, s0, nsteps, graph, 3, v,
                                zc)
                     _same(got, ref)
@@ -129,7 +130,7 @@ def test_one_launch_step_long_z_chunks(ctx, precision):
     for nsteps in (1, 3):
         ref = _run(ctx, shape, precision, s0, nsteps, False, 0)
         for zc in (0, 64, 128, 96):
-            for v in (-1, 3, 4):
+            for v in (-1, 3, 4, 5, 6):
                 got = _run(ctx, shape, precision, s0, nsteps, True, 3, v, zc)
                 _same(got, ref)
 
