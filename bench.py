@@ -62,13 +62,16 @@ MEASURED_S23_BYTES = {(20, 32): 86.03}
 MEASURED_PAIR2_BYTES = {1: {(-1, 0): 117.40}, 2: {(-1, 0): 103.48}}
 # ... and of the one-launch step kernel (mode 3), keyed by (pair variant,
 # z_chunk) as the level reports them (profiles/k_step_1024cubed_*.md)
-MEASURED_STEP_BYTES = {(4, 64): 88.99, (3, 0): 74.58, (2, 0): 97.07}
+MEASURED_STEP_BYTES = {(6, 256): 97.74, (4, 64): 88.99, (3, 0): 74.58,
+                       (2, 0): 97.07}
 
 
 def step_kernel_name(pair_variant):
     """The one-launch step kernel a pair variant selects."""
-    if pair_variant in (-1, 4):
-        return f"k_step_r<FP32C, 11, {2 if pair_variant == 4 else 1}>"
+    if pair_variant in (-1, 4, 5, 6):
+        d = 2 if pair_variant in (4, 5) else 1
+        order = ", 4 x 8 tile patches" if pair_variant in (5, 6) else ""
+        return f"k_step_r<FP32C, 11, {d}>{order}"
     ty = {0: 8, 1: 4, 3: 11}.get(pair_variant, 12)
     return f"k_step<FP32C, {ty}, {'aligned' if ty == 11 else 'unaligned'}>"
 
