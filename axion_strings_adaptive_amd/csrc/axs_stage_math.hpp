@@ -61,16 +61,21 @@ __device__ __forceinline__ double star_lap(const Star &s, double i2)
     return l;
 }
 
+// The weights carry 1/dx^2: (4/3) i2 and (1/12) i2 are formed once per
+// kernel (i2 is invariant), so a Laplacian costs one multiply and one fma
+// after the sums.  For dx a power of two (every level here: dx = 0.5 and
+// its refinements) the products are exact and the result is bitwise the
+// unscaled form times i2.
 __device__ __forceinline__ float star_lap(const StarF &s, float i2)
 {
 #pragma clang fp contract(off)
-    const float w_near = 4.0f / 3.0f, w_far = 1.0f / 12.0f;
+    const float w_near = (4.0f / 3.0f) * i2, w_far = (1.0f / 12.0f) * i2;
     const float c = s.c;
     const float nsum = ((s.xm1 - c) + (s.xp1 - c)) + ((s.ym1 - c) + (s.yp1 - c)) +
                        ((s.zm1 - c) + (s.zp1 - c));
     const float fsum = ((s.xm2 - c) + (s.xp2 - c)) + ((s.ym2 - c) + (s.yp2 - c)) +
                        ((s.zm2 - c) + (s.zp2 - c));
-    return fmaf(w_near, nsum, -(w_far * fsum)) * i2;
+    return fmaf(w_near, nsum, -(w_far * fsum));
 }
 
 // Pi' = lap psi + c psi - (lambda/2)(|psi|^2 - R^2) psi
@@ -86,16 +91,20 @@ __device__ __forceinline__ void force(double lap1, double lap2, double psi1,
 }
 
 // FP32 force with the stage scalars already rounded: ccf = (float)c,
-// hlam = 0.5f * (float)lambda, r2f = (float)R^2.
+// hlam = 0.5f * (float)lambda, r2f = (float)R^2.  The two linear terms
+// share one coefficient, c - (lambda/2)(|psi|^2 - R^2), formed by one fma
+// (five operations for both components); where c = 0 (radiation era) it is
+// exactly -(lambda/2)(|psi|^2 - R^2) and the bits are those of the
+// term-by-term form.
 __device__ __forceinline__ void force_f(float lap1, float lap2, float psi1,
                                         float psi2, float ccf, float hlam,
                                         float r2f, float &F1, float &F2)
 {
 #pragma clang fp contract(off)
     const float d = fmaf(psi1, psi1, fmaf(psi2, psi2, -r2f));
-    const float pc = hlam * d;
-    F1 = fmaf(-pc, psi1, fmaf(ccf, psi1, lap1));
-    F2 = fmaf(-pc, psi2, fmaf(ccf, psi2, lap2));
+    const float cm = fmaf(-hlam, d, ccf);
+    F1 = fmaf(cm, psi1, lap1);
+    F2 = fmaf(cm, psi2, lap2);
 }
 
 __device__ __forceinline__ void force(float lap1, float lap2, float psi1,
