@@ -52,18 +52,48 @@ PAIR_BYTES_NOACC = {1: [32, 80], 0: [64, 96]}
 # ... of the one launch per step (fusion mode 3, FP32C): read y_n and lo,
 # write y_{n+1} and lo (4S)
 STEP_BYTES_ONE = 64
-# HBM bytes per cell of one FP32C stage-2/3 launch at 1024^3, from
+# HBM bytes per cell of the dominant FP32C launch at 1024^3, from
 # rocprofv3 --pmc FETCH_SIZE (x 2.0) and WRITE_SIZE (x 1.0), the scales
-# measured on device copies and reads of known size; keyed by the tile
-# (variant, z_chunk) the launch used.
-MEASURED_S23_BYTES = {(20, 32): 86.03}
-# the same for the fused FP32C stages 3+4 launch, keyed by (pair variant,
-# z_chunk): pairs with a stored accumulator (mode 1) and without (mode 2)
-MEASURED_PAIR2_BYTES = {1: {(-1, 0): 117.40}, 2: {(-1, 0): 103.48}}
-# ... and of the one-launch step kernel (mode 3), keyed by (pair variant,
-# z_chunk) as the level reports them (profiles/k_step_1024cubed_*.md)
-MEASURED_STEP_BYTES = {(6, 256): 97.74, (4, 64): 88.99, (3, 0): 74.58,
-                       (2, 0): 97.07}
+# measured on device copies and reads of known size; keyed by the launch
+# shape the level reports ((tile variant, z chunk) for one launch per
+# stage, (pair variant, z chunk) otherwise).  Each table holds the digest
+# of the kernel sources it was measured on: when the sources differ the
+# stored figure describes another kernel, and `traffic` is reported as null
+# (source_digest() of the files named).
+MEASURED_TRAFFIC = {
+    # one launch per step (stage fusion 3): k_step_r / k_step
+    "step": {"sources": ["axs_step.hip", "axs_stage_math.hpp"],
+             "digest": "eba392a364dd3360",
+             "profile": "profiles/k_step_1024cubed_trace_and_traffic.md",
+             "bytes": {(4, 128): 73.43}},
+    # fused pairs, stages 3+4 (modes 1 and 2)
+    "pair": {"sources": ["axs_pair.hip", "axs_stage_math.hpp"],
+             "digest": None, "profile": None, "bytes": {}},
+    # one launch per stage, stages 2/3
+    "stage": {"sources": ["axs_rk.hip", "axs_stage_math.hpp"],
+              "digest": None, "profile": None, "bytes": {}},
+}
+CSRC = os.path.join(ROOT, "axion_strings_adaptive_amd", "csrc")
+
+
+def source_digest(names):
+    """sha256 (first 16 hex digits) of the named kernel sources, in order."""
+    import hashlib
+    h = hashlib.sha256()
+    for name in names:
+        with open(os.path.join(CSRC, name), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def measured_traffic(family, key):
+    """Stored HBM bytes per cell for a launch shape, or (None, reason)."""
+    t = MEASURED_TRAFFIC[family]
+    if key not in t["bytes"]:
+        return None, "no stored measurement for this launch shape"
+    if t["digest"] != source_digest(t["sources"]):
+        return None, "stored measurement is of other kernel sources"
+    return t["bytes"][key], t["profile"]
 
 
 def step_kernel_name(pair_variant):
@@ -482,12 +512,13 @@ def main(argv=None):
     if fused == 3:
         # one launch per step: the step kernel is the only one
         achieved = STEP_BYTES_ONE * ncell / (st[0] * 1e-3) / 1e9
-        per_cell = MEASURED_STEP_BYTES.get(pkey)
+        per_cell, tsrc = measured_traffic("step", pkey)
         roofline = {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
             "traffic": per_cell * ncell if per_cell and n == 1024 else None,
             "traffic_unit": "bytes per launch",
+            "traffic_source": tsrc if n == 1024 else "1024^3 only",
             "kernel": step_kernel_name(pkey[0]) + " (the whole RK4 step)",
             "algorithmic_bytes_per_cell": STEP_BYTES_ONE,
             "step_launch_ms": st[0], "step_gbs": achieved}
@@ -497,13 +528,14 @@ def main(argv=None):
         k = 0 if st[0] > st[1] else 1
         pb = PAIR_BYTES if fused == 1 else PAIR_BYTES_NOACC
         achieved = pb[1][k] * ncell / (st[k] * 1e-3) / 1e9
-        per_cell = (MEASURED_PAIR2_BYTES[fused].get(pkey) if k == 1
-                    else None)
+        per_cell, tsrc = (measured_traffic("pair", (fused,) + pkey)
+                          if k == 1 else (None, "not measured"))
         roofline = {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
             "traffic": per_cell * ncell if per_cell and n == 1024 else None,
             "traffic_unit": "bytes per launch",
+            "traffic_source": tsrc if n == 1024 else "1024^3 only",
             "kernel": f"k_pair<float, FP32C, {k + 1 + 2 * (fused - 1)}> "
                       f"(RK stages {'1+2' if k == 0 else '3+4'})",
             "algorithmic_bytes_per_cell": pb[1][k],
@@ -515,13 +547,14 @@ def main(argv=None):
         # roofline of the dominant kernel: the stage-2/3 launch (two per step)
         bytes23 = STAGE_BYTES[1][1] * ncell
         achieved = bytes23 / (0.5 * (st[1] + st[2]) * 1e-3) / 1e9
-        per_cell = MEASURED_S23_BYTES.get(
-            (shape32["tile_variant"], shape32["tile_z_chunk"]))
+        per_cell, tsrc = measured_traffic(
+            "stage", (shape32["tile_variant"], shape32["tile_z_chunk"]))
         roofline = {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
             "traffic": per_cell * ncell if per_cell and n == 1024 else None,
             "traffic_unit": "bytes per launch",
+            "traffic_source": tsrc if n == 1024 else "1024^3 only",
             "kernel": "k_stage<float, FP32C, MODE_S23> (RK stages 2/3)",
             "algorithmic_bytes_per_cell": STAGE_BYTES[1][1],
             "stage_ms": st,
