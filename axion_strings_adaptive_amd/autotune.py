@@ -6,7 +6,8 @@ bits; only the time differs):
 * stage fusion 3 (FP32 / FP32C, the whole step in one launch): the step
   kernel's variant (-1 = the 64 x 11 ring kernel loading one plane ahead,
   4 = the same loading two planes ahead, 6 / 5 = those two with the tiles
-  walked in 4 x 8 patches, 3 = 64 x 11 with rows aligned to cache lines,
+  walked in 4 x 8 patches, 8 / 7 = those two with the 8 XCDs on adjacent
+  bands of 32 tiles, 3 = 64 x 11 with rows aligned to cache lines,
   2 = 60 x 12, 0 = 60 x 8, 1 = 60 x 4) and its z chunk;
 * stage fusions 1 and 2 (the fused pairs, with and without a stored
   accumulator): the pair tile (TY x VX cells per lane, centre-load prefetch,
@@ -42,7 +43,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tune_table.txt")
 RESULTS = os.path.join(HERE, "tune_results.json")
 
-STEP_VARIANTS = (-1, 4, 6, 5, 3, 2, 0, 1)
+STEP_VARIANTS = (-1, 4, 6, 5, 8, 7, 3, 2, 0, 1)
 STEP_Z_CHUNKS = (32, 64, 128, 256)
 PAIR_Z_CHUNKS = (16, 32, 64)
 STAGE_Z_CHUNKS = (16, 32, 64)

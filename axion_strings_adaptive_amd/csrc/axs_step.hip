@@ -40,7 +40,9 @@ struct StepParams
     double inv_dx2;
     int z_chunk;
     int tiles_x, tiles_y, tiles_z;
-    int xcd_remap;
+    int xcd_remap; // 0: none; 1: XCD k takes the k-th eighth of the tiles;
+                   // G >= 2: bands of G consecutive tiles per XCD, the 8
+                   // XCDs' bands adjacent
     int patch; // tile order: 0 x-major; 1 patches of 4 x 8 tiles (x, y)
 };
 
@@ -70,7 +72,35 @@ __device__ __forceinline__ void step_tile(const StepParams &p, int s, int &bx,
     bx = g * 4 + (w & 3);
 }
 
+// Launch position b -> tile position.  Workgroups are dealt to the 8 XCDs
+// round robin (b mod 8).  Mode 1 gives XCD k the k-th eighth of the tile
+// list, so the 32 workgroups one XCD runs at a time are consecutive tiles
+// (two tile rows of a 1024-wide level), whose x and inner y halos are each
+// other's interior and meet in that XCD's L2.  Mode G >= 2 deals the list in
+// super-bands of 8 G tiles: XCD k takes the k-th band of G consecutive tiles
+// of each, so the 8 XCDs work on 8 adjacent bands of one z chunk at the
+// same time and the rows between bands are read by two XCDs at about the
+// same time (the last partial super-band keeps launch order).  A bijection
+// on [0, ntiles) either way.
+__device__ __forceinline__ int xcd_tile(const StepParams &p, int b)
+{
+    const int ntiles = p.tiles_x * p.tiles_y * p.tiles_z;
+    if (p.xcd_remap == 1)
+        return (b & 7) * (ntiles >> 3) + (b >> 3);
+    if (p.xcd_remap >= 2)
+    {
+        const int G = p.xcd_remap;
+        if (b < ntiles / (8 * G) * (8 * G))
+        {
+            const int j = b >> 3;
+            return (j / G) * (8 * G) + (b & 7) * G + j % G;
+        }
+    }
+    return b;
+}
+
 constexpr int kStepTX = 60; // interior columns per tile (64 lanes - 2 x 2)
+constexpr int kStepBand = 32; // tiles per XCD band (variants 7, 8)
 
 // Block of 64 (TY + 4) threads (AL: 68 (TY + 4)): each owns one point of
 // the tile grown by 2.  Loop variable q: the plane where F(psi_n), F(psi_1)
@@ -89,12 +119,7 @@ __global__ void __launch_bounds__((AL ? 68 : 64) * (TY + 4)) k_step(StepParams p
     __shared__ float sa[2][4][HYA][EXA]; // [buffer][psn1 psn2 ps11 ps12]
     __shared__ float sb[3][4][HYB][EXB]; // [q % 3][ps21 ps22 ps31 ps32]
 
-    int bid = blockIdx.x;
-    if (p.xcd_remap)
-    {
-        const int ntiles = p.tiles_x * p.tiles_y * p.tiles_z;
-        bid = (bid & 7) * (ntiles >> 3) + (bid >> 3);
-    }
+    const int bid = xcd_tile(p, blockIdx.x);
     const int bx = bid % p.tiles_x;
     const int rem = bid / p.tiles_x;
     const int by = rem % p.tiles_y;
@@ -455,12 +480,7 @@ __global__ void __launch_bounds__(68 * (TY + 4)) k_step_r(StepParams p)
     __shared__ float2 sa[2 * 2 * HYA * EXA];
     __shared__ float2 sb[6 * 2 * HYB * EXB];
 
-    int bid = blockIdx.x;
-    if (p.xcd_remap)
-    {
-        const int ntiles = p.tiles_x * p.tiles_y * p.tiles_z;
-        bid = (bid & 7) * (ntiles >> 3) + (bid >> 3);
-    }
+    const int bid = xcd_tile(p, blockIdx.x);
     int bx, by, bz;
     step_tile(p, bid, bx, by, bz);
 
@@ -875,7 +895,9 @@ static void launch_step_ty(const StepParams &p, int prec, long ntiles,
 // One RK4 step in one launch: y_n = yin -> y_{n+1} = yout (yout != yin),
 // FP32C lo in place.  The level's pair variant selects the kernel and tile:
 // -1 (default): k_step_r over 64 x 11 tiles with rows aligned to cache
-// lines (1020 threads, z chunk 128); 3: k_step over the same tiles; 2:
+// lines (1020 threads, z chunk 128), loading one plane ahead; 4: two planes
+// ahead; 6 / 5: -1 / 4 with the tiles in 4 x 8 patches; 8 / 7: -1 / 4 with
+// the XCDs on adjacent bands of 32 tiles; 3: k_step over the same tiles; 2:
 // k_step over 60 x 12 tiles (1024 threads); 0: 60 x 8; 1: 60 x 4.  All give
 // the same bits.
 int launch_rk_step(axs_level *L, const void *yin, void *yout,
@@ -887,7 +909,7 @@ int launch_rk_step(axs_level *L, const void *yin, void *yout,
     // ring kernel: -1 (loads one plane ahead), 4 (two), 5 / 6 (4 / -1 with
     // the tiles in 4 x 8 patches)
     const int pv = L->pair_variant;
-    const bool ring = pv < 0 || pv == 4 || pv == 5 || pv == 6;
+    const bool ring = pv < 0 || (pv >= 4 && pv <= 8);
     const bool al = ring || L->pair_variant == 3;
     const int TY =
         al ? 11 : (L->pair_variant == 1 ? 4 : (L->pair_variant == 0 ? 8 : 12));
@@ -928,13 +950,15 @@ int launch_rk_step(axs_level *L, const void *yin, void *yout,
     p.tiles_z = (p.nz + zc - 1) / zc;
     const long ntiles = (long)p.tiles_x * p.tiles_y * p.tiles_z;
     p.xcd_remap = (ntiles % 8 == 0) ? 1 : 0;
+    if (pv == 7 || pv == 8)
+        p.xcd_remap = kStepBand;
     p.patch = (ring && (pv == 5 || pv == 6) && p.tiles_x % 4 == 0) ? 1 : 0;
     hipStream_t s = L->ctx->stream;
     if (ring)
     {
         const dim3 block(68 * (11 + 4));
-        // variants 4, 5: loads two planes ahead; else one
-        const bool far = pv == 4 || pv == 5;
+        // variants 4, 5, 7: loads two planes ahead; else one
+        const bool far = pv == 4 || pv == 5 || pv == 7;
         if (L->precision == AXS_FP32C && far)
             hipLaunchKernelGGL((k_step_r<AXS_FP32C, 11, 2>), dim3(ntiles),
                                block, 0, s, p);

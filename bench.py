@@ -98,9 +98,11 @@ def measured_traffic(family, key):
 
 def step_kernel_name(pair_variant):
     """The one-launch step kernel a pair variant selects."""
-    if pair_variant in (-1, 4, 5, 6):
-        d = 2 if pair_variant in (4, 5) else 1
-        order = ", 4 x 8 tile patches" if pair_variant in (5, 6) else ""
+    if pair_variant in (-1, 4, 5, 6, 7, 8):
+        d = 2 if pair_variant in (4, 5, 7) else 1
+        order = {5: ", 4 x 8 tile patches", 6: ", 4 x 8 tile patches",
+                 7: ", XCDs on adjacent tile bands",
+                 8: ", XCDs on adjacent tile bands"}.get(pair_variant, "")
         return f"k_step_r<FP32C, 11, {d}>{order}"
     ty = {0: 8, 1: 4, 3: 11}.get(pair_variant, 12)
     return f"k_step<FP32C, {ty}, {'aligned' if ty == 11 else 'unaligned'}>"

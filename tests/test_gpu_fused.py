@@ -113,9 +113,10 @@ def test_one_launch_step_equals_per_stage(ctx, shape, precision):
         for graph in (False, True):
             for zc in (0, 4, 16):
                 # -1, 4: ring kernel, loads one / two planes ahead (6, 5:
-                # the same with tiles in 4 x 8 patches); 3: register-queue
-                # kernel (all 64 x 11 aligned); 0, 1, 2: 60 x 8, 4, 12
-                for v in (-1, 0, 1, 2, 3, 4, 5, 6):
+                # the same with tiles in 4 x 8 patches; 8, 7: XCDs on
+                # adjacent bands of tiles); 3: register-queue kernel (all
+                # 64 x 11 aligned); 0, 1, 2: 60 x 8, 4, 12
+                for v in (-1, 0, 1, 2, 3, 4, 5, 6, 7, 8):
                     got = _run(ctx, shape, precision, s0, nsteps, graph, 3, v,
                                zc)
                     _same(got, ref)
@@ -130,7 +131,7 @@ def test_one_launch_step_long_z_chunks(ctx, precision):
     for nsteps in (1, 3):
         ref = _run(ctx, shape, precision, s0, nsteps, False, 0)
         for zc in (0, 64, 128, 96):
-            for v in (-1, 3, 4, 5, 6):
+            for v in (-1, 3, 4, 5, 6, 7, 8):
                 got = _run(ctx, shape, precision, s0, nsteps, True, 3, v, zc)
                 _same(got, ref)
 
