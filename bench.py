@@ -63,9 +63,9 @@ STEP_BYTES_ONE = 64
 MEASURED_TRAFFIC = {
     # one launch per step (stage fusion 3): k_step_r / k_step
     "step": {"sources": ["axs_step.hip", "axs_stage_math.hpp"],
-             "digest": "eba392a364dd3360",
+             "digest": "e642f85f2cbfd8b0",
              "profile": "profiles/k_step_1024cubed_trace_and_traffic.md",
-             "bytes": {(4, 128): 73.43}},
+             "bytes": {(7, 128): 77.32}},
     # fused pairs, stages 3+4 (modes 1 and 2)
     "pair": {"sources": ["axs_pair.hip", "axs_stage_math.hpp"],
              "digest": None, "profile": None, "bytes": {}},
