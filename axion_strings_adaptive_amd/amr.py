@@ -205,7 +205,8 @@ class Hierarchy:
                  force_full_refinement: bool = False, use_graph: bool = True,
                  regrid_int: int = 0, gradient_threshold: float = capi.DBL_MAX,
                  radial_gradient_threshold: float = capi.DBL_MAX,
-                 level_add_log_mr_over_h=None):
+                 level_add_log_mr_over_h=None,
+                 per_stage_launches: bool = False):
         self.ctx = ctx
         self.lib = ctx.lib
         self.h = None
@@ -249,6 +250,7 @@ class Hierarchy:
         for i, v in enumerate(sched):
             p.level_add_log_mr_over_h[i] = v
         p.use_graph = int(use_graph)
+        p.per_stage_launches = int(per_stage_launches)
         self.params = p
         # ---- level 0: seeded IC on a uniform level ----
         base = capi.Level(ctx, n0, dx0, precision)
@@ -425,6 +427,9 @@ def main(argv=None):
                          "--log-mr-over-h-i, stopping at tau_f")
     ap.add_argument("--c0", type=float, default=0.0)
     ap.add_argument("--log-mr-over-h-i", type=float, default=0.0)
+    ap.add_argument("--per-stage-launches", action="store_true",
+                    help="step the box levels with one launch per RK stage "
+                         "instead of the fused stage pairs (same bits)")
     args = ap.parse_args(argv)
     ctx = capi.Context(0)
     kw = {}
@@ -446,7 +451,7 @@ def main(argv=None):
                   regrid_int=args.regrid_int,
                   gradient_threshold=args.gradient_threshold,
                   radial_gradient_threshold=args.radial_gradient_threshold,
-                  **kw)
+                  per_stage_launches=args.per_stage_launches, **kw)
     h.step()  # warm-up
     ctx.synchronize()
     rs0, n_regrid0, parts0 = h.regrid_s, h.nregrids, dict(h.regrid_parts)

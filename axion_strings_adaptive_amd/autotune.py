@@ -226,7 +226,8 @@ def write_table(results: dict, path: str = TABLE,
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--n", type=int, nargs="+", default=[1024])
+    ap.add_argument("--n", type=int, nargs="*", default=[1024],
+                    help="level sizes to tune (none: box shapes only)")
     ap.add_argument("--precision", type=int, nargs="+", default=[1, 0])
     ap.add_argument("--fusion", type=int, nargs="*", default=None,
                     help="stage fusion modes to tune (default: all the "

@@ -73,7 +73,7 @@ class AmrParams(C.Structure):
                 ("amplitude_threshold", C.c_double),
                 ("n_schedule", C.c_int),
                 ("level_add_log_mr_over_h", C.c_double * AXS_MAX_LEVELS),
-                ("use_graph", C.c_int)]
+                ("use_graph", C.c_int), ("per_stage_launches", C.c_int)]
 
 
 class CoeffBlend(C.Structure):
@@ -154,6 +154,8 @@ _SIGS = {
     "axs_boxlevel_destroy": (_I, [_P]),
     "axs_boxlevel_num_cells": (C.c_int64, [_P]),
     "axs_boxlevel_set_work_shape": (_I, [_P, _I, _I]),
+    "axs_boxlevel_set_stage_fusion": (_I, [_P, _I]),
+    "axs_boxlevel_get_stage_fusion": (_I, [_P, C.POINTER(_I)]),
     "axs_boxlevel_get_work_shape": (_I, [_P, C.POINTER(_I), C.POINTER(_I),
                                          C.POINTER(_I)]),
     "axs_boxlevel_upload_dense": (_I, [_P, _PD]),
@@ -622,6 +624,15 @@ class BoxLevel:
         check(self.lib.axs_boxlevel_get_work_shape(self.h, C.byref(r),
                                                    C.byref(g), C.byref(n)))
         return r.value, g.value, n.value
+
+    def set_fusion(self, mode: int):
+        """RK4 launches: 0 one per stage, 1 fused stage pairs."""
+        check(self.lib.axs_boxlevel_set_stage_fusion(self.h, mode))
+
+    def fusion(self) -> int:
+        m = C.c_int()
+        check(self.lib.axs_boxlevel_get_stage_fusion(self.h, C.byref(m)))
+        return m.value
 
     def upload_dense(self, dense: np.ndarray):
         d = np.ascontiguousarray(dense, dtype=np.float64)

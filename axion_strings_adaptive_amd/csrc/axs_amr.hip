@@ -358,8 +358,12 @@ int create_level(axs_hierarchy *h, const std::vector<BoxSpec> &boxes,
             lo[3 * b + d] = boxes[b].lo[d];
             hi[3 * b + d] = boxes[b].hi[d];
         }
-    return axs_boxlevel_create(h->ctx, (int)boxes.size(), lo.data(), hi.data(),
-                               dom.data(), 3, dx, h->precision, out);
+    const int rc = axs_boxlevel_create(h->ctx, (int)boxes.size(), lo.data(),
+                                       hi.data(), dom.data(), 3, dx,
+                                       h->precision, out);
+    if (rc == AXS_OK && h->p.per_stage_launches)
+        return axs_boxlevel_set_stage_fusion(*out, 0);
+    return rc;
 }
 
 std::array<int, 3> block_dims(const axs_hierarchy *h, int l)

@@ -80,12 +80,18 @@ struct axs_boxlevel
     // identity of this level for the graph caches of finer levels: unique
     // over the process, unlike an address that a later level may reuse
     unsigned long long uid;
+    // RK4 stage launches: 0 one per stage, 1 fused pairs (stages 1+2, 3+4;
+    // the state then alternates between the arrays behind y and acc)
+    int fusion;
     // captured step graphs, keyed by (dt, coarse level uid, t0_frac,
-    // dt_frac); uid 0 = no coarse level
+    // dt_frac) and the state arrays they read (this level's and the coarse
+    // level's, which the fused pairs alternate); uid 0 = no coarse level
     struct GraphEntry
     {
         double dt, t0, df;
         unsigned long long coarse_uid;
+        const void *state, *coarse_state;
+        int fusion;
         hipGraph_t g;
         hipGraphExec_t ge;
     };
@@ -631,6 +637,278 @@ __global__ void __launch_bounds__(256) k_box_stage(BoxStageParams<T> p)
             q1[4] = nq1;
             q2[4] = nq2;
             (void)plane;
+        }
+        __syncthreads(); // LDS reuse by the next item
+    }
+}
+
+// ------------------------------------------------- fused stage pairs --
+// Two launches per RK4 step (stages 1+2, stages 3+4) instead of four, with
+// the bits of the per-stage launches.  The second stage of each pair needs
+// its input psi on the 13-point star.  At a cell of the box (and at the
+// star's points inside the box) that psi is pointwise in what the pair
+// reads -- psi_1 = psi_n + (dt/2) Pi_n, psi_3 = psi_n + dt Pi_2, formed by
+// the expression the first stage stores -- and at the star's ghost points
+// it comes from the ghost shell of a scratch array, filled before the pair
+// by k_fill_axpy with what the per-stage fill of that stage's input writes
+// there.  The first stage of a pair reads its input as the per-stage
+// kernel does (psi_n, or y_2 after a standard fill).
+//
+// Ghost fill of a pair's second-stage psi into the psi shell (width 2) of
+// `dst`: a ghost cell owned by a box of this level gets the value that
+// box's first stage forms there, stage_axpy(psi, a, Pi) of the owner's psi
+// (array ps, components 0, 1) and Pi (array pp, components 2, 3); a
+// coarse-fine ghost gets the quartic interpolation at the stage time, as
+// k_fill<T, 2> writes it.
+template <typename T, int PREC>
+__global__ void k_fill_axpy(LevelView f, T *dst, const T *ps, const T *pp,
+                            double a, int have_coarse, CoarseSrc<T> c,
+                            unsigned long long *err)
+{
+    const BoxD B = f.box[blockIdx.y];
+    const int ng = f.ng, w = 2;
+    const uint32_t gx = B.n[0] + 2 * w, gy = B.n[1] + 2 * w,
+                   gz = B.n[2] + 2 * w;
+    const uint32_t cells =
+        gx * gy * gz - (uint32_t)B.n[0] * B.n[1] * B.n[2];
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < cells;
+         t += gridDim.x * blockDim.x)
+    {
+        int lx, ly, lz;
+        ghost_cell(t, B.n[0], B.n[1], B.n[2], w, lx, ly, lz);
+        const uint32_t d = sidx(B, ng, lx, ly, lz);
+        const int X = B.lo[0] + lx, Y = B.lo[1] + ly, Z = B.lo[2] + lz;
+        uint32_t src;
+        if (locate(f, X, Y, Z, src) >= 0)
+        {
+            for (int comp = 0; comp < 2; ++comp)
+                dst[(size_t)comp * f.total + d] = stage_axpy<T, PREC>(
+                    ps[(size_t)comp * f.total + src], a,
+                    pp[(size_t)(2 + comp) * f.total + src]);
+            continue;
+        }
+        if (!have_coarse)
+        {
+            atomicAdd(err, 1ull);
+            continue;
+        }
+        const int px = pmod(X, f.dom[0]), py = pmod(Y, f.dom[1]),
+                  pz = pmod(Z, f.dom[2]);
+        double v2[2];
+        interp_quartic<T, 2>(c, px, py, pz, v2, err);
+        for (int comp = 0; comp < 2; ++comp)
+            dst[(size_t)comp * f.total + d] = (T)v2[comp];
+    }
+}
+
+template <typename T> struct BoxPairParams
+{
+    const BoxD *box;
+    const WorkItem *items;
+    int nitems;
+    int ng;
+    uint32_t total;
+    const T *yn;  // y_n
+    const T *y2;  // pair 2: y_2, the stage-3 input (psi ghosts filled)
+    T *y2out;     // pair 1: y_2 written
+    const T *gb;  // psi ghost shell of the second stage's input
+    T *acc;       // pair 1: k1 + 2 k2 written; pair 2: read, then y_{n+1}
+                  // written over it (own cell only)
+    float *comp;  // FP32C low words of y_n (pair 2, own cell, in place)
+    const double *coeff; // 12 doubles: stage s at [3s .. 3s+2]
+    double dt, h6;
+    float h6_lo;
+    double inv_dx2;
+};
+
+// PAIR 1: stages 1+2 (star fields psi_n and psi_1); PAIR 2: stages 3+4
+// (psi_2 and psi_3).  Work items, LDS planes and register queues as in
+// k_box_stage, with two star fields.
+template <typename T, int PREC, int PAIR>
+__global__ void __launch_bounds__(256) k_box_pair(BoxPairParams<T> p)
+{
+    __shared__ T sm[2][4][kBoxLds]; // [buffer][A1 A2 B1 B2]
+    const int sa_ = PAIR == 1 ? 0 : 2, sb_ = sa_ + 1; // coefficient stages
+    const double *cfa = p.coeff + 3 * sa_, *cfb = p.coeff + 3 * sb_;
+    const T i2 = (T)p.inv_dx2;
+    const size_t tot = p.total;
+    const int ng = p.ng;
+    // first stage's step (a of its RK update) and the second stage's psi
+    // formation: pair 1 a = dt/2 for both; pair 2 a = dt
+    const double a = PAIR == 1 ? 0.5 * p.dt : p.dt;
+    const T *pA = PAIR == 1 ? p.yn : p.y2; // first stage's input
+    for (int it = blockIdx.x; it < p.nitems; it += gridDim.x)
+    {
+        const WorkItem w = p.items[it];
+        const BoxD B = p.box[w.box];
+        const int W = w.w, H = w.h, nyb = B.n[1], nzb = B.n[2];
+        const int SW = W + 4;
+        const int t = threadIdx.x;
+        const int lx = t % W, ly = t / W;
+        const bool lane = ly < H;
+        const int yy = w.y0 + ly;
+        const bool active = lane && yy < nyb;
+        const int yl = min(yy, nyb + 1);
+        const uint32_t gxs = W + 2 * ng, gys = nyb + 2 * ng;
+        auto at = [&](int x, int y, int z) -> uint32_t {
+            return B.off + ((uint32_t)(z + ng) * gys + (uint32_t)(y + ng)) * gxs +
+                   (uint32_t)(x + ng);
+        };
+        // field A (the first stage's psi) and B (the second stage's psi)
+        // of the point (x, y, z) of the box's grown region
+        auto load = [&](int x, int y, int z, T (&va)[2], T (&vb)[2]) {
+            const uint32_t o = at(x, y, z);
+            const bool ghost =
+                x < 0 || x >= W || y < 0 || y >= nyb || z < 0 || z >= nzb;
+            for (int c = 0; c < 2; ++c)
+            {
+                va[c] = pA[c * tot + o];
+                if (ghost)
+                    vb[c] = p.gb[c * tot + o];
+                else if (PAIR == 1)
+                    vb[c] = stage_axpy<T, PREC>(va[c], a, p.yn[(2 + c) * tot + o]);
+                else
+                    vb[c] = stage_axpy<T, PREC>(p.yn[c * tot + o], a,
+                                                p.y2[(2 + c) * tot + o]);
+            }
+        };
+        const bool hxl = lane && lx < 2, hxr = lane && lx >= W - 2;
+        const bool hyl = lane && ly < 2, hyr = lane && ly >= H - 2;
+        const int hxx = hxl ? lx - 2 : W + (lx - (W - 2));
+        const int hyy = hyl ? w.y0 + ly - 2
+                            : min(w.y0 + H + (ly - (H - 2)), nyb + 1);
+        T qa[5][2] = {}, qb[5][2] = {};
+        if (lane)
+            for (int i = 0; i < 5; ++i)
+                load(lx, yl, i - 2, qa[i], qb[i]);
+        T hxa[2] = {0, 0}, hxb[2] = {0, 0}, hya[2] = {0, 0}, hyb[2] = {0, 0};
+        if (hxl || hxr)
+            load(hxx, yl, 0, hxa, hxb);
+        if (hyl || hyr)
+            load(lx, hyy, 0, hya, hyb);
+        for (int z = 0; z < nzb; ++z)
+        {
+            const int bf = z & 1;
+            T *s[4] = {sm[bf][0], sm[bf][1], sm[bf][2], sm[bf][3]};
+            if (lane)
+            {
+                const int r = (ly + 2) * SW + lx + 2;
+                for (int c = 0; c < 2; ++c)
+                {
+                    s[c][r] = qa[2][c];
+                    s[2 + c][r] = qb[2][c];
+                }
+                if (hxl || hxr)
+                {
+                    const int rx = (ly + 2) * SW + (hxl ? lx : lx + 4);
+                    for (int c = 0; c < 2; ++c)
+                    {
+                        s[c][rx] = hxa[c];
+                        s[2 + c][rx] = hxb[c];
+                    }
+                }
+                if (hyl || hyr)
+                {
+                    const int ry = (hyl ? ly : ly + 4) * SW + lx + 2;
+                    for (int c = 0; c < 2; ++c)
+                    {
+                        s[c][ry] = hya[c];
+                        s[2 + c][ry] = hyb[c];
+                    }
+                }
+            }
+            T nqa[2] = {qa[4][0], qa[4][1]}, nqb[2] = {qb[4][0], qb[4][1]};
+            if (lane && z + 1 < nzb)
+            {
+                load(lx, yl, z + 3, nqa, nqb);
+                if (hxl || hxr)
+                    load(hxx, yl, z + 1, hxa, hxb);
+                if (hyl || hyr)
+                    load(lx, hyy, z + 1, hya, hyb);
+            }
+            const uint32_t oc = active ? at(lx, yy, z) : 0;
+            T yn[4] = {0, 0, 0, 0}, ac[4] = {0, 0, 0, 0}, pi2[2] = {0, 0};
+            float lo[4] = {0, 0, 0, 0};
+            if (active)
+            {
+                for (int c = 0; c < 4; ++c)
+                    yn[c] = p.yn[c * tot + oc];
+                if (PAIR == 2)
+                {
+                    pi2[0] = p.y2[2 * tot + oc];
+                    pi2[1] = p.y2[3 * tot + oc];
+                    for (int c = 0; c < 4; ++c)
+                        ac[c] = p.acc[c * tot + oc];
+                    if (PREC == AXS_FP32C)
+                        for (int c = 0; c < 4; ++c)
+                            lo[c] = p.comp[c * tot + oc];
+                }
+            }
+            __syncthreads();
+            if (active)
+            {
+                const int r = (ly + 2) * SW + lx + 2;
+                using S = typename StarOfB<T>::type;
+                auto star = [&](const T *f, const T (&q)[5][2], int c) {
+                    return S{f[r - 2], f[r - 1], f[r + 1], f[r + 2],
+                             f[r - 2 * SW], f[r - SW], f[r + SW], f[r + 2 * SW],
+                             q[0][c], q[1][c], q[3][c], q[4][c], q[2][c]};
+                };
+                T FA1, FA2, FB1, FB2;
+                force(star_lap(star(s[0], qa, 0), i2),
+                      star_lap(star(s[1], qa, 1), i2), qa[2][0], qa[2][1],
+                      cfa[0], cfa[1], cfa[2], FA1, FA2);
+                force(star_lap(star(s[2], qb, 0), i2),
+                      star_lap(star(s[3], qb, 1), i2), qb[2][0], qb[2][1],
+                      cfb[0], cfb[1], cfb[2], FB1, FB2);
+                if constexpr (PAIR == 1)
+                {
+                    // stage 1 (input y_n), stage 2 (input y_1)
+                    const T k1[4] = {yn[2], yn[3], FA1, FA2};
+                    T acc1[4], y1[4];
+                    stage_update<T, PREC, MODE_S1>(k1, yn, ac, lo, 0.5 * p.dt,
+                                                   p.h6, p.h6_lo, acc1, y1);
+                    const T k2[4] = {y1[2], y1[3], FB1, FB2};
+                    T acc2[4], y2[4];
+                    stage_update<T, PREC, MODE_S23>(k2, yn, acc1, lo,
+                                                    0.5 * p.dt, p.h6, p.h6_lo,
+                                                    acc2, y2);
+                    for (int c = 0; c < 4; ++c)
+                    {
+                        p.y2out[c * tot + oc] = y2[c];
+                        p.acc[c * tot + oc] = acc2[c];
+                    }
+                }
+                else
+                {
+                    // stage 3 (input y_2), stage 4 (input y_3)
+                    const T k3[4] = {pi2[0], pi2[1], FA1, FA2};
+                    T acc3[4], y3[4];
+                    stage_update<T, PREC, MODE_S23>(k3, yn, ac, lo, p.dt, p.h6,
+                                                    p.h6_lo, acc3, y3);
+                    const T k4[4] = {y3[2], y3[3], FB1, FB2};
+                    T unused[4], y4[4];
+                    stage_update<T, PREC, MODE_S4>(k4, yn, acc3, lo, p.dt,
+                                                   p.h6, p.h6_lo, unused, y4);
+                    for (int c = 0; c < 4; ++c)
+                    {
+                        p.acc[c * tot + oc] = y4[c];
+                        if (PREC == AXS_FP32C)
+                            p.comp[c * tot + oc] = lo[c];
+                    }
+                }
+            }
+            for (int i = 0; i < 4; ++i)
+                for (int c = 0; c < 2; ++c)
+                {
+                    qa[i][c] = qa[i + 1][c];
+                    qb[i][c] = qb[i + 1][c];
+                }
+            for (int c = 0; c < 2; ++c)
+            {
+                qa[4][c] = nqa[c];
+                qb[4][c] = nqb[c];
+            }
         }
         __syncthreads(); // LDS reuse by the next item
     }
@@ -1354,6 +1632,134 @@ static int stage(axs_boxlevel *bl, int st, const void *yin, void *yout,
     return stage_t<float, AXS_FP32>(bl, st, yin, yout, dt);
 }
 
+// psi ghosts (shell 2) of a pair's second-stage input into dst's shell:
+// stage_axpy(psi of ps, a, Pi of pp) where a box of this level owns the
+// cell, else the coarse interpolation at t_frac
+template <typename T, int PREC>
+static int fill_axpy_t(axs_boxlevel *bl, void *dst, const void *ps,
+                       const void *pp, double a, const axs_boxlevel *coarse,
+                       double t_frac)
+{
+    CoarseSrc<T> c{};
+    if (coarse)
+    {
+        c.v = view_of(coarse);
+        c.cur = (const T *)coarse->y;
+        c.cur_lo = coarse->comp;
+        if (bl->snap && bl->snap_uid == coarse->uid)
+        {
+            c.snap = (const T *)bl->snap;
+            c.snap_lo = bl->snap_lo;
+        }
+        c.t_frac = t_frac;
+    }
+    hipLaunchKernelGGL((k_fill_axpy<T, PREC>), box_grid(bl, max_shell(bl, 2)),
+                       dim3(256), 0, bl->ctx->stream, view_of(bl), (T *)dst,
+                       (const T *)ps, (const T *)pp, a, coarse ? 1 : 0, c,
+                       bl->derr);
+    AXS_HIP_TRY(hipGetLastError());
+    return AXS_OK;
+}
+
+static int fill_axpy(axs_boxlevel *bl, void *dst, const void *ps,
+                     const void *pp, double a, const axs_boxlevel *coarse,
+                     double t_frac)
+{
+    if (bl->precision == AXS_FP64)
+        return fill_axpy_t<double, AXS_FP64>(bl, dst, ps, pp, a, coarse,
+                                             t_frac);
+    if (bl->precision == AXS_FP32C)
+        return fill_axpy_t<float, AXS_FP32C>(bl, dst, ps, pp, a, coarse,
+                                             t_frac);
+    return fill_axpy_t<float, AXS_FP32>(bl, dst, ps, pp, a, coarse, t_frac);
+}
+
+template <typename T, int PREC>
+static int pair_t(axs_boxlevel *bl, int pair, double dt)
+{
+    BoxPairParams<T> p{};
+    p.box = bl->dbox;
+    p.items = bl->ditems;
+    p.nitems = bl->nitems;
+    p.ng = bl->ng;
+    p.total = bl->total;
+    p.yn = (const T *)bl->y;
+    p.y2 = (const T *)bl->ya;
+    p.y2out = (T *)bl->ya;
+    p.gb = (const T *)bl->yb;
+    p.acc = (T *)bl->acc;
+    p.comp = bl->comp;
+    p.coeff = bl->dcoeff;
+    p.dt = dt;
+    p.h6 = dt / 6.0;
+    p.h6_lo = (float)(p.h6 - (double)(float)p.h6);
+    p.inv_dx2 = 1.0 / (bl->dx * bl->dx);
+    const int grid = std::min(bl->nitems, bl->grid_blocks);
+    hipStream_t s = bl->ctx->stream;
+    if (pair == 1)
+        hipLaunchKernelGGL((k_box_pair<T, PREC, 1>), dim3(grid), dim3(256), 0,
+                           s, p);
+    else
+        hipLaunchKernelGGL((k_box_pair<T, PREC, 2>), dim3(grid), dim3(256), 0,
+                           s, p);
+    AXS_HIP_TRY(hipGetLastError());
+    return AXS_OK;
+}
+
+static int pair(axs_boxlevel *bl, int pr, double dt)
+{
+    if (bl->precision == AXS_FP64)
+        return pair_t<double, AXS_FP64>(bl, pr, dt);
+    if (bl->precision == AXS_FP32C)
+        return pair_t<float, AXS_FP32C>(bl, pr, dt);
+    return pair_t<float, AXS_FP32>(bl, pr, dt);
+}
+
+// The launches of one RK4 step on the level's stream (direct or under
+// capture).  Stage times of the coarse interval: t0, t0 + df/2, t0 + df/2,
+// t0 + df.  One launch per stage: the ghost fill of each stage input, then
+// the stage (inputs y, ya, yb, ya).  Fused pairs: psi_n ghosts of y and
+// psi_1 ghosts into yb's shell, stages 1+2 (acc, y_2 into ya); psi_2 ghosts
+// of ya and psi_3 ghosts into yb's shell, stages 3+4 (y_{n+1} into acc,
+// which the caller then makes the state).
+static int step_launches(axs_boxlevel *b, const axs_boxlevel *coarse,
+                         double dt, double t0, double df)
+{
+    const double tf[4] = {t0, t0 + 0.5 * df, t0 + 0.5 * df, t0 + df};
+    int rc = AXS_OK;
+    if (b->fusion == 1)
+    {
+        rc = fill(b, b->y, nullptr, coarse, tf[0], 0, true);
+        if (!rc)
+            rc = fill_axpy(b, b->yb, b->y, b->y, 0.5 * dt, coarse, tf[1]);
+        if (!rc)
+            rc = pair(b, 1, dt);
+        if (!rc)
+            rc = fill(b, b->ya, nullptr, coarse, tf[2], 0, true);
+        if (!rc)
+            rc = fill_axpy(b, b->yb, b->y, b->ya, dt, coarse, tf[3]);
+        if (!rc)
+            rc = pair(b, 2, dt);
+        return rc;
+    }
+    void *in[4] = {b->y, b->ya, b->yb, b->ya};
+    void *out[4] = {b->ya, b->yb, b->ya, b->y};
+    for (int st = 0; st < 4 && rc == AXS_OK; ++st)
+    {
+        rc = fill(b, in[st], nullptr, coarse, tf[st], 0, true);
+        if (rc == AXS_OK)
+            rc = stage(b, st + 1, in[st], out[st], dt);
+    }
+    return rc;
+}
+
+// after a fused-pair step the state is in the array behind acc
+static void step_done(axs_boxlevel *b)
+{
+    if (b->fusion == 1)
+        std::swap(b->y, b->acc);
+}
+
 } // namespace axs
 
 using namespace axs;
@@ -1444,6 +1850,7 @@ int axs_boxlevel_create(axs_ctx *ctx, int nboxes, const int *lo, const int *hi,
     // 2048 blocks unless the tuning table has one for this precision
     b->item_rows = 256;
     b->grid_blocks = 2048;
+    b->fusion = 1; // fused stage pairs
     (void)tuned_box(precision, &b->item_rows, &b->grid_blocks);
     const std::vector<WorkItem> items = make_items(b->hbox, b->item_rows);
     b->nitems = (int)items.size();
@@ -1701,6 +2108,23 @@ int axs_boxlevel_set_work_shape(axs_boxlevel *b, int item_rows, int grid_blocks)
     return AXS_OK;
 }
 
+int axs_boxlevel_set_stage_fusion(axs_boxlevel *b, int mode)
+{
+    AXS_CHECK_ARG(b, "axs_boxlevel_set_stage_fusion: NULL level");
+    AXS_CHECK_ARG(mode == 0 || mode == 1,
+                  "axs_boxlevel_set_stage_fusion: mode 0 (one launch per "
+                  "stage) or 1 (fused stage pairs)");
+    b->fusion = mode;
+    return AXS_OK;
+}
+
+int axs_boxlevel_get_stage_fusion(const axs_boxlevel *b, int *mode)
+{
+    AXS_CHECK_ARG(b && mode, "axs_boxlevel_get_stage_fusion: NULL argument");
+    *mode = b->fusion;
+    return AXS_OK;
+}
+
 int axs_boxlevel_get_work_shape(const axs_boxlevel *b, int *item_rows,
                                 int *grid_blocks, int *nitems)
 {
@@ -1953,17 +2377,10 @@ int axs_boxlevel_rk4_step(axs_boxlevel *b, const axs_rhs_coeffs coeffs[4],
     if (rc0)
         return rc0;
     hipLaunchKernelGGL(k_zero_err, dim3(1), dim3(64), 0, s, b->derr);
-    const double tf[4] = {t0, t0 + 0.5 * df, t0 + 0.5 * df, t0 + df};
-    void *in[4] = {b->y, b->ya, b->yb, b->ya};
-    void *out[4] = {b->ya, b->yb, b->ya, b->y};
-    for (int st = 0; st < 4; ++st)
-    {
-        int rc = fill(b, in[st], nullptr, coarse, tf[st], 0, true);
-        if (!rc)
-            rc = stage(b, st + 1, in[st], out[st], dt);
-        if (rc)
-            return rc;
-    }
+    const int rc = step_launches(b, coarse, dt, t0, df);
+    if (rc)
+        return rc;
+    step_done(b);
     return check_err(b, "axs_boxlevel_rk4_step");
 }
 
@@ -1983,22 +2400,16 @@ int axs_boxlevel_rk4_step_graph(axs_boxlevel *b, const axs_rhs_coeffs coeffs[4],
         return rc0;
     axs_boxlevel::GraphEntry *hit = nullptr;
     const unsigned long long cuid = coarse ? coarse->uid : 0ull;
+    const void *cstate = coarse ? coarse->y : nullptr;
     for (auto &e : b->graphs)
-        if (e.dt == dt && e.t0 == t0 && e.df == df && e.coarse_uid == cuid)
+        if (e.dt == dt && e.t0 == t0 && e.df == df && e.coarse_uid == cuid &&
+            e.state == b->y && e.coarse_state == cstate &&
+            e.fusion == b->fusion)
             hit = &e;
     if (!hit)
     {
         AXS_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        const double tf[4] = {t0, t0 + 0.5 * df, t0 + 0.5 * df, t0 + df};
-        void *in[4] = {b->y, b->ya, b->yb, b->ya};
-        void *out[4] = {b->ya, b->yb, b->ya, b->y};
-        int rc = AXS_OK;
-        for (int st = 0; st < 4 && rc == AXS_OK; ++st)
-        {
-            rc = fill(b, in[st], nullptr, coarse, tf[st], 0, true);
-            if (rc == AXS_OK)
-                rc = stage(b, st + 1, in[st], out[st], dt);
-        }
+        const int rc = step_launches(b, coarse, dt, t0, df);
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(s, &g);
         if (rc != AXS_OK || e != hipSuccess)
@@ -2016,7 +2427,7 @@ int axs_boxlevel_rk4_step_graph(axs_boxlevel *b, const axs_rhs_coeffs coeffs[4],
             (void)hipGraphDestroy(g);
             return set_error(AXS_ERR_HIP, "hipGraphInstantiate failed");
         }
-        b->graphs.push_back({dt, t0, df, cuid, g, ge});
+        b->graphs.push_back({dt, t0, df, cuid, b->y, cstate, b->fusion, g, ge});
         hit = &b->graphs.back();
     }
     // the ghost fills of the replay count cells without an owning box in
@@ -2025,6 +2436,7 @@ int axs_boxlevel_rk4_step_graph(axs_boxlevel *b, const axs_rhs_coeffs coeffs[4],
     hipLaunchKernelGGL(k_zero_err, dim3(1), dim3(64), 0, s, b->derr);
     AXS_HIP_TRY(hipGetLastError());
     AXS_HIP_TRY(hipGraphLaunch(hit->ge, s));
+    step_done(b);
     return check_err(b, "axs_boxlevel_rk4_step_graph");
 }
 

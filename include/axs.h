@@ -419,6 +419,18 @@ int axs_boxlevel_set_work_shape(axs_boxlevel *bl, int item_rows,
 /* Current item_rows, grid_blocks and number of items (NULL skips). */
 int axs_boxlevel_get_work_shape(const axs_boxlevel *bl, int *item_rows,
                                 int *grid_blocks, int *nitems);
+/* Launches of the box level's RK4 step; no counterpart in the reference
+ * (GRTeclyn runs one RK stage per pass over the boxes).  mode 0: one
+ * persistent launch per stage, each after the ghost fill of its input;
+ * mode 1: two launches, stages 1+2 and 3+4, whose second stage forms its
+ * psi (psi_n + (dt/2) Pi_n, psi_n + dt Pi_2) on the stencil inside the
+ * boxes and reads it at the ghost cells from a fill that writes what the
+ * per-stage fill of that stage's input writes (the owner's value, or the
+ * coarse interpolation at the stage time) -- the same bits, 144 instead of
+ * 304 bytes per FP32C cell and step.  After a mode-1 step the state lives
+ * in the other of two arrays; every entry point follows it.  Default 1. */
+int axs_boxlevel_set_stage_fusion(axs_boxlevel *bl, int mode);
+int axs_boxlevel_get_stage_fusion(const axs_boxlevel *bl, int *mode);
 /* Copies values from / to a dense periodic host array of the whole domain
  * (4 * domain cells, doubles, x fastest); cells outside every box are not
  * read / not written. */
@@ -573,6 +585,9 @@ typedef struct axs_amr_params
                        exist from this log(m_r/H) on (the box plan's
                        axion_strings.derived_level_add_log_mr_over_h) */
     int use_graph; /* replay each level's step from a captured graph */
+    int per_stage_launches; /* 0: every box level steps with fused stage
+                               pairs (axs_boxlevel_set_stage_fusion mode 1);
+                               1: one launch per stage */
 } axs_amr_params;
 /* The stage scalars' blend after the pre-evolution handoff
  * (AxionStringsLevel.cpp:889-907): lambda and the curvature coefficient go

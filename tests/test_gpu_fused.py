@@ -136,6 +136,20 @@ def test_one_launch_step_long_z_chunks(ctx, precision):
                 _same(got, ref)
 
 
+@pytest.mark.parametrize("precision", [1, 2])
+def test_one_launch_step_xcd_bands(ctx, precision):
+    """Variants 7 and 8 deal the tiles to the XCDs in super-bands of 8 x 32
+    tiles; a grid of 4 x 10 tiles in x, y and 8 z chunks has 320 tiles:
+    one full super-band of 256 and 64 tiles in launch order.  Both parts,
+    and the other tile orders, give the per-stage bits."""
+    shape = (256, 100, 64)
+    s0 = random_state(shape, seed=313 + precision)
+    ref = _run(ctx, shape, precision, s0, 2, False, 0)
+    for v in (7, 8, 4, 5):
+        got = _run(ctx, shape, precision, s0, 2, False, 3, v, 8)
+        _same(got, ref)
+
+
 def test_fused_timing_reports_pairs(ctx):
     capi = _capi()
     lvl = capi.Level(ctx, 64, 0.5, capi.AXS_FP32C)
