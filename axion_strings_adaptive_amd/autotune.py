@@ -47,7 +47,7 @@ STEP_VARIANTS = (-1, 4, 6, 5, 8, 7, 3, 2, 0, 1)
 STEP_Z_CHUNKS = (32, 64, 128, 256)
 PAIR_Z_CHUNKS = (16, 32, 64)
 STAGE_Z_CHUNKS = (16, 32, 64)
-BOX_ROWS = (4, 8, 16, 32, 256)
+BOX_ROWS = (4, 8, 16, 32, 64, 512)
 BOX_GRIDS = (512, 1024, 2048, 4096, 8192)
 BOX_EDGES = (16, 32, 64)
 BOX_DOMAIN = 256

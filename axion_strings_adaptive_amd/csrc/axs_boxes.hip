@@ -10,7 +10,7 @@
 // coarse lookups need no host-built index lists.
 //
 // Work-list: every box is cut into items of H rows x full box width W
-// (W * H <= 256 lanes), each item streams through the box's z extent with
+// (W * H <= 512 lanes), each item streams through the box's z extent with
 // the same register-queue + LDS-plane scheme as the uniform kernel, and a
 // persistent grid strides over the items.
 #include "axs_internal.hpp"
@@ -27,8 +27,11 @@
 namespace axs
 {
 
-constexpr int kBlock = 8;   // blocking factor of box corners
-constexpr int kBoxLds = 576; // (W + 4) (H + 4) <= 544 for 4 <= W <= 64
+constexpr int kBlock = 8;     // blocking factor of box corners
+constexpr int kBoxLanes = 512; // threads of a work-list block: one cell of an
+                               // item's W x H plane each
+constexpr int kBoxLds = 816;   // (W + 4) (H + 4) <= 816 for 8 <= W <= 64,
+                               // 4 <= H <= kBoxLanes / W
 
 struct BoxD
 {
@@ -434,7 +437,7 @@ __global__ void k_fill(LevelView f, T *buf, float *lo, int have_coarse,
 }
 
 // --------------------------------------------------- work-list stage ----
-// Items of box b: rows y0, y0 + H, ... of H = clamp(rows, 4, 256 / W)
+// Items of box b: rows y0, y0 + H, ... of H = clamp(rows, 4, 512 / W)
 // rows each (W = the box's x edge, 8..64); a lane row of the item loads the
 // y halo below (its first two rows) or above (its last two), so H >= 4.
 static std::vector<WorkItem> make_items(const std::vector<BoxD> &boxes,
@@ -444,7 +447,7 @@ static std::vector<WorkItem> make_items(const std::vector<BoxD> &boxes,
     for (size_t i = 0; i < boxes.size(); ++i)
     {
         const int W = boxes[i].n[0];
-        const int H = std::max(4, std::min(rows, 256 / W));
+        const int H = std::max(4, std::min(rows, kBoxLanes / W));
         for (int y0 = 0; y0 < boxes[i].n[1]; y0 += H)
             items.push_back(WorkItem{(int)i, y0, H, W});
     }
@@ -481,7 +484,7 @@ template <> struct StarOfB<float>
 };
 
 template <typename T, int PREC, int MODE>
-__global__ void __launch_bounds__(256) k_box_stage(BoxStageParams<T> p)
+__global__ void __launch_bounds__(kBoxLanes) k_box_stage(BoxStageParams<T> p)
 {
     __shared__ T sm[2][2][kBoxLds];
     const double *cf = p.coeff + 3 * p.coeff_stage;
@@ -725,7 +728,7 @@ template <typename T> struct BoxPairParams
 // (psi_2 and psi_3).  Work items, LDS planes and register queues as in
 // k_box_stage, with two star fields.
 template <typename T, int PREC, int PAIR>
-__global__ void __launch_bounds__(256) k_box_pair(BoxPairParams<T> p)
+__global__ void __launch_bounds__(kBoxLanes) k_box_pair(BoxPairParams<T> p)
 {
     __shared__ T sm[2][4][kBoxLds]; // [buffer][A1 A2 B1 B2]
     const int sa_ = PAIR == 1 ? 0 : 2, sb_ = sa_ + 1; // coefficient stages
@@ -1607,16 +1610,16 @@ static int stage_t(axs_boxlevel *bl, int stage, const void *yin, void *yout,
     switch (stage)
     {
     case 1:
-        hipLaunchKernelGGL((k_box_stage<T, PREC, MODE_S1>), dim3(grid), dim3(256),
-                           0, s, p);
+        hipLaunchKernelGGL((k_box_stage<T, PREC, MODE_S1>), dim3(grid),
+                           dim3(kBoxLanes), 0, s, p);
         break;
     case 4:
-        hipLaunchKernelGGL((k_box_stage<T, PREC, MODE_S4>), dim3(grid), dim3(256),
-                           0, s, p);
+        hipLaunchKernelGGL((k_box_stage<T, PREC, MODE_S4>), dim3(grid),
+                           dim3(kBoxLanes), 0, s, p);
         break;
     default:
         hipLaunchKernelGGL((k_box_stage<T, PREC, MODE_S23>), dim3(grid),
-                           dim3(256), 0, s, p);
+                           dim3(kBoxLanes), 0, s, p);
     }
     AXS_HIP_TRY(hipGetLastError());
     return AXS_OK;
@@ -1697,11 +1700,11 @@ static int pair_t(axs_boxlevel *bl, int pair, double dt)
     const int grid = std::min(bl->nitems, bl->grid_blocks);
     hipStream_t s = bl->ctx->stream;
     if (pair == 1)
-        hipLaunchKernelGGL((k_box_pair<T, PREC, 1>), dim3(grid), dim3(256), 0,
-                           s, p);
+        hipLaunchKernelGGL((k_box_pair<T, PREC, 1>), dim3(grid), dim3(kBoxLanes),
+                           0, s, p);
     else
-        hipLaunchKernelGGL((k_box_pair<T, PREC, 2>), dim3(grid), dim3(256), 0,
-                           s, p);
+        hipLaunchKernelGGL((k_box_pair<T, PREC, 2>), dim3(grid), dim3(kBoxLanes),
+                           0, s, p);
     AXS_HIP_TRY(hipGetLastError());
     return AXS_OK;
 }
@@ -1846,9 +1849,9 @@ int axs_boxlevel_create(axs_ctx *ctx, int nboxes, const int *lo, const int *hi,
     }
     b->total = (uint32_t)off;
     b->nvalid = nvalid;
-    // work shape: whole-width items of up to 256 lanes and a grid of up to
+    // work shape: whole-width items of up to 512 lanes and a grid of up to
     // 2048 blocks unless the tuning table has one for this precision
-    b->item_rows = 256;
+    b->item_rows = kBoxLanes;
     b->grid_blocks = 2048;
     b->fusion = 1; // fused stage pairs
     (void)tuned_box(precision, &b->item_rows, &b->grid_blocks);
@@ -2073,8 +2076,8 @@ int64_t axs_boxlevel_num_cells(axs_boxlevel *b) { return b ? b->nvalid : 0; }
 int axs_boxlevel_set_work_shape(axs_boxlevel *b, int item_rows, int grid_blocks)
 {
     AXS_CHECK_ARG(b, "axs_boxlevel_set_work_shape: NULL level");
-    AXS_CHECK_ARG(item_rows >= 1 && item_rows <= 256,
-                  "axs_boxlevel_set_work_shape: 1 <= item_rows <= 256");
+    AXS_CHECK_ARG(item_rows >= 1 && item_rows <= kBoxLanes,
+                  "axs_boxlevel_set_work_shape: 1 <= item_rows <= 512");
     AXS_CHECK_ARG(grid_blocks >= 1 && grid_blocks <= 65536,
                   "axs_boxlevel_set_work_shape: 1 <= grid_blocks <= 65536");
     if (item_rows == b->item_rows && grid_blocks == b->grid_blocks)

@@ -408,9 +408,10 @@ int axs_boxlevel_destroy(axs_boxlevel *bl);
 int64_t axs_boxlevel_num_cells(axs_boxlevel *bl);
 /* Work shape of the box level's RK4 stage launches; no counterpart in the
  * reference (AMReX's ParallelFor picks it there).  Every box is cut into
- * items of whole box rows, H = clamp(item_rows, 4, 256 / box x edge) rows
- * each, and a grid of min(items, grid_blocks) blocks strides over them.
- * Defaults: item_rows 256 (the widest items), grid_blocks 2048, or the
+ * items of whole box rows, H = clamp(item_rows, 4, 512 / box x edge) rows
+ * each (512-thread blocks), and a grid of min(items, grid_blocks) blocks
+ * strides over them.
+ * Defaults: item_rows 512 (the widest items), grid_blocks 2048, or the
  * tuning table's `box` line.  Every shape gives the same bits.  Setting a
  * new shape waits for the level's stream, drops its captured graphs and
  * may reallocate the item list. */

@@ -101,13 +101,13 @@ def test_box_work_shapes_give_the_same_bits(ctx, oracle, table, precision):
         return got, out
 
     (r0, g0, n0), base = run()
-    assert (r0, g0) == (256, 2048)
-    # items of whole-width rows: W x H <= 256 with H >= 4
+    assert (r0, g0) == (512, 2048)
+    # items of whole-width rows: W x H <= 512 with H >= 4
     def items(b, rows):
         w, ny = b[1][0] - b[0][0] + 1, b[1][1] - b[0][1] + 1
-        h = max(4, min(rows, 256 // w))
+        h = max(4, min(rows, 512 // w))
         return -(-ny // h)
-    want = sum(items(b, 256) for b in boxes)
+    want = sum(items(b, 512) for b in boxes)
     assert n0 == want
     for shape in [(4, 7), (8, 2048), (16, 100), (1, 65536)]:
         (r, g, ni), out = run(shape)
