@@ -277,23 +277,23 @@ __global__ void __launch_bounds__(64 * TY) k_pair(PairParams<T> p)
             const uint32_t o = (uint32_t)zz * plane + rowc;
 #pragma unroll
             for (int c = 0; c < 4; ++c)
-                vload<float, VX>(p.comp + c * n + o, lo_[c]);
+                vload_nt<float, VX>(p.comp + c * n + o, lo_[c]);
         }
         if constexpr (PAIR == 2)
         {
             if (!active)
                 return;
             const uint32_t o = (uint32_t)zz * plane + rowc;
-            vload<T, VX>(p.yn + 2 * n + o, pn_[0]);
-            vload<T, VX>(p.yn + 3 * n + o, pn_[1]);
+            vload_nt<T, VX>(p.yn + 2 * n + o, pn_[0]);
+            vload_nt<T, VX>(p.yn + 3 * n + o, pn_[1]);
 #pragma unroll
             for (int c = 0; c < 4; ++c)
-                vload<T, VX>(p.acc + c * n + o, ac_[c]);
+                vload_nt<T, VX>(p.acc + c * n + o, ac_[c]);
             if constexpr (PREC == AXS_FP32C)
             {
 #pragma unroll
                 for (int c = 0; c < 4; ++c)
-                    vload<float, VX>(p.comp + c * n + o, lo_[c]);
+                    vload_nt<float, VX>(p.comp + c * n + o, lo_[c]);
             }
         }
     };

@@ -44,6 +44,23 @@ __device__ __forceinline__ void vload(const T *p, T (&o)[VX])
     }
 }
 
+// the same with the non-temporal cache policy: words read once, at the own
+// cell, which no neighbouring tile reads (the L2 keeps the stencil fields)
+template <typename T, int VX>
+__device__ __forceinline__ void vload_nt(const T *p, T (&o)[VX])
+{
+    if constexpr (VX == 1)
+        o[0] = __builtin_nontemporal_load(p);
+    else
+    {
+        const typename VecOf<T, VX>::type v = __builtin_nontemporal_load(
+            reinterpret_cast<const typename VecOf<T, VX>::type *>(p));
+#pragma unroll
+        for (int i = 0; i < VX; ++i)
+            o[i] = v[i];
+    }
+}
+
 template <typename T, int VX>
 __device__ __forceinline__ void vstore(T *p, const T (&x)[VX], bool nt)
 {
