@@ -653,20 +653,26 @@ __global__ void __launch_bounds__(kBoxLanes) k_box_stage(BoxStageParams<T> p)
 // reads -- psi_1 = psi_n + (dt/2) Pi_n, psi_3 = psi_n + dt Pi_2, formed by
 // the expression the first stage stores -- and at the star's ghost points
 // it comes from the ghost shell of a scratch array, filled before the pair
-// by k_fill_axpy with what the per-stage fill of that stage's input writes
-// there.  The first stage of a pair reads its input as the per-stage
-// kernel does (psi_n, or y_2 after a standard fill).
+// with what the per-stage fill of that stage's input writes there: a ghost
+// cell owned by a box of this level gets the value that box's first stage
+// forms there, stage_axpy(psi, a, Pi) of the owner's psi and Pi; a
+// coarse-fine ghost gets the quartic interpolation at the stage time.  The
+// first stage of a pair reads its input as the per-stage kernel does (psi_n,
+// or y_2, whose ghosts the same fill writes as k_fill<T, 2> would).
 //
-// Ghost fill of a pair's second-stage psi into the psi shell (width 2) of
-// `dst`: a ghost cell owned by a box of this level gets the value that
-// box's first stage forms there, stage_axpy(psi, a, Pi) of the owner's psi
-// (array ps, components 0, 1) and Pi (array pp, components 2, 3); a
-// coarse-fine ghost gets the quartic interpolation at the stage time, as
-// k_fill<T, 2> writes it.
+// Both ghost fills of a fused pair in one pass over the psi shells (width
+// 2): the first stage's psi into dstA's shell (what k_fill<T, 2> of srcA
+// writes at tA) and the second stage's psi into dstB's shell
+// (stage_axpy(psi of ps, a, Pi of pp) of the owner cell, or the coarse
+// interpolation at tB).  Each ghost cell's owner is looked up once; at a
+// coarse-fine cell the coarse stencil is read once and blended at both
+// times, each sum in the order of interp_quartic, so both outputs have the
+// bits of the per-stage fills of those stage inputs.
 template <typename T, int PREC>
-__global__ void k_fill_axpy(LevelView f, T *dst, const T *ps, const T *pp,
-                            double a, int have_coarse, CoarseSrc<T> c,
-                            unsigned long long *err)
+__global__ void k_fill_pair(LevelView f, T *dstA, const T *srcA, T *dstB,
+                            const T *ps, const T *pp, double a,
+                            int have_coarse, CoarseSrc<T> c, double tA,
+                            double tB, unsigned long long *err)
 {
     const BoxD B = f.box[blockIdx.y];
     const int ng = f.ng, w = 2;
@@ -685,22 +691,79 @@ __global__ void k_fill_axpy(LevelView f, T *dst, const T *ps, const T *pp,
         if (locate(f, X, Y, Z, src) >= 0)
         {
             for (int comp = 0; comp < 2; ++comp)
-                dst[(size_t)comp * f.total + d] = stage_axpy<T, PREC>(
-                    ps[(size_t)comp * f.total + src], a,
-                    pp[(size_t)(2 + comp) * f.total + src]);
+            {
+                const size_t o = (size_t)comp * f.total;
+                dstA[o + d] = srcA[o + src];
+                dstB[o + d] = stage_axpy<T, PREC>(
+                    ps[o + src], a, pp[(size_t)(2 + comp) * f.total + src]);
+            }
             continue;
         }
         if (!have_coarse)
         {
-            atomicAdd(err, 1ull);
+            atomicAdd(err, 2ull);
             continue;
         }
-        const int px = pmod(X, f.dom[0]), py = pmod(Y, f.dom[1]),
-                  pz = pmod(Z, f.dom[2]);
-        double v2[2];
-        interp_quartic<T, 2>(c, px, py, pz, v2, err);
+        const int cx = pmod(X, f.dom[0]) >> 1, cy = pmod(Y, f.dom[1]) >> 1,
+                  cz = pmod(Z, f.dom[2]) >> 1;
+        const double *wx = kQuartW[pmod(X, f.dom[0]) & 1],
+                     *wy = kQuartW[pmod(Y, f.dom[1]) & 1],
+                     *wz = kQuartW[pmod(Z, f.dom[2]) & 1];
+        double accA[2] = {}, accB[2] = {};
+        for (int k = 0; k < 5; ++k)
+        {
+            double ayA[2] = {}, ayB[2] = {};
+            for (int j = 0; j < 5; ++j)
+            {
+                double axA[2] = {}, axB[2] = {};
+                for (int i = 0; i < 5; ++i)
+                {
+                    uint32_t ci;
+                    const bool ok = locate(c.v, cx + i - 2, cy + j - 2,
+                                           cz + k - 2, ci) >= 0;
+                    if (!ok)
+                        atomicAdd(err, 2ull);
+                    for (int q = 0; q < 2; ++q)
+                    {
+                        double vA = 0.0, vB = 0.0;
+                        if (ok)
+                        {
+                            const size_t o = (size_t)q * c.v.total + ci;
+                            const double cur = (double)c.cur[o] +
+                                               (c.cur_lo ? (double)c.cur_lo[o] : 0.0);
+                            vA = vB = cur;
+                            if (c.snap)
+                            {
+                                const double sv =
+                                    (double)c.snap[o] +
+                                    (c.snap_lo ? (double)c.snap_lo[o] : 0.0);
+                                if (tA < 1.0)
+                                    vA = (1.0 - tA) * sv + tA * cur;
+                                if (tB < 1.0)
+                                    vB = (1.0 - tB) * sv + tB * cur;
+                            }
+                        }
+                        axA[q] += wx[i] * vA;
+                        axB[q] += wx[i] * vB;
+                    }
+                }
+                for (int q = 0; q < 2; ++q)
+                {
+                    ayA[q] += wy[j] * axA[q];
+                    ayB[q] += wy[j] * axB[q];
+                }
+            }
+            for (int q = 0; q < 2; ++q)
+            {
+                accA[q] += wz[k] * ayA[q];
+                accB[q] += wz[k] * ayB[q];
+            }
+        }
         for (int comp = 0; comp < 2; ++comp)
-            dst[(size_t)comp * f.total + d] = (T)v2[comp];
+        {
+            dstA[(size_t)comp * f.total + d] = (T)accA[comp];
+            dstB[(size_t)comp * f.total + d] = (T)accB[comp];
+        }
     }
 }
 
@@ -1635,13 +1698,12 @@ static int stage(axs_boxlevel *bl, int st, const void *yin, void *yout,
     return stage_t<float, AXS_FP32>(bl, st, yin, yout, dt);
 }
 
-// psi ghosts (shell 2) of a pair's second-stage input into dst's shell:
-// stage_axpy(psi of ps, a, Pi of pp) where a box of this level owns the
-// cell, else the coarse interpolation at t_frac
+// both fills of a pair in one launch: srcA's psi into dstA's shell at tA,
+// stage_axpy(psi of ps, a, Pi of pp) into dstB's shell at tB
 template <typename T, int PREC>
-static int fill_axpy_t(axs_boxlevel *bl, void *dst, const void *ps,
-                       const void *pp, double a, const axs_boxlevel *coarse,
-                       double t_frac)
+static int fill_pair_t(axs_boxlevel *bl, void *dstA, const void *srcA,
+                       void *dstB, const void *ps, const void *pp, double a,
+                       const axs_boxlevel *coarse, double tA, double tB)
 {
     CoarseSrc<T> c{};
     if (coarse)
@@ -1654,27 +1716,27 @@ static int fill_axpy_t(axs_boxlevel *bl, void *dst, const void *ps,
             c.snap = (const T *)bl->snap;
             c.snap_lo = bl->snap_lo;
         }
-        c.t_frac = t_frac;
     }
-    hipLaunchKernelGGL((k_fill_axpy<T, PREC>), box_grid(bl, max_shell(bl, 2)),
-                       dim3(256), 0, bl->ctx->stream, view_of(bl), (T *)dst,
-                       (const T *)ps, (const T *)pp, a, coarse ? 1 : 0, c,
-                       bl->derr);
+    hipLaunchKernelGGL((k_fill_pair<T, PREC>), box_grid(bl, max_shell(bl, 2)),
+                       dim3(256), 0, bl->ctx->stream, view_of(bl), (T *)dstA,
+                       (const T *)srcA, (T *)dstB, (const T *)ps,
+                       (const T *)pp, a, coarse ? 1 : 0, c, tA, tB, bl->derr);
     AXS_HIP_TRY(hipGetLastError());
     return AXS_OK;
 }
 
-static int fill_axpy(axs_boxlevel *bl, void *dst, const void *ps,
-                     const void *pp, double a, const axs_boxlevel *coarse,
-                     double t_frac)
+static int fill_pair(axs_boxlevel *bl, void *dstA, const void *srcA,
+                     void *dstB, const void *ps, const void *pp, double a,
+                     const axs_boxlevel *coarse, double tA, double tB)
 {
     if (bl->precision == AXS_FP64)
-        return fill_axpy_t<double, AXS_FP64>(bl, dst, ps, pp, a, coarse,
-                                             t_frac);
+        return fill_pair_t<double, AXS_FP64>(bl, dstA, srcA, dstB, ps, pp, a,
+                                             coarse, tA, tB);
     if (bl->precision == AXS_FP32C)
-        return fill_axpy_t<float, AXS_FP32C>(bl, dst, ps, pp, a, coarse,
-                                             t_frac);
-    return fill_axpy_t<float, AXS_FP32>(bl, dst, ps, pp, a, coarse, t_frac);
+        return fill_pair_t<float, AXS_FP32C>(bl, dstA, srcA, dstB, ps, pp, a,
+                                             coarse, tA, tB);
+    return fill_pair_t<float, AXS_FP32>(bl, dstA, srcA, dstB, ps, pp, a,
+                                        coarse, tA, tB);
 }
 
 template <typename T, int PREC>
@@ -1721,10 +1783,10 @@ static int pair(axs_boxlevel *bl, int pr, double dt)
 // The launches of one RK4 step on the level's stream (direct or under
 // capture).  Stage times of the coarse interval: t0, t0 + df/2, t0 + df/2,
 // t0 + df.  One launch per stage: the ghost fill of each stage input, then
-// the stage (inputs y, ya, yb, ya).  Fused pairs: psi_n ghosts of y and
-// psi_1 ghosts into yb's shell, stages 1+2 (acc, y_2 into ya); psi_2 ghosts
-// of ya and psi_3 ghosts into yb's shell, stages 3+4 (y_{n+1} into acc,
-// which the caller then makes the state).
+// the stage (inputs y, ya, yb, ya).  Fused pairs: one fill of psi_n ghosts
+// of y and psi_1 ghosts into yb's shell, stages 1+2 (acc, y_2 into ya); one
+// fill of psi_2 ghosts of ya and psi_3 ghosts into yb's shell, stages 3+4
+// (y_{n+1} into acc, which the caller then makes the state).
 static int step_launches(axs_boxlevel *b, const axs_boxlevel *coarse,
                          double dt, double t0, double df)
 {
@@ -1732,15 +1794,13 @@ static int step_launches(axs_boxlevel *b, const axs_boxlevel *coarse,
     int rc = AXS_OK;
     if (b->fusion == 1)
     {
-        rc = fill(b, b->y, nullptr, coarse, tf[0], 0, true);
-        if (!rc)
-            rc = fill_axpy(b, b->yb, b->y, b->y, 0.5 * dt, coarse, tf[1]);
+        rc = fill_pair(b, b->y, b->y, b->yb, b->y, b->y, 0.5 * dt, coarse,
+                       tf[0], tf[1]);
         if (!rc)
             rc = pair(b, 1, dt);
         if (!rc)
-            rc = fill(b, b->ya, nullptr, coarse, tf[2], 0, true);
-        if (!rc)
-            rc = fill_axpy(b, b->yb, b->y, b->ya, dt, coarse, tf[3]);
+            rc = fill_pair(b, b->ya, b->ya, b->yb, b->y, b->ya, dt, coarse,
+                           tf[2], tf[3]);
         if (!rc)
             rc = pair(b, 2, dt);
         return rc;
