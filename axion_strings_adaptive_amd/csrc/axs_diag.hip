@@ -1311,6 +1311,62 @@ __global__ void k_fill(const T *y, const float *lo, uint32_t n, int kind,
     }
 }
 
+// The same fill four cells per lane for FP32 / FP32C levels (n a multiple of
+// 4, 16-byte aligned arrays): one 16-byte load per component (and low word)
+// and one 16-byte store per lane, the per-cell arithmetic of k_fill.
+template <bool HASLO>
+__global__ void k_fill_f4(const float *y, const float *lo, uint32_t n, int kind,
+                          int scheme, double thr, double R, float *buf)
+{
+    const uint32_t n4 = n / 4;
+    for (uint32_t q = blockIdx.x * 256 + threadIdx.x; q < n4;
+         q += gridDim.x * 256)
+    {
+        const uint32_t id = 4 * q;
+        auto get = [&](uint32_t off, double (&v)[4]) {
+            const float4 h = *reinterpret_cast<const float4 *>(y + off + id);
+            v[0] = h.x;
+            v[1] = h.y;
+            v[2] = h.z;
+            v[3] = h.w;
+            if constexpr (HASLO)
+            {
+                const float4 l = *reinterpret_cast<const float4 *>(lo + off + id);
+                v[0] = (double)h.x + (double)l.x;
+                v[1] = (double)h.y + (double)l.y;
+                v[2] = (double)h.z + (double)l.z;
+                v[3] = (double)h.w + (double)l.w;
+            }
+        };
+        float out[4];
+        if (kind == AXS_FIELD_MASKED_A_DOT)
+        {
+            double psi1[4], psi2[4], Pi1[4], Pi2[4];
+            get(0, psi1);
+            get(n, psi2);
+            get(2 * n, Pi1);
+            get(3 * n, Pi2);
+            for (int e = 0; e < 4; ++e)
+                out[e] = (float)masked_a_dot_d(scheme, thr, psi1[e], psi2[e],
+                                               Pi1[e], Pi2[e], R);
+        }
+        else
+        {
+            double v[4];
+            get((uint32_t)(kind + 1) * n, v);
+            for (int e = 0; e < 4; ++e)
+                out[e] = (float)v[e];
+        }
+        *reinterpret_cast<float4 *>(buf + id) =
+            make_float4(out[0], out[1], out[2], out[3]);
+    }
+}
+
+static bool aligned16(const void *p)
+{
+    return ((uintptr_t)p & 15) == 0;
+}
+
 static int nblocks(size_t n, int bs, int cap)
 {
     size_t b = (n + bs - 1) / bs;
@@ -1727,6 +1783,19 @@ int axs_fill_field(axs_level *L, int kind, const axs_masking *m, double R,
         hipLaunchKernelGGL((k_fill<double, double>), dim3(nb), dim3(256), 0, s,
                            (const double *)L->y, (const float *)nullptr, n, kind,
                            m->scheme, m->threshold, R, (double *)buf);
+    else if (n % 4 == 0 && aligned16(L->y) && aligned16(L->comp) &&
+             aligned16(buf))
+    {
+        const int nb4 = nblocks(n / 4, 256, 8192);
+        if (L->comp)
+            hipLaunchKernelGGL(k_fill_f4<true>, dim3(nb4), dim3(256), 0, s,
+                               (const float *)L->y, (const float *)L->comp, n,
+                               kind, m->scheme, m->threshold, R, (float *)buf);
+        else
+            hipLaunchKernelGGL(k_fill_f4<false>, dim3(nb4), dim3(256), 0, s,
+                               (const float *)L->y, (const float *)nullptr, n,
+                               kind, m->scheme, m->threshold, R, (float *)buf);
+    }
     else
         hipLaunchKernelGGL((k_fill<float, float>), dim3(nb), dim3(256), 0, s,
                            (const float *)L->y, (const float *)L->comp, n, kind,

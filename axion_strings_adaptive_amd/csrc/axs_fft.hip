@@ -4,9 +4,10 @@
 //     with a deterministic device reduction: along one (ky,kz) row the shell
 //     index floor(|p|+0.5) is non-decreasing in kx, so a wavefront covering
 //     64 consecutive kx holds contiguous runs of equal shells; a segmented
-//     shuffle scan sums each run and the run's last lane adds it to the
-//     wavefront's private LDS histogram (no two lanes of one instruction hit
-//     the same shell).  Wave histograms are merged in fixed order into
+//     scan over DPP lane patterns sums each run and the run's last lane adds
+//     it to the wavefront's private LDS histogram (no two lanes of one
+//     instruction hit the same shell).  Shells come from integer |p|^2, the
+//     row indices from wave-uniform counters.  Wave histograms are merged in fixed order into
 //     per-block partials, summed in fixed block order by a second pass.
 //   * FourierIC::generate (FourierIC.hpp:237-285): hash-seeded spectral
 //     fill, Hermitian projection of the self-conjugate planes, hipFFT C2R,
@@ -39,13 +40,15 @@ __device__ __forceinline__ int wrapped_freq_d(int idx, int n)
 
 // ------------------------------------------------------- shell binning --
 constexpr int BIN_WAVES = 4;
+constexpr int kBinBatch = 8; // units whose loads a wave keeps in flight
 constexpr int kMaxShells = 1025; // up to 2048^3
 
 struct BinParams
 {
     int nx, ny, nz, nkx; // nkx = nx/2 + 1
     int n_shells, n_max;
-    long chunks_per_row; // ceil(nkx/64)
+    long chunks_per_row; // 64-lane chunks per (ky, kz) row
+    int extra_kx;        // 1: kx = nx/2 is past the last chunk (nkx = 64 c + 1)
     long total_units;    // ny*nz*chunks_per_row
     long units_per_wave;
     double *partials; // [gridDim.x][2 n_shells + 2]
@@ -58,13 +61,60 @@ __device__ __forceinline__ double mag2(const C *spec, size_t i)
     return re * re + im * im;
 }
 
+// floor(|p| + 0.5) for |p|^2 = p2 (an integer below 2^22), in integers:
+// the s with (2s - 1)^2 < 4 p2 < (2s + 1)^2 (4 p2 is even, the bounds
+// odd, so neither can be equal).  This is the shell floor(sqrt(p2) + 0.5)
+// of double arithmetic exactly: sqrt(p2) is never closer to a half-integer
+// than 0.25 / (2 sqrt(p2) + 1) (> 7e-5 here), far more than double
+// rounding moves it.  The float estimate (the hardware square root, within
+// an ulp) is within one of s.
+__device__ __forceinline__ int shell_of(int p2)
+{
+    int s = (int)(__builtin_amdgcn_sqrtf((float)p2) + 0.5f);
+    const int q = 4 * p2;
+    if ((2 * s + 1) * (2 * s + 1) < q)
+        ++s;
+    else if (s > 0 && (2 * s - 1) * (2 * s - 1) > q)
+        --s;
+    return s;
+}
+
+// One step of a segmented inclusive scan through a DPP lane pattern: each
+// lane adds the value of the lane the pattern names if that lane is in the
+// same run (equal shell); lanes the pattern gives no source keep the
+// neighbour-shell sentinel -2 and add nothing.
+template <int CTRL, int ROWS, bool ZERO>
+__device__ __forceinline__ void seg_scan_step(int shell, double &v, int &cnt)
+{
+    // ZERO: lanes without a source read 0 (for the row shifts: v and cnt
+    // add nothing, and shell 0 -- the DC mode alone -- matches only with a
+    // zero value); else they keep the sentinel -2 and zeros
+    const int os = ZERO ? 0 : -2;
+    const long long b = __double_as_longlong(v);
+    const int so = __builtin_amdgcn_update_dpp(os, shell, CTRL, ROWS, 0xf, ZERO);
+    const int vlo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROWS, 0xf, ZERO);
+    const int vhi =
+        __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWS, 0xf, ZERO);
+    const int co = __builtin_amdgcn_update_dpp(0, cnt, CTRL, ROWS, 0xf, ZERO);
+    if (so == shell)
+    {
+        v += __longlong_as_double(((long long)vhi << 32) | (unsigned)vlo);
+        cnt += co;
+    }
+}
+
+// DPP controls of the wave64 scan: shift right by 1, 2, 4, 8 lanes inside
+// each row of 16 lanes, then lane 15 of rows 0 / 2 to rows 1 / 3 and lane
+// 31 to rows 2 and 3.
+constexpr int kDppRowShr = 0x110, kDppBcast15 = 0x142, kDppBcast31 = 0x143;
+
 template <typename C>
 __global__ void __launch_bounds__(64 * BIN_WAVES)
     k_bin(const C *spec, BinParams p)
 {
     extern __shared__ double hist[]; // [BIN_WAVES][2][n_shells]
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     double *h = hist + (size_t)wave * 2 * p.n_shells;
     double *hc = h + p.n_shells;
     for (int s = lane; s < 2 * p.n_shells; s += 64)
@@ -73,60 +123,127 @@ __global__ void __launch_bounds__(64 * BIN_WAVES)
 
     double full = 0.0, insc = 0.0; // per lane
     const long gw = (long)blockIdx.x * BIN_WAVES + wave;
-    const long u0 = gw * p.units_per_wave;
-    const long u1 = min(u0 + p.units_per_wave, p.total_units);
-    const double half = p.n_max / 2.0;
-    for (long u = u0; u < u1; ++u)
+    long u = gw * p.units_per_wave;
+    const long u1 = min(u + p.units_per_wave, p.total_units);
+    const double half = p.n_max / 2.0, hh = half * half;
+    const int nk = p.extra_kx ? p.nkx - 1 : p.nkx; // kx of the chunks
+    // the wave's units run through (k, j, chunk) with chunk fastest: one
+    // division for the first unit, then counters (all wave-uniform).  Units
+    // go in batches whose loads are all issued before the first is used, so
+    // each wave keeps kBinBatch loads in flight.
+    const long row0 = u / p.chunks_per_row;
+    int chunk = (int)(u - row0 * p.chunks_per_row);
+    int j = (int)(row0 % p.ny), k = (int)(row0 / p.ny);
+    size_t rowbase = (size_t)row0 * p.nkx; // spectrum index of (0, j, k)
+    auto kyz2_of = [&](int jj, int kk) {
+        const int ky = wrapped_freq_d(jj, p.ny), kz = wrapped_freq_d(kk, p.nz);
+        return ky * ky + kz * kz;
+    };
+    int kyz2 = kyz2_of(j, k);
+    for (; u < u1; u += kBinBatch)
     {
-        const long row = u / p.chunks_per_row;
-        const int chunk = (int)(u % p.chunks_per_row);
-        const int j = (int)(row % p.ny);
-        const int k = (int)(row / p.ny);
-        const int i = chunk * 64 + lane;
-        const int ky = wrapped_freq_d(j, p.ny), kz = wrapped_freq_d(k, p.nz);
-        int shell = -1;
-        double v = 0.0;
-        double cnt = 0.0;
-        if (i < p.nkx)
+        int bc[kBinBatch], bq[kBinBatch]; // chunk, ky^2 + kz^2
+        size_t br[kBinBatch];             // row base
+        C bre[kBinBatch], bim[kBinBatch];
+#pragma unroll
+        for (int g = 0; g < kBinBatch; ++g)
         {
-            const int kx = i;
-            const double p_mod_sq = (double)(kx * kx + ky * ky + kz * kz);
-            const double p_mod = sqrt(p_mod_sq);
-            const double m2 = mag2(spec, ((size_t)k * p.ny + j) * p.nkx + i);
-            const int wgt = (kx == 0 || kx == p.nx / 2) ? 1 : 2;
-            full += wgt * m2;
-            if (p_mod < half)
+            bc[g] = chunk;
+            bq[g] = kyz2;
+            br[g] = rowbase;
+            // every lane loads (lanes past the row and units past the
+            // wave's range read an element in bounds that is not used), so
+            // no branch separates the batch's loads
+            const int i = min(chunk * 64 + lane, nk - 1);
+            const size_t o = u + g < u1 ? rowbase + i : 0;
+            bre[g] = spec[2 * o];
+            bim[g] = spec[2 * o + 1];
+            if (++chunk == p.chunks_per_row)
             {
-                insc += wgt * m2;
-                const int sh = (int)floor(p_mod + 0.5);
-                if (sh < p.n_shells)
+                chunk = 0;
+                rowbase += p.nkx;
+                if (++j == p.ny)
                 {
-                    shell = sh;
-                    v = wgt * 4.0 * 3.14159265358979323846 * p_mod_sq * m2;
-                    cnt = (double)wgt;
+                    j = 0;
+                    ++k;
+                }
+                kyz2 = kyz2_of(j, k);
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < kBinBatch; ++g)
+        {
+            if (u + g >= u1)
+                break;
+            const int kx = bc[g] * 64 + lane;
+            const int kx0 = bc[g] * 64; // the chunk's smallest kx
+            const double re = (double)bre[g], im = (double)bim[g];
+            const double m2 = re * re + im * im;
+            const int wgt = (kx == 0 || kx == p.nx / 2) ? 1 : 2;
+            if ((double)(kx0 * kx0 + bq[g]) >= hh)
+            {
+                // the whole chunk lies outside the inscribed sphere: the
+                // full-cube sum only (wave-uniform branch)
+                if (kx < nk)
+                    full += wgt * m2;
+            }
+            else
+            {
+                int shell = -1, cnt = 0;
+                double v = 0.0;
+                if (kx < nk)
+                {
+                    const int p2 = kx * kx + bq[g];
+                    full += wgt * m2;
+                    if ((double)p2 < hh) // |p| < n_max / 2
+                    {
+                        insc += wgt * m2;
+                        const int sh = shell_of(p2);
+                        if (sh < p.n_shells)
+                        {
+                            shell = sh;
+                            v = wgt * 4.0 * 3.14159265358979323846 * (double)p2 *
+                                m2;
+                            cnt = wgt;
+                        }
+                    }
+                }
+                // segmented inclusive scan over runs of equal shell: along
+                // a row the shell is non-decreasing in kx, so equal shells
+                // are contiguous
+                seg_scan_step<kDppRowShr + 1, 0xf, true>(shell, v, cnt);
+                seg_scan_step<kDppRowShr + 2, 0xf, true>(shell, v, cnt);
+                seg_scan_step<kDppRowShr + 4, 0xf, true>(shell, v, cnt);
+                seg_scan_step<kDppRowShr + 8, 0xf, true>(shell, v, cnt);
+                seg_scan_step<kDppBcast15, 0xa, false>(shell, v, cnt);
+                seg_scan_step<kDppBcast31, 0xc, false>(shell, v, cnt);
+                const int snext = __shfl_down(shell, 1, 64);
+                const bool tail = (lane == 63) || (snext != shell);
+                if (shell >= 0 && tail)
+                {
+                    h[shell] += v;
+                    hc[shell] += (double)cnt;
                 }
             }
-        }
-        // segmented inclusive scan over runs of equal shell: along a row
-        // the shell is monotone in kx, so equal shells are contiguous
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1)
-        {
-            const double vo = __shfl_up(v, o, 64);
-            const double co = __shfl_up(cnt, o, 64);
-            const int so = __shfl_up(shell, o, 64);
-            if (lane >= o && so == shell)
+            // kx = nx/2 past the last 64-lane chunk (nx a multiple of 128):
+            // lane 63, whose run ended just above, bins it on its own
+            if (p.extra_kx && bc[g] == p.chunks_per_row - 1 && lane == 63)
             {
-                v += vo;
-                cnt += co;
+                const int kxe = p.nkx - 1;
+                const int p2 = kxe * kxe + bq[g];
+                const double m2e = mag2(spec, br[g] + kxe);
+                full += 1 * m2e;
+                if ((double)p2 < hh)
+                {
+                    insc += 1 * m2e;
+                    const int sh = shell_of(p2);
+                    if (sh < p.n_shells)
+                    {
+                        h[sh] += 1 * 4.0 * 3.14159265358979323846 * (double)p2 * m2e;
+                        hc[sh] += 1.0;
+                    }
+                }
             }
-        }
-        const int snext = __shfl_down(shell, 1, 64);
-        const bool tail = (lane == 63) || (snext != shell);
-        if (shell >= 0 && tail)
-        {
-            h[shell] += v;
-            hc[shell] += cnt;
         }
     }
     // wave totals (fixed order)
@@ -412,7 +529,10 @@ int axs_level_spectrum(axs_level *L, int kind, const axs_masking *m, double R,
     p.nkx = p.nx / 2 + 1;
     p.n_shells = n_shells;
     p.n_max = n_max;
-    p.chunks_per_row = (p.nkx + 63) / 64;
+    // nkx = 64 c + 1 (nx a multiple of 128): c chunks, and lane 63 of each
+    // row's last chunk takes kx = nx/2, rather than a chunk of one lane
+    p.extra_kx = (p.nkx % 64 == 1 && p.nkx > 1) ? 1 : 0;
+    p.chunks_per_row = (p.nkx - p.extra_kx + 63) / 64;
     p.total_units = (long)p.ny * p.nz * p.chunks_per_row;
     int nb = 2048;
     const int width = 2 * n_shells + 2;

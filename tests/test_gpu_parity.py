@@ -377,6 +377,30 @@ def test_fill_field_parity(ctx, oracle, kind, scheme):
     np.testing.assert_allclose(got, want, rtol=1e-14, atol=1e-300)
 
 
+@pytest.mark.parametrize("precision", [1, 2])
+@pytest.mark.parametrize("shape", [(32, 16, 8), (9, 5, 7)])
+@pytest.mark.parametrize("kind,scheme", [(0, 0), (0, 1), (0, 2), (1, 0), (2, 0)])
+def test_fill_field_parity_fp32(ctx, oracle, precision, shape, kind, scheme):
+    """FP32 / FP32C levels -- four cells per lane where the cell count is a
+    multiple of 4, one otherwise -- against the oracle's fill of the level's
+    own state (hi + lo in double), rounded to float: within one float ulp
+    (the oracle's double may round the other way at a float tie)."""
+    capi = _capi()
+    nx, ny, nz = shape
+    s0 = np.random.default_rng(sum(shape)).normal(size=(4, nz, ny, nx)) * 0.6
+    s0[0] += 0.5
+    lvl = capi.Level(ctx, shape, 0.5, precision)
+    lvl.upload(s0)
+    lvl.advance(capi.make_background(2.0), 1.0, 0.1, 5)
+    s = lvl.download()
+    got = lvl.fill_field(kind, scheme, 0.8, 2.0)
+    want = oracle.fill_field(s, kind, scheme, 0.8, 2.0).astype(np.float32)
+    lvl.close()
+    assert got.dtype == np.float32
+    np.testing.assert_allclose(got, want, rtol=1.2e-7, atol=1e-30)
+    assert np.count_nonzero(got != want) <= got.size // 1000
+
+
 @pytest.mark.parametrize("kind", [0, 1])
 @pytest.mark.parametrize("shape", [(32, 32, 32), (64, 32, 16)])
 def test_spectrum_parity(ctx, oracle, kind, shape):
