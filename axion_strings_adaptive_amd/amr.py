@@ -232,6 +232,7 @@ class Hierarchy:
         self.rgrad_thr = radial_gradient_threshold
         self.verbose = verbose
         self._views = {}
+        self._composite = None  # collapsed level-0 field of the spectra
         self.levels: list[capi.BoxLevel] = []
         p = capi.AmrParams()
         p.max_level = levels
@@ -385,14 +386,20 @@ class Hierarchy:
         (AxionStringsLevel.cpp:1617-1619 on collapse_to_level0's result):
         (shell sums, shell counts, [full cube, inscribed sphere], sum of
         squares)."""
-        lvl = self.composite_level0()
-        try:
-            R = capi.background_eval(self.bg, 0, self.tau)
-            return lvl.spectrum(kind, scheme, thr, R)
-        finally:
-            lvl.close()
+        # the collapsed level is kept between calls, so its FFT plans
+        # (built on a level's first spectrum) are built once
+        lvl = self._composite
+        if lvl is None:
+            lvl = self._composite = capi.Level(
+                self.ctx, self.domains[0], self.dx[0], self.precision)
+        capi.check(self.lib.axs_hierarchy_collapse(self.h, lvl.h))
+        R = capi.background_eval(self.bg, 0, self.tau)
+        return lvl.spectrum(kind, scheme, thr, R)
 
     def close(self):
+        if self._composite is not None:
+            self._composite.close()
+            self._composite = None
         if self.h:
             for v in self._views.values():
                 v.h = None
@@ -470,6 +477,10 @@ def main(argv=None):
     t0 = time.perf_counter()
     ss, sc, tot, _ = h.composite_spectrum()
     ctx.synchronize()
+    t_spec_first = time.perf_counter() - t0  # builds the FFT plans
+    t0 = time.perf_counter()
+    ss, sc, tot, _ = h.composite_spectrum()
+    ctx.synchronize()
     t_spec = time.perf_counter() - t0
     print(json.dumps({
         "levels": len(h.levels), "cells": h.ncells,
@@ -495,6 +506,7 @@ def main(argv=None):
         "L_tilde^3": (n0 * h.dx[0]) ** 3,
         "rho_tot_unscreened": e["rho_tot_unscreened"],
         "composite_spectrum_ms": 1e3 * t_spec,
+        "composite_spectrum_first_call_ms": 1e3 * t_spec_first,
         "composite_spectrum_nonempty_shells": int((sc > 0).sum())}))
     h.close()
     ctx.close()

@@ -161,6 +161,21 @@ def test_composite_level0_collapses_by_average_down(ctx):
         assert np.array_equal(L.download_dense(), d, equal_nan=True)
     ss, sc, tot, sumsq = h.composite_spectrum()
     assert (sc > 0).sum() > 0 and np.all(np.isfinite(ss))
+    # the spectrum of the collapsed field, whose level is kept between calls
+    lvl = capi.Level(ctx, 32, h.dx[0], capi.AXS_FP64)
+    lvl.upload(g)
+    R = capi.background_eval(h.bg, 0, h.tau)
+    want_spec = lvl.spectrum(capi.AXS_FIELD_MASKED_A_DOT, capi.AXS_MASK_B,
+                             0.8, R)
+    lvl.close()
+    np.testing.assert_allclose(ss, want_spec[0], rtol=1e-12,
+                               atol=1e-14 * np.abs(want_spec[0]).max())
+    assert np.array_equal(sc, want_spec[1])
+    again = h.composite_spectrum()
+    assert np.array_equal(again[0], ss) and np.array_equal(again[1], sc)
+    h.step()  # a later call collapses the advanced hierarchy
+    later = h.composite_spectrum()
+    assert not np.array_equal(later[0], ss)
     h.close()
 
 
