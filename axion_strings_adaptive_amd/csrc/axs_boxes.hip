@@ -38,7 +38,7 @@ struct BoxD
     int lo[3];
     int n[3];
     uint32_t off; // grown-region start in every component array
-    int pad;
+    int erow;     // the box's first row (y, z) in the x-edge arrays
 };
 
 struct WorkItem
@@ -86,6 +86,19 @@ struct axs_boxlevel
     // RK4 stage launches: 0 one per stage, 1 fused pairs (stages 1+2, 3+4;
     // the state then alternates between the arrays behind y and acc)
     int fusion;
+    // x-edge arrays of the fused pairs: columns 0, 1, W-2, W-1 of every
+    // box row (y, z), 4 components, packed (4 values per row, rows of a box
+    // consecutive, z major), so the ghost fills read the x-face sources of
+    // their neighbours contiguously instead of 2 cells per 128-byte line.
+    // edge_n: the state's (valid when edge_gen == ygen and edge_state ==
+    // y; the second pair writes it for y_{n+1}); edge_2: y_2's (the first
+    // pair writes it).  ygen counts writes of the state's valid cells.
+    void *edge_n, *edge_2;
+    void *edge_alloc;
+    size_t edge_alloc_bytes;
+    size_t edge_rows;
+    unsigned long long ygen, edge_gen;
+    const void *edge_state;
     // captured step graphs, keyed by (dt, coarse level uid, t0_frac,
     // dt_frac) and the state arrays they read (this level's and the coarse
     // level's, which the fused pairs alternate); uid 0 = no coarse level
@@ -143,6 +156,13 @@ __device__ __forceinline__ uint32_t sidx(const BoxD &B, int ng, int lx, int ly,
     const uint32_t gx = B.n[0] + 2 * ng, gy = B.n[1] + 2 * ng;
     return B.off + ((uint32_t)(lz + ng) * gy + (uint32_t)(ly + ng)) * gx +
            (uint32_t)(lx + ng);
+}
+
+// index (component 0) in the x-edge arrays of column col (0, 1, 2, 3 for
+// x = 0, 1, W-2, W-1) of row (y, z) of box B
+__device__ __forceinline__ size_t edge_idx(const BoxD &B, int y, int z, int col)
+{
+    return ((size_t)B.erow + (size_t)z * B.n[1] + y) * 4 + col;
 }
 
 // owner box of a periodic global cell and its storage index, or -1
@@ -668,11 +688,16 @@ __global__ void __launch_bounds__(kBoxLanes) k_box_stage(BoxStageParams<T> p)
 // coarse-fine cell the coarse stencil is read once and blended at both
 // times, each sum in the order of interp_quartic, so both outputs have the
 // bits of the per-stage fills of those stage inputs.
+//
+// With x-edge arrays (eA, eps, epp: the x-edge columns of srcA, ps and pp;
+// null without), a ghost cell of an x face (y and z inside the box) owned
+// by a box of this level reads its source there: the same values, packed.
 template <typename T, int PREC>
 __global__ void k_fill_pair(LevelView f, T *dstA, const T *srcA, T *dstB,
                             const T *ps, const T *pp, double a,
                             int have_coarse, CoarseSrc<T> c, double tA,
-                            double tB, unsigned long long *err)
+                            double tB, const T *eA, const T *eps,
+                            const T *epp, size_t es, unsigned long long *err)
 {
     const BoxD B = f.box[blockIdx.y];
     const int ng = f.ng, w = 2;
@@ -687,9 +712,30 @@ __global__ void k_fill_pair(LevelView f, T *dstA, const T *srcA, T *dstB,
         ghost_cell(t, B.n[0], B.n[1], B.n[2], w, lx, ly, lz);
         const uint32_t d = sidx(B, ng, lx, ly, lz);
         const int X = B.lo[0] + lx, Y = B.lo[1] + ly, Z = B.lo[2] + lz;
-        uint32_t src;
-        if (locate(f, X, Y, Z, src) >= 0)
+        const int xw = pmod(X, f.dom[0]), yw = pmod(Y, f.dom[1]),
+                  zw = pmod(Z, f.dom[2]);
+        const int own = f.owner[((zw / kBlock) * f.bdim[1] + yw / kBlock) *
+                                    f.bdim[0] +
+                                xw / kBlock];
+        if (own >= 0)
         {
+            const BoxD S = f.box[own];
+            const int sx = xw - S.lo[0], sy = yw - S.lo[1], sz = zw - S.lo[2];
+            const bool xface = ly >= 0 && ly < B.n[1] && lz >= 0 && lz < B.n[2];
+            if (eA && xface && (sx < 2 || sx >= S.n[0] - 2))
+            {
+                const size_t e = edge_idx(S, sy, sz, sx < 2 ? sx : sx - S.n[0] + 4);
+                for (int comp = 0; comp < 2; ++comp)
+                {
+                    const size_t o = (size_t)comp * f.total;
+                    dstA[o + d] = eA[(size_t)comp * es + e];
+                    dstB[o + d] = stage_axpy<T, PREC>(
+                        eps[(size_t)comp * es + e], a,
+                        epp[(size_t)(2 + comp) * es + e]);
+                }
+                continue;
+            }
+            const uint32_t src = sidx(S, ng, sx, sy, sz);
             for (int comp = 0; comp < 2; ++comp)
             {
                 const size_t o = (size_t)comp * f.total;
@@ -767,6 +813,30 @@ __global__ void k_fill_pair(LevelView f, T *dstA, const T *srcA, T *dstB,
     }
 }
 
+// The x-edge columns (0, 1, W-2, W-1) of every row of every box of y, 4
+// components, into the packed x-edge arrays (grid: x over a box's rows x 4,
+// y over the boxes).
+template <typename T>
+__global__ void k_edges(const BoxD *box, int ng, uint32_t total, const T *y,
+                        T *edge, size_t es)
+{
+    const BoxD B = box[blockIdx.y];
+    const uint32_t n = 4u * (uint32_t)B.n[1] * (uint32_t)B.n[2];
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += gridDim.x * blockDim.x)
+    {
+        const int col = (int)(t & 3);
+        const uint32_t row = t >> 2;
+        const int sy = (int)(row % (uint32_t)B.n[1]),
+                  sz = (int)(row / (uint32_t)B.n[1]);
+        const int sx = col < 2 ? col : B.n[0] - 4 + col;
+        const uint32_t src = sidx(B, ng, sx, sy, sz);
+        const size_t e = edge_idx(B, sy, sz, col);
+        for (int c = 0; c < 4; ++c)
+            edge[c * es + e] = y[(size_t)c * total + src];
+    }
+}
+
 template <typename T> struct BoxPairParams
 {
     const BoxD *box;
@@ -785,6 +855,9 @@ template <typename T> struct BoxPairParams
     double dt, h6;
     float h6_lo;
     double inv_dx2;
+    T *edge;   // x-edge arrays of the pair's output (pair 1: y_2; pair 2:
+               // y_{n+1}), or null
+    size_t es; // their component stride
 };
 
 // PAIR 1: stages 1+2 (star fields psi_n and psi_1); PAIR 2: stages 3+4
@@ -944,6 +1017,13 @@ __global__ void __launch_bounds__(kBoxLanes) k_box_pair(BoxPairParams<T> p)
                         p.y2out[c * tot + oc] = y2[c];
                         p.acc[c * tot + oc] = acc2[c];
                     }
+                    if (p.edge && (lx < 2 || lx >= W - 2))
+                    {
+                        const size_t e =
+                            edge_idx(B, yy, z, lx < 2 ? lx : lx - W + 4);
+                        for (int c = 0; c < 4; ++c)
+                            p.edge[c * p.es + e] = y2[c];
+                    }
                 }
                 else
                 {
@@ -961,6 +1041,13 @@ __global__ void __launch_bounds__(kBoxLanes) k_box_pair(BoxPairParams<T> p)
                         p.acc[c * tot + oc] = y4[c];
                         if (PREC == AXS_FP32C)
                             p.comp[c * tot + oc] = lo[c];
+                    }
+                    if (p.edge && (lx < 2 || lx >= W - 2))
+                    {
+                        const size_t e =
+                            edge_idx(B, yy, z, lx < 2 ? lx : lx - W + 4);
+                        for (int c = 0; c < 4; ++c)
+                            p.edge[c * p.es + e] = y4[c];
                     }
                 }
             }
@@ -1700,10 +1787,12 @@ static int stage(axs_boxlevel *bl, int st, const void *yin, void *yout,
 
 // both fills of a pair in one launch: srcA's psi into dstA's shell at tA,
 // stage_axpy(psi of ps, a, Pi of pp) into dstB's shell at tB
+// (eA, eps, epp: the x-edge arrays of srcA, ps and pp)
 template <typename T, int PREC>
 static int fill_pair_t(axs_boxlevel *bl, void *dstA, const void *srcA,
                        void *dstB, const void *ps, const void *pp, double a,
-                       const axs_boxlevel *coarse, double tA, double tB)
+                       const axs_boxlevel *coarse, double tA, double tB,
+                       const void *eA, const void *eps, const void *epp)
 {
     CoarseSrc<T> c{};
     if (coarse)
@@ -1720,23 +1809,54 @@ static int fill_pair_t(axs_boxlevel *bl, void *dstA, const void *srcA,
     hipLaunchKernelGGL((k_fill_pair<T, PREC>), box_grid(bl, max_shell(bl, 2)),
                        dim3(256), 0, bl->ctx->stream, view_of(bl), (T *)dstA,
                        (const T *)srcA, (T *)dstB, (const T *)ps,
-                       (const T *)pp, a, coarse ? 1 : 0, c, tA, tB, bl->derr);
+                       (const T *)pp, a, coarse ? 1 : 0, c, tA, tB,
+                       (const T *)eA, (const T *)eps, (const T *)epp,
+                       4 * bl->edge_rows, bl->derr);
     AXS_HIP_TRY(hipGetLastError());
     return AXS_OK;
 }
 
 static int fill_pair(axs_boxlevel *bl, void *dstA, const void *srcA,
                      void *dstB, const void *ps, const void *pp, double a,
-                     const axs_boxlevel *coarse, double tA, double tB)
+                     const axs_boxlevel *coarse, double tA, double tB,
+                     const void *eA, const void *eps, const void *epp)
 {
     if (bl->precision == AXS_FP64)
         return fill_pair_t<double, AXS_FP64>(bl, dstA, srcA, dstB, ps, pp, a,
-                                             coarse, tA, tB);
+                                             coarse, tA, tB, eA, eps, epp);
     if (bl->precision == AXS_FP32C)
         return fill_pair_t<float, AXS_FP32C>(bl, dstA, srcA, dstB, ps, pp, a,
-                                             coarse, tA, tB);
+                                             coarse, tA, tB, eA, eps, epp);
     return fill_pair_t<float, AXS_FP32>(bl, dstA, srcA, dstB, ps, pp, a,
-                                        coarse, tA, tB);
+                                        coarse, tA, tB, eA, eps, epp);
+}
+
+// The x-edge arrays of the state, from y, unless they already hold the
+// state's (the last fused-pair step wrote them and nothing has written the
+// state since).  Run outside any capture, before a fused-pair step.
+template <typename T>
+static int ensure_edges_t(axs_boxlevel *bl)
+{
+    if (bl->edge_gen == bl->ygen && bl->edge_state == bl->y)
+        return AXS_OK;
+    size_t rows = 0;
+    for (const BoxD &B : bl->hbox)
+        rows = std::max(rows, (size_t)B.n[1] * B.n[2]);
+    hipLaunchKernelGGL(k_edges<T>, box_grid(bl, 4 * rows), dim3(256), 0,
+                       bl->ctx->stream, bl->dbox, bl->ng, bl->total,
+                       (const T *)bl->y, (T *)bl->edge_n, 4 * bl->edge_rows);
+    AXS_HIP_TRY(hipGetLastError());
+    bl->edge_gen = bl->ygen;
+    bl->edge_state = bl->y;
+    return AXS_OK;
+}
+
+static int ensure_edges(axs_boxlevel *bl)
+{
+    if (bl->fusion != 1)
+        return AXS_OK;
+    return bl->precision == AXS_FP64 ? ensure_edges_t<double>(bl)
+                                     : ensure_edges_t<float>(bl);
 }
 
 template <typename T, int PREC>
@@ -1759,6 +1879,8 @@ static int pair_t(axs_boxlevel *bl, int pair, double dt)
     p.h6 = dt / 6.0;
     p.h6_lo = (float)(p.h6 - (double)(float)p.h6);
     p.inv_dx2 = 1.0 / (bl->dx * bl->dx);
+    p.edge = (T *)(pair == 1 ? bl->edge_2 : bl->edge_n);
+    p.es = 4 * bl->edge_rows;
     const int grid = std::min(bl->nitems, bl->grid_blocks);
     hipStream_t s = bl->ctx->stream;
     if (pair == 1)
@@ -1786,7 +1908,11 @@ static int pair(axs_boxlevel *bl, int pr, double dt)
 // the stage (inputs y, ya, yb, ya).  Fused pairs: one fill of psi_n ghosts
 // of y and psi_1 ghosts into yb's shell, stages 1+2 (acc, y_2 into ya); one
 // fill of psi_2 ghosts of ya and psi_3 ghosts into yb's shell, stages 3+4
-// (y_{n+1} into acc, which the caller then makes the state).
+// (y_{n+1} into acc, which the caller then makes the state).  The fills'
+// fine-fine x-face sources come from the x-edge arrays: the state's
+// (edge_n, made current by ensure_edges before the step) and y_2's
+// (edge_2, written by the first pair); the second pair writes y_{n+1}'s
+// into edge_n.
 static int step_launches(axs_boxlevel *b, const axs_boxlevel *coarse,
                          double dt, double t0, double df)
 {
@@ -1795,12 +1921,12 @@ static int step_launches(axs_boxlevel *b, const axs_boxlevel *coarse,
     if (b->fusion == 1)
     {
         rc = fill_pair(b, b->y, b->y, b->yb, b->y, b->y, 0.5 * dt, coarse,
-                       tf[0], tf[1]);
+                       tf[0], tf[1], b->edge_n, b->edge_n, b->edge_n);
         if (!rc)
             rc = pair(b, 1, dt);
         if (!rc)
             rc = fill_pair(b, b->ya, b->ya, b->yb, b->y, b->ya, dt, coarse,
-                           tf[2], tf[3]);
+                           tf[2], tf[3], b->edge_2, b->edge_n, b->edge_2);
         if (!rc)
             rc = pair(b, 2, dt);
         return rc;
@@ -1816,11 +1942,17 @@ static int step_launches(axs_boxlevel *b, const axs_boxlevel *coarse,
     return rc;
 }
 
-// after a fused-pair step the state is in the array behind acc
+// after a fused-pair step the state is in the array behind acc, and
+// edge_n holds its x-edge columns
 static void step_done(axs_boxlevel *b)
 {
+    ++b->ygen;
     if (b->fusion == 1)
+    {
         std::swap(b->y, b->acc);
+        b->edge_gen = b->ygen;
+        b->edge_state = b->y;
+    }
 }
 
 } // namespace axs
@@ -1861,7 +1993,7 @@ int axs_boxlevel_create(axs_ctx *ctx, int nboxes, const int *lo, const int *hi,
         b->bdim[d] = domain[d] / kBlock;
     }
     std::vector<int> owner((size_t)b->bdim[0] * b->bdim[1] * b->bdim[2], -1);
-    uint64_t off = 0;
+    uint64_t off = 0, erows = 0;
     int64_t nvalid = 0;
     for (int i = 0; i < nboxes; ++i)
     {
@@ -1883,6 +2015,8 @@ int axs_boxlevel_create(axs_ctx *ctx, int nboxes, const int *lo, const int *hi,
         B.off = (uint32_t)off;
         off += (uint64_t)(B.n[0] + 2 * nghost) * (B.n[1] + 2 * nghost) *
                (B.n[2] + 2 * nghost);
+        B.erow = (int)erows;
+        erows += (uint64_t)B.n[1] * B.n[2];
         nvalid += (int64_t)B.n[0] * B.n[1] * B.n[2];
         for (int z = B.lo[2] / kBlock; z < (B.lo[2] + B.n[2]) / kBlock; ++z)
             for (int y = B.lo[1] / kBlock; y < (B.lo[1] + B.n[1]) / kBlock; ++y)
@@ -1909,6 +2043,7 @@ int axs_boxlevel_create(axs_ctx *ctx, int nboxes, const int *lo, const int *hi,
     }
     b->total = (uint32_t)off;
     b->nvalid = nvalid;
+    b->edge_rows = (size_t)erows; // < 2^32 / 64 since off < 2^32
     // work shape: whole-width items of up to 512 lanes and a grid of up to
     // 2048 blocks unless the tuning table has one for this precision
     b->item_rows = kBoxLanes;
@@ -1936,10 +2071,17 @@ int axs_boxlevel_create(axs_ctx *ctx, int nboxes, const int *lo, const int *hi,
         b->alloc[3] = pool_alloc(ctx, bytes4 + pad, &b->alloc_bytes[3]);
         ok = b->alloc[3] != nullptr;
     }
+    const size_t ebytes = (size_t)16 * b->edge_rows * b->word; // one edge array
+    if (ok)
+    {
+        b->edge_alloc = pool_alloc(ctx, 2 * ebytes, &b->edge_alloc_bytes);
+        ok = b->edge_alloc != nullptr;
+    }
     if (!ok)
     {
         for (int i = 0; i < 4; ++i)
             pool_release(ctx, b->alloc[i], b->alloc_bytes[i]);
+        pool_release(ctx, b->edge_alloc, b->edge_alloc_bytes);
         (void)hipFree(b->dbox);
         (void)hipFree(b->downer);
         (void)hipFree(b->ditems);
@@ -1955,6 +2097,11 @@ int axs_boxlevel_create(axs_ctx *ctx, int nboxes, const int *lo, const int *hi,
     b->yb = (char *)b->alloc[2] + bytes4 + 11 * S;
     if (b->alloc[3])
         b->comp = (float *)((char *)b->alloc[3] + 4 * S);
+    b->edge_n = b->edge_alloc;
+    b->edge_2 = (char *)b->edge_alloc + ebytes;
+    b->ygen = 1; // edge_gen 0: the x-edge arrays hold nothing yet
+    b->edge_gen = 0;
+    b->edge_state = nullptr;
     hipStream_t s = ctx->stream;
     AXS_HIP_TRY(hipMemcpyAsync(b->dbox, b->hbox.data(), nboxes * sizeof(BoxD),
                                hipMemcpyHostToDevice, s));
@@ -2120,6 +2267,7 @@ int axs_boxlevel_destroy(axs_boxlevel *b)
     }
     for (int i = 0; i < 4; ++i)
         pool_release(b->ctx, b->alloc[i], b->alloc_bytes[i]);
+    pool_release(b->ctx, b->edge_alloc, b->edge_alloc_bytes);
     (void)hipFree(b->dbox);
     (void)hipFree(b->downer);
     (void)hipFree(b->ditems);
@@ -2204,6 +2352,7 @@ int axs_boxlevel_get_work_shape(const axs_boxlevel *b, int *item_rows,
 int axs_boxlevel_upload_dense(axs_boxlevel *b, const double *dense)
 {
     AXS_CHECK_ARG(b && dense, "axs_boxlevel_upload_dense: NULL argument");
+    ++b->ygen; // writes the state
     const size_t n4 = (size_t)4 * b->total;
     const size_t nd = (size_t)b->dom[0] * b->dom[1] * b->dom[2];
     std::vector<double> st(n4, 0.0);
@@ -2302,6 +2451,8 @@ static int copy_level(const axs_boxlevel *b, axs_level *L, int to_level)
 
 int axs_boxlevel_copy_from_level(axs_boxlevel *b, const axs_level *src)
 {
+    if (b)
+        ++b->ygen; // writes the state
     return copy_level(b, const_cast<axs_level *>(src), 0);
 }
 
@@ -2315,6 +2466,7 @@ int axs_boxlevel_copy_valid(axs_boxlevel *dst, const axs_boxlevel *src,
 {
     AXS_CHECK_ARG(dst && src, "axs_boxlevel_copy_valid: NULL argument");
     AXS_CHECK_ARG(dst != src, "axs_boxlevel_copy_valid: same level");
+    ++dst->ygen; // writes the state
     for (int d = 0; d < 3; ++d)
         AXS_CHECK_ARG(dst->dom[d] == src->dom[d],
                       "axs_boxlevel_copy_valid: domains differ");
@@ -2387,6 +2539,8 @@ int axs_boxlevel_fill_ghosts(axs_boxlevel *b, const axs_boxlevel *coarse,
                              double t_frac, int interior)
 {
     AXS_CHECK_ARG(b, "axs_boxlevel_fill_ghosts: NULL level");
+    if (interior)
+        ++b->ygen; // writes the state's valid cells
     AXS_CHECK_ARG(!interior || coarse,
                   "axs_boxlevel_fill_ghosts: interior fill needs a coarse level");
     if (coarse)
@@ -2439,10 +2593,15 @@ int axs_boxlevel_rk4_step(axs_boxlevel *b, const axs_rhs_coeffs coeffs[4],
     int rc0 = upload_stage_scalars(b, coeffs);
     if (rc0)
         return rc0;
+    if ((rc0 = ensure_edges(b)))
+        return rc0;
     hipLaunchKernelGGL(k_zero_err, dim3(1), dim3(64), 0, s, b->derr);
     const int rc = step_launches(b, coarse, dt, t0, df);
     if (rc)
+    {
+        ++b->ygen; // launches may have run part of the step
         return rc;
+    }
     step_done(b);
     return check_err(b, "axs_boxlevel_rk4_step");
 }
@@ -2460,6 +2619,8 @@ int axs_boxlevel_rk4_step_graph(axs_boxlevel *b, const axs_rhs_coeffs coeffs[4],
     hipStream_t s = b->ctx->stream;
     int rc0 = upload_stage_scalars(b, coeffs);
     if (rc0)
+        return rc0;
+    if ((rc0 = ensure_edges(b)))
         return rc0;
     axs_boxlevel::GraphEntry *hit = nullptr;
     const unsigned long long cuid = coarse ? coarse->uid : 0ull;
@@ -2592,6 +2753,7 @@ int axs_boxlevel_coverage_mask(const axs_boxlevel *f, const int cd[3],
 int axs_boxlevel_average_down(const axs_boxlevel *f, axs_boxlevel *c)
 {
     AXS_CHECK_ARG(f && c, "axs_boxlevel_average_down: NULL argument");
+    ++c->ygen; // writes the state
     for (int d = 0; d < 3; ++d)
         AXS_CHECK_ARG(c->dom[d] * 2 == f->dom[d],
                       "axs_boxlevel_average_down: coarse domain must be half");

@@ -152,3 +152,87 @@ def test_fusion_setter_rejects_bad_modes(ctx):
     with pytest.raises(capi.AxsError, match="fused stage pairs"):
         bl.set_fusion(2)
     bl.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_pairs_after_state_writes(ctx, precision):
+    """The fills of the fused pairs read their neighbours' x-face cells from
+    packed x-edge arrays that the second pair writes for the next step; every
+    other write of the state (an upload, an average-down onto the level, a
+    copy from a uniform level, a regrid copy, a step with one launch per
+    stage, an interior fill from a coarse level) makes the next step rebuild
+    them.  Each such write followed by steps gives the per-stage bits."""
+    capi = _capi()
+    n = 64
+    bg = capi.make_background(2.0, c0=1.0)
+    boxes = tiling(n, [32, 16])
+
+    def run(mode):
+        bl = capi.BoxLevel(ctx, boxes, (n,) * 3, 0.5, precision)
+        bl.set_fusion(mode)
+        bl.upload_dense(_state(n, 31))
+        clock = [1.0]
+
+        def step(k=1):
+            for _ in range(k):
+                dt = 0.1
+                bl.rk4_step(_coeffs(capi, bg, clock[0], dt), dt)
+                clock[0] += dt
+
+        step(2)
+        bl.upload_dense(_state(n, 32))
+        step(2)
+        fine = capi.BoxLevel(ctx, [((0, 0, 0), (31, 31, 31)),
+                                   ((64, 32, 0), (95, 63, 31))],
+                             (2 * n,) * 3, 0.25, precision)
+        fine.upload_dense(_state(2 * n, 33))
+        fine.average_down_to(bl)
+        fine.close()
+        step()
+        lvl = capi.Level(ctx, n, 0.5, precision)
+        lvl.upload(_state(n, 34))
+        bl.copy_from_level(lvl)
+        lvl.close()
+        step()
+        other = capi.BoxLevel(ctx, tiling(n, [16]), (n,) * 3, 0.5, precision)
+        other.upload_dense(_state(n, 35))
+        bl.copy_valid_from(other)
+        other.close()
+        step(2)
+        bl.set_fusion(0)
+        step()
+        bl.set_fusion(mode)
+        step()
+        coarse = capi.BoxLevel(ctx, tiling(n // 2, [16]), (n // 2,) * 3, 1.0,
+                               precision)
+        coarse.upload_dense(_state(n // 2, 36))
+        bl.fill_ghosts(coarse, 1.0, interior=True)
+        coarse.close()
+        step(2)
+        out = bl.download_dense(fill=0.0)
+        bl.close()
+        return out
+
+    assert np.array_equal(run(0), run(1))
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_hierarchy_pairs_equal_per_stage(ctx, use_graph):
+    """A whole AMR hierarchy (two refined levels, subcycling, regrids every
+    coarse step, average-down) stepped with the fused pairs gives the bits
+    of one launch per stage on every level."""
+    from axion_strings_adaptive_amd import amr
+    capi = _capi()
+    out = {}
+    for per_stage in (True, False):
+        h = amr.Hierarchy(ctx, n0=32, levels=2, precision=capi.AXS_FP32C,
+                          verbose=False, regrid_int=1, use_graph=use_graph,
+                          per_stage_launches=per_stage)
+        for _ in range(3):
+            h.step()
+        out[per_stage] = ([L.boxes for L in h.levels],
+                          [L.download_dense(fill=0.0) for L in h.levels])
+        h.close()
+    assert out[True][0] == out[False][0]
+    for a, b in zip(out[True][1], out[False][1]):
+        assert np.array_equal(a, b)
