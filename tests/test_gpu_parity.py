@@ -402,8 +402,11 @@ def test_fill_field_parity_fp32(ctx, oracle, precision, shape, kind, scheme):
 
 
 @pytest.mark.parametrize("kind", [0, 1])
-@pytest.mark.parametrize("shape", [(32, 32, 32), (64, 32, 16)])
+@pytest.mark.parametrize("shape", [(32, 32, 32), (64, 32, 16), (128, 16, 8),
+                                   (256, 8, 8)])
 def test_spectrum_parity(ctx, oracle, kind, shape):
+    """nx = 128 and 256: kx = nx/2 lies past the last 64-wide chunk of a
+    row and is binned on its own."""
     capi = _capi()
     nx, ny, nz = shape
     lvl = capi.Level(ctx, shape, 0.5, capi.AXS_FP64)
@@ -421,12 +424,16 @@ def test_spectrum_parity(ctx, oracle, kind, shape):
     assert tot[0] == pytest.approx(nx * ny * nz * ssq, rel=1e-10)
 
 
-def test_spectrum_fp32c_close(ctx, oracle):
+@pytest.mark.parametrize("shape", [(32, 32, 32), (128, 16, 8)])
+def test_spectrum_fp32c_close(ctx, oracle, shape):
     capi = _capi()
-    lvl = evolved(ctx, oracle, n=32, steps=10, precision=capi.AXS_FP32C)
+    lvl = capi.Level(ctx, shape, 0.5, capi.AXS_FP32C)
+    lvl.upload(oracle.seeded_ic(shape))
+    lvl.advance(capi.make_background(2.0), 1.0, 0.1, 10)
     s = lvl.download()
     ss, sc, tot, _ = lvl.spectrum(capi.AXS_FIELD_PI1)
     wss, wsc, wtot = oracle.spectrum(oracle.fill_field(s, 1))
+    lvl.close()
     assert np.array_equal(sc, wsc)
     np.testing.assert_allclose(ss, wss, rtol=2e-5, atol=1e-6 * wss.max())
 
