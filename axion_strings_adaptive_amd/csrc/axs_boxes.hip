@@ -911,15 +911,50 @@ __global__ void __launch_bounds__(kBoxLanes) k_box_pair(BoxPairParams<T> p)
                                                 p.y2[(2 + c) * tot + o]);
             }
         };
+        // the same for the lane's own column, also keeping the values read
+        // to form field B (pair 1: Pi_n; pair 2: psi_n, Pi_2), which the
+        // stage update of that cell uses again when it reaches the centre
+        // of the queue (so they are not read twice).  FP64 pair 2 reads them
+        // again instead: its queue would take it past 128 VGPRs.
+        constexpr int NR = PAIR == 1 ? 2 : 4;
+        constexpr bool keep = PAIR == 1 || sizeof(T) == 4;
+        auto load_own = [&](int x, int y, int z, T (&va)[2], T (&vb)[2],
+                            T (&vr)[NR]) {
+            const uint32_t o = at(x, y, z);
+            const bool ghost =
+                x < 0 || x >= W || y < 0 || y >= nyb || z < 0 || z >= nzb;
+            for (int c = 0; c < 2; ++c)
+            {
+                va[c] = pA[c * tot + o];
+                if (ghost)
+                    vb[c] = p.gb[c * tot + o];
+                else if constexpr (PAIR == 1)
+                {
+                    const T pin = p.yn[(2 + c) * tot + o];
+                    vr[c] = pin;
+                    vb[c] = stage_axpy<T, PREC>(va[c], a, pin);
+                }
+                else
+                {
+                    const T psn = p.yn[c * tot + o], pi2v = p.y2[(2 + c) * tot + o];
+                    if constexpr (keep)
+                    {
+                        vr[c] = psn;
+                        vr[2 + c] = pi2v;
+                    }
+                    vb[c] = stage_axpy<T, PREC>(psn, a, pi2v);
+                }
+            }
+        };
         const bool hxl = lane && lx < 2, hxr = lane && lx >= W - 2;
         const bool hyl = lane && ly < 2, hyr = lane && ly >= H - 2;
         const int hxx = hxl ? lx - 2 : W + (lx - (W - 2));
         const int hyy = hyl ? w.y0 + ly - 2
                             : min(w.y0 + H + (ly - (H - 2)), nyb + 1);
-        T qa[5][2] = {}, qb[5][2] = {};
+        T qa[5][2] = {}, qb[5][2] = {}, qr[5][NR] = {};
         if (lane)
             for (int i = 0; i < 5; ++i)
-                load(lx, yl, i - 2, qa[i], qb[i]);
+                load_own(lx, yl, i - 2, qa[i], qb[i], qr[i]);
         T hxa[2] = {0, 0}, hxb[2] = {0, 0}, hya[2] = {0, 0}, hyb[2] = {0, 0};
         if (hxl || hxr)
             load(hxx, yl, 0, hxa, hxb);
@@ -957,9 +992,12 @@ __global__ void __launch_bounds__(kBoxLanes) k_box_pair(BoxPairParams<T> p)
                 }
             }
             T nqa[2] = {qa[4][0], qa[4][1]}, nqb[2] = {qb[4][0], qb[4][1]};
+            T nqr[NR];
+            for (int c = 0; c < NR; ++c)
+                nqr[c] = qr[4][c];
             if (lane && z + 1 < nzb)
             {
-                load(lx, yl, z + 3, nqa, nqb);
+                load_own(lx, yl, z + 3, nqa, nqb, nqr);
                 if (hxl || hxr)
                     load(hxx, yl, z + 1, hxa, hxb);
                 if (hyl || hyr)
@@ -970,12 +1008,37 @@ __global__ void __launch_bounds__(kBoxLanes) k_box_pair(BoxPairParams<T> p)
             float lo[4] = {0, 0, 0, 0};
             if (active)
             {
-                for (int c = 0; c < 4; ++c)
-                    yn[c] = p.yn[c * tot + oc];
+                // the cell's own y_n (and Pi_2): from the queues where the
+                // loads of field B put them, Pi_n of pair 2 from memory
+                if constexpr (PAIR == 1)
+                {
+                    yn[0] = qa[2][0];
+                    yn[1] = qa[2][1];
+                    yn[2] = qr[2][0];
+                    yn[3] = qr[2][1];
+                }
+                else if constexpr (keep)
+                {
+                    yn[0] = qr[2][0];
+                    yn[1] = qr[2][1];
+                    yn[2] = p.yn[2 * tot + oc];
+                    yn[3] = p.yn[3 * tot + oc];
+                }
+                else
+                    for (int c = 0; c < 4; ++c)
+                        yn[c] = p.yn[c * tot + oc];
                 if (PAIR == 2)
                 {
-                    pi2[0] = p.y2[2 * tot + oc];
-                    pi2[1] = p.y2[3 * tot + oc];
+                    if constexpr (keep)
+                    {
+                        pi2[0] = qr[2][2];
+                        pi2[1] = qr[2][3];
+                    }
+                    else
+                    {
+                        pi2[0] = p.y2[2 * tot + oc];
+                        pi2[1] = p.y2[3 * tot + oc];
+                    }
                     for (int c = 0; c < 4; ++c)
                         ac[c] = p.acc[c * tot + oc];
                     if (PREC == AXS_FP32C)
@@ -1052,16 +1115,22 @@ __global__ void __launch_bounds__(kBoxLanes) k_box_pair(BoxPairParams<T> p)
                 }
             }
             for (int i = 0; i < 4; ++i)
+            {
                 for (int c = 0; c < 2; ++c)
                 {
                     qa[i][c] = qa[i + 1][c];
                     qb[i][c] = qb[i + 1][c];
                 }
+                for (int c = 0; c < NR; ++c)
+                    qr[i][c] = qr[i + 1][c];
+            }
             for (int c = 0; c < 2; ++c)
             {
                 qa[4][c] = nqa[c];
                 qb[4][c] = nqb[c];
             }
+            for (int c = 0; c < NR; ++c)
+                qr[4][c] = nqr[c];
         }
         __syncthreads(); // LDS reuse by the next item
     }
