@@ -955,24 +955,11 @@ __global__ void __launch_bounds__(kBoxLanes) k_box_pair(BoxPairParams<T> p)
         if (lane)
             for (int i = 0; i < 5; ++i)
                 load_own(lx, yl, i - 2, qa[i], qb[i], qr[i]);
-        // halo points of the plane the next LDS write takes (h*) and, for
-        // FP32 / FP32C, of the plane after it (h*2): their loads are then
-        // issued two planes ahead of their use, like the own column's
-        constexpr bool far_halo = sizeof(T) == 4;
         T hxa[2] = {0, 0}, hxb[2] = {0, 0}, hya[2] = {0, 0}, hyb[2] = {0, 0};
-        T hxa2[2] = {0, 0}, hxb2[2] = {0, 0}, hya2[2] = {0, 0}, hyb2[2] = {0, 0};
         if (hxl || hxr)
             load(hxx, yl, 0, hxa, hxb);
         if (hyl || hyr)
             load(lx, hyy, 0, hya, hyb);
-        if constexpr (far_halo)
-            if (nzb > 1)
-            {
-                if (hxl || hxr)
-                    load(hxx, yl, 1, hxa2, hxb2);
-                if (hyl || hyr)
-                    load(lx, hyy, 1, hya2, hyb2);
-            }
         for (int z = 0; z < nzb; ++z)
         {
             const int bf = z & 1;
@@ -1011,30 +998,10 @@ __global__ void __launch_bounds__(kBoxLanes) k_box_pair(BoxPairParams<T> p)
             if (lane && z + 1 < nzb)
             {
                 load_own(lx, yl, z + 3, nqa, nqb, nqr);
-                if constexpr (far_halo)
-                {
-                    for (int c = 0; c < 2; ++c)
-                    {
-                        hxa[c] = hxa2[c];
-                        hxb[c] = hxb2[c];
-                        hya[c] = hya2[c];
-                        hyb[c] = hyb2[c];
-                    }
-                    if (z + 2 < nzb)
-                    {
-                        if (hxl || hxr)
-                            load(hxx, yl, z + 2, hxa2, hxb2);
-                        if (hyl || hyr)
-                            load(lx, hyy, z + 2, hya2, hyb2);
-                    }
-                }
-                else
-                {
-                    if (hxl || hxr)
-                        load(hxx, yl, z + 1, hxa, hxb);
-                    if (hyl || hyr)
-                        load(lx, hyy, z + 1, hya, hyb);
-                }
+                if (hxl || hxr)
+                    load(hxx, yl, z + 1, hxa, hxb);
+                if (hyl || hyr)
+                    load(lx, hyy, z + 1, hya, hyb);
             }
             const uint32_t oc = active ? at(lx, yy, z) : 0;
             T yn[4] = {0, 0, 0, 0}, ac[4] = {0, 0, 0, 0}, pi2[2] = {0, 0};
