@@ -219,101 +219,122 @@ __device__ double coarse_value(const CoarseSrc<T> &c, int comp, int cx, int cy,
     return v;
 }
 
-// Interior prolongation: one thread per coarse cell under a fine box's valid
-// region (box bounds are even) computes its 8 fine children at once, so each
-// coarse neighbour is located once for the 4 components and read once for
-// the 8 children.  Per child the tensor-product sums run in the same order
-// as interp_quartic's (x innermost, then y, then z), so the values are the
-// same.
+// Interior prolongation.  A block takes a tile of 8 x 8 x 4 coarse cells
+// under a fine box's valid region (box bounds are even): it stages the
+// tile's coarse neighbourhood (the tile grown by 2, clipped to what the
+// tile's cells use), each cell located once and its 4 components (time
+// blended, FP64) put in LDS, then one thread per coarse cell computes its 8
+// fine children from there.  Per child the tensor-product sums run in the
+// same order as interp_quartic's (x innermost, then y, then z), with the
+// same coarse values (0 where a coarse cell has no owner, which is
+// counted), so the values are the same.
+constexpr int kProTX = 8, kProTY = 8, kProTZ = 4;
+constexpr int kProStage = (kProTX + 4) * (kProTY + 4) * (kProTZ + 4);
+
 template <typename T>
 __global__ void __launch_bounds__(256)
     k_prolong(LevelView f, T *buf, float *lo, CoarseSrc<T> c,
               unsigned long long *err)
 {
+    __shared__ double st[4][kProStage];
     const BoxD B = f.box[blockIdx.y];
     const int cn0 = B.n[0] / 2, cn1 = B.n[1] / 2, cn2 = B.n[2] / 2;
-    const uint32_t ncc = (uint32_t)cn0 * cn1 * cn2;
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < ncc;
-         t += gridDim.x * blockDim.x)
+    const int ntx = (cn0 + kProTX - 1) / kProTX, nty = (cn1 + kProTY - 1) / kProTY,
+              ntz = (cn2 + kProTZ - 1) / kProTZ;
+    const int tile = (int)blockIdx.x;
+    if (tile >= ntx * nty * ntz) // the whole block: before any barrier
+        return;
+    const int tx0 = (tile % ntx) * kProTX, ty0 = ((tile / ntx) % nty) * kProTY,
+              tz0 = (tile / (ntx * nty)) * kProTZ;
+    const int ex = min(kProTX, cn0 - tx0), ey = min(kProTY, cn1 - ty0),
+              ez = min(kProTZ, cn2 - tz0);
+    const int sx = ex + 4, sy = ey + 4, sz = ez + 4;
+    // coarse global coordinates of the staged region's corner
+    const int gx0 = (B.lo[0] >> 1) + tx0 - 2, gy0 = (B.lo[1] >> 1) + ty0 - 2,
+              gz0 = (B.lo[2] >> 1) + tz0 - 2;
+    for (int s = threadIdx.x; s < sx * sy * sz; s += blockDim.x)
     {
-        const int ix = (int)(t % cn0), iy = (int)((t / cn0) % cn1),
-                  iz = (int)(t / ((uint32_t)cn0 * cn1));
-        // coarse cell of the children (B.lo is even and inside the domain)
-        const int cx = (B.lo[0] >> 1) + ix, cy = (B.lo[1] >> 1) + iy,
-                  cz = (B.lo[2] >> 1) + iz;
-        double acc[4][2][2][2]; // comp, pz, py, px
+        const int lx = s % sx, ly = (s / sx) % sy, lz = s / (sx * sy);
+        uint32_t ci;
+        const bool ok = locate(c.v, gx0 + lx, gy0 + ly, gz0 + lz, ci) >= 0;
+        if (!ok)
+            atomicAdd(err, 1ull);
         for (int q = 0; q < 4; ++q)
-            for (int a = 0; a < 2; ++a)
-                for (int b = 0; b < 2; ++b)
-                    acc[q][a][b][0] = acc[q][a][b][1] = 0.0;
-        for (int k = 0; k < 5; ++k)
         {
-            double ay[4][2][2]; // comp, py, px
-            for (int q = 0; q < 4; ++q)
-                ay[q][0][0] = ay[q][0][1] = ay[q][1][0] = ay[q][1][1] = 0.0;
-            for (int j = 0; j < 5; ++j)
+            double v = 0.0;
+            if (ok)
             {
-                double ax[4][2]; // comp, px
-                for (int q = 0; q < 4; ++q)
-                    ax[q][0] = ax[q][1] = 0.0;
-                for (int i = 0; i < 5; ++i)
+                const size_t o = (size_t)q * c.v.total + ci;
+                v = (double)c.cur[o] + (c.cur_lo ? (double)c.cur_lo[o] : 0.0);
+                if (c.snap && c.t_frac < 1.0)
                 {
-                    uint32_t ci;
-                    const bool ok =
-                        locate(c.v, cx + i - 2, cy + j - 2, cz + k - 2, ci) >= 0;
-                    if (!ok)
-                        atomicAdd(err, 1ull);
-                    for (int q = 0; q < 4; ++q)
-                    {
-                        double v = 0.0;
-                        if (ok)
-                        {
-                            const size_t o = (size_t)q * c.v.total + ci;
-                            v = (double)c.cur[o] +
-                                (c.cur_lo ? (double)c.cur_lo[o] : 0.0);
-                            if (c.snap && c.t_frac < 1.0)
-                            {
-                                const double sv =
-                                    (double)c.snap[o] +
-                                    (c.snap_lo ? (double)c.snap_lo[o] : 0.0);
-                                v = (1.0 - c.t_frac) * sv + c.t_frac * v;
-                            }
-                        }
-                        ax[q][0] += kQuartW[0][i] * v;
-                        ax[q][1] += kQuartW[1][i] * v;
-                    }
+                    const double sv =
+                        (double)c.snap[o] + (c.snap_lo ? (double)c.snap_lo[o] : 0.0);
+                    v = (1.0 - c.t_frac) * sv + c.t_frac * v;
                 }
-                for (int q = 0; q < 4; ++q)
-                    for (int py = 0; py < 2; ++py)
-                    {
-                        ay[q][py][0] += kQuartW[py][j] * ax[q][0];
-                        ay[q][py][1] += kQuartW[py][j] * ax[q][1];
-                    }
             }
-            for (int q = 0; q < 4; ++q)
-                for (int pz = 0; pz < 2; ++pz)
-                    for (int py = 0; py < 2; ++py)
-                    {
-                        acc[q][pz][py][0] += kQuartW[pz][k] * ay[q][py][0];
-                        acc[q][pz][py][1] += kQuartW[pz][k] * ay[q][py][1];
-                    }
+            st[q][s] = v;
         }
-        for (int pz = 0; pz < 2; ++pz)
-            for (int py = 0; py < 2; ++py)
-                for (int px = 0; px < 2; ++px)
+    }
+    __syncthreads();
+    const int ix = threadIdx.x % kProTX, iy = (threadIdx.x / kProTX) % kProTY,
+              iz = threadIdx.x / (kProTX * kProTY);
+    if (ix >= ex || iy >= ey || iz >= ez)
+        return;
+    double acc[4][2][2][2]; // comp, pz, py, px
+    for (int q = 0; q < 4; ++q)
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b)
+                acc[q][a][b][0] = acc[q][a][b][1] = 0.0;
+    for (int k = 0; k < 5; ++k)
+    {
+        double ay[4][2][2]; // comp, py, px
+        for (int q = 0; q < 4; ++q)
+            ay[q][0][0] = ay[q][0][1] = ay[q][1][0] = ay[q][1][1] = 0.0;
+        for (int j = 0; j < 5; ++j)
+        {
+            double ax[4][2]; // comp, px
+            for (int q = 0; q < 4; ++q)
+                ax[q][0] = ax[q][1] = 0.0;
+            const int row = ((iz + k) * sy + (iy + j)) * sx + ix;
+            for (int i = 0; i < 5; ++i)
+                for (int q = 0; q < 4; ++q)
                 {
-                    const uint32_t dst =
-                        sidx(B, f.ng, 2 * ix + px, 2 * iy + py, 2 * iz + pz);
-                    for (int q = 0; q < 4; ++q)
-                    {
-                        const double v = acc[q][pz][py][px];
-                        const T h = (T)v;
-                        buf[(size_t)q * f.total + dst] = h;
-                        if (lo)
-                            lo[(size_t)q * f.total + dst] = (float)(v - (double)h);
-                    }
+                    const double v = st[q][row + i];
+                    ax[q][0] += kQuartW[0][i] * v;
+                    ax[q][1] += kQuartW[1][i] * v;
+                }
+            for (int q = 0; q < 4; ++q)
+                for (int py = 0; py < 2; ++py)
+                {
+                    ay[q][py][0] += kQuartW[py][j] * ax[q][0];
+                    ay[q][py][1] += kQuartW[py][j] * ax[q][1];
+                }
+        }
+        for (int q = 0; q < 4; ++q)
+            for (int pz = 0; pz < 2; ++pz)
+                for (int py = 0; py < 2; ++py)
+                {
+                    acc[q][pz][py][0] += kQuartW[pz][k] * ay[q][py][0];
+                    acc[q][pz][py][1] += kQuartW[pz][k] * ay[q][py][1];
                 }
     }
+    const int bx = tx0 + ix, by = ty0 + iy, bz = tz0 + iz; // coarse local
+    for (int pz = 0; pz < 2; ++pz)
+        for (int py = 0; py < 2; ++py)
+            for (int px = 0; px < 2; ++px)
+            {
+                const uint32_t dst =
+                    sidx(B, f.ng, 2 * bx + px, 2 * by + py, 2 * bz + pz);
+                for (int q = 0; q < 4; ++q)
+                {
+                    const double v = acc[q][pz][py][px];
+                    const T h = (T)v;
+                    buf[(size_t)q * f.total + dst] = h;
+                    if (lo)
+                        lo[(size_t)q * f.total + dst] = (float)(v - (double)h);
+                }
+            }
 }
 
 // Ghost index t of a grown box -> local cell (lx, ly, lz): the ghost shell
@@ -1776,7 +1797,12 @@ static int fill_t(axs_boxlevel *bl, void *buf, float *lo,
     if (interior)
     {
         // valid cells from the coarse level, then the ghosts as usual
-        hipLaunchKernelGGL(k_prolong<T>, box_grid(bl, max_valid(bl) / 8),
+        int tiles = 1;
+        for (const BoxD &B : bl->hbox)
+            tiles = std::max(tiles, ((B.n[0] / 2 + kProTX - 1) / kProTX) *
+                                        ((B.n[1] / 2 + kProTY - 1) / kProTY) *
+                                        ((B.n[2] / 2 + kProTZ - 1) / kProTZ));
+        hipLaunchKernelGGL(k_prolong<T>, dim3((unsigned)tiles, (unsigned)bl->nb),
                            dim3(256), 0, bl->ctx->stream, view_of(bl),
                            (T *)buf, lo, c, bl->derr);
         AXS_HIP_TRY(hipGetLastError());
