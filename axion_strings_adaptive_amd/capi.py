@@ -193,6 +193,7 @@ _SIGS = {
     "axs_hierarchy_plaquette_counts": (_I, [_P, _PI64]),
     "axs_hierarchy_collapse": (_I, [_P, _P]),
     "axs_amr_tagged_blocks": (_I, [_P, C.POINTER(_I), _I, _I, _P]),
+    "axs_amr_tagged_blocks_device": (_I, [_P, _P, C.POINTER(_I), _I, _I, _P]),
     "axs_amr_erode": (_I, [_P, C.POINTER(_I), _I, _P]),
     "axs_amr_boxes_from_blocks": (_I, [_P, C.POINTER(_I), _I, _I, _I,
                                        C.POINTER(_I), C.POINTER(_I),

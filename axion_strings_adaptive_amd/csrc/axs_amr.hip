@@ -81,6 +81,41 @@ std::vector<uint8_t> tagged_blocks(const uint8_t *tags, const int dom[3],
     return out;
 }
 
+// tagged_blocks on the device, from device tags: block (bx, by, bz) is set
+// where a tag lies in [b block - r, (b + 1) block + r) along every axis
+// (periodic), the set the axis-by-axis reduction above forms.  One
+// wavefront per block, its lanes striding the (block + 2r)^3 window; only
+// the block mask (dom / block cells) then goes to the host.
+__global__ void k_tagged_blocks(const uint8_t *tags, int nx, int ny, int nz,
+                                int block, int r, uint8_t *out)
+{
+    const int bx = nx / block, by = ny / block, bz = nz / block;
+    const size_t nb = (size_t)bx * by * bz;
+    const int lane = threadIdx.x & 63;
+    const size_t nw = ((size_t)gridDim.x * blockDim.x) >> 6;
+    const int w = block + 2 * r, w3 = w * w * w;
+    for (size_t b = (((size_t)blockIdx.x * blockDim.x) + threadIdx.x) >> 6;
+         b < nb; b += nw)
+    {
+        const int ix = (int)(b % bx), iy = (int)((b / bx) % by),
+                  iz = (int)(b / ((size_t)bx * by));
+        bool any = false;
+        for (int s = lane; s < w3 && !any; s += 64)
+        {
+            const int dx = s % w, dy = (s / w) % w, dz = s / (w * w);
+            int x = ix * block - r + dx, y = iy * block - r + dy,
+                z = iz * block - r + dz;
+            x = x < 0 ? x + nx : (x >= nx ? x - nx : x);
+            y = y < 0 ? y + ny : (y >= ny ? y - ny : y);
+            z = z < 0 ? z + nz : (z >= nz ? z - nz : z);
+            any = tags[((size_t)z * ny + y) * nx + x] != 0;
+        }
+        const bool set = __ballot(any) != 0ull;
+        if (lane == 0)
+            out[b] = set ? 1 : 0;
+    }
+}
+
 // a block mask kept where its whole periodic (2r+1)^3 neighbourhood is set
 std::vector<uint8_t> erode(const uint8_t *m, const int bd[3], int r)
 {
@@ -305,12 +340,16 @@ int fine_boxes(axs_hierarchy *h, int l, const std::vector<uint8_t> &covered,
         return AXS_OK;
     const std::array<int, 3> &cd = h->domains[l];
     const size_t n = cells_of(cd);
-    std::vector<uint8_t> tags(n, 1);
+    const int cblock = h->p.blocking_factor / 2;
+    const int bd[3] = {cd[0] / cblock, cd[1] / cblock, cd[2] / cblock};
+    const size_t nb = (size_t)bd[0] * bd[1] * bd[2];
+    blk->assign(nb, 1); // every block tagged under force_full_refinement
     if (!h->p.force_full_refinement)
     {
         double R = 1.0;
         AXS_TRY(axs_background_eval(&h->bg, 0, h->tau, &R));
-        AXS_TRY(device_buffer(h, &h->dtags, &h->dtags_bytes, n));
+        // dense tags, then the block mask behind them
+        AXS_TRY(device_buffer(h, &h->dtags, &h->dtags_bytes, n + nb));
         AXS_TRY(axs_device_memset(h->ctx, h->dtags, 0, n));
         const axs_tagger_params tp{h->p.gradient_threshold,
                                    h->p.radial_gradient_threshold,
@@ -318,11 +357,15 @@ int fine_boxes(axs_hierarchy *h, int l, const std::vector<uint8_t> &covered,
         int64_t counts[2];
         AXS_TRY(axs_boxlevel_tag(h->levels[l], &tp, (uint8_t *)h->dtags,
                                  nullptr, counts));
-        AXS_TRY(axs_device_download(h->ctx, tags.data(), h->dtags, n));
+        uint8_t *dmask = (uint8_t *)h->dtags + n;
+        hipLaunchKernelGGL(k_tagged_blocks,
+                           dim3((unsigned)std::min<size_t>(4096, (nb + 3) / 4)),
+                           dim3(256), 0, h->ctx->stream,
+                           (const uint8_t *)h->dtags, cd[0], cd[1], cd[2],
+                           cblock, h->p.n_error_buf, dmask);
+        AXS_HIP_TRY(hipGetLastError());
+        AXS_TRY(axs_device_download(h->ctx, blk->data(), dmask, nb));
     }
-    const int cblock = h->p.blocking_factor / 2;
-    *blk = tagged_blocks(tags.data(), cd.data(), cblock, h->p.n_error_buf);
-    const int bd[3] = {cd[0] / cblock, cd[1] / cblock, cd[2] / cblock};
     if (l >= 1)
     {
         const std::vector<uint8_t> inner = erode(covered.data(), bd, 1);
@@ -776,6 +819,26 @@ int axs_amr_tagged_blocks(const uint8_t *tags, const int dom[3], int block,
     const std::vector<uint8_t> b = tagged_blocks(tags, dom, block, r);
     std::memcpy(out, b.data(), b.size());
     return AXS_OK;
+}
+
+int axs_amr_tagged_blocks_device(axs_ctx *ctx, const uint8_t *tags_dev,
+                                 const int dom[3], int block, int r,
+                                 uint8_t *out_dev)
+{
+    AXS_CHECK_ARG(ctx && tags_dev && dom && out_dev,
+                  "axs_amr_tagged_blocks_device: NULL argument");
+    AXS_CHECK_ARG(block >= 1 && r >= 0 && r <= block,
+                  "axs_amr_tagged_blocks_device: need block >= 1, 0 <= r <= block");
+    for (int d = 0; d < 3; ++d)
+        AXS_CHECK_ARG(dom[d] > 0 && dom[d] % block == 0,
+                      "axs_amr_tagged_blocks_device: dom must be multiples of block");
+    const size_t nb = (size_t)(dom[0] / block) * (dom[1] / block) * (dom[2] / block);
+    hipLaunchKernelGGL(k_tagged_blocks,
+                       dim3((unsigned)std::min<size_t>(4096, (nb + 3) / 4)),
+                       dim3(256), 0, ctx->stream, tags_dev, dom[0], dom[1],
+                       dom[2], block, r, out_dev);
+    AXS_HIP_TRY(hipGetLastError());
+    return axs_ctx_synchronize(ctx);
 }
 
 int axs_amr_erode(const uint8_t *mask, const int bdim[3], int r, uint8_t *out)

@@ -659,6 +659,12 @@ int axs_hierarchy_collapse(axs_hierarchy *h, axs_level *dst);
  * to lo / hi (3 ints each) if at most max_boxes, else AXS_ERR_ARG. */
 int axs_amr_tagged_blocks(const uint8_t *tags, const int dom[3], int block,
                           int r, uint8_t *out);
+/* No reference counterpart: the same block mask from tags in device memory
+ * into device memory (dom/block bytes), as the hierarchy's regrid forms it
+ * on the device; synchronizes the context's stream. */
+int axs_amr_tagged_blocks_device(axs_ctx *ctx, const uint8_t *tags_dev,
+                                 const int dom[3], int block, int r,
+                                 uint8_t *out_dev);
 int axs_amr_erode(const uint8_t *mask, const int bdim[3], int r,
                   uint8_t *out);
 int axs_amr_boxes_from_blocks(const uint8_t *blk, const int bdim[3],

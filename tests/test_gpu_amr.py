@@ -212,3 +212,33 @@ def test_schedule_gates_level_creation(ctx):
                       force_full_refinement=True)
     assert len(h.levels) == 2
     h.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("block,r", [(8, 2), (8, 0), (8, 8), (4, 3)])
+def test_tagged_blocks_device_equals_host(ctx, block, r):
+    """The regrid's block mask, formed on the device from device tags, is
+    the host reduction's (tags grown by r cells, periodic, any per block)
+    for sparse, medium and dense tags, including tags next to the seam."""
+    import ctypes as C
+    from axion_strings_adaptive_amd import capi
+    lib = capi.load()
+    u8 = C.POINTER(C.c_uint8)
+    rng = np.random.default_rng(10 * block + r)
+    nx, ny, nz = 64, 32, 48
+    for p in (0.0005, 0.01, 0.2):
+        tags = (rng.random((nz, ny, nx)) < p).astype(np.uint8)
+        tags[0, 0, nx - 1] = 1  # grows across the periodic seams
+        want = np.zeros((nz // block, ny // block, nx // block), np.uint8)
+        capi.check(lib.axs_amr_tagged_blocks(
+            tags.ctypes.data_as(u8), (C.c_int * 3)(nx, ny, nz), block, r,
+            want.ctypes.data_as(u8)))
+        tb = capi.DeviceBuffer(tags.size)
+        tb.from_numpy(tags)
+        ob = capi.DeviceBuffer(want.size)
+        capi.check(lib.axs_amr_tagged_blocks_device(
+            ctx.h, C.c_void_p(tb.ptr), (C.c_int * 3)(nx, ny, nz), block, r,
+            C.c_void_p(ob.ptr)))
+        got = ob.to_numpy(np.uint8).reshape(want.shape)
+        assert np.array_equal(got, want)
+        assert 0 < want.sum()
