@@ -1455,15 +1455,25 @@ __device__ __forceinline__ double d1r(double fm2, double fm1, double fp1,
     return (w_far * fm2 - w_near * fm1 + w_near * fp1 - w_far * fp2) * i1;
 }
 
+// compute_total_energy_sums over a box level's valid cells
+// (EnergyKernel.hpp:106-233, Energy.hpp pointwise forms): a block streams an item's
+// W x H rows through the box's planes with psi (FP64, FP32C as hi + lo) of
+// the current plane in LDS with a 2-cell x / y halo (from the filled
+// ghosts) and the z neighbours in a 5-deep register queue, so each psi value
+// is read about once per plane instead of once per stencil use.  Per cell
+// the 14 terms are the reference's, cells the coverage mask excludes
+// skipped; the sums are fixed-order per block, then over blocks.
 template <typename T>
-__global__ void k_box_energy(LevelView v, const T *y, const float *lo,
-                             const uint8_t *cover, double dx, double R,
-                             double lambda, double RpR, int scheme, double thr,
-                             double *partials)
+__global__ void __launch_bounds__(kBoxLanes)
+    k_box_energy_items(const BoxD *box, const WorkItem *items, int nitems,
+                       int ng, uint32_t total, int domx, int domy,
+                       const T *y, const float *lo, const uint8_t *cover,
+                       double dx, double R, double lambda, double RpR,
+                       int scheme, double thr, double *partials)
 {
-    const BoxD B = v.box[blockIdx.y];
-    const uint32_t cells = (uint32_t)B.n[0] * B.n[1] * B.n[2];
-    const size_t tot = v.total;
+    __shared__ double sp[2][kBoxLds];
+    __shared__ double red[kBoxLanes];
+    const size_t tot = total;
     const double i1 = 1.0 / dx, R2 = R * R, R4 = R2 * R2, f_a = 1.4142135623730951;
     double acc[14];
     for (int q = 0; q < 14; ++q)
@@ -1472,84 +1482,158 @@ __global__ void k_box_energy(LevelView v, const T *y, const float *lo,
         const size_t o = (size_t)comp * tot + i;
         return (double)y[o] + (lo ? (double)lo[o] : 0.0);
     };
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < cells;
-         t += gridDim.x * blockDim.x)
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x)
     {
-        const int lx = t % B.n[0], ly = (t / B.n[0]) % B.n[1],
-                  lz = t / ((uint32_t)B.n[0] * B.n[1]);
-        const uint32_t dense = ((uint32_t)(B.lo[2] + lz) * v.dom[1] +
-                                (B.lo[1] + ly)) *
-                                   v.dom[0] +
-                               (B.lo[0] + lx);
-        if (cover && !cover[dense])
-            continue;
-        const uint32_t c = sidx(B, v.ng, lx, ly, lz);
-        const double psi1 = val(0, c), psi2 = val(1, c), Pi1 = val(2, c),
-                     Pi2 = val(3, c);
-        const double psi_sq = psi1 * psi1 + psi2 * psi2;
-        double g1sq = 0.0, g2sq = 0.0, gtsq = 0.0;
-        for (int d = 0; d < 3; ++d)
-        {
-            const int ex = d == 0, ey = d == 1, ez = d == 2;
-            auto s = [&](int comp, int m) -> double {
-                return val(comp, sidx(B, v.ng, lx + m * ex, ly + m * ey,
-                                      lz + m * ez));
-            };
-            const double a = d1r(s(0, -2), s(0, -1), s(0, 1), s(0, 2), i1);
-            const double b = d1r(s(1, -2), s(1, -1), s(1, 1), s(1, 2), i1);
-            g1sq += a * a;
-            g2sq += b * b;
-            const double dth = psi_sq > 0.0 ? (psi1 * b - psi2 * a) / psi_sq : 0.0;
-            gtsq += dth * dth;
-        }
-        const double c1 = Pi1 - RpR * psi1, c2 = Pi2 - RpR * psi2;
-        const double dp = psi_sq - R2;
-        const double pot = 0.25 * lambda * dp * dp;
-        const double rho = (c1 * c1 + c2 * c2 + (g1sq + g2sq) + pot) / R4;
-        const double tp = psi_sq > 0.0 ? (psi1 * Pi2 - Pi1 * psi2) / psi_sq : 0.0;
-        const double adot = f_a * tp / R;
-        const double rak = 0.5 * adot * adot;
-        const double rag = 0.5 * f_a * f_a * gtsq / R2;
-        double rrk = 0.0;
-        if (psi_sq > 0.0)
-        {
-            const double rd = (psi1 * c1 + psi2 * c2) / sqrt(psi_sq);
-            rrk = rd * rd / R4;
-        }
-        const double rrg = (g1sq + g2sq - psi_sq * gtsq) / R4;
-        const double rrm = pot / R4;
-        double wgt = 1.0;
-        if (scheme != AXS_MASK_NONE)
-        {
-            const double mod = sqrt(psi_sq) / R;
-            if (scheme == AXS_MASK_B)
-                wgt = mod >= thr ? 1.0 : 0.0;
-            else
+        const WorkItem w = items[it];
+        const BoxD B = box[w.box];
+        const int W = w.w, H = w.h, nyb = B.n[1], nzb = B.n[2];
+        const int SW = W + 4;
+        const int t = threadIdx.x;
+        const int lx = t % W, ly = t / W;
+        const bool lane = ly < H;
+        const int yy = w.y0 + ly;
+        const bool active = lane && yy < nyb;
+        const int yl = min(yy, nyb + 1);
+        const bool hxl = lane && lx < 2, hxr = lane && lx >= W - 2;
+        const bool hyl = lane && ly < 2, hyr = lane && ly >= H - 2;
+        const int hxx = hxl ? lx - 2 : W + (lx - (W - 2));
+        const int hyy = hyl ? w.y0 + ly - 2
+                            : min(w.y0 + H + (ly - (H - 2)), nyb + 1);
+        double q1[5] = {}, q2[5] = {}; // own psi at planes z-2 .. z+2
+        if (lane)
+            for (int i = 0; i < 5; ++i)
             {
-                const double f = 1.0 + (mod - 1.0) / f_a;
-                wgt = f * f;
+                const uint32_t o = sidx(B, ng, lx, yl, i - 2);
+                q1[i] = val(0, o);
+                q2[i] = val(1, o);
             }
+        for (int z = 0; z < nzb; ++z)
+        {
+            if (lane)
+            {
+                const int r = (ly + 2) * SW + lx + 2;
+                sp[0][r] = q1[2];
+                sp[1][r] = q2[2];
+                if (hxl || hxr)
+                {
+                    const uint32_t o = sidx(B, ng, hxx, yl, z);
+                    const int rx = (ly + 2) * SW + (hxl ? lx : lx + 4);
+                    sp[0][rx] = val(0, o);
+                    sp[1][rx] = val(1, o);
+                }
+                if (hyl || hyr)
+                {
+                    const uint32_t o = sidx(B, ng, lx, hyy, z);
+                    const int ry = (hyl ? ly : ly + 4) * SW + lx + 2;
+                    sp[0][ry] = val(0, o);
+                    sp[1][ry] = val(1, o);
+                }
+            }
+            double n1 = 0.0, n2 = 0.0; // own psi of plane z + 3
+            if (lane && z + 1 < nzb)
+            {
+                const uint32_t o = sidx(B, ng, lx, yl, z + 3);
+                n1 = val(0, o);
+                n2 = val(1, o);
+            }
+            __syncthreads();
+            const uint32_t dense =
+                active ? ((uint32_t)(B.lo[2] + z) * domy + (B.lo[1] + yy)) * domx +
+                             (B.lo[0] + lx)
+                       : 0;
+            if (active && !(cover && !cover[dense]))
+            {
+                const uint32_t c = sidx(B, ng, lx, yy, z);
+                const double psi1 = q1[2], psi2 = q2[2], Pi1 = val(2, c),
+                             Pi2 = val(3, c);
+                const double psi_sq = psi1 * psi1 + psi2 * psi2;
+                const int r = (ly + 2) * SW + lx + 2;
+                double g1sq = 0.0, g2sq = 0.0, gtsq = 0.0;
+                for (int d = 0; d < 3; ++d)
+                {
+                    double a, b;
+                    if (d == 0)
+                    {
+                        a = d1r(sp[0][r - 2], sp[0][r - 1], sp[0][r + 1],
+                                sp[0][r + 2], i1);
+                        b = d1r(sp[1][r - 2], sp[1][r - 1], sp[1][r + 1],
+                                sp[1][r + 2], i1);
+                    }
+                    else if (d == 1)
+                    {
+                        a = d1r(sp[0][r - 2 * SW], sp[0][r - SW], sp[0][r + SW],
+                                sp[0][r + 2 * SW], i1);
+                        b = d1r(sp[1][r - 2 * SW], sp[1][r - SW], sp[1][r + SW],
+                                sp[1][r + 2 * SW], i1);
+                    }
+                    else
+                    {
+                        a = d1r(q1[0], q1[1], q1[3], q1[4], i1);
+                        b = d1r(q2[0], q2[1], q2[3], q2[4], i1);
+                    }
+                    g1sq += a * a;
+                    g2sq += b * b;
+                    const double dth =
+                        psi_sq > 0.0 ? (psi1 * b - psi2 * a) / psi_sq : 0.0;
+                    gtsq += dth * dth;
+                }
+                const double c1 = Pi1 - RpR * psi1, c2 = Pi2 - RpR * psi2;
+                const double dp = psi_sq - R2;
+                const double pot = 0.25 * lambda * dp * dp;
+                const double rho = (c1 * c1 + c2 * c2 + (g1sq + g2sq) + pot) / R4;
+                const double tp =
+                    psi_sq > 0.0 ? (psi1 * Pi2 - Pi1 * psi2) / psi_sq : 0.0;
+                const double adot = f_a * tp / R;
+                const double rak = 0.5 * adot * adot;
+                const double rag = 0.5 * f_a * f_a * gtsq / R2;
+                double rrk = 0.0;
+                if (psi_sq > 0.0)
+                {
+                    const double rd = (psi1 * c1 + psi2 * c2) / sqrt(psi_sq);
+                    rrk = rd * rd / R4;
+                }
+                const double rrg = (g1sq + g2sq - psi_sq * gtsq) / R4;
+                const double rrm = pot / R4;
+                double wgt = 1.0;
+                if (scheme != AXS_MASK_NONE)
+                {
+                    const double mod = sqrt(psi_sq) / R;
+                    if (scheme == AXS_MASK_B)
+                        wgt = mod >= thr ? 1.0 : 0.0;
+                    else
+                    {
+                        const double f = 1.0 + (mod - 1.0) / f_a;
+                        wgt = f * f;
+                    }
+                }
+                const double vals[14] = {rho, rho * wgt, rak, rak * wgt, rag,
+                                         rag * wgt, rrk, rrk * wgt, rrg,
+                                         rrg * wgt, rrm, rrm * wgt, wgt, 1.0};
+                for (int q = 0; q < 14; ++q)
+                    acc[q] += vals[q];
+            }
+            __syncthreads(); // the plane's LDS is rewritten next
+            for (int i = 0; i < 4; ++i)
+            {
+                q1[i] = q1[i + 1];
+                q2[i] = q2[i + 1];
+            }
+            q1[4] = n1;
+            q2[4] = n2;
         }
-        const double vals[14] = {rho, rho * wgt, rak, rak * wgt, rag, rag * wgt,
-                                 rrk, rrk * wgt, rrg, rrg * wgt, rrm, rrm * wgt,
-                                 wgt, 1.0};
-        for (int q = 0; q < 14; ++q)
-            acc[q] += vals[q];
     }
-    __shared__ double red[256];
-    const uint32_t row = blockIdx.y * gridDim.x + blockIdx.x;
     for (int q = 0; q < 14; ++q)
     {
         red[threadIdx.x] = acc[q];
         __syncthreads();
-        for (int o = 128; o > 0; o >>= 1)
+        for (int o = kBoxLanes / 2; o > 0; o >>= 1)
         {
             if ((int)threadIdx.x < o)
                 red[threadIdx.x] += red[threadIdx.x + o];
             __syncthreads();
         }
         if (threadIdx.x == 0)
-            partials[(size_t)row * 14 + q] = red[0];
+            partials[(size_t)blockIdx.x * 14 + q] = red[0];
         __syncthreads();
     }
 }
@@ -2799,22 +2883,21 @@ int axs_boxlevel_energy_sums(axs_boxlevel *b, double R, double lambda,
     AXS_CHECK_ARG(b && m && out, "axs_boxlevel_energy_sums: NULL argument");
     AXS_CHECK_ARG(R > 0.0, "axs_boxlevel_energy_sums: R must be > 0");
     hipStream_t s = b->ctx->stream;
-    dim3 g = box_grid(b, max_valid(b));
-    // one row of 14 partials per block: fewer blocks per box when there
-    // are many boxes (the kernel strides over a box's cells)
+    // one row of 14 partials per block of the work-item pass
     const size_t cap_rows = ((size_t)kMaxPartials * 16 - 14) / 14;
-    const size_t per_box = cap_rows / g.y;
-    AXS_CHECK_ARG(per_box >= 1, "axs_boxlevel_energy_sums: too many boxes");
-    if (g.x > per_box)
-        g.x = (unsigned)per_box;
-    const size_t rows = (size_t)g.x * g.y;
     double *part = b->ctx->d_partials;
+    const size_t rows = (size_t)std::max(
+        1, std::min<int>(b->nitems, (int)std::min<size_t>(cap_rows, 4096)));
     if (b->precision == AXS_FP64)
-        hipLaunchKernelGGL(k_box_energy<double>, g, dim3(256), 0, s, view_of(b),
+        hipLaunchKernelGGL(k_box_energy_items<double>, dim3((unsigned)rows),
+                           dim3(kBoxLanes), 0, s, b->dbox, b->ditems, b->nitems,
+                           b->ng, b->total, b->dom[0], b->dom[1],
                            (const double *)b->y, (const float *)nullptr, cover,
                            b->dx, R, lambda, RpR, m->scheme, m->threshold, part);
     else
-        hipLaunchKernelGGL(k_box_energy<float>, g, dim3(256), 0, s, view_of(b),
+        hipLaunchKernelGGL(k_box_energy_items<float>, dim3((unsigned)rows),
+                           dim3(kBoxLanes), 0, s, b->dbox, b->ditems, b->nitems,
+                           b->ng, b->total, b->dom[0], b->dom[1],
                            (const float *)b->y, (const float *)b->comp, cover,
                            b->dx, R, lambda, RpR, m->scheme, m->threshold, part);
     hipLaunchKernelGGL(k_box_sum_cols, dim3(14), dim3(256), 0, s,
